@@ -1,0 +1,212 @@
+/*
+ * kiyosi_amd — C ABI of the MI355X (gfx950) Monte Carlo pricing engine.
+ *
+ * The library replaces the CPU Monte Carlo engines of kiyosi 0.23.0
+ * (crates/kiyosi/src/engines/monte_carlo.rs and engines/monte_carlo/). Each
+ * entry point below corresponds to one `PricingEngine<I>` method of one
+ * engine (crates/kiyosi/src/risk/analytics.rs:554-834) and names the
+ * reference function it stands in for. Plain C types only: pointers, sizes,
+ * integers and doubles. Times are signed microseconds since
+ * 1970-01-01T00:00:00 UTC, the resolution of kiyosi's `Timestamp`
+ * (crates/kiyosi/src/time/date.rs:255-257).
+ *
+ * Ownership follows the reference: inputs are borrowed and read-only, every
+ * call owns its device buffers ("every call owns its buffers",
+ * monte_carlo.rs:738-740), outputs are written to caller memory. A context is
+ * bound to one GPU; calls on one context are serialised by an internal mutex,
+ * so a context may be shared between threads like the reference's `Send +
+ * Sync` engines (crates/kiyosi/src/lib.rs:103-125).
+ */
+#ifndef KIYOSI_AMD_H
+#define KIYOSI_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error kinds of crates/kiyosi/src/error.rs:18-57. */
+typedef enum kg_status {
+  KG_OK = 0,
+  KG_INVALID_INPUT = 1,         /* Error::InvalidInput */
+  KG_UNSUPPORTED_OPERATION = 2, /* Error::UnsupportedOperation */
+  KG_COMPUTATION_FAILURE = 3,   /* Error::ComputationFailure (non-finite path or result) */
+  KG_INTERRUPTED = 4,           /* Error::Io(Interrupted): the cancel flag was raised */
+  KG_DEVICE_ERROR = 5,          /* Error::Io: a HIP runtime failure */
+  KG_MISSING_HISTORY = 6        /* Error::MissingHistory */
+} kg_status;
+
+/* MonteCarloSampling (monte_carlo.rs:76-82). */
+typedef enum kg_sampling { KG_PSEUDO_RANDOM = 0, KG_SOBOL = 1 } kg_sampling;
+
+/* Arithmetic of the path loop. KG_FP64 is the plain form that restates the
+ * reference arithmetic; KG_FP32_STEP steps paths in FP32 and accumulates
+ * discounted payoffs in FP64. */
+typedef enum kg_precision { KG_FP64 = 0, KG_FP32_STEP = 1 } kg_precision;
+
+/* MonteCarloUncertaintyStatus (monte_carlo.rs:869-894). */
+typedef enum kg_uncertainty {
+  KG_AVAILABLE = 0,
+  KG_UNSUPPORTED_SAMPLING = 1,
+  KG_INSUFFICIENT_OBSERVATIONS = 2,
+  KG_INCOMPLETE_ANTITHETIC_PAIR = 3,
+  KG_ZERO_OBSERVED_VARIANCE = 4
+} kg_uncertainty;
+
+/* MonteCarloSettings (monte_carlo.rs:90-118) plus the GPU precision knob.
+ * The seed must be resolved by the caller (`with_resolved_seed`,
+ * monte_carlo.rs:163-177): has_seed == 0 is rejected. */
+typedef struct kg_settings {
+  int64_t path_count;         /* 1..=10,000,000 as in the reference, or up to 2^26 with allow_large */
+  uint64_t seed;
+  int32_t has_seed;
+  int32_t sampling;           /* kg_sampling */
+  int32_t antithetic;         /* pairs Z and -Z */
+  int32_t precision;          /* kg_precision */
+  int32_t max_active_workers; /* validated (>= 0) and otherwise unused on the GPU */
+  int32_t allow_large;        /* lifts the 1e7-path and 1e9-path-step caps (GPU limit 2^26 paths) */
+} kg_settings;
+
+/* Moments (monte_carlo.rs:2155-2161): scale-normalised mean and M2. */
+typedef struct kg_moments {
+  uint64_t count;
+  double scale;
+  double mean;
+  double m2;
+} kg_moments;
+
+/* MonteCarloEstimate (monte_carlo.rs:859-1001) plus GPU diagnostics. */
+typedef struct kg_estimate {
+  double price;
+  double standard_error; /* valid iff has_standard_error */
+  double ci_lower;       /* price -/+ 1.959963984540054 * standard_error */
+  double ci_upper;
+  int32_t has_standard_error;
+  int32_t uncertainty_status; /* kg_uncertainty */
+  int32_t has_seed;           /* 0 when no sampling occurred */
+  int32_t reserved;
+  uint64_t seed;
+  /* Diagnostics, not part of the reference result: the standard error of
+   * the per-path discounted values computed as if they were independent
+   * (also for Sobol, where the reference reports none), the number of
+   * simulated path-steps and the device time of the sampling kernels. */
+  double path_standard_error;
+  double path_steps;
+  double device_ms;
+} kg_estimate;
+
+/* Flat-rate Black-Scholes-Merton market (market/parameters.rs:314-518,
+ * BlackScholesMertonParameters with flat curves). */
+typedef struct kg_bsm_market {
+  double spot;
+  double rate;
+  double dividend;
+  double volatility;
+  int64_t valuation_time;
+} kg_bsm_market;
+
+typedef struct kg_context kg_context;
+
+/* Creates a context on HIP device `device`. */
+kg_status kg_context_create(int32_t device, kg_context** out);
+void kg_context_destroy(kg_context* ctx);
+/* Message of the last failed call on this context (empty if none). */
+const char* kg_last_error(const kg_context* ctx);
+/* Version string of the library. */
+const char* kg_version(void);
+
+/* ---- European vanilla: MonteCarloVanillaEngine::price_with_statistics
+ * (monte_carlo.rs:1307-1320), sample (:1093-1141), EuropeanPayoff
+ * (:1165-1252). option_type: 1 call, -1 put. */
+typedef struct kg_european_terms {
+  int32_t option_type;
+  int32_t reserved;
+  double strike;
+  int64_t effective_time;
+  int64_t expiry_time;
+} kg_european_terms;
+
+kg_status kg_vanilla_price(kg_context* ctx, const kg_settings* settings,
+                           const kg_european_terms* option, const kg_bsm_market* market,
+                           kg_estimate* out);
+
+/* ---- Autocallable notes (snowball, binary/ternary snowball, Phoenix):
+ * MonteCarloAutocallableEngine::price_with_statistics
+ * (engines/monte_carlo/autocallable.rs:874-918), plan building
+ * prepare_sample_on_grid (:293-430), path payoffs bridge_payoffs
+ * (:628-736) and payoff (:823-862), note terms AutocallableNote
+ * (instruments/autocallable.rs:700-870, 1389-1424).
+ *
+ * `events` are the note's merged observation events
+ * (AutocallableNote::new, instruments/autocallable.rs:972-1040): strictly
+ * increasing times, each with an optional knock-out level, an optional
+ * coupon barrier and the snowball coupon rate accrued on knock-out. */
+typedef struct kg_autocall_event {
+  int64_t time;
+  int32_t has_knock_out;
+  int32_t has_coupon_barrier;
+  double knock_out_level;
+  double coupon_barrier_level;
+  double coupon_rate;
+} kg_autocall_event;
+
+enum {
+  KG_CALENDAR_WEEKDAYS = 0, /* TradingCalendar::weekdays (time/calendar.rs:156-161) */
+  KG_CALENDAR_ALL_DAYS = 1  /* TradingCalendar::all_days (time/calendar.rs:146-151) */
+};
+enum { KG_KNOCK_IN_NONE = 0, KG_KNOCK_IN_DAILY = 1, KG_KNOCK_IN_AT_EXPIRY = 2 };
+enum { KG_REDEMPTION_DOWNSIDE = 0, KG_REDEMPTION_FIXED_COUPON = 1 };
+enum { KG_COUPON_SNOWBALL = 0, KG_COUPON_PHOENIX = 1 };
+enum {
+  KG_STATE_UNKNOWN = -1, /* barrier_state None */
+  KG_STATE_NO_EVENT = 0,
+  KG_STATE_KNOCKED_OUT = 1,
+  KG_STATE_KNOCKED_IN = 2
+};
+
+typedef struct kg_autocall_terms {
+  int32_t calendar;
+  int32_t knock_in_kind;   /* KG_KNOCK_IN_* */
+  int32_t redemption_kind; /* KG_REDEMPTION_* (ignored without knock-in) */
+  int32_t coupon_kind;     /* KG_COUPON_* */
+  int32_t barrier_state;   /* KG_STATE_* */
+  int32_t n_events;
+  int64_t observation_clock_micros; /* time of day of daily knock-in checks */
+  double knock_in_level;
+  double initial_spot;
+  double upper_strike;
+  double lower_strike;
+  double fixed_coupon_rate; /* ternary snowball minimum coupon */
+  double coupon;            /* maturity coupon rate (snowball) or coupon per period (Phoenix) */
+  double principal_redemption_ratio;
+  double rebate_rate;
+  double annual_rebate_rate;
+  int64_t effective_time;
+  int64_t expiry_time;
+  const kg_autocall_event* events;
+} kg_autocall_terms;
+
+kg_status kg_autocallable_price(kg_context* ctx, const kg_settings* settings,
+                                const kg_autocall_terms* note, const kg_bsm_market* market,
+                                kg_estimate* out);
+
+/* ---- Generator diagnostics, computed on the GPU with per-thread skip-ahead.
+ * kg_pcg32_outputs: outputs first..first+count of Pcg32::new(seed, stream)
+ * (monte_carlo.rs:2247-2267). kg_sobol_points: the 52-bit scrambled Sobol
+ * point bits of points first..first+count for every dimension, point-major
+ * (out[(p - first) * dims + d]), as SobolSequence::cursor yields them
+ * (engines/monte_carlo/sobol.rs:136-243). kg_normals: normals of a block's
+ * stream as NormalDraws yields them (monte_carlo.rs:1678-1703). */
+kg_status kg_pcg32_outputs(kg_context* ctx, uint64_t seed, uint64_t stream, uint64_t first,
+                           uint64_t count, uint32_t* out);
+kg_status kg_sobol_points(kg_context* ctx, uint32_t dims, uint64_t seed, uint64_t first,
+                          uint64_t count, uint64_t* out);
+kg_status kg_pcg_normals(kg_context* ctx, uint64_t seed, uint64_t stream, uint64_t first,
+                         uint64_t count, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KIYOSI_AMD_H */

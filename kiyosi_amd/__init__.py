@@ -1,0 +1,34 @@
+"""kiyosi_amd: a Monte Carlo pricing engine for one AMD Instinct MI355X (gfx950).
+
+The engines below take the same contracts, ``PricingContext`` and pricing
+calls as kiyosi's CPU Monte Carlo engines and run them as hand-written HIP
+kernels through the C ABI in include/kiyosi_amd.h.
+"""
+
+from .engines import (
+    MonteCarloAutocallableEngine,
+    MonteCarloEstimate,
+    MonteCarloVanillaEngine,
+)
+from .errors import (
+    ComputationFailureError,
+    DeviceError,
+    InvalidInputError,
+    KiyosiError,
+    MissingHistoryError,
+    UnsupportedOperationError,
+)
+from .instruments import (
+    BinarySnowballOption,
+    CouponObservation,
+    EuropeanOption,
+    KnockInObservation,
+    KnockOutObservation,
+    PhoenixOption,
+    SnowballObservation,
+    SnowballOption,
+    TernarySnowballOption,
+)
+from .market import BlackScholesMertonParameters, PricingContext, TradingCalendar
+
+__version__ = "0.1.0"

@@ -1,0 +1,117 @@
+"""ctypes mirror of the structs and enums in include/kiyosi_amd.h."""
+
+import ctypes as C
+
+OK, INVALID_INPUT, UNSUPPORTED_OPERATION, COMPUTATION_FAILURE = 0, 1, 2, 3
+INTERRUPTED, DEVICE_ERROR, MISSING_HISTORY = 4, 5, 6
+
+PSEUDO_RANDOM, SOBOL = 0, 1
+FP64, FP32_STEP = 0, 1
+
+UNCERTAINTY_NAMES = {
+    0: "available",
+    1: "unsupported_sampling",
+    2: "insufficient_observations",
+    3: "incomplete_antithetic_pair",
+    4: "zero_observed_variance",
+}
+
+CALENDAR_WEEKDAYS, CALENDAR_ALL_DAYS = 0, 1
+KNOCK_IN_NONE, KNOCK_IN_DAILY, KNOCK_IN_AT_EXPIRY = 0, 1, 2
+REDEMPTION_DOWNSIDE, REDEMPTION_FIXED_COUPON = 0, 1
+COUPON_SNOWBALL, COUPON_PHOENIX = 0, 1
+STATE_UNKNOWN, STATE_NO_EVENT, STATE_KNOCKED_OUT, STATE_KNOCKED_IN = -1, 0, 1, 2
+
+
+class Settings(C.Structure):
+    _fields_ = [
+        ("path_count", C.c_int64),
+        ("seed", C.c_uint64),
+        ("has_seed", C.c_int32),
+        ("sampling", C.c_int32),
+        ("antithetic", C.c_int32),
+        ("precision", C.c_int32),
+        ("max_active_workers", C.c_int32),
+        ("allow_large", C.c_int32),
+    ]
+
+
+class Moments(C.Structure):
+    _fields_ = [
+        ("count", C.c_uint64),
+        ("scale", C.c_double),
+        ("mean", C.c_double),
+        ("m2", C.c_double),
+    ]
+
+
+class Estimate(C.Structure):
+    _fields_ = [
+        ("price", C.c_double),
+        ("standard_error", C.c_double),
+        ("ci_lower", C.c_double),
+        ("ci_upper", C.c_double),
+        ("has_standard_error", C.c_int32),
+        ("uncertainty_status", C.c_int32),
+        ("has_seed", C.c_int32),
+        ("reserved", C.c_int32),
+        ("seed", C.c_uint64),
+        ("path_standard_error", C.c_double),
+        ("path_steps", C.c_double),
+        ("device_ms", C.c_double),
+    ]
+
+
+class BsmMarket(C.Structure):
+    _fields_ = [
+        ("spot", C.c_double),
+        ("rate", C.c_double),
+        ("dividend", C.c_double),
+        ("volatility", C.c_double),
+        ("valuation_time", C.c_int64),
+    ]
+
+
+class EuropeanTerms(C.Structure):
+    _fields_ = [
+        ("option_type", C.c_int32),
+        ("reserved", C.c_int32),
+        ("strike", C.c_double),
+        ("effective_time", C.c_int64),
+        ("expiry_time", C.c_int64),
+    ]
+
+
+class AutocallEvent(C.Structure):
+    _fields_ = [
+        ("time", C.c_int64),
+        ("has_knock_out", C.c_int32),
+        ("has_coupon_barrier", C.c_int32),
+        ("knock_out_level", C.c_double),
+        ("coupon_barrier_level", C.c_double),
+        ("coupon_rate", C.c_double),
+    ]
+
+
+class AutocallTerms(C.Structure):
+    _fields_ = [
+        ("calendar", C.c_int32),
+        ("knock_in_kind", C.c_int32),
+        ("redemption_kind", C.c_int32),
+        ("coupon_kind", C.c_int32),
+        ("barrier_state", C.c_int32),
+        ("n_events", C.c_int32),
+        ("observation_clock_micros", C.c_int64),
+        ("knock_in_level", C.c_double),
+        ("initial_spot", C.c_double),
+        ("upper_strike", C.c_double),
+        ("lower_strike", C.c_double),
+        ("fixed_coupon_rate", C.c_double),
+        ("coupon", C.c_double),
+        ("principal_redemption_ratio", C.c_double),
+        ("rebate_rate", C.c_double),
+        ("annual_rebate_rate", C.c_double),
+        ("effective_time", C.c_int64),
+        ("expiry_time", C.c_int64),
+        ("events", C.POINTER(AutocallEvent)),
+    ]

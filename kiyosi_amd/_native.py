@@ -1,0 +1,82 @@
+"""Loads libkiyosi_amd.so (the HIP engine) and declares its C ABI.
+
+The library is built in-tree by ``python -m kiyosi_amd.build``. There is no
+CPU fallback: pricing fails loudly when the library or a GPU is missing.
+"""
+
+import ctypes as C
+import pathlib
+import threading
+
+from . import _abi
+from .errors import DeviceError, raise_for_status
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "libkiyosi_amd.so"
+
+_lib = None
+_lock = threading.Lock()
+_contexts = {}
+
+# Every symbol declared in include/kiyosi_amd.h, with (restype, argtypes).
+P = C.POINTER
+SIGNATURES = {
+    "kg_version": (C.c_char_p, []),
+    "kg_context_create": (C.c_int, [C.c_int32, P(C.c_void_p)]),
+    "kg_context_destroy": (None, [C.c_void_p]),
+    "kg_last_error": (C.c_char_p, [C.c_void_p]),
+    "kg_pcg32_outputs": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_uint32)]),
+    "kg_pcg_normals": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_double)]),
+    "kg_sobol_points": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_uint64)]),
+    "kg_vanilla_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.EuropeanTerms), P(_abi.BsmMarket),
+                                   P(_abi.Estimate)]),
+    "kg_autocallable_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AutocallTerms), P(_abi.BsmMarket),
+                                        P(_abi.Estimate)]),
+}
+
+
+def lib():
+    """Returns the loaded library, raising if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise DeviceError(
+                    f"{LIB_PATH} is missing: build it with `python -m kiyosi_amd.build`")
+            handle = C.CDLL(str(LIB_PATH))
+            for name, (restype, argtypes) in SIGNATURES.items():
+                fn = getattr(handle, name)
+                fn.restype = restype
+                fn.argtypes = argtypes
+            _lib = handle
+        return _lib
+
+
+class Context:
+    """One HIP device context (stream, workspace, Sobol cache)."""
+
+    def __init__(self, device: int = 0):
+        handle = C.c_void_p()
+        status = lib().kg_context_create(device, C.byref(handle))
+        if status:
+            raise_for_status(status, f"cannot create a context on HIP device {device}")
+        self.handle = handle
+        self.device = device
+
+    def check(self, status: int) -> None:
+        if status:
+            raise_for_status(status, lib().kg_last_error(self.handle).decode())
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.kg_context_destroy(self.handle)
+            self.handle = None
+
+
+def context(device: int = 0) -> Context:
+    with _lock:
+        ctx = _contexts.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        with _lock:
+            ctx = _contexts.setdefault(device, ctx)
+    return ctx

@@ -1,0 +1,182 @@
+// Context lifecycle, the ordered partial merge, flags, timing and the host
+// side of the scrambled Sobol setup.
+#include <algorithm>
+
+#include "host_common.hpp"
+
+extern "C" const uint32_t kg_joe_kuo_words[];
+extern "C" const uint32_t kg_joe_kuo_end[];
+
+namespace kg {
+
+// Merges per-block partials [block][slot] for `slots` independent Moments.
+// Thread t folds a contiguous run of blocks in block order; the 256 runs are
+// then merged as a fixed tree. The result depends only on the partials, not
+// on scheduling, so a seed replays exactly (monte_carlo.rs:1800-1809 merges
+// the same partials sequentially).
+__global__ void __launch_bounds__(256) merge_partials(const Moments* __restrict__ partials,
+                                                      int64_t blocks, int slots,
+                                                      Moments* __restrict__ out) {
+  __shared__ Moments smem[4];
+  const int64_t per = (blocks + blockDim.x - 1) / blockDim.x;
+  const int64_t begin = per * threadIdx.x;
+  const int64_t end = begin + per < blocks ? begin + per : blocks;
+  for (int s = 0; s < slots; ++s) {
+    Moments acc[1] = {moments_zero()};
+    for (int64_t b = begin; b < end; ++b) acc[0] = moments_merge(acc[0], partials[b * slots + s]);
+    block_merge<1>(acc, smem);
+    if (threadIdx.x == 0) out[s] = acc[0];
+  }
+}
+
+void merge_and_fetch(kg_context* ctx, const Moments* partials, int64_t blocks, int slots,
+                     Moments* host_out) {
+  Moments* d_out = (Moments*)ctx->results.reserve(sizeof(Moments) * slots);
+  hipLaunchKernelGGL(merge_partials, dim3(1), dim3(256), 0, ctx->stream, partials, blocks, slots,
+                     d_out);
+  KG_HIP(hipGetLastError());
+  KG_HIP(hipMemcpyAsync(host_out, d_out, sizeof(Moments) * slots, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  KG_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+__global__ void clear_word(uint32_t* w) { *w = 0u; }
+
+uint32_t* reset_flags(kg_context* ctx) {
+  uint32_t* f = (uint32_t*)ctx->flags.reserve(256);
+  // A kernel rather than hipMemsetAsync keeps the sequence capturable.
+  hipLaunchKernelGGL(clear_word, dim3(1), dim3(1), 0, ctx->stream, f);
+  KG_HIP(hipGetLastError());
+  return f;
+}
+
+void check_flags(kg_context* ctx) {
+  uint32_t h = 0;
+  KG_HIP(hipMemcpyAsync(&h, ctx->flags.ptr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP(hipStreamSynchronize(ctx->stream));
+  if (h & FLAG_CANCELLED) fail(KG_INTERRUPTED, "operation interrupted");
+  if (h & FLAG_NONFINITE) unstable();
+}
+
+void timing_begin(kg_context* ctx) { KG_HIP(hipEventRecord(ctx->start, ctx->stream)); }
+double timing_end(kg_context* ctx) {
+  KG_HIP(hipEventRecord(ctx->stop, ctx->stream));
+  KG_HIP(hipEventSynchronize(ctx->stop));
+  float ms = 0.f;
+  KG_HIP(hipEventElapsedTime(&ms, ctx->start, ctx->stop));
+  return ms;
+}
+
+// SobolSequence::new (engines/monte_carlo/sobol.rs:85-134): Joe-Kuo
+// recurrence to 52-bit direction integers, then linear matrix scrambling and
+// a digital shift drawn from Pcg32(seed, dimension).
+static void build_sobol(const std::vector<uint32_t>& table, uint32_t dims, uint64_t seed,
+                        std::vector<uint64_t>& dirs, std::vector<uint64_t>& shifts) {
+  constexpr int B = 52;
+  constexpr uint64_t MASK = (1ULL << B) - 1;
+  dirs.assign((size_t)dims * B, 0);
+  shifts.assign(dims, 0);
+  size_t pos = 0;
+  for (uint32_t d = 0; d < dims; ++d) {
+    uint64_t raw[B] = {};
+    if (d == 0) {
+      for (int c = 0; c < B; ++c) raw[c] = 1ULL << (B - 1 - c);
+    } else {
+      if (pos + 2 > table.size()) fail(KG_INVALID_INPUT, "Sobol direction table is truncated");
+      const int degree = (int)table[pos];
+      const uint32_t poly = table[pos + 1];
+      for (int c = 0; c < degree && c < B; ++c) raw[c] = (uint64_t)table[pos + 2 + c] << (B - 1 - c);
+      pos += (size_t)degree + 2;
+      for (int c = degree; c < B; ++c) {
+        uint64_t v = raw[c - degree] ^ (raw[c - degree] >> degree);
+        for (int k = 1; k < degree; ++k)
+          if ((poly >> (degree - 1 - k)) & 1u) v ^= raw[c - k];
+        raw[c] = v;
+      }
+    }
+    Pcg32 g;
+    g.init(seed, d);
+    auto word = [&g]() {
+      uint64_t hi = g.next();
+      return (hi << 32) | (uint64_t)g.next();
+    };
+    shifts[d] = word() & MASK;
+    uint64_t rows[B];
+    for (int row = 0; row < B; ++row) {
+      const int bit = B - 1 - row;
+      rows[row] = (word() & (MASK << (bit + 1))) | (1ULL << bit);
+    }
+    for (int c = 0; c < B; ++c) {
+      uint64_t v = 0;
+      for (int row = 0; row < B; ++row)
+        v |= (uint64_t)(__builtin_popcountll(raw[c] & rows[row]) & 1) << (B - 1 - row);
+      dirs[(size_t)c * dims + d] = v;
+    }
+  }
+}
+
+const SobolSetup& sobol_setup(kg_context* ctx, uint32_t dims, uint64_t seed) {
+  SobolSetup& s = ctx->sobol;
+  if (s.valid && s.dims == dims && s.seed == seed) return s;
+  if (ctx->joe_kuo.empty())
+    ctx->joe_kuo.assign(kg_joe_kuo_words, kg_joe_kuo_end);
+  s.valid = false;
+  build_sobol(ctx->joe_kuo, dims, seed, s.directions, s.shifts);
+  void* dd = s.d_directions.reserve(s.directions.size() * sizeof(uint64_t));
+  void* ds = s.d_shifts.reserve(s.shifts.size() * sizeof(uint64_t));
+  KG_HIP(hipMemcpyAsync(dd, s.directions.data(), s.directions.size() * sizeof(uint64_t),
+                        hipMemcpyHostToDevice, ctx->stream));
+  KG_HIP(hipMemcpyAsync(ds, s.shifts.data(), s.shifts.size() * sizeof(uint64_t),
+                        hipMemcpyHostToDevice, ctx->stream));
+  KG_HIP(hipStreamSynchronize(ctx->stream));
+  s.dims = dims;
+  s.seed = seed;
+  s.valid = true;
+  return s;
+}
+
+}  // namespace kg
+
+extern "C" {
+
+const char* kg_version(void) { return "kiyosi_amd 0.1.0 (gfx950)"; }
+
+kg_status kg_context_create(int32_t device, kg_context** out) {
+  if (!out) return KG_INVALID_INPUT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return KG_DEVICE_ERROR;
+  if (device < 0 || device >= count) return KG_INVALID_INPUT;
+  kg_context* ctx = new (std::nothrow) kg_context();
+  if (!ctx) return KG_COMPUTATION_FAILURE;
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
+      hipEventCreate(&ctx->start) != hipSuccess || hipEventCreate(&ctx->stop) != hipSuccess) {
+    delete ctx;
+    return KG_DEVICE_ERROR;
+  }
+  *out = ctx;
+  return KG_OK;
+}
+
+void kg_context_destroy(kg_context* ctx) {
+  if (!ctx) return;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->partials.release();
+    ctx->scratch.release();
+    ctx->flags.release();
+    ctx->results.release();
+    ctx->sobol.d_directions.release();
+    ctx->sobol.d_shifts.release();
+    if (ctx->start) (void)hipEventDestroy(ctx->start);
+    if (ctx->stop) (void)hipEventDestroy(ctx->stop);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+}
+
+const char* kg_last_error(const kg_context* ctx) { return ctx ? ctx->error.c_str() : ""; }
+}

@@ -1,0 +1,320 @@
+// Device building blocks shared by the Monte Carlo kernels (gfx950, wave64).
+//
+// Generators restate kiyosi 0.23.0 bit for bit:
+//   Pcg32            crates/kiyosi/src/engines/monte_carlo.rs:2224-2277
+//   Box-Muller       monte_carlo.rs:1830-1894 (sin_cos_turns polynomial)
+//   Cephes ndtri     engines/monte_carlo/sobol.rs:249-346
+//   Moments          monte_carlo.rs:2155-2222 (merge formula, applied as a tree)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kg {
+
+constexpr int BLOCK_SAMPLES = 256;  // monte_carlo.rs:20
+constexpr int WAVE = 64;
+
+// ------------------------------------------------------------------ PCG32
+struct Pcg32 {
+  uint64_t state, inc;
+  static constexpr uint64_t MUL = 6364136223846793005ULL;
+  __host__ __device__ inline void init(uint64_t seed, uint64_t stream) {
+    state = 0;
+    inc = (stream << 1) | 1;
+    step();
+    state += seed;
+    step();
+  }
+  __host__ __device__ inline void step() { state = state * MUL + inc; }
+  __host__ __device__ inline uint32_t next() {
+    uint64_t old = state;
+    state = old * MUL + inc;
+    uint32_t x = (uint32_t)(((old >> 18) ^ old) >> 27);
+    uint32_t r = (uint32_t)(old >> 59);
+    return (x >> r) | (x << ((32u - r) & 31u));
+  }
+  // O(log n) affine jump of the LCG (monte_carlo.rs:2233-2245).
+  __host__ __device__ inline void advance(uint64_t distance) {
+    uint64_t m = MUL, c = inc;
+    while (distance) {
+      if (distance & 1) state = state * m + c;
+      c = c * (m + 1);
+      m = m * m;
+      distance >>= 1;
+    }
+  }
+  // 52-bit uniform strictly inside (0, 1) (monte_carlo.rs:2272-2276).
+  __device__ inline double uniform() {
+    uint64_t hi = next();
+    uint64_t lo = next();
+    uint64_t bits = (hi << 20) | (lo >> 12);
+    return ((double)bits + 0.5) * 0x1p-52;
+  }
+};
+
+// ------------------------------------------------------------------ Box-Muller
+// Sine and cosine of 2*pi*turns by quadrant reduction in turns and Taylor
+// polynomials of degree 15/16 on [-pi/4, pi/4] (monte_carlo.rs:1852-1894).
+__device__ inline void sin_cos_turns(double turns, double* s, double* c) {
+  double q = turns * 4.0 + 0.5;
+  uint32_t quadrant = (uint32_t)q;  // turns lies in (0, 1): q in (0.5, 4.5)
+  double angle = 6.283185307179586 * (turns - 0.25 * (double)quadrant);
+  double sq = angle * angle;
+  double ps = -1.0 / 1307674368000.0;
+  ps = ps * sq + 1.0 / 6227020800.0;
+  ps = ps * sq + -1.0 / 39916800.0;
+  ps = ps * sq + 1.0 / 362880.0;
+  ps = ps * sq + -1.0 / 5040.0;
+  ps = ps * sq + 1.0 / 120.0;
+  ps = ps * sq + -1.0 / 6.0;
+  double pc = 1.0 / 20922789888000.0;
+  pc = pc * sq + -1.0 / 87178291200.0;
+  pc = pc * sq + 1.0 / 479001600.0;
+  pc = pc * sq + -1.0 / 3628800.0;
+  pc = pc * sq + 1.0 / 40320.0;
+  pc = pc * sq + -1.0 / 720.0;
+  pc = pc * sq + 1.0 / 24.0;
+  pc = pc * sq + -0.5;
+  double sn = angle + angle * sq * ps;
+  double cs = 1.0 + sq * pc;
+  if (quadrant & 1) {
+    double t = sn;
+    sn = cs;
+    cs = -t;
+  }
+  if (quadrant & 2) {
+    sn = -sn;
+    cs = -cs;
+  }
+  *s = sn;
+  *c = cs;
+}
+
+// One Box-Muller pair: normal 2k is r*cos, normal 2k+1 is r*sin.
+__device__ inline void box_muller_pair(Pcg32& g, double* even, double* odd) {
+  double radius = sqrt(-2.0 * log(g.uniform()));
+  double s, c;
+  sin_cos_turns(g.uniform(), &s, &c);
+  *even = radius * c;
+  *odd = radius * s;
+}
+
+// Stream of normals for one sample: normal index k of a block stream comes
+// from Box-Muller pair k/2, i.e. PCG outputs 4*(k/2) .. 4*(k/2)+3
+// (monte_carlo.rs:1689-1703, 1830-1845). Positioning is by skip-ahead, so a
+// thread can start at any sample of the block.
+struct NormalStream {
+  Pcg32 g;
+  double spare;
+  bool has_spare;
+  __device__ inline void start(uint64_t seed, uint64_t block, uint64_t first_normal) {
+    g.init(seed, block);
+    g.advance((first_normal >> 1) * 4);
+    has_spare = false;
+    if (first_normal & 1) {
+      double e;
+      box_muller_pair(g, &e, &spare);
+      has_spare = true;
+    }
+  }
+  __device__ inline double next() {
+    if (has_spare) {
+      has_spare = false;
+      return spare;
+    }
+    double e;
+    box_muller_pair(g, &e, &spare);
+    has_spare = true;
+    return e;
+  }
+  // Moves the stream forward by `count` normals without drawing them.
+  __device__ inline void skip(uint64_t count) {
+    if (count == 0) return;
+    if (has_spare) {
+      has_spare = false;
+      count -= 1;
+    }
+    g.advance((count >> 1) * 4);
+    if (count & 1) next();
+  }
+};
+
+// ------------------------------------------------------------------ ndtri
+// Cephes ndtri in FP64 (engines/monte_carlo/sobol.rs:262-346).
+__device__ inline double inverse_normal_cdf(double p) {
+  const double TAIL = 0.1353352832366127;
+  bool upper = p > 1.0 - TAIL;
+  double tail = upper ? 1.0 - p : p;
+  if (tail > TAIL) {
+    double c = tail - 0.5;
+    double s = c * c;
+    double num = -59.96335010141079;
+    num = num * s + 98.00107541859997;
+    num = num * s + -56.67628574690703;
+    num = num * s + 13.931260938727968;
+    num = num * s + -1.2391658386738125;
+    double den = 1.0;
+    den = den * s + 1.9544885833814176;
+    den = den * s + 4.676279128988815;
+    den = den * s + 86.36024213908905;
+    den = den * s + -225.46268785411937;
+    den = den * s + 200.26021238006066;
+    den = den * s + -82.03722561683334;
+    den = den * s + 15.90562251262117;
+    den = den * s + -1.1833162112133;
+    return (c + c * s * num / den) * 2.5066282746310002;
+  }
+  double root = sqrt(-2.0 * log(tail));
+  double inv = 1.0 / root;
+  double num, den;
+  if (root < 8.0) {
+    num = 4.0554489230596245;
+    num = num * inv + 31.525109459989388;
+    num = num * inv + 57.16281922464213;
+    num = num * inv + 44.08050738932008;
+    num = num * inv + 14.684956192885803;
+    num = num * inv + 2.1866330685079025;
+    num = num * inv + -0.1402560791713545;
+    num = num * inv + -0.03504246268278482;
+    num = num * inv + -0.0008574567851546854;
+    den = 1.0;
+    den = den * inv + 15.779988325646675;
+    den = den * inv + 45.39076351288792;
+    den = den * inv + 41.3172038254672;
+    den = den * inv + 15.04253856929075;
+    den = den * inv + 2.504649462083094;
+    den = den * inv + -0.14218292285478779;
+    den = den * inv + -0.03808064076915783;
+    den = den * inv + -0.0009332594808954574;
+  } else {
+    num = 3.2377489177694603;
+    num = num * inv + 6.915228890689842;
+    num = num * inv + 3.9388102529247444;
+    num = num * inv + 1.3330346081580755;
+    num = num * inv + 0.20148538954917908;
+    num = num * inv + 0.012371663481782003;
+    num = num * inv + 0.00030158155350823543;
+    num = num * inv + 2.6580697468673755e-06;
+    num = num * inv + 6.239745391849833e-09;
+    den = 1.0;
+    den = den * inv + 6.02427039364742;
+    den = den * inv + 3.6798356385616087;
+    den = den * inv + 1.3770209948908132;
+    den = den * inv + 0.21623699359449663;
+    den = den * inv + 0.013420400608854318;
+    den = den * inv + 0.00032801446468212774;
+    den = den * inv + 2.8924786474538068e-06;
+    den = den * inv + 6.790194080099813e-09;
+  }
+  double q = root - log(root) / root - inv * (num / den);
+  return upper ? q : -q;
+}
+__device__ inline double normal_from_bits(uint64_t bits) {
+  return inverse_normal_cdf(((double)bits + 0.5) * 0x1p-52);
+}
+
+// ------------------------------------------------------------------ Moments
+// Scale-normalised count/mean/M2 with the reference's add and merge rules.
+struct Moments {
+  double count;  // exact below 2^53
+  double scale, mean, m2;
+};
+
+__device__ inline Moments moments_zero() { return Moments{0.0, 0.0, 0.0, 0.0}; }
+
+__device__ inline void moments_add(Moments& m, double value, bool variance) {
+  double a = fabs(value);
+  if (a > m.scale) {
+    double ratio = m.scale / a;
+    m.mean *= ratio;
+    if (variance) m.m2 *= ratio * ratio;
+    m.scale = a;
+  }
+  m.count += 1.0;
+  double v = m.scale == 0.0 ? 0.0 : value / m.scale;
+  double delta = v - m.mean;
+  m.mean += delta / m.count;
+  if (variance) m.m2 += delta * (v - m.mean);
+}
+
+__device__ inline Moments moments_merge(const Moments& a, const Moments& b) {
+  if (a.count == 0.0) return b;
+  if (b.count == 0.0) return a;
+  double sc = fmax(a.scale, b.scale);
+  double left = sc == 0.0 ? 0.0 : a.scale / sc;
+  double right = sc == 0.0 ? 0.0 : b.scale / sc;
+  double delta = b.mean * right - a.mean * left;
+  double n = a.count + b.count;
+  double wb = b.count / n;
+  Moments r;
+  r.mean = a.mean * left + delta * wb;
+  r.m2 = a.m2 * left * left + b.m2 * right * right + delta * delta * (a.count * wb);
+  r.count = n;
+  r.scale = sc;
+  return r;
+}
+
+__device__ inline Moments moments_shfl_xor(const Moments& m, int mask) {
+  Moments r;
+  r.count = __shfl_xor(m.count, mask, WAVE);
+  r.scale = __shfl_xor(m.scale, mask, WAVE);
+  r.mean = __shfl_xor(m.mean, mask, WAVE);
+  r.m2 = __shfl_xor(m.m2, mask, WAVE);
+  return r;
+}
+
+// Merges the Moments of all lanes of a wave; the lower lane of each pair is
+// the left operand, so the order is fixed by lane index.
+__device__ inline Moments wave_merge(Moments m) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  for (int mask = 1; mask < WAVE; mask <<= 1) {
+    Moments o = moments_shfl_xor(m, mask);
+    m = (lane & mask) ? moments_merge(o, m) : moments_merge(m, o);
+  }
+  return m;
+}
+
+// Block-wide merge of N independent Moments sets; returns the result in
+// thread 0. `smem` needs N * (blockDim.x / 64) entries.
+template <int N>
+__device__ inline void block_merge(Moments (&m)[N], Moments* smem) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = threadIdx.x / WAVE;
+  const int waves = blockDim.x / WAVE;
+  for (int i = 0; i < N; ++i) m[i] = wave_merge(m[i]);
+  if (lane == 0)
+    for (int i = 0; i < N; ++i) smem[i * waves + wave] = m[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < N; ++i) {
+      Moments acc = smem[i * waves];
+      for (int w = 1; w < waves; ++w) acc = moments_merge(acc, smem[i * waves + w]);
+      m[i] = acc;
+    }
+  }
+  __syncthreads();
+}
+
+// f64::midpoint for finite inputs well inside the exponent range.
+__device__ inline double midpoint(double a, double b) {
+  const double HI = 8.98846567431158e307;  // f64::MAX / 2
+  if (fabs(a) <= HI && fabs(b) <= HI) return (a + b) / 2.0;
+  return a / 2.0 + b / 2.0;
+}
+
+// discounted_product (crates/kiyosi/src/numerics.rs:226-238).
+__device__ inline double discounted_product(double scale, double weight, double log_discount) {
+  double value = scale * weight;
+  double discount = exp(log_discount);
+  if (scale == 0.0 || weight == 0.0) return 0.0;
+  if (isnormal(discount) && isnormal(value)) return value * discount;
+  return copysign(exp(log(scale) + log(fabs(weight)) + log_discount), weight);
+}
+
+// Flags raised by kernels and read back by the host.
+enum : uint32_t { FLAG_NONFINITE = 1u, FLAG_CANCELLED = 2u };
+
+__device__ inline void raise_flag(uint32_t* flags, uint32_t bit) { atomicOr(flags, bit); }
+
+}  // namespace kg

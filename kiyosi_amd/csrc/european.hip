@@ -1,0 +1,208 @@
+// European vanilla engine: MonteCarloVanillaEngine (monte_carlo.rs:832-1329).
+//
+// Plan building restates EuropeanPayoff::new (:1165-1223) on the host; the
+// kernel restates TerminalPayoff::at_normal (:1238-1252) over the block
+// sampler of sample_paths (:1761-1811): one workgroup per 256-sample block,
+// PCG stream = block index, normal j of the block for sample j (one
+// dimension), antithetic samples paired while the sample index is below
+// paths/2. Each thread owns two consecutive samples, i.e. one Box-Muller pair.
+#include "host_common.hpp"
+#include "fp32_math.hpp"
+
+namespace kg {
+
+struct EuroParams {
+  double sign, diffusion, half_variance, moneyness, log_scale;
+  double peak_shift;  // log scale of the second peak-scaling pass
+  int mode;           // 0: payoff, 1: peak pass (:1118-1125), 2: rescaled pass (:1127-1129)
+};
+
+__device__ inline double euro_value(const EuroParams& p, double z, uint32_t& bad) {
+  double distance = fma(p.diffusion, z, p.moneyness);
+  if (p.mode == 0) {
+    double v;
+    if (p.sign * distance <= 0.0)
+      v = 0.0;
+    else if (p.sign < 0.0)
+      v = -expm1(distance);
+    else
+      v = -expm1(-distance) * exp(fma(p.diffusion, z, -p.half_variance));
+    if (!isfinite(v)) bad = 1;
+    return v;
+  }
+  double lv = distance > 0.0
+                  ? fma(p.diffusion, z, p.log_scale - p.half_variance) + log(-expm1(-distance))
+                  : -INFINITY;
+  if (p.mode == 1) {
+    if (lv > 1.7976931348623157e308 || isnan(lv)) bad = 1;
+    return fmax(lv + 746.0, 0.0);
+  }
+  return exp(lv - p.peak_shift);
+}
+
+// FP32 stepping: the payoff of one normal in single precision; the value
+// leaves in FP64 for accumulation.
+__device__ inline double euro_value_fp32(const EuroParams& p, float z, uint32_t& bad) {
+  float diffusion = (float)p.diffusion;
+  float distance = fmaf(diffusion, z, (float)p.moneyness);
+  float v;
+  if (p.sign * (double)distance <= 0.0)
+    v = 0.f;
+  else if (p.sign < 0.0)
+    v = -f32::expm1(distance);
+  else
+    v = -f32::expm1(-distance) * f32::exp(fmaf(diffusion, z, -(float)p.half_variance));
+  if (!isfinite(v)) bad = 1;
+  return (double)v;
+}
+
+__device__ inline uint64_t sobol_dim0(const uint64_t* __restrict__ dirs, uint64_t shift,
+                                      uint64_t n) {
+  uint64_t gray = n ^ (n >> 1);
+  uint64_t v = shift;
+  while (gray) {
+    v ^= dirs[__builtin_ctzll(gray)];
+    gray &= gray - 1;
+  }
+  return v;
+}
+
+template <bool FP32>
+__global__ void __launch_bounds__(128)
+    european_kernel(EuroParams p, uint64_t seed, int64_t samples, int64_t paths, int antithetic,
+                    int sobol, const uint64_t* __restrict__ dirs, uint64_t shift,
+                    Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
+  __shared__ Moments smem[2 * 2];
+  const int64_t block = blockIdx.x;
+  const int j = 2 * threadIdx.x;
+  const int64_t n0 = block * BLOCK_SAMPLES + j;
+  Moments m[2] = {moments_zero(), moments_zero()};
+  uint32_t bad = 0;
+  if (n0 < samples) {
+    double z[2];
+    float zf[2];
+    if (sobol) {
+      for (int s = 0; s < 2; ++s) {
+        uint64_t bits = sobol_dim0(dirs, shift, (uint64_t)(n0 + s));
+        if (FP32)
+          zf[s] = f32::normal_from_bits((uint32_t)(bits >> 20));
+        else
+          z[s] = normal_from_bits(bits);
+      }
+    } else {
+      Pcg32 g;
+      g.init(seed, (uint64_t)block);
+      g.advance((uint64_t)j * 2);  // pair j/2 starts at output 4*(j/2)
+      if (FP32)
+        f32::box_muller_pair(g, &zf[0], &zf[1]);
+      else
+        box_muller_pair(g, &z[0], &z[1]);
+    }
+    for (int s = 0; s < 2; ++s) {
+      const int64_t n = n0 + s;
+      if (n >= samples) break;
+      double pos = FP32 ? euro_value_fp32(p, zf[s], bad) : euro_value(p, z[s], bad);
+      moments_add(m[0], pos, true);
+      if (antithetic && n < paths / 2) {
+        double neg = FP32 ? euro_value_fp32(p, -zf[s], bad) : euro_value(p, -z[s], bad);
+        moments_add(m[0], neg, true);
+        moments_add(m[1], midpoint(pos, neg), true);
+      }
+    }
+  }
+  if (bad) raise_flag(flags, FLAG_NONFINITE);
+  block_merge<2>(m, smem);
+  if (threadIdx.x == 0) {
+    partials[block * 2 + 0] = m[0];
+    partials[block * 2 + 1] = m[1];
+  }
+}
+
+// One pass of the sampler: returns total and pair moments.
+static void european_pass(kg_context* ctx, const Sampler& s, const EuroParams& p,
+                          Moments out[2], double* device_ms) {
+  const int64_t blocks = s.blocks();
+  Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 2 * blocks);
+  uint32_t* flags = reset_flags(ctx);
+  const uint64_t* dirs = nullptr;
+  uint64_t shift = 0;
+  if (s.sobol) {
+    const SobolSetup& so = sobol_setup(ctx, 1, s.seed);
+    dirs = (const uint64_t*)so.d_directions.ptr;
+    shift = so.shifts[0];
+  }
+  timing_begin(ctx);
+  if (s.fp32)
+    hipLaunchKernelGGL(european_kernel<true>, dim3(blocks), dim3(128), 0, ctx->stream, p, s.seed,
+                       s.samples(), s.path_count, (int)s.antithetic, (int)s.sobol, dirs, shift,
+                       parts, flags);
+  else
+    hipLaunchKernelGGL(european_kernel<false>, dim3(blocks), dim3(128), 0, ctx->stream, p, s.seed,
+                       s.samples(), s.path_count, (int)s.antithetic, (int)s.sobol, dirs, shift,
+                       parts, flags);
+  KG_HIP(hipGetLastError());
+  *device_ms += timing_end(ctx);
+  check_flags(ctx);
+  merge_and_fetch(ctx, parts, blocks, 2, out);
+}
+
+}  // namespace kg
+
+using namespace kg;
+
+extern "C" kg_status kg_vanilla_price(kg_context* ctx, const kg_settings* settings,
+                                      const kg_european_terms* option,
+                                      const kg_bsm_market* market, kg_estimate* out) {
+  return guarded(ctx, [&] {
+    if (!option || !market || !out) fail(KG_INVALID_INPUT, "arguments must not be null");
+    Sampler s = validate_settings(settings, 1);
+    const kg_european_terms& o = *option;
+    const kg_bsm_market& m = *market;
+    if (o.option_type != 1 && o.option_type != -1)
+      fail(KG_INVALID_INPUT, "option_type must be 1 (call) or -1 (put)");
+    if (!(m.spot > 0.0) || !std::isfinite(m.spot)) fail(KG_INVALID_INPUT, "spot must be positive");
+    if (!(m.volatility >= 0.0)) fail(KG_INVALID_INPUT, "volatility must be nonnegative");
+    if (!(o.strike > 0.0)) fail(KG_INVALID_INPUT, "strike must be positive");
+    // validate_life (instruments/monitoring.rs:470-481)
+    if (m.valuation_time < o.effective_time || m.valuation_time > o.expiry_time)
+      fail(KG_INVALID_INPUT, "valuation outside contract life");
+    const double sign = o.option_type;
+    const double time = (double)(o.expiry_time - m.valuation_time) / (365.0 * 86400e6);
+    *out = kg_estimate{};
+    if (time == 0.0) {  // monte_carlo.rs:1188-1192
+      fill_known(std::max(sign * (m.spot - o.strike), 0.0), out);
+      return;
+    }
+    if (m.volatility == 0.0) {  // deterministic::vanilla_to_expiry for flat curves
+      double fwd = m.spot * std::exp(-m.dividend * time) - o.strike * std::exp(-m.rate * time);
+      fill_known(std::max(sign * fwd, 0.0), out);
+      return;
+    }
+    EuroParams p{};
+    p.sign = sign;
+    p.diffusion = m.volatility * std::sqrt(time);
+    const double carry = (m.rate - m.dividend) * time;
+    p.half_variance = (0.5 * p.diffusion) * p.diffusion;
+    const double drift = carry - p.half_variance;
+    if (!std::isfinite(drift) || !std::isfinite(p.diffusion)) unstable();
+    p.moneyness = log_ratio(m.spot, o.strike) + carry - p.half_variance;
+    p.log_scale = sign > 0.0 ? std::log(m.spot) - m.dividend * time
+                             : std::log(o.strike) - m.rate * time;
+    double log_scale = p.log_scale;
+    Moments r[2];
+    double ms = 0.0;
+    if (sign > 0.0 && p.diffusion > 25.0) {
+      // Peak scaling (monte_carlo.rs:1104-1131): a first pass finds the
+      // largest log payoff, a second pass accumulates in units of it.
+      p.mode = 1;
+      european_pass(ctx, s, p, r, &ms);
+      log_scale = r[0].scale - 746.0;
+      p.mode = 2;
+      p.peak_shift = log_scale;
+    }
+    european_pass(ctx, s, p, r, &ms);
+    out->device_ms = ms;
+    out->path_steps = (double)s.path_count;
+    fill_estimate(s, r[0], r[1], log_scale, m.valuation_time == o.expiry_time, true, out);
+  });
+}

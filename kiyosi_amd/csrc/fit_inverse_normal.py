@@ -1,0 +1,73 @@
+"""Fits the single-precision inverse normal used by the FP32 path kernels.
+
+Form (after M. Giles, "Approximating the erfinv function", GPU Computing Gems
+Jade Edition, 2011, which writes erfinv(x) = x * P(w) with
+w = -ln((1 - x)(1 + x))): for p in (0, 1) let x = 2p - 1 and
+w = -ln(4 p (1 - p)); then
+
+    Phi^-1(p) = x * P_c(w - 2.5)          if w < 5
+    Phi^-1(p) = x * P_t(sqrt(w) - 3)      otherwise
+
+with degree-8 polynomials fitted here, by least squares on Chebyshev nodes,
+to SciPy's double-precision ndtri. The coefficients are this script's own fit,
+printed as C++ for fp32_math.hpp. Run: python fit_inverse_normal.py
+"""
+
+import numpy as np
+from scipy.special import ndtri
+
+DEGREE = 8
+W_SPLIT = 5.0
+P_MIN = 2.0 ** -33  # smallest midpoint of a 32-bit uniform
+
+
+def target(w):
+    # p < 1/2 branch: 1 - x^2 = exp(-w), x = -sqrt(-expm1(-w)).
+    x = -np.sqrt(-np.expm1(-w))
+    p = 0.5 * (1.0 + x)
+    return ndtri(p) / x
+
+
+def cheb_nodes(a, b, n):
+    k = np.arange(n)
+    return 0.5 * (a + b) + 0.5 * (b - a) * np.cos((2 * k + 1) * np.pi / (2 * n))
+
+
+def fit(u, values):
+    coeffs = np.polynomial.polynomial.polyfit(u, values, DEGREE)
+    return coeffs[::-1]  # highest degree first, for Horner
+
+
+def main():
+    w_max = -np.log(4.0 * P_MIN * (1.0 - P_MIN))
+    wc = cheb_nodes(1e-12, W_SPLIT, 4000)
+    central = fit(wc - 2.5, target(wc))
+    wt = cheb_nodes(W_SPLIT, w_max * 1.01, 4000)
+    tail = fit(np.sqrt(wt) - 3.0, target(wt))
+
+    # Float32 evaluation check against ndtri on a dense grid of 32-bit inputs.
+    bits = np.unique(np.concatenate([
+        np.arange(0, 1 << 16, dtype=np.uint64),
+        np.linspace(0, 2**32 - 1, 2_000_001).astype(np.uint64),
+        (2**32 - 1 - np.arange(0, 1 << 16, dtype=np.uint64)),
+    ]))
+    p32 = ((bits.astype(np.float64) + 0.5) * 2.0 ** -32).astype(np.float32)
+    q32 = ((2.0**32 - 1 - bits.astype(np.float64) + 0.5) * 2.0 ** -32).astype(np.float32)
+    x = (p32 - q32).astype(np.float32)
+    w = (-np.log(np.float32(4) * p32 * q32)).astype(np.float32)
+    c32 = central.astype(np.float32)
+    t32 = tail.astype(np.float32)
+    pc = np.polyval(c32, (w - np.float32(2.5)).astype(np.float32)).astype(np.float32)
+    pt = np.polyval(t32, (np.sqrt(w) - np.float32(3)).astype(np.float32)).astype(np.float32)
+    z = np.where(w < W_SPLIT, pc, pt).astype(np.float32) * x
+    exact = ndtri((bits.astype(np.float64) + 0.5) * 2.0 ** -32)
+    err = np.abs(z.astype(np.float64) - exact)
+    rel = err / np.maximum(np.abs(exact), 1e-3)
+    print(f"// max abs error {err.max():.3e}, max rel error {rel.max():.3e} over {bits.size} inputs")
+    for name, c in (("CENTRAL", central), ("TAIL", tail)):
+        body = ", ".join(f"{v:.9e}f" for v in c.astype(np.float32))
+        print(f"constexpr float {name}[{DEGREE + 1}] = {{{body}}};")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,92 @@
+// Single-precision path math on the gfx950 transcendental unit.
+//
+// The FP32 stepping kernels draw the same integers as the FP64 kernels (same
+// PCG outputs, same Sobol point bits) and map them to normals in FP32:
+//   * Box-Muller with v_log_f32 / v_sqrt_f32 and v_sin_f32 / v_cos_f32,
+//     whose inputs are in turns, so the reference's turn-based angle
+//     (monte_carlo.rs:1852-1894) maps onto them directly;
+//   * an inverse normal of the form x * P(w), w = -ln(4 p (1 - p)), after
+//     M. Giles (GPU Computing Gems Jade Edition, 2011), with coefficients
+//     fitted by kiyosi_amd/csrc/fit_inverse_normal.py (max abs error 2.3e-6
+//     over all 32-bit inputs, including both tails).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.hpp"
+
+namespace kg {
+namespace f32 {
+
+constexpr float LN2 = 0.693147180559945309f;
+constexpr float LOG2E = 1.44269504088896341f;
+
+__device__ inline float log(float x) { return __builtin_amdgcn_logf(x) * LN2; }
+__device__ inline float exp(float x) { return __builtin_amdgcn_exp2f(x * LOG2E); }
+__device__ inline float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// exp(x) - 1 without cancellation for |x| < 1/2 (Taylor to x^9).
+__device__ inline float expm1(float x) {
+  if (fabsf(x) < 0.5f) {
+    float p = 1.0f / 362880.0f;
+    p = fmaf(p, x, 1.0f / 40320.0f);
+    p = fmaf(p, x, 1.0f / 5040.0f);
+    p = fmaf(p, x, 1.0f / 720.0f);
+    p = fmaf(p, x, 1.0f / 120.0f);
+    p = fmaf(p, x, 1.0f / 24.0f);
+    p = fmaf(p, x, 1.0f / 6.0f);
+    p = fmaf(p, x, 0.5f);
+    return fmaf(p * x, x, x);
+  }
+  return exp(x) - 1.0f;
+}
+
+// Inverse normal CDF of the midpoint of a 32-bit uniform:
+// p = (bits + 1/2) / 2^32.
+__device__ inline float normal_from_bits(uint32_t bits) {
+  constexpr float CENTRAL[9] = {3.293115824e-08f,  4.844033015e-07f, -4.897925464e-06f,
+                                -6.199391464e-06f, 3.087891673e-04f, -1.773068332e-03f,
+                                -5.907721352e-03f, 3.488027155e-01f, 2.123313427e+00f};
+  constexpr float TAIL[9] = {1.557544238e-06f,  -2.612999815e-04f, 1.683062175e-03f,
+                             -4.751312081e-03f, 8.147120476e-03f,  -1.092399564e-02f,
+                             1.335252635e-02f,  1.416593313e+00f,  4.006433964e+00f};
+  const bool upper = (bits >> 31) != 0;
+  const uint32_t k = upper ? ~bits : bits;            // distance to the nearer end
+  const float ps = fmaf((float)k, 0x1p-32f, 0x1p-33f);  // min(p, 1 - p) in (0, 1/2]
+  const float ax = fmaf(-2.0f, ps, 1.0f);              // |2p - 1|
+  const float w = -log(ps * fmaf(-4.0f, ps, 4.0f));    // -ln(4 p (1 - p))
+  float r;
+  if (w < 5.0f) {
+    const float u = w - 2.5f;
+    r = CENTRAL[0];
+#pragma unroll
+    for (int i = 1; i < 9; ++i) r = fmaf(r, u, CENTRAL[i]);
+  } else {
+    const float u = sqrt(w) - 3.0f;
+    r = TAIL[0];
+#pragma unroll
+    for (int i = 1; i < 9; ++i) r = fmaf(r, u, TAIL[i]);
+  }
+  return upper ? ax * r : -ax * r;
+}
+
+// Uniform midpoint in (0, 1) from the high PCG word of a 52-bit uniform.
+__device__ inline float uniform_hi(uint32_t hi) { return fmaf((float)hi, 0x1p-32f, 0x1p-33f); }
+
+// One Box-Muller pair from four PCG outputs, consumed exactly as the FP64
+// form consumes them (two 52-bit uniforms), so both forms stay on one stream.
+__device__ inline void box_muller_pair(Pcg32& g, float* even, float* odd) {
+  const uint32_t h1 = g.next();
+  (void)g.next();
+  const uint32_t h2 = g.next();
+  (void)g.next();
+  const float radius = sqrt(-2.0f * log(uniform_hi(h1)));
+  const float turns = uniform_hi(h2);
+  *even = radius * __builtin_amdgcn_cosf(turns);
+  *odd = radius * __builtin_amdgcn_sinf(turns);
+}
+
+}  // namespace f32
+}  // namespace kg

@@ -1,0 +1,252 @@
+// Host-side runtime shared by the engine entry points: the context (device,
+// stream, workspace, Sobol setup cache), error propagation to the C ABI and
+// the estimate logic of MonteCarloEstimate (monte_carlo.rs:859-1001,
+// 1412-1452).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/kiyosi_amd.h"
+#include "device_common.hpp"
+
+namespace kg {
+
+struct Error : std::runtime_error {
+  kg_status status;
+  Error(kg_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+[[noreturn]] inline void fail(kg_status s, const std::string& m) { throw Error(s, m); }
+[[noreturn]] inline void unstable() {
+  fail(KG_COMPUTATION_FAILURE, "Monte Carlo produced a non-finite parameter, path or result");
+}
+
+#define KG_HIP(call)                                                                     \
+  do {                                                                                   \
+    hipError_t kg_err_ = (call);                                                         \
+    if (kg_err_ != hipSuccess)                                                           \
+      ::kg::fail(KG_DEVICE_ERROR, std::string(#call) + ": " + hipGetErrorString(kg_err_)); \
+  } while (0)
+
+// A grow-only device buffer owned by a context.
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void* reserve(size_t n) {
+    if (n > bytes) {
+      if (ptr) KG_HIP(hipFree(ptr));
+      ptr = nullptr;
+      bytes = 0;
+      KG_HIP(hipMalloc(&ptr, n));
+      bytes = n;
+    }
+    return ptr;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+};
+
+// Host-built scrambled Sobol directions for (dims, seed), cached like the
+// reference's request cache (engines/monte_carlo/sobol.rs:12-69).
+struct SobolSetup {
+  uint32_t dims = 0;
+  uint64_t seed = 0;
+  bool valid = false;
+  std::vector<uint64_t> directions;  // [column * dims + dim], 52 columns
+  std::vector<uint64_t> shifts;      // [dim]
+  DeviceBuffer d_directions, d_shifts;
+};
+
+}  // namespace kg
+
+struct kg_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t start = nullptr, stop = nullptr;
+  std::mutex mu;
+  std::string error;
+  kg::DeviceBuffer partials, scratch, flags, results;
+  kg::SobolSetup sobol;
+  std::vector<uint32_t> joe_kuo;  // direction-number records, loaded once
+};
+
+namespace kg {
+
+// Runs `body` with the context's device current and its lock held,
+// translating exceptions into status codes and messages.
+template <class F>
+kg_status guarded(kg_context* ctx, F&& body) {
+  if (!ctx) return KG_INVALID_INPUT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  try {
+    KG_HIP(hipSetDevice(ctx->device));
+    body();
+    ctx->error.clear();
+    return KG_OK;
+  } catch (const Error& e) {
+    ctx->error = e.what();
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    ctx->error = "host allocation failed";
+    return KG_COMPUTATION_FAILURE;
+  } catch (const std::exception& e) {
+    ctx->error = e.what();
+    return KG_DEVICE_ERROR;
+  }
+}
+
+// ------------------------------------------------------------------ settings
+struct Sampler {
+  int64_t path_count;
+  uint64_t seed;
+  bool sobol;
+  bool antithetic;
+  bool fp32;
+  int64_t samples() const { return antithetic ? (path_count + 1) / 2 : path_count; }
+  int64_t blocks() const { return (samples() + BLOCK_SAMPLES - 1) / BLOCK_SAMPLES; }
+  // statistics flag of sample_paths (monte_carlo.rs:1771-1774)
+  bool statistics() const { return !sobol && (!antithetic || path_count % 2 == 0); }
+};
+
+constexpr int64_t MAX_GPU_PATHS = int64_t(1) << 26;
+
+// MonteCarloSettings::validate (monte_carlo.rs:198-217) and the path-step cap
+// (:219-227); allow_large lifts both to the GPU limits.
+inline Sampler validate_settings(const kg_settings* s, uint64_t dims) {
+  if (!s) fail(KG_INVALID_INPUT, "settings must not be null");
+  bool large = s->allow_large != 0;
+  int64_t cap = large ? MAX_GPU_PATHS : 10000000;
+  if (s->path_count < 1 || s->path_count > cap)
+    fail(KG_INVALID_INPUT, large ? "path_count must be between 1 and 67108864"
+                                 : "path_count must be between 1 and 10000000");
+  if (s->max_active_workers < 0) fail(KG_INVALID_INPUT, "max_active_workers must be nonnegative");
+  if (s->sampling != KG_PSEUDO_RANDOM && s->sampling != KG_SOBOL)
+    fail(KG_INVALID_INPUT, "unknown sampling");
+  if (s->precision != KG_FP64 && s->precision != KG_FP32_STEP)
+    fail(KG_INVALID_INPUT, "unknown precision");
+  if (s->sampling == KG_SOBOL && (s->path_count & (s->path_count - 1)))
+    fail(KG_INVALID_INPUT, "path_count must be a power of two for Sobol sampling");
+  if (!s->has_seed)
+    fail(KG_INVALID_INPUT,
+         "the GPU engine needs a resolved seed; call with_resolved_seed() before pricing");
+  if (!large && (double)s->path_count * (double)dims > 1e9)
+    fail(KG_INVALID_INPUT,
+         "Monte Carlo simulation exceeds one billion path steps; reduce paths or sampling "
+         "dimensions");
+  if (s->sampling == KG_SOBOL && (dims < 1 || dims > 21201))
+    fail(KG_INVALID_INPUT, "Sobol dimension must be in 1..=21,201");
+  return Sampler{s->path_count, s->seed, s->sampling == KG_SOBOL, s->antithetic != 0,
+                 s->precision == KG_FP32_STEP};
+}
+
+// ------------------------------------------------------------------ numerics
+inline double h_discounted_product(double scale, double weight, double log_discount) {
+  double value = scale * weight;
+  double discount = std::exp(log_discount);
+  if (scale == 0.0 || weight == 0.0) return 0.0;
+  if (std::isnormal(discount) && std::isnormal(value)) return value * discount;
+  return std::copysign(std::exp(std::log(scale) + std::log(std::fabs(weight)) + log_discount),
+                       weight);
+}
+inline double scaled_value(double scale, double weight, double log_discount) {
+  double r = h_discounted_product(scale, weight, log_discount);
+  if (!std::isfinite(r)) unstable();
+  return r;
+}
+inline double log_ratio(double num, double den) {  // numerics.rs:80-92
+  double rel = (num - den) / den;
+  if (std::fabs(rel) < 0.5) return std::log1p(rel);
+  double ratio = num / den;
+  if (std::isnormal(ratio)) return std::log(ratio);
+  return std::log(num) - std::log(den);
+}
+
+// ------------------------------------------------------------------ estimate
+// estimate + sampling_standard_error + MonteCarloEstimate::from_sampling
+// (monte_carlo.rs:1412-1452, 938-960, 897-936).
+inline void fill_estimate(const Sampler& s, const Moments& sample, const Moments& pairs,
+                          double log_discount, bool known, bool has_seed, kg_estimate* out) {
+  kg_estimate e{};
+  e.price = scaled_value(sample.scale, sample.mean, log_discount);
+  e.has_seed = has_seed;
+  e.seed = has_seed ? s.seed : 0;
+  bool has_se = false;
+  double se = 0.0;
+  if (s.sobol) {
+  } else if (known) {
+    has_se = true;
+  } else {
+    const Moments& u = s.antithetic ? pairs : sample;
+    if (u.count >= 2.0 && !(s.antithetic && s.path_count % 2 != 0)) {
+      double dev = std::sqrt(std::max(u.m2, 0.0) / (u.count - 1.0) / u.count);
+      se = scaled_value(u.scale, dev, log_discount);
+      has_se = true;
+    }
+  }
+  if (has_seed && has_se && se == 0.0) {
+    e.uncertainty_status = KG_ZERO_OBSERVED_VARIANCE;
+  } else if (has_se) {
+    e.uncertainty_status = KG_AVAILABLE;
+    e.has_standard_error = 1;
+    e.standard_error = se;
+    double hw = 1.959963984540054 * se;
+    e.ci_lower = e.price - hw;
+    e.ci_upper = e.price + hw;
+    if (!std::isfinite(e.ci_lower) || !std::isfinite(e.ci_upper)) unstable();
+  } else if (s.sobol) {
+    e.uncertainty_status = KG_UNSUPPORTED_SAMPLING;
+  } else if (s.antithetic && s.path_count % 2 != 0) {
+    e.uncertainty_status = KG_INCOMPLETE_ANTITHETIC_PAIR;
+  } else {
+    e.uncertainty_status = KG_INSUFFICIENT_OBSERVATIONS;
+  }
+  // Diagnostic: standard error of the per-path values as if independent.
+  if (sample.count >= 2.0) {
+    double dev = std::sqrt(std::max(sample.m2, 0.0) / (sample.count - 1.0) / sample.count);
+    e.path_standard_error = h_discounted_product(sample.scale, dev, log_discount);
+  }
+  double keep_steps = out->path_steps, keep_ms = out->device_ms;
+  *out = e;
+  out->path_steps = keep_steps;
+  out->device_ms = keep_ms;
+}
+
+// A known value: one Moments add, no sampling (e.g. monte_carlo.rs:1099-1103).
+inline void fill_known(double value, kg_estimate* out) {
+  if (!std::isfinite(value)) unstable();
+  kg_estimate e{};
+  e.price = value;
+  e.has_standard_error = 1;
+  e.standard_error = 0.0;
+  e.ci_lower = e.ci_upper = value;
+  e.uncertainty_status = KG_AVAILABLE;
+  *out = e;
+}
+
+// ------------------------------------------------------------------ runtime
+// Merges per-block partials [block][slot] into `slots` results on the device
+// and copies them back (see merge_partials in context.hip).
+void merge_and_fetch(kg_context* ctx, const Moments* partials, int64_t blocks, int slots,
+                     Moments* host_out);
+// Clears the flag word, returns a device pointer to it.
+uint32_t* reset_flags(kg_context* ctx);
+// Reads the flag word back (synchronising the stream) and throws on a raised flag.
+void check_flags(kg_context* ctx);
+// Builds (or reuses) the scrambled Sobol directions on the host and device.
+const SobolSetup& sobol_setup(kg_context* ctx, uint32_t dims, uint64_t seed);
+// Event timing on the context stream.
+void timing_begin(kg_context* ctx);
+double timing_end(kg_context* ctx);
+
+}  // namespace kg

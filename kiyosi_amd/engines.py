@@ -1,0 +1,131 @@
+"""Monte Carlo engines with kiyosi's Python interface, running on the GPU.
+
+Each engine mirrors the constructor keywords and pricing methods of the
+kiyosi class of the same name (crates/kiyosi-python/src/engines/monte_carlo.rs
+:212-342, 476-507): ``price``, ``price_with_seed`` and
+``price_with_statistics``. Two keywords are added for the GPU: ``precision``
+("fp64" restates the reference arithmetic, "fp32" steps paths in single
+precision with FP64 accumulation) and ``device``. ``max_active_workers`` is
+validated and otherwise unused.
+"""
+
+import ctypes as C
+import os
+
+from . import _abi, _native
+from .errors import InvalidInputError
+from .instruments import EuropeanOption, _Autocallable
+
+_SAMPLING = {"pseudo_random": _abi.PSEUDO_RANDOM, "sobol": _abi.SOBOL}
+_PRECISION = {"fp64": _abi.FP64, "fp32": _abi.FP32_STEP}
+
+
+class MonteCarloEstimate:
+    """Price, standard error, 95% interval, uncertainty status and seed
+    (monte_carlo.rs:859-1001), plus GPU diagnostics."""
+
+    def __init__(self, raw: _abi.Estimate):
+        self.price = raw.price
+        self.standard_error = raw.standard_error if raw.has_standard_error else None
+        self.confidence_interval = (raw.ci_lower, raw.ci_upper) if raw.has_standard_error else None
+        self.uncertainty_status = _abi.UNCERTAINTY_NAMES[raw.uncertainty_status]
+        self.seed = raw.seed if raw.has_seed else None
+        self.path_standard_error = raw.path_standard_error
+        self.path_steps = raw.path_steps
+        self.device_ms = raw.device_ms
+
+    def __repr__(self):
+        return (f"MonteCarloEstimate(price={self.price!r}, standard_error={self.standard_error!r}, "
+                f"uncertainty_status={self.uncertainty_status!r}, seed={self.seed!r})")
+
+
+class _MonteCarloEngine:
+    def __init__(self, *, path_count=131_072, seed=None, max_active_workers=0,
+                 sampling="pseudo_random", antithetic=False, precision="fp64", device=0,
+                 allow_large=False):
+        if isinstance(path_count, bool) or not isinstance(path_count, int):
+            raise TypeError("path_count must be an integer")
+        limit = 1 << 26 if allow_large else 10_000_000
+        if not 1 <= path_count <= limit:
+            raise InvalidInputError(f"path_count must be between 1 and {limit}")
+        if max_active_workers < 0:
+            raise InvalidInputError("max_active_workers must be nonnegative")
+        if sampling not in _SAMPLING:
+            raise InvalidInputError("sampling must be 'pseudo_random' or 'sobol'")
+        if sampling == "sobol" and path_count & (path_count - 1):
+            raise InvalidInputError("path_count must be a power of two for Sobol sampling")
+        if precision not in _PRECISION:
+            raise InvalidInputError("precision must be 'fp64' or 'fp32'")
+        if seed is not None and not 0 <= seed < 1 << 64:
+            raise OverflowError("seed must fit in an unsigned 64-bit integer")
+        self.path_count = path_count
+        self.seed = seed
+        self.max_active_workers = max_active_workers
+        self.sampling = sampling
+        self.antithetic = antithetic
+        self.precision = precision
+        self.device = device
+        self.allow_large = allow_large
+
+    def _kwargs(self):
+        return dict(path_count=self.path_count, seed=self.seed,
+                    max_active_workers=self.max_active_workers, sampling=self.sampling,
+                    antithetic=self.antithetic, precision=self.precision, device=self.device,
+                    allow_large=self.allow_large)
+
+    def with_resolved_seed(self):
+        """A replayable copy; draws a seed from system entropy only when none is set
+        (monte_carlo.rs:163-177)."""
+        kwargs = self._kwargs()
+        if kwargs["seed"] is None:
+            kwargs["seed"] = int.from_bytes(os.urandom(8), "little")
+        return type(self)(**kwargs)
+
+    def _settings(self) -> _abi.Settings:
+        return _abi.Settings(self.path_count, self.seed or 0, int(self.seed is not None),
+                             _SAMPLING[self.sampling], int(self.antithetic),
+                             _PRECISION[self.precision], self.max_active_workers,
+                             int(self.allow_large))
+
+    def price_with_statistics(self, instrument, context) -> MonteCarloEstimate:
+        engine = self if self.seed is not None else self.with_resolved_seed()
+        ctx = _native.context(engine.device)
+        out = _abi.Estimate()
+        fn, terms, market = engine._entry(instrument, context)
+        ctx.check(fn(ctx.handle, C.byref(engine._settings()), C.byref(terms), C.byref(market),
+                     C.byref(out)))
+        return MonteCarloEstimate(out)
+
+    def price_with_seed(self, instrument, context):
+        estimate = self.price_with_statistics(instrument, context)
+        return estimate.price, estimate.seed
+
+    def price(self, instrument, context) -> float:
+        return self.price_with_statistics(instrument, context).price
+
+
+class MonteCarloVanillaEngine(_MonteCarloEngine):
+    """European vanilla Monte Carlo (monte_carlo.rs:832-1329)."""
+
+    def __init__(self, *, step_count=50, **kwargs):
+        if not 1 <= step_count <= 10_000:
+            raise InvalidInputError("step_count is outside the supported Monte Carlo range")
+        self.step_count = step_count
+        super().__init__(**kwargs)
+
+    def _kwargs(self):
+        return dict(super()._kwargs(), step_count=self.step_count)
+
+    def _entry(self, instrument, context):
+        if not isinstance(instrument, EuropeanOption):
+            raise TypeError("MonteCarloVanillaEngine prices EuropeanOption")
+        return _native.lib().kg_vanilla_price, instrument._terms(), context.bsm_market()
+
+
+class MonteCarloAutocallableEngine(_MonteCarloEngine):
+    """Snowball and Phoenix Monte Carlo (engines/monte_carlo/autocallable.rs)."""
+
+    def _entry(self, instrument, context):
+        if not isinstance(instrument, _Autocallable):
+            raise TypeError("MonteCarloAutocallableEngine prices autocallable notes")
+        return _native.lib().kg_autocallable_price, instrument._terms(), context.bsm_market()

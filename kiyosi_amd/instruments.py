@@ -1,0 +1,269 @@
+"""Contracts priced by the Monte Carlo engines.
+
+Field names and meanings follow kiyosi's Python classes
+(python/src/kiyosi/_native.pyi) and the Rust terms they wrap
+(crates/kiyosi/src/instruments/{vanilla,autocallable}.rs). Each contract
+converts itself to the plain C structs of include/kiyosi_amd.h.
+"""
+
+import ctypes as C
+import datetime as _dt
+import math
+
+from . import _abi
+from .errors import InvalidInputError
+from .market import TradingCalendar, clock_micros, to_micros
+
+
+def _positive(name, value):
+    value = float(value)
+    if not (math.isfinite(value) and value > 0.0):
+        raise InvalidInputError(f"{name} must be positive and finite")
+    return value
+
+
+def _finite(name, value):
+    value = float(value)
+    if not math.isfinite(value):
+        raise InvalidInputError(f"{name} must be finite")
+    return value
+
+
+class EuropeanOption:
+    """European vanilla call or put (instruments/vanilla.rs)."""
+
+    def __init__(self, *, option_type, strike, effective_time, expiry_time):
+        if option_type not in ("call", "put"):
+            raise InvalidInputError("option_type must be 'call' or 'put'")
+        self.option_type = option_type
+        self.strike = _positive("strike", strike)
+        self.effective_time = effective_time
+        self.expiry_time = expiry_time
+        if to_micros(expiry_time) < to_micros(effective_time):
+            raise InvalidInputError("expiry precedes the effective time")
+
+    def _terms(self):
+        return _abi.EuropeanTerms(1 if self.option_type == "call" else -1, 0, self.strike,
+                                  to_micros(self.effective_time), to_micros(self.expiry_time))
+
+
+class KnockInObservation:
+    """Daily knock-in at a clock time, or knock-in observed only at expiry."""
+
+    def __init__(self, kind, observation_clock_time=None):
+        self._kind = kind
+        self._clock = observation_clock_time
+
+    @staticmethod
+    def every_trading_day(observation_clock_time=_dt.time(0, 0)):
+        return KnockInObservation("every_trading_day", observation_clock_time)
+
+    @staticmethod
+    def at_expiry():
+        return KnockInObservation("at_expiry")
+
+    def kind(self):
+        return self._kind
+
+    def observation_clock_time(self):
+        return self._clock
+
+
+class SnowballObservation:
+    """Knock-out observation with an annual coupon accrued on knock-out."""
+
+    def __init__(self, time, knock_out_level, coupon_rate):
+        self.time = time
+        self.knock_out_level = _positive("knock_out_level", knock_out_level)
+        self.coupon_rate = _finite("coupon_rate", coupon_rate)
+
+
+class KnockOutObservation:
+    def __init__(self, time, knock_out_level):
+        self.time = time
+        self.knock_out_level = _positive("knock_out_level", knock_out_level)
+
+
+class CouponObservation:
+    def __init__(self, time, coupon_barrier_level):
+        self.time = time
+        self.coupon_barrier_level = _finite("coupon_barrier_level", coupon_barrier_level)
+        if self.coupon_barrier_level < 0:
+            raise InvalidInputError("coupon_barrier_level must be nonnegative")
+
+
+_STATES = {None: _abi.STATE_UNKNOWN, "no_event": _abi.STATE_NO_EVENT,
+           "knocked_out": _abi.STATE_KNOCKED_OUT, "knocked_in": _abi.STATE_KNOCKED_IN}
+
+
+class _Autocallable:
+    """Shared terms of snowball variants and Phoenix notes
+    (instruments/autocallable.rs:700-870)."""
+
+    _coupon_kind = _abi.COUPON_SNOWBALL
+    _redemption_kind = _abi.REDEMPTION_DOWNSIDE
+
+    def _common(self, *, initial_spot, knock_in_level, upper_strike, lower_strike,
+                knock_in_observation, effective_time, expiry_time, barrier_state,
+                principal_redemption_ratio, rebate_rate, annual_rebate_rate, calendar):
+        self.initial_spot = _positive("initial_spot", initial_spot)
+        self.knock_in_level = None if knock_in_level is None else _positive("knock_in_level", knock_in_level)
+        self.upper_strike = upper_strike
+        self.lower_strike = lower_strike
+        if upper_strike is not None and lower_strike is not None and not (0 <= lower_strike <= upper_strike):
+            raise InvalidInputError("lower_strike must lie in [0, upper_strike]")
+        self.knock_in_observation = knock_in_observation
+        self.effective_time = effective_time
+        self.expiry_time = expiry_time
+        if to_micros(expiry_time) <= to_micros(effective_time):
+            raise InvalidInputError("expiry must follow the effective time")
+        if barrier_state not in _STATES:
+            raise InvalidInputError("barrier_state must be None, 'no_event', 'knocked_in' or 'knocked_out'")
+        self.barrier_state = barrier_state
+        self.principal_redemption_ratio = _finite("principal_redemption_ratio", principal_redemption_ratio)
+        self.rebate_rate = _finite("rebate_rate", rebate_rate)
+        self.annual_rebate_rate = _finite("annual_rebate_rate", annual_rebate_rate)
+        self.calendar = calendar if calendar is not None else TradingCalendar.weekdays()
+
+    def _events(self):
+        """Merged, sorted observation events (AutocallableNote::new,
+        instruments/autocallable.rs:972-1040)."""
+        events = {}
+        for obs in getattr(self, "autocall_observations", ()):
+            t = to_micros(obs.time)
+            events.setdefault(t, [t, 0, 0, 0.0, 0.0, 0.0])
+            e = events[t]
+            e[1], e[3], e[5] = 1, obs.knock_out_level, obs.coupon_rate
+        for obs in getattr(self, "knock_out_observations", ()):
+            t = to_micros(obs.time)
+            e = events.setdefault(t, [t, 0, 0, 0.0, 0.0, 0.0])
+            if not e[1]:
+                e[1], e[3] = 1, obs.knock_out_level
+        for obs in getattr(self, "coupon_observations", ()):
+            t = to_micros(obs.time)
+            e = events.setdefault(t, [t, 0, 0, 0.0, 0.0, 0.0])
+            if not e[2]:
+                e[2], e[4] = 1, obs.coupon_barrier_level
+        start, end = to_micros(self.effective_time), to_micros(self.expiry_time)
+        ordered = [events[t] for t in sorted(events)]
+        for e in ordered:
+            if not (start <= e[0] <= end):
+                raise InvalidInputError("observations must lie within the contract life")
+        return ordered
+
+    def _terms(self):
+        events = self._events()
+        array = (_abi.AutocallEvent * max(len(events), 1))()
+        for i, e in enumerate(events):
+            array[i] = _abi.AutocallEvent(*e)
+        ki = self.knock_in_observation
+        if self.knock_in_level is None:
+            kind, clock = _abi.KNOCK_IN_NONE, 0
+        elif ki.kind() == "every_trading_day":
+            kind, clock = _abi.KNOCK_IN_DAILY, clock_micros(ki.observation_clock_time())
+        else:
+            kind, clock = _abi.KNOCK_IN_AT_EXPIRY, 0
+        terms = _abi.AutocallTerms(
+            calendar=self.calendar.kind, knock_in_kind=kind, redemption_kind=self._redemption_kind,
+            coupon_kind=self._coupon_kind, barrier_state=_STATES[self.barrier_state],
+            n_events=len(events), observation_clock_micros=clock,
+            knock_in_level=self.knock_in_level or 0.0, initial_spot=self.initial_spot,
+            upper_strike=self.upper_strike or 0.0, lower_strike=self.lower_strike or 0.0,
+            fixed_coupon_rate=getattr(self, "minimum_coupon_rate", 0.0), coupon=self._coupon(),
+            principal_redemption_ratio=self.principal_redemption_ratio, rebate_rate=self.rebate_rate,
+            annual_rebate_rate=self.annual_rebate_rate, effective_time=to_micros(self.effective_time),
+            expiry_time=to_micros(self.expiry_time), events=C.cast(array, C.POINTER(_abi.AutocallEvent)))
+        terms._keep = array
+        return terms
+
+
+class SnowballOption(_Autocallable):
+    """Snowball with knock-out coupons and capped knock-in downside loss."""
+
+    def __init__(self, *, maturity_coupon_rate, initial_spot, knock_in_level, upper_strike,
+                 lower_strike, knock_in_observation, effective_time, expiry_time,
+                 autocall_observations, barrier_state=None, principal_redemption_ratio=1.0,
+                 rebate_rate=0.0, annual_rebate_rate=0.0, calendar=None):
+        self.maturity_coupon_rate = _finite("maturity_coupon_rate", maturity_coupon_rate)
+        self.autocall_observations = list(autocall_observations)
+        if not self.autocall_observations:
+            raise InvalidInputError("autocall_observations must be nonempty")
+        self._common(initial_spot=initial_spot, knock_in_level=knock_in_level,
+                     upper_strike=upper_strike, lower_strike=lower_strike,
+                     knock_in_observation=knock_in_observation, effective_time=effective_time,
+                     expiry_time=expiry_time, barrier_state=barrier_state,
+                     principal_redemption_ratio=principal_redemption_ratio,
+                     rebate_rate=rebate_rate, annual_rebate_rate=annual_rebate_rate,
+                     calendar=calendar)
+
+    def _coupon(self):
+        return self.maturity_coupon_rate
+
+
+class PhoenixOption(_Autocallable):
+    """Phoenix note: per-period coupons above a barrier, knock-out, knock-in downside."""
+
+    _coupon_kind = _abi.COUPON_PHOENIX
+
+    def __init__(self, *, coupon_per_period, initial_spot, knock_in_level, upper_strike,
+                 lower_strike, knock_in_observation, effective_time, expiry_time,
+                 knock_out_observations, coupon_observations, barrier_state=None,
+                 principal_redemption_ratio=1.0, rebate_rate=0.0, annual_rebate_rate=0.0,
+                 calendar=None):
+        self.coupon_per_period = _finite("coupon_per_period", coupon_per_period)
+        self.knock_out_observations = list(knock_out_observations)
+        self.coupon_observations = list(coupon_observations)
+        self._common(initial_spot=initial_spot, knock_in_level=knock_in_level,
+                     upper_strike=upper_strike, lower_strike=lower_strike,
+                     knock_in_observation=knock_in_observation, effective_time=effective_time,
+                     expiry_time=expiry_time, barrier_state=barrier_state,
+                     principal_redemption_ratio=principal_redemption_ratio,
+                     rebate_rate=rebate_rate, annual_rebate_rate=annual_rebate_rate,
+                     calendar=calendar)
+
+    def _coupon(self):
+        return self.coupon_per_period
+
+
+class BinarySnowballOption(_Autocallable):
+    """Snowball without knock-in: knock-out coupons, else the maturity coupon."""
+
+    def __init__(self, *, maturity_coupon_rate, effective_time, expiry_time, autocall_observations,
+                 barrier_state=None, principal_redemption_ratio=1.0, rebate_rate=0.0,
+                 annual_rebate_rate=0.0, calendar=None, initial_spot=None):
+        self.maturity_coupon_rate = _finite("maturity_coupon_rate", maturity_coupon_rate)
+        self.autocall_observations = list(autocall_observations)
+        spot = initial_spot or self.autocall_observations[0].knock_out_level
+        self._common(initial_spot=spot, knock_in_level=None, upper_strike=None, lower_strike=None,
+                     knock_in_observation=None, effective_time=effective_time,
+                     expiry_time=expiry_time, barrier_state=barrier_state,
+                     principal_redemption_ratio=principal_redemption_ratio,
+                     rebate_rate=rebate_rate, annual_rebate_rate=annual_rebate_rate,
+                     calendar=calendar)
+
+    def _coupon(self):
+        return self.maturity_coupon_rate
+
+
+class TernarySnowballOption(_Autocallable):
+    """Snowball whose knock-in pays a fixed minimum coupon instead of a loss."""
+
+    _redemption_kind = _abi.REDEMPTION_FIXED_COUPON
+
+    def __init__(self, *, maturity_coupon_rate, minimum_coupon_rate, knock_in_level,
+                 effective_time, expiry_time, autocall_observations, knock_in_observation,
+                 barrier_state=None, principal_redemption_ratio=1.0, rebate_rate=0.0,
+                 annual_rebate_rate=0.0, calendar=None):
+        self.maturity_coupon_rate = _finite("maturity_coupon_rate", maturity_coupon_rate)
+        self.minimum_coupon_rate = _finite("minimum_coupon_rate", minimum_coupon_rate)
+        self.autocall_observations = list(autocall_observations)
+        self._common(initial_spot=knock_in_level, knock_in_level=knock_in_level,
+                     upper_strike=None, lower_strike=None,
+                     knock_in_observation=knock_in_observation, effective_time=effective_time,
+                     expiry_time=expiry_time, barrier_state=barrier_state,
+                     principal_redemption_ratio=principal_redemption_ratio,
+                     rebate_rate=rebate_rate, annual_rebate_rate=annual_rebate_rate,
+                     calendar=calendar)
+
+    def _coupon(self):
+        return self.maturity_coupon_rate

@@ -1,0 +1,173 @@
+// Test infrastructure, not product code: the C entry points through which
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg drive the
+// FP64 CPU restatement of kiyosi's Monte Carlo engines (core.hpp, plans.hpp).
+#include <cstdio>
+#include <mutex>
+
+#include "core.hpp"
+#include "plans.hpp"
+
+using namespace oracle;
+
+namespace {
+std::vector<uint32_t> g_table;
+thread_local std::string g_error;
+
+Settings to_settings(const kg_settings* s, int workers) {
+  if (!s->has_seed) throw Failure(INVALID_INPUT, "the oracle needs an explicit seed");
+  if (s->path_count < 1) throw Failure(INVALID_INPUT, "path_count must be positive");
+  Settings st;
+  st.path_count = s->path_count;
+  st.seed = s->seed;
+  st.sampling = s->sampling;
+  st.antithetic = s->antithetic != 0;
+  st.workers = workers;
+  if (st.sampling == SOBOL && (st.path_count & (st.path_count - 1)))
+    throw Failure(INVALID_INPUT, "path_count must be a power of two for Sobol sampling");
+  return st;
+}
+Context context() { return Context{g_table.data(), g_table.size()}; }
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_error.clear();
+    return 0;
+  } catch (const Failure& e) {
+    g_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 3;
+  }
+}
+
+void write(const Estimate& e, kg_estimate* out) {
+  *out = kg_estimate{};
+  out->price = e.price;
+  out->has_standard_error = e.has_standard_error;
+  out->standard_error = e.standard_error;
+  if (e.has_standard_error) {
+    double hw = 1.959963984540054 * e.standard_error;
+    out->ci_lower = e.price - hw;
+    out->ci_upper = e.price + hw;
+  }
+  out->uncertainty_status = e.status;
+  out->has_seed = e.has_seed;
+  out->seed = e.seed;
+}
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error() { return g_error.c_str(); }
+
+void oracle_set_sobol_table(const uint32_t* words, size_t n) { g_table.assign(words, words + n); }
+
+void oracle_pcg32_outputs(uint64_t seed, uint64_t stream, uint64_t first, uint64_t count,
+                          uint32_t* out) {
+  Pcg32 g(seed, stream);
+  g.advance(first);
+  for (uint64_t i = 0; i < count; ++i) out[i] = g.next();
+}
+
+void oracle_pcg32_uniforms(uint64_t seed, uint64_t stream, uint64_t count, double* out) {
+  Pcg32 g(seed, stream);
+  for (uint64_t i = 0; i < count; ++i) out[i] = g.uniform();
+}
+
+// Normals first..first+count of a block stream, drawn one by one as
+// NormalDraws::normal does (monte_carlo.rs:1678-1687).
+void oracle_pcg_normals(uint64_t seed, uint64_t stream, uint64_t first, uint64_t count,
+                        double* out) {
+  NormalDraws d;
+  d.gen = Pcg32(seed, stream);
+  d.skip_normals(first);
+  for (uint64_t i = 0; i < count; ++i) out[i] = d.normal();
+}
+
+void oracle_sin_cos_turns(const double* turns, size_t n, double* s, double* c) {
+  for (size_t i = 0; i < n; ++i) sin_cos_turns(turns[i], &s[i], &c[i]);
+}
+
+void oracle_inverse_normal_cdf(const double* p, size_t n, double* out) {
+  for (size_t i = 0; i < n; ++i) out[i] = inverse_normal_cdf(p[i]);
+}
+
+int oracle_sobol_points(uint32_t dims, uint64_t seed, uint64_t first, uint64_t count,
+                        uint64_t* out) {
+  return guarded([&] {
+    SobolSequence s = sobol_new(dims, seed, g_table.data(), g_table.size());
+    SobolCursor c(&s, first);
+    for (uint64_t p = 0; p < count; ++p) {
+      for (uint32_t d = 0; d < dims; ++d) out[p * dims + d] = c.values[d];
+      c.next_path();
+    }
+  });
+}
+
+// Unscrambled, unshifted direction integers (52 bits) for dimension checks.
+int oracle_sobol_raw_directions(uint32_t dims, uint64_t* out) {
+  return guarded([&] {
+    // Build with an identity scramble by reusing sobol_new's recurrence on a
+    // copy of the table: the scramble is the identity when every matrix row
+    // is a single bit, which sobol_new cannot express, so redo the recurrence.
+    size_t pos = 0;
+    for (uint32_t d = 0; d < dims; ++d) {
+      uint64_t raw[SOBOL_BITS] = {};
+      if (d == 0) {
+        for (int c = 0; c < SOBOL_BITS; ++c) raw[c] = 1ULL << (SOBOL_BITS - c - 1);
+      } else {
+        int degree = (int)g_table.at(pos);
+        uint32_t coeff = g_table.at(pos + 1);
+        for (int c = 0; c < degree && c < SOBOL_BITS; ++c)
+          raw[c] = (uint64_t)g_table.at(pos + 2 + c) << (SOBOL_BITS - c - 1);
+        pos += degree + 2;
+        for (int c = degree; c < SOBOL_BITS; ++c) {
+          uint64_t v = raw[c - degree] ^ (raw[c - degree] >> degree);
+          for (int o = 1; o < degree; ++o)
+            if ((coeff >> (degree - o - 1)) & 1) v ^= raw[c - o];
+          raw[c] = v;
+        }
+      }
+      for (int c = 0; c < SOBOL_BITS; ++c) out[(size_t)d * SOBOL_BITS + c] = raw[c];
+    }
+  });
+}
+
+int oracle_vanilla_price(const kg_settings* s, const kg_european_terms* o,
+                         const kg_bsm_market* m, int workers, kg_estimate* out) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    write(european_price(st, *o, *m, context()), out);
+  });
+}
+
+int oracle_autocallable_price(const kg_settings* s, const kg_autocall_terms* t,
+                              const kg_bsm_market* m, int workers, kg_estimate* out) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    write(autocall_price(st, *t, *m, context()), out);
+  });
+}
+
+// Plan of an autocallable note as the oracle builds it, for plan parity tests:
+// per step (drift, diffusion, discount, event, observe), and bridge flag.
+int oracle_autocallable_plan(const kg_autocall_terms* t, const kg_bsm_market* m, double* steps,
+                             size_t cap, size_t* n_steps, int* bridge) {
+  return guarded([&] {
+    Note note{t};
+    AutocallPlan p = autocall_plan(note, *m);
+    *n_steps = p.steps.size();
+    *bridge = p.bridge;
+    for (size_t i = 0; i < p.steps.size() && i < cap; ++i) {
+      steps[i * 5 + 0] = p.steps[i].drift;
+      steps[i * 5 + 1] = p.steps[i].diffusion;
+      steps[i * 5 + 2] = p.steps[i].discount;
+      steps[i * 5 + 3] = p.steps[i].event;
+      steps[i * 5 + 4] = p.steps[i].observe_knock_in;
+    }
+  });
+}
+}

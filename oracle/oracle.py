@@ -1,0 +1,128 @@
+"""Test infrastructure, not product code.
+
+ctypes loader for the FP64 CPU restatement of kiyosi's Monte Carlo engines
+(``oracle/_build/liboracle.so``, built by ``make -C oracle``). Only tests/,
+``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg import it, as the
+checker of the HIP engine; the library never does.
+"""
+
+import ctypes as C
+import pathlib
+import subprocess
+
+import numpy as np
+
+from kiyosi_amd import _abi
+from kiyosi_amd.errors import raise_for_status
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+TABLE_PATH = HERE.parent / "kiyosi_amd" / "data" / "joe_kuo_6_21201.u32"
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        handle = C.CDLL(str(LIB_PATH))
+        P = C.POINTER
+        handle.oracle_last_error.restype = C.c_char_p
+        handle.oracle_set_sobol_table.argtypes = [P(C.c_uint32), C.c_size_t]
+        handle.oracle_pcg32_outputs.argtypes = [C.c_uint64] * 4 + [P(C.c_uint32)]
+        handle.oracle_pcg32_uniforms.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_double)]
+        handle.oracle_pcg_normals.argtypes = [C.c_uint64] * 4 + [P(C.c_double)]
+        handle.oracle_sin_cos_turns.argtypes = [P(C.c_double), C.c_size_t, P(C.c_double), P(C.c_double)]
+        handle.oracle_inverse_normal_cdf.argtypes = [P(C.c_double), C.c_size_t, P(C.c_double)]
+        handle.oracle_sobol_points.argtypes = [C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_uint64)]
+        handle.oracle_sobol_raw_directions.argtypes = [C.c_uint32, P(C.c_uint64)]
+        handle.oracle_vanilla_price.argtypes = [
+            P(_abi.Settings), P(_abi.EuropeanTerms), P(_abi.BsmMarket), C.c_int, P(_abi.Estimate)]
+        handle.oracle_autocallable_price.argtypes = [
+            P(_abi.Settings), P(_abi.AutocallTerms), P(_abi.BsmMarket), C.c_int, P(_abi.Estimate)]
+        handle.oracle_autocallable_plan.argtypes = [
+            P(_abi.AutocallTerms), P(_abi.BsmMarket), P(C.c_double), C.c_size_t,
+            P(C.c_size_t), P(C.c_int)]
+        table = np.fromfile(TABLE_PATH, dtype="<u4")
+        handle.oracle_set_sobol_table(table.ctypes.data_as(P(C.c_uint32)), table.size)
+        _lib = handle
+    return _lib
+
+
+def _check(status: int) -> None:
+    if status:
+        raise_for_status(status, lib().oracle_last_error().decode())
+
+
+def _ptr(array, ctype):
+    return array.ctypes.data_as(C.POINTER(ctype))
+
+
+def pcg32_outputs(seed: int, stream: int, first: int, count: int) -> np.ndarray:
+    out = np.empty(count, dtype=np.uint32)
+    lib().oracle_pcg32_outputs(seed, stream, first, count, _ptr(out, C.c_uint32))
+    return out
+
+
+def pcg32_uniforms(seed: int, stream: int, count: int) -> np.ndarray:
+    out = np.empty(count, dtype=np.float64)
+    lib().oracle_pcg32_uniforms(seed, stream, count, _ptr(out, C.c_double))
+    return out
+
+
+def pcg_normals(seed: int, stream: int, first: int, count: int) -> np.ndarray:
+    out = np.empty(count, dtype=np.float64)
+    lib().oracle_pcg_normals(seed, stream, first, count, _ptr(out, C.c_double))
+    return out
+
+
+def sin_cos_turns(turns) -> tuple:
+    turns = np.ascontiguousarray(turns, dtype=np.float64)
+    s, c = np.empty_like(turns), np.empty_like(turns)
+    lib().oracle_sin_cos_turns(_ptr(turns, C.c_double), turns.size, _ptr(s, C.c_double), _ptr(c, C.c_double))
+    return s, c
+
+
+def inverse_normal_cdf(p) -> np.ndarray:
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    out = np.empty_like(p)
+    lib().oracle_inverse_normal_cdf(_ptr(p, C.c_double), p.size, _ptr(out, C.c_double))
+    return out
+
+
+def sobol_points(dims: int, seed: int, first: int, count: int) -> np.ndarray:
+    out = np.empty((count, dims), dtype=np.uint64)
+    _check(lib().oracle_sobol_points(dims, seed, first, count, _ptr(out, C.c_uint64)))
+    return out
+
+
+def sobol_raw_directions(dims: int) -> np.ndarray:
+    out = np.empty((dims, 52), dtype=np.uint64)
+    _check(lib().oracle_sobol_raw_directions(dims, _ptr(out, C.c_uint64)))
+    return out
+
+
+def vanilla_price(settings, terms, market, workers: int = 1):
+    out = _abi.Estimate()
+    _check(lib().oracle_vanilla_price(C.byref(settings), C.byref(terms), C.byref(market), workers, C.byref(out)))
+    return out
+
+
+def autocallable_price(settings, terms, market, workers: int = 1):
+    out = _abi.Estimate()
+    _check(lib().oracle_autocallable_price(C.byref(settings), C.byref(terms), C.byref(market), workers, C.byref(out)))
+    return out
+
+
+def autocallable_plan(terms, market, cap: int = 100000):
+    steps = np.zeros((cap, 5), dtype=np.float64)
+    n, bridge = C.c_size_t(), C.c_int()
+    _check(lib().oracle_autocallable_plan(C.byref(terms), C.byref(market), _ptr(steps, C.c_double), cap,
+                                          C.byref(n), C.byref(bridge)))
+    return steps[: n.value].copy(), bool(bridge.value)

@@ -1,0 +1,128 @@
+"""GPU parity of the generators and of the European engine (config 1) against
+the FP64 CPU oracle, through the C ABI."""
+
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from conftest import END, START, golden, us
+from kiyosi_amd import _abi, _native
+
+pytestmark = pytest.mark.gpu
+
+
+def market(spot=100.0, rate=0.05, dividend=0.02, sigma=0.2, valuation=START):
+    return _abi.BsmMarket(spot, rate, dividend, sigma, us(valuation))
+
+
+def option(strike=100.0, kind=1):
+    return _abi.EuropeanTerms(kind, 0, strike, us(START), us(END))
+
+
+def settings(paths, seed, sampling=_abi.PSEUDO_RANDOM, antithetic=False, precision=_abi.FP64):
+    return _abi.Settings(paths, seed, 1, sampling, int(antithetic), precision, 0, 0)
+
+
+def gpu_price(ctx, s, o, m):
+    out = _abi.Estimate()
+    ctx.check(_native.lib().kg_vanilla_price(ctx.handle, C.byref(s), C.byref(o), C.byref(m), C.byref(out)))
+    return out
+
+
+def test_pcg32_outputs_are_bit_exact(gpu, oracle):
+    lib = _native.lib()
+    for seed, stream, first, count in ((42, 54, 0, 6), (1729, 0, 0, 4096), (2**64 - 1, 1953, 12345, 8192),
+                                       (0, 7, 2**40, 1000)):
+        out = np.empty(count, dtype=np.uint32)
+        gpu.check(lib.kg_pcg32_outputs(gpu.handle, seed, stream, first, count,
+                                       out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        np.testing.assert_array_equal(out, oracle.pcg32_outputs(seed, stream, first, count))
+
+
+def test_pcg_normals_match_the_oracle(gpu, oracle):
+    count = 4096
+    out = np.empty(count)
+    gpu.check(_native.lib().kg_pcg_normals(gpu.handle, 1729, 5, 3, count, out.ctypes.data_as(C.POINTER(C.c_double))))
+    ref = oracle.pcg_normals(1729, 5, 3, count)
+    # Device log may differ from glibc's in the last bit; the integers do not.
+    np.testing.assert_allclose(out, ref, rtol=4e-16 * 8, atol=1e-300)
+    assert np.mean(out == ref) > 0.9
+
+
+def test_sobol_points_are_bit_exact(gpu, oracle):
+    for dims, seed, first, count in ((1, 42, 0, 1), (5, 1729, 0, 1024), (252, 1729, 4096, 300), (21201, 3, 7, 2)):
+        out = np.empty((count, dims), dtype=np.uint64)
+        gpu.check(_native.lib().kg_sobol_points(gpu.handle, dims, seed, first, count,
+                                                out.ctypes.data_as(C.POINTER(C.c_uint64))))
+        np.testing.assert_array_equal(out, oracle.sobol_points(dims, seed, first, count))
+
+
+@pytest.mark.parametrize("paths,antithetic,sampling", [
+    (1, False, _abi.PSEUDO_RANDOM), (255, False, _abi.PSEUDO_RANDOM), (257, True, _abi.PSEUDO_RANDOM),
+    (4097, True, _abi.PSEUDO_RANDOM), (65536, False, _abi.PSEUDO_RANDOM), (1, False, _abi.SOBOL),
+    (2, True, _abi.SOBOL), (4096, False, _abi.SOBOL), (32768, True, _abi.SOBOL)])
+@pytest.mark.parametrize("kind", [1, -1])
+def test_european_fp64_matches_the_oracle(gpu, oracle, paths, antithetic, sampling, kind):
+    s = settings(paths, 42, sampling, antithetic)
+    o, m = option(kind=kind), market()
+    got, ref = gpu_price(gpu, s, o, m), oracle.vanilla_price(s, o, m, workers=8)
+    assert got.price == pytest.approx(ref.price, rel=1e-10)
+    assert got.has_standard_error == ref.has_standard_error
+    assert got.uncertainty_status == ref.uncertainty_status
+    if ref.has_standard_error:
+        assert got.standard_error == pytest.approx(ref.standard_error, rel=1e-9)
+
+
+def test_config1_european_call_one_million_antithetic_pcg_paths(gpu, oracle):
+    """1,000,000 PCG paths in antithetic pairs, seed 1729: price within 1e-10
+    relative of the FP64 oracle, standard error equal."""
+    s = settings(1_000_000, 1729, antithetic=True)
+    o, m = option(), market()
+    got, ref = gpu_price(gpu, s, o, m), oracle.vanilla_price(s, o, m, workers=8)
+    assert abs(got.price / ref.price - 1) < 1e-10
+    assert got.standard_error == pytest.approx(ref.standard_error, rel=1e-9)
+    analytic = {c["id"]: c["value"] for c in golden("references.json")["monte_carlo"]}["mc-european-call"]
+    assert abs(got.price - analytic) < 4 * got.standard_error
+
+
+def test_calibration_vectors_on_the_gpu(gpu):
+    for run in golden("calibration_monte_carlo.json")["runs"]:
+        if not run["id"].startswith("mc-european"):
+            continue
+        kind = 1 if run["id"].endswith("call") else -1
+        for sample in run["samples"][:4]:
+            got = gpu_price(gpu, settings(run["path_count"], sample["seed"]), option(kind=kind), market())
+            assert got.price == pytest.approx(sample["price"], rel=1e-10)
+
+
+def test_python_golden_tail_case_on_the_gpu(gpu):
+    tail = golden("python_golden.json")["sobol_high_variance_tail"]
+    got = gpu_price(gpu, settings(1, 42, _abi.SOBOL), option(strike=1.0),
+                    market(spot=1e308, rate=800.0, dividend=0.0, sigma=40.0))
+    assert got.price == pytest.approx(tail["value"], rel=1e-10)
+
+
+def test_fp32_step_is_within_half_a_standard_error(gpu):
+    o, m = option(), market()
+    plain = gpu_price(gpu, settings(1_000_000, 1729, antithetic=True), o, m)
+    fast = gpu_price(gpu, settings(1_000_000, 1729, antithetic=True, precision=_abi.FP32_STEP), o, m)
+    assert abs(fast.price - plain.price) <= 0.5 * plain.standard_error
+
+
+def test_python_engine_interface(gpu):
+    import kiyosi_amd as k
+    opt = k.EuropeanOption(option_type="call", strike=100, effective_time=START, expiry_time=END)
+    ctx = k.PricingContext(model_parameters=k.BlackScholesMertonParameters(
+        risk_free_rate=0.05, dividend_yield=0.02, volatility=0.2), spot_price=100, valuation_time=START)
+    engine = k.MonteCarloVanillaEngine(path_count=65536, seed=1009)
+    est = engine.price_with_statistics(opt, ctx)
+    assert est.seed == 1009 and est.uncertainty_status == "available"
+    assert est.price == engine.price(opt, ctx)
+    assert est.price == pytest.approx(9.249574403327761, rel=1e-10)
+    assert math.isclose(est.confidence_interval[1] - est.price, 1.959963984540054 * est.standard_error)
+    unseeded = k.MonteCarloVanillaEngine(path_count=1024).price_with_statistics(opt, ctx)
+    assert unseeded.seed is not None
+    with pytest.raises(k.InvalidInputError):
+        k.MonteCarloVanillaEngine(path_count=1000, sampling="sobol")
