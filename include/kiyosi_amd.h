@@ -192,6 +192,160 @@ kg_status kg_autocallable_price(kg_context* ctx, const kg_settings* settings,
                                 const kg_autocall_terms* note, const kg_bsm_market* market,
                                 kg_estimate* out);
 
+/* ---- Heston path stepping. kiyosi has Heston only in its Fourier and FD
+ * engines (market/heston.rs:58-227, engines/analytic/heston.rs,
+ * engines/finite_difference/heston.rs); every reference MC engine rejects
+ * Heston contexts with UnsupportedOperation (market/parameters.rs:271-278).
+ * This entry point extends MonteCarloAsianEngine / MonteCarloVanillaEngine
+ * to HestonParameters with a full-truncation Euler scheme in log-spot on
+ * `step_count` equal steps: per step, normals 2k and 2k+1 of the sample
+ * (one Box-Muller pair) drive variance and spot with correlation rho.
+ * payoff: KG_HESTON_EUROPEAN pays on the terminal spot, KG_HESTON_ASIAN_*
+ * on the arithmetic or geometric mean of the step_count post-step spots. */
+enum {
+  KG_HESTON_EUROPEAN = 0,
+  KG_HESTON_ASIAN_ARITHMETIC = 1,
+  KG_HESTON_ASIAN_GEOMETRIC = 2
+};
+
+typedef struct kg_heston_market {
+  double spot;
+  double rate;
+  double dividend;
+  double v0;    /* initial variance */
+  double kappa; /* mean reversion speed */
+  double theta; /* long-run variance */
+  double xi;    /* volatility of variance */
+  double rho;   /* spot-variance correlation */
+  int64_t valuation_time;
+} kg_heston_market;
+
+typedef struct kg_heston_terms {
+  int32_t payoff;      /* KG_HESTON_* */
+  int32_t option_type; /* 1 call, -1 put */
+  int32_t step_count;  /* 1..=10,000 equal steps from valuation to expiry */
+  int32_t reserved;
+  double strike;
+  int64_t effective_time;
+  int64_t expiry_time;
+} kg_heston_terms;
+
+kg_status kg_heston_price(kg_context* ctx, const kg_settings* settings,
+                          const kg_heston_terms* option, const kg_heston_market* market,
+                          kg_estimate* out);
+
+/* ---- Single barrier options: MonteCarloBarrierEngine
+ * (engines/monte_carlo/barrier.rs:160-541): scheduled fixings walk the
+ * cumulative noise against the barrier on each observation date (knock-ins
+ * with more than 8 fixings left are conditioned on the hit with a lognormal
+ * price, :320-333); continuous monitoring draws the terminal value and
+ * weights it by the Brownian-bridge survival (:402-440). Values only: the
+ * likelihood-ratio Greeks of :459-472 are not computed. */
+enum { KG_DOWN_AND_OUT = 0, KG_DOWN_AND_IN = 1, KG_UP_AND_OUT = 2, KG_UP_AND_IN = 3 };
+enum { KG_OBSERVE_CONTINUOUS = 0, KG_OBSERVE_SCHEDULED = 1 };
+enum { KG_REBATE_AT_EXPIRY = 0, KG_REBATE_AT_HIT = 1 };
+
+typedef struct kg_barrier_terms {
+  int32_t option_type;  /* 1 call, -1 put */
+  int32_t barrier_type; /* KG_DOWN_AND_OUT .. KG_UP_AND_IN */
+  int32_t observation;  /* KG_OBSERVE_* */
+  int32_t rebate_timing;
+  double strike;
+  double barrier;
+  double rebate;
+  int64_t effective_time;
+  int64_t expiry_time;
+  int32_t n_observations; /* scheduled fixings, strictly increasing */
+  int32_t reserved;
+  const int64_t* observation_times;
+} kg_barrier_terms;
+
+kg_status kg_barrier_price(kg_context* ctx, const kg_settings* settings,
+                           const kg_barrier_terms* option, const kg_bsm_market* market,
+                           kg_estimate* out);
+
+/* ---- Average-price options: MonteCarloAsianEngine
+ * (engines/monte_carlo/asian.rs:374-1257). Geometric averages sample the
+ * exact one-dimensional lognormal (:741-784); arithmetic averages integrate
+ * the continuous average by the trapezoid rule over step_count intervals
+ * with a conditional last half interval and a geometric control variate
+ * (:640-712, 1016-1061, 1100-1256). */
+enum { KG_AVERAGE_ARITHMETIC = 0, KG_AVERAGE_GEOMETRIC = 1 };
+
+typedef struct kg_asian_terms {
+  int32_t option_type; /* 1 call, -1 put */
+  int32_t averaging;   /* KG_AVERAGE_* */
+  int32_t step_count;  /* integration intervals, 2..=10,000 */
+  int32_t has_realized;
+  double strike;
+  double realized_average; /* average fixed before valuation, if any */
+  int64_t effective_time;
+  int64_t averaging_start_time;
+  int64_t expiry_time;
+} kg_asian_terms;
+
+kg_status kg_asian_price(kg_context* ctx, const kg_settings* settings, const kg_asian_terms* option,
+                         const kg_bsm_market* market, kg_estimate* out);
+
+/* ---- Accumulators and decumulators: MonteCarloAccumulatorEngine
+ * (engines/monte_carlo/accumulator.rs:91-218; instrument
+ * instruments/accumulator.rs:401-526). One step per trading-day fixing at
+ * the observation clock: knock-out settles the accumulated quantity at the
+ * knock-out fixing, otherwise the daily quantity (times the acceleration
+ * factor beyond the strike) accrues and settles at expiry. */
+typedef struct kg_accumulator_terms {
+  int32_t option_type; /* 1 accumulator (call), -1 decumulator (put) */
+  int32_t calendar;    /* KG_CALENDAR_* */
+  double strike;
+  double knock_out_level;
+  double daily_quantity;
+  double acceleration_factor;
+  double accumulated_quantity; /* quantity accrued before valuation */
+  int64_t observation_clock_micros;
+  int64_t effective_time;
+  int64_t expiry_time;
+} kg_accumulator_terms;
+
+kg_status kg_accumulator_price(kg_context* ctx, const kg_settings* settings,
+                               const kg_accumulator_terms* option, const kg_bsm_market* market,
+                               kg_estimate* out);
+
+/* ---- Heterogeneous book in one persistent launch. Every contract keeps
+ * its own settings (path count, seed, sampling) and market, so each result
+ * equals the single-contract entry point's on the same inputs. The kernel
+ * pulls (contract, 256-sample block) tiles from a device queue, longest
+ * contracts first; per-block partials are merged in block order. The
+ * launch sequence is captured once in a hipGraph and replayed; tile shape
+ * (samples per thread, workgroups per CU) comes from the autotuner unless
+ * fixed in `tuning`. No counterpart in kiyosi, which prices one contract
+ * per call (autocallable.rs:119-152 batches one note type only). */
+enum {
+  KG_BOOK_BARRIER = 0,
+  KG_BOOK_ASIAN = 1,
+  KG_BOOK_AUTOCALL = 2,
+  KG_BOOK_ACCUMULATOR = 3
+};
+
+typedef struct kg_book_contract {
+  int32_t kind; /* KG_BOOK_* */
+  int32_t reserved;
+  kg_settings settings;
+  kg_bsm_market market;
+  const void* terms; /* kg_barrier_terms, kg_asian_terms, kg_autocall_terms or kg_accumulator_terms */
+} kg_book_contract;
+
+typedef struct kg_book_tuning {
+  int32_t samples_per_thread;  /* 1, 2 or 4; 0 = autotune */
+  int32_t workgroups_per_cu;   /* 1..8; 0 = autotune */
+  int32_t use_graph;           /* capture and replay the launch as a hipGraph */
+  int32_t replays;             /* extra timed replays of the captured graph (benchmarking) */
+  double device_ms;            /* out: mean device time of one book launch */
+  double path_steps;           /* out: simulated path-steps of the book */
+} kg_book_tuning;
+
+kg_status kg_price_book(kg_context* ctx, const kg_book_contract* contracts, size_t n,
+                        int32_t precision, kg_book_tuning* tuning, kg_estimate* out);
+
 /* ---- Generator diagnostics, computed on the GPU with per-thread skip-ahead.
  * kg_pcg32_outputs: outputs first..first+count of Pcg32::new(seed, stream)
  * (monte_carlo.rs:2247-2267). kg_sobol_points: the 52-bit scrambled Sobol

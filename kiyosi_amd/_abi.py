@@ -115,3 +115,106 @@ class AutocallTerms(C.Structure):
         ("expiry_time", C.c_int64),
         ("events", C.POINTER(AutocallEvent)),
     ]
+
+
+HESTON_EUROPEAN, HESTON_ASIAN_ARITHMETIC, HESTON_ASIAN_GEOMETRIC = 0, 1, 2
+
+
+class HestonMarket(C.Structure):
+    _fields_ = [
+        ("spot", C.c_double),
+        ("rate", C.c_double),
+        ("dividend", C.c_double),
+        ("v0", C.c_double),
+        ("kappa", C.c_double),
+        ("theta", C.c_double),
+        ("xi", C.c_double),
+        ("rho", C.c_double),
+        ("valuation_time", C.c_int64),
+    ]
+
+
+class HestonTerms(C.Structure):
+    _fields_ = [
+        ("payoff", C.c_int32),
+        ("option_type", C.c_int32),
+        ("step_count", C.c_int32),
+        ("reserved", C.c_int32),
+        ("strike", C.c_double),
+        ("effective_time", C.c_int64),
+        ("expiry_time", C.c_int64),
+    ]
+
+
+DOWN_AND_OUT, DOWN_AND_IN, UP_AND_OUT, UP_AND_IN = 0, 1, 2, 3
+OBSERVE_CONTINUOUS, OBSERVE_SCHEDULED = 0, 1
+REBATE_AT_EXPIRY, REBATE_AT_HIT = 0, 1
+AVERAGE_ARITHMETIC, AVERAGE_GEOMETRIC = 0, 1
+BOOK_BARRIER, BOOK_ASIAN, BOOK_AUTOCALL, BOOK_ACCUMULATOR = 0, 1, 2, 3
+
+
+class BarrierTerms(C.Structure):
+    _fields_ = [
+        ("option_type", C.c_int32),
+        ("barrier_type", C.c_int32),
+        ("observation", C.c_int32),
+        ("rebate_timing", C.c_int32),
+        ("strike", C.c_double),
+        ("barrier", C.c_double),
+        ("rebate", C.c_double),
+        ("effective_time", C.c_int64),
+        ("expiry_time", C.c_int64),
+        ("n_observations", C.c_int32),
+        ("reserved", C.c_int32),
+        ("observation_times", C.POINTER(C.c_int64)),
+    ]
+
+
+class AsianTerms(C.Structure):
+    _fields_ = [
+        ("option_type", C.c_int32),
+        ("averaging", C.c_int32),
+        ("step_count", C.c_int32),
+        ("has_realized", C.c_int32),
+        ("strike", C.c_double),
+        ("realized_average", C.c_double),
+        ("effective_time", C.c_int64),
+        ("averaging_start_time", C.c_int64),
+        ("expiry_time", C.c_int64),
+    ]
+
+
+class AccumulatorTerms(C.Structure):
+    _fields_ = [
+        ("option_type", C.c_int32),
+        ("calendar", C.c_int32),
+        ("strike", C.c_double),
+        ("knock_out_level", C.c_double),
+        ("daily_quantity", C.c_double),
+        ("acceleration_factor", C.c_double),
+        ("accumulated_quantity", C.c_double),
+        ("observation_clock_micros", C.c_int64),
+        ("effective_time", C.c_int64),
+        ("expiry_time", C.c_int64),
+    ]
+
+
+class BookContract(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("reserved", C.c_int32),
+        ("settings", Settings),
+        ("market", BsmMarket),
+        ("terms", C.c_void_p),
+    ]
+
+
+class BookTuning(C.Structure):
+    _fields_ = [
+        ("samples_per_thread", C.c_int32),
+        ("workgroups_per_cu", C.c_int32),
+        ("use_graph", C.c_int32),
+        ("replays", C.c_int32),
+        ("device_ms", C.c_double),
+        ("path_steps", C.c_double),
+    ]

@@ -31,6 +31,16 @@ SIGNATURES = {
                                    P(_abi.Estimate)]),
     "kg_autocallable_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AutocallTerms), P(_abi.BsmMarket),
                                         P(_abi.Estimate)]),
+    "kg_heston_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.HestonTerms), P(_abi.HestonMarket),
+                                  P(_abi.Estimate)]),
+    "kg_barrier_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.BarrierTerms), P(_abi.BsmMarket),
+                                   P(_abi.Estimate)]),
+    "kg_asian_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AsianTerms), P(_abi.BsmMarket),
+                                 P(_abi.Estimate)]),
+    "kg_accumulator_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AccumulatorTerms), P(_abi.BsmMarket),
+                                       P(_abi.Estimate)]),
+    "kg_price_book": (C.c_int, [C.c_void_p, P(_abi.BookContract), C.c_size_t, C.c_int32, P(_abi.BookTuning),
+                                P(_abi.Estimate)]),
 }
 
 

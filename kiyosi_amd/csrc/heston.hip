@@ -1,0 +1,219 @@
+// Heston path stepping for European and Asian payoffs (extends kiyosi's
+// Monte Carlo engines to HestonParameters, market/heston.rs:58-227, which
+// the reference engines reject).
+//
+// Scheme: full-truncation Euler in log-spot (Lord, Koekkoek, van Dijk 2010)
+// on `steps` equal steps of size dt:
+//   v+ = max(v, 0)
+//   x += (r - q) dt - v+ dt / 2 + sqrt(v+ dt) (rho z1 + sqrt(1 - rho^2) z2)
+//   v += kappa (theta - v+) dt + xi sqrt(v+ dt) z1
+// z1, z2 are normals 2k, 2k+1 of the sample, i.e. the cosine and sine of
+// one Box-Muller pair of the block's PCG stream (monte_carlo.rs:1830-1845),
+// so sample j of block b starts at PCG output 4 * j * steps.
+//
+// Layout: one thread per sample, 256 per workgroup (one sample block); an
+// antithetic pair steps both legs from one draw. FP32 stepping keeps the
+// running average in FP64 and updates the variance in FP64 while it is
+// below V_NEAR_ZERO, where the truncation decides the next step.
+#include "fp32_math.hpp"
+#include "host_common.hpp"
+
+namespace kg {
+
+constexpr double V_NEAR_ZERO = 2.5e-4;
+
+struct HestonParams {
+  int payoff, steps;
+  double sign, strike, spot;
+  double a, hdt, kdt, kth, xi, rho, rho_c, sdt, v0, log_discount;
+};
+
+template <bool FP32>
+struct Leg;
+
+template <>
+struct Leg<false> {
+  double x, v, sum;
+  __device__ inline void init(const HestonParams& p) {
+    x = 0.0;
+    v = p.v0;
+    sum = 0.0;
+  }
+  __device__ inline void step(const HestonParams& p, double z1, double z2) {
+    const double vp = fmax(v, 0.0);
+    const double sq = sqrt(vp) * p.sdt;
+    const double zs = fma(p.rho, z1, p.rho_c * z2);
+    x += fma(sq, zs, fma(-p.hdt, vp, p.a));
+    v += fma(p.xi * sq, z1, fma(-p.kdt, vp, p.kth));
+    if (p.payoff == KG_HESTON_ASIAN_ARITHMETIC)
+      sum += exp(x);
+    else if (p.payoff == KG_HESTON_ASIAN_GEOMETRIC)
+      sum += x;
+  }
+};
+
+template <>
+struct Leg<true> {
+  float x, v;
+  double sum;
+  __device__ inline void init(const HestonParams& p) {
+    x = 0.f;
+    v = (float)p.v0;
+    sum = 0.0;
+  }
+  __device__ inline void step(const HestonParams& p, float z1, float z2) {
+    float sq;
+    if (v < (float)V_NEAR_ZERO) {
+      // Near-zero variance in FP64: truncation and the square root decide
+      // whether the path leaves zero.
+      const double vd = fmax((double)v, 0.0);
+      const double sqd = sqrt(vd) * p.sdt;
+      sq = (float)sqd;
+      const float vp = (float)vd;
+      x += fmaf(sq, fmaf((float)p.rho, z1, (float)p.rho_c * z2), fmaf(-(float)p.hdt, vp, (float)p.a));
+      v = (float)((double)v + fma(p.xi * sqd, (double)z1, fma(-p.kdt, vd, p.kth)));
+    } else {
+      const float vp = v;
+      sq = f32::sqrt(vp) * (float)p.sdt;
+      x += fmaf(sq, fmaf((float)p.rho, z1, (float)p.rho_c * z2), fmaf(-(float)p.hdt, vp, (float)p.a));
+      v += fmaf((float)p.xi * sq, z1, fmaf(-(float)p.kdt, vp, (float)p.kth));
+    }
+    if (p.payoff == KG_HESTON_ASIAN_ARITHMETIC)
+      sum += (double)f32::exp(x);
+    else if (p.payoff == KG_HESTON_ASIAN_GEOMETRIC)
+      sum += (double)x;
+  }
+};
+
+template <bool FP32>
+__device__ inline double leg_value(const HestonParams& p, const Leg<FP32>& l, uint32_t& bad) {
+  double u;
+  if (p.payoff == KG_HESTON_EUROPEAN)
+    u = p.spot * exp((double)l.x);
+  else if (p.payoff == KG_HESTON_ASIAN_ARITHMETIC)
+    u = p.spot * (l.sum / p.steps);
+  else
+    u = p.spot * exp(l.sum / p.steps);
+  const double v = fmax(p.sign * (u - p.strike), 0.0) * exp(p.log_discount);
+  if (!isfinite(v)) bad = 1;
+  return v;
+}
+
+template <bool FP32>
+__global__ void __launch_bounds__(256)
+    heston_kernel(HestonParams p, uint64_t seed, int64_t samples, int64_t paths, int antithetic,
+                  Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
+  __shared__ Moments smem[2 * 4];
+  const int64_t block = blockIdx.x;
+  const int j = threadIdx.x;
+  const int64_t n = block * BLOCK_SAMPLES + j;
+  const bool live = n < samples;
+  const bool paired = antithetic && n < paths / 2;
+  uint32_t bad = 0;
+  Moments m[2] = {moments_zero(), moments_zero()};
+  if (live) {
+    Pcg32 g;
+    g.init(seed, (uint64_t)block);
+    g.advance((uint64_t)j * (uint64_t)p.steps * 4);
+    Leg<FP32> pos, neg;
+    pos.init(p);
+    neg.init(p);
+    for (int k = 0; k < p.steps; ++k) {
+      if constexpr (FP32) {
+        float z1, z2;
+        f32::box_muller_pair(g, &z1, &z2);
+        pos.step(p, z1, z2);
+        if (paired) neg.step(p, -z1, -z2);
+      } else {
+        double z1, z2;
+        box_muller_pair(g, &z1, &z2);
+        pos.step(p, z1, z2);
+        if (paired) neg.step(p, -z1, -z2);
+      }
+    }
+    const double vp = leg_value<FP32>(p, pos, bad);
+    moments_add(m[0], vp, true);
+    if (paired) {
+      const double vn = leg_value<FP32>(p, neg, bad);
+      moments_add(m[0], vn, true);
+      moments_add(m[1], midpoint(vp, vn), true);
+    }
+  }
+  if (bad) raise_flag(flags, FLAG_NONFINITE);
+  block_merge<2>(m, smem);
+  if (threadIdx.x == 0) {
+    partials[block * 2 + 0] = m[0];
+    partials[block * 2 + 1] = m[1];
+  }
+}
+
+}  // namespace kg
+
+using namespace kg;
+
+extern "C" kg_status kg_heston_price(kg_context* ctx, const kg_settings* settings,
+                                     const kg_heston_terms* option,
+                                     const kg_heston_market* market, kg_estimate* out) {
+  return guarded(ctx, [&] {
+    if (!option || !market || !out) fail(KG_INVALID_INPUT, "arguments must not be null");
+    const kg_heston_terms& o = *option;
+    const kg_heston_market& m = *market;
+    if (o.option_type != 1 && o.option_type != -1)
+      fail(KG_INVALID_INPUT, "option_type must be 1 (call) or -1 (put)");
+    if (o.payoff < KG_HESTON_EUROPEAN || o.payoff > KG_HESTON_ASIAN_GEOMETRIC)
+      fail(KG_INVALID_INPUT, "unknown payoff");
+    if (o.step_count < 1 || o.step_count > 10000)
+      fail(KG_INVALID_INPUT, "step_count is outside the supported Monte Carlo range");
+    if (!(o.strike > 0.0) || !std::isfinite(o.strike)) fail(KG_INVALID_INPUT, "strike must be positive");
+    if (!(m.spot > 0.0) || !std::isfinite(m.spot)) fail(KG_INVALID_INPUT, "spot must be positive");
+    // HestonParameters::new (market/heston.rs:100-125)
+    for (double v : {m.v0, m.theta, m.kappa, m.xi})
+      if (!std::isfinite(v) || v < 0.0)
+        fail(KG_INVALID_INPUT, "Heston variances, speed and vol of vol must be finite and nonnegative");
+    if (!std::isfinite(m.rho) || m.rho < -1.0 || m.rho > 1.0)
+      fail(KG_INVALID_INPUT, "correlation must be finite and in [-1, 1]");
+    if (m.valuation_time < o.effective_time || m.valuation_time > o.expiry_time)
+      fail(KG_INVALID_INPUT, "valuation outside contract life");
+    if (m.valuation_time == o.expiry_time)
+      fail(KG_INVALID_INPUT, "the Heston path engine needs time to expiry");
+    if (settings && settings->sampling == KG_SOBOL)
+      fail(KG_UNSUPPORTED_OPERATION, "Heston path stepping draws pseudorandom normals only");
+    Sampler s = validate_settings(settings, 2 * (uint64_t)o.step_count);
+    const double T = (double)(o.expiry_time - m.valuation_time) / (365.0 * 86400e6);
+    const double dt = T / o.step_count;
+    HestonParams p{};
+    p.payoff = o.payoff;
+    p.steps = o.step_count;
+    p.sign = o.option_type;
+    p.strike = o.strike;
+    p.spot = m.spot;
+    p.a = (m.rate - m.dividend) * dt;
+    p.hdt = 0.5 * dt;
+    p.kdt = m.kappa * dt;
+    p.kth = m.kappa * m.theta * dt;
+    p.xi = m.xi;
+    p.rho = m.rho;
+    p.rho_c = std::sqrt(1.0 - m.rho * m.rho);
+    p.sdt = std::sqrt(dt);
+    p.v0 = m.v0;
+    p.log_discount = -m.rate * T;
+    const int64_t blocks = s.blocks();
+    Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 2 * blocks);
+    uint32_t* flags = reset_flags(ctx);
+    *out = kg_estimate{};
+    timing_begin(ctx);
+    if (s.fp32)
+      hipLaunchKernelGGL(heston_kernel<true>, dim3(blocks), dim3(256), 0, ctx->stream, p, s.seed,
+                         s.samples(), s.path_count, (int)s.antithetic, parts, flags);
+    else
+      hipLaunchKernelGGL(heston_kernel<false>, dim3(blocks), dim3(256), 0, ctx->stream, p, s.seed,
+                         s.samples(), s.path_count, (int)s.antithetic, parts, flags);
+    KG_HIP(hipGetLastError());
+    out->device_ms = timing_end(ctx);
+    check_flags(ctx);
+    Moments r[2];
+    merge_and_fetch(ctx, parts, blocks, 2, r);
+    out->path_steps = (double)s.path_count * (double)o.step_count;
+    fill_estimate(s, r[0], r[1], 0.0, false, true, out);
+  });
+}

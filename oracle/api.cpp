@@ -4,7 +4,9 @@
 #include <cstdio>
 #include <mutex>
 
+#include "book.hpp"
 #include "core.hpp"
+#include "heston.hpp"
 #include "plans.hpp"
 
 using namespace oracle;
@@ -149,6 +151,38 @@ int oracle_autocallable_price(const kg_settings* s, const kg_autocall_terms* t,
   return guarded([&] {
     Settings st = to_settings(s, workers);
     write(autocall_price(st, *t, *m, context()), out);
+  });
+}
+
+int oracle_heston_price(const kg_settings* s, const kg_heston_terms* o, const kg_heston_market* m,
+                        int workers, kg_estimate* out) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    write(heston_price(st, *o, *m, context()), out);
+  });
+}
+
+int oracle_barrier_price(const kg_settings* s, const kg_barrier_terms* o, const kg_bsm_market* m,
+                         int workers, kg_estimate* out) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    write(barrier_price(st, *o, *m, context()), out);
+  });
+}
+
+int oracle_asian_price(const kg_settings* s, const kg_asian_terms* o, const kg_bsm_market* m,
+                       int workers, kg_estimate* out) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    write(asian_price(st, *o, *m, context()), out);
+  });
+}
+
+int oracle_accumulator_price(const kg_settings* s, const kg_accumulator_terms* o,
+                             const kg_bsm_market* m, int workers, kg_estimate* out) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    write(accumulator_price(st, *o, *m, context()), out);
   });
 }
 

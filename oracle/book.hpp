@@ -1,0 +1,509 @@
+// Test infrastructure, not product code: FP64 CPU restatements of kiyosi's
+// barrier, Asian and accumulator Monte Carlo estimators (value only), the
+// checker of the HIP book kernel.
+#pragma once
+
+#include "../include/kiyosi_amd.h"
+#include "core.hpp"
+#include "plans.hpp"
+
+namespace oracle {
+
+// ---------------------------------------------------------------- numerics
+// numerics.rs:192-215
+inline double normal_cdf(double x) { return 0.5 * std::erfc(-x * 0.7071067811865476); }
+inline double log_normal_tail(double value, double exponent) {
+  double inv_sq = (1.0 / value) / value;
+  double term = 1.0, sum = 1.0;
+  for (int i = 1; i <= 20; ++i) {
+    term *= -(double)(2 * i - 1) * inv_sq;
+    sum += term;
+  }
+  return exponent - std::log(-value) + std::log(sum / std::sqrt(2.0 * M_PI));
+}
+inline double log_normal_cdf(double value) {
+  if (value >= -10.0) return std::log(normal_cdf(value));
+  return log_normal_tail(value, -0.5 * value * value);
+}
+inline double log_add(double a, double b) {  // numerics.rs:320-327
+  double high = std::max(a, b);
+  if (high == -INFINITY) return high;
+  return high + std::log1p(std::exp(-std::fabs(a - b)));
+}
+inline double log_difference(double high, double low) {  // numerics.rs:329-334
+  if (high == -INFINITY) return -INFINITY;
+  return high + std::log(-std::expm1(low - high));
+}
+inline double positive_difference(double a, double b) {  // asian.rs:971-984
+  if (a == INFINITY) return a;
+  if (b == -INFINITY) return std::exp(a);
+  double d = -std::expm1(b - a);
+  return d > 0.0 ? std::exp(a + std::log(d)) : 0.0;
+}
+inline double weighted_cdf(double amount, double exponent, double argument) {  // numerics.rs:217-224
+  double discount = std::exp(exponent);
+  double discounted = amount * discount;
+  if (argument < -10.0 || !std::isnormal(discount) || !std::isfinite(discounted))
+    return std::exp(std::log(amount) + exponent + log_normal_cdf(argument));
+  return discounted * normal_cdf(argument);
+}
+inline double standardize(double value, double vol, double root) {  // numerics.rs:160-167
+  double width = vol * root;
+  if (width < std::numeric_limits<double>::min()) return (value / vol) / root;
+  return value / width;
+}
+// lognormal_price (engines/analytic/european.rs:317-400), the branch for
+// widths of at least 1e-5.
+inline double lognormal_price(double spot, double strike, double log_moneyness, double asset_log,
+                              double risk_log, double vol, double root_time, double sign) {
+  double width = vol * root_time;
+  if (!std::isfinite(width)) unstable();
+  if (width < 1e-5) throw Failure(UNSUPPORTED, "lognormal price below width 1e-5");
+  double st = standardize(log_moneyness, vol, root_time);
+  double d1 = st + 0.5 * width, d2 = st - 0.5 * width;
+  double asset = weighted_cdf(spot, asset_log, sign * d1);
+  double cash = weighted_cdf(strike, risk_log, sign * d2);
+  double direct = sign * (asset - cash);
+  double value;
+  if (std::isfinite(direct)) {
+    value = direct;
+  } else {
+    double a = log_moneyness + log_normal_cdf(sign * d1);
+    double c = log_normal_cdf(sign * d2);
+    value = std::exp(std::log(strike) + risk_log + (sign > 0 ? log_difference(a, c) : log_difference(c, a)));
+  }
+  if (!std::isfinite(value)) unstable();
+  return value;
+}
+
+// Means and pair statistics of N path functionals as path_means returns them
+// (monte_carlo.rs:1475-1496).
+struct Means {
+  double mean;
+  bool has_se;
+  double se;
+};
+inline Means path_mean(const Settings& st, const Moments& m, const Moments& p) {
+  Means r;
+  r.mean = scaled_value(m.scale, m.mean, 0.0);
+  r.has_se = sampling_standard_error(st, m, p, 0.0, &r.se);
+  return r;
+}
+inline Estimate from_sampling(const Settings& st, double price, bool has_se, double se) {
+  Estimate e;
+  e.price = price;
+  e.has_seed = 1;
+  e.seed = st.seed;
+  if (has_se && se == 0.0) {
+    e.status = ZERO_OBSERVED_VARIANCE;
+  } else if (has_se) {
+    e.status = AVAILABLE;
+    e.has_standard_error = 1;
+    e.standard_error = se;
+  } else if (st.sampling == SOBOL) {
+    e.status = UNSUPPORTED_SAMPLING;
+  } else if (st.antithetic && st.path_count % 2) {
+    e.status = INCOMPLETE_ANTITHETIC_PAIR;
+  } else {
+    e.status = INSUFFICIENT_OBSERVATIONS;
+  }
+  return e;
+}
+inline Estimate known_estimate(const Settings& st, double value) {
+  if (!std::isfinite(value)) unstable();
+  Estimate e;
+  e.price = value;
+  e.has_seed = 0;
+  if (st.sampling == PSEUDO_RANDOM) {
+    e.status = AVAILABLE;
+    e.has_standard_error = 1;
+    e.standard_error = 0.0;
+  } else {
+    e.status = UNSUPPORTED_SAMPLING;
+  }
+  return e;
+}
+
+// ---------------------------------------------------------------- barrier
+inline Estimate barrier_price(const Settings& st, const kg_barrier_terms& o, const kg_bsm_market& m,
+                              const Context& cx) {
+  const int64_t val = m.valuation_time, expiry = o.expiry_time;
+  if (val < o.effective_time || val > expiry)
+    throw Failure(INVALID_INPUT, "valuation outside contract life");
+  if (m.volatility == 0.0) throw Failure(UNSUPPORTED, "zero volatility is deterministic");
+  const bool up = o.barrier_type >= KG_UP_AND_OUT;
+  const bool knock_in = o.barrier_type == KG_DOWN_AND_IN || o.barrier_type == KG_UP_AND_IN;
+  const bool continuous = o.observation == KG_OBSERVE_CONTINUOUS;
+  const bool at_hit = o.rebate_timing == KG_REBATE_AT_HIT;
+  const double sign = o.option_type;
+  const double r = m.rate, q = m.dividend, sigma = m.volatility, S = m.spot;
+  const double T = years_until(val, expiry);
+  // resolved_price_with_state (engines/analytic/barrier.rs:32-80) for a note
+  // without earlier touches: touched now, expired, or a knock-out whose
+  // payoff needs the touch.
+  bool observed_now = continuous;
+  for (int i = 0; i < o.n_observations && !continuous; ++i) observed_now |= o.observation_times[i] == val;
+  const bool touched = observed_now && (up ? S >= o.barrier : S <= o.barrier);
+  const bool finished = val == expiry;
+  bool last_is_expiry = continuous || (o.n_observations > 0 && o.observation_times[o.n_observations - 1] == expiry);
+  const bool requires_touch = last_is_expiry && (up ? (sign > 0 && o.strike >= o.barrier)
+                                                    : (sign < 0 && o.strike <= o.barrier));
+  auto cash = [&] { return o.rebate == 0.0 ? 0.0 : o.rebate * std::exp(-r * T); };
+  auto vanilla = [&] {
+    if (T == 0.0) return std::max(sign * (S - o.strike), 0.0);
+    return lognormal_price(S, o.strike, log_ratio(S, o.strike) + (r - q) * T, -q * T, -r * T, sigma,
+                           std::sqrt(T), sign);
+  };
+  if (touched && !knock_in) return known_estimate(st, at_hit ? o.rebate : cash());
+  if (!touched && finished && knock_in) return known_estimate(st, cash());
+  const bool fixed = o.rebate == 0.0 && requires_touch;
+  if (fixed && !knock_in) return known_estimate(st, 0.0);
+  if (touched || finished || fixed) return known_estimate(st, vanilla());
+  if (continuous && !knock_in && at_hit && o.rebate != 0.0)
+    throw Failure(UNSUPPORTED, "continuous at-hit rebates need the analytic hit value");
+  // Plan (barrier.rs:228-290).
+  const double carry = r - q, hvr = 0.5 * sigma * sigma, drift = carry - hvr;
+  auto mean_distance = [&](int64_t date, double level) {
+    double e = years_until(val, date);
+    return log_ratio(S, level) + carry * e - hvr * e;
+  };
+  struct Step {
+    double mean, width, discount, lower;
+    bool observe;
+    int64_t date;
+  };
+  std::vector<Step> steps;
+  int64_t prev = val;
+  double width_sum = 0.0;
+  std::vector<int64_t> dates;
+  if (!continuous)
+    for (int i = 0; i < o.n_observations; ++i)
+      if (o.observation_times[i] > val) dates.push_back(o.observation_times[i]);
+  std::vector<std::pair<int64_t, bool>> all;
+  for (int64_t d : dates) all.push_back({d, true});
+  if (dates.empty() || dates.back() != expiry) all.push_back({expiry, false});
+  for (auto [date, observe] : all) {
+    double dt = years_until(prev, date);
+    double width = sigma * std::sqrt(dt);
+    double elapsed = years_until(val, date);
+    double mean = drift * elapsed;
+    if (width <= 0.0 || !std::isfinite(width) || !std::isfinite(mean)) unstable();
+    steps.push_back({mean, width, -r * elapsed, observe ? mean_distance(date, o.barrier) : 0.0, observe, date});
+    width_sum += width;
+    prev = date;
+  }
+  const double moneyness = mean_distance(expiry, o.strike);
+  const double barrier = log_ratio(o.barrier, S);
+  const double terminal_barrier = mean_distance(expiry, o.barrier);
+  const double log_discount = -r * T;
+  const bool bounded = std::fabs(std::log(S)) + 10.0 * width_sum + std::fabs(drift * T) < 600.0;
+  const bool condition_on_hit = knock_in && !continuous && steps.size() > 8 && bounded;
+  auto weighted = [&](double amount, double lw) {
+    return scaled_value(std::fabs(amount), amount < 0 ? -1.0 : 1.0, log_discount + lw);
+  };
+  Moments total[1], pairs[1];
+  path_samples_lazy<1>(
+      st, steps.size(), true, cx,
+      [&](LazyPath& normals, double* out) {
+        double terminal = 0.0, noise = 0.0, knocked_out = 0.0;
+        bool touched_p = false, has_cond = false;
+        double conditioned = 0.0;
+        for (size_t index = 0; index < steps.size(); ++index) {
+          const Step& s = steps[index];
+          double z = normals.normal();
+          noise = std::fma(s.width, z, noise);
+          terminal = s.mean + noise;
+          if (!bounded && scaled_value(S, 1.0, terminal) <= 0.0) unstable();
+          if (s.observe && (up ? noise >= -s.lower : noise <= -s.lower)) {
+            touched_p = true;
+            if (condition_on_hit && steps.size() - index > 8) {
+              double asset = scaled_value(S, 1.0, terminal);
+              double remaining = years_until(s.date, expiry);
+              conditioned = lognormal_price(asset, o.strike, moneyness + noise + hvr * remaining,
+                                            s.discount - q * remaining, log_discount, sigma,
+                                            std::sqrt(remaining), sign);
+              has_cond = true;
+              break;
+            }
+          }
+          if (touched_p && !knock_in) {
+            knocked_out = scaled_value(o.rebate, 1.0, at_hit ? s.discount : log_discount);
+            break;
+          }
+        }
+        double asset = scaled_value(S, 1.0, terminal);
+        double distance = moneyness + noise;
+        double payoff = std::fabs(distance) < 0.5 ? o.strike * std::expm1(distance) : asset - o.strike;
+        double intrinsic = std::max(sign * payoff, 0.0);
+        double value;
+        if (has_cond) {
+          value = conditioned;
+        } else if (continuous) {
+          double w0 = steps[0].width;
+          double dtb = terminal_barrier + noise;
+          bool live = up ? dtb < 0.0 : dtb > 0.0;
+          double slope = live ? 2.0 * (std::fabs(dtb) / w0) / w0 : 0.0;
+          double exponent = -std::fabs(barrier) * slope;
+          double survival = live ? std::log(-std::expm1(exponent)) : -INFINITY;
+          double rebate = (!knock_in && at_hit) ? 0.0 : o.rebate;
+          value = weighted(intrinsic, knock_in ? exponent : survival) +
+                  weighted(rebate, knock_in ? survival : exponent);
+        } else if (touched_p && !knock_in) {
+          value = knocked_out;
+        } else {
+          value = weighted(touched_p == knock_in ? intrinsic : o.rebate, 0.0);
+        }
+        if (!std::isfinite(value)) unstable();
+        out[0] = value;
+      },
+      total, pairs);
+  Means mm = path_mean(st, total[0], pairs[0]);
+  return from_sampling(st, mm.mean, mm.has_se, mm.se);
+}
+
+// ---------------------------------------------------------------- Asian
+struct AsianArith {
+  double spot, strike, weight, log_discount, drift, width, log_width, sign;
+  int n;
+  double interval_forward;
+};
+
+inline double geometric_values(const AsianArith& p, double lf, double ls, double width) {
+  double d1 = (lf - ls) / width + 0.5 * width;
+  double d2 = d1 - width;
+  double la = lf + p.log_discount + log_normal_cdf(p.sign * d1);
+  double lc = ls + p.log_discount + log_normal_cdf(p.sign * d2);
+  return p.sign > 0 ? positive_difference(la, lc) : positive_difference(lc, la);
+}
+
+inline Estimate asian_price(const Settings& st, const kg_asian_terms& o, const kg_bsm_market& m,
+                            const Context& cx) {
+  const int64_t val = m.valuation_time;
+  if (val < o.effective_time || val > o.expiry_time)
+    throw Failure(INVALID_INPUT, "valuation must be within contract life");
+  if (val > o.averaging_start_time) throw Failure(UNSUPPORTED, "seasoned averages are not restated");
+  if (o.step_count < 2 || o.step_count > 10000) throw Failure(INVALID_INPUT, "step_count out of range");
+  const double tau = years_until(val, o.expiry_time);
+  const double sign = o.option_type;
+  if (tau == 0.0) return known_estimate(st, std::max(sign * (m.spot - o.strike), 0.0));
+  if (m.volatility == 0.0) throw Failure(UNSUPPORTED, "zero volatility is deterministic");
+  const double period = years_until(o.averaging_start_time, o.expiry_time);
+  const bool single = period == 0.0;
+  if (single) throw Failure(UNSUPPORTED, "single-fixing averages are not restated");
+  const double lead = std::max(years_until(val, o.averaging_start_time), 0.0);
+  const double future = years_until(std::max(val, o.averaging_start_time), o.expiry_time);
+  const double weight = future / period;
+  const double sigma = m.volatility, r = m.rate, q = m.dividend;
+  const double carry = r - q;
+  const double drift_rate = carry - 0.5 * sigma * sigma;
+  const double log_discount = -r * tau;
+  Moments total[1], pairs[1];
+  if (o.averaging == KG_AVERAGE_GEOMETRIC) {
+    const double vt = lead + future / 3.0;
+    const double im = weight * log_ratio(m.spot, o.strike);
+    const double drift = weight * drift_rate * (lead + future / 2.0);
+    const double width = sigma * (weight * std::sqrt(vt));
+    const double log_width = std::log(sigma) + std::log(weight) + 0.5 * std::log(vt);
+    const double log_strike = std::log(o.strike);
+    path_samples<1>(
+        st, 1, true, cx,
+        [&](const double* n, double* out) {
+          // GeometricParameters::geometric (asian.rs:741-784)
+          double distance = im + drift + width * n[0];
+          double center = (im + drift) / width;
+          double standardized = center + n[0];
+          if (sign * standardized <= 0.0) {
+            out[0] = 0.0;
+            return;
+          }
+          double lp;
+          if (std::fabs(distance) < 1e-5 && std::isfinite(standardized))
+            lp = log_strike + log_width + std::log(std::fabs(standardized)) +
+                 (distance == 0.0 ? 0.0 : std::log(std::fabs(std::expm1(distance)) / std::fabs(distance)));
+          else if (sign > 0)
+            lp = log_strike + distance + std::log(-std::expm1(-distance));
+          else
+            lp = log_strike + std::log(-std::expm1(distance));
+          out[0] = std::exp(lp + log_discount);
+          if (!std::isfinite(out[0])) unstable();
+        },
+        total, pairs);
+    Means mm = path_mean(st, total[0], pairs[0]);
+    return from_sampling(st, mm.mean, mm.has_se, mm.se);
+  }
+  if (lead > 0.0) throw Failure(UNSUPPORTED, "forward-start arithmetic averages are not restated");
+  AsianArith p;
+  p.spot = m.spot;
+  p.strike = o.strike;
+  p.weight = weight;
+  p.log_discount = log_discount;
+  p.n = o.step_count;
+  const double dt = future / o.step_count;
+  p.interval_forward = carry * dt;
+  p.drift = drift_rate * dt;
+  p.width = sigma * std::sqrt(dt);
+  p.log_width = std::log(sigma) + 0.5 * std::log(dt);
+  p.sign = sign;
+  const size_t dims = (size_t)o.step_count - 1;
+  // geometric_control (asian.rs:1016-1061)
+  bool control = false;
+  double c_log_strike = 0, c_las = 0, c_fw = 0, c_expected = 0;
+  {
+    const double n = o.step_count;
+    const double fw = 1.0 - 0.5 / n;
+    const double las = std::log(p.spot) + std::log(p.weight);
+    const double ls = std::log(p.strike);
+    if (!(p.width < 1e-5 || p.width > 0.8 || std::fabs(p.log_discount) > 50.0 ||
+          std::fabs(std::log(p.spot)) > 200.0 || std::fabs(las) > 200.0 || !std::isfinite(ls))) {
+      double mean = 0.0 + fw * p.drift;
+      for (int s = 1; s < o.step_count; ++s) mean += (fw - (double)s / n) * p.drift;
+      double variance = 0.0 * 0.0 + (p.width * p.width) * (n / 3.0 - 1.0 / (12.0 * n));
+      double w = std::sqrt(variance);
+      double lf = las + mean + 0.5 * variance;
+      if (w >= 0.02 && w <= 0.8 && std::fabs(lf - ls) <= 2.0 * w) {
+        control = true;
+        c_log_strike = ls;
+        c_las = las;
+        c_fw = fw;
+        c_expected = geometric_values(p, lf, ls, w);
+      }
+    }
+  }
+  auto run = [&](bool use_control) {
+    path_samples<1>(
+        st, dims, true, cx,
+        [&](const double* normals, double* out) {
+          // ArithmeticGrid::path (asian.rs:640-712), flat rates, no lead
+          double growth = 0.0, rel = 0.0, log_sum = -INFINITY, gg = 0.0;
+          for (size_t i = 0; i < dims; ++i) {
+            growth += p.drift + p.width * normals[i];
+            double fraction = ((int)i + 2 == p.n ? 0.5 : 1.0) / (double)p.n;
+            if (use_control) gg += fraction * growth;
+            if (growth > 600.0 || log_sum != -INFINITY) {
+              if (log_sum == -INFINITY) log_sum = std::log(1.0 - 0.5 / (double)p.n + rel);
+              log_sum = log_add(log_sum, std::log(fraction) + growth);
+            } else {
+              rel += fraction * std::expm1(growth);
+            }
+          }
+          const double half = 0.5 / (double)p.n, rest = 1.0 - half;
+          const double log_spot = std::log(p.spot);
+          const double log_rest = log_sum == -INFINITY ? std::log(rest) + std::log1p(rel / rest) : log_sum;
+          const double log_q = log_spot + std::log(p.weight) + 0.0 + log_rest + p.interval_forward;
+          const double log_c = log_spot + std::log(p.weight) + 0.0 + std::log(half);
+          const double log_accrued = log_add(-INFINITY, log_c);
+          const bool has_extra = log_sum == -INFINITY;
+          const double extra = has_extra ? std::expm1(0.0) + std::exp(0.0) * (rest * std::expm1(p.interval_forward) +
+                                                                              rel * std::exp(p.interval_forward))
+                                         : 0.0;
+          // ArithmeticParameters::arithmetic (asian.rs:1100-1256)
+          double residual = 0.0;
+          if (use_control) {
+            double width = c_fw * p.width;
+            double lf = c_las + 0.0 + gg + c_fw * p.drift + 0.5 * width * width;
+            residual = geometric_values(p, lf, c_log_strike, width);
+          }
+          const double log_strike = std::log(p.strike);
+          double price;
+          if (log_accrued >= log_strike) {
+            if (p.sign < 0) {
+              price = 0.0;
+            } else {
+              double c = positive_difference(log_accrued + p.log_discount, log_strike + p.log_discount);
+              price = std::exp(log_q + p.log_discount) + c;
+            }
+          } else {
+            const double log_b = log_strike + std::log(-std::expm1(log_accrued - log_strike));
+            double forward = log_q - log_b;
+            const double baseline = 0.0 + p.weight * ((p.spot - p.strike) / p.spot);
+            if (has_extra) {
+              double excess = baseline + p.weight * extra;
+              double b = std::exp(log_b - log_spot);
+              if (b > 0.0 && std::fabs(excess / b) < 0.5) forward = std::log1p(excess / b);
+            }
+            if (!(p.width >= 1e-5)) throw Failure(UNSUPPORTED, "arithmetic widths below 1e-5");
+            double center = forward / p.width;
+            double d1 = center + 0.5 * p.width, d2 = center - 0.5 * p.width;
+            double la = log_q + p.log_discount + log_normal_cdf(p.sign * d1);
+            double lc = log_b + p.log_discount + log_normal_cdf(p.sign * d2);
+            price = p.sign > 0 ? positive_difference(la, lc) : positive_difference(lc, la);
+          }
+          out[0] = price - residual;
+          if (!std::isfinite(out[0])) unstable();
+        },
+        total, pairs);
+  };
+  run(control);
+  Means mm = path_mean(st, total[0], pairs[0]);
+  double price = mm.mean + (control ? c_expected : 0.0);
+  if (price < 0.0) {  // asian.rs:744-752: resample without the control
+    run(false);
+    mm = path_mean(st, total[0], pairs[0]);
+    price = mm.mean;
+  }
+  return from_sampling(st, price, mm.has_se, mm.se);
+}
+
+// ---------------------------------------------------------------- accumulator
+inline Estimate accumulator_price(const Settings& st, const kg_accumulator_terms& o,
+                                  const kg_bsm_market& m, const Context& cx) {
+  const int64_t val = m.valuation_time;
+  if (val < o.effective_time || val >= o.expiry_time)
+    throw Failure(UNSUPPORTED, "valuation must lie inside the accumulation period");
+  if (m.volatility == 0.0) throw Failure(UNSUPPORTED, "zero volatility is deterministic");
+  const double sign = o.option_type;
+  const double S = m.spot;
+  const double log_spot = std::log(S);
+  const double lo = std::log(std::numeric_limits<double>::min()) + 1.0;
+  const double hi = std::log(std::numeric_limits<double>::max()) - 1.0;
+  const double safe_lo = std::max(lo, lo - log_spot), safe_hi = std::min(hi, hi - log_spot);
+  const double strike = log_ratio(o.strike, S), knock_out = log_ratio(o.knock_out_level, S);
+  // Daily fixings after valuation at the observation clock (accumulator.rs:150-175).
+  struct Step {
+    double drift, diffusion, discount;
+  };
+  std::vector<Step> steps;
+  int64_t prev = val;
+  for (int64_t d = day_of(val); d <= day_of(o.expiry_time); ++d) {
+    int64_t at = d * MICROS_PER_DAY + o.observation_clock_micros;
+    if (at <= val || at > o.expiry_time || !is_trading_day(o.calendar, d)) continue;
+    Step s;
+    transition(m.rate, m.dividend, m.volatility, years_until(prev, at), years_until(val, at), &s.drift,
+               &s.diffusion, &s.discount);
+    steps.push_back(s);
+    prev = at;
+  }
+  if (steps.empty()) throw Failure(UNSUPPORTED, "no fixing remains");
+  const double terminal_discount = steps.back().discount;
+  auto unit_payoff = [&](double spot, double lr) {
+    double distance = lr - strike;
+    if (std::fabs(distance) < 0.5) return sign * o.strike * std::expm1(distance);
+    return sign > 0 ? spot - o.strike : o.strike - spot;
+  };
+  auto spot_at = [&](double lr) {
+    double s = scaled_value(S, 1.0, lr);
+    if (s <= 0.0) unstable();
+    return s;
+  };
+  Moments total[1], pairs[1];
+  path_samples_lazy<1>(
+      st, steps.size(), true, cx,
+      [&](LazyPath& normals, double* out) {
+        double lr = 0.0, quantity = o.accumulated_quantity;
+        for (const Step& s : steps) {
+          lr += std::fma(s.diffusion, normals.normal(), s.drift);
+          if (!(lr >= safe_lo && lr <= safe_hi)) spot_at(lr);
+          if (sign * lr >= sign * knock_out) {
+            out[0] = scaled_value(quantity, unit_payoff(spot_at(lr), lr), s.discount);
+            return;
+          }
+          quantity += o.daily_quantity * (sign * lr < sign * strike ? o.acceleration_factor : 1.0);
+          if (!std::isfinite(quantity)) unstable();
+        }
+        out[0] = scaled_value(quantity, unit_payoff(spot_at(lr), lr), terminal_discount);
+      },
+      total, pairs);
+  return estimate(st, total[0], pairs[0], 0.0, false, true, st.seed);
+}
+
+}  // namespace oracle

@@ -46,6 +46,8 @@ def lib():
             P(_abi.Settings), P(_abi.EuropeanTerms), P(_abi.BsmMarket), C.c_int, P(_abi.Estimate)]
         handle.oracle_autocallable_price.argtypes = [
             P(_abi.Settings), P(_abi.AutocallTerms), P(_abi.BsmMarket), C.c_int, P(_abi.Estimate)]
+        handle.oracle_heston_price.argtypes = [
+            P(_abi.Settings), P(_abi.HestonTerms), P(_abi.HestonMarket), C.c_int, P(_abi.Estimate)]
         handle.oracle_autocallable_plan.argtypes = [
             P(_abi.AutocallTerms), P(_abi.BsmMarket), P(C.c_double), C.c_size_t,
             P(C.c_size_t), P(C.c_int)]
@@ -117,6 +119,12 @@ def vanilla_price(settings, terms, market, workers: int = 1):
 def autocallable_price(settings, terms, market, workers: int = 1):
     out = _abi.Estimate()
     _check(lib().oracle_autocallable_price(C.byref(settings), C.byref(terms), C.byref(market), workers, C.byref(out)))
+    return out
+
+
+def heston_price(settings, terms, market, workers: int = 1):
+    out = _abi.Estimate()
+    _check(lib().oracle_heston_price(C.byref(settings), C.byref(terms), C.byref(market), workers, C.byref(out)))
     return out
 
 
