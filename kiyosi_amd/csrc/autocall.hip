@@ -17,289 +17,9 @@
 // has a bridge and payoff (:823-862) otherwise. Control flow is uniform per
 // wave: every lane walks the same steps with per-lane predicates, so plan
 // tables are read once per wave.
-#include <algorithm>
-
-#include "fp32_math.hpp"
-#include "host_common.hpp"
-#include "sobol_device.hpp"
+#include "autocall_common.hpp"
 
 namespace kg {
-
-constexpr int64_t MICROS_PER_DAY = 86400LL * 1000000LL;
-
-// ------------------------------------------------------------------ plan
-struct AcStepD {  // per step, FP64
-  double drift, diffusion;
-};
-struct AcEventD {
-  double knock_out;       // log level, or +inf without knock-out
-  double coupon_barrier;  // log level
-  double pay_coupon;      // period coupon * discount factor
-  double pay_out;         // redemption * discount factor
-  double pay_both;        // (redemption + period coupon) * discount factor
-  double pay_coupon_raw, pay_out_raw, pay_both_raw;  // undiscounted amounts
-  double log_discount;
-  int32_t has_knock_out;
-  int32_t pad;
-};
-struct AcIntervalD {
-  int32_t end;
-  int32_t pad;
-  double drift, diffusion;
-};
-
-struct AcPlanDev {
-  int32_t n_steps;
-  int32_t n_intervals;
-  int32_t bridge;
-  int32_t knocked_in;
-  int32_t has_ki;
-  int32_t expiry_knock_in;
-  double knock_in;  // log level
-  double coupon;    // current coupon from pricing::initial
-  double initial_spot;
-  double safe_lo, safe_hi;
-  double final_log_discount;
-  // terminal(spot, ki): principal + rebate + coupon*years or the downside
-  double term_base;       // principal + rebate(expiry)
-  double term_no_ki;      // coupon * years when not knocked in
-  double term_ki_fixed;   // coupon * years when knocked in with a fixed coupon
-  double lower, upper, inv_initial;
-  int32_t downside;       // knock-in redemption is the downside
-  int32_t pad;
-  const AcStepD* steps;
-  const int32_t* step_event;    // event index or -1
-  const uint8_t* step_observe;  // observe_knock_in
-  const AcEventD* events;
-  const AcIntervalD* intervals;
-  const double* weight;  // bridge fill weights per step
-  const double* fill;    // bridge fill diffusions per step
-  // FP32 copies of the per-step tables
-  const float* steps32;     // drift, diffusion pairs
-  const float* weight32;
-  const float* fill32;
-  const float* events32;    // knock_out, coupon_barrier pairs
-  const float* intervals32; // drift, diffusion pairs
-};
-
-template <typename R>
-struct Tab;
-template <>
-struct Tab<double> {
-  static __device__ inline double drift(const AcPlanDev& p, int i) { return p.steps[i].drift; }
-  static __device__ inline double diff(const AcPlanDev& p, int i) { return p.steps[i].diffusion; }
-  static __device__ inline double w(const AcPlanDev& p, int i) { return p.weight[i]; }
-  static __device__ inline double f(const AcPlanDev& p, int i) { return p.fill[i]; }
-  static __device__ inline double ko(const AcPlanDev& p, int e) { return p.events[e].knock_out; }
-  static __device__ inline double cb(const AcPlanDev& p, int e) { return p.events[e].coupon_barrier; }
-  static __device__ inline double idrift(const AcPlanDev& p, int k) { return p.intervals[k].drift; }
-  static __device__ inline double idiff(const AcPlanDev& p, int k) { return p.intervals[k].diffusion; }
-};
-template <>
-struct Tab<float> {
-  static __device__ inline float drift(const AcPlanDev& p, int i) { return p.steps32[2 * i]; }
-  static __device__ inline float diff(const AcPlanDev& p, int i) { return p.steps32[2 * i + 1]; }
-  static __device__ inline float w(const AcPlanDev& p, int i) { return p.weight32[i]; }
-  static __device__ inline float f(const AcPlanDev& p, int i) { return p.fill32[i]; }
-  static __device__ inline float ko(const AcPlanDev& p, int e) { return p.events32[2 * e]; }
-  static __device__ inline float cb(const AcPlanDev& p, int e) { return p.events32[2 * e + 1]; }
-  static __device__ inline float idrift(const AcPlanDev& p, int k) { return p.intervals32[2 * k]; }
-  static __device__ inline float idiff(const AcPlanDev& p, int k) { return p.intervals32[2 * k + 1]; }
-};
-
-// discounted_payment (autocallable.rs:552-561) with the discount factor
-// folded in on the host; the log-space form covers non-normal products.
-__device__ inline double event_payment(const AcEventD& e, bool due, bool out) {
-  double v = out ? (due ? e.pay_both : e.pay_out) : (due ? e.pay_coupon : 0.0);
-  if (v == 0.0 || isnormal(v)) return v;
-  double raw = out ? (due ? e.pay_both_raw : e.pay_out_raw) : (due ? e.pay_coupon_raw : 0.0);
-  return discounted_product(1.0, raw, e.log_discount);
-}
-
-// discounted_terminal (autocallable.rs:798-801) + AutocallableNote::terminal
-// (instruments/autocallable.rs:1396-1414).
-__device__ inline double terminal_value(const AcPlanDev& p, double lr, bool ki, uint32_t& bad) {
-  double spot = discounted_product(p.initial_spot, 1.0, lr);
-  if (!isfinite(spot)) bad = 1;
-  double amount;
-  if (ki && p.downside) {
-    double c = fmin(fmax(spot, p.lower), p.upper);
-    amount = p.term_base + (c - p.upper) * p.inv_initial;
-  } else {
-    amount = p.term_base + (ki ? p.term_ki_fixed : p.term_no_ki);
-  }
-  return discounted_product(1.0, amount, p.final_log_discount);
-}
-
-// Source of the normals of one sample.
-template <bool SOBOL, bool FP32>
-struct Draws {
-  // Sobol: LDS tables of this block and the lane's Gray-code digits
-  const void* lo;
-  const void* hi;
-  uint32_t glo, ghi;
-  int dims;
-  // Sobol fallback for long paths: point index and global directions
-  const uint64_t* dirs;
-  const uint64_t* shifts;
-  uint64_t point;
-  bool direct;
-  // PCG
-  NormalStream s;
-  Pcg32 g32;
-  float spare32;
-  bool has32;
-
-  __device__ inline double normal64(int d) {
-    if (SOBOL) {
-      uint64_t bits = direct ? sobol_point_direct(dirs, shifts, dims, point, d)
-                             : sobol_lds_bits64((const uint64_t*)lo, (const uint64_t*)hi, d, glo, ghi);
-      return normal_from_bits(bits);
-    }
-    return s.next();
-  }
-  __device__ inline float normal32(int d) {
-    if (SOBOL) {
-      uint32_t bits =
-          direct ? (uint32_t)(sobol_point_direct(dirs, shifts, dims, point, d) >> 20)
-                 : sobol_lds_bits32((const uint32_t*)lo, (const uint32_t*)hi, d, glo, ghi);
-      return f32::normal_from_bits(bits);
-    }
-    if (has32) {
-      has32 = false;
-      return spare32;
-    }
-    float e;
-    f32::box_muller_pair(g32, &e, &spare32);
-    has32 = true;
-    return e;
-  }
-  template <typename R>
-  __device__ inline R normal(int d) {
-    if constexpr (FP32)
-      return normal32(d);
-    else
-      return normal64(d);
-  }
-  __device__ inline void skip(int count) {
-    if (SOBOL || count <= 0) return;
-    if (FP32) {
-      if (has32) {
-        has32 = false;
-        count -= 1;
-      }
-      g32.advance((uint64_t)(count >> 1) * 4);
-      if (count & 1) (void)normal32(0);
-    } else {
-      s.skip((uint64_t)count);
-    }
-  }
-};
-
-// Bridge path (bridge_payoffs::<1, true>, autocallable.rs:628-736). `sign`
-// is -1 for the antithetic leg. Returns the discounted cash of the path.
-template <bool SOBOL, bool FP32>
-__device__ double bridge_path(const AcPlanDev& p, Draws<SOBOL, FP32>& dr, bool live, float sign,
-                              uint32_t& bad) {
-  using R = typename std::conditional<FP32, float, double>::type;
-  double cash = p.coupon;
-  bool monitoring = !p.knocked_in && p.has_ki;
-  const R level = (R)p.knock_in;
-  bool ki = p.knocked_in;
-  bool active = live;
-  R lr = 0;
-  int start = 0;
-  for (int k = 0; k < p.n_intervals; ++k) {
-    if (__ballot(active) == 0) break;
-    const int end = p.intervals[k].end;
-    R target = lr;
-    bool filling = false;
-    if (active) {
-      R z = (R)sign * dr.template normal<R>(start);
-      target = lr + (FP32 ? fmaf((float)Tab<R>::idiff(p, k), (float)z, (float)Tab<R>::idrift(p, k))
-                          : fma((double)Tab<R>::idiff(p, k), (double)z, (double)Tab<R>::idrift(p, k)));
-      filling = monitoring;
-      if (!filling) dr.skip(end - start);
-    }
-    R minimum = (R)INFINITY;
-    for (int i = start; i < end; ++i) {
-      if (__ballot(filling) == 0) break;
-      if (filling && minimum < level) {
-        dr.skip(end - i);
-        filling = false;
-      }
-      if (filling) {
-        R z = (R)sign * dr.template normal<R>(i + 1);
-        R w = Tab<R>::w(p, i), f = Tab<R>::f(p, i);
-        lr += FP32 ? fmaf((float)w, (float)(target - lr), (float)(f * z))
-                   : fma((double)w, (double)(target - lr), (double)(f * z));
-        minimum = fmin(minimum, lr);
-      }
-    }
-    lr = target;
-    start = end + 1;
-    if (active) {
-      if (monitoring && (minimum < level || (p.step_observe[end] && lr < level))) {
-        monitoring = false;
-        ki = true;
-      }
-      const int e = p.step_event[end];
-      if (e >= 0) {
-        const AcEventD& ev = p.events[e];
-        const bool hit = ev.has_knock_out && lr >= Tab<R>::ko(p, e);
-        cash += event_payment(ev, lr >= Tab<R>::cb(p, e), hit);
-        if (hit) {
-          active = false;
-          monitoring = false;
-        }
-      }
-    }
-  }
-  if (active) {
-    const bool hit = ki || (p.expiry_knock_in && p.has_ki && lr < level);
-    cash += terminal_value(p, (double)lr, hit, bad);
-  }
-  if (live && !isfinite(cash)) bad = 1;
-  return cash;
-}
-
-// Daily or event-only path (payoff, autocallable.rs:823-862).
-template <bool SOBOL, bool FP32>
-__device__ double step_path(const AcPlanDev& p, Draws<SOBOL, FP32>& dr, bool live, float sign,
-                            uint32_t& bad) {
-  using R = typename std::conditional<FP32, float, double>::type;
-  double cash = p.coupon;
-  bool ki = p.knocked_in;
-  const R level = (R)p.knock_in;
-  bool active = live;
-  R lr = 0;
-  for (int i = 0; i < p.n_steps; ++i) {
-    if (__ballot(active) == 0) break;
-    if (!active) continue;
-    R z = (R)sign * dr.template normal<R>(i);
-    lr += FP32 ? fmaf((float)Tab<R>::diff(p, i), (float)z, (float)Tab<R>::drift(p, i))
-               : fma((double)Tab<R>::diff(p, i), (double)z, (double)Tab<R>::drift(p, i));
-    if (!((double)lr >= p.safe_lo && (double)lr <= p.safe_hi)) {
-      // validate_return (autocallable.rs:606-613)
-      double s = discounted_product(p.initial_spot, 1.0, (double)lr);
-      if (!isfinite(s) || s <= 0.0) bad = 1;
-    }
-    if (p.step_observe[i]) ki = ki || (p.has_ki && lr < level);
-    const int e = p.step_event[i];
-    if (e >= 0) {
-      const AcEventD& ev = p.events[e];
-      const bool out = ev.has_knock_out && lr >= Tab<R>::ko(p, e);
-      cash += event_payment(ev, lr >= Tab<R>::cb(p, e), out);
-      if (out) active = false;
-    }
-  }
-  if (active) {
-    ki = ki || (p.expiry_knock_in && p.has_ki && lr < level);
-    cash += terminal_value(p, (double)lr, ki, bad);
-  }
-  if (live && !isfinite(cash)) bad = 1;
-  return cash;
-}
 
 template <bool SOBOL, bool FP32>
 __global__ void __launch_bounds__(256)
@@ -369,346 +89,6 @@ __global__ void __launch_bounds__(256)
     partials[block * 2 + 0] = m[0];
     partials[block * 2 + 1] = m[1];
   }
-}
-
-// ------------------------------------------------------------------ host plan
-static inline int64_t floor_div(int64_t a, int64_t b) {
-  int64_t q = a / b;
-  return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
-}
-static inline double years(int64_t a, int64_t b) { return (double)(b - a) / (365.0 * 86400e6); }
-static inline bool trading_day(int calendar, int64_t day) {
-  if (calendar == KG_CALENDAR_ALL_DAYS) return true;
-  return ((day % 7) + 7 + 3) % 7 < 5;  // 1970-01-01 was a Thursday
-}
-static inline void transition(double r, double q, double sigma, double time, double discount_time,
-                              double* drift, double* diffusion, double* log_discount) {
-  *drift = (r - q - 0.5 * sigma * sigma) * time;  // monte_carlo.rs:2123-2137
-  *diffusion = sigma * std::sqrt(time);
-  *log_discount = -r * discount_time;
-  if (!std::isfinite(*drift) || !std::isfinite(*diffusion) || !std::isfinite(*log_discount))
-    unstable();
-}
-
-struct AcHostPlan {
-  bool fixed = false;
-  double fixed_value = 0.0;
-  AcPlanDev dev{};
-  std::vector<AcStepD> steps;
-  std::vector<int32_t> step_event;
-  std::vector<uint8_t> step_observe;
-  std::vector<AcEventD> events;
-  std::vector<AcIntervalD> intervals;
-  std::vector<double> weight, fill;
-  std::vector<float> steps32, weight32, fill32, events32, intervals32;
-};
-
-struct NoteView {
-  const kg_autocall_terms& t;
-  bool has_ki() const { return t.knock_in_kind != KG_KNOCK_IN_NONE; }
-  bool daily() const { return t.knock_in_kind == KG_KNOCK_IN_DAILY; }
-  double maturity_coupon() const { return t.coupon_kind == KG_COUPON_SNOWBALL ? t.coupon : 0.0; }
-  double period_coupon() const { return t.coupon_kind == KG_COUPON_PHOENIX ? t.coupon : 0.0; }
-  double rebate(int64_t time) const {
-    return t.rebate_rate + t.annual_rebate_rate * years(t.effective_time, time);
-  }
-  double knock_out_payment(const kg_autocall_event& e) const {
-    return t.principal_redemption_ratio + rebate(e.time) +
-           e.coupon_rate * years(t.effective_time, e.time);
-  }
-  double terminal(double spot, bool ki) const {
-    double coupon = maturity_coupon();
-    if (ki && has_ki()) {
-      if (t.redemption_kind == KG_REDEMPTION_DOWNSIDE)
-        return t.principal_redemption_ratio + rebate(t.expiry_time) +
-               (std::min(std::max(spot, t.lower_strike), t.upper_strike) - t.upper_strike) /
-                   t.initial_spot;
-      coupon = t.fixed_coupon_rate;
-    }
-    return t.principal_redemption_ratio + rebate(t.expiry_time) +
-           coupon * years(t.effective_time, t.expiry_time);
-  }
-  bool affects() const {
-    if (!has_ki()) return false;
-    if (t.redemption_kind == KG_REDEMPTION_FIXED_COUPON) return t.fixed_coupon_rate != maturity_coupon();
-    return t.lower_strike != t.upper_strike || maturity_coupon() != 0.0;
-  }
-  bool daily_time(int64_t time) const {
-    return daily() && time >= t.effective_time && time <= t.expiry_time &&
-           time - floor_div(time, MICROS_PER_DAY) * MICROS_PER_DAY == t.observation_clock_micros &&
-           trading_day(t.calendar, floor_div(time, MICROS_PER_DAY));
-  }
-  bool observes(int64_t time) const {
-    if (t.knock_in_kind == KG_KNOCK_IN_DAILY) return daily_time(time);
-    return t.knock_in_kind == KG_KNOCK_IN_AT_EXPIRY && time == t.expiry_time;
-  }
-};
-
-static void validate_terms(const kg_autocall_terms& t) {
-  if (t.n_events < 0 || (t.n_events > 0 && !t.events))
-    fail(KG_INVALID_INPUT, "events must not be null");
-  if (!(t.initial_spot > 0.0)) fail(KG_INVALID_INPUT, "initial_spot must be positive");
-  if (t.expiry_time <= t.effective_time)
-    fail(KG_INVALID_INPUT, "expiry must follow the effective time");
-  for (int i = 0; i < t.n_events; ++i) {
-    if (i && t.events[i].time <= t.events[i - 1].time)
-      fail(KG_INVALID_INPUT, "observations must be strictly ordered");
-    if (t.events[i].time < t.effective_time || t.events[i].time > t.expiry_time)
-      fail(KG_INVALID_INPUT, "observations must lie within the contract life");
-  }
-  if (t.knock_in_kind != KG_KNOCK_IN_NONE && !(t.knock_in_level > 0.0))
-    fail(KG_INVALID_INPUT, "knock_in_level must be positive");
-  if (t.knock_in_kind != KG_KNOCK_IN_NONE && t.redemption_kind == KG_REDEMPTION_DOWNSIDE &&
-      !(0.0 <= t.lower_strike && t.lower_strike <= t.upper_strike))
-    fail(KG_INVALID_INPUT, "lower_strike must lie in [0, upper_strike]");
-}
-
-static AcHostPlan build_plan(const kg_autocall_terms& t, const kg_bsm_market& m) {
-  validate_terms(t);
-  NoteView note{t};
-  AcHostPlan h;
-  if (!(m.spot > 0.0) || !std::isfinite(m.spot)) fail(KG_INVALID_INPUT, "spot must be positive");
-  if (!(m.volatility >= 0.0)) fail(KG_INVALID_INPUT, "volatility must be nonnegative");
-  if (m.volatility == 0.0)
-    fail(KG_UNSUPPORTED_OPERATION,
-         "zero volatility selects the deterministic engine, which is not part of the GPU engine");
-  const int64_t valuation = m.valuation_time;
-  if (valuation < t.effective_time || valuation > t.expiry_time)
-    fail(KG_INVALID_INPUT, "valuation outside contract life");
-  // prior_with_work_check (instruments/autocallable.rs:1321-1370)
-  int64_t first_out = INT64_MAX;
-  for (int i = 0; i < t.n_events; ++i)
-    if (t.events[i].has_knock_out) {
-      first_out = t.events[i].time;
-      break;
-    }
-  bool had_out = first_out < valuation;
-  bool had_in = false;
-  if (note.daily() && valuation > t.effective_time) {
-    const int64_t end = std::min(valuation - 1, t.expiry_time);
-    for (int64_t d = floor_div(t.effective_time, MICROS_PER_DAY);
-         d <= floor_div(end, MICROS_PER_DAY) && !had_in; ++d) {
-      const int64_t at = d * MICROS_PER_DAY + t.observation_clock_micros;
-      had_in = at >= t.effective_time && at <= end && trading_day(t.calendar, d);
-    }
-  }
-  if (t.barrier_state == KG_STATE_UNKNOWN && (had_out || had_in))
-    fail(KG_MISSING_HISTORY, "prior autocallable state is required");
-  if (t.barrier_state == KG_STATE_KNOCKED_OUT && !had_out)
-    fail(KG_INVALID_INPUT, "knock-out predates observations");
-  if (t.barrier_state == KG_STATE_KNOCKED_IN && !had_in)
-    fail(KG_INVALID_INPUT, "knock-in predates monitoring");
-  // pricing::initial (instruments/autocallable/pricing.rs:12-46)
-  if (t.barrier_state == KG_STATE_KNOCKED_OUT) {
-    h.fixed = true;
-    return h;
-  }
-  bool knocked_in = t.barrier_state == KG_STATE_KNOCKED_IN;
-  if (note.daily_time(valuation)) knocked_in = knocked_in || (note.has_ki() && m.spot < t.knock_in_level);
-  const kg_autocall_event* current = nullptr;
-  for (int i = 0; i < t.n_events; ++i)
-    if (t.events[i].time == valuation) current = &t.events[i];
-  double coupon = 0.0;
-  if (current && current->has_coupon_barrier && m.spot >= current->coupon_barrier_level)
-    coupon = note.period_coupon();
-  if (valuation == t.expiry_time) {  // pricing::expiry_payment (:48-68)
-    double v;
-    if (current && current->has_knock_out && m.spot >= current->knock_out_level)
-      v = note.knock_out_payment(*current);
-    else
-      v = note.terminal(m.spot, knocked_in || (note.observes(t.expiry_time) && note.has_ki() &&
-                                               m.spot < t.knock_in_level));
-    h.fixed = true;
-    h.fixed_value = coupon + v;
-    return h;
-  }
-  if (current && current->has_knock_out && m.spot >= current->knock_out_level) {
-    h.fixed = true;
-    h.fixed_value = note.knock_out_payment(*current) + coupon;
-    return h;
-  }
-  const double r = m.rate, q = m.dividend, sigma = m.volatility;
-  const bool daily = note.daily() && !knocked_in && note.affects();
-  AcPlanDev& p = h.dev;
-  p.knocked_in = knocked_in;
-  p.coupon = coupon;
-  p.initial_spot = m.spot;
-  const double log_spot = std::log(m.spot);
-  const double low = std::log(std::numeric_limits<double>::min()) + 1.0;
-  const double high = std::log(std::numeric_limits<double>::max()) - 1.0;
-  p.safe_lo = std::max(low, low - log_spot);  // safe_log_returns (monte_carlo.rs:2139-2145)
-  p.safe_hi = std::min(high, high - log_spot);
-  p.has_ki = note.has_ki();
-  p.knock_in = p.has_ki ? log_ratio(t.knock_in_level, m.spot) : 0.0;
-  p.expiry_knock_in = note.observes(t.expiry_time);
-  // future_event_times (pricing.rs:70-106)
-  std::vector<int64_t> dates;
-  for (int i = 0; i < t.n_events; ++i)
-    if (t.events[i].time > valuation) dates.push_back(t.events[i].time);
-  if (daily)
-    for (int64_t d = floor_div(valuation, MICROS_PER_DAY); d <= floor_div(t.expiry_time, MICROS_PER_DAY);
-         ++d) {
-      const int64_t at = d * MICROS_PER_DAY + t.observation_clock_micros;
-      if (at > valuation && at <= t.expiry_time && trading_day(t.calendar, d)) dates.push_back(at);
-    }
-  dates.push_back(t.expiry_time);
-  std::sort(dates.begin(), dates.end());
-  dates.erase(std::unique(dates.begin(), dates.end()), dates.end());
-  if (dates.size() > 100000) fail(KG_INVALID_INPUT, "the note has more than 100,000 path steps");
-  int next = 0;
-  while (next < t.n_events && t.events[next].time <= valuation) ++next;
-  int64_t previous = valuation;
-  double excursion = 0.0, last_discount = 0.0;
-  for (int64_t date : dates) {
-    AcStepD s;
-    double disc;
-    transition(r, q, sigma, years(previous, date), years(valuation, date), &s.drift, &s.diffusion,
-               &disc);
-    excursion += std::fabs(s.drift) + 20.0 * s.diffusion;
-    int32_t ev = -1;
-    if (next < t.n_events && t.events[next].time == date) {
-      const kg_autocall_event& e = t.events[next++];
-      AcEventD c{};
-      c.has_knock_out = e.has_knock_out != 0;
-      c.knock_out = c.has_knock_out ? log_ratio(e.knock_out_level, m.spot) : INFINITY;
-      c.coupon_barrier = log_ratio(e.has_coupon_barrier ? e.coupon_barrier_level : 0.0, m.spot);
-      const double pc = e.has_coupon_barrier ? note.period_coupon() : 0.0;
-      const double red = c.has_knock_out ? note.knock_out_payment(e) : 0.0;
-      c.log_discount = disc;
-      c.pay_coupon_raw = pc;
-      c.pay_out_raw = red;
-      c.pay_both_raw = red + pc;
-      c.pay_coupon = h_discounted_product(1.0, pc, disc);
-      c.pay_out = h_discounted_product(1.0, red, disc);
-      c.pay_both = h_discounted_product(1.0, red + pc, disc);
-      ev = (int32_t)h.events.size();
-      h.events.push_back(c);
-    }
-    h.steps.push_back(s);
-    h.step_event.push_back(ev);
-    h.step_observe.push_back(daily && note.daily_time(date));
-    last_discount = disc;
-    previous = date;
-  }
-  p.final_log_discount = last_discount;
-  p.term_base = t.principal_redemption_ratio + note.rebate(t.expiry_time);
-  const double life = years(t.effective_time, t.expiry_time);
-  p.term_no_ki = note.maturity_coupon() * life;
-  p.term_ki_fixed = (t.redemption_kind == KG_REDEMPTION_FIXED_COUPON || !note.has_ki())
-                        ? (note.has_ki() ? t.fixed_coupon_rate : note.maturity_coupon()) * life
-                        : 0.0;
-  p.downside = note.has_ki() && t.redemption_kind == KG_REDEMPTION_DOWNSIDE;
-  p.lower = t.lower_strike;
-  p.upper = t.upper_strike;
-  p.inv_initial = 1.0 / t.initial_spot;
-  // BridgePlan::new (autocallable.rs:463-515)
-  const bool safe = excursion >= p.safe_lo && excursion <= p.safe_hi && -excursion >= p.safe_lo &&
-                    -excursion <= p.safe_hi;
-  if (daily && safe) {
-    std::vector<AcIntervalD> iv;
-    size_t first = 0;
-    bool gap = false;
-    for (size_t end = 0; end < h.steps.size(); ++end) {
-      if (h.step_event[end] >= 0 || end + 1 == h.steps.size()) {
-        gap |= end - first >= 8;
-        const int64_t start = first == 0 ? valuation : dates[first - 1];
-        AcIntervalD b{};
-        b.end = (int32_t)end;
-        double unused;
-        transition(r, q, sigma, years(start, dates[end]), 0.0, &b.drift, &b.diffusion, &unused);
-        iv.push_back(b);
-        first = end + 1;
-      }
-    }
-    if (gap) {
-      std::vector<double> w(h.steps.size(), 0.0), f(h.steps.size(), 0.0);
-      bool ok = true;
-      size_t from = 0;
-      int64_t prev = valuation;
-      for (const AcIntervalD& b : iv) {
-        const int64_t end_t = dates[b.end];
-        for (size_t i = from; i < (size_t)b.end; ++i) {
-          const double dt = years(prev, dates[i]);
-          const double fraction = dt / years(prev, end_t);
-          if (!(0.0 < fraction && fraction < 1.0)) {
-            ok = false;
-            break;
-          }
-          w[i] = fraction;
-          f[i] = sigma * std::sqrt(dt * (1.0 - fraction));
-          prev = dates[i];
-        }
-        if (!ok) break;
-        from = b.end + 1;
-        prev = end_t;
-      }
-      if (ok) {
-        p.bridge = 1;
-        h.intervals = iv;
-        h.weight = w;
-        h.fill = f;
-      }
-    }
-  }
-  p.n_steps = (int32_t)h.steps.size();
-  p.n_intervals = (int32_t)h.intervals.size();
-  for (const AcStepD& s : h.steps) {
-    h.steps32.push_back((float)s.drift);
-    h.steps32.push_back((float)s.diffusion);
-  }
-  for (double v : h.weight) h.weight32.push_back((float)v);
-  for (double v : h.fill) h.fill32.push_back((float)v);
-  for (const AcEventD& e : h.events) {
-    h.events32.push_back((float)e.knock_out);
-    h.events32.push_back((float)e.coupon_barrier);
-  }
-  for (const AcIntervalD& b : h.intervals) {
-    h.intervals32.push_back((float)b.drift);
-    h.intervals32.push_back((float)b.diffusion);
-  }
-  return h;
-}
-
-// Copies the plan tables into one device buffer and points the plan at them.
-template <class T>
-static void stage(std::vector<unsigned char>& blob, const std::vector<T>& v, size_t* offset) {
-  *offset = (blob.size() + 255) & ~size_t(255);
-  blob.resize(*offset + v.size() * sizeof(T) + 1);
-  if (!v.empty()) std::memcpy(blob.data() + *offset, v.data(), v.size() * sizeof(T));
-}
-
-static AcPlanDev upload(kg_context* ctx, AcHostPlan& h) {
-  std::vector<unsigned char> blob;
-  size_t o[12];
-  stage(blob, h.steps, &o[0]);
-  stage(blob, h.step_event, &o[1]);
-  stage(blob, h.step_observe, &o[2]);
-  stage(blob, h.events, &o[3]);
-  stage(blob, h.intervals, &o[4]);
-  stage(blob, h.weight, &o[5]);
-  stage(blob, h.fill, &o[6]);
-  stage(blob, h.steps32, &o[7]);
-  stage(blob, h.weight32, &o[8]);
-  stage(blob, h.fill32, &o[9]);
-  stage(blob, h.events32, &o[10]);
-  stage(blob, h.intervals32, &o[11]);
-  unsigned char* d = (unsigned char*)ctx->scratch.reserve(blob.size());
-  KG_HIP(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream));
-  AcPlanDev p = h.dev;
-  p.steps = (const AcStepD*)(d + o[0]);
-  p.step_event = (const int32_t*)(d + o[1]);
-  p.step_observe = (const uint8_t*)(d + o[2]);
-  p.events = (const AcEventD*)(d + o[3]);
-  p.intervals = (const AcIntervalD*)(d + o[4]);
-  p.weight = (const double*)(d + o[5]);
-  p.fill = (const double*)(d + o[6]);
-  p.steps32 = (const float*)(d + o[7]);
-  p.weight32 = (const float*)(d + o[8]);
-  p.fill32 = (const float*)(d + o[9]);
-  p.events32 = (const float*)(d + o[10]);
-  p.intervals32 = (const float*)(d + o[11]);
-  KG_HIP(hipStreamSynchronize(ctx->stream));
-  return p;
 }
 
 template <bool SOBOL, bool FP32>

@@ -1,0 +1,305 @@
+// Device path code of the barrier, Asian and accumulator engines, written
+// against the Draws normal source of autocall_common.hpp so that one
+// persistent kernel (book.hip) runs every contract type.
+//
+//   barrier_path   MonteCarloBarrierEngine sample closure
+//                  (engines/monte_carlo/barrier.rs:291-477), values only
+//   geo_asian      GeometricParameters::geometric (asian.rs:741-784)
+//   arith_asian    ArithmeticGrid::path + ArithmeticParameters::arithmetic
+//                  with the geometric control residual (asian.rs:640-712,
+//                  1100-1256)
+//   accum_path     MonteCarloAccumulatorEngine sample closure
+//                  (engines/monte_carlo/accumulator.rs:180-216)
+#pragma once
+
+#include "autocall_common.hpp"
+
+namespace kg {
+
+// ------------------------------------------------------------------ numerics
+// numerics.rs:192-215, 320-334; asian.rs:971-984; numerics.rs:160-167, 217-224
+__device__ inline double d_normal_cdf(double x) { return 0.5 * erfc(-x * 0.7071067811865476); }
+__device__ inline double d_log_normal_cdf(double value) {
+  if (value >= -10.0) return log(d_normal_cdf(value));
+  const double inv_sq = (1.0 / value) / value;
+  double term = 1.0, sum = 1.0;
+  for (int i = 1; i <= 20; ++i) {
+    term *= -(double)(2 * i - 1) * inv_sq;
+    sum += term;
+  }
+  return -0.5 * value * value - log(-value) + log(sum / 2.5066282746310002);
+}
+__device__ inline double d_log_add(double a, double b) {
+  const double high = fmax(a, b);
+  if (high == -INFINITY) return high;
+  return high + log1p(exp(-fabs(a - b)));
+}
+__device__ inline double d_log_difference(double high, double low) {
+  if (high == -INFINITY) return -INFINITY;
+  return high + log(-expm1(low - high));
+}
+__device__ inline double d_positive_difference(double a, double b) {
+  if (a == INFINITY) return a;
+  if (b == -INFINITY) return exp(a);
+  const double d = -expm1(b - a);
+  return d > 0.0 ? exp(a + log(d)) : 0.0;
+}
+__device__ inline double d_weighted_cdf(double amount, double exponent, double argument) {
+  const double discount = exp(exponent);
+  const double discounted = amount * discount;
+  if (argument < -10.0 || !isnormal(discount) || !isfinite(discounted))
+    return exp(log(amount) + exponent + d_log_normal_cdf(argument));
+  return discounted * d_normal_cdf(argument);
+}
+// lognormal_price (engines/analytic/european.rs:317-400) for widths >= 1e-5;
+// narrower widths are rejected on the host.
+__device__ inline double d_lognormal_price(double spot, double strike, double log_moneyness,
+                                           double asset_log, double risk_log, double vol,
+                                           double root_time, double sign) {
+  const double width = vol * root_time;
+  const double st = log_moneyness / width;
+  const double d1 = st + 0.5 * width, d2 = st - 0.5 * width;
+  const double asset = d_weighted_cdf(spot, asset_log, sign * d1);
+  const double cash = d_weighted_cdf(strike, risk_log, sign * d2);
+  const double direct = sign * (asset - cash);
+  if (isfinite(direct)) return direct;
+  const double a = log_moneyness + d_log_normal_cdf(sign * d1);
+  const double c = d_log_normal_cdf(sign * d2);
+  return exp(log(strike) + risk_log + (sign > 0 ? d_log_difference(a, c) : d_log_difference(c, a)));
+}
+
+// ------------------------------------------------------------------ barrier
+struct BarrierStep {
+  double mean, width, discount, lower, remaining;
+  int32_t observe;
+  int32_t pad;
+};
+struct BarrierDev {
+  int32_t n_steps, up, knock_in, continuous, at_hit, condition_on_hit, bounded, pad;
+  double sign, strike, rebate, spot, q, sigma, hvr, moneyness, barrier_abs, terminal_barrier;
+  double log_discount;
+  const BarrierStep* steps;
+  const float* steps32;  // mean, width, lower per step
+};
+
+template <bool FP32, class D>
+__device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign, uint32_t& bad) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  R noise = 0, terminal = 0;
+  bool touched = false, has_cond = false, active = live;
+  double knocked_out = 0.0, conditioned = 0.0;
+  for (int i = 0; i < b.n_steps; ++i) {
+    if (__ballot(active) == 0) break;
+    if (!active) continue;
+    const BarrierStep& s = b.steps[i];
+    const R width = FP32 ? (R)b.steps32[3 * i + 1] : (R)s.width;
+    const R mean = FP32 ? (R)b.steps32[3 * i] : (R)s.mean;
+    const R z = (R)sign * dr.template normal<R>(i);
+    noise = FP32 ? (R)fmaf((float)width, (float)z, (float)noise) : (R)fma((double)width, (double)z, (double)noise);
+    terminal = mean + noise;
+    if (!b.bounded) {
+      const double sp = discounted_product(b.spot, 1.0, (double)terminal);
+      if (!isfinite(sp) || sp <= 0.0) bad = 1;
+    }
+    if (s.observe) {
+      const R lower = FP32 ? (R)b.steps32[3 * i + 2] : (R)s.lower;
+      if (b.up ? noise >= -lower : noise <= -lower) {
+        touched = true;
+        if (b.condition_on_hit && b.n_steps - i > 8) {
+          const double asset = discounted_product(b.spot, 1.0, (double)terminal);
+          conditioned = d_lognormal_price(asset, b.strike,
+                                          b.moneyness + (double)noise + b.hvr * s.remaining,
+                                          s.discount - b.q * s.remaining, b.log_discount, b.sigma,
+                                          sqrt(s.remaining), b.sign);
+          has_cond = true;
+          active = false;
+          continue;
+        }
+      }
+    }
+    if (touched && !b.knock_in) {
+      knocked_out = discounted_product(b.rebate, 1.0, b.at_hit ? s.discount : b.log_discount);
+      active = false;
+    }
+  }
+  if (!live) return 0.0;
+  const double asset = discounted_product(b.spot, 1.0, (double)terminal);
+  const double distance = b.moneyness + (double)noise;
+  const double payoff = fabs(distance) < 0.5 ? b.strike * expm1(distance) : asset - b.strike;
+  const double intrinsic = fmax(b.sign * payoff, 0.0);
+  auto weighted = [&](double amount, double lw) {
+    return discounted_product(fabs(amount), amount < 0.0 ? -1.0 : 1.0, b.log_discount + lw);
+  };
+  double value;
+  if (has_cond) {
+    value = conditioned;
+  } else if (b.continuous) {
+    const double w0 = b.steps[0].width;
+    const double dtb = b.terminal_barrier + (double)noise;
+    const bool alive = b.up ? dtb < 0.0 : dtb > 0.0;
+    const double slope = alive ? 2.0 * (fabs(dtb) / w0) / w0 : 0.0;
+    const double exponent = -b.barrier_abs * slope;
+    const double survival = alive ? log(-expm1(exponent)) : -INFINITY;
+    const double rebate = (!b.knock_in && b.at_hit) ? 0.0 : b.rebate;
+    value = weighted(intrinsic, b.knock_in ? exponent : survival) +
+            weighted(rebate, b.knock_in ? survival : exponent);
+  } else if (touched && !b.knock_in) {
+    value = knocked_out;
+  } else {
+    value = weighted(touched == (bool)b.knock_in ? intrinsic : b.rebate, 0.0);
+  }
+  if (!isfinite(value)) bad = 1;
+  return value;
+}
+
+// ------------------------------------------------------------------ Asians
+struct GeoAsianDev {
+  double im, drift, width, log_width, log_strike, sign, log_discount;
+};
+
+template <bool FP32, class D>
+__device__ double geo_asian(const GeoAsianDev& g, D& dr, bool live, float sign, uint32_t& bad) {
+  if (!live) return 0.0;
+  const double z = (double)sign * (double)(FP32 ? (double)dr.template normal<float>(0)
+                                                 : dr.template normal<double>(0));
+  const double distance = g.im + g.drift + g.width * z;
+  const double center = (g.im + g.drift) / g.width;
+  const double standardized = center + z;
+  if (g.sign * standardized <= 0.0) return 0.0;
+  double lp;
+  if (fabs(distance) < 1e-5 && isfinite(standardized))
+    lp = g.log_strike + g.log_width + log(fabs(standardized)) +
+         (distance == 0.0 ? 0.0 : log(fabs(expm1(distance)) / fabs(distance)));
+  else if (g.sign > 0)
+    lp = g.log_strike + distance + log(-expm1(-distance));
+  else
+    lp = g.log_strike + log(-expm1(distance));
+  const double v = exp(lp + g.log_discount);
+  if (!isfinite(v)) bad = 1;
+  return v;
+}
+
+struct ArithAsianDev {
+  int32_t n, control;
+  double spot, strike, weight, log_discount, drift, width, sign, interval_forward;
+  double c_log_strike, c_las, c_fw;
+};
+
+__device__ inline double geometric_values_d(const ArithAsianDev& p, double lf, double ls,
+                                            double width) {
+  const double d1 = (lf - ls) / width + 0.5 * width;
+  const double d2 = d1 - width;
+  const double la = lf + p.log_discount + d_log_normal_cdf(p.sign * d1);
+  const double lc = ls + p.log_discount + d_log_normal_cdf(p.sign * d2);
+  return p.sign > 0 ? d_positive_difference(la, lc) : d_positive_difference(lc, la);
+}
+
+template <bool FP32, class D>
+__device__ double arith_asian(const ArithAsianDev& p, D& dr, bool live, float sign, uint32_t& bad) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  const int dims = p.n - 1;
+  R growth = 0;
+  double rel = 0.0, log_sum = -INFINITY, gg = 0.0;
+  const R drift = (R)p.drift, width = (R)p.width;
+  const double inv_n = 1.0 / (double)p.n;
+  for (int i = 0; i < dims; ++i) {
+    if (__ballot(live) == 0) break;
+    if (!live) continue;
+    const R z = (R)sign * dr.template normal<R>(i);
+    growth += drift + width * z;
+    const double fraction = (i + 2 == p.n ? 0.5 : 1.0) * inv_n;
+    if (p.control) gg += fraction * (double)growth;
+    if ((double)growth > 600.0 || log_sum != -INFINITY) {
+      if (log_sum == -INFINITY) log_sum = log(1.0 - 0.5 * inv_n + rel);
+      log_sum = d_log_add(log_sum, log(fraction) + (double)growth);
+    } else {
+      rel += fraction * (FP32 ? (double)f32::expm1((float)growth) : expm1((double)growth));
+    }
+  }
+  if (!live) return 0.0;
+  const double half = 0.5 * inv_n, rest = 1.0 - half;
+  const double log_spot = log(p.spot), log_weight = log(p.weight);
+  const double log_rest = log_sum == -INFINITY ? log(rest) + log1p(rel / rest) : log_sum;
+  const double log_q = log_spot + log_weight + log_rest + p.interval_forward;
+  const double log_c = log_spot + log_weight + log(half);
+  const bool has_extra = log_sum == -INFINITY;
+  const double extra = has_extra ? rest * expm1(p.interval_forward) + rel * exp(p.interval_forward) : 0.0;
+  double residual = 0.0;
+  if (p.control) {
+    const double w = p.c_fw * p.width;
+    const double lf = p.c_las + gg + p.c_fw * p.drift + 0.5 * w * w;
+    residual = geometric_values_d(p, lf, p.c_log_strike, w);
+  }
+  const double log_strike = log(p.strike);
+  double price;
+  if (log_c >= log_strike) {
+    price = p.sign < 0 ? 0.0
+                       : exp(log_q + p.log_discount) +
+                             d_positive_difference(log_c + p.log_discount, log_strike + p.log_discount);
+  } else {
+    const double log_b = log_strike + log(-expm1(log_c - log_strike));
+    double forward = log_q - log_b;
+    const double baseline = p.weight * ((p.spot - p.strike) / p.spot);
+    if (has_extra) {
+      const double excess = baseline + p.weight * extra;
+      const double bb = exp(log_b - log_spot);
+      if (bb > 0.0 && fabs(excess / bb) < 0.5) forward = log1p(excess / bb);
+    }
+    const double center = forward / p.width;
+    const double d1 = center + 0.5 * p.width, d2 = center - 0.5 * p.width;
+    const double la = log_q + p.log_discount + d_log_normal_cdf(p.sign * d1);
+    const double lc = log_b + p.log_discount + d_log_normal_cdf(p.sign * d2);
+    price = p.sign > 0 ? d_positive_difference(la, lc) : d_positive_difference(lc, la);
+  }
+  const double v = price - residual;
+  if (!isfinite(v)) bad = 1;
+  return v;
+}
+
+// ------------------------------------------------------------------ accumulator
+struct AccumDev {
+  int32_t n_steps, pad;
+  double sign, strike_log, ko_log, strike, spot, daily_qty, accel, acc_qty, safe_lo, safe_hi;
+  double terminal_discount;
+  const double* steps;  // drift, diffusion, discount per step
+  const float* steps32; // drift, diffusion per step
+};
+
+template <bool FP32, class D>
+__device__ double accum_path(const AccumDev& a, D& dr, bool live, float sign, uint32_t& bad) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  R lr = 0;
+  double quantity = a.acc_qty;
+  bool active = live;
+  double value = 0.0;
+  const R ko = (R)a.ko_log, sk = (R)a.strike_log, sg = (R)a.sign;
+  auto unit = [&](double l) {
+    const double spot = discounted_product(a.spot, 1.0, l);
+    if (!isfinite(spot) || spot <= 0.0) bad = 1;
+    const double distance = l - a.strike_log;
+    if (fabs(distance) < 0.5) return a.sign * a.strike * expm1(distance);
+    return a.sign > 0 ? spot - a.strike : a.strike - spot;
+  };
+  for (int i = 0; i < a.n_steps; ++i) {
+    if (__ballot(active) == 0) break;
+    if (!active) continue;
+    const R z = (R)sign * dr.template normal<R>(i);
+    lr += FP32 ? (R)fmaf(a.steps32[2 * i + 1], (float)z, a.steps32[2 * i])
+               : (R)fma(a.steps[3 * i + 1], (double)z, a.steps[3 * i]);
+    if (!((double)lr >= a.safe_lo && (double)lr <= a.safe_hi)) {
+      const double sp = discounted_product(a.spot, 1.0, (double)lr);
+      if (!isfinite(sp) || sp <= 0.0) bad = 1;
+    }
+    if (sg * lr >= sg * ko) {
+      value = discounted_product(quantity, unit((double)lr), a.steps[3 * i + 2]);
+      active = false;
+      continue;
+    }
+    quantity += a.daily_qty * (sg * lr < sg * sk ? a.accel : 1.0);
+  }
+  if (active) value = discounted_product(quantity, unit((double)lr), a.terminal_discount);
+  if (live && !isfinite(value)) bad = 1;
+  return live ? value : 0.0;
+}
+
+}  // namespace kg

@@ -48,6 +48,10 @@ def lib():
             P(_abi.Settings), P(_abi.AutocallTerms), P(_abi.BsmMarket), C.c_int, P(_abi.Estimate)]
         handle.oracle_heston_price.argtypes = [
             P(_abi.Settings), P(_abi.HestonTerms), P(_abi.HestonMarket), C.c_int, P(_abi.Estimate)]
+        for name, terms in (("barrier", _abi.BarrierTerms), ("asian", _abi.AsianTerms),
+                            ("accumulator", _abi.AccumulatorTerms)):
+            getattr(handle, f"oracle_{name}_price").argtypes = [
+                P(_abi.Settings), P(terms), P(_abi.BsmMarket), C.c_int, P(_abi.Estimate)]
         handle.oracle_autocallable_plan.argtypes = [
             P(_abi.AutocallTerms), P(_abi.BsmMarket), P(C.c_double), C.c_size_t,
             P(C.c_size_t), P(C.c_int)]
@@ -134,3 +138,22 @@ def autocallable_plan(terms, market, cap: int = 100000):
     _check(lib().oracle_autocallable_plan(C.byref(terms), C.byref(market), _ptr(steps, C.c_double), cap,
                                           C.byref(n), C.byref(bridge)))
     return steps[: n.value].copy(), bool(bridge.value)
+
+
+def barrier_price(settings, terms, market, workers: int = 1):
+    out = _abi.Estimate()
+    _check(lib().oracle_barrier_price(C.byref(settings), C.byref(terms), C.byref(market), workers, C.byref(out)))
+    return out
+
+
+def asian_price(settings, terms, market, workers: int = 1):
+    out = _abi.Estimate()
+    _check(lib().oracle_asian_price(C.byref(settings), C.byref(terms), C.byref(market), workers, C.byref(out)))
+    return out
+
+
+def accumulator_price(settings, terms, market, workers: int = 1):
+    out = _abi.Estimate()
+    _check(lib().oracle_accumulator_price(C.byref(settings), C.byref(terms), C.byref(market), workers,
+                                          C.byref(out)))
+    return out
