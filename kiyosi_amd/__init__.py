@@ -6,7 +6,11 @@ kernels through the C ABI in include/kiyosi_amd.h.
 """
 
 from .engines import (
+    Book,
+    MonteCarloAccumulatorEngine,
+    MonteCarloAsianEngine,
     MonteCarloAutocallableEngine,
+    MonteCarloBarrierEngine,
     MonteCarloEstimate,
     MonteCarloVanillaEngine,
 )
@@ -19,7 +23,11 @@ from .errors import (
     UnsupportedOperationError,
 )
 from .instruments import (
+    Accumulator,
+    ArithmeticAveragePriceOption,
+    BarrierOption,
     BinarySnowballOption,
+    GeometricAveragePriceOption,
     CouponObservation,
     EuropeanOption,
     KnockInObservation,

@@ -123,7 +123,7 @@ extern "C" kg_status kg_autocallable_price(kg_context* ctx, const kg_settings* s
     AcHostPlan h = build_plan(*note, *market);
     *out = kg_estimate{};
     if (h.fixed) {
-      fill_known(h.fixed_value, out);
+      fill_known(h.fixed_value, settings && settings->sampling == KG_SOBOL, out);
       return;
     }
     Sampler s = validate_settings(settings, (uint64_t)h.dev.n_steps);

@@ -1,22 +1,845 @@
-// Barrier, Asian, accumulator engines and the persistent book kernel
-// (placeholders until the book kernel lands).
-#include "host_common.hpp"
+// Barrier, Asian and accumulator engines and the heterogeneous book.
+//
+// Every contract becomes a BookItem: a plan built on the host (restating the
+// reference engine's own plan building, cited per builder below) plus its
+// sampler settings. One persistent kernel pulls (contract, 256-sample block)
+// tiles from an atomic queue, longest contracts first, and writes one
+// partial per tile; a second kernel merges each contract's partials in block
+// order. Tile identity fixes the stream of every sample (PCG stream = block
+// index of the contract, Sobol point = sample index), so results do not
+// depend on scheduling. The single-contract entry points run a book of one.
+//
+// Within a tile the 256 samples are spread over 256/spt threads (spt =
+// samples per thread); per-sample values go through LDS and are folded by
+// the first wave in sample order, so the tile shape chosen by the autotuner
+// does not change the arithmetic. The reset/book/merge sequence is captured
+// once into a hipGraph and replayed.
+#include <algorithm>
+#include <numeric>
+
+#include "book_device.hpp"
+
+namespace kg {
+
+enum : int32_t { IK_BARRIER = 0, IK_GEO_ASIAN = 1, IK_ARITH_ASIAN = 2, IK_AUTOCALL = 3, IK_ACCUM = 4 };
+
+struct BookItem {
+  int32_t kind, dims;
+  int32_t antithetic, sobol;
+  int64_t samples, paths;
+  int64_t first_block;  // offset of this contract's partials
+  uint64_t seed;
+  const uint64_t* dirs;    // Sobol directions [column * dims + dim]
+  const uint64_t* shifts;  // Sobol shifts [dim]
+  union {
+    BarrierDev b;
+    GeoAsianDev g;
+    ArithAsianDev a;
+    AcPlanDev ac;
+    AccumDev acc;
+  } u;
+};
+
+// ------------------------------------------------------------------ device
+template <bool FP32, bool SOBOL>
+__device__ void book_sample(const BookItem& it, int64_t block, int j, double* pos, double* neg,
+                            bool* live_out, bool* paired_out, uint32_t& bad) {
+  const int64_t n = block * BLOCK_SAMPLES + j;
+  const bool live = n < it.samples;
+  const bool paired = it.antithetic && live && n < it.paths / 2;
+  Draws<SOBOL, FP32> dr;
+  dr.dims = it.dims;
+  dr.direct = true;
+  dr.dirs = it.dirs;
+  dr.shifts = it.shifts;
+  dr.point = (uint64_t)n;
+  if (!SOBOL) {
+    const uint64_t first = (uint64_t)j * (uint64_t)it.dims;
+    if (FP32) {
+      dr.g32.init(it.seed, (uint64_t)block);
+      dr.g32.advance((first >> 1) * 4);
+      dr.has32 = false;
+      if (first & 1) {
+        float e;
+        f32::box_muller_pair(dr.g32, &e, &dr.spare32);
+        dr.has32 = true;
+      }
+    } else {
+      dr.s.start(it.seed, (uint64_t)block, first);
+    }
+  }
+  const Draws<SOBOL, FP32> origin = dr;
+  auto run = [&](bool on, float sign) -> double {
+    switch (it.kind) {
+      case IK_BARRIER:
+        return barrier_path<FP32>(it.u.b, dr, on, sign, bad);
+      case IK_GEO_ASIAN:
+        return geo_asian<FP32>(it.u.g, dr, on, sign, bad);
+      case IK_ARITH_ASIAN:
+        return arith_asian<FP32>(it.u.a, dr, on, sign, bad);
+      case IK_AUTOCALL:
+        return it.u.ac.bridge ? bridge_path<SOBOL, FP32>(it.u.ac, dr, on, sign, bad)
+                              : step_path<SOBOL, FP32>(it.u.ac, dr, on, sign, bad);
+      default:
+        return accum_path<FP32>(it.u.acc, dr, on, sign, bad);
+    }
+  };
+  *pos = run(live, 1.0f);
+  *neg = 0.0;
+  if (it.antithetic && __ballot(paired) != 0) {
+    dr = origin;
+    *neg = run(paired, -1.0f);
+  }
+  *live_out = live;
+  *paired_out = paired;
+}
+
+template <bool FP32, int SPT>
+__global__ void __launch_bounds__(256)
+    book_kernel(const BookItem* __restrict__ items, const int2* __restrict__ tiles, int n_tiles,
+                int* __restrict__ counter, Moments* __restrict__ partials,
+                uint32_t* __restrict__ flags) {
+  __shared__ int s_tile;
+  __shared__ double s_pos[BLOCK_SAMPLES], s_neg[BLOCK_SAMPLES];
+  __shared__ unsigned char s_state[BLOCK_SAMPLES];
+  uint32_t bad = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+    __syncthreads();
+    const int t = s_tile;
+    if (t >= n_tiles) break;  // every wave reads the same index: a uniform exit
+    const int2 tb = tiles[t];
+    const BookItem& it = items[tb.x];
+    for (int s = 0; s < SPT; ++s) {
+      const int j = threadIdx.x + s * (BLOCK_SAMPLES / SPT);
+      double pos, neg;
+      bool live, paired;
+      if (it.sobol)
+        book_sample<FP32, true>(it, tb.y, j, &pos, &neg, &live, &paired, bad);
+      else
+        book_sample<FP32, false>(it, tb.y, j, &pos, &neg, &live, &paired, bad);
+      s_pos[j] = pos;
+      s_neg[j] = neg;
+      s_state[j] = (unsigned char)(live | (paired << 1));
+    }
+    __syncthreads();
+    if (threadIdx.x < WAVE) {
+      // Lane l folds samples 4l .. 4l+3 in order, then the wave merges.
+      Moments m = moments_zero(), p = moments_zero();
+      for (int k = 0; k < 4; ++k) {
+        const int j = threadIdx.x * 4 + k;
+        const unsigned char st = s_state[j];
+        if (st & 1) moments_add(m, s_pos[j], true);
+        if (st & 2) {
+          moments_add(m, s_neg[j], true);
+          moments_add(p, midpoint(s_pos[j], s_neg[j]), true);
+        }
+      }
+      m = wave_merge(m);
+      p = wave_merge(p);
+      if (threadIdx.x == 0) {
+        partials[(it.first_block + tb.y) * 2 + 0] = m;
+        partials[(it.first_block + tb.y) * 2 + 1] = p;
+      }
+    }
+    __syncthreads();
+  }
+  if (bad) raise_flag(flags, FLAG_NONFINITE);
+}
+
+// One workgroup per contract: its block partials in order (see merge_partials).
+__global__ void __launch_bounds__(256)
+    merge_book(const BookItem* __restrict__ items, const Moments* __restrict__ partials,
+               Moments* __restrict__ out) {
+  __shared__ Moments smem[4];
+  const BookItem& it = items[blockIdx.x];
+  const int64_t blocks = (it.samples + BLOCK_SAMPLES - 1) / BLOCK_SAMPLES;
+  const int64_t per = (blocks + blockDim.x - 1) / blockDim.x;
+  const int64_t begin = per * threadIdx.x;
+  const int64_t end = begin + per < blocks ? begin + per : blocks;
+  for (int s = 0; s < 2; ++s) {
+    Moments acc[1] = {moments_zero()};
+    for (int64_t b = begin; b < end; ++b)
+      acc[0] = moments_merge(acc[0], partials[(it.first_block + b) * 2 + s]);
+    block_merge<1>(acc, smem);
+    if (threadIdx.x == 0) out[blockIdx.x * 2 + s] = acc[0];
+  }
+}
+
+__global__ void book_reset(int* counter, uint32_t* flags) {
+  *counter = 0;
+  *flags = 0u;
+}
+
+// ------------------------------------------------------------------ host
+// Result of planning one contract: either a known value or a sampled item.
+enum class Estimator { PathMeans, Estimate };
+struct Planned {
+  bool known = false;
+  double value = 0.0;
+  Estimator kind = Estimator::Estimate;
+  double control_expected = 0.0;  // arithmetic Asian control mean
+  BookItem item{};
+  Sampler sampler{};
+  // tables staged into the shared blob: (offset, pointer slot) fixups
+  std::vector<std::pair<size_t, const void**>> fixups;
+};
+
+struct Blob {
+  std::vector<unsigned char> bytes;
+  template <class T>
+  size_t add(const std::vector<T>& v) {
+    const size_t off = (bytes.size() + 255) & ~size_t(255);
+    bytes.resize(off + std::max<size_t>(v.size() * sizeof(T), 1));
+    if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+  }
+};
+
+static inline int64_t floor_div64(int64_t a, int64_t b) {
+  int64_t q = a / b;
+  return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+}
+
+static void check_market(const kg_bsm_market& m) {
+  if (!(m.spot > 0.0) || !std::isfinite(m.spot)) fail(KG_INVALID_INPUT, "spot must be positive");
+  if (!(m.volatility >= 0.0) || !std::isfinite(m.volatility))
+    fail(KG_INVALID_INPUT, "volatility must be nonnegative");
+  if (!std::isfinite(m.rate) || !std::isfinite(m.dividend)) fail(KG_INVALID_INPUT, "rates must be finite");
+  if (m.volatility == 0.0)
+    fail(KG_UNSUPPORTED_OPERATION,
+         "zero volatility selects the deterministic engine, which is not part of the GPU engine");
+}
+
+// Plain BSM price through lognormal_price's main branch (barrier resolution).
+static double vanilla_bsm(double S, double K, double r, double q, double sigma, double T, double sign) {
+  if (T == 0.0) return std::max(sign * (S - K), 0.0);
+  const double w = sigma * std::sqrt(T);
+  const double lm = log_ratio(S, K) + (r - q) * T;
+  const double d1 = lm / w + 0.5 * w, d2 = d1 - w;
+  auto cdf = [](double x) { return 0.5 * std::erfc(-x * 0.7071067811865476); };
+  return sign * (S * std::exp(-q * T) * cdf(sign * d1) - K * std::exp(-r * T) * cdf(sign * d2));
+}
+
+// MonteCarloBarrierEngine::native plan (engines/monte_carlo/barrier.rs:160-290)
+// and resolved_price_with_state (engines/analytic/barrier.rs:32-80).
+static Planned plan_barrier(const kg_barrier_terms& o, const kg_bsm_market& m, Blob& blob) {
+  Planned pl;
+  pl.kind = Estimator::PathMeans;
+  if (o.option_type != 1 && o.option_type != -1) fail(KG_INVALID_INPUT, "option_type must be 1 or -1");
+  if (o.barrier_type < KG_DOWN_AND_OUT || o.barrier_type > KG_UP_AND_IN)
+    fail(KG_INVALID_INPUT, "unknown barrier type");
+  if (!(o.strike > 0.0) || !(o.barrier > 0.0) || !(o.rebate >= 0.0) || !std::isfinite(o.rebate))
+    fail(KG_INVALID_INPUT, "strike and barrier must be positive and the rebate nonnegative");
+  if (o.observation == KG_OBSERVE_SCHEDULED && (o.n_observations < 1 || !o.observation_times))
+    fail(KG_INVALID_INPUT, "scheduled monitoring needs observation times");
+  for (int i = 1; i < o.n_observations; ++i)
+    if (o.observation_times[i] <= o.observation_times[i - 1])
+      fail(KG_INVALID_INPUT, "observation times must be strictly increasing");
+  const int64_t val = m.valuation_time, expiry = o.expiry_time;
+  if (val < o.effective_time || val > expiry) fail(KG_INVALID_INPUT, "valuation outside contract life");
+  check_market(m);
+  const bool up = o.barrier_type >= KG_UP_AND_OUT;
+  const bool knock_in = o.barrier_type == KG_DOWN_AND_IN || o.barrier_type == KG_UP_AND_IN;
+  const bool continuous = o.observation == KG_OBSERVE_CONTINUOUS;
+  const bool at_hit = o.rebate_timing == KG_REBATE_AT_HIT;
+  const double sign = o.option_type, r = m.rate, q = m.dividend, sigma = m.volatility, S = m.spot;
+  const double T = (double)(expiry - val) / (365.0 * 86400e6);
+  bool observed_now = continuous;
+  for (int i = 0; i < o.n_observations && !continuous; ++i) observed_now |= o.observation_times[i] == val;
+  const bool touched = observed_now && (up ? S >= o.barrier : S <= o.barrier);
+  const bool finished = val == expiry;
+  const bool last_is_expiry =
+      continuous || (o.n_observations > 0 && o.observation_times[o.n_observations - 1] == expiry);
+  const bool requires_touch =
+      last_is_expiry && (up ? (sign > 0 && o.strike >= o.barrier) : (sign < 0 && o.strike <= o.barrier));
+  const double cash = o.rebate == 0.0 ? 0.0 : o.rebate * std::exp(-r * T);
+  const bool fixed = o.rebate == 0.0 && requires_touch;
+  if (touched && !knock_in) {
+    pl.known = true;
+    pl.value = at_hit ? o.rebate : cash;
+    return pl;
+  }
+  if (!touched && finished && knock_in) {
+    pl.known = true;
+    pl.value = cash;
+    return pl;
+  }
+  if (fixed && !knock_in) {
+    pl.known = true;
+    return pl;
+  }
+  if (touched || finished || fixed) {
+    pl.known = true;
+    pl.value = vanilla_bsm(S, o.strike, r, q, sigma, T, sign);
+    return pl;
+  }
+  if (continuous && !knock_in && at_hit && o.rebate != 0.0)
+    fail(KG_UNSUPPORTED_OPERATION, "continuous at-hit rebates need the analytic hit value");
+  const double carry = r - q, hvr = 0.5 * sigma * sigma, drift = carry - hvr;
+  auto years = [](int64_t a, int64_t b) { return (double)(b - a) / (365.0 * 86400e6); };
+  auto mean_distance = [&](int64_t date, double level) {
+    const double e = years(val, date);
+    return log_ratio(S, level) + carry * e - hvr * e;
+  };
+  std::vector<std::pair<int64_t, bool>> dates;
+  if (!continuous)
+    for (int i = 0; i < o.n_observations; ++i)
+      if (o.observation_times[i] > val) dates.push_back({o.observation_times[i], true});
+  if (dates.empty() || dates.back().first != expiry) dates.push_back({expiry, false});
+  std::vector<BarrierStep> steps;
+  std::vector<float> steps32;
+  int64_t prev = val;
+  double width_sum = 0.0;
+  for (auto [date, observe] : dates) {
+    BarrierStep s{};
+    const double dt = years(prev, date);
+    s.width = sigma * std::sqrt(dt);
+    const double elapsed = years(val, date);
+    s.mean = drift * elapsed;
+    if (s.width <= 0.0 || !std::isfinite(s.width) || !std::isfinite(s.mean)) unstable();
+    s.discount = -r * elapsed;
+    s.observe = observe;
+    s.lower = observe ? mean_distance(date, o.barrier) : 0.0;
+    s.remaining = years(date, expiry);
+    steps.push_back(s);
+    steps32.push_back((float)s.mean);
+    steps32.push_back((float)s.width);
+    steps32.push_back((float)s.lower);
+    width_sum += s.width;
+    prev = date;
+  }
+  BarrierDev b{};
+  b.n_steps = (int32_t)steps.size();
+  b.up = up;
+  b.knock_in = knock_in;
+  b.continuous = continuous;
+  b.at_hit = at_hit;
+  b.bounded = std::fabs(std::log(S)) + 10.0 * width_sum + std::fabs(drift * T) < 600.0;
+  b.condition_on_hit = knock_in && !continuous && steps.size() > 8 && b.bounded;
+  if (b.condition_on_hit)
+    for (const BarrierStep& s : steps)
+      if (s.observe && sigma * std::sqrt(s.remaining) < 1e-5 && s.remaining > 0)
+        fail(KG_UNSUPPORTED_OPERATION, "conditional prices below width 1e-5 are not supported");
+  b.sign = sign;
+  b.strike = o.strike;
+  b.rebate = o.rebate;
+  b.spot = S;
+  b.q = q;
+  b.sigma = sigma;
+  b.hvr = hvr;
+  b.moneyness = mean_distance(expiry, o.strike);
+  b.barrier_abs = std::fabs(log_ratio(o.barrier, S));
+  b.terminal_barrier = mean_distance(expiry, o.barrier);
+  b.log_discount = -r * T;
+  pl.item.kind = IK_BARRIER;
+  pl.item.dims = b.n_steps;
+  pl.item.u.b = b;
+  const size_t o1 = blob.add(steps), o2 = blob.add(steps32);
+  pl.fixups.push_back({o1, (const void**)&pl.item.u.b.steps});
+  pl.fixups.push_back({o2, (const void**)&pl.item.u.b.steps32});
+  return pl;
+}
+
+// MonteCarloAsianEngine::prepare (engines/monte_carlo/asian.rs:374-551) and
+// geometric_control (:1016-1061) for flat rates and a fresh average.
+static Planned plan_asian(const kg_asian_terms& o, const kg_bsm_market& m, Blob&) {
+  Planned pl;
+  pl.kind = Estimator::PathMeans;
+  if (o.option_type != 1 && o.option_type != -1) fail(KG_INVALID_INPUT, "option_type must be 1 or -1");
+  if (o.step_count < 2 || o.step_count > 10000)
+    fail(KG_INVALID_INPUT, "step_count is outside the supported Monte Carlo range");
+  if (!(o.strike > 0.0)) fail(KG_INVALID_INPUT, "strike must be positive");
+  const int64_t val = m.valuation_time;
+  if (val < o.effective_time || val > o.expiry_time)
+    fail(KG_INVALID_INPUT, "valuation must be within contract life");
+  if (o.averaging_start_time < o.effective_time || o.averaging_start_time > o.expiry_time)
+    fail(KG_INVALID_INPUT, "averaging must start within the contract life");
+  if (val > o.averaging_start_time && !o.has_realized)
+    fail(KG_MISSING_HISTORY, "prior realized average is required");
+  if (val > o.averaging_start_time)
+    fail(KG_UNSUPPORTED_OPERATION, "seasoned averages are priced by the CPU engine");
+  const double sign = o.option_type;
+  auto years = [](int64_t a, int64_t b) { return (double)(b - a) / (365.0 * 86400e6); };
+  const double tau = years(val, o.expiry_time);
+  if (tau == 0.0) {
+    pl.known = true;
+    pl.value = std::max(sign * (m.spot - o.strike), 0.0);
+    return pl;
+  }
+  check_market(m);
+  const double period = years(o.averaging_start_time, o.expiry_time);
+  if (period == 0.0) fail(KG_UNSUPPORTED_OPERATION, "single-fixing averages use the vanilla engine");
+  const double lead = std::max(years(val, o.averaging_start_time), 0.0);
+  const double future = years(std::max(val, o.averaging_start_time), o.expiry_time);
+  const double weight = future / period;
+  const double sigma = m.volatility, r = m.rate, q = m.dividend;
+  const double carry = r - q, drift_rate = carry - 0.5 * sigma * sigma;
+  const double log_discount = -r * tau;
+  if (o.averaging == KG_AVERAGE_GEOMETRIC) {
+    const double vt = lead + future / 3.0;
+    GeoAsianDev g{};
+    g.im = weight * log_ratio(m.spot, o.strike);
+    g.drift = weight * drift_rate * (lead + future / 2.0);
+    g.width = sigma * (weight * std::sqrt(vt));
+    g.log_width = std::log(sigma) + std::log(weight) + 0.5 * std::log(vt);
+    g.log_strike = std::log(o.strike);
+    g.sign = sign;
+    g.log_discount = log_discount;
+    if (!std::isfinite(g.drift) || !std::isfinite(g.width)) unstable();
+    pl.item.kind = IK_GEO_ASIAN;
+    pl.item.dims = 1;
+    pl.item.u.g = g;
+    return pl;
+  }
+  if (lead > 0.0) fail(KG_UNSUPPORTED_OPERATION, "forward-start arithmetic averages are not supported");
+  ArithAsianDev a{};
+  a.n = o.step_count;
+  a.spot = m.spot;
+  a.strike = o.strike;
+  a.weight = weight;
+  a.log_discount = log_discount;
+  const double dt = future / o.step_count;
+  a.interval_forward = carry * dt;
+  a.drift = drift_rate * dt;
+  a.width = sigma * std::sqrt(dt);
+  a.sign = sign;
+  if (a.width < 1e-5) fail(KG_UNSUPPORTED_OPERATION, "arithmetic widths below 1e-5 are not supported");
+  const double n = o.step_count, fw = 1.0 - 0.5 / n;
+  const double las = std::log(a.spot) + std::log(a.weight), ls = std::log(a.strike);
+  if (!(a.width > 0.8 || std::fabs(a.log_discount) > 50.0 || std::fabs(std::log(a.spot)) > 200.0 ||
+        std::fabs(las) > 200.0 || !std::isfinite(ls))) {
+    double mean = fw * a.drift;
+    for (int s = 1; s < o.step_count; ++s) mean += (fw - (double)s / n) * a.drift;
+    const double variance = (a.width * a.width) * (n / 3.0 - 1.0 / (12.0 * n));
+    const double w = std::sqrt(variance);
+    const double lf = las + mean + 0.5 * variance;
+    if (w >= 0.02 && w <= 0.8 && std::fabs(lf - ls) <= 2.0 * w) {
+      a.control = 1;
+      a.c_log_strike = ls;
+      a.c_las = las;
+      a.c_fw = fw;
+      const double d1 = (lf - ls) / w + 0.5 * w, d2 = d1 - w;
+      auto lcdf = [](double x) {
+        if (x >= -10.0) return std::log(0.5 * std::erfc(-x * 0.7071067811865476));
+        const double inv_sq = (1.0 / x) / x;
+        double term = 1.0, sum = 1.0;
+        for (int i = 1; i <= 20; ++i) {
+          term *= -(double)(2 * i - 1) * inv_sq;
+          sum += term;
+        }
+        return -0.5 * x * x - std::log(-x) + std::log(sum / std::sqrt(2.0 * M_PI));
+      };
+      auto pdiff = [](double x, double y) {
+        if (x == INFINITY) return x;
+        if (y == -INFINITY) return std::exp(x);
+        const double d = -std::expm1(y - x);
+        return d > 0.0 ? std::exp(x + std::log(d)) : 0.0;
+      };
+      const double la = lf + log_discount + lcdf(sign * d1);
+      const double lc = ls + log_discount + lcdf(sign * d2);
+      pl.control_expected = sign > 0 ? pdiff(la, lc) : pdiff(lc, la);
+    }
+  }
+  pl.item.kind = IK_ARITH_ASIAN;
+  pl.item.dims = o.step_count - 1;
+  pl.item.u.a = a;
+  return pl;
+}
+
+// MonteCarloAccumulatorEngine::sample plan (engines/monte_carlo/accumulator.rs:91-180),
+// daily accrual without fixed-purchase grouping.
+static Planned plan_accumulator(const kg_accumulator_terms& o, const kg_bsm_market& m, Blob& blob) {
+  Planned pl;
+  if (o.option_type != 1 && o.option_type != -1) fail(KG_INVALID_INPUT, "option_type must be 1 or -1");
+  if (!(o.strike > 0.0) || !(o.knock_out_level > 0.0) || !(o.daily_quantity >= 0.0) ||
+      !(o.acceleration_factor >= 0.0))
+    fail(KG_INVALID_INPUT, "accumulator terms must be positive");
+  const int64_t val = m.valuation_time;
+  if (val < o.effective_time || val >= o.expiry_time)
+    fail(KG_UNSUPPORTED_OPERATION, "valuation must lie inside the accumulation period");
+  check_market(m);
+  const double S = m.spot;
+  std::vector<double> steps;
+  std::vector<float> steps32;
+  int64_t prev = val;
+  const int64_t day_ms = 86400LL * 1000000LL;
+  for (int64_t d = floor_div64(val, day_ms); d <= floor_div64(o.expiry_time, day_ms); ++d) {
+    const int64_t at = d * day_ms + o.observation_clock_micros;
+    const bool trading = o.calendar == KG_CALENDAR_ALL_DAYS || ((d % 7) + 7 + 3) % 7 < 5;
+    if (at <= val || at > o.expiry_time || !trading) continue;
+    double drift, diff, disc;
+    transition(m.rate, m.dividend, m.volatility, years(prev, at), years(val, at), &drift, &diff, &disc);
+    steps.push_back(drift);
+    steps.push_back(diff);
+    steps.push_back(disc);
+    steps32.push_back((float)drift);
+    steps32.push_back((float)diff);
+    prev = at;
+  }
+  if (steps.empty()) fail(KG_UNSUPPORTED_OPERATION, "no fixing remains");
+  AccumDev a{};
+  a.n_steps = (int32_t)(steps.size() / 3);
+  a.sign = o.option_type;
+  a.strike_log = log_ratio(o.strike, S);
+  a.ko_log = log_ratio(o.knock_out_level, S);
+  a.strike = o.strike;
+  a.spot = S;
+  a.daily_qty = o.daily_quantity;
+  a.accel = o.acceleration_factor;
+  a.acc_qty = o.accumulated_quantity;
+  const double lo = std::log(std::numeric_limits<double>::min()) + 1.0;
+  const double hi = std::log(std::numeric_limits<double>::max()) - 1.0;
+  a.safe_lo = std::max(lo, lo - std::log(S));
+  a.safe_hi = std::min(hi, hi - std::log(S));
+  a.terminal_discount = steps[steps.size() - 1];
+  pl.item.kind = IK_ACCUM;
+  pl.item.dims = a.n_steps;
+  pl.item.u.acc = a;
+  const size_t o1 = blob.add(steps), o2 = blob.add(steps32);
+  pl.fixups.push_back({o1, (const void**)&pl.item.u.acc.steps});
+  pl.fixups.push_back({o2, (const void**)&pl.item.u.acc.steps32});
+  return pl;
+}
+
+static Planned plan_autocall(const kg_autocall_terms& t, const kg_bsm_market& m, Blob& blob) {
+  Planned pl;
+  AcHostPlan h = build_plan(t, m);
+  if (h.fixed) {
+    pl.known = true;
+    pl.value = h.fixed_value;
+    return pl;
+  }
+  pl.item.kind = IK_AUTOCALL;
+  pl.item.dims = h.dev.n_steps;
+  pl.item.u.ac = h.dev;
+  AcPlanDev& p = pl.item.u.ac;
+  pl.fixups.push_back({blob.add(h.steps), (const void**)&p.steps});
+  pl.fixups.push_back({blob.add(h.step_event), (const void**)&p.step_event});
+  pl.fixups.push_back({blob.add(h.step_observe), (const void**)&p.step_observe});
+  pl.fixups.push_back({blob.add(h.events), (const void**)&p.events});
+  pl.fixups.push_back({blob.add(h.intervals), (const void**)&p.intervals});
+  pl.fixups.push_back({blob.add(h.weight), (const void**)&p.weight});
+  pl.fixups.push_back({blob.add(h.fill), (const void**)&p.fill});
+  pl.fixups.push_back({blob.add(h.steps32), (const void**)&p.steps32});
+  pl.fixups.push_back({blob.add(h.weight32), (const void**)&p.weight32});
+  pl.fixups.push_back({blob.add(h.fill32), (const void**)&p.fill32});
+  pl.fixups.push_back({blob.add(h.events32), (const void**)&p.events32});
+  pl.fixups.push_back({blob.add(h.intervals32), (const void**)&p.intervals32});
+  return pl;
+}
+
+static int cu_count(kg_context* ctx) {
+  int cus = 0;
+  KG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  return std::max(cus, 1);
+}
+
+template <bool FP32>
+static void launch_book(kg_context* ctx, int spt, int wpc, const BookItem* items, const int2* tiles,
+                        int n_tiles, int* counter, Moments* parts, uint32_t* flags) {
+  const int threads = BLOCK_SAMPLES / spt;
+  const int grid = std::max(1, std::min(n_tiles, cu_count(ctx) * wpc));
+  switch (spt) {
+    case 1:
+      hipLaunchKernelGGL((book_kernel<FP32, 1>), dim3(grid), dim3(threads), 0, ctx->stream, items, tiles,
+                         n_tiles, counter, parts, flags);
+      break;
+    case 2:
+      hipLaunchKernelGGL((book_kernel<FP32, 2>), dim3(grid), dim3(threads), 0, ctx->stream, items, tiles,
+                         n_tiles, counter, parts, flags);
+      break;
+    default:
+      hipLaunchKernelGGL((book_kernel<FP32, 4>), dim3(grid), dim3(threads), 0, ctx->stream, items, tiles,
+                         n_tiles, counter, parts, flags);
+      break;
+  }
+  KG_HIP(hipGetLastError());
+}
+
+static void run_book_once(kg_context* ctx, bool fp32, int spt, int wpc, const BookItem* items,
+                          const int2* tiles, int n_tiles, int* counter, Moments* parts,
+                          uint32_t* flags) {
+  hipLaunchKernelGGL(book_reset, dim3(1), dim3(1), 0, ctx->stream, counter, flags);
+  KG_HIP(hipGetLastError());
+  if (fp32)
+    launch_book<true>(ctx, spt, wpc, items, tiles, n_tiles, counter, parts, flags);
+  else
+    launch_book<false>(ctx, spt, wpc, items, tiles, n_tiles, counter, parts, flags);
+}
+
+// Times each tile shape on a strided eighth of the tiles and keeps the
+// fastest (samples per thread x workgroups per CU).
+static void autotune(kg_context* ctx, bool fp32, const BookItem* items,
+                     const std::vector<int2>& tiles, int* counter, Moments* parts, uint32_t* flags) {
+  const int slot = fp32 ? 1 : 0;
+  if (ctx->tuned_spt[slot]) return;
+  std::vector<int2> sample;
+  const int stride = tiles.size() >= 8192 ? 8 : 1;
+  for (size_t i = 0; i < tiles.size(); i += stride) sample.push_back(tiles[i]);
+  int2* d_sample = (int2*)ctx->scratch.reserve(sample.size() * sizeof(int2));
+  KG_HIP(hipMemcpyAsync(d_sample, sample.data(), sample.size() * sizeof(int2), hipMemcpyHostToDevice,
+                        ctx->stream));
+  double best = 1e300;
+  int best_spt = 1, best_wpc = 4;
+  for (int spt : {1, 2, 4})
+    for (int wpc : {2, 4, 8}) {
+      run_book_once(ctx, fp32, spt, wpc, items, d_sample, (int)sample.size(), counter, parts, flags);
+      timing_begin(ctx);
+      run_book_once(ctx, fp32, spt, wpc, items, d_sample, (int)sample.size(), counter, parts, flags);
+      const double ms = timing_end(ctx);
+      if (ms < best) {
+        best = ms;
+        best_spt = spt;
+        best_wpc = wpc;
+      }
+    }
+  check_flags(ctx);
+  ctx->tuned_spt[slot] = best_spt;
+  ctx->tuned_wpc[slot] = best_wpc;
+}
+
+struct BookRun {
+  std::vector<Planned> plans;
+  std::vector<int> sampled;  // indices into plans with sampling work
+};
+
+static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_t n,
+                       int32_t precision, kg_book_tuning* tuning, kg_estimate* out) {
+  if (n == 0) return;
+  if (!contracts || !out) fail(KG_INVALID_INPUT, "arguments must not be null");
+  if (precision != KG_FP64 && precision != KG_FP32_STEP) fail(KG_INVALID_INPUT, "unknown precision");
+  if (n > (size_t(1) << 20)) fail(KG_INVALID_INPUT, "a book holds at most 1,048,576 contracts");
+  const bool fp32 = precision == KG_FP32_STEP;
+  Blob blob;
+  std::vector<Planned> plans(n);
+  std::vector<std::vector<uint64_t>> sob_dirs(n), sob_shifts(n);
+  std::vector<size_t> sob_off(n, 0), sob_shift_off(n, 0);
+  double path_steps = 0.0;
+  int64_t total_blocks = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const kg_book_contract& c = contracts[i];
+    if (!c.terms) fail(KG_INVALID_INPUT, "contract terms must not be null");
+    validate_settings(&c.settings, 1);
+    switch (c.kind) {
+      case KG_BOOK_BARRIER:
+        plans[i] = plan_barrier(*(const kg_barrier_terms*)c.terms, c.market, blob);
+        break;
+      case KG_BOOK_ASIAN:
+        plans[i] = plan_asian(*(const kg_asian_terms*)c.terms, c.market, blob);
+        break;
+      case KG_BOOK_AUTOCALL:
+        plans[i] = plan_autocall(*(const kg_autocall_terms*)c.terms, c.market, blob);
+        break;
+      case KG_BOOK_ACCUMULATOR:
+        plans[i] = plan_accumulator(*(const kg_accumulator_terms*)c.terms, c.market, blob);
+        break;
+      default:
+        fail(KG_INVALID_INPUT, "unknown contract kind");
+    }
+    Planned& p = plans[i];
+    if (p.known) continue;
+    p.sampler = validate_settings(&c.settings, (uint64_t)p.item.dims);
+    const Sampler& s = p.sampler;
+    p.item.samples = s.samples();
+    p.item.paths = s.path_count;
+    p.item.seed = s.seed;
+    p.item.antithetic = s.antithetic;
+    p.item.sobol = s.sobol;
+    p.item.first_block = total_blocks;
+    total_blocks += s.blocks();
+    path_steps += (double)s.path_count * (double)p.item.dims;
+    if (s.sobol) {
+      sobol_tables(ctx, (uint32_t)p.item.dims, s.seed, sob_dirs[i], sob_shifts[i]);
+      sob_off[i] = blob.add(sob_dirs[i]);
+      sob_shift_off[i] = blob.add(sob_shifts[i]);
+    }
+  }
+  std::vector<int> sampled;
+  for (size_t i = 0; i < n; ++i)
+    if (!plans[i].known) sampled.push_back((int)i);
+  std::vector<Moments> results(2 * sampled.size());
+  double device_ms = 0.0;
+  if (!sampled.empty()) {
+    if (total_blocks > (int64_t(1) << 31) - 1) fail(KG_INVALID_INPUT, "the book has too many sample blocks");
+    unsigned char* d_blob = (unsigned char*)ctx->book_blob.reserve(blob.bytes.size());
+    std::vector<BookItem> items;
+    for (int i : sampled) {
+      Planned& p = plans[i];
+      for (auto& f : p.fixups) *f.second = d_blob + f.first;
+      if (p.item.sobol) {
+        p.item.dirs = (const uint64_t*)(d_blob + sob_off[i]);
+        p.item.shifts = (const uint64_t*)(d_blob + sob_shift_off[i]);
+      }
+      items.push_back(p.item);
+    }
+    // Tiles, most expensive contracts first (LPT order for the queue).
+    std::vector<int2> tiles;
+    tiles.reserve((size_t)total_blocks);
+    for (size_t k = 0; k < items.size(); ++k) {
+      const int64_t blocks = (items[k].samples + BLOCK_SAMPLES - 1) / BLOCK_SAMPLES;
+      for (int64_t b = 0; b < blocks; ++b) tiles.push_back(make_int2((int)k, (int)b));
+    }
+    std::stable_sort(tiles.begin(), tiles.end(), [&](const int2& x, const int2& y) {
+      return items[x.x].dims > items[y.x].dims;
+    });
+    BookItem* d_items = (BookItem*)ctx->book_items.reserve(items.size() * sizeof(BookItem));
+    int2* d_tiles = (int2*)ctx->book_tiles.reserve(tiles.size() * sizeof(int2));
+    Moments* d_parts = (Moments*)ctx->book_parts.reserve((size_t)total_blocks * 2 * sizeof(Moments));
+    Moments* d_out = (Moments*)ctx->book_out.reserve(items.size() * 2 * sizeof(Moments));
+    int* d_counter = (int*)ctx->book_counter.reserve(256);
+    uint32_t* flags = (uint32_t*)ctx->flags.reserve(256);
+    KG_HIP(hipMemcpyAsync(d_blob, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
+    KG_HIP(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(BookItem), hipMemcpyHostToDevice,
+                          ctx->stream));
+    KG_HIP(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice,
+                          ctx->stream));
+    int spt = tuning ? tuning->samples_per_thread : 0;
+    int wpc = tuning ? tuning->workgroups_per_cu : 0;
+    if (spt != 0 && spt != 1 && spt != 2 && spt != 4) fail(KG_INVALID_INPUT, "samples_per_thread must be 1, 2 or 4");
+    if (wpc < 0 || wpc > 16) fail(KG_INVALID_INPUT, "workgroups_per_cu must be in 0..16");
+    if (!spt || !wpc) {
+      autotune(ctx, fp32, d_items, tiles, d_counter, d_parts, flags);
+      const int slot = fp32 ? 1 : 0;
+      if (!spt) spt = ctx->tuned_spt[slot];
+      if (!wpc) wpc = ctx->tuned_wpc[slot];
+    }
+    const bool use_graph = tuning && tuning->use_graph;
+    const int replays = tuning ? std::max(0, tuning->replays) : 0;
+    if (use_graph) {
+      hipGraph_t graph;
+      hipGraphExec_t exec;
+      KG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      run_book_once(ctx, fp32, spt, wpc, d_items, d_tiles, (int)tiles.size(), d_counter, d_parts, flags);
+      hipLaunchKernelGGL(merge_book, dim3(items.size()), dim3(256), 0, ctx->stream, d_items, d_parts, d_out);
+      KG_HIP(hipStreamEndCapture(ctx->stream, &graph));
+      KG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      timing_begin(ctx);
+      KG_HIP(hipGraphLaunch(exec, ctx->stream));
+      device_ms = timing_end(ctx);
+      if (replays) {
+        timing_begin(ctx);
+        for (int k = 0; k < replays; ++k) KG_HIP(hipGraphLaunch(exec, ctx->stream));
+        device_ms = timing_end(ctx) / replays;
+      }
+      KG_HIP(hipGraphExecDestroy(exec));
+      KG_HIP(hipGraphDestroy(graph));
+    } else {
+      for (int k = 0; k <= replays; ++k) {
+        timing_begin(ctx);
+        run_book_once(ctx, fp32, spt, wpc, d_items, d_tiles, (int)tiles.size(), d_counter, d_parts, flags);
+        hipLaunchKernelGGL(merge_book, dim3(items.size()), dim3(256), 0, ctx->stream, d_items, d_parts, d_out);
+        KG_HIP(hipGetLastError());
+        device_ms = timing_end(ctx);
+      }
+    }
+    check_flags(ctx);
+    KG_HIP(hipMemcpyAsync(results.data(), d_out, results.size() * sizeof(Moments), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    KG_HIP(hipStreamSynchronize(ctx->stream));
+    if (tuning) {
+      tuning->samples_per_thread = spt;
+      tuning->workgroups_per_cu = wpc;
+    }
+    // Arithmetic Asians whose controlled price is negative are resampled
+    // without the control on the same seed (asian.rs:744-752).
+    std::vector<kg_book_contract> redo;
+    std::vector<int> redo_index;
+    for (size_t k = 0; k < sampled.size(); ++k) {
+      Planned& p = plans[sampled[k]];
+      if (p.item.kind == IK_ARITH_ASIAN && p.item.u.a.control) {
+        const double mean = scaled_value(results[2 * k].scale, results[2 * k].mean, 0.0);
+        if (mean + p.control_expected < 0.0) redo_index.push_back((int)k);
+      }
+    }
+    for (int k : redo_index) {
+      Planned& p = plans[sampled[k]];
+      BookItem it = items[k];
+      it.u.a.control = 0;
+      it.first_block = 0;
+      const int64_t blocks = (it.samples + BLOCK_SAMPLES - 1) / BLOCK_SAMPLES;
+      std::vector<int2> t2;
+      for (int64_t b = 0; b < blocks; ++b) t2.push_back(make_int2(0, (int)b));
+      KG_HIP(hipMemcpyAsync(d_items, &it, sizeof(BookItem), hipMemcpyHostToDevice, ctx->stream));
+      KG_HIP(hipMemcpyAsync(d_tiles, t2.data(), t2.size() * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
+      run_book_once(ctx, fp32, spt, wpc, d_items, d_tiles, (int)t2.size(), d_counter, d_parts, flags);
+      hipLaunchKernelGGL(merge_book, dim3(1), dim3(256), 0, ctx->stream, d_items, d_parts, d_out);
+      check_flags(ctx);
+      KG_HIP(hipMemcpyAsync(&results[2 * k], d_out, 2 * sizeof(Moments), hipMemcpyDeviceToHost, ctx->stream));
+      KG_HIP(hipStreamSynchronize(ctx->stream));
+      p.control_expected = 0.0;
+    }
+  }
+  // Estimates in input order.
+  size_t k = 0;
+  for (size_t i = 0; i < n; ++i) {
+    Planned& p = plans[i];
+    const kg_settings& cs = contracts[i].settings;
+    out[i] = kg_estimate{};
+    if (p.known) {
+      fill_known(p.value, cs.sampling == KG_SOBOL, &out[i]);
+      continue;
+    }
+    const Moments& m = results[2 * k];
+    const Moments& pm = results[2 * k + 1];
+    ++k;
+    fill_estimate(p.sampler, m, pm, 0.0, false, true, &out[i]);
+    if (p.kind == Estimator::PathMeans && p.control_expected != 0.0) {
+      out[i].price += p.control_expected;
+      if (out[i].has_standard_error) {
+        const double hw = 1.959963984540054 * out[i].standard_error;
+        out[i].ci_lower = out[i].price - hw;
+        out[i].ci_upper = out[i].price + hw;
+      }
+    }
+    out[i].path_steps = (double)p.sampler.path_count * (double)p.item.dims;
+    out[i].device_ms = device_ms;
+  }
+  if (tuning) {
+    tuning->device_ms = device_ms;
+    tuning->path_steps = path_steps;
+  }
+}
+
+template <class Terms>
+static void price_single(kg_context* ctx, int kind, const kg_settings* settings, const Terms* option,
+                         const kg_bsm_market* market, kg_estimate* out) {
+  if (!settings || !option || !market || !out) fail(KG_INVALID_INPUT, "arguments must not be null");
+  kg_book_contract c{};
+  c.kind = kind;
+  c.settings = *settings;
+  c.market = *market;
+  c.terms = option;
+  kg_book_tuning t{};
+  t.samples_per_thread = 1;
+  t.workgroups_per_cu = 4;
+  price_book(ctx, &c, 1, settings->precision, &t, out);
+}
+
+}  // namespace kg
+
 using namespace kg;
+
 extern "C" {
-kg_status kg_barrier_price(kg_context* ctx, const kg_settings*, const kg_barrier_terms*,
-                           const kg_bsm_market*, kg_estimate*) {
-  return guarded(ctx, [&] { fail(KG_UNSUPPORTED_OPERATION, "not built yet"); });
+
+kg_status kg_barrier_price(kg_context* ctx, const kg_settings* settings, const kg_barrier_terms* option,
+                           const kg_bsm_market* market, kg_estimate* out) {
+  return guarded(ctx, [&] { price_single(ctx, KG_BOOK_BARRIER, settings, option, market, out); });
 }
-kg_status kg_asian_price(kg_context* ctx, const kg_settings*, const kg_asian_terms*,
-                         const kg_bsm_market*, kg_estimate*) {
-  return guarded(ctx, [&] { fail(KG_UNSUPPORTED_OPERATION, "not built yet"); });
+
+kg_status kg_asian_price(kg_context* ctx, const kg_settings* settings, const kg_asian_terms* option,
+                         const kg_bsm_market* market, kg_estimate* out) {
+  return guarded(ctx, [&] { price_single(ctx, KG_BOOK_ASIAN, settings, option, market, out); });
 }
-kg_status kg_accumulator_price(kg_context* ctx, const kg_settings*, const kg_accumulator_terms*,
-                               const kg_bsm_market*, kg_estimate*) {
-  return guarded(ctx, [&] { fail(KG_UNSUPPORTED_OPERATION, "not built yet"); });
+
+kg_status kg_accumulator_price(kg_context* ctx, const kg_settings* settings,
+                               const kg_accumulator_terms* option, const kg_bsm_market* market,
+                               kg_estimate* out) {
+  return guarded(ctx, [&] { price_single(ctx, KG_BOOK_ACCUMULATOR, settings, option, market, out); });
 }
-kg_status kg_price_book(kg_context* ctx, const kg_book_contract*, size_t, int32_t, kg_book_tuning*,
-                        kg_estimate*) {
-  return guarded(ctx, [&] { fail(KG_UNSUPPORTED_OPERATION, "not built yet"); });
+
+kg_status kg_price_book(kg_context* ctx, const kg_book_contract* contracts, size_t n, int32_t precision,
+                        kg_book_tuning* tuning, kg_estimate* out) {
+  return guarded(ctx, [&] { price_book(ctx, contracts, n, precision, tuning, out); });
 }
 }

@@ -115,6 +115,12 @@ static void build_sobol(const std::vector<uint32_t>& table, uint32_t dims, uint6
   }
 }
 
+void sobol_tables(kg_context* ctx, uint32_t dims, uint64_t seed, std::vector<uint64_t>& dirs,
+                  std::vector<uint64_t>& shifts) {
+  if (ctx->joe_kuo.empty()) ctx->joe_kuo.assign(kg_joe_kuo_words, kg_joe_kuo_end);
+  build_sobol(ctx->joe_kuo, dims, seed, dirs, shifts);
+}
+
 const SobolSetup& sobol_setup(kg_context* ctx, uint32_t dims, uint64_t seed) {
   SobolSetup& s = ctx->sobol;
   if (s.valid && s.dims == dims && s.seed == seed) return s;
@@ -171,6 +177,9 @@ void kg_context_destroy(kg_context* ctx) {
     ctx->results.release();
     ctx->sobol.d_directions.release();
     ctx->sobol.d_shifts.release();
+    for (kg::DeviceBuffer* b : {&ctx->book_blob, &ctx->book_items, &ctx->book_tiles, &ctx->book_parts,
+                                &ctx->book_out, &ctx->book_counter})
+      b->release();
     if (ctx->start) (void)hipEventDestroy(ctx->start);
     if (ctx->stop) (void)hipEventDestroy(ctx->stop);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -170,12 +170,12 @@ extern "C" kg_status kg_vanilla_price(kg_context* ctx, const kg_settings* settin
     const double time = (double)(o.expiry_time - m.valuation_time) / (365.0 * 86400e6);
     *out = kg_estimate{};
     if (time == 0.0) {  // monte_carlo.rs:1188-1192
-      fill_known(std::max(sign * (m.spot - o.strike), 0.0), out);
+      fill_known(std::max(sign * (m.spot - o.strike), 0.0), settings->sampling == KG_SOBOL, out);
       return;
     }
     if (m.volatility == 0.0) {  // deterministic::vanilla_to_expiry for flat curves
       double fwd = m.spot * std::exp(-m.dividend * time) - o.strike * std::exp(-m.rate * time);
-      fill_known(std::max(sign * fwd, 0.0), out);
+      fill_known(std::max(sign * fwd, 0.0), settings->sampling == KG_SOBOL, out);
       return;
     }
     EuroParams p{};

@@ -79,6 +79,10 @@ struct kg_context {
   kg::DeviceBuffer partials, scratch, flags, results;
   kg::SobolSetup sobol;
   std::vector<uint32_t> joe_kuo;  // direction-number records, loaded once
+  // Book engine: device buffers and the autotuned tile shape per precision.
+  kg::DeviceBuffer book_blob, book_items, book_tiles, book_parts, book_out, book_counter;
+  int tuned_spt[2] = {0, 0};
+  int tuned_wpc[2] = {0, 0};
 };
 
 namespace kg {
@@ -222,15 +226,20 @@ inline void fill_estimate(const Sampler& s, const Moments& sample, const Moments
   out->device_ms = keep_ms;
 }
 
-// A known value: one Moments add, no sampling (e.g. monte_carlo.rs:1099-1103).
-inline void fill_known(double value, kg_estimate* out) {
+// A known value: one Moments add, no sampling (e.g. monte_carlo.rs:1099-1103);
+// estimate() reports zero uncertainty for it except under Sobol sampling.
+inline void fill_known(double value, bool sobol, kg_estimate* out) {
   if (!std::isfinite(value)) unstable();
   kg_estimate e{};
   e.price = value;
-  e.has_standard_error = 1;
-  e.standard_error = 0.0;
-  e.ci_lower = e.ci_upper = value;
-  e.uncertainty_status = KG_AVAILABLE;
+  if (sobol) {
+    e.uncertainty_status = KG_UNSUPPORTED_SAMPLING;
+  } else {
+    e.has_standard_error = 1;
+    e.standard_error = 0.0;
+    e.ci_lower = e.ci_upper = value;
+    e.uncertainty_status = KG_AVAILABLE;
+  }
   *out = e;
 }
 
@@ -245,6 +254,10 @@ uint32_t* reset_flags(kg_context* ctx);
 void check_flags(kg_context* ctx);
 // Builds (or reuses) the scrambled Sobol directions on the host and device.
 const SobolSetup& sobol_setup(kg_context* ctx, uint32_t dims, uint64_t seed);
+// Builds scrambled Sobol directions ([column * dims + dim], 52 columns) and
+// shifts on the host without touching the cache.
+void sobol_tables(kg_context* ctx, uint32_t dims, uint64_t seed, std::vector<uint64_t>& dirs,
+                  std::vector<uint64_t>& shifts);
 // Event timing on the context stream.
 void timing_begin(kg_context* ctx);
 double timing_end(kg_context* ctx);

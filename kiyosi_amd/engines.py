@@ -129,3 +129,85 @@ class MonteCarloAutocallableEngine(_MonteCarloEngine):
         if not isinstance(instrument, _Autocallable):
             raise TypeError("MonteCarloAutocallableEngine prices autocallable notes")
         return _native.lib().kg_autocallable_price, instrument._terms(), context.bsm_market()
+
+
+class MonteCarloBarrierEngine(_MonteCarloEngine):
+    """Single-barrier Monte Carlo (engines/monte_carlo/barrier.rs)."""
+
+    def _entry(self, instrument, context):
+        from .instruments import BarrierOption
+        if not isinstance(instrument, BarrierOption):
+            raise TypeError("MonteCarloBarrierEngine prices BarrierOption")
+        return _native.lib().kg_barrier_price, instrument._terms(), context.bsm_market()
+
+
+class MonteCarloAsianEngine(_MonteCarloEngine):
+    """Average-price Monte Carlo (engines/monte_carlo/asian.rs)."""
+
+    def __init__(self, *, step_count=50, **kwargs):
+        if not 2 <= step_count <= 10_000:
+            raise InvalidInputError("step_count is outside the supported Monte Carlo range")
+        self.step_count = step_count
+        super().__init__(**kwargs)
+
+    def _kwargs(self):
+        return dict(super()._kwargs(), step_count=self.step_count)
+
+    def _entry(self, instrument, context):
+        from .instruments import _AveragePrice
+        if not isinstance(instrument, _AveragePrice):
+            raise TypeError("MonteCarloAsianEngine prices average-price options")
+        return _native.lib().kg_asian_price, instrument._terms(self.step_count), context.bsm_market()
+
+
+class MonteCarloAccumulatorEngine(_MonteCarloEngine):
+    """Accumulator Monte Carlo (engines/monte_carlo/accumulator.rs)."""
+
+    def _entry(self, instrument, context):
+        from .instruments import Accumulator
+        if not isinstance(instrument, Accumulator):
+            raise TypeError("MonteCarloAccumulatorEngine prices Accumulator")
+        return _native.lib().kg_accumulator_price, instrument._terms(), context.bsm_market()
+
+
+_BOOK_KINDS = {
+    "MonteCarloBarrierEngine": _abi.BOOK_BARRIER,
+    "MonteCarloAsianEngine": _abi.BOOK_ASIAN,
+    "MonteCarloAutocallableEngine": _abi.BOOK_AUTOCALL,
+    "MonteCarloAccumulatorEngine": _abi.BOOK_ACCUMULATOR,
+}
+
+
+class Book:
+    """A heterogeneous set of (engine, instrument, context) entries priced in one
+    persistent GPU launch; each entry's result equals its engine's own price."""
+
+    def __init__(self, entries):
+        self.entries = list(entries)
+        n = len(self.entries)
+        self._array = (_abi.BookContract * max(n, 1))()
+        self._keep = []
+        for i, (engine, instrument, context) in enumerate(self.entries):
+            kind = _BOOK_KINDS.get(type(engine).__name__)
+            if kind is None:
+                raise TypeError(f"{type(engine).__name__} cannot join a book")
+            engine = engine if engine.seed is not None else engine.with_resolved_seed()
+            if kind == _abi.BOOK_ASIAN:
+                terms = instrument._terms(engine.step_count)
+            else:
+                terms = instrument._terms()
+            self._keep.append(terms)
+            self._array[i] = _abi.BookContract(kind, 0, engine._settings(), context.bsm_market(),
+                                               C.cast(C.pointer(terms), C.c_void_p))
+
+    def price(self, precision="fp64", device=0, samples_per_thread=0, workgroups_per_cu=0,
+              use_graph=True, replays=0):
+        """Prices every entry; returns (estimates, tuning) where tuning reports the
+        tile shape used, the mean device time of one launch and the path-steps."""
+        ctx = _native.context(device)
+        n = len(self.entries)
+        out = (_abi.Estimate * max(n, 1))()
+        tuning = _abi.BookTuning(samples_per_thread, workgroups_per_cu, int(use_graph), replays, 0.0, 0.0)
+        ctx.check(_native.lib().kg_price_book(ctx.handle, self._array, n, _PRECISION[precision],
+                                              C.byref(tuning), out))
+        return [MonteCarloEstimate(out[i]) for i in range(n)], tuning

@@ -267,3 +267,108 @@ class TernarySnowballOption(_Autocallable):
 
     def _coupon(self):
         return self.maturity_coupon_rate
+
+
+_BARRIER_TYPES = {"down_and_out": _abi.DOWN_AND_OUT, "down_and_in": _abi.DOWN_AND_IN,
+                  "up_and_out": _abi.UP_AND_OUT, "up_and_in": _abi.UP_AND_IN}
+
+
+class BarrierOption:
+    """Single barrier call or put (instruments/barrier.rs); continuous monitoring
+    without an observation schedule, scheduled fixings otherwise."""
+
+    def __init__(self, *, option_type, strike, effective_time, expiry_time, barrier_level, barrier_type,
+                 rebate=0.0, rebate_timing="at_expiry", observation_schedule=None, touch_state=None,
+                 calendar=None):
+        if option_type not in ("call", "put"):
+            raise InvalidInputError("option_type must be 'call' or 'put'")
+        if barrier_type not in _BARRIER_TYPES:
+            raise InvalidInputError("unknown barrier_type")
+        if rebate_timing not in ("at_hit", "at_expiry"):
+            raise InvalidInputError("rebate_timing must be 'at_hit' or 'at_expiry'")
+        if touch_state == "touched":
+            raise InvalidInputError("touched barriers are resolved by the CPU engine")
+        self.option_type = option_type
+        self.strike = _positive("strike", strike)
+        self.barrier_level = _positive("barrier_level", barrier_level)
+        self.barrier_type = barrier_type
+        self.rebate = _finite("rebate", rebate)
+        self.rebate_timing = rebate_timing
+        self.effective_time = effective_time
+        self.expiry_time = expiry_time
+        self.observation_schedule = None if observation_schedule is None else list(observation_schedule)
+        self.calendar = calendar
+
+    def _terms(self):
+        times = self.observation_schedule
+        arr, n = None, 0
+        if times is not None:
+            arr = (C.c_int64 * max(len(times), 1))(*[to_micros(t) for t in times])
+            n = len(times)
+        terms = _abi.BarrierTerms(
+            1 if self.option_type == "call" else -1, _BARRIER_TYPES[self.barrier_type],
+            _abi.OBSERVE_CONTINUOUS if times is None else _abi.OBSERVE_SCHEDULED,
+            _abi.REBATE_AT_HIT if self.rebate_timing == "at_hit" else _abi.REBATE_AT_EXPIRY,
+            self.strike, self.barrier_level, self.rebate, to_micros(self.effective_time),
+            to_micros(self.expiry_time), n, 0, C.cast(arr, C.POINTER(C.c_int64)) if arr is not None else None)
+        terms._keep = arr
+        return terms
+
+
+class _AveragePrice:
+    _averaging = _abi.AVERAGE_ARITHMETIC
+
+    def __init__(self, *, option_type, strike, effective_time, averaging_start_time, expiry_time,
+                 realized_average=None):
+        if option_type not in ("call", "put"):
+            raise InvalidInputError("option_type must be 'call' or 'put'")
+        self.option_type = option_type
+        self.strike = _positive("strike", strike)
+        self.effective_time = effective_time
+        self.averaging_start_time = averaging_start_time
+        self.expiry_time = expiry_time
+        self.realized_average = realized_average
+
+    def _terms(self, step_count=50):
+        return _abi.AsianTerms(1 if self.option_type == "call" else -1, self._averaging, step_count,
+                               int(self.realized_average is not None), self.strike,
+                               float(self.realized_average or 0.0), to_micros(self.effective_time),
+                               to_micros(self.averaging_start_time), to_micros(self.expiry_time))
+
+
+class ArithmeticAveragePriceOption(_AveragePrice):
+    """Continuously averaged arithmetic-average-price option (instruments/asian.rs)."""
+
+
+class GeometricAveragePriceOption(_AveragePrice):
+    """Continuously averaged geometric-average-price option (instruments/asian.rs)."""
+
+    _averaging = _abi.AVERAGE_GEOMETRIC
+
+
+class Accumulator:
+    """Accumulator (call) or decumulator (put) with daily accrual and knock-out
+    (instruments/accumulator.rs)."""
+
+    def __init__(self, *, strike, knock_out_level, daily_quantity, acceleration_factor, effective_time,
+                 expiry_time, observation_clock_time=_dt.time(0, 0), accumulated_quantity=None,
+                 option_type="call", calendar=None):
+        if option_type not in ("call", "put"):
+            raise InvalidInputError("option_type must be 'call' or 'put'")
+        self.strike = _positive("strike", strike)
+        self.knock_out_level = _positive("knock_out_level", knock_out_level)
+        self.daily_quantity = _finite("daily_quantity", daily_quantity)
+        self.acceleration_factor = _finite("acceleration_factor", acceleration_factor)
+        self.effective_time = effective_time
+        self.expiry_time = expiry_time
+        self.observation_clock_time = observation_clock_time
+        self.accumulated_quantity = float(accumulated_quantity or 0.0)
+        self.option_type = option_type
+        self.calendar = calendar if calendar is not None else TradingCalendar.weekdays()
+
+    def _terms(self):
+        return _abi.AccumulatorTerms(
+            1 if self.option_type == "call" else -1, self.calendar.kind, self.strike, self.knock_out_level,
+            self.daily_quantity, self.acceleration_factor, self.accumulated_quantity,
+            clock_micros(self.observation_clock_time), to_micros(self.effective_time),
+            to_micros(self.expiry_time))

@@ -56,3 +56,75 @@ def snowball_252d_sobol_2p20():
                                                       volatility=0.25),
         spot_price=100.0, valuation_time=VALUATION)
     return note, context, dict(path_count=1 << 20, seed=1729, sampling="sobol")
+
+
+BOOK_SEED = 20251203
+BOOK_KINDS = ("up_and_out", "down_and_in", "geometric_asian", "arithmetic_asian", "snowball",
+              "phoenix", "accumulator")
+
+
+def book_4096(n=4096, seed=BOOK_SEED, path_scale=1.0):
+    """Heterogeneous book drawn from `seed`: kinds uniform over up-and-out and
+    down-and-in barriers (daily fixings), geometric and arithmetic Asians
+    (daily grid), snowballs, Phoenix notes and accumulators; maturity
+    U[0.25, 3] years as trading days (x252); K/S0 ~ U[0.8, 1.2]; barriers
+    U[0.6, 0.9] (down) or U[1.05, 1.3] (up) of S0; sigma ~ U[0.15, 0.45],
+    r ~ U[0, 0.05], q ~ U[0, 0.03]; paths 256 * round(2^U[14, 18] / 256),
+    scaled by `path_scale`; contract i is seeded seed + i. Returns a list of
+    (engine, instrument, context)."""
+    import numpy as np
+
+    from .engines import (MonteCarloAccumulatorEngine, MonteCarloAsianEngine,
+                          MonteCarloAutocallableEngine, MonteCarloBarrierEngine)
+    from .instruments import (Accumulator, ArithmeticAveragePriceOption, BarrierOption,
+                              CouponObservation, GeometricAveragePriceOption, KnockOutObservation,
+                              PhoenixOption)
+
+    rng = np.random.default_rng(seed)
+    calendar = TradingCalendar.weekdays()
+    trading = [VALUATION]
+    while len(trading) <= 3 * 252 + 1:
+        trading.append(calendar.add_trading_days(trading[-1], 1))
+    entries = []
+    for i in range(n):
+        kind = BOOK_KINDS[int(rng.integers(len(BOOK_KINDS)))]
+        days = int(round(rng.uniform(0.25, 3.0) * 252))
+        expiry = trading[days]
+        moneyness = rng.uniform(0.8, 1.2)
+        sigma, r, q = rng.uniform(0.15, 0.45), rng.uniform(0.0, 0.05), rng.uniform(0.0, 0.03)
+        paths = max(256, int(256 * round(2 ** rng.uniform(14, 18) / 256) * path_scale) // 256 * 256)
+        down, up = rng.uniform(0.6, 0.9) * 100, rng.uniform(1.05, 1.3) * 100
+        option_type = "call" if rng.uniform() < 0.5 else "put"
+        context = PricingContext(model_parameters=BlackScholesMertonParameters(
+            risk_free_rate=r, dividend_yield=q, volatility=sigma), spot_price=100.0, valuation_time=VALUATION)
+        kw = dict(path_count=paths, seed=seed + i)
+        if kind in ("up_and_out", "down_and_in"):
+            instrument = BarrierOption(option_type=option_type, strike=100 * moneyness, effective_time=VALUATION,
+                                       expiry_time=expiry, barrier_level=up if kind == "up_and_out" else down,
+                                       barrier_type=kind, observation_schedule=trading[1:days + 1])
+            engine = MonteCarloBarrierEngine(**kw)
+        elif kind.endswith("asian"):
+            cls = GeometricAveragePriceOption if kind.startswith("geo") else ArithmeticAveragePriceOption
+            instrument = cls(option_type=option_type, strike=100 * moneyness, effective_time=VALUATION,
+                             averaging_start_time=VALUATION, expiry_time=expiry)
+            engine = MonteCarloAsianEngine(step_count=max(2, days), **kw)
+        elif kind == "snowball":
+            months = max(1, days // 21)
+            instrument = snowball_note(trading_days=21 * months, months=months, knock_in=down)
+            engine = MonteCarloAutocallableEngine(**kw)
+        elif kind == "phoenix":
+            months = max(1, days // 21)
+            dates = [trading[21 * k] for k in range(1, months + 1)]
+            instrument = PhoenixOption(
+                coupon_per_period=0.01, initial_spot=100.0, knock_in_level=down, upper_strike=100.0,
+                lower_strike=0.0, knock_in_observation=KnockInObservation.every_trading_day(dt.time(0, 0)),
+                effective_time=VALUATION, expiry_time=dates[-1],
+                knock_out_observations=[KnockOutObservation(d, 100.0) for d in dates],
+                coupon_observations=[CouponObservation(d, 80.0) for d in dates], calendar=calendar)
+            engine = MonteCarloAutocallableEngine(**kw)
+        else:
+            instrument = Accumulator(strike=100 * moneyness, knock_out_level=up, daily_quantity=1.0,
+                                     acceleration_factor=2.0, effective_time=VALUATION, expiry_time=expiry)
+            engine = MonteCarloAccumulatorEngine(**kw)
+        entries.append((engine, instrument, context))
+    return entries

@@ -1,0 +1,156 @@
+"""GPU parity of the barrier, Asian and accumulator engines and of the
+persistent book kernel with the FP64 oracle."""
+
+import ctypes as C
+import datetime as dt
+import math
+
+import numpy as np
+import pytest
+
+from conftest import END, START, us
+from kiyosi_amd import _abi, _native, workloads
+from kiyosi_amd.engines import Book
+
+pytestmark = pytest.mark.gpu
+
+
+def market(spot=100.0, rate=0.05, dividend=0.02, sigma=0.2):
+    return _abi.BsmMarket(spot, rate, dividend, sigma, us(START))
+
+
+def settings(paths, seed=7, antithetic=False, sampling=_abi.PSEUDO_RANDOM, precision=_abi.FP64):
+    return _abi.Settings(paths, seed, 1, sampling, int(antithetic), precision, 0, 0)
+
+
+def daily(days=365):
+    return [us(START + dt.timedelta(days=k)) for k in range(1, days + 1)]
+
+
+def barrier_terms(option_type, barrier_type, level, strike=100.0, rebate=0.0, times=None, at_hit=False):
+    arr = (C.c_int64 * len(times))(*times) if times else None
+    t = _abi.BarrierTerms(option_type, barrier_type,
+                          _abi.OBSERVE_SCHEDULED if times else _abi.OBSERVE_CONTINUOUS, int(at_hit), strike, level,
+                          rebate, us(START), us(END), len(times) if times else 0, 0,
+                          C.cast(arr, C.POINTER(C.c_int64)) if arr else None)
+    t._keep = arr
+    return t
+
+
+def gpu(ctx, fn, s, terms, m):
+    out = _abi.Estimate()
+    ctx.check(getattr(_native.lib(), fn)(ctx.handle, C.byref(s), C.byref(terms), C.byref(m), C.byref(out)))
+    return out
+
+
+BARRIERS = [
+    ("continuous down-out call", lambda: barrier_terms(1, _abi.DOWN_AND_OUT, 90.0), 65536, False),
+    ("continuous up-in put", lambda: barrier_terms(-1, _abi.UP_AND_IN, 120.0), 4097, True),
+    ("daily up-out call rebate at hit", lambda: barrier_terms(1, _abi.UP_AND_OUT, 125.0, rebate=3.0,
+                                                              times=daily(), at_hit=True), 5000, False),
+    ("daily down-in put conditioned", lambda: barrier_terms(-1, _abi.DOWN_AND_IN, 88.0, times=daily()), 6000, True),
+    ("weekly down-in call", lambda: barrier_terms(1, _abi.DOWN_AND_IN, 90.0, times=daily()[6::7]), 3000, False),
+]
+
+
+@pytest.mark.parametrize("name,make,paths,antithetic", BARRIERS, ids=[b[0] for b in BARRIERS])
+def test_barrier_fp64_matches_the_oracle(gpu, oracle, name, make, paths, antithetic):
+    terms, s = make(), settings(paths, antithetic=antithetic)
+    got = gpu(gpu, "kg_barrier_price", s, terms, market())
+    ref = oracle.barrier_price(s, terms, market(), workers=8)
+    assert got.price == pytest.approx(ref.price, rel=1e-10, abs=1e-13)
+    assert got.uncertainty_status == ref.uncertainty_status
+    if ref.has_standard_error:
+        assert got.standard_error == pytest.approx(ref.standard_error, rel=1e-8)
+
+
+def test_barrier_calibration_vector_on_the_gpu(gpu):
+    got = gpu(gpu, "kg_barrier_price", settings(65536, seed=1009), barrier_terms(1, _abi.DOWN_AND_OUT, 90.0),
+              market())
+    assert got.price == pytest.approx(7.616185456813128, rel=1e-10)
+
+
+@pytest.mark.parametrize("averaging", [_abi.AVERAGE_GEOMETRIC, _abi.AVERAGE_ARITHMETIC])
+@pytest.mark.parametrize("kind", [1, -1])
+@pytest.mark.parametrize("paths,antithetic,sampling", [(4096, False, _abi.PSEUDO_RANDOM),
+                                                      (3001, True, _abi.PSEUDO_RANDOM),
+                                                      (2048, False, _abi.SOBOL)])
+def test_asian_fp64_matches_the_oracle(gpu, oracle, averaging, kind, paths, antithetic, sampling):
+    t = _abi.AsianTerms(kind, averaging, 64, 0, 100.0, 0.0, us(START), us(START), us(END))
+    s = settings(paths, antithetic=antithetic, sampling=sampling)
+    m = market(rate=0.04, dividend=0.01)
+    got, ref = gpu(gpu, "kg_asian_price", s, t, m), oracle.asian_price(s, t, m, workers=8)
+    assert got.price == pytest.approx(ref.price, rel=1e-10)
+    assert got.uncertainty_status == ref.uncertainty_status
+
+
+@pytest.mark.parametrize("kind,strike,ko", [(1, 97.0, 106.0), (-1, 103.0, 94.0)])
+def test_accumulator_fp64_matches_the_oracle(gpu, oracle, kind, strike, ko):
+    t = _abi.AccumulatorTerms(kind, _abi.CALENDAR_WEEKDAYS, strike, ko, 1.0, 2.0, 0.0, 0, us(START),
+                              us(START + dt.timedelta(days=180)))
+    s = settings(5000, antithetic=True)
+    m = market(sigma=0.25)
+    got, ref = gpu(gpu, "kg_accumulator_price", s, t, m), oracle.accumulator_price(s, t, m, workers=8)
+    assert got.price == pytest.approx(ref.price, rel=1e-10)
+    assert got.standard_error == pytest.approx(ref.standard_error, rel=1e-8)
+
+
+def oracle_entry(oracle, engine, instrument, context):
+    s = engine._settings()
+    m = context.bsm_market()
+    name = type(engine).__name__
+    if name == "MonteCarloBarrierEngine":
+        return oracle.barrier_price(s, instrument._terms(), m, workers=16)
+    if name == "MonteCarloAsianEngine":
+        return oracle.asian_price(s, instrument._terms(engine.step_count), m, workers=16)
+    if name == "MonteCarloAutocallableEngine":
+        return oracle.autocallable_price(s, instrument._terms(), m, workers=16)
+    return oracle.accumulator_price(s, instrument._terms(), m, workers=16)
+
+
+def small_book(n=70):
+    return workloads.book_4096(n=n, path_scale=1 / 64)
+
+
+def test_book_fp64_matches_the_oracle_contract_by_contract(gpu, oracle):
+    entries = small_book()
+    estimates, tuning = Book(entries).price("fp64")
+    for (engine, instrument, context), got in zip(entries, estimates):
+        ref = oracle_entry(oracle, engine, instrument, context)
+        assert got.price == pytest.approx(ref.price, rel=1e-10, abs=1e-12), type(instrument).__name__
+        assert got.uncertainty_status == _abi.UNCERTAINTY_NAMES[ref.uncertainty_status]
+    assert tuning.samples_per_thread in (1, 2, 4) and tuning.workgroups_per_cu in (2, 4, 8)
+
+
+def test_book_results_do_not_depend_on_tile_shape_or_graph(gpu):
+    book = Book(small_book(40))
+    base, _ = book.price("fp64", samples_per_thread=1, workgroups_per_cu=2, use_graph=False)
+    for spt, wpc, graph in ((2, 4, True), (4, 8, True), (4, 2, False)):
+        other, _ = book.price("fp64", samples_per_thread=spt, workgroups_per_cu=wpc, use_graph=graph, replays=3)
+        assert [e.price for e in other] == [e.price for e in base]
+
+
+def test_graph_replays_repeat_the_eager_result(gpu):
+    book = Book(small_book(30))
+    eager, _ = book.price("fp32", samples_per_thread=2, workgroups_per_cu=4, use_graph=False)
+    for _ in range(3):
+        replayed, _ = book.price("fp32", samples_per_thread=2, workgroups_per_cu=4, use_graph=True, replays=2)
+        assert [e.price for e in replayed] == [e.price for e in eager]
+
+
+def test_book_matches_single_contract_engines(gpu):
+    entries = small_book(21)
+    estimates, _ = Book(entries).price("fp64")
+    for (engine, instrument, context), got in zip(entries, estimates):
+        single = engine.price_with_statistics(instrument, context)
+        assert got.price == pytest.approx(single.price, rel=1e-12, abs=1e-14)
+
+
+def test_fp32_book_within_half_a_standard_error(gpu):
+    book = Book(workloads.book_4096(n=280, path_scale=1 / 4))
+    plain, _ = book.price("fp64")
+    fast, _ = book.price("fp32")
+    errors = np.array([abs(f.price - p.price) / p.path_standard_error for f, p in zip(fast, plain)
+                       if p.path_standard_error > 0])
+    assert np.mean(errors <= 0.5) >= 0.99, np.sort(errors)[-5:]
+    assert errors.max() < 1.0
