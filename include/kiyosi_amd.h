@@ -346,6 +346,76 @@ typedef struct kg_book_tuning {
 kg_status kg_price_book(kg_context* ctx, const kg_book_contract* contracts, size_t n,
                         int32_t precision, kg_book_tuning* tuning, kg_estimate* out);
 
+/* ---- American options by least-squares Monte Carlo:
+ * MonteCarloVanillaEngine::price for AmericanOption (in-sample,
+ * engines/monte_carlo/american.rs:955-1455) and
+ * price_american_out_of_sample (:574-637).
+ *
+ * Model: flat BSM (surface == NULL) follows the reference exactly:
+ * normalised Brownian coordinates per path, spot = S0 e^{drift (k+1)}
+ * e^{diffusion W_k}, block-ordered normal equations. A local-volatility
+ * surface (LocalVolatilitySurface, market/local_volatility.rs:61-359:
+ * bilinear in Actual/365 years from the first row and in spot, spot clamped
+ * to the grid) extends the engine: log-Euler steps with sigma(S_k, t_k)
+ * taken at the start of each exercise interval.
+ *
+ * Regressors: KG_BASIS_QUADRATIC is the reference's (1, S/K, (S/K)^2) with
+ * Gaussian elimination and degree fallback (:1693-1749); KG_BASIS_POLY4 uses
+ * (1 .. (S/K)^4) with an FP64 Cholesky solve; KG_BASIS_MLP fits a
+ * 3-64-64-1 GELU network on (S/K, t, sigma_loc) per exercise date with
+ * bf16 MFMA forward/backward and Adam, warm-started from the later date. */
+enum { KG_BASIS_QUADRATIC = 0, KG_BASIS_POLY4 = 1, KG_BASIS_MLP = 2 };
+
+typedef struct kg_local_vol_surface {
+  int32_t n_times; /* >= 2 rows */
+  int32_t n_spots; /* >= 2 columns */
+  const int64_t* times;  /* strictly increasing row times */
+  const double* spots;   /* strictly increasing positive spot nodes */
+  const double* values;  /* [n_times][n_spots] positive volatilities */
+} kg_local_vol_surface;
+
+typedef struct kg_american_terms {
+  int32_t option_type; /* 1 call, -1 put */
+  int32_t step_count;  /* exercise intervals, >= 2 */
+  double strike;
+  int64_t effective_time;
+  int64_t expiry_time;
+} kg_american_terms;
+
+typedef struct kg_american_model {
+  double spot;
+  double rate;
+  double dividend;
+  double volatility; /* BSM volatility when surface is NULL */
+  int64_t valuation_time;
+  const kg_local_vol_surface* surface;
+} kg_american_model;
+
+typedef struct kg_lsmc_options {
+  int32_t basis;               /* KG_BASIS_* */
+  int32_t out_of_sample;       /* 0: in-sample price on the training paths */
+  int64_t training_path_count; /* training paths for out-of-sample pricing */
+  int32_t epochs;              /* MLP: passes over each date's samples (default 4) */
+  int32_t batch;               /* MLP: minibatch size (default 4096) */
+  double learning_rate;        /* MLP: Adam step (default 1e-3) */
+} kg_lsmc_options;
+
+typedef struct kg_lsmc_report {
+  double price;          /* the engine's price: in-sample, or out-of-sample when requested */
+  double standard_error; /* of the out-of-sample estimate (pseudorandom sampling) */
+  double in_sample_price;
+  double training_ms;    /* device time of path generation and regression */
+  double pricing_ms;     /* device time of the out-of-sample pass */
+  double path_steps;     /* simulated path-steps (training + pricing) */
+  double mean_loss;      /* MLP: mean squared error of the last epoch over all dates */
+  int32_t dates_fitted;  /* exercise dates with a fitted continuation value */
+  int32_t reserved;
+} kg_lsmc_report;
+
+kg_status kg_american_price(kg_context* ctx, const kg_settings* settings,
+                            const kg_american_terms* option, const kg_american_model* model,
+                            const kg_lsmc_options* options, kg_lsmc_report* out);
+
 /* ---- Generator diagnostics, computed on the GPU with per-thread skip-ahead.
  * kg_pcg32_outputs: outputs first..first+count of Pcg32::new(seed, stream)
  * (monte_carlo.rs:2247-2267). kg_sobol_points: the 52-bit scrambled Sobol

@@ -24,6 +24,7 @@ from .errors import (
 )
 from .instruments import (
     Accumulator,
+    AmericanOption,
     ArithmeticAveragePriceOption,
     BarrierOption,
     BinarySnowballOption,
@@ -37,6 +38,12 @@ from .instruments import (
     SnowballOption,
     TernarySnowballOption,
 )
-from .market import BlackScholesMertonParameters, PricingContext, TradingCalendar
+from .market import (
+    BlackScholesMertonParameters,
+    LocalVolatilityParameters,
+    LocalVolatilitySurface,
+    PricingContext,
+    TradingCalendar,
+)
 
 __version__ = "0.1.0"

@@ -218,3 +218,62 @@ class BookTuning(C.Structure):
         ("device_ms", C.c_double),
         ("path_steps", C.c_double),
     ]
+
+
+BASIS_QUADRATIC, BASIS_POLY4, BASIS_MLP = 0, 1, 2
+
+
+class LocalVolSurface(C.Structure):
+    _fields_ = [
+        ("n_times", C.c_int32),
+        ("n_spots", C.c_int32),
+        ("times", C.POINTER(C.c_int64)),
+        ("spots", C.POINTER(C.c_double)),
+        ("values", C.POINTER(C.c_double)),
+    ]
+
+
+class AmericanTerms(C.Structure):
+    _fields_ = [
+        ("option_type", C.c_int32),
+        ("step_count", C.c_int32),
+        ("strike", C.c_double),
+        ("effective_time", C.c_int64),
+        ("expiry_time", C.c_int64),
+    ]
+
+
+class AmericanModel(C.Structure):
+    _fields_ = [
+        ("spot", C.c_double),
+        ("rate", C.c_double),
+        ("dividend", C.c_double),
+        ("volatility", C.c_double),
+        ("valuation_time", C.c_int64),
+        ("surface", C.POINTER(LocalVolSurface)),
+    ]
+
+
+class LsmcOptions(C.Structure):
+    _fields_ = [
+        ("basis", C.c_int32),
+        ("out_of_sample", C.c_int32),
+        ("training_path_count", C.c_int64),
+        ("epochs", C.c_int32),
+        ("batch", C.c_int32),
+        ("learning_rate", C.c_double),
+    ]
+
+
+class LsmcReport(C.Structure):
+    _fields_ = [
+        ("price", C.c_double),
+        ("standard_error", C.c_double),
+        ("in_sample_price", C.c_double),
+        ("training_ms", C.c_double),
+        ("pricing_ms", C.c_double),
+        ("path_steps", C.c_double),
+        ("mean_loss", C.c_double),
+        ("dates_fitted", C.c_int32),
+        ("reserved", C.c_int32),
+    ]

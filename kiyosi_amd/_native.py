@@ -39,6 +39,8 @@ SIGNATURES = {
                                  P(_abi.Estimate)]),
     "kg_accumulator_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AccumulatorTerms), P(_abi.BsmMarket),
                                        P(_abi.Estimate)]),
+    "kg_american_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AmericanTerms), P(_abi.AmericanModel),
+                                    P(_abi.LsmcOptions), P(_abi.LsmcReport)]),
     "kg_price_book": (C.c_int, [C.c_void_p, P(_abi.BookContract), C.c_size_t, C.c_int32, P(_abi.BookTuning),
                                 P(_abi.Estimate)]),
 }

@@ -181,8 +181,14 @@ struct Planned {
   double control_expected = 0.0;  // arithmetic Asian control mean
   BookItem item{};
   Sampler sampler{};
-  // tables staged into the shared blob: (offset, pointer slot) fixups
-  std::vector<std::pair<size_t, const void**>> fixups;
+  // Tables staged into the shared blob: (blob offset, byte offset of the
+  // pointer field inside `item`). Offsets, not addresses, because the plan
+  // is moved after it is built.
+  std::vector<std::pair<size_t, size_t>> fixups;
+  template <class T>
+  void stage(size_t blob_offset, T* const& field) {
+    fixups.push_back({blob_offset, (size_t)((const unsigned char*)&field - (const unsigned char*)&item)});
+  }
 };
 
 struct Blob {
@@ -336,8 +342,8 @@ static Planned plan_barrier(const kg_barrier_terms& o, const kg_bsm_market& m, B
   pl.item.dims = b.n_steps;
   pl.item.u.b = b;
   const size_t o1 = blob.add(steps), o2 = blob.add(steps32);
-  pl.fixups.push_back({o1, (const void**)&pl.item.u.b.steps});
-  pl.fixups.push_back({o2, (const void**)&pl.item.u.b.steps32});
+  pl.stage(o1, pl.item.u.b.steps);
+  pl.stage(o2, pl.item.u.b.steps32);
   return pl;
 }
 
@@ -497,8 +503,8 @@ static Planned plan_accumulator(const kg_accumulator_terms& o, const kg_bsm_mark
   pl.item.dims = a.n_steps;
   pl.item.u.acc = a;
   const size_t o1 = blob.add(steps), o2 = blob.add(steps32);
-  pl.fixups.push_back({o1, (const void**)&pl.item.u.acc.steps});
-  pl.fixups.push_back({o2, (const void**)&pl.item.u.acc.steps32});
+  pl.stage(o1, pl.item.u.acc.steps);
+  pl.stage(o2, pl.item.u.acc.steps32);
   return pl;
 }
 
@@ -514,18 +520,18 @@ static Planned plan_autocall(const kg_autocall_terms& t, const kg_bsm_market& m,
   pl.item.dims = h.dev.n_steps;
   pl.item.u.ac = h.dev;
   AcPlanDev& p = pl.item.u.ac;
-  pl.fixups.push_back({blob.add(h.steps), (const void**)&p.steps});
-  pl.fixups.push_back({blob.add(h.step_event), (const void**)&p.step_event});
-  pl.fixups.push_back({blob.add(h.step_observe), (const void**)&p.step_observe});
-  pl.fixups.push_back({blob.add(h.events), (const void**)&p.events});
-  pl.fixups.push_back({blob.add(h.intervals), (const void**)&p.intervals});
-  pl.fixups.push_back({blob.add(h.weight), (const void**)&p.weight});
-  pl.fixups.push_back({blob.add(h.fill), (const void**)&p.fill});
-  pl.fixups.push_back({blob.add(h.steps32), (const void**)&p.steps32});
-  pl.fixups.push_back({blob.add(h.weight32), (const void**)&p.weight32});
-  pl.fixups.push_back({blob.add(h.fill32), (const void**)&p.fill32});
-  pl.fixups.push_back({blob.add(h.events32), (const void**)&p.events32});
-  pl.fixups.push_back({blob.add(h.intervals32), (const void**)&p.intervals32});
+  pl.stage(blob.add(h.steps), p.steps);
+  pl.stage(blob.add(h.step_event), p.step_event);
+  pl.stage(blob.add(h.step_observe), p.step_observe);
+  pl.stage(blob.add(h.events), p.events);
+  pl.stage(blob.add(h.intervals), p.intervals);
+  pl.stage(blob.add(h.weight), p.weight);
+  pl.stage(blob.add(h.fill), p.fill);
+  pl.stage(blob.add(h.steps32), p.steps32);
+  pl.stage(blob.add(h.weight32), p.weight32);
+  pl.stage(blob.add(h.fill32), p.fill32);
+  pl.stage(blob.add(h.events32), p.events32);
+  pl.stage(blob.add(h.intervals32), p.intervals32);
   return pl;
 }
 
@@ -599,6 +605,28 @@ static void autotune(kg_context* ctx, bool fp32, const BookItem* items,
   ctx->tuned_wpc[slot] = best_wpc;
 }
 
+// Every table a kernel dereferences must point into the uploaded blob.
+static void check_tables(const BookItem& it) {
+  bool ok = !it.sobol || (it.dirs && it.shifts);
+  switch (it.kind) {
+    case IK_BARRIER:
+      ok = ok && it.u.b.steps && it.u.b.steps32;
+      break;
+    case IK_ACCUM:
+      ok = ok && it.u.acc.steps && it.u.acc.steps32;
+      break;
+    case IK_AUTOCALL: {
+      const AcPlanDev& p = it.u.ac;
+      ok = ok && p.steps && p.step_event && p.step_observe && p.events && p.intervals && p.weight &&
+           p.fill && p.steps32 && p.weight32 && p.fill32 && p.events32 && p.intervals32;
+      break;
+    }
+    default:
+      break;
+  }
+  if (!ok) fail(KG_COMPUTATION_FAILURE, "a contract table was not staged on the device");
+}
+
 struct BookRun {
   std::vector<Planned> plans;
   std::vector<int> sampled;  // indices into plans with sampling work
@@ -666,11 +694,15 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
     std::vector<BookItem> items;
     for (int i : sampled) {
       Planned& p = plans[i];
-      for (auto& f : p.fixups) *f.second = d_blob + f.first;
+      for (auto& f : p.fixups) {
+        const void* at = d_blob + f.first;
+        std::memcpy((unsigned char*)&p.item + f.second, &at, sizeof(at));
+      }
       if (p.item.sobol) {
         p.item.dirs = (const uint64_t*)(d_blob + sob_off[i]);
         p.item.shifts = (const uint64_t*)(d_blob + sob_shift_off[i]);
       }
+      check_tables(p.item);
       items.push_back(p.item);
     }
     // Tiles, most expensive contracts first (LPT order for the queue).
@@ -689,7 +721,8 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
     Moments* d_out = (Moments*)ctx->book_out.reserve(items.size() * 2 * sizeof(Moments));
     int* d_counter = (int*)ctx->book_counter.reserve(256);
     uint32_t* flags = (uint32_t*)ctx->flags.reserve(256);
-    KG_HIP(hipMemcpyAsync(d_blob, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
+    if (!blob.bytes.empty())
+      KG_HIP(hipMemcpyAsync(d_blob, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
     KG_HIP(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(BookItem), hipMemcpyHostToDevice,
                           ctx->stream));
     KG_HIP(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice,

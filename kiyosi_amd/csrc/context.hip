@@ -178,7 +178,8 @@ void kg_context_destroy(kg_context* ctx) {
     ctx->sobol.d_directions.release();
     ctx->sobol.d_shifts.release();
     for (kg::DeviceBuffer* b : {&ctx->book_blob, &ctx->book_items, &ctx->book_tiles, &ctx->book_parts,
-                                &ctx->book_out, &ctx->book_counter})
+                                &ctx->book_out, &ctx->book_counter, &ctx->policy, &ctx->lsmc_tables,
+                                &ctx->mlp_state})
       b->release();
     if (ctx->start) (void)hipEventDestroy(ctx->start);
     if (ctx->stop) (void)hipEventDestroy(ctx->stop);

@@ -81,6 +81,8 @@ struct kg_context {
   std::vector<uint32_t> joe_kuo;  // direction-number records, loaded once
   // Book engine: device buffers and the autotuned tile shape per precision.
   kg::DeviceBuffer book_blob, book_items, book_tiles, book_parts, book_out, book_counter;
+  // American engine: exercise policy rows, model tables and MLP state.
+  kg::DeviceBuffer policy, lsmc_tables, mlp_state;
   int tuned_spt[2] = {0, 0};
   int tuned_wpc[2] = {0, 0};
 };

@@ -1,0 +1,147 @@
+// Shared declarations of the American (LSMC) engine: the device model, the
+// host plan and the MLP regressor entry (lsmc.hip, mlp.hip).
+#pragma once
+
+#include <algorithm>
+#include <vector>
+
+#include "host_common.hpp"
+
+namespace kg {
+
+struct LsmcModel {
+  int32_t lv, steps;
+  double spot, strike, sign, rate, dividend, sigma, time, dt;
+  double drift, diffusion, discount, log_discount;
+  double year0;  // years from the surface's first row to valuation
+  // BSM reference reconstruction per exercise date (spots_at, american.rs:1166-1215)
+  const double* scale;
+  const double* group_cutoff;
+  const int32_t* screen;
+  // local-volatility table (market/local_volatility.rs:61-359)
+  int32_t nt, ns;
+  const double* years;
+  const double* spots;
+  const double* vols;
+};
+
+struct LsmcHost {
+  bool known = false;
+  double known_value = 0.0;
+  LsmcModel model{};
+  Sampler pricing{}, training{};
+  bool oos = false;
+  double immediate = 0.0;
+};
+
+inline LsmcHost lsmc_plan(kg_context* ctx, const kg_settings& settings, const kg_american_terms& o,
+                          const kg_american_model& mk, const kg_lsmc_options& opt) {
+  LsmcHost h;
+  if (o.option_type != 1 && o.option_type != -1) fail(KG_INVALID_INPUT, "option_type must be 1 or -1");
+  if (!(o.strike > 0.0) || !std::isfinite(o.strike)) fail(KG_INVALID_INPUT, "strike must be positive");
+  if (o.step_count < 2 || o.step_count > 10000)
+    fail(KG_INVALID_INPUT, "American Monte Carlo requires 2..=10,000 exercise intervals");
+  if (opt.basis < KG_BASIS_QUADRATIC || opt.basis > KG_BASIS_MLP) fail(KG_INVALID_INPUT, "unknown basis");
+  if (!(mk.spot > 0.0) || !std::isfinite(mk.spot)) fail(KG_INVALID_INPUT, "spot must be positive");
+  if (mk.valuation_time < o.effective_time || mk.valuation_time > o.expiry_time)
+    fail(KG_INVALID_INPUT, "valuation outside contract life");
+  if (settings.sampling == KG_SOBOL)
+    fail(KG_UNSUPPORTED_OPERATION, "the GPU LSMC engine draws pseudorandom normals");
+  const double sign = o.option_type;
+  h.immediate = std::max(sign * (mk.spot - o.strike), 0.0);
+  const double time = (double)(o.expiry_time - mk.valuation_time) / (365.0 * 86400e6);
+  h.pricing = validate_settings(&settings, (uint64_t)o.step_count);
+  if (time == 0.0) {
+    h.known = true;
+    h.known_value = h.immediate;
+    return h;
+  }
+  h.oos = opt.out_of_sample != 0;
+  if (h.oos) {
+    kg_settings t = settings;
+    t.path_count = opt.training_path_count;
+    t.seed = settings.seed ^ 0x9e3779b97f4a7c15ULL;  // american.rs:593
+    h.training = validate_settings(&t, (uint64_t)o.step_count);
+  }
+  const bool lv = mk.surface != nullptr;
+  if (!lv) {
+    if (!(mk.volatility > 0.0) || !std::isfinite(mk.volatility))
+      fail(KG_UNSUPPORTED_OPERATION, "the LSMC engine needs a positive volatility");
+    if (o.option_type > 0 && mk.dividend <= 0.0 && mk.rate >= 0.0)
+      fail(KG_UNSUPPORTED_OPERATION, "calls without dividends are priced by the European engine");
+  }
+  LsmcModel& m = h.model;
+  m.lv = lv;
+  m.steps = o.step_count;
+  m.spot = mk.spot;
+  m.strike = o.strike;
+  m.sign = sign;
+  m.rate = mk.rate;
+  m.dividend = mk.dividend;
+  m.sigma = mk.volatility;
+  m.time = time;
+  m.dt = time / o.step_count;
+  m.drift = (mk.rate - mk.dividend - 0.5 * mk.volatility * mk.volatility) * m.dt;
+  m.diffusion = mk.volatility * std::sqrt(m.dt);
+  m.log_discount = -mk.rate * m.dt;
+  m.discount = std::exp(m.log_discount);
+  std::vector<double> tables;
+  const int S = o.step_count;
+  tables.resize(2 * (size_t)S + (size_t)S);  // scale, cutoff, screen (as int pairs)
+  if (!lv) {
+    const bool normalized =
+        std::fabs(std::log(mk.spot)) + (std::fabs(m.drift) + 10.0 * std::fabs(m.diffusion)) * S < 600.0;
+    if (!normalized) fail(KG_UNSUPPORTED_OPERATION, "anchored spot paths are not supported on the GPU");
+    const double lg = log_ratio(o.strike, mk.spot);
+    const double cutoff = sign * lg - 32.0 * std::numeric_limits<double>::epsilon() * (1.0 + std::fabs(lg));
+    for (int k = 0; k < S; ++k) {
+      const double elapsed = k + 1;
+      const double drifts = m.drift * elapsed;
+      const double gc = cutoff - sign * drifts;
+      tables[k] = mk.spot * std::exp(drifts);
+      tables[S + k] = gc;
+      // Screen flags are packed as int32 in the third block of S doubles.
+      reinterpret_cast<int32_t*>(&tables[2 * (size_t)S])[k] = gc > 0.5 * m.diffusion * std::sqrt(elapsed);
+    }
+  }
+  const kg_local_vol_surface* sf = mk.surface;
+  size_t lv_off = tables.size();
+  if (lv) {
+    if (sf->n_times < 2 || sf->n_spots < 2 || !sf->times || !sf->spots || !sf->values)
+      fail(KG_INVALID_INPUT, "a local-volatility surface needs at least two rows and columns");
+    for (int i = 1; i < sf->n_times; ++i)
+      if (sf->times[i] <= sf->times[i - 1]) fail(KG_INVALID_INPUT, "surface times must increase");
+    for (int i = 0; i < sf->n_spots; ++i)
+      if (!(sf->spots[i] > 0.0) || (i && sf->spots[i] <= sf->spots[i - 1]))
+        fail(KG_INVALID_INPUT, "surface spots must be positive and increasing");
+    for (int i = 0; i < sf->n_times * sf->n_spots; ++i)
+      if (!(sf->values[i] > 0.0) || !std::isfinite(sf->values[i]))
+        fail(KG_INVALID_INPUT, "surface volatilities must be positive and finite");
+    if (mk.valuation_time < sf->times[0] || o.expiry_time > sf->times[sf->n_times - 1])
+      fail(KG_INVALID_INPUT, "time outside local-volatility coverage");
+    m.nt = sf->n_times;
+    m.ns = sf->n_spots;
+    m.year0 = (double)(mk.valuation_time - sf->times[0]) / (365.0 * 86400e6);
+    for (int i = 0; i < sf->n_times; ++i)
+      tables.push_back((double)(sf->times[i] - sf->times[0]) / (365.0 * 86400e6));
+    for (int i = 0; i < sf->n_spots; ++i) tables.push_back(sf->spots[i]);
+    for (int i = 0; i < sf->n_times * sf->n_spots; ++i) tables.push_back(sf->values[i]);
+  }
+  double* d = (double*)ctx->lsmc_tables.reserve(std::max<size_t>(tables.size(), 1) * sizeof(double));
+  KG_HIP(hipMemcpyAsync(d, tables.data(), tables.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  KG_HIP(hipStreamSynchronize(ctx->stream));
+  m.scale = d;
+  m.group_cutoff = d + S;
+  m.screen = (const int32_t*)(d + 2 * S);
+  if (lv) {
+    m.years = d + lv_off;
+    m.spots = m.years + sf->n_times;
+    m.vols = m.spots + sf->n_spots;
+  }
+  return h;
+}
+
+// Fits and prices with the MLP continuation regressor (mlp.hip).
+void american_mlp(kg_context* ctx, LsmcHost& h, const kg_lsmc_options& opt, kg_lsmc_report* out);
+
+}  // namespace kg

@@ -121,6 +121,43 @@ class MonteCarloVanillaEngine(_MonteCarloEngine):
             raise TypeError("MonteCarloVanillaEngine prices EuropeanOption")
         return _native.lib().kg_vanilla_price, instrument._terms(), context.bsm_market()
 
+    def price_with_statistics(self, instrument, context):
+        from .instruments import AmericanOption
+        if isinstance(instrument, AmericanOption):
+            report = self.american_report(instrument, context)
+            raw = _abi.Estimate(price=report.price, has_seed=1, seed=self._resolved().seed,
+                                uncertainty_status=2)
+            return MonteCarloEstimate(raw)
+        return super().price_with_statistics(instrument, context)
+
+    def _resolved(self):
+        return self if self.seed is not None else self.with_resolved_seed()
+
+    def american_report(self, option, context, *, training_path_count=None, basis="quadratic",
+                        epochs=4, batch=4096, learning_rate=1e-3):
+        """Least-squares Monte Carlo on the exercise grid of step_count intervals:
+        in-sample (kiyosi's price for AmericanOption) or, with training_path_count,
+        fitted on separate training paths and priced out of sample. basis is
+        "quadratic" (kiyosi's), "poly4" or "mlp". Returns the raw report."""
+        from .market import american_model
+        engine = self._resolved()
+        ctx = _native.context(engine.device)
+        bases = {"quadratic": _abi.BASIS_QUADRATIC, "poly4": _abi.BASIS_POLY4, "mlp": _abi.BASIS_MLP}
+        options = _abi.LsmcOptions(bases[basis], int(training_path_count is not None),
+                                   int(training_path_count or 0), epochs, batch, learning_rate)
+        report = _abi.LsmcReport()
+        model = american_model(context)
+        ctx.check(_native.lib().kg_american_price(ctx.handle, C.byref(engine._settings()),
+                                                  C.byref(option._american_terms(engine.step_count)),
+                                                  C.byref(model), C.byref(options), C.byref(report)))
+        return report
+
+    def price_american_out_of_sample(self, option, context, *, training_path_count, basis="quadratic"):
+        """Fits the exercise rule on training paths (seed ^ 0x9e3779b97f4a7c15) and
+        prices it on this engine's paths (american.rs:574-637)."""
+        return self.american_report(option, context, training_path_count=training_path_count,
+                                    basis=basis).price
+
 
 class MonteCarloAutocallableEngine(_MonteCarloEngine):
     """Snowball and Phoenix Monte Carlo (engines/monte_carlo/autocallable.rs)."""

@@ -372,3 +372,12 @@ class Accumulator:
             self.daily_quantity, self.acceleration_factor, self.accumulated_quantity,
             clock_micros(self.observation_clock_time), to_micros(self.effective_time),
             to_micros(self.expiry_time))
+
+
+class AmericanOption(EuropeanOption):
+    """American call or put (instruments/vanilla.rs), exercisable on the
+    engine's exercise grid."""
+
+    def _american_terms(self, step_count):
+        return _abi.AmericanTerms(1 if self.option_type == "call" else -1, step_count, self.strike,
+                                  to_micros(self.effective_time), to_micros(self.expiry_time))

@@ -119,3 +119,55 @@ class PricingContext:
             raise UnsupportedOperationError("this engine needs Black-Scholes-Merton parameters")
         return _abi.BsmMarket(self.spot_price, model.risk_free_rate, model.dividend_yield,
                               model.volatility, to_micros(self.valuation_time))
+
+
+class LocalVolatilitySurface:
+    """Bilinear local-volatility table (market/local_volatility.rs:61-359):
+    rows at `times`, columns at `spots`, values[time][spot]."""
+
+    def __init__(self, times, spots, values):
+        import numpy as np
+        self.times = [to_micros(t) for t in times]
+        self.spots = np.ascontiguousarray(spots, dtype=np.float64)
+        self.values = np.ascontiguousarray(values, dtype=np.float64)
+        if len(self.times) < 2 or self.spots.size < 2:
+            raise InvalidInputError("a surface needs at least two times and two spots")
+        if self.values.shape != (len(self.times), self.spots.size):
+            raise InvalidInputError("values must have one row per time and one column per spot")
+
+    def _struct(self):
+        import ctypes as C
+        from . import _abi
+        times = (C.c_int64 * len(self.times))(*self.times)
+        s = _abi.LocalVolSurface(len(self.times), self.spots.size, times,
+                                 self.spots.ctypes.data_as(C.POINTER(C.c_double)),
+                                 self.values.ctypes.data_as(C.POINTER(C.c_double)))
+        s._keep = (times, self.spots, self.values)
+        return s
+
+
+class LocalVolatilityParameters:
+    """Flat rates with a local-volatility surface (market/parameters.rs:217-224)."""
+
+    def __init__(self, *, risk_free_rate, dividend_yield, surface):
+        self.risk_free_rate = _finite("risk_free_rate", risk_free_rate)
+        self.dividend_yield = _finite("dividend_yield", dividend_yield)
+        self.surface = surface
+
+
+def american_model(context):
+    """The C model struct of a BSM or local-volatility pricing context."""
+    import ctypes as C
+    from . import _abi
+    model = context.model_parameters
+    if isinstance(model, BlackScholesMertonParameters):
+        return _abi.AmericanModel(context.spot_price, model.risk_free_rate, model.dividend_yield,
+                                  model.volatility, to_micros(context.valuation_time), None)
+    if isinstance(model, LocalVolatilityParameters):
+        surface = model.surface._struct()
+        m = _abi.AmericanModel(context.spot_price, model.risk_free_rate, model.dividend_yield, 0.0,
+                               to_micros(context.valuation_time), C.pointer(surface))
+        m._keep = surface
+        return m
+    from .errors import UnsupportedOperationError
+    raise UnsupportedOperationError("American Monte Carlo needs BSM or local-volatility parameters")

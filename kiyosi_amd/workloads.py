@@ -128,3 +128,34 @@ def book_4096(n=4096, seed=BOOK_SEED, path_scale=1.0):
             engine = MonteCarloAccumulatorEngine(**kw)
         entries.append((engine, instrument, context))
     return entries
+
+
+def config5_surface(times=51, spots=601):
+    """sigma(S, t) = 0.2 + 0.1 exp(-(ln(S/100))^2 / 0.08) (1 + 0.5 t), tabulated
+    for bilinear interpolation on `times` rows over one year from valuation and
+    `spots` log-spaced columns on S in [100 e^-3, 100 e^3]."""
+    import numpy as np
+
+    from .market import LocalVolatilitySurface
+    year = 365 * 86_400_000_000
+    t = np.arange(times) / (times - 1)
+    stamps = [VALUATION + dt.timedelta(microseconds=int(round(year * f))) for f in t]
+    s = 100.0 * np.exp(np.linspace(-3.0, 3.0, spots))
+    values = 0.2 + 0.1 * np.exp(-np.log(s / 100.0) ** 2 / 0.08)[None, :] * (1.0 + 0.5 * t)[:, None]
+    return LocalVolatilitySurface(stamps, s, values)
+
+
+def american_put_local_vol(paths=1 << 20, training=1 << 20, seed=1729):
+    """American put under the local-volatility surface above: S0 = K = 100,
+    r = 5%, q = 0, T = 1y, 50 exercise dates; `training` paths fit the
+    exercise rule and `paths` independent paths price it."""
+    from .engines import MonteCarloVanillaEngine
+    from .instruments import AmericanOption
+    from .market import LocalVolatilityParameters
+    option = AmericanOption(option_type="put", strike=100.0, effective_time=VALUATION,
+                            expiry_time=dt.datetime(2026, 1, 1))
+    context = PricingContext(model_parameters=LocalVolatilityParameters(
+        risk_free_rate=0.05, dividend_yield=0.0, surface=config5_surface()), spot_price=100.0,
+        valuation_time=VALUATION)
+    engine = MonteCarloVanillaEngine(path_count=paths, step_count=50, seed=seed, allow_large=paths > 10_000_000)
+    return option, context, engine, training

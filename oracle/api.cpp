@@ -4,9 +4,11 @@
 #include <cstdio>
 #include <mutex>
 
+#include "american.hpp"
 #include "book.hpp"
 #include "core.hpp"
 #include "heston.hpp"
+#include "lv.hpp"
 #include "plans.hpp"
 
 using namespace oracle;
@@ -183,6 +185,77 @@ int oracle_accumulator_price(const kg_settings* s, const kg_accumulator_terms* o
   return guarded([&] {
     Settings st = to_settings(s, workers);
     write(accumulator_price(st, *o, *m, context()), out);
+  });
+}
+
+// In-sample LSMC price (american_scenarios) and the fitted policy
+// (steps-1 rows of: valid, c0, c1, c2).
+int oracle_american_lsmc(const kg_settings* s, const kg_european_terms* o, const kg_bsm_market* m,
+                         int steps, int workers, double* price, double* policy) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    std::vector<double> pol;
+    *price = american_lsmc(st, *o, *m, steps, context(), &pol);
+    if (policy) std::copy(pol.begin(), pol.end(), policy);
+  });
+}
+
+int oracle_american_out_of_sample(const kg_settings* s, int64_t training_paths,
+                                  const kg_european_terms* o, const kg_bsm_market* m, int steps,
+                                  int workers, double* price) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    *price = american_out_of_sample(st, training_paths, *o, *m, steps, context());
+  });
+}
+
+// Finite-difference American price under a local-volatility surface.
+int oracle_fd_american(const kg_local_vol_surface* surface, int64_t valuation, double spot, double strike,
+                       double rate, double dividend, int64_t expiry, int option_type, int nx, int nt,
+                       double* price) {
+  return guarded([&] {
+    Surface sf = make_surface(*surface);
+    *price = fd_american(sf, years_until(surface->times[0], valuation), spot, strike, rate, dividend,
+                         years_until(valuation, expiry), option_type, nx, nt);
+  });
+}
+
+static LvSetup lv_setup(const kg_american_terms& o, const kg_american_model& m) {
+  if (!m.surface) throw Failure(INVALID_INPUT, "a local-volatility surface is required");
+  LvSetup L;
+  L.surface = make_surface(*m.surface);
+  L.year0 = years_until(m.surface->times[0], m.valuation_time);
+  L.spot = m.spot;
+  L.strike = o.strike;
+  L.sign = o.option_type;
+  L.rate = m.rate;
+  L.dividend = m.dividend;
+  L.steps = o.step_count;
+  L.time = years_until(m.valuation_time, o.expiry_time);
+  L.dt = L.time / o.step_count;
+  L.log_discount = -m.rate * L.dt;
+  L.discount = std::exp(L.log_discount);
+  return L;
+}
+
+// LSMC under local volatility: in-sample price, and the out-of-sample price
+// when training_paths > 0 (training seed = seed ^ 0x9e3779b97f4a7c15).
+int oracle_lv_lsmc(const kg_settings* s, const kg_american_terms* o, const kg_american_model* m, int basis,
+                   int64_t training_paths, int workers, double* in_sample, double* out_of_sample) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    LvSetup L = lv_setup(*o, *m);
+    std::vector<double> pol;
+    if (training_paths > 0) {
+      Settings tr = st;
+      tr.path_count = training_paths;
+      tr.seed = st.seed ^ 0x9e3779b97f4a7c15ULL;
+      *in_sample = lv_lsmc(tr, L, basis, &pol);
+      *out_of_sample = lv_price(st, L, basis, pol);
+    } else {
+      *in_sample = lv_lsmc(st, L, basis, &pol);
+      *out_of_sample = *in_sample;
+    }
   });
 }
 

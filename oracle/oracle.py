@@ -52,6 +52,15 @@ def lib():
                             ("accumulator", _abi.AccumulatorTerms)):
             getattr(handle, f"oracle_{name}_price").argtypes = [
                 P(_abi.Settings), P(terms), P(_abi.BsmMarket), C.c_int, P(_abi.Estimate)]
+        handle.oracle_american_lsmc.argtypes = [P(_abi.Settings), P(_abi.EuropeanTerms), P(_abi.BsmMarket),
+                                                 C.c_int, C.c_int, P(C.c_double), P(C.c_double)]
+        handle.oracle_american_out_of_sample.argtypes = [P(_abi.Settings), C.c_int64, P(_abi.EuropeanTerms),
+                                                         P(_abi.BsmMarket), C.c_int, C.c_int, P(C.c_double)]
+        handle.oracle_fd_american.argtypes = [P(_abi.LocalVolSurface), C.c_int64, C.c_double, C.c_double,
+                                              C.c_double, C.c_double, C.c_int64, C.c_int, C.c_int, C.c_int,
+                                              P(C.c_double)]
+        handle.oracle_lv_lsmc.argtypes = [P(_abi.Settings), P(_abi.AmericanTerms), P(_abi.AmericanModel), C.c_int,
+                                          C.c_int64, C.c_int, P(C.c_double), P(C.c_double)]
         handle.oracle_autocallable_plan.argtypes = [
             P(_abi.AutocallTerms), P(_abi.BsmMarket), P(C.c_double), C.c_size_t,
             P(C.c_size_t), P(C.c_int)]
@@ -157,3 +166,35 @@ def accumulator_price(settings, terms, market, workers: int = 1):
     _check(lib().oracle_accumulator_price(C.byref(settings), C.byref(terms), C.byref(market), workers,
                                           C.byref(out)))
     return out
+
+
+def american_lsmc(settings, terms, market, steps, workers: int = 1):
+    """In-sample LSMC price and the fitted policy, shape (steps - 1, 4)."""
+    price = C.c_double()
+    policy = np.zeros((steps - 1, 4))
+    _check(lib().oracle_american_lsmc(C.byref(settings), C.byref(terms), C.byref(market), steps, workers,
+                                      C.byref(price), _ptr(policy, C.c_double)))
+    return price.value, policy
+
+
+def american_out_of_sample(settings, training_paths, terms, market, steps, workers: int = 1):
+    price = C.c_double()
+    _check(lib().oracle_american_out_of_sample(C.byref(settings), training_paths, C.byref(terms), C.byref(market),
+                                               steps, workers, C.byref(price)))
+    return price.value
+
+
+def fd_american(surface, valuation, spot, strike, rate, dividend, expiry, option_type=-1, nx=1000, nt=1000):
+    """Crank-Nicolson American price under a local-volatility surface (kg LocalVolSurface)."""
+    price = C.c_double()
+    _check(lib().oracle_fd_american(C.byref(surface), valuation, spot, strike, rate, dividend, expiry,
+                                    option_type, nx, nt, C.byref(price)))
+    return price.value
+
+
+def lv_lsmc(settings, terms, model, basis, training_paths=0, workers: int = 1):
+    """(in-sample, out-of-sample) LSMC prices under local volatility."""
+    a, b = C.c_double(), C.c_double()
+    _check(lib().oracle_lv_lsmc(C.byref(settings), C.byref(terms), C.byref(model), basis, training_paths,
+                                workers, C.byref(a), C.byref(b)))
+    return a.value, b.value

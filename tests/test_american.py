@@ -1,0 +1,155 @@
+"""Least-squares Monte Carlo: the oracle pinned by the reference's per-seed
+LSMC prices and its finite-difference references (CPU); the GPU engine
+checked against the oracle and the FD judge."""
+
+import ctypes as C
+import datetime as dt
+
+import pytest
+
+from conftest import END, START, golden, us
+from kiyosi_amd import _abi, _native, workloads
+from kiyosi_amd.market import LocalVolatilitySurface, american_model
+
+
+def bsm(spot=100.0, rate=0.05, dividend=0.02, sigma=0.2):
+    return _abi.BsmMarket(spot, rate, dividend, sigma, us(START))
+
+
+def put(strike=100.0):
+    return _abi.EuropeanTerms(-1, 0, strike, us(START), us(END))
+
+
+def settings(paths, seed=1009, antithetic=False, precision=_abi.FP64, large=False):
+    return _abi.Settings(paths, seed, 1, _abi.PSEUDO_RANDOM, int(antithetic), precision, 0, int(large))
+
+
+def fd_richardson(oracle, surface, spot=100.0, strike=100.0, rate=0.05, dividend=0.02, n=2000):
+    coarse = oracle.fd_american(surface, us(START), spot, strike, rate, dividend, us(END), -1, n // 2, n // 2)
+    fine = oracle.fd_american(surface, us(START), spot, strike, rate, dividend, us(END), -1, n, n)
+    return fine + (fine - coarse) / 3
+
+
+def test_lsmc_calibration_vectors(oracle):
+    """Per-seed in-sample LSMC prices of kiyosi's American engine."""
+    for run in golden("calibration_american.json")["runs"]:
+        p = run["profile"]
+        for sample in run["samples"][:4] + [run["timing_seed"]]:
+            price, _ = oracle.american_lsmc(settings(p["path_count"], sample["seed"]), put(), bsm(), p["step_count"],
+                                            workers=8)
+            assert price == pytest.approx(sample["price"], rel=1e-13), (run["dimension"], sample)
+
+
+def test_fd_judge_matches_the_reference_fd_prices(oracle):
+    refs = {c["id"]: c["value"] for k in ("fd_local_vol", "fd_bsm") for c in golden("references.json")[k]}
+    two = LocalVolatilitySurface([START, END], [80.0, 120.0], [[0.30, 0.15], [0.35, 0.20]])._struct()
+    assert fd_richardson(oracle, two) == pytest.approx(refs["fd-local-vol-american-put"], abs=1e-4)
+    flat = LocalVolatilitySurface([START, END], [80.0, 120.0], [[0.2, 0.2], [0.2, 0.2]])._struct()
+    assert fd_richardson(oracle, flat) == pytest.approx(refs["fd-bsm-american-put"], abs=1e-4)
+
+
+def test_local_vol_lsmc_with_a_flat_surface_is_the_reference_lsmc(oracle):
+    """Log-Euler steps under a constant surface are the reference's exact BSM
+    steps, so the local-vol LSMC reproduces kiyosi's quadratic LSMC."""
+    flat = LocalVolatilitySurface([START, END], [80.0, 120.0], [[0.2, 0.2], [0.2, 0.2]])
+    from kiyosi_amd.market import LocalVolatilityParameters, PricingContext
+    ctx = PricingContext(model_parameters=LocalVolatilityParameters(risk_free_rate=0.05, dividend_yield=0.02,
+                                                                    surface=flat),
+                         spot_price=100.0, valuation_time=START)
+    terms = _abi.AmericanTerms(-1, 50, 100.0, us(START), us(END))
+    lv, _ = oracle.lv_lsmc(settings(65536), terms, american_model(ctx), _abi.BASIS_QUADRATIC, workers=8)
+    ref = golden("calibration_american.json")["runs"][0]["samples"][0]
+    assert lv == pytest.approx(ref["price"], rel=1e-9)
+
+
+def test_out_of_sample_is_below_in_sample_and_near_fd(oracle):
+    s = settings(131072, seed=3)
+    ins, _ = oracle.american_lsmc(s, put(), bsm(), 50, workers=8)
+    oos = oracle.american_out_of_sample(s, 131072, put(), bsm(), 50, workers=8)
+    assert oos < ins + 0.02
+    assert abs(oos - 6.660684545893456) < 0.06
+
+
+# ---------------------------------------------------------------- GPU
+def gpu_american(ctx, s, terms, model, basis=_abi.BASIS_QUADRATIC, training=0):
+    options = _abi.LsmcOptions(basis, int(training > 0), training, 4, 4096, 1e-3)
+    report = _abi.LsmcReport()
+    ctx.check(_native.lib().kg_american_price(ctx.handle, C.byref(s), C.byref(terms), C.byref(model),
+                                              C.byref(options), C.byref(report)))
+    return report
+
+
+def bsm_model(sigma=0.2):
+    return _abi.AmericanModel(100.0, 0.05, 0.02, sigma, us(START), None)
+
+
+def am_terms(steps=50):
+    return _abi.AmericanTerms(-1, steps, 100.0, us(START), us(END))
+
+
+@pytest.mark.gpu
+def test_gpu_quadratic_lsmc_reproduces_the_calibration_vectors(gpu):
+    for run in golden("calibration_american.json")["runs"]:
+        p = run["profile"]
+        for sample in run["samples"][:2]:
+            got = gpu_american(gpu, settings(p["path_count"], sample["seed"]), am_terms(p["step_count"]), bsm_model())
+            assert got.price == pytest.approx(sample["price"], rel=1e-9), (run["dimension"], sample)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("antithetic", [False, True])
+def test_gpu_quadratic_lsmc_matches_the_oracle(gpu, oracle, antithetic):
+    s = settings(40_001, seed=11, antithetic=antithetic)
+    got = gpu_american(gpu, s, am_terms(), bsm_model())
+    ref, _ = oracle.american_lsmc(s, put(), bsm(), 50, workers=8)
+    assert got.price == pytest.approx(ref, rel=1e-9)
+
+
+@pytest.mark.gpu
+def test_gpu_out_of_sample_matches_the_oracle(gpu, oracle):
+    s = settings(30_000, seed=5)
+    got = gpu_american(gpu, s, am_terms(), bsm_model(), training=50_000)
+    ref = oracle.american_out_of_sample(s, 50_000, put(), bsm(), 50, workers=8)
+    assert got.price == pytest.approx(ref, rel=1e-9)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("basis", [_abi.BASIS_QUADRATIC, _abi.BASIS_POLY4])
+def test_gpu_local_vol_lsmc_matches_the_oracle(gpu, oracle, basis):
+    option, context, engine, _ = workloads.american_put_local_vol()
+    model = american_model(context)
+    terms = option._american_terms(50)
+    s = settings(20_000, seed=7, antithetic=True)
+    got = gpu_american(gpu, s, terms, model, basis, training=30_000)
+    ins, oos = oracle.lv_lsmc(s, terms, model, basis, 30_000, workers=8)
+    assert got.in_sample_price == pytest.approx(ins, rel=1e-8)
+    assert got.price == pytest.approx(oos, rel=1e-8)
+
+
+@pytest.mark.gpu
+def test_gpu_fp32_lsmc_within_half_a_standard_error(gpu):
+    option, context, engine, _ = workloads.american_put_local_vol()
+    model = american_model(context)
+    terms = option._american_terms(50)
+    plain = gpu_american(gpu, settings(1 << 18, seed=9), terms, model, _abi.BASIS_POLY4, training=1 << 18)
+    fast = gpu_american(gpu, settings(1 << 18, seed=9, precision=_abi.FP32_STEP), terms, model, _abi.BASIS_POLY4,
+                        training=1 << 18)
+    assert abs(fast.price - plain.price) <= 0.5 * plain.standard_error
+
+
+@pytest.mark.gpu
+def test_config5_poly4_lower_bound_near_the_fd_judge(gpu, oracle):
+    """2^20 training + 2^20 pricing paths: the poly-4 out-of-sample price is a
+    lower bound of the 50-date Bermudan value, within 1 SE plus the
+    Bermudan/Euler gap (0.02) of the FD American price."""
+    option, context, engine, training = workloads.american_put_local_vol()
+    model = american_model(context)
+    terms = option._american_terms(50)
+    rep = gpu_american(gpu, settings(1 << 20, seed=1729, precision=_abi.FP32_STEP), terms, model, _abi.BASIS_POLY4,
+                       training=training)
+    quad = gpu_american(gpu, settings(1 << 20, seed=1729, precision=_abi.FP32_STEP), terms, model,
+                        _abi.BASIS_QUADRATIC, training=training)
+    fd = fd_richardson(oracle, context.model_parameters.surface._struct(), dividend=0.0)
+    assert rep.price <= fd + 3 * rep.standard_error
+    assert abs(rep.price - fd) < rep.standard_error + 0.02
+    assert rep.price >= quad.price - 2 * rep.standard_error
