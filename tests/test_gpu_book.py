@@ -37,7 +37,7 @@ def barrier_terms(option_type, barrier_type, level, strike=100.0, rebate=0.0, ti
     return t
 
 
-def gpu(ctx, fn, s, terms, m):
+def price_one(ctx, fn, s, terms, m):
     out = _abi.Estimate()
     ctx.check(getattr(_native.lib(), fn)(ctx.handle, C.byref(s), C.byref(terms), C.byref(m), C.byref(out)))
     return out
@@ -56,7 +56,7 @@ BARRIERS = [
 @pytest.mark.parametrize("name,make,paths,antithetic", BARRIERS, ids=[b[0] for b in BARRIERS])
 def test_barrier_fp64_matches_the_oracle(gpu, oracle, name, make, paths, antithetic):
     terms, s = make(), settings(paths, antithetic=antithetic)
-    got = gpu(gpu, "kg_barrier_price", s, terms, market())
+    got = price_one(gpu, "kg_barrier_price", s, terms, market())
     ref = oracle.barrier_price(s, terms, market(), workers=8)
     assert got.price == pytest.approx(ref.price, rel=1e-10, abs=1e-13)
     assert got.uncertainty_status == ref.uncertainty_status
@@ -65,7 +65,7 @@ def test_barrier_fp64_matches_the_oracle(gpu, oracle, name, make, paths, antithe
 
 
 def test_barrier_calibration_vector_on_the_gpu(gpu):
-    got = gpu(gpu, "kg_barrier_price", settings(65536, seed=1009), barrier_terms(1, _abi.DOWN_AND_OUT, 90.0),
+    got = price_one(gpu, "kg_barrier_price", settings(65536, seed=1009), barrier_terms(1, _abi.DOWN_AND_OUT, 90.0),
               market())
     assert got.price == pytest.approx(7.616185456813128, rel=1e-10)
 
@@ -79,7 +79,7 @@ def test_asian_fp64_matches_the_oracle(gpu, oracle, averaging, kind, paths, anti
     t = _abi.AsianTerms(kind, averaging, 64, 0, 100.0, 0.0, us(START), us(START), us(END))
     s = settings(paths, antithetic=antithetic, sampling=sampling)
     m = market(rate=0.04, dividend=0.01)
-    got, ref = gpu(gpu, "kg_asian_price", s, t, m), oracle.asian_price(s, t, m, workers=8)
+    got, ref = price_one(gpu, "kg_asian_price", s, t, m), oracle.asian_price(s, t, m, workers=8)
     assert got.price == pytest.approx(ref.price, rel=1e-10)
     assert got.uncertainty_status == ref.uncertainty_status
 
@@ -90,7 +90,7 @@ def test_accumulator_fp64_matches_the_oracle(gpu, oracle, kind, strike, ko):
                               us(START + dt.timedelta(days=180)))
     s = settings(5000, antithetic=True)
     m = market(sigma=0.25)
-    got, ref = gpu(gpu, "kg_accumulator_price", s, t, m), oracle.accumulator_price(s, t, m, workers=8)
+    got, ref = price_one(gpu, "kg_accumulator_price", s, t, m), oracle.accumulator_price(s, t, m, workers=8)
     assert got.price == pytest.approx(ref.price, rel=1e-10)
     assert got.standard_error == pytest.approx(ref.standard_error, rel=1e-8)
 
