@@ -430,6 +430,23 @@ kg_status kg_sobol_points(kg_context* ctx, uint32_t dims, uint64_t seed, uint64_
 kg_status kg_pcg_normals(kg_context* ctx, uint64_t seed, uint64_t stream, uint64_t first,
                          uint64_t count, double* out);
 
+/* ---- MLP regressor diagnostics (no reference counterpart: they expose the
+ * kernels of KG_BASIS_MLP so their gradients and Adam steps can be checked
+ * against a CPU restatement). `params` holds the 4,481 FP32 parameters
+ * W1[64][3], b1[64], W2[64 out][64 in], b2[64], W3[64], b3; `samples` holds n
+ * rows (u0, u1, u2, y) of FP32.
+ * kg_mlp_gradient: gradient of sum((out - y)^2) / n and that loss.
+ * kg_mlp_train: `steps` Adam steps (beta 0.9/0.999, eps 1e-8, moments from
+ * zero) over consecutive batches of `batch` rows, wrapping to row 0 when a
+ * batch reaches the end; updates params in place.
+ * kg_mlp_forward: the network output of every row. */
+kg_status kg_mlp_gradient(kg_context* ctx, const float* params, const float* samples, int64_t n,
+                          float* gradient, double* loss);
+kg_status kg_mlp_train(kg_context* ctx, float* params, const float* samples, int64_t n, int32_t batch,
+                       int32_t steps, double learning_rate);
+kg_status kg_mlp_forward(kg_context* ctx, const float* params, const float* samples, int64_t n,
+                         float* out);
+
 #ifdef __cplusplus
 }
 #endif

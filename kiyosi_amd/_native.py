@@ -43,6 +43,11 @@ SIGNATURES = {
                                     P(_abi.LsmcOptions), P(_abi.LsmcReport)]),
     "kg_price_book": (C.c_int, [C.c_void_p, P(_abi.BookContract), C.c_size_t, C.c_int32, P(_abi.BookTuning),
                                 P(_abi.Estimate)]),
+    "kg_mlp_gradient": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int64, P(C.c_float),
+                                  P(C.c_double)]),
+    "kg_mlp_train": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int64, C.c_int32, C.c_int32,
+                               C.c_double]),
+    "kg_mlp_forward": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int64, P(C.c_float)]),
 }
 
 

@@ -28,6 +28,14 @@ START = dt.datetime(2025, 1, 1)
 END = dt.datetime(2026, 1, 1)
 
 
+def fd_richardson(oracle, surface, spot=100.0, strike=100.0, rate=0.05, dividend=0.02, n=2000):
+    """American put on START..END by the oracle's Crank-Nicolson/PSOR solver,
+    Richardson-extrapolated over n and n/2 space and time steps."""
+    coarse = oracle.fd_american(surface, us(START), spot, strike, rate, dividend, us(END), -1, n // 2, n // 2)
+    fine = oracle.fd_american(surface, us(START), spot, strike, rate, dividend, us(END), -1, n, n)
+    return fine + (fine - coarse) / 3
+
+
 @pytest.fixture(scope="session")
 def oracle():
     from oracle import oracle as module

@@ -3,11 +3,9 @@ LSMC prices and its finite-difference references (CPU); the GPU engine
 checked against the oracle and the FD judge."""
 
 import ctypes as C
-import datetime as dt
-
 import pytest
 
-from conftest import END, START, golden, us
+from conftest import END, START, fd_richardson, golden, us
 from kiyosi_amd import _abi, _native, workloads
 from kiyosi_amd.market import LocalVolatilitySurface, american_model
 
@@ -22,12 +20,6 @@ def put(strike=100.0):
 
 def settings(paths, seed=1009, antithetic=False, precision=_abi.FP64, large=False):
     return _abi.Settings(paths, seed, 1, _abi.PSEUDO_RANDOM, int(antithetic), precision, 0, int(large))
-
-
-def fd_richardson(oracle, surface, spot=100.0, strike=100.0, rate=0.05, dividend=0.02, n=2000):
-    coarse = oracle.fd_american(surface, us(START), spot, strike, rate, dividend, us(END), -1, n // 2, n // 2)
-    fine = oracle.fd_american(surface, us(START), spot, strike, rate, dividend, us(END), -1, n, n)
-    return fine + (fine - coarse) / 3
 
 
 def test_lsmc_calibration_vectors(oracle):
