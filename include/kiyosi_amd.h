@@ -336,7 +336,7 @@ typedef struct kg_book_contract {
 
 typedef struct kg_book_tuning {
   int32_t samples_per_thread;  /* 1, 2 or 4; 0 = autotune */
-  int32_t workgroups_per_cu;   /* 1..8; 0 = autotune */
+  int32_t workgroups_per_cu;   /* 1..16; 0 = autotune */
   int32_t use_graph;           /* capture and replay the launch as a hipGraph */
   int32_t replays;             /* extra timed replays of the captured graph (benchmarking) */
   double device_ms;            /* out: mean device time of one book launch */
