@@ -1,20 +1,23 @@
 """Throughput of the Monte Carlo engine on one MI355X.
 
-Workload: a snowball under BSM with 252 daily steps, 12 monthly knock-out
-observations stepping down 0.5% from 100%, daily knock-in at 75%, coupon 15%
-p.a., sigma 25%, r 2%, q 0, priced with 2^20 scrambled Sobol paths (seed
-1729). One step = one complete pricing call through the C ABI (plan build,
-sampling kernel, block merge, result copy) with the FP32-stepping /
-FP64-accumulating engine. The plain FP64 engine runs the same paths for the
-speed ratio and the price error, which is stated in units of the path
+Headline workload (config 2): a snowball under BSM with 252 daily steps, 12
+monthly knock-out observations stepping down 0.5% from 100%, daily knock-in
+at 75%, coupon 15% p.a., sigma 25%, r 2%, q 0, priced with 2^20 scrambled
+Sobol paths (seed 1729). One step = one complete pricing call through the C
+ABI (plan build, sampling kernel, block merge, result copy) with the
+FP32-stepping / FP64-accumulating engine. The plain FP64 engine runs the same
+paths for the speed ratio and the price error, stated in units of the path
 standard error.
 
 Prints one JSON line. ``value`` is path-steps per second of the FP32-step
-engine; ``roofline`` compares the sampling kernel's algorithmic FLOP rate
-(9 FLOP per path-step: Sobol inverse normal 3, bridge fill 4, knock-in
-compare 1, event/terminal share 1) with the FP32 vector peak, using the
-kernel duration from HIP events on the engine's stream; ``cpu_baseline``
-times the FP64 CPU restatement of the reference engine on the host cores.
+engine; ``roofline`` compares the sampling kernel's algorithmic FLOP rate (9
+FLOP per path-step: Sobol inverse normal 3, bridge fill 4, knock-in compare
+1, event/terminal share 1) with the FP32 vector peak, using the kernel
+duration from HIP events on the engine's stream; ``cpu_baseline`` times the
+FP64 CPU restatement of the reference engine on the host cores. ``configs``
+adds the other workloads: the European call (config 1), the Heston Asian
+(config 3), the 4,096-contract book (config 4) and the local-volatility
+American put with polynomial and MLP regressors (config 5).
 """
 
 import argparse
@@ -36,38 +39,148 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_engine(engine, note, context, steps, warmup):
+def run_engine(engine, instrument, context, steps, warmup):
     for _ in range(warmup):
-        engine.price_with_statistics(note, context)
+        engine.price_with_statistics(instrument, context)
     device_ms = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        est = engine.price_with_statistics(note, context)
+        est = engine.price_with_statistics(instrument, context)
         device_ms.append(est.device_ms)
     wall = time.perf_counter() - t0
     return est, wall, sum(device_ms) / len(device_ms)
 
 
-def cpu_baseline(note, context, kw, seconds=10.0):
-    """The FP64 CPU restatement (oracle) on the host cores, repeated full
-    pricings until about `seconds` have elapsed."""
+def pair(fp64, fp32, path_steps, reps64, reps32):
+    """Speed and agreement of an FP32-step run against the FP64 plain run of
+    the same paths: (est, wall, kernel_ms) tuples."""
+    (e64, w64, d64), (e32, w32, d32) = fp64, fp32
+    se = e64.path_standard_error
+    return {"fp64_path_steps_per_s": path_steps * reps64 / w64, "fp32_path_steps_per_s": path_steps * reps32 / w32,
+            "speedup": (path_steps * reps32 / w32) / (path_steps * reps64 / w64),
+            "fp64_kernel_ms": d64, "fp32_kernel_ms": d32, "kernel_speedup": d64 / d32 if d32 else None,
+            "price_fp64": e64.price, "price_fp32": e32.price,
+            "price_error_se": abs(e32.price - e64.price) / se if se else None, "path_steps": path_steps}
+
+
+def config1(reps):
+    import kiyosi_amd as k
+    from kiyosi_amd import workloads
+    option, context, kw = workloads.european_call_1m_pcg_antithetic()
+    r64 = run_engine(k.MonteCarloVanillaEngine(precision="fp64", **kw), option, context, reps, 2)
+    r32 = run_engine(k.MonteCarloVanillaEngine(precision="fp32", **kw), option, context, reps, 2)
+    out = pair(r64, r32, r32[0].path_steps, reps, reps)
+    out["workload"] = "European call, 1e6 antithetic PCG paths, 1 step"
+    return out
+
+
+def config3(reps):
+    import ctypes as C
+
+    from kiyosi_amd import _abi, _native, workloads
+    ctx = _native.context(0)
+
+    def run(precision):
+        s, o, m = workloads.heston_asian_252_pcg_2p22(precision)
+        est = _abi.Estimate()
+        ctx.check(_native.lib().kg_heston_price(ctx.handle, C.byref(s), C.byref(o), C.byref(m), C.byref(est)))
+        t0, dev = time.perf_counter(), 0.0
+        for _ in range(reps):
+            ctx.check(_native.lib().kg_heston_price(ctx.handle, C.byref(s), C.byref(o), C.byref(m), C.byref(est)))
+            dev += est.device_ms
+        return est, time.perf_counter() - t0, dev / reps
+
+    r64, r32 = run("fp64"), run("fp32")
+    out = pair(r64, r32, r32[0].path_steps, reps, reps)
+    out["workload"] = "Heston arithmetic Asian call, 2^22 PCG paths x 252 steps"
+    return out
+
+
+def config4(reps):
+    from kiyosi_amd import workloads
+    from kiyosi_amd.engines import Book
+    t0 = time.perf_counter()
+    book = Book(workloads.book_4096())
+    plan_s = time.perf_counter() - t0
+    out = {"workload": "4,096-contract book (barriers, Asians, snowballs, Phoenix, accumulators), one launch",
+           "contracts": len(book.entries), "python_plan_s": plan_s}
+    est = {}
+    for precision in ("fp64", "fp32"):
+        book.price(precision)  # autotune + warm-up
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            est[precision], tuning = book.price(precision, use_graph=True, replays=2)
+        wall = (time.perf_counter() - t0) / reps
+        out[precision] = {"path_steps": tuning.path_steps, "device_ms": tuning.device_ms,
+                          "path_steps_per_s_device": tuning.path_steps / (tuning.device_ms * 1e-3),
+                          "contracts_per_s_device": len(book.entries) / (tuning.device_ms * 1e-3),
+                          "call_s": wall, "contracts_per_s_call": len(book.entries) / wall,
+                          "samples_per_thread": tuning.samples_per_thread,
+                          "workgroups_per_cu": tuning.workgroups_per_cu}
+    errors = [abs(a.price - b.price) / b.path_standard_error for a, b in zip(est["fp32"], est["fp64"])
+              if b.path_standard_error > 0]
+    out["speedup_device"] = out["fp64"]["device_ms"] / out["fp32"]["device_ms"]
+    out["fp32_error_se_max"] = max(errors)
+    out["fp32_within_half_se"] = sum(e <= 0.5 for e in errors) / len(errors)
+    return out
+
+
+def config5():
+    from kiyosi_amd import workloads
+    from kiyosi_amd.engines import MonteCarloVanillaEngine
+    option, context, engine, training = workloads.american_put_local_vol()
+    out = {"workload": "American put, local vol, 50 dates, 2^20 training + 2^20 pricing paths, FP32 step"}
+    eng = MonteCarloVanillaEngine(path_count=1 << 20, step_count=50, seed=1729, precision="fp32")
+    eng.american_report(option, context, training_path_count=training, basis="poly4")  # warm-up
+    for basis in ("quadratic", "poly4", "mlp"):
+        t0 = time.perf_counter()
+        r = eng.american_report(option, context, training_path_count=training, basis=basis)
+        wall = time.perf_counter() - t0
+        out[basis] = {"price": r.price, "standard_error": r.standard_error, "in_sample": r.in_sample_price,
+                      "training_ms": r.training_ms, "pricing_ms": r.pricing_ms, "call_s": wall,
+                      "path_steps": r.path_steps, "path_steps_per_s_call": r.path_steps / wall,
+                      "mean_loss": r.mean_loss if basis == "mlp" else None}
+    return out
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(note, context, kw, seconds=8.0):
+    """The FP64 CPU restatement (oracle) on the host cores: repeated full
+    2^20-path pricings on up to 16 threads, and 2^17-path pricings on one
+    thread, each for about `seconds`."""
     from kiyosi_amd import _abi
     from oracle import oracle
     cores = min(os.cpu_count() or 1, 16)
-    s = _abi.Settings(kw["path_count"], kw["seed"], 1, _abi.SOBOL, 0, 0, 0, 1)
     terms, market = note._terms(), context.bsm_market()
-    oracle.autocallable_price(s, terms, market, workers=cores)  # warm-up
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        oracle.autocallable_price(s, terms, market, workers=cores)
-        reps += 1
-        elapsed = time.perf_counter() - t0
-        if elapsed >= seconds:
-            break
-    path_steps = kw["path_count"] * 252 * reps
-    return {"value": path_steps / elapsed, "unit": "path-steps/s", "cores": cores, "kind": "port",
-            "sample": f"{reps} full pricings of the 2^20-path snowball (FP64 CPU restatement, "
-                      f"{cores} threads, {elapsed:.1f} s)"}
+
+    def rate(paths, workers):
+        s = _abi.Settings(paths, kw["seed"], 1, _abi.SOBOL, 0, 0, 0, 1)
+        oracle.autocallable_price(s, terms, market, workers=workers)  # warm-up
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            oracle.autocallable_price(s, terms, market, workers=workers)
+            reps += 1
+            elapsed = time.perf_counter() - t0
+            if elapsed >= seconds:
+                break
+        return paths * 252 * reps / elapsed, reps, elapsed
+
+    multi, reps, elapsed = rate(kw["path_count"], cores)
+    single, reps1, elapsed1 = rate(1 << 17, 1)
+    return {"value": multi, "unit": "path-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{reps} full pricings of the 2^20-path snowball on {cores} threads ({elapsed:.1f} s) and "
+                      f"{reps1} pricings of a 2^17-path prefix on 1 thread ({elapsed1:.1f} s); FP64 C++ "
+                      f"restatement of kiyosi 0.23.0 MC",
+            "single_core_value": single, "cpu": cpu_model(), "logical_cpus": os.cpu_count()}
 
 
 def main():
@@ -76,6 +189,7 @@ def main():
     parser.add_argument("--steps", type=int, default=20)
     parser.add_argument("--warmup", type=int, default=3)
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    parser.add_argument("--headline-only", action="store_true", help="skip configs 1, 3, 4 and 5")
     args = parser.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -89,15 +203,15 @@ def main():
     note, context, kw = workloads.snowball_252d_sobol_2p20()
     fp32 = k.MonteCarloAutocallableEngine(precision="fp32", **kw)
     fp64 = k.MonteCarloAutocallableEngine(precision="fp64", **kw)
-    log("bench: FP64 plain engine")
-    est64, wall64, dev64 = run_engine(fp64, note, context, max(3, args.steps // 4), 1)
+    log("bench: config 2, FP64 plain engine")
+    steps64 = max(3, args.steps // 4)
+    est64, wall64, dev64 = run_engine(fp64, note, context, steps64, 1)
     log(f"bench: fp64 {wall64:.3f}s, kernel {dev64:.3f} ms")
-    log("bench: FP32-step engine")
+    log("bench: config 2, FP32-step engine")
     est32, wall32, dev32 = run_engine(fp32, note, context, args.steps, args.warmup)
     log(f"bench: fp32 {wall32:.3f}s, kernel {dev32:.3f} ms")
 
     path_steps = est32.path_steps
-    steps64 = max(3, args.steps // 4)
     value = path_steps * args.steps / wall32
     value64 = path_steps * steps64 / wall64
     se = est64.path_standard_error
@@ -128,6 +242,15 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                      "kernel": "autocall_kernel<sobol, fp32>", "kernel_ms": dev32},
     }
+    if not args.headline_only:
+        configs = {}
+        for name, fn in (("c1", lambda: config1(20)), ("c3", lambda: config3(3)), ("c4", lambda: config4(2)),
+                         ("c5", config5)):
+            log(f"bench: {name}")
+            t0 = time.perf_counter()
+            configs[name] = fn()
+            log(f"bench: {name} done in {time.perf_counter() - t0:.1f}s")
+        result["configs"] = configs
     if not args.no_cpu:
         log("bench: CPU baseline")
         result["cpu_baseline"] = cpu_baseline(note, context, kw)

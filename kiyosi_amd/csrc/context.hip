@@ -9,30 +9,31 @@ extern "C" const uint32_t kg_joe_kuo_end[];
 
 namespace kg {
 
-// Merges per-block partials [block][slot] for `slots` independent Moments.
-// Thread t folds a contiguous run of blocks in block order; the 256 runs are
-// then merged as a fixed tree. The result depends only on the partials, not
-// on scheduling, so a seed replays exactly (monte_carlo.rs:1800-1809 merges
-// the same partials sequentially).
-__global__ void __launch_bounds__(256) merge_partials(const Moments* __restrict__ partials,
-                                                      int64_t blocks, int slots,
-                                                      Moments* __restrict__ out) {
-  __shared__ Moments smem[4];
+// Merges per-block partials [block][slot] for `slots` independent Moments,
+// one workgroup per slot. Thread t folds a contiguous run of blocks in block
+// order; the runs are then merged as a fixed tree. The result depends only
+// on the partials, not on scheduling, so a seed replays exactly
+// (monte_carlo.rs:1800-1809 merges the same partials sequentially).
+constexpr int MERGE_THREADS = 1024;
+
+__global__ void __launch_bounds__(MERGE_THREADS) merge_partials(const Moments* __restrict__ partials,
+                                                                int64_t blocks, int slots,
+                                                                Moments* __restrict__ out) {
+  __shared__ Moments smem[MERGE_THREADS / WAVE];
+  const int s = blockIdx.x;
   const int64_t per = (blocks + blockDim.x - 1) / blockDim.x;
   const int64_t begin = per * threadIdx.x;
   const int64_t end = begin + per < blocks ? begin + per : blocks;
-  for (int s = 0; s < slots; ++s) {
-    Moments acc[1] = {moments_zero()};
-    for (int64_t b = begin; b < end; ++b) acc[0] = moments_merge(acc[0], partials[b * slots + s]);
-    block_merge<1>(acc, smem);
-    if (threadIdx.x == 0) out[s] = acc[0];
-  }
+  Moments acc[1] = {moments_zero()};
+  for (int64_t b = begin; b < end; ++b) acc[0] = moments_merge(acc[0], partials[b * slots + s]);
+  block_merge<1>(acc, smem);
+  if (threadIdx.x == 0) out[s] = acc[0];
 }
 
 void merge_and_fetch(kg_context* ctx, const Moments* partials, int64_t blocks, int slots,
                      Moments* host_out) {
   Moments* d_out = (Moments*)ctx->results.reserve(sizeof(Moments) * slots);
-  hipLaunchKernelGGL(merge_partials, dim3(1), dim3(256), 0, ctx->stream, partials, blocks, slots,
+  hipLaunchKernelGGL(merge_partials, dim3(slots), dim3(MERGE_THREADS), 0, ctx->stream, partials, blocks, slots,
                      d_out);
   KG_HIP(hipGetLastError());
   KG_HIP(hipMemcpyAsync(host_out, d_out, sizeof(Moments) * slots, hipMemcpyDeviceToHost,

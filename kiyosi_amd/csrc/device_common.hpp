@@ -18,21 +18,34 @@ constexpr int WAVE = 64;
 // ------------------------------------------------------------------ PCG32
 struct Pcg32 {
   uint64_t state, inc;
+  uint64_t inc2;  // increment of two LCG steps: inc * (MUL + 1)
   static constexpr uint64_t MUL = 6364136223846793005ULL;
+  static constexpr uint64_t MUL2 = MUL * MUL;  // multiplier of two steps (mod 2^64)
   __host__ __device__ inline void init(uint64_t seed, uint64_t stream) {
     state = 0;
     inc = (stream << 1) | 1;
+    inc2 = inc * (MUL + 1);
     step();
     state += seed;
     step();
   }
   __host__ __device__ inline void step() { state = state * MUL + inc; }
-  __host__ __device__ inline uint32_t next() {
-    uint64_t old = state;
-    state = old * MUL + inc;
+  __host__ __device__ static inline uint32_t output(uint64_t old) {
     uint32_t x = (uint32_t)(((old >> 18) ^ old) >> 27);
     uint32_t r = (uint32_t)(old >> 59);
     return (x >> r) | (x << ((32u - r) & 31u));
+  }
+  __host__ __device__ inline uint32_t next() {
+    uint64_t old = state;
+    state = old * MUL + inc;
+    return output(old);
+  }
+  // The next output, then one more output skipped: two LCG steps in one
+  // affine jump (same stream position as two next() calls).
+  __host__ __device__ inline uint32_t next_skip_one() {
+    uint64_t old = state;
+    state = old * MUL2 + inc2;
+    return output(old);
   }
   // O(log n) affine jump of the LCG (monte_carlo.rs:2233-2245).
   __host__ __device__ inline void advance(uint64_t distance) {

@@ -76,12 +76,12 @@ __device__ inline float normal_from_bits(uint32_t bits) {
 __device__ inline float uniform_hi(uint32_t hi) { return fmaf((float)hi, 0x1p-32f, 0x1p-33f); }
 
 // One Box-Muller pair from four PCG outputs, consumed exactly as the FP64
-// form consumes them (two 52-bit uniforms), so both forms stay on one stream.
+// form consumes them (two 52-bit uniforms), so both forms stay on one
+// stream. Only the high word of each uniform is used, so the low words'
+// outputs are skipped by two-step LCG jumps.
 __device__ inline void box_muller_pair(Pcg32& g, float* even, float* odd) {
-  const uint32_t h1 = g.next();
-  (void)g.next();
-  const uint32_t h2 = g.next();
-  (void)g.next();
+  const uint32_t h1 = g.next_skip_one();
+  const uint32_t h2 = g.next_skip_one();
   const float radius = sqrt(-2.0f * log(uniform_hi(h1)));
   const float turns = uniform_hi(h2);
   *even = radius * __builtin_amdgcn_cosf(turns);

@@ -13,14 +13,11 @@
 //
 // Layout: one thread per sample, 256 per workgroup (one sample block); an
 // antithetic pair steps both legs from one draw. FP32 stepping keeps the
-// running average in FP64 and updates the variance in FP64 while it is
-// below V_NEAR_ZERO, where the truncation decides the next step.
+// running average and the variance recursion in FP64.
 #include "fp32_math.hpp"
 #include "host_common.hpp"
 
 namespace kg {
-
-constexpr double V_NEAR_ZERO = 2.5e-4;
 
 struct HestonParams {
   int payoff, steps;
@@ -54,30 +51,23 @@ struct Leg<false> {
 
 template <>
 struct Leg<true> {
-  float x, v;
-  double sum;
+  float x;
+  double v, sum;
   __device__ inline void init(const HestonParams& p) {
     x = 0.f;
-    v = (float)p.v0;
+    v = p.v0;
     sum = 0.0;
   }
+  // The variance recursion stays in FP64 on every step: under xi^2 > 2 kappa
+  // theta nearly every wave holds a lane near zero, so a branch into FP64
+  // only near zero would run both forms anyway. The truncation is decided in
+  // FP64; the square root and the log-spot step run in FP32.
   __device__ inline void step(const HestonParams& p, float z1, float z2) {
-    float sq;
-    if (v < (float)V_NEAR_ZERO) {
-      // Near-zero variance in FP64: truncation and the square root decide
-      // whether the path leaves zero.
-      const double vd = fmax((double)v, 0.0);
-      const double sqd = sqrt(vd) * p.sdt;
-      sq = (float)sqd;
-      const float vp = (float)vd;
-      x += fmaf(sq, fmaf((float)p.rho, z1, (float)p.rho_c * z2), fmaf(-(float)p.hdt, vp, (float)p.a));
-      v = (float)((double)v + fma(p.xi * sqd, (double)z1, fma(-p.kdt, vd, p.kth)));
-    } else {
-      const float vp = v;
-      sq = f32::sqrt(vp) * (float)p.sdt;
-      x += fmaf(sq, fmaf((float)p.rho, z1, (float)p.rho_c * z2), fmaf(-(float)p.hdt, vp, (float)p.a));
-      v += fmaf((float)p.xi * sq, z1, fmaf(-(float)p.kdt, vp, (float)p.kth));
-    }
+    const double vd = fmax(v, 0.0);
+    const float vp = (float)vd;
+    const float sq = f32::sqrt(vp) * (float)p.sdt;
+    x += fmaf(sq, fmaf((float)p.rho, z1, (float)p.rho_c * z2), fmaf(-(float)p.hdt, vp, (float)p.a));
+    v += fma((double)((float)p.xi * sq), (double)z1, fma(-p.kdt, vd, p.kth));
     if (p.payoff == KG_HESTON_ASIAN_ARITHMETIC)
       sum += (double)f32::exp(x);
     else if (p.payoff == KG_HESTON_ASIAN_GEOMETRIC)

@@ -230,19 +230,47 @@ __device__ inline bool solve_poly4_d(const double* e, double* out) {
   return false;
 }
 
-// Sums each entry over the blocks in block order, then solves. Writes
-// policy row `step`: [valid, c0 .. c4].
+// Sums each entry over the blocks, then solves; writes policy row `step`:
+// [valid, c0 .. c4]. The quadratic basis adds the block sums in block order,
+// as the reference does, streaming them through LDS so the sequential adds
+// wait on LDS rather than on global loads; the poly-4 sums use a fixed
+// tree (thread t folds blocks t, t + 256, ..., then waves and threads
+// combine in a fixed order).
+constexpr int SOLVE_CHUNK = 128;
+
 template <int BASIS>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(256)
     lsmc_solve(const double* __restrict__ partials, int64_t blocks, double* __restrict__ pol_row,
                uint32_t* __restrict__ flags) {
   constexpr int E = Entries<BASIS>::N;
   __shared__ double sums[E];
   const int t = threadIdx.x;
-  if (t < E) {
+  if (BASIS == KG_BASIS_QUADRATIC) {
+    __shared__ double buf[SOLVE_CHUNK * E];
     double s = 0.0;
-    for (int64_t b = 0; b < blocks; ++b) s += partials[b * E + t];
-    sums[t] = s;
+    for (int64_t b0 = 0; b0 < blocks; b0 += SOLVE_CHUNK) {
+      const int nb = (int)(blocks - b0 < SOLVE_CHUNK ? blocks - b0 : SOLVE_CHUNK);
+      for (int i = t; i < nb * E; i += blockDim.x) buf[i] = partials[b0 * E + i];
+      __syncthreads();
+      if (t < E)
+        for (int k = 0; k < nb; ++k) s += buf[k * E + t];
+      __syncthreads();
+    }
+    if (t < E) sums[t] = s;
+  } else {
+    __shared__ double red[E * 4];
+    double acc[E];
+    for (int e = 0; e < E; ++e) acc[e] = 0.0;
+    for (int64_t b = t; b < blocks; b += blockDim.x)
+      for (int e = 0; e < E; ++e) acc[e] += partials[b * E + e];
+    const int lane = t & (WAVE - 1), wave = t / WAVE;
+    for (int e = 0; e < E; ++e) {
+      double v = acc[e];
+      for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_down(v, o, WAVE);
+      if (lane == 0) red[e * 4 + wave] = v;
+    }
+    __syncthreads();
+    if (t < E) sums[t] = ((red[t * 4] + red[t * 4 + 1]) + red[t * 4 + 2]) + red[t * 4 + 3];
   }
   __syncthreads();
   if (t != 0) return;
@@ -272,52 +300,62 @@ __global__ void __launch_bounds__(64)
 }
 
 // Out-of-sample valuation (price_american_out_of_sample, american.rs:574-637):
-// each sample streams until the fitted rule exercises; an antithetic pair
-// replays the same normals negated.
+// each sample streams until the fitted rule exercises; the two legs of an
+// antithetic pair step together on the same normals with opposite signs.
+// The step loop is block-uniform so the local-volatility rows can be
+// staged in LDS.
 template <bool FP32, bool LV, int BASIS>
 __global__ void __launch_bounds__(256)
     lsmc_price(LsmcModel m, const double* __restrict__ policy, uint64_t seed, int64_t samples,
                int64_t paths, int antithetic, Moments* __restrict__ parts, uint32_t* __restrict__ flags) {
   __shared__ Moments smem[2 * 4];
+  __shared__ double lv_lds[LV ? 3 * LV_LDS_NODES : 1];
   const int64_t block = blockIdx.x;
   const int j = threadIdx.x;
   const int64_t n = block * BLOCK_SAMPLES + j;
   const bool live = n < samples;
   const bool paired = antithetic && live && n < paths / 2;
   uint32_t bad = 0;
-  Moments mm[2] = {moments_zero(), moments_zero()};
-  double values[2] = {0.0, 0.0};
-  for (int leg = 0; leg < (paired ? 2 : 1) && live; ++leg) {
-    Normals<FP32> z;
-    z.start(seed, (uint64_t)block, (uint64_t)j * (uint64_t)m.steps);
-    const double sg = leg ? -1.0 : 1.0;
-    double current = m.spot, x = 0.0, value = NAN;
-    for (int step = 0; step < m.steps; ++step) {
-      const double e = sg * z.next();
+  Normals<FP32> z;
+  if (live) z.start(seed, (uint64_t)block, (uint64_t)j * (uint64_t)m.steps);
+  LvRows rows{};
+  rows.row = -1;
+  double current[2] = {m.spot, m.spot}, x[2] = {0.0, 0.0}, values[2] = {0.0, 0.0};
+  int bracket[2] = {-1, -1};
+  bool alive[2] = {live, paired};
+  for (int step = 0; step < m.steps; ++step) {
+    if (LV) lv_stage(m, step, lv_lds, rows);  // block-uniform
+    if (!alive[0] && !alive[1]) continue;
+    const double e = z.next();
+    for (int leg = 0; leg < 2; ++leg) {
+      if (!alive[leg]) continue;
+      const double d = leg ? -e : e;
       if (LV) {
-        x = lv_step<FP32>(m, step, x, e);
-        current = m.spot * (FP32 ? (double)f32::exp((float)x) : exp(x));
+        x[leg] = lv_step<FP32>(m, rows, x[leg], d, bracket[leg]);
+        current[leg] = m.spot * (FP32 ? (double)f32::exp((float)x[leg]) : exp(x[leg]));
       } else if (FP32) {
-        current = (double)((float)current * f32::exp(fmaf((float)m.diffusion, (float)e, (float)m.drift)));
+        current[leg] =
+            (double)((float)current[leg] * f32::exp(fmaf((float)m.diffusion, (float)d, (float)m.drift)));
       } else {
-        current *= exp(fma(m.diffusion, e, m.drift));
+        current[leg] *= exp(fma(m.diffusion, d, m.drift));
       }
-      if (!isfinite(current) || current <= 0.0) bad = 1;
-      if (step == m.steps - 1) break;
-      const double intrinsic = payoff(m, current);
+      if (!isfinite(current[leg]) || current[leg] <= 0.0) bad = 1;
+      if (step == m.steps - 1) continue;
+      const double intrinsic = payoff(m, current[leg]);
       const double* pol = policy + (size_t)step * 8;
       if (intrinsic > 0.0 && pol[0] != 0.0) {
-        const double cont = continuation(pol, BASIS, current / m.strike);
+        const double cont = continuation(pol, BASIS, current[leg] / m.strike);
         if (!isfinite(cont)) bad = 1;
         if (intrinsic / m.strike > cont) {
-          value = discounted_product(1.0, intrinsic, -m.rate * (m.dt * (double)(step + 1)));
-          break;
+          values[leg] = discounted_product(1.0, intrinsic, -m.rate * (m.dt * (double)(step + 1)));
+          alive[leg] = false;
         }
       }
     }
-    if (isnan(value)) value = discounted_product(1.0, payoff(m, current), -m.rate * m.time);
-    values[leg] = value;
   }
+  for (int leg = 0; leg < 2; ++leg)
+    if (alive[leg]) values[leg] = discounted_product(1.0, payoff(m, current[leg]), -m.rate * m.time);
+  Moments mm[2] = {moments_zero(), moments_zero()};
   if (live) {
     moments_add(mm[0], values[0], true);
     if (paired) {
@@ -361,7 +399,7 @@ static double run_lsmc(kg_context* ctx, const LsmcModel& m, const Sampler& s, do
                        d_policy + (size_t)(step + 1) * 8, have, samples, s.path_count, (int)s.antithetic,
                        coords, spot, cash, partials, flags);
     KG_HIP(hipGetLastError());
-    hipLaunchKernelGGL((lsmc_solve<BASIS>), dim3(1), dim3(64), 0, ctx->stream, partials, blocks,
+    hipLaunchKernelGGL((lsmc_solve<BASIS>), dim3(1), dim3(256), 0, ctx->stream, partials, blocks,
                        d_policy + (size_t)step * 8, flags);
     KG_HIP(hipGetLastError());
     have = 1;  // the kernels read the valid flag of the row they use

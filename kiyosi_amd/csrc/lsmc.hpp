@@ -23,6 +23,10 @@ struct LsmcModel {
   const double* years;
   const double* spots;
   const double* vols;
+  // Spot nodes equally spaced in log (to 1e-12): bracket index
+  // floor((ln S - log_s0) inv_h), then corrected against the nodes.
+  int32_t log_uniform;
+  double log_s0, inv_h, log_spot;
 };
 
 struct LsmcHost {
@@ -121,6 +125,14 @@ inline LsmcHost lsmc_plan(kg_context* ctx, const kg_settings& settings, const kg
       fail(KG_INVALID_INPUT, "time outside local-volatility coverage");
     m.nt = sf->n_times;
     m.ns = sf->n_spots;
+    m.log_spot = std::log(mk.spot);
+    m.log_s0 = std::log(sf->spots[0]);
+    const double h = std::log(sf->spots[sf->n_spots - 1] / sf->spots[0]) / (sf->n_spots - 1);
+    bool uniform = h > 0.0;
+    for (int i = 1; i < sf->n_spots && uniform; ++i)
+      uniform = std::fabs(std::log(sf->spots[i] / sf->spots[i - 1]) - h) <= 1e-12 * h + 1e-15;
+    m.log_uniform = uniform;
+    m.inv_h = uniform ? 1.0 / h : 0.0;
     m.year0 = (double)(mk.valuation_time - sf->times[0]) / (365.0 * 86400e6);
     for (int i = 0; i < sf->n_times; ++i)
       tables.push_back((double)(sf->times[i] - sf->times[0]) / (365.0 * 86400e6));

@@ -36,6 +36,64 @@ __device__ inline double lv_sigma(const LsmcModel& m, double year, double spot) 
   return lv_interp(lv_interp(v0[c], v0[c + 1], ws), lv_interp(v1[c], v1[c + 1], ws), wt);
 }
 
+// The two rows of a local-volatility table that bracket one time, with the
+// spot nodes: in LDS when the table has at most LV_LDS_NODES spot nodes,
+// otherwise in global memory. `row` is the staged row (-1 before the first).
+constexpr int LV_LDS_NODES = 1024;
+
+struct LvRows {
+  const double* spots;
+  const double* row0;
+  const double* row1;
+  double wt;
+  int row;
+};
+
+// Stages the rows of time index i (year0 + dt i). Every thread of the block
+// calls it with the same i; it synchronises the block only when the row
+// changes.
+__device__ inline void lv_stage(const LsmcModel& m, int i, double* lds, LvRows& t) {
+  double wt;
+  const int r = lv_bracket(m.years, m.nt, m.year0 + m.dt * i, &wt);
+  t.wt = wt;
+  if (m.ns > LV_LDS_NODES) {
+    t.spots = m.spots;
+    t.row0 = m.vols + (size_t)r * m.ns;
+    t.row1 = t.row0 + m.ns;
+    t.row = r;
+    return;
+  }
+  if (t.row == r) return;
+  __syncthreads();
+  if (t.row < 0)
+    for (int k = threadIdx.x; k < m.ns; k += blockDim.x) lds[k] = m.spots[k];
+  for (int k = threadIdx.x; k < 2 * m.ns; k += blockDim.x) lds[m.ns + k] = m.vols[(size_t)r * m.ns + k];
+  __syncthreads();
+  t.spots = lds;
+  t.row0 = lds + m.ns;
+  t.row1 = lds + 2 * m.ns;
+  t.row = r;
+}
+
+// lv_sigma at the staged time for `spot` = S0 exp(log_rel). `c` carries
+// the spot bracket between calls (-1 before the first: a binary search); on
+// a log-uniform grid it starts from the grid index of log_rel. The walk
+// leaves c = clamp(partition_point - 1, 0, ns - 2), the bracket lv_bracket
+// returns.
+__device__ inline double lv_sigma_at(const LsmcModel& m, const LvRows& t, double spot, double log_rel, int& c) {
+  if (m.log_uniform) {
+    const double g = floor((log_rel + m.log_spot - m.log_s0) * m.inv_h);
+    c = (int)fmin(fmax(g, 0.0), (double)(m.ns - 2));
+  } else if (c < 0) {
+    double w;
+    c = lv_bracket(t.spots, m.ns, spot, &w);
+  }
+  while (c > 0 && spot < t.spots[c]) --c;
+  while (c < m.ns - 2 && spot >= t.spots[c + 1]) ++c;
+  const double ws = fmin(fmax((spot - t.spots[c]) / (t.spots[c + 1] - t.spots[c]), 0.0), 1.0);
+  return lv_interp(lv_interp(t.row0[c], t.row0[c + 1], ws), lv_interp(t.row1[c], t.row1[c + 1], ws), t.wt);
+}
+
 // spots_at (american.rs:1166-1215): spot after step+1 intervals.
 __device__ inline double bsm_spot(const LsmcModel& m, int step, double w) {
   const double lg = m.diffusion * w;
@@ -55,17 +113,17 @@ __device__ inline double continuation(const double* pol, int basis, double x) {
   return pol[1] + x * (pol[2] + x * (pol[3] + x * (pol[4] + x * pol[5])));
 }
 
-// One log-Euler step under local volatility from the start-of-step sigma.
+// One log-Euler step of x = ln(S / S0) under local volatility from the
+// start-of-step sigma; `t` holds the rows staged for the step's start.
 template <bool FP32>
-__device__ inline double lv_step(const LsmcModel& m, int k, double x, double z) {
-  const double year = m.year0 + m.dt * k;
+__device__ inline double lv_step(const LsmcModel& m, const LvRows& t, double x, double z, int& c) {
   if (FP32) {
-    const float sig = (float)lv_sigma(m, year, m.spot * (double)f32::exp((float)x));
+    const float sig = (float)lv_sigma_at(m, t, m.spot * (double)f32::exp((float)x), x, c);
     const float dt = (float)m.dt;
     return (double)((float)x + fmaf(sig * f32::sqrt(dt), (float)z,
                                     ((float)(m.rate - m.dividend) - 0.5f * sig * sig) * dt));
   }
-  const double sig = lv_sigma(m, year, m.spot * exp(x));
+  const double sig = lv_sigma_at(m, t, m.spot * exp(x), x, c);
   return x + fma(sig * sqrt(m.dt), z, (m.rate - m.dividend - 0.5 * sig * sig) * m.dt);
 }
 
@@ -106,21 +164,27 @@ template <bool FP32, bool LV>
 __global__ void __launch_bounds__(256)
     lsmc_generate(LsmcModel m, uint64_t seed, int64_t samples, int64_t paths, int antithetic,
                   void* __restrict__ coords, double* __restrict__ spot, double* __restrict__ cash) {
+  __shared__ double lv_lds[LV ? 3 * LV_LDS_NODES : 1];
   const int64_t block = blockIdx.x;
   const int j = threadIdx.x;
   const int64_t n = block * BLOCK_SAMPLES + j;
-  if (n >= samples) return;
-  const bool paired = antithetic && n < paths / 2;
+  const bool live = n < samples;
+  const bool paired = antithetic && live && n < paths / 2;
   const int legs = antithetic ? 2 : 1;
   const int64_t slots = samples * legs;
   Normals<FP32> z;
-  z.start(seed, (uint64_t)block, (uint64_t)j * (uint64_t)m.steps);
+  if (live) z.start(seed, (uint64_t)block, (uint64_t)j * (uint64_t)m.steps);
+  LvRows rows{};
+  rows.row = -1;
+  int cp = -1, cn = -1;  // spot brackets of the two legs
   double w = 0.0, xp = 0.0, xn = 0.0;
   for (int k = 0; k < m.steps; ++k) {
+    if (LV) lv_stage(m, k, lv_lds, rows);  // block-uniform
+    if (!live) continue;
     const double e = z.next();
     if (LV) {
-      xp = lv_step<FP32>(m, k, xp, e);
-      if (paired) xn = lv_step<FP32>(m, k, xn, -e);
+      xp = lv_step<FP32>(m, rows, xp, e, cp);
+      if (paired) xn = lv_step<FP32>(m, rows, xn, -e, cn);
       if (FP32) {
         ((float*)coords)[(size_t)k * slots + n * legs] = (float)xp;
         if (antithetic) ((float*)coords)[(size_t)k * slots + n * legs + 1] = (float)xn;
@@ -136,6 +200,7 @@ __global__ void __launch_bounds__(256)
         ((double*)coords)[(size_t)k * samples + n] = w;
     }
   }
+  if (!live) return;
   for (int leg = 0; leg < (paired ? 2 : 1); ++leg) {
     double s;
     if (LV)

@@ -34,8 +34,9 @@
 //     again hold u(mt, i) for the GELU derivative of layer 1;
 //   * dW2 = dZ2^T . H1 contracts over samples and reads both operands from
 //     a 32-sample LDS transpose.
-// Gradient partials are written per wave and summed in wave order by the
-// update kernel, so training is deterministic.
+// Gradient partials are summed over the waves of a workgroup in wave order
+// and over workgroups in workgroup order by the update kernel, so training
+// is deterministic.
 #include "lsmc_device.hpp"
 
 namespace kg {
@@ -54,7 +55,7 @@ constexpr int PK_W1 = 0, PK_B1 = 192, PK_B2 = 256, PK_W3 = 320, PK_B3 = 384;  //
 constexpr int PACK_BYTES = ((PK_FLOATS + 385 * 4) + 15) & ~15;
 constexpr int PACK_WORDS = PACK_BYTES / 16;
 constexpr int T_ROW = 40;  // padded row of the 32-sample transposes (bf16)
-constexpr int PPW = 2;     // 32-sample pairs per wave in a gradient launch
+constexpr int PPW = 1;     // 32-sample pairs per wave in a gradient launch
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -207,8 +208,9 @@ __device__ inline float sum_columns(float v) {
 }
 
 // Gradient partials of the batch samples[begin, begin + n): one partial of
-// P_STRIDE floats per wave (gradient of sum((out - y)^2) * inv_n, loss in
-// slot P_LOSS). Every wave runs PPW pairs so the block barriers are uniform.
+// P_STRIDE floats per workgroup (gradient of sum((out - y)^2) * inv_n, loss
+// in slot P_LOSS). Every wave runs PPW pairs so the block barriers are
+// uniform.
 __global__ void __launch_bounds__(256)
     mlp_grad(const uint4* __restrict__ packed, const float4* __restrict__ samples, int64_t begin, int n,
              float inv_n, float* __restrict__ parts, uint32_t* __restrict__ flags) {
@@ -256,33 +258,46 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
   }
   if (!finite) raise_flag(flags, FLAG_NONFINITE);
-  float* out = parts + ((int64_t)blockIdx.x * 4 + wave) * P_STRIDE;
-  for (int mt = 0; mt < 4; ++mt)
-    for (int nt = 0; nt < 4; ++nt)
-      for (int i = 0; i < 4; ++i) out[P_W2 + (16 * mt + 4 * g + i) * HID + 16 * nt + c] = a.w2[mt][nt][i];
   for (int r = 0; r < 16; ++r) {
-    const int u = 16 * (r >> 2) + 4 * g + (r & 3);
-    const float w1a = sum_columns(a.w1[r][0]), w1b = sum_columns(a.w1[r][1]), w1c = sum_columns(a.w1[r][2]);
-    const float b1 = sum_columns(a.b1[r]), b2 = sum_columns(a.b2[r]), w3 = sum_columns(a.w3[r]);
-    if (c == 0) {
-      out[P_W1 + 3 * u] = w1a;
-      out[P_W1 + 3 * u + 1] = w1b;
-      out[P_W1 + 3 * u + 2] = w1c;
-      out[P_B1 + u] = b1;
-      out[P_B2 + u] = b2;
-      out[P_W3 + u] = w3;
+    for (int k = 0; k < 3; ++k) a.w1[r][k] = sum_columns(a.w1[r][k]);
+    a.b1[r] = sum_columns(a.b1[r]);
+    a.b2[r] = sum_columns(a.b2[r]);
+    a.w3[r] = sum_columns(a.w3[r]);
+  }
+  a.b3 = sum_columns(a.b3);
+  a.loss = sum_columns(a.loss);
+  // The four waves' gradients are summed in LDS in wave order (the dZ2
+  // transpose is free again), then written as the workgroup's partial.
+  float* red = reinterpret_cast<float*>(&s_dz[0][0]);
+  static_assert(sizeof(s_dz) >= (P_COUNT + 1) * sizeof(float), "reduction buffer");
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
+      auto put = [&](int k, float v) { red[k] = w == 0 ? v : red[k] + v; };
+      for (int mt = 0; mt < 4; ++mt)
+        for (int nt = 0; nt < 4; ++nt)
+          for (int i = 0; i < 4; ++i) put(P_W2 + (16 * mt + 4 * g + i) * HID + 16 * nt + c, a.w2[mt][nt][i]);
+      if (c == 0)
+        for (int r = 0; r < 16; ++r) {
+          const int u = 16 * (r >> 2) + 4 * g + (r & 3);
+          for (int k = 0; k < 3; ++k) put(P_W1 + 3 * u + k, a.w1[r][k]);
+          put(P_B1 + u, a.b1[r]);
+          put(P_B2 + u, a.b2[r]);
+          put(P_W3 + u, a.w3[r]);
+        }
+      if (lane == 0) {
+        put(P_B3, a.b3);
+        put(P_LOSS, a.loss);
+      }
     }
+    __syncthreads();
   }
-  const float b3 = sum_columns(a.b3), loss = sum_columns(a.loss);
-  if (lane == 0) {
-    out[P_B3] = b3;
-    out[P_LOSS] = loss;
-  }
+  float* out = parts + (int64_t)blockIdx.x * P_STRIDE;
+  for (int k = threadIdx.x; k <= P_COUNT; k += blockDim.x) out[k] = red[k];
 }
 
 enum : int { UPDATE_ADAM = 0, UPDATE_GRADIENT = 1, UPDATE_PACK = 2, UPDATE_ADOPT = 3 };
 
-// Sums the partials of each parameter in wave order, then (ADAM) takes one
+// Sums the partials of each parameter in workgroup order, then (ADAM) takes one
 // Adam step and refreshes the packed compute copy, (GRADIENT) writes the
 // summed gradient, (PACK) only packs `params`, or (ADOPT) replaces the
 // parameters by their running average and packs them. With avg_k > 0 an
@@ -297,6 +312,7 @@ __global__ void __launch_bounds__(256)
   if (p > P_COUNT) return;
   double sum = 0.0;
   if (mode == UPDATE_ADAM || mode == UPDATE_GRADIENT)
+#pragma unroll 8
     for (int w = 0; w < n_parts; ++w) sum += (double)parts[(int64_t)w * P_STRIDE + p];
   if (p == P_LOSS) {
     if (loss) *loss += (float)sum;
@@ -505,6 +521,7 @@ __global__ void __launch_bounds__(256)
               uint32_t* __restrict__ flags) {
   __shared__ uint4 s_pack[PACK_WORDS];
   __shared__ Moments smem[2 * 4];
+  __shared__ double lv_lds[LV ? 3 * LV_LDS_NODES : 1];
   const int64_t block = blockIdx.x;
   const int j = threadIdx.x;
   const int lane = j & (WAVE - 1), g = lane >> 4, c = lane & 15;
@@ -514,19 +531,23 @@ __global__ void __launch_bounds__(256)
   Normals<FP32> z;
   z.start(seed, (uint64_t)block, (uint64_t)j * (uint64_t)m.steps);
   double current[2] = {m.spot, m.spot}, x[2] = {0.0, 0.0}, value[2] = {0.0, 0.0};
+  int bracket[2] = {-1, -1};
   bool alive[2] = {live, paired};
   bool bad = false;
+  LvRows rows{};
+  rows.row = -1;
   for (int step = 0; step < m.steps; ++step) {
     const bool fit = step < m.steps - 1 && fitted[step] != 0;
     __syncthreads();
     if (fit) stage(s_pack, nets + (size_t)step * PACK_WORDS);
     __syncthreads();
+    if (LV) lv_stage(m, step, lv_lds, rows);  // block-uniform
     const double e = live ? z.next() : 0.0;
     for (int leg = 0; leg < 2; ++leg) {
       if (!alive[leg]) continue;
       const double d = leg ? -e : e;
       if (LV) {
-        x[leg] = lv_step<FP32>(m, step, x[leg], d);
+        x[leg] = lv_step<FP32>(m, rows, x[leg], d, bracket[leg]);
         current[leg] = m.spot * (FP32 ? (double)f32::exp((float)x[leg]) : exp(x[leg]));
       } else if (FP32) {
         current[leg] = (double)((float)current[leg] * f32::exp(fmaf((float)m.diffusion, (float)d, (float)m.drift)));
@@ -543,11 +564,12 @@ __global__ void __launch_bounds__(256)
     if (!fit) continue;
     const Pack P(s_pack);
     const double t = m.dt * (step + 1);
+    if (LV) lv_stage(m, step + 1, lv_lds, rows);  // the date's rows; also the next step's start
     for (int leg = 0; leg < 2; ++leg) {
       const double intrinsic = payoff(m, current[leg]);
       const bool ask = alive[leg] && intrinsic > 0.0;
       if (__ballot(ask) == 0) continue;
-      const double sigma = LV ? lv_sigma(m, m.year0 + t, current[leg]) : m.sigma;
+      const double sigma = LV ? lv_sigma_at(m, rows, current[leg], x[leg], bracket[leg]) : m.sigma;
       const float u0 = (float)(4.0 * (current[leg] / m.strike - 1.0));
       const float u1 = (float)((t - sc.t_mid) * sc.t_scale);
       const float u2 = (float)((sigma - sc.s_mid) * sc.s_scale);
@@ -613,10 +635,11 @@ struct Net {
   int max_parts;
 };
 
+// Partials (workgroups of four waves) of a gradient launch over n samples.
 static int parts_for(int n) {
   const int pairs = (n + 31) / 32;
   const int waves = (pairs + PPW - 1) / PPW;
-  return ((waves + 3) / 4) * 4;
+  return (waves + 3) / 4;
 }
 
 static Net net_buffers(kg_context* ctx, DeviceBuffer& buf, int max_batch, size_t extra_bytes, void** extra) {
@@ -658,7 +681,7 @@ static int launch_grad(kg_context* ctx, const Net& net, const float4* samples, i
                        uint32_t* flags) {
   const int parts = parts_for(n);
   if (parts > net.max_parts) fail(KG_INVALID_INPUT, "batch larger than the reserved partials");
-  hipLaunchKernelGGL(mlp_grad, dim3(parts / 4), dim3(256), 0, ctx->stream, net.packed, samples, begin, n,
+  hipLaunchKernelGGL(mlp_grad, dim3(parts), dim3(256), 0, ctx->stream, net.packed, samples, begin, n,
                      1.0f / (float)n, net.parts, flags);
   KG_HIP(hipGetLastError());
   return parts;
@@ -694,7 +717,7 @@ static void run(kg_context* ctx, LsmcHost& h, const kg_lsmc_options& opt, kg_lsm
   const LsmcModel& m = h.model;
   const Sampler& tr = h.oos ? h.training : h.pricing;
   const int epochs = opt.epochs > 0 ? opt.epochs : 4;
-  const int batch = opt.batch > 0 ? opt.batch : 4096;
+  const int batch = opt.batch > 0 ? opt.batch : 16384;
   const float lr = opt.learning_rate > 0.0 ? (float)opt.learning_rate : 1e-3f;
   if (epochs > 1000 || batch < 32 || batch > (1 << 20) || !(lr < 1.0f))
     fail(KG_INVALID_INPUT, "MLP options: epochs <= 1000, batch in 32..=1048576, learning rate below 1");

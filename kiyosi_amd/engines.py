@@ -134,7 +134,7 @@ class MonteCarloVanillaEngine(_MonteCarloEngine):
         return self if self.seed is not None else self.with_resolved_seed()
 
     def american_report(self, option, context, *, training_path_count=None, basis="quadratic",
-                        epochs=4, batch=4096, learning_rate=1e-3):
+                        epochs=4, batch=16384, learning_rate=1e-3):
         """Least-squares Monte Carlo on the exercise grid of step_count intervals:
         in-sample (kiyosi's price for AmericanOption) or, with training_path_count,
         fitted on separate training paths and priced out of sample. basis is

@@ -58,6 +58,22 @@ def snowball_252d_sobol_2p20():
     return note, context, dict(path_count=1 << 20, seed=1729, sampling="sobol")
 
 
+def heston_asian_252_pcg_2p22(precision="fp64"):
+    """Arithmetic-average Asian call under Heston: kappa=2, theta=0.04, xi=0.5,
+    rho=-0.7, v0=0.04, S0=K=100, r=3%, q=0, T=1y; 252 equal steps, the average
+    of the 252 post-step spots; 2^22 PCG paths, seed 1729 (beyond the
+    reference's one-billion path-step cap, hence allow_large). Returns the C
+    structs (settings, terms, market)."""
+    from . import _abi
+    from .market import to_micros
+    settings = _abi.Settings(1 << 22, 1729, 1, _abi.PSEUDO_RANDOM, 0,
+                             _abi.FP32_STEP if precision == "fp32" else _abi.FP64, 0, 1)
+    terms = _abi.HestonTerms(_abi.HESTON_ASIAN_ARITHMETIC, 1, 252, 0, 100.0, to_micros(VALUATION),
+                             to_micros(dt.datetime(2026, 1, 1)))
+    market = _abi.HestonMarket(100.0, 0.03, 0.0, 0.04, 2.0, 0.04, 0.5, -0.7, to_micros(VALUATION))
+    return settings, terms, market
+
+
 BOOK_SEED = 20251203
 BOOK_KINDS = ("up_and_out", "down_and_in", "geometric_asian", "arithmetic_asian", "snowball",
               "phoenix", "accumulator")
