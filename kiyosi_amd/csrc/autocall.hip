@@ -147,10 +147,8 @@ extern "C" kg_status kg_autocallable_price(kg_context* ctx, const kg_settings* s
       launch<false, true>(ctx, p, s, dirs, shifts, parts, flags);
     else
       launch<false, false>(ctx, p, s, dirs, shifts, parts, flags);
-    out->device_ms = timing_end(ctx);
-    check_flags(ctx);
     Moments r[2];
-    merge_and_fetch(ctx, parts, blocks, 2, r);
+    finish_sampling(ctx, parts, blocks, 2, r, &out->device_ms);
     out->path_steps = (double)s.path_count * (double)p.n_steps;
     fill_estimate(s, r[0], r[1], 0.0, false, true, out);
   });

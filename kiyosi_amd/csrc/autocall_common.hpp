@@ -611,7 +611,7 @@ inline AcPlanDev upload(kg_context* ctx, AcHostPlan& h) {
   stage(blob, h.events32, &o[10]);
   stage(blob, h.intervals32, &o[11]);
   unsigned char* d = (unsigned char*)ctx->scratch.reserve(blob.size());
-  KG_HIP(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream));
+  upload_async(ctx, d, blob.data(), blob.size());
   AcPlanDev p = h.dev;
   p.steps = (const AcStepD*)(d + o[0]);
   p.step_event = (const int32_t*)(d + o[1]);
@@ -625,7 +625,6 @@ inline AcPlanDev upload(kg_context* ctx, AcHostPlan& h) {
   p.fill32 = (const float*)(d + o[9]);
   p.events32 = (const float*)(d + o[10]);
   p.intervals32 = (const float*)(d + o[11]);
-  KG_HIP(hipStreamSynchronize(ctx->stream));
   return p;
 }
 

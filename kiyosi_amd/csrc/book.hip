@@ -158,9 +158,7 @@ __global__ void __launch_bounds__(256)
   const int64_t begin = per * threadIdx.x;
   const int64_t end = begin + per < blocks ? begin + per : blocks;
   for (int s = 0; s < 2; ++s) {
-    Moments acc[1] = {moments_zero()};
-    for (int64_t b = begin; b < end; ++b)
-      acc[0] = moments_merge(acc[0], partials[(it.first_block + b) * 2 + s]);
+    Moments acc[1] = {fold_partials(moments_zero(), partials + it.first_block * 2 + s, 2, begin, end)};
     block_merge<1>(acc, smem);
     if (threadIdx.x == 0) out[blockIdx.x * 2 + s] = acc[0];
   }

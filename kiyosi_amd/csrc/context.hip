@@ -1,6 +1,7 @@
 // Context lifecycle, the ordered partial merge, flags, timing and the host
 // side of the scrambled Sobol setup.
 #include <algorithm>
+#include <cstring>
 
 #include "host_common.hpp"
 
@@ -24,8 +25,7 @@ __global__ void __launch_bounds__(MERGE_THREADS) merge_partials(const Moments* _
   const int64_t per = (blocks + blockDim.x - 1) / blockDim.x;
   const int64_t begin = per * threadIdx.x;
   const int64_t end = begin + per < blocks ? begin + per : blocks;
-  Moments acc[1] = {moments_zero()};
-  for (int64_t b = begin; b < end; ++b) acc[0] = moments_merge(acc[0], partials[b * slots + s]);
+  Moments acc[1] = {fold_partials(moments_zero(), partials + s, slots, begin, end)};
   block_merge<1>(acc, smem);
   if (threadIdx.x == 0) out[s] = acc[0];
 }
@@ -57,6 +57,60 @@ void check_flags(kg_context* ctx) {
   KG_HIP(hipStreamSynchronize(ctx->stream));
   if (h & FLAG_CANCELLED) fail(KG_INTERRUPTED, "operation interrupted");
   if (h & FLAG_NONFINITE) unstable();
+}
+
+constexpr size_t PINNED_RESULT_BYTES = 4096;
+
+static void reserve_pinned(kg_context* ctx, size_t bytes) {
+  if (bytes <= ctx->pinned_bytes) return;
+  KG_HIP(hipStreamSynchronize(ctx->stream));  // no copy may still read the old buffer
+  if (ctx->pinned) KG_HIP(hipHostFree(ctx->pinned));
+  ctx->pinned = nullptr;
+  ctx->pinned_bytes = 0;
+  const size_t size = std::max<size_t>(bytes, size_t(1) << 20);
+  KG_HIP(hipHostMalloc(&ctx->pinned, size, hipHostMallocDefault));
+  ctx->pinned_bytes = size;
+}
+
+void upload_async(kg_context* ctx, void* dst, const void* src, size_t n) {
+  if (n == 0) return;
+  const size_t at = PINNED_RESULT_BYTES + ((ctx->staging_used + 255) & ~size_t(255));
+  if (at + n > ctx->pinned_bytes) {
+    // Growing the buffer synchronises, so the staged bytes restart at zero.
+    reserve_pinned(ctx, 2 * (PINNED_RESULT_BYTES + n));
+    ctx->staging_used = 0;
+    upload_async(ctx, dst, src, n);
+    return;
+  }
+  unsigned char* stage = (unsigned char*)ctx->pinned + at;
+  std::memcpy(stage, src, n);
+  KG_HIP(hipMemcpyAsync(dst, stage, n, hipMemcpyHostToDevice, ctx->stream));
+  ctx->staging_used = at + n - PINNED_RESULT_BYTES;
+}
+
+void finish_sampling(kg_context* ctx, const Moments* partials, int64_t blocks, int slots, Moments* host_out,
+                     double* device_ms) {
+  KG_HIP(hipEventRecord(ctx->stop, ctx->stream));
+  reserve_pinned(ctx, PINNED_RESULT_BYTES);
+  Moments* d_out = (Moments*)ctx->results.reserve(sizeof(Moments) * slots);
+  hipLaunchKernelGGL(merge_partials, dim3(slots), dim3(MERGE_THREADS), 0, ctx->stream, partials, blocks, slots,
+                     d_out);
+  KG_HIP(hipGetLastError());
+  unsigned char* pin = (unsigned char*)ctx->pinned;
+  const size_t result_bytes = sizeof(Moments) * slots;
+  if (result_bytes + sizeof(uint32_t) > PINNED_RESULT_BYTES) fail(KG_INVALID_INPUT, "too many merged outputs");
+  KG_HIP(hipMemcpyAsync(pin, d_out, result_bytes, hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP(hipMemcpyAsync(pin + result_bytes, ctx->flags.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  KG_HIP(hipStreamSynchronize(ctx->stream));
+  uint32_t h;
+  std::memcpy(&h, pin + result_bytes, sizeof(h));
+  if (h & FLAG_CANCELLED) fail(KG_INTERRUPTED, "operation interrupted");
+  if (h & FLAG_NONFINITE) unstable();
+  std::memcpy(host_out, pin, result_bytes);
+  float ms = 0.f;
+  KG_HIP(hipEventElapsedTime(&ms, ctx->start, ctx->stop));
+  *device_ms = ms;
 }
 
 void timing_begin(kg_context* ctx) { KG_HIP(hipEventRecord(ctx->start, ctx->stream)); }
@@ -182,6 +236,8 @@ void kg_context_destroy(kg_context* ctx) {
                                 &ctx->book_out, &ctx->book_counter, &ctx->policy, &ctx->lsmc_tables,
                                 &ctx->mlp_state})
       b->release();
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    ctx->pinned = nullptr;
     if (ctx->start) (void)hipEventDestroy(ctx->start);
     if (ctx->stop) (void)hipEventDestroy(ctx->stop);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -277,19 +277,37 @@ __device__ inline Moments moments_shfl_xor(const Moments& m, int mask) {
   return r;
 }
 
-// Merges the Moments of all lanes of a wave; the lower lane of each pair is
-// the left operand, so the order is fixed by lane index.
-__device__ inline Moments wave_merge(Moments m) {
+// Merges the Moments of lanes 0..lanes-1 of a wave (lanes a power of two);
+// the lower lane of each pair is the left operand, so the order is fixed by
+// lane index.
+__device__ inline Moments wave_merge(Moments m, int lanes = WAVE) {
   const int lane = threadIdx.x & (WAVE - 1);
-  for (int mask = 1; mask < WAVE; mask <<= 1) {
+  for (int mask = 1; mask < lanes; mask <<= 1) {
     Moments o = moments_shfl_xor(m, mask);
     m = (lane & mask) ? moments_merge(o, m) : moments_merge(m, o);
   }
   return m;
 }
 
+// Folds partials[b * stride] for b in [begin, end) into acc in block order,
+// issuing the loads eight at a time so the merge chain does not wait on each
+// load in turn.
+__device__ inline Moments fold_partials(Moments acc, const Moments* __restrict__ partials, int64_t stride,
+                                        int64_t begin, int64_t end) {
+  constexpr int CHUNK = 8;
+  for (int64_t b0 = begin; b0 < end; b0 += CHUNK) {
+    Moments tmp[CHUNK];
+#pragma unroll
+    for (int k = 0; k < CHUNK; ++k) tmp[k] = b0 + k < end ? partials[(b0 + k) * stride] : moments_zero();
+#pragma unroll
+    for (int k = 0; k < CHUNK; ++k) acc = moments_merge(acc, tmp[k]);
+  }
+  return acc;
+}
+
 // Block-wide merge of N independent Moments sets; returns the result in
-// thread 0. `smem` needs N * (blockDim.x / 64) entries.
+// thread 0. Each wave merges its lanes, then wave 0 merges the waves' results
+// the same way. `smem` needs N * (blockDim.x / 64) entries.
 template <int N>
 __device__ inline void block_merge(Moments (&m)[N], Moments* smem) {
   const int lane = threadIdx.x & (WAVE - 1);
@@ -299,11 +317,84 @@ __device__ inline void block_merge(Moments (&m)[N], Moments* smem) {
   if (lane == 0)
     for (int i = 0; i < N; ++i) smem[i * waves + wave] = m[i];
   __syncthreads();
+  if (wave == 0 && waves > 1) {
+    int lanes = 1;
+    while (lanes < waves) lanes <<= 1;
+    for (int i = 0; i < N; ++i) {
+      Moments acc = lane < waves ? smem[i * waves + lane] : moments_zero();
+      acc = wave_merge(acc, lanes);
+      if (lane == 0) m[i] = acc;
+    }
+  }
+  __syncthreads();
+}
+
+// Moments of the values a block's threads hold (at most K per thread and
+// output; n[i] of them valid), formed from FP64 sums instead of a Welford
+// merge tree: the block's largest magnitude is the scale, the scaled values
+// and their squares are summed in a fixed shuffle order and then in wave
+// order, and count, mean and M2 come out in thread 0. For kernels whose
+// threads hold only a few values, where merge trees would cost more than the
+// sampling. `smem` needs 3 * N * (blockDim.x / 64) doubles.
+template <int N, int K>
+__device__ inline void block_moments(const double (&v)[N][K], const int (&n)[N], Moments (&out)[N],
+                                     double* smem) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = threadIdx.x / WAVE;
+  const int waves = blockDim.x / WAVE;
+  for (int i = 0; i < N; ++i) {
+    double a = 0.0;
+    for (int k = 0; k < K; ++k)
+      if (k < n[i]) a = fmax(a, fabs(v[i][k]));
+    for (int o = WAVE / 2; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o, WAVE));
+    if (lane == 0) smem[i * waves + wave] = a;
+  }
+  __syncthreads();
+  double scale[N];
+  for (int i = 0; i < N; ++i) {
+    double a = 0.0;
+    for (int w = 0; w < waves; ++w) a = fmax(a, smem[i * waves + w]);
+    scale[i] = a;
+  }
+  __syncthreads();
+  for (int i = 0; i < N; ++i) {
+    // Dividing keeps full precision when 1 / scale would be subnormal.
+    const bool divide = scale[i] > 1e300;
+    const double inv = scale[i] > 0.0 ? 1.0 / scale[i] : 0.0;
+    double s1 = 0.0, s2 = 0.0, c = 0.0;
+    for (int k = 0; k < K; ++k)
+      if (k < n[i]) {
+        const double x = divide ? v[i][k] / scale[i] : v[i][k] * inv;
+        s1 += x;
+        s2 = fma(x, x, s2);
+        c += 1.0;
+      }
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+      s1 += __shfl_down(s1, o, WAVE);
+      s2 += __shfl_down(s2, o, WAVE);
+      c += __shfl_down(c, o, WAVE);
+    }
+    if (lane == 0) {
+      smem[3 * (i * waves + wave)] = s1;
+      smem[3 * (i * waves + wave) + 1] = s2;
+      smem[3 * (i * waves + wave) + 2] = c;
+    }
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 0; i < N; ++i) {
-      Moments acc = smem[i * waves];
-      for (int w = 1; w < waves; ++w) acc = moments_merge(acc, smem[i * waves + w]);
-      m[i] = acc;
+      double s1 = 0.0, s2 = 0.0, c = 0.0;
+      for (int w = 0; w < waves; ++w) {
+        s1 += smem[3 * (i * waves + w)];
+        s2 += smem[3 * (i * waves + w) + 1];
+        c += smem[3 * (i * waves + w) + 2];
+      }
+      if (c == 0.0) {
+        out[i] = moments_zero();
+      } else {
+        const double mean = s1 / c;
+        out[i] = Moments{c, scale[i], mean, fmax(s2 - s1 * mean, 0.0)};
+      }
     }
   }
   __syncthreads();

@@ -72,11 +72,13 @@ __global__ void __launch_bounds__(128)
     european_kernel(EuroParams p, uint64_t seed, int64_t samples, int64_t paths, int antithetic,
                     int sobol, const uint64_t* __restrict__ dirs, uint64_t shift,
                     Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
-  __shared__ Moments smem[2 * 2];
+  __shared__ double smem[3 * 2 * 2];
   const int64_t block = blockIdx.x;
   const int j = 2 * threadIdx.x;
   const int64_t n0 = block * BLOCK_SAMPLES + j;
-  Moments m[2] = {moments_zero(), moments_zero()};
+  // Path values (both legs) and pair means of the thread's two samples.
+  double vals[2][4];
+  int count[2] = {0, 0};
   uint32_t bad = 0;
   if (n0 < samples) {
     double z[2];
@@ -102,16 +104,17 @@ __global__ void __launch_bounds__(128)
       const int64_t n = n0 + s;
       if (n >= samples) break;
       double pos = FP32 ? euro_value_fp32(p, zf[s], bad) : euro_value(p, z[s], bad);
-      moments_add(m[0], pos, true);
+      vals[0][count[0]++] = pos;
       if (antithetic && n < paths / 2) {
         double neg = FP32 ? euro_value_fp32(p, -zf[s], bad) : euro_value(p, -z[s], bad);
-        moments_add(m[0], neg, true);
-        moments_add(m[1], midpoint(pos, neg), true);
+        vals[0][count[0]++] = neg;
+        vals[1][count[1]++] = midpoint(pos, neg);
       }
     }
   }
   if (bad) raise_flag(flags, FLAG_NONFINITE);
-  block_merge<2>(m, smem);
+  Moments m[2];
+  block_moments<2, 4>(vals, count, m, smem);
   if (threadIdx.x == 0) {
     partials[block * 2 + 0] = m[0];
     partials[block * 2 + 1] = m[1];
@@ -141,9 +144,9 @@ static void european_pass(kg_context* ctx, const Sampler& s, const EuroParams& p
                        s.samples(), s.path_count, (int)s.antithetic, (int)s.sobol, dirs, shift,
                        parts, flags);
   KG_HIP(hipGetLastError());
-  *device_ms += timing_end(ctx);
-  check_flags(ctx);
-  merge_and_fetch(ctx, parts, blocks, 2, out);
+  double ms = 0.0;
+  finish_sampling(ctx, parts, blocks, 2, out, &ms);
+  *device_ms += ms;
 }
 
 }  // namespace kg

@@ -199,10 +199,8 @@ extern "C" kg_status kg_heston_price(kg_context* ctx, const kg_settings* setting
       hipLaunchKernelGGL(heston_kernel<false>, dim3(blocks), dim3(256), 0, ctx->stream, p, s.seed,
                          s.samples(), s.path_count, (int)s.antithetic, parts, flags);
     KG_HIP(hipGetLastError());
-    out->device_ms = timing_end(ctx);
-    check_flags(ctx);
     Moments r[2];
-    merge_and_fetch(ctx, parts, blocks, 2, r);
+    finish_sampling(ctx, parts, blocks, 2, r, &out->device_ms);
     out->path_steps = (double)s.path_count * (double)o.step_count;
     fill_estimate(s, r[0], r[1], 0.0, false, true, out);
   });

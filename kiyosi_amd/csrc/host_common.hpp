@@ -85,6 +85,11 @@ struct kg_context {
   kg::DeviceBuffer policy, lsmc_tables, mlp_state;
   int tuned_spt[2] = {0, 0};
   int tuned_wpc[2] = {0, 0};
+  // Pinned host memory (hipHostMalloc): results and flags of a call in the
+  // first PINNED_RESULT_BYTES, then a staging area for asynchronous uploads
+  // whose bump offset restarts with every call.
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0, staging_used = 0;
 };
 
 namespace kg {
@@ -97,17 +102,21 @@ kg_status guarded(kg_context* ctx, F&& body) {
   std::lock_guard<std::mutex> lock(ctx->mu);
   try {
     KG_HIP(hipSetDevice(ctx->device));
+    ctx->staging_used = 0;  // the previous call ended with a synchronisation
     body();
     ctx->error.clear();
     return KG_OK;
   } catch (const Error& e) {
     ctx->error = e.what();
+    (void)hipStreamSynchronize(ctx->stream);  // nothing queued may outlive the call
     return e.status;
   } catch (const std::bad_alloc&) {
     ctx->error = "host allocation failed";
+    (void)hipStreamSynchronize(ctx->stream);
     return KG_COMPUTATION_FAILURE;
   } catch (const std::exception& e) {
     ctx->error = e.what();
+    (void)hipStreamSynchronize(ctx->stream);
     return KG_DEVICE_ERROR;
   }
 }
@@ -263,5 +272,14 @@ void sobol_tables(kg_context* ctx, uint32_t dims, uint64_t seed, std::vector<uin
 // Event timing on the context stream.
 void timing_begin(kg_context* ctx);
 double timing_end(kg_context* ctx);
+// Copies n host bytes to device memory through the pinned staging area:
+// asynchronous on the context's stream, and `src` may be released at once.
+void upload_async(kg_context* ctx, void* dst, const void* src, size_t n);
+// Ends a sampling launch with one synchronisation: records the stop event
+// (call right after the sampling kernel), merges the partials, fetches the
+// merged Moments and the device flags, raises on a flag and returns the
+// sampling kernel's time since timing_begin.
+void finish_sampling(kg_context* ctx, const Moments* partials, int64_t blocks, int slots, Moments* host_out,
+                     double* device_ms);
 
 }  // namespace kg
