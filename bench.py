@@ -33,6 +33,10 @@ sys.path.insert(0, str(ROOT))
 FLOP_PER_PATH_STEP = 9.0
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the config-2 FP32 sampling kernel: FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0 from the two counter passes of `bench.py --profile-pass`,
+# scaled on a known 256 MiB read/write (profiles/hbm_traffic_fetch_write_counters.md).
+C2_KERNEL_HBM_BYTES = 961_200
 
 
 def log(msg):
@@ -183,6 +187,31 @@ def cpu_baseline(note, context, kw, seconds=8.0):
             "single_core_value": single, "cpu": cpu_model(), "logical_cpus": os.cpu_count()}
 
 
+def profile_pass():
+    """A short run for hardware-counter passes: a torch elementwise multiply
+    that reads and writes 256 MiB each (the calibration of FETCH_SIZE and
+    WRITE_SIZE), three config-2 FP32 pricings and one config-5 poly-4 LSMC."""
+    import torch
+
+    import kiyosi_amd as k
+    from kiyosi_amd import workloads
+    n = 64 << 20
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    log(f"profile pass: calibration multiply of {n * 4} bytes read and written per launch")
+    note, context, kw = workloads.snowball_252d_sobol_2p20()
+    engine = k.MonteCarloAutocallableEngine(precision="fp32", **kw)
+    for _ in range(3):
+        engine.price_with_statistics(note, context)
+    option, context5, _, training = workloads.american_put_local_vol()
+    e5 = k.MonteCarloVanillaEngine(path_count=1 << 20, step_count=50, seed=1729, precision="fp32")
+    e5.american_report(option, context5, training_path_count=training, basis="poly4")
+    log("profile pass: done")
+
+
 def main():
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -190,12 +219,17 @@ def main():
     parser.add_argument("--warmup", type=int, default=3)
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     parser.add_argument("--headline-only", action="store_true", help="skip configs 1, 3, 4 and 5")
+    parser.add_argument("--profile-pass", action="store_true",
+                        help="run the kernels once for a hardware-counter pass; prints no result line")
     args = parser.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the engine shards paths by block across GPUs by design, not in this build)",
               file=sys.stderr)
         sys.exit(2)
+    if args.profile_pass:
+        profile_pass()
+        return
 
     import kiyosi_amd as k
     from kiyosi_amd import workloads
@@ -239,7 +273,7 @@ def main():
         "price_error_se": abs(est32.price - est64.price) / se if se else None,
         "contracts_per_sec": args.steps / wall32,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": C2_KERNEL_HBM_BYTES,
                      "kernel": "autocall_kernel<sobol, fp32>", "kernel_ms": dev32},
     }
     if not args.headline_only:

@@ -42,8 +42,9 @@ struct BookItem {
 
 // ------------------------------------------------------------------ device
 template <bool FP32, bool SOBOL>
-__device__ void book_sample(const BookItem& it, int64_t block, int j, double* pos, double* neg,
-                            bool* live_out, bool* paired_out, uint32_t& bad) {
+__device__ __attribute__((always_inline)) void sample_body(const BookItem& it, int64_t block, int j, double* pos,
+                                                           double* neg, bool* live_out, bool* paired_out,
+                                                           uint32_t& bad) {
   const int64_t n = block * BLOCK_SAMPLES + j;
   const bool live = n < it.samples;
   const bool paired = it.antithetic && live && n < it.paths / 2;
@@ -94,6 +95,25 @@ __device__ void book_sample(const BookItem& it, int64_t block, int j, double* po
   *paired_out = paired;
 }
 
+// The FP32 kernel inlines the sample body, so the uniform contract fields
+// and step tables are read with scalar loads. The FP64 body stays a call:
+// inlined, its register demand would cut the kernel's occupancy.
+template <bool SOBOL>
+__device__ __attribute__((noinline)) void sample_fp64(const BookItem& it, int64_t block, int j, double* pos,
+                                                      double* neg, bool* live_out, bool* paired_out, uint32_t& bad) {
+  sample_body<false, SOBOL>(it, block, j, pos, neg, live_out, paired_out, bad);
+}
+
+template <bool FP32, bool SOBOL>
+__device__ __attribute__((always_inline)) void book_sample(const BookItem& it, int64_t block, int j, double* pos,
+                                                           double* neg, bool* live_out, bool* paired_out,
+                                                           uint32_t& bad) {
+  if constexpr (FP32)
+    sample_body<true, SOBOL>(it, block, j, pos, neg, live_out, paired_out, bad);
+  else
+    sample_fp64<SOBOL>(it, block, j, pos, neg, live_out, paired_out, bad);
+}
+
 template <bool FP32, int SPT>
 __global__ void __launch_bounds__(256)
     book_kernel(const BookItem* __restrict__ items, const int2* __restrict__ tiles, int n_tiles,
@@ -106,7 +126,9 @@ __global__ void __launch_bounds__(256)
   for (;;) {
     if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
     __syncthreads();
-    const int t = s_tile;
+    // The tile index is the same in every lane; saying so lets the contract's
+    // fields and step tables be read with scalar loads.
+    const int t = __builtin_amdgcn_readfirstlane(s_tile);
     if (t >= n_tiles) break;  // every wave reads the same index: a uniform exit
     const int2 tb = tiles[t];
     const BookItem& it = items[tb.x];

@@ -88,6 +88,11 @@ __device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign
   R noise = 0, terminal = 0;
   bool touched = false, has_cond = false, active = live;
   double knocked_out = 0.0, conditioned = 0.0;
+  // The step and state where the path knocked out or was conditioned on the
+  // hit; the FP64 values are formed after the loop, once for all lanes,
+  // instead of once per lane inside it.
+  int hit = -1;
+  R hit_noise = 0, hit_terminal = 0;
   for (int i = 0; i < b.n_steps; ++i) {
     if (__ballot(active) == 0) break;
     if (!active) continue;
@@ -106,11 +111,9 @@ __device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign
       if (b.up ? noise >= -lower : noise <= -lower) {
         touched = true;
         if (b.condition_on_hit && b.n_steps - i > 8) {
-          const double asset = discounted_product(b.spot, 1.0, (double)terminal);
-          conditioned = d_lognormal_price(asset, b.strike,
-                                          b.moneyness + (double)noise + b.hvr * s.remaining,
-                                          s.discount - b.q * s.remaining, b.log_discount, b.sigma,
-                                          sqrt(s.remaining), b.sign);
+          hit = i;
+          hit_noise = noise;
+          hit_terminal = terminal;
           has_cond = true;
           active = false;
           continue;
@@ -118,9 +121,18 @@ __device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign
       }
     }
     if (touched && !b.knock_in) {
-      knocked_out = discounted_product(b.rebate, 1.0, b.at_hit ? s.discount : b.log_discount);
+      hit = i;
       active = false;
     }
+  }
+  if (has_cond) {
+    const BarrierStep& s = b.steps[hit];
+    const double asset = discounted_product(b.spot, 1.0, (double)hit_terminal);
+    conditioned = d_lognormal_price(asset, b.strike, b.moneyness + (double)hit_noise + b.hvr * s.remaining,
+                                    s.discount - b.q * s.remaining, b.log_discount, b.sigma, sqrt(s.remaining),
+                                    b.sign);
+  } else if (hit >= 0) {
+    knocked_out = discounted_product(b.rebate, 1.0, b.at_hit ? b.steps[hit].discount : b.log_discount);
   }
   if (!live) return 0.0;
   const double asset = discounted_product(b.spot, 1.0, (double)terminal);
@@ -272,6 +284,11 @@ __device__ double accum_path(const AccumDev& a, D& dr, bool live, float sign, ui
   double quantity = a.acc_qty;
   bool active = live;
   double value = 0.0;
+  // Knock-out step and state; the settlement is formed after the loop, once
+  // for all lanes.
+  int ko_step = -1;
+  R ko_lr = 0;
+  double ko_quantity = 0.0;
   const R ko = (R)a.ko_log, sk = (R)a.strike_log, sg = (R)a.sign;
   auto unit = [&](double l) {
     const double spot = discounted_product(a.spot, 1.0, l);
@@ -291,13 +308,18 @@ __device__ double accum_path(const AccumDev& a, D& dr, bool live, float sign, ui
       if (!isfinite(sp) || sp <= 0.0) bad = 1;
     }
     if (sg * lr >= sg * ko) {
-      value = discounted_product(quantity, unit((double)lr), a.steps[3 * i + 2]);
+      ko_step = i;
+      ko_lr = lr;
+      ko_quantity = quantity;
       active = false;
       continue;
     }
     quantity += a.daily_qty * (sg * lr < sg * sk ? a.accel : 1.0);
   }
-  if (active) value = discounted_product(quantity, unit((double)lr), a.terminal_discount);
+  if (ko_step >= 0)
+    value = discounted_product(ko_quantity, unit((double)ko_lr), a.steps[3 * ko_step + 2]);
+  else if (active)
+    value = discounted_product(quantity, unit((double)lr), a.terminal_discount);
   if (live && !isfinite(value)) bad = 1;
   return live ? value : 0.0;
 }
