@@ -132,6 +132,19 @@ kg_status kg_vanilla_price(kg_context* ctx, const kg_settings* settings,
                            const kg_european_terms* option, const kg_bsm_market* market,
                            kg_estimate* out);
 
+/* ---- Shared-draw scenarios of a European option: the batched Greek stencil
+ * of MonteCarloVanillaEngine::price_with_greeks_with_settings
+ * (monte_carlo.rs:1334-1409, european_greek_prices :1143-1162). Prices the
+ * option under n scenario markets on the same normals; out[i].price equals
+ * kg_vanilla_price under markets[i] with the same settings, bit for bit.
+ * Scenario estimates carry the price and path_standard_error only
+ * (has_standard_error = 0). Peak-scaled payoffs (volatility * sqrt(T) > 25
+ * for calls) return KG_UNSUPPORTED_OPERATION: price them one at a time. */
+#define KG_MAX_SCENARIOS 64
+kg_status kg_vanilla_scenarios(kg_context* ctx, const kg_settings* settings,
+                               const kg_european_terms* option, const kg_bsm_market* markets,
+                               int32_t n, kg_estimate* out);
+
 /* ---- Autocallable notes (snowball, binary/ternary snowball, Phoenix):
  * MonteCarloAutocallableEngine::price_with_statistics
  * (engines/monte_carlo/autocallable.rs:874-918), plan building

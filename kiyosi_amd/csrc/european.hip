@@ -67,6 +67,31 @@ __device__ inline uint64_t sobol_dim0(const uint64_t* __restrict__ dirs, uint64_
   return v;
 }
 
+// The normals of a thread's two samples n0, n0 + 1: Sobol dimension 0, or
+// one Box-Muller pair of the block's PCG stream (pair j/2 starts at output
+// 4 * (j/2)).
+template <bool FP32>
+__device__ inline void euro_normals(uint64_t seed, int64_t block, int j, int64_t n0, int sobol,
+                                    const uint64_t* __restrict__ dirs, uint64_t shift, double z[2], float zf[2]) {
+  if (sobol) {
+    for (int s = 0; s < 2; ++s) {
+      uint64_t bits = sobol_dim0(dirs, shift, (uint64_t)(n0 + s));
+      if (FP32)
+        zf[s] = f32::normal_from_bits((uint32_t)(bits >> 20));
+      else
+        z[s] = normal_from_bits(bits);
+    }
+  } else {
+    Pcg32 g;
+    g.init(seed, (uint64_t)block);
+    g.advance((uint64_t)j * 2);
+    if (FP32)
+      f32::box_muller_pair(g, &zf[0], &zf[1]);
+    else
+      box_muller_pair(g, &z[0], &z[1]);
+  }
+}
+
 template <bool FP32>
 __global__ void __launch_bounds__(128)
     european_kernel(EuroParams p, uint64_t seed, int64_t samples, int64_t paths, int antithetic,
@@ -83,23 +108,7 @@ __global__ void __launch_bounds__(128)
   if (n0 < samples) {
     double z[2];
     float zf[2];
-    if (sobol) {
-      for (int s = 0; s < 2; ++s) {
-        uint64_t bits = sobol_dim0(dirs, shift, (uint64_t)(n0 + s));
-        if (FP32)
-          zf[s] = f32::normal_from_bits((uint32_t)(bits >> 20));
-        else
-          z[s] = normal_from_bits(bits);
-      }
-    } else {
-      Pcg32 g;
-      g.init(seed, (uint64_t)block);
-      g.advance((uint64_t)j * 2);  // pair j/2 starts at output 4*(j/2)
-      if (FP32)
-        f32::box_muller_pair(g, &zf[0], &zf[1]);
-      else
-        box_muller_pair(g, &z[0], &z[1]);
-    }
+    euro_normals<FP32>(seed, block, j, n0, sobol, dirs, shift, z, zf);
     for (int s = 0; s < 2; ++s) {
       const int64_t n = n0 + s;
       if (n >= samples) break;
@@ -119,6 +128,44 @@ __global__ void __launch_bounds__(128)
     partials[block * 2 + 0] = m[0];
     partials[block * 2 + 1] = m[1];
   }
+}
+
+// Shared-draw scenarios (the batched Greek stencil of
+// MonteCarloVanillaEngine::european_greek_prices, monte_carlo.rs:1143-1162):
+// each thread draws its two normals once and evaluates every scenario's
+// payoff on them. Scenario s's block moments, formed exactly as the
+// single-scenario kernel forms its totals, go to partials[block * n + s].
+template <bool FP32>
+__global__ void __launch_bounds__(128)
+    european_scenarios_kernel(const EuroParams* __restrict__ params, int n_scen, uint64_t seed, int64_t samples,
+                              int64_t paths, int antithetic, int sobol, const uint64_t* __restrict__ dirs,
+                              uint64_t shift, Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
+  __shared__ double smem[3 * 1 * 2];
+  const int64_t block = blockIdx.x;
+  const int j = 2 * threadIdx.x;
+  const int64_t n0 = block * BLOCK_SAMPLES + j;
+  double z[2] = {0.0, 0.0};
+  float zf[2] = {0.f, 0.f};
+  if (n0 < samples) euro_normals<FP32>(seed, block, j, n0, sobol, dirs, shift, z, zf);
+  uint32_t bad = 0;
+  for (int sc = 0; sc < n_scen; ++sc) {
+    const EuroParams p = params[sc];
+    double vals[1][4];
+    int count[1] = {0};
+    if (n0 < samples) {
+      for (int s = 0; s < 2; ++s) {
+        const int64_t n = n0 + s;
+        if (n >= samples) break;
+        vals[0][count[0]++] = FP32 ? euro_value_fp32(p, zf[s], bad) : euro_value(p, z[s], bad);
+        if (antithetic && n < paths / 2)
+          vals[0][count[0]++] = FP32 ? euro_value_fp32(p, -zf[s], bad) : euro_value(p, -z[s], bad);
+      }
+    }
+    Moments m[1];
+    block_moments<1, 4>(vals, count, m, smem);
+    if (threadIdx.x == 0) partials[block * n_scen + sc] = m[0];
+  }
+  if (bad) raise_flag(flags, FLAG_NONFINITE);
 }
 
 // One pass of the sampler: returns total and pair moments.
@@ -149,6 +196,43 @@ static void european_pass(kg_context* ctx, const Sampler& s, const EuroParams& p
   *device_ms += ms;
 }
 
+// EuropeanPayoff::new (monte_carlo.rs:1171-1252) for a flat BSM market:
+// either a known value or the sampled payoff's parameters.
+static bool plan_european(const kg_european_terms& o, const kg_bsm_market& m, EuroParams* p, double* known) {
+  if (o.option_type != 1 && o.option_type != -1)
+    fail(KG_INVALID_INPUT, "option_type must be 1 (call) or -1 (put)");
+  if (!(m.spot > 0.0) || !std::isfinite(m.spot)) fail(KG_INVALID_INPUT, "spot must be positive");
+  if (!(m.volatility >= 0.0)) fail(KG_INVALID_INPUT, "volatility must be nonnegative");
+  if (!(o.strike > 0.0)) fail(KG_INVALID_INPUT, "strike must be positive");
+  // validate_life (instruments/monitoring.rs:470-481): a European option has
+  // no valuation start (instruments/vanilla.rs:198-200), so only valuation
+  // after expiry is rejected.
+  if (m.valuation_time > o.expiry_time) fail(KG_INVALID_INPUT, "valuation outside contract life");
+  const double sign = o.option_type;
+  const double time = (double)(o.expiry_time - m.valuation_time) / (365.0 * 86400e6);
+  if (time == 0.0) {  // monte_carlo.rs:1188-1192
+    *known = std::max(sign * (m.spot - o.strike), 0.0);
+    return true;
+  }
+  if (m.volatility == 0.0) {  // deterministic::vanilla_to_expiry for flat curves
+    double fwd = m.spot * std::exp(-m.dividend * time) - o.strike * std::exp(-m.rate * time);
+    *known = std::max(sign * fwd, 0.0);
+    return true;
+  }
+  *p = EuroParams{};
+  p->sign = sign;
+  p->diffusion = m.volatility * std::sqrt(time);
+  const double carry = (m.rate - m.dividend) * time;
+  p->half_variance = (0.5 * p->diffusion) * p->diffusion;
+  const double drift = carry - p->half_variance;
+  if (!std::isfinite(drift) || !std::isfinite(p->diffusion)) unstable();
+  p->moneyness = log_ratio(m.spot, o.strike) + carry - p->half_variance;
+  p->log_scale = sign > 0.0 ? std::log(m.spot) - m.dividend * time : std::log(o.strike) - m.rate * time;
+  return false;
+}
+
+static bool needs_peak_scaling(const EuroParams& p) { return p.sign > 0.0 && p.diffusion > 25.0; }
+
 }  // namespace kg
 
 using namespace kg;
@@ -161,40 +245,17 @@ extern "C" kg_status kg_vanilla_price(kg_context* ctx, const kg_settings* settin
     Sampler s = validate_settings(settings, 1);
     const kg_european_terms& o = *option;
     const kg_bsm_market& m = *market;
-    if (o.option_type != 1 && o.option_type != -1)
-      fail(KG_INVALID_INPUT, "option_type must be 1 (call) or -1 (put)");
-    if (!(m.spot > 0.0) || !std::isfinite(m.spot)) fail(KG_INVALID_INPUT, "spot must be positive");
-    if (!(m.volatility >= 0.0)) fail(KG_INVALID_INPUT, "volatility must be nonnegative");
-    if (!(o.strike > 0.0)) fail(KG_INVALID_INPUT, "strike must be positive");
-    // validate_life (instruments/monitoring.rs:470-481)
-    if (m.valuation_time < o.effective_time || m.valuation_time > o.expiry_time)
-      fail(KG_INVALID_INPUT, "valuation outside contract life");
-    const double sign = o.option_type;
-    const double time = (double)(o.expiry_time - m.valuation_time) / (365.0 * 86400e6);
+    EuroParams p;
+    double known = 0.0;
     *out = kg_estimate{};
-    if (time == 0.0) {  // monte_carlo.rs:1188-1192
-      fill_known(std::max(sign * (m.spot - o.strike), 0.0), settings->sampling == KG_SOBOL, out);
+    if (plan_european(o, m, &p, &known)) {
+      fill_known(known, settings->sampling == KG_SOBOL, out);
       return;
     }
-    if (m.volatility == 0.0) {  // deterministic::vanilla_to_expiry for flat curves
-      double fwd = m.spot * std::exp(-m.dividend * time) - o.strike * std::exp(-m.rate * time);
-      fill_known(std::max(sign * fwd, 0.0), settings->sampling == KG_SOBOL, out);
-      return;
-    }
-    EuroParams p{};
-    p.sign = sign;
-    p.diffusion = m.volatility * std::sqrt(time);
-    const double carry = (m.rate - m.dividend) * time;
-    p.half_variance = (0.5 * p.diffusion) * p.diffusion;
-    const double drift = carry - p.half_variance;
-    if (!std::isfinite(drift) || !std::isfinite(p.diffusion)) unstable();
-    p.moneyness = log_ratio(m.spot, o.strike) + carry - p.half_variance;
-    p.log_scale = sign > 0.0 ? std::log(m.spot) - m.dividend * time
-                             : std::log(o.strike) - m.rate * time;
     double log_scale = p.log_scale;
     Moments r[2];
     double ms = 0.0;
-    if (sign > 0.0 && p.diffusion > 25.0) {
+    if (needs_peak_scaling(p)) {
       // Peak scaling (monte_carlo.rs:1104-1131): a first pass finds the
       // largest log payoff, a second pass accumulates in units of it.
       p.mode = 1;
@@ -207,5 +268,69 @@ extern "C" kg_status kg_vanilla_price(kg_context* ctx, const kg_settings* settin
     out->device_ms = ms;
     out->path_steps = (double)s.path_count;
     fill_estimate(s, r[0], r[1], log_scale, m.valuation_time == o.expiry_time, true, out);
+  });
+}
+
+extern "C" kg_status kg_vanilla_scenarios(kg_context* ctx, const kg_settings* settings,
+                                          const kg_european_terms* option, const kg_bsm_market* markets,
+                                          int32_t n, kg_estimate* out) {
+  return guarded(ctx, [&] {
+    if (!option || !markets || !out) fail(KG_INVALID_INPUT, "arguments must not be null");
+    if (n < 1 || n > KG_MAX_SCENARIOS) fail(KG_INVALID_INPUT, "1..=64 scenarios per call");
+    Sampler s = validate_settings(settings, 1);
+    std::vector<EuroParams> params;
+    std::vector<int> index(n, -1);
+    std::vector<double> known(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+      EuroParams p;
+      if (plan_european(*option, markets[i], &p, &known[i])) continue;
+      if (needs_peak_scaling(p))
+        fail(KG_UNSUPPORTED_OPERATION, "peak-scaled payoffs are priced one scenario at a time");
+      index[i] = (int)params.size();
+      params.push_back(p);
+    }
+    std::vector<Moments> r(params.size());
+    double ms = 0.0;
+    if (!params.empty()) {
+      const int64_t blocks = s.blocks();
+      const int sc = (int)params.size();
+      EuroParams* d_params = (EuroParams*)ctx->scratch.reserve(params.size() * sizeof(EuroParams));
+      upload_async(ctx, d_params, params.data(), params.size() * sizeof(EuroParams));
+      Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * (size_t)sc * blocks);
+      const uint64_t* dirs = nullptr;
+      uint64_t shift = 0;
+      if (s.sobol) {
+        const SobolSetup& so = sobol_setup(ctx, 1, s.seed);
+        dirs = (const uint64_t*)so.d_directions.ptr;
+        shift = so.shifts[0];
+      }
+      uint32_t* flags = reset_flags(ctx);
+      timing_begin(ctx);
+      if (s.fp32)
+        hipLaunchKernelGGL(european_scenarios_kernel<true>, dim3(blocks), dim3(128), 0, ctx->stream, d_params, sc,
+                           s.seed, s.samples(), s.path_count, (int)s.antithetic, (int)s.sobol, dirs, shift, parts,
+                           flags);
+      else
+        hipLaunchKernelGGL(european_scenarios_kernel<false>, dim3(blocks), dim3(128), 0, ctx->stream, d_params, sc,
+                           s.seed, s.samples(), s.path_count, (int)s.antithetic, (int)s.sobol, dirs, shift, parts,
+                           flags);
+      KG_HIP(hipGetLastError());
+      finish_sampling(ctx, parts, blocks, sc, r.data(), &ms);
+    }
+    for (int i = 0; i < n; ++i) {
+      out[i] = kg_estimate{};
+      if (index[i] < 0) {
+        fill_known(known[i], settings->sampling == KG_SOBOL, &out[i]);
+        continue;
+      }
+      const Moments none{};
+      fill_estimate(s, r[index[i]], none, params[index[i]].log_scale,
+                    markets[i].valuation_time == option->expiry_time, true, &out[i]);
+      // Scenario estimates carry the price and the path standard error only.
+      out[i].has_standard_error = 0;
+      out[i].standard_error = out[i].ci_lower = out[i].ci_upper = 0.0;
+      out[i].device_ms = ms;
+      out[i].path_steps = (double)s.path_count;
+    }
   });
 }

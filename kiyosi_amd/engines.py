@@ -133,6 +133,35 @@ class MonteCarloVanillaEngine(_MonteCarloEngine):
     def _resolved(self):
         return self if self.seed is not None else self.with_resolved_seed()
 
+    def price_with_greeks(self, option, context, request=None, settings=None):
+        """Price and Greeks of a European option by central differences on
+        shared draws: the stencil's scenarios are priced in one GPU launch on
+        the same normals (price_with_greeks_with_settings,
+        engines/monte_carlo.rs:1334-1409). Returns a PricingResult whose
+        values equal numerical_greeks(engine, ...) with the same seed."""
+        from . import greeks
+        from .instruments import EuropeanOption
+        if not isinstance(option, EuropeanOption):
+            return greeks.numerical_greeks(self, option, context, request, settings)
+        settings = settings or greeks.NumericalShiftSettings()
+        engine = self._resolved()
+        scenarios = greeks.plan(option, context, request, settings)
+        if len(scenarios) > _abi.MAX_SCENARIOS:
+            return greeks.numerical_greeks(engine, option, context, request, settings)
+        markets = (_abi.BsmMarket * len(scenarios))(*[ctx.bsm_market() for ctx in scenarios])
+        out = (_abi.Estimate * len(scenarios))()
+        ctx = _native.context(engine.device)
+        status = _native.lib().kg_vanilla_scenarios(ctx.handle, C.byref(engine._settings()),
+                                                    C.byref(option._terms()), markets, len(scenarios), out)
+        if status == _abi.UNSUPPORTED_OPERATION:  # peak-scaled payoffs
+            return greeks.numerical_greeks(engine, option, context, request, settings)
+        ctx.check(status)
+        prices = {greeks.key(scn): out[i].price for i, scn in enumerate(scenarios)}
+        values, missing = greeks.complete(option, context, request, settings, out[0].price,
+                                          lambda scn: prices[greeks.key(scn)])
+        seed = engine.seed if any(out[i].has_seed for i in range(len(scenarios))) else None
+        return greeks.PricingResult(out[0].price, greeks.normalize_request(request), values, missing, seed)
+
     def american_report(self, option, context, *, training_path_count=None, basis="quadratic",
                         epochs=4, batch=16384, learning_rate=1e-3):
         """Least-squares Monte Carlo on the exercise grid of step_count intervals:

@@ -65,8 +65,9 @@ inline bool european_plan(const kg_european_terms& o, const kg_bsm_market& m, Te
                           double* known) {
   if (o.option_type != 1 && o.option_type != -1)
     throw Failure(INVALID_INPUT, "option_type must be 1 (call) or -1 (put)");
-  if (m.valuation_time < o.effective_time || m.valuation_time > o.expiry_time)
-    throw Failure(INVALID_INPUT, "valuation outside contract life");
+  // validate_life with EuropeanOption::valuation_start() == None
+  // (instruments/monitoring.rs:470-481, instruments/vanilla.rs:198-200).
+  if (m.valuation_time > o.expiry_time) throw Failure(INVALID_INPUT, "valuation outside contract life");
   double sign = o.option_type;
   double time = years_until(m.valuation_time, o.expiry_time);
   if (time == 0.0) {
