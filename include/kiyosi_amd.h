@@ -205,6 +205,23 @@ kg_status kg_autocallable_price(kg_context* ctx, const kg_settings* settings,
                                 const kg_autocall_terms* note, const kg_bsm_market* market,
                                 kg_estimate* out);
 
+/* Prices one note in n market scenarios (1..KG_MAX_AUTOCALL_SCENARIOS) on the
+ * same draws: the shared-draw batches of
+ * MonteCarloAutocallableEngine::price_with_greeks_with_settings
+ * (engines/monte_carlo/autocallable.rs:919-999), i.e. spot_prices (:174-197)
+ * for spot bumps and market_prices (:199-241) for volatility and rate bumps,
+ * both through PathPlan::sample with several plans (:580-599). out[i] is
+ * exactly what kg_autocallable_price returns for markets[i] with the same
+ * settings, device_ms aside (shared by the lanes). Scenarios with a known
+ * value (knock-out or expiry at the valuation) take it. The sampled scenarios
+ * must share the note's step grid (same valuation time, knock-in state and
+ * bridge); otherwise the call returns KG_UNSUPPORTED_OPERATION and the
+ * scenarios are priced one at a time. */
+#define KG_MAX_AUTOCALL_SCENARIOS 16
+kg_status kg_autocallable_scenarios(kg_context* ctx, const kg_settings* settings,
+                                    const kg_autocall_terms* note, const kg_bsm_market* markets,
+                                    int32_t n, kg_estimate* out);
+
 /* ---- Heston path stepping. kiyosi has Heston only in its Fourier and FD
  * engines (market/heston.rs:58-227, engines/analytic/heston.rs,
  * engines/finite_difference/heston.rs); every reference MC engine rejects

@@ -21,18 +21,14 @@
 
 namespace kg {
 
+// The draw source of sample j of `block`: the block's Sobol LDS tables (or
+// direct points for long paths), or the block's PCG stream advanced to
+// normal j * dims.
 template <bool SOBOL, bool FP32>
-__global__ void __launch_bounds__(256)
-    autocall_kernel(AcPlanDev p, uint64_t seed, int64_t samples, int64_t paths, int antithetic,
-                    const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shifts,
-                    int sobol_lds, Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
-  extern __shared__ __align__(16) unsigned char lds[];
-  __shared__ Moments smem[2 * 4];
-  const int64_t block = blockIdx.x;
-  const int j = threadIdx.x;
-  const int64_t n = block * BLOCK_SAMPLES + j;
-  const bool live = n < samples;
-  const int dims = p.n_steps;
+__device__ inline Draws<SOBOL, FP32> setup_draws(int dims, uint64_t seed, int64_t block, int j, int64_t n,
+                                                 const uint64_t* __restrict__ dirs,
+                                                 const uint64_t* __restrict__ shifts, int sobol_lds,
+                                                 unsigned char* lds) {
   Draws<SOBOL, FP32> dr;
   dr.dims = dims;
   dr.direct = !sobol_lds;
@@ -64,6 +60,21 @@ __global__ void __launch_bounds__(256)
       dr.s.start(seed, (uint64_t)block, first);
     }
   }
+  return dr;
+}
+
+template <bool SOBOL, bool FP32>
+__global__ void __launch_bounds__(256)
+    autocall_kernel(AcPlanDev p, uint64_t seed, int64_t samples, int64_t paths, int antithetic,
+                    const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shifts,
+                    int sobol_lds, Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ Moments smem[2 * 4];
+  const int64_t block = blockIdx.x;
+  const int j = threadIdx.x;
+  const int64_t n = block * BLOCK_SAMPLES + j;
+  const bool live = n < samples;
+  Draws<SOBOL, FP32> dr = setup_draws<SOBOL, FP32>(p.n_steps, seed, block, j, n, dirs, shifts, sobol_lds, lds);
   uint32_t bad = 0;
   const bool paired = antithetic && live && n < paths / 2;
   const Draws<SOBOL, FP32> origin = dr;
@@ -91,10 +102,62 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-template <bool SOBOL, bool FP32>
-static void launch(kg_context* ctx, const AcPlanDev& p, const Sampler& s, const uint64_t* dirs,
+// Scenario lanes on shared draws: lane l's total and pair moments of the
+// block go to partials[block * slots + slot_base + 2l] and [... + 1],
+// formed exactly as the single-plan kernel forms its own.
+template <int L, bool SOBOL, bool FP32>
+__global__ void __launch_bounds__(256)
+    autocall_lanes_kernel(const AcLanes* __restrict__ lanes, uint64_t seed, int64_t samples, int64_t paths, int antithetic,
+                          const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shifts, int sobol_lds,
+                          Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ Moments smem[2 * 4];
+  const int64_t block = blockIdx.x;
+  const int j = threadIdx.x;
+  const int64_t n = block * BLOCK_SAMPLES + j;
+  const bool live = n < samples;
+  const AcLanes& P = *lanes;
+  Draws<SOBOL, FP32> dr = setup_draws<SOBOL, FP32>(P.n_steps, seed, block, j, n, dirs, shifts, sobol_lds, lds);
+  uint32_t bad = 0;
+  const bool paired = antithetic && live && n < paths / 2;
+  const Draws<SOBOL, FP32> origin = dr;
+  double pos[L], neg[L];
+  if (P.bridge)
+    bridge_lanes<L, SOBOL, FP32>(P, dr, live, 1.0f, bad, pos);
+  else
+    step_lanes<L, SOBOL, FP32>(P, dr, live, 1.0f, bad, pos);
+#pragma unroll
+  for (int l = 0; l < L; ++l) neg[l] = 0.0;
+  if (antithetic && __ballot(paired) != 0) {
+    dr = origin;
+    if (P.bridge)
+      bridge_lanes<L, SOBOL, FP32>(P, dr, paired, -1.0f, bad, neg);
+    else
+      step_lanes<L, SOBOL, FP32>(P, dr, paired, -1.0f, bad, neg);
+  }
+  if (bad) raise_flag(flags, FLAG_NONFINITE);
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    if (l >= P.n) break;
+    Moments m[2] = {moments_zero(), moments_zero()};
+    if (live) {
+      moments_add(m[0], pos[l], true);
+      if (paired) {
+        moments_add(m[0], neg[l], true);
+        moments_add(m[1], midpoint(pos[l], neg[l]), true);
+      }
+    }
+    block_merge<2>(m, smem);
+    if (threadIdx.x == 0) {
+      partials[block * P.slots + P.slot_base + 2 * l] = m[0];
+      partials[block * P.slots + P.slot_base + 2 * l + 1] = m[1];
+    }
+  }
+}
+
+template <bool SOBOL, bool FP32, int L, class Plan>
+static void launch(kg_context* ctx, const Plan& p, int dims, const Sampler& s, const uint64_t* dirs,
                    const uint64_t* shifts, Moments* parts, uint32_t* flags) {
-  const int dims = p.n_steps;
   size_t lds = 0;
   int sobol_lds = 0;
   if (SOBOL) {
@@ -102,13 +165,110 @@ static void launch(kg_context* ctx, const AcPlanDev& p, const Sampler& s, const 
     sobol_lds = lds <= SOBOL_LDS_LIMIT;
     if (!sobol_lds) lds = 0;
   }
-  auto kernel = autocall_kernel<SOBOL, FP32>;
+  void (*kernel)(Plan, uint64_t, int64_t, int64_t, int, const uint64_t*, const uint64_t*, int, Moments*,
+                 uint32_t*);
+  if constexpr (std::is_same<Plan, const AcLanes*>::value)
+    kernel = autocall_lanes_kernel<L, SOBOL, FP32>;
+  else
+    kernel = autocall_kernel<SOBOL, FP32>;
   if (lds > 64 * 1024)
     KG_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
   hipLaunchKernelGGL(kernel, dim3(s.blocks()), dim3(256), lds, ctx->stream, p, s.seed, s.samples(),
                      s.path_count, (int)s.antithetic, dirs, shifts, sobol_lds, parts, flags);
   KG_HIP(hipGetLastError());
+}
+
+template <int L = 1, class Plan>
+static void launch_any(kg_context* ctx, const Plan& p, int dims, const Sampler& s, Moments* parts,
+                       uint32_t* flags) {
+  const uint64_t* dirs = nullptr;
+  const uint64_t* shifts = nullptr;
+  if (s.sobol) {
+    const SobolSetup& so = sobol_setup(ctx, (uint32_t)dims, s.seed);
+    dirs = (const uint64_t*)so.d_directions.ptr;
+    shifts = (const uint64_t*)so.d_shifts.ptr;
+  }
+  if (s.sobol && s.fp32)
+    launch<true, true, L>(ctx, p, dims, s, dirs, shifts, parts, flags);
+  else if (s.sobol)
+    launch<true, false, L>(ctx, p, dims, s, dirs, shifts, parts, flags);
+  else if (s.fp32)
+    launch<false, true, L>(ctx, p, dims, s, dirs, shifts, parts, flags);
+  else
+    launch<false, false, L>(ctx, p, dims, s, dirs, shifts, parts, flags);
+}
+
+// The lanes' tables interleaved row by row ([row][lane]) in one staging
+// blob; table pointers are offsets into the blob until bind_lanes.
+template <class T>
+static std::vector<T> interleave(const std::vector<const AcHostPlan*>& L, std::vector<T> AcHostPlan::*table,
+                                 size_t width) {
+  const size_t n = L.size(), rows = (L[0]->*table).size() / width;
+  std::vector<T> out(rows * n * width);
+  for (size_t l = 0; l < n; ++l)
+    for (size_t r = 0; r < rows; ++r)
+      for (size_t c = 0; c < width; ++c) out[(r * n + l) * width + c] = (L[l]->*table)[r * width + c];
+  return out;
+}
+
+struct LaneOffsets {
+  size_t o[13];
+};
+
+static AcLanes stage_lanes(const std::vector<const AcHostPlan*>& L, std::vector<unsigned char>& blob,
+                           LaneOffsets& off) {
+  const AcHostPlan& a = *L[0];
+  AcLanes P{};
+  P.n = (int32_t)L.size();
+  P.n_steps = a.dev.n_steps;
+  P.n_intervals = a.dev.n_intervals;
+  P.bridge = a.dev.bridge;
+  for (size_t l = 0; l < L.size(); ++l) {
+    const AcPlanDev& d = L[l]->dev;
+    P.s[l] = AcLaneScalars{d.knock_in,  d.coupon,     d.initial_spot,  d.safe_lo, d.safe_hi,
+                           d.final_log_discount,   d.term_base,     d.term_no_ki, d.term_ki_fixed,
+                           d.lower,     d.upper,      d.inv_initial,   d.knocked_in, d.has_ki,
+                           d.expiry_knock_in,      d.downside};
+  }
+  std::vector<int32_t> ends;
+  for (const AcIntervalD& b : a.intervals) ends.push_back(b.end);
+  std::vector<AcStepD> intervals(a.intervals.size() * L.size());
+  for (size_t l = 0; l < L.size(); ++l)
+    for (size_t k = 0; k < a.intervals.size(); ++k)
+      intervals[k * L.size() + l] = AcStepD{L[l]->intervals[k].drift, L[l]->intervals[k].diffusion};
+  size_t* o = off.o;
+  stage(blob, a.step_event, &o[0]);
+  stage(blob, a.step_observe, &o[1]);
+  stage(blob, ends, &o[2]);
+  stage(blob, interleave(L, &AcHostPlan::steps, 1), &o[3]);
+  stage(blob, interleave(L, &AcHostPlan::weight, 1), &o[4]);
+  stage(blob, interleave(L, &AcHostPlan::fill, 1), &o[5]);
+  stage(blob, interleave(L, &AcHostPlan::events, 1), &o[6]);
+  stage(blob, intervals, &o[7]);
+  stage(blob, interleave(L, &AcHostPlan::steps32, 2), &o[8]);
+  stage(blob, interleave(L, &AcHostPlan::weight32, 1), &o[9]);
+  stage(blob, interleave(L, &AcHostPlan::fill32, 1), &o[10]);
+  stage(blob, interleave(L, &AcHostPlan::events32, 2), &o[11]);
+  stage(blob, interleave(L, &AcHostPlan::intervals32, 2), &o[12]);
+  return P;
+}
+
+static void bind_lanes(AcLanes& P, unsigned char* d, const LaneOffsets& off) {
+  const size_t* o = off.o;
+  P.step_event = (const int32_t*)(d + o[0]);
+  P.step_observe = (const uint8_t*)(d + o[1]);
+  P.interval_end = (const int32_t*)(d + o[2]);
+  P.steps = (const AcStepD*)(d + o[3]);
+  P.weight = (const double*)(d + o[4]);
+  P.fill = (const double*)(d + o[5]);
+  P.events = (const AcEventD*)(d + o[6]);
+  P.intervals = (const AcStepD*)(d + o[7]);
+  P.steps32 = (const float*)(d + o[8]);
+  P.weight32 = (const float*)(d + o[9]);
+  P.fill32 = (const float*)(d + o[10]);
+  P.events32 = (const float*)(d + o[11]);
+  P.intervals32 = (const float*)(d + o[12]);
 }
 
 }  // namespace kg
@@ -128,28 +288,90 @@ extern "C" kg_status kg_autocallable_price(kg_context* ctx, const kg_settings* s
     }
     Sampler s = validate_settings(settings, (uint64_t)h.dev.n_steps);
     AcPlanDev p = upload(ctx, h);
-    const uint64_t* dirs = nullptr;
-    const uint64_t* shifts = nullptr;
-    if (s.sobol) {
-      const SobolSetup& so = sobol_setup(ctx, (uint32_t)p.n_steps, s.seed);
-      dirs = (const uint64_t*)so.d_directions.ptr;
-      shifts = (const uint64_t*)so.d_shifts.ptr;
-    }
     const int64_t blocks = s.blocks();
     Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 2 * blocks);
     uint32_t* flags = reset_flags(ctx);
+    if (s.sobol) sobol_setup(ctx, (uint32_t)p.n_steps, s.seed);
     timing_begin(ctx);
-    if (s.sobol && s.fp32)
-      launch<true, true>(ctx, p, s, dirs, shifts, parts, flags);
-    else if (s.sobol)
-      launch<true, false>(ctx, p, s, dirs, shifts, parts, flags);
-    else if (s.fp32)
-      launch<false, true>(ctx, p, s, dirs, shifts, parts, flags);
-    else
-      launch<false, false>(ctx, p, s, dirs, shifts, parts, flags);
+    launch_any(ctx, p, p.n_steps, s, parts, flags);
     Moments r[2];
     finish_sampling(ctx, parts, blocks, 2, r, &out->device_ms);
     out->path_steps = (double)s.path_count * (double)p.n_steps;
     fill_estimate(s, r[0], r[1], 0.0, false, true, out);
+  });
+}
+
+extern "C" kg_status kg_autocallable_scenarios(kg_context* ctx, const kg_settings* settings,
+                                               const kg_autocall_terms* note, const kg_bsm_market* markets,
+                                               int32_t n, kg_estimate* out) {
+  return guarded(ctx, [&] {
+    if (!note || !markets || !out) fail(KG_INVALID_INPUT, "arguments must not be null");
+    if (n < 1 || n > AC_MAX_SCENARIOS) fail(KG_INVALID_INPUT, "scenario count must be between 1 and 16");
+    std::vector<AcHostPlan> h(n);
+    for (int i = 0; i < n; ++i) {
+      h[i] = build_plan(*note, markets[i]);
+      out[i] = kg_estimate{};
+    }
+    // Scenarios with a known value take it; the others become lanes.
+    std::vector<int> lane_of;
+    for (int i = 0; i < n; ++i) {
+      if (h[i].fixed) continue;
+      if (!lane_of.empty() && !same_grid(h[lane_of[0]], h[i]))
+        fail(KG_UNSUPPORTED_OPERATION, "the scenarios do not share the path grid; price them one at a time");
+      lane_of.push_back(i);
+    }
+    const bool sobol = settings && settings->sampling == KG_SOBOL;
+    if (lane_of.empty()) {
+      for (int i = 0; i < n; ++i) fill_known(h[i].fixed_value, sobol, &out[i]);
+      return;
+    }
+    const int dims = h[lane_of[0]].dev.n_steps;
+    Sampler s = validate_settings(settings, (uint64_t)dims);
+    // Launch groups of at most AC_MAX_LANES lanes; the lane sets follow
+    // their tables in one upload.
+    const int m = (int)lane_of.size();
+    const int slots = 2 * m;
+    std::vector<unsigned char> blob;
+    std::vector<AcLanes> groups;
+    std::vector<LaneOffsets> offs;
+    for (int g = 0; g < m; g += AC_MAX_LANES) {
+      std::vector<const AcHostPlan*> L;
+      for (int l = g; l < std::min(m, g + AC_MAX_LANES); ++l) L.push_back(&h[lane_of[l]]);
+      offs.emplace_back();
+      groups.push_back(stage_lanes(L, blob, offs.back()));
+      groups.back().slot_base = 2 * g;
+      groups.back().slots = slots;
+    }
+    const size_t at = (blob.size() + 255) & ~size_t(255);
+    blob.resize(at + sizeof(AcLanes) * groups.size());
+    unsigned char* d = (unsigned char*)ctx->scratch.reserve(blob.size());
+    for (size_t g = 0; g < groups.size(); ++g) {
+      bind_lanes(groups[g], d, offs[g]);
+      std::memcpy(blob.data() + at + g * sizeof(AcLanes), &groups[g], sizeof(AcLanes));
+    }
+    upload_async(ctx, d, blob.data(), blob.size());
+    const int64_t blocks = s.blocks();
+    Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * slots * blocks);
+    uint32_t* flags = reset_flags(ctx);
+    if (s.sobol) sobol_setup(ctx, (uint32_t)dims, s.seed);
+    timing_begin(ctx);
+    for (size_t g = 0; g < groups.size(); ++g) {
+      const AcLanes* d_lanes = (const AcLanes*)(d + at + g * sizeof(AcLanes));
+      if (groups[g].n <= 2)
+        launch_any<2>(ctx, d_lanes, dims, s, parts, flags);
+      else
+        launch_any<4>(ctx, d_lanes, dims, s, parts, flags);
+    }
+    Moments r[2 * AC_MAX_SCENARIOS];
+    double ms = 0.0;
+    finish_sampling(ctx, parts, blocks, slots, r, &ms);
+    for (int i = 0; i < n; ++i)
+      if (h[i].fixed) fill_known(h[i].fixed_value, sobol, &out[i]);
+    for (int l = 0; l < m; ++l) {
+      kg_estimate* o = &out[lane_of[l]];
+      o->device_ms = ms;
+      o->path_steps = (double)s.path_count * (double)dims;
+      fill_estimate(s, r[2 * l], r[2 * l + 1], 0.0, false, true, o);
+    }
   });
 }

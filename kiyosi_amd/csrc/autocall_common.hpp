@@ -106,7 +106,9 @@ __device__ inline double event_payment(const AcEventD& e, bool due, bool out) {
 
 // discounted_terminal (autocallable.rs:798-801) + AutocallableNote::terminal
 // (instruments/autocallable.rs:1396-1414).
-__device__ inline double terminal_value(const AcPlanDev& p, double lr, bool ki, uint32_t& bad) {
+template <class Plan>
+__device__ inline double terminal_value(const Plan& p, double lr, bool ki, uint32_t& bad) {
+#pragma clang fp contract(off)
   double spot = discounted_product(p.initial_spot, 1.0, lr);
   if (!isfinite(spot)) bad = 1;
   double amount;
@@ -287,6 +289,220 @@ __device__ double step_path(const AcPlanDev& p, Draws<SOBOL, FP32>& dr, bool liv
   }
   if (live && !isfinite(cash)) bad = 1;
   return cash;
+}
+
+// ------------------------------------------------------------------ scenario lanes
+// Up to AC_MAX_LANES market scenarios of one note on the same draws
+// (PathPlan::sample with several plans, autocallable.rs:580-599, and the
+// bridge_payoffs / spot_payoffs lane loops, :628-800). The lanes share the
+// step grid (step_event, step_observe, interval ends; checked on the host)
+// but may differ in every level, transition and payment. Each lane repeats
+// the single-plan arithmetic of bridge_path / step_path on its own state, so
+// its value equals that of pricing its scenario alone; a normal is drawn
+// while any lane of the sample still needs it. Per-step tables interleave
+// the lanes ([step][lane]) so that one base pointer serves every lane. The
+// lane loops are unrolled to L (a power of two at least n) so that each
+// lane's state stays in registers; more lanes than AC_MAX_LANES take
+// further launches over the same draws.
+constexpr int AC_MAX_LANES = 4;   // lanes per launch
+constexpr int AC_MAX_SCENARIOS = 16;  // lanes per call, in launches of AC_MAX_LANES
+
+struct AcLaneScalars {
+  double knock_in, coupon, initial_spot, safe_lo, safe_hi, final_log_discount;
+  double term_base, term_no_ki, term_ki_fixed, lower, upper, inv_initial;
+  int32_t knocked_in, has_ki, expiry_knock_in, downside;
+};
+
+struct AcLanes {
+  int32_t n, n_steps, n_intervals, bridge;
+  int32_t slot_base, slots;  // lane l's moments: partials[block * slots + slot_base + 2l (+1)]
+  AcLaneScalars s[AC_MAX_LANES];
+  const int32_t* step_event;    // [step], shared
+  const uint8_t* step_observe;  // [step], shared
+  const int32_t* interval_end;  // [interval], shared
+  const AcStepD* steps;         // [step][lane]
+  const double* weight;         // [step][lane]
+  const double* fill;           // [step][lane]
+  const AcEventD* events;       // [event][lane]
+  const AcStepD* intervals;     // [interval][lane] drift, diffusion
+  const float* steps32;         // [step][lane] drift, diffusion
+  const float* weight32;        // [step][lane]
+  const float* fill32;          // [step][lane]
+  const float* events32;        // [event][lane] knock_out, coupon_barrier
+  const float* intervals32;     // [interval][lane] drift, diffusion
+};
+
+template <typename R>
+struct LaneTab;
+template <>
+struct LaneTab<double> {
+  static __device__ inline double drift(const AcLanes& P, int i, int l) { return P.steps[i * P.n + l].drift; }
+  static __device__ inline double diff(const AcLanes& P, int i, int l) { return P.steps[i * P.n + l].diffusion; }
+  static __device__ inline double w(const AcLanes& P, int i, int l) { return P.weight[i * P.n + l]; }
+  static __device__ inline double f(const AcLanes& P, int i, int l) { return P.fill[i * P.n + l]; }
+  static __device__ inline double ko(const AcLanes& P, int e, int l) { return P.events[e * P.n + l].knock_out; }
+  static __device__ inline double cb(const AcLanes& P, int e, int l) { return P.events[e * P.n + l].coupon_barrier; }
+  static __device__ inline double idrift(const AcLanes& P, int k, int l) { return P.intervals[k * P.n + l].drift; }
+  static __device__ inline double idiff(const AcLanes& P, int k, int l) {
+    return P.intervals[k * P.n + l].diffusion;
+  }
+};
+template <>
+struct LaneTab<float> {
+  static __device__ inline float drift(const AcLanes& P, int i, int l) { return P.steps32[2 * (i * P.n + l)]; }
+  static __device__ inline float diff(const AcLanes& P, int i, int l) { return P.steps32[2 * (i * P.n + l) + 1]; }
+  static __device__ inline float w(const AcLanes& P, int i, int l) { return P.weight32[i * P.n + l]; }
+  static __device__ inline float f(const AcLanes& P, int i, int l) { return P.fill32[i * P.n + l]; }
+  static __device__ inline float ko(const AcLanes& P, int e, int l) { return P.events32[2 * (e * P.n + l)]; }
+  static __device__ inline float cb(const AcLanes& P, int e, int l) { return P.events32[2 * (e * P.n + l) + 1]; }
+  static __device__ inline float idrift(const AcLanes& P, int k, int l) { return P.intervals32[2 * (k * P.n + l)]; }
+  static __device__ inline float idiff(const AcLanes& P, int k, int l) {
+    return P.intervals32[2 * (k * P.n + l) + 1];
+  }
+};
+
+// Per-lane flags of one sample are bit masks (bit l for lane l), so that
+// they occupy one register each rather than one lane mask per lane.
+__device__ inline bool bit(uint32_t mask, int l) { return (mask >> l) & 1u; }
+
+template <int L, bool SOBOL, bool FP32>
+__device__ void bridge_lanes(const AcLanes& P, Draws<SOBOL, FP32>& dr, bool live, float sign, uint32_t& bad,
+                             double (&cash)[L]) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  using T = LaneTab<R>;
+  const int n = P.n;
+  uint32_t monitoring = 0, ki = 0, active = 0;
+  R lr[L], target[L], minimum[L];
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const bool on = l < n;
+    cash[l] = on ? P.s[l].coupon : 0.0;
+    if (on && !P.s[l].knocked_in && P.s[l].has_ki) monitoring |= 1u << l;
+    if (on && P.s[l].knocked_in) ki |= 1u << l;
+    if (on && live) active |= 1u << l;
+    lr[l] = 0;
+  }
+  int start = 0;
+  for (int k = 0; k < P.n_intervals; ++k) {
+    if (__ballot(active != 0) == 0) break;
+    const int end = P.interval_end[k];
+    R z = 0;
+    if (active) z = (R)sign * dr.template normal<R>(start);
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      target[l] = lr[l];
+      minimum[l] = (R)INFINITY;
+      if (bit(active, l))
+        target[l] = lr[l] + (FP32 ? fmaf((float)T::idiff(P, k, l), (float)z, (float)T::idrift(P, k, l))
+                                  : fma((double)T::idiff(P, k, l), (double)z, (double)T::idrift(P, k, l)));
+    }
+    // monitoring implies active: a knock-out ends both.
+    uint32_t filling = monitoring;
+    if (active && !filling) dr.skip(end - start);
+    for (int i = start; i < end; ++i) {
+      if (__ballot(filling != 0) == 0) break;
+      if (filling) {
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+          if (bit(filling, l) && minimum[l] < (R)P.s[l].knock_in) filling &= ~(1u << l);
+        if (!filling) dr.skip(end - i);
+      }
+      if (filling) {
+        const R zf = (R)sign * dr.template normal<R>(i + 1);
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+          if (bit(filling, l)) {
+            const R w = T::w(P, i, l), f = T::f(P, i, l);
+            lr[l] += FP32 ? fmaf((float)w, (float)(target[l] - lr[l]), (float)(f * zf))
+                          : fma((double)w, (double)(target[l] - lr[l]), (double)(f * zf));
+            minimum[l] = fmin(minimum[l], lr[l]);
+          }
+      }
+    }
+    start = end + 1;
+    const bool observe = P.step_observe[end];
+    const int e = P.step_event[end];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      lr[l] = target[l];
+      if (!bit(active, l)) continue;
+      const R level = (R)P.s[l].knock_in;
+      if (bit(monitoring, l) && (minimum[l] < level || (observe && lr[l] < level))) {
+        monitoring &= ~(1u << l);
+        ki |= 1u << l;
+      }
+      if (e >= 0) {
+        const AcEventD& ev = P.events[e * n + l];
+        const bool hit = ev.has_knock_out && lr[l] >= T::ko(P, e, l);
+        cash[l] += event_payment(ev, lr[l] >= T::cb(P, e, l), hit);
+        if (hit) {
+          active &= ~(1u << l);
+          monitoring &= ~(1u << l);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    if (bit(active, l)) {
+      const AcLaneScalars& p = P.s[l];
+      const bool hit = bit(ki, l) || (p.expiry_knock_in && p.has_ki && lr[l] < (R)p.knock_in);
+      cash[l] += terminal_value(p, (double)lr[l], hit, bad);
+    }
+    if (live && l < n && !isfinite(cash[l])) bad = 1;
+  }
+}
+
+template <int L, bool SOBOL, bool FP32>
+__device__ void step_lanes(const AcLanes& P, Draws<SOBOL, FP32>& dr, bool live, float sign, uint32_t& bad,
+                           double (&cash)[L]) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  using T = LaneTab<R>;
+  const int n = P.n;
+  uint32_t ki = 0, active = 0;
+  R lr[L];
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const bool on = l < n;
+    cash[l] = on ? P.s[l].coupon : 0.0;
+    if (on && P.s[l].knocked_in) ki |= 1u << l;
+    if (on && live) active |= 1u << l;
+    lr[l] = 0;
+  }
+  for (int i = 0; i < P.n_steps; ++i) {
+    if (__ballot(active != 0) == 0) break;
+    if (!active) continue;
+    const R z = (R)sign * dr.template normal<R>(i);
+    const bool observe = P.step_observe[i];
+    const int e = P.step_event[i];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      if (!bit(active, l)) continue;
+      const AcLaneScalars& p = P.s[l];
+      lr[l] += FP32 ? fmaf((float)T::diff(P, i, l), (float)z, (float)T::drift(P, i, l))
+                    : fma((double)T::diff(P, i, l), (double)z, (double)T::drift(P, i, l));
+      if (!((double)lr[l] >= p.safe_lo && (double)lr[l] <= p.safe_hi)) {
+        double s = discounted_product(p.initial_spot, 1.0, (double)lr[l]);
+        if (!isfinite(s) || s <= 0.0) bad = 1;
+      }
+      if (observe && p.has_ki && lr[l] < (R)p.knock_in) ki |= 1u << l;
+      if (e >= 0) {
+        const AcEventD& ev = P.events[e * n + l];
+        const bool out = ev.has_knock_out && lr[l] >= T::ko(P, e, l);
+        cash[l] += event_payment(ev, lr[l] >= T::cb(P, e, l), out);
+        if (out) active &= ~(1u << l);
+      }
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    if (bit(active, l)) {
+      const AcLaneScalars& p = P.s[l];
+      const bool hit = bit(ki, l) || (p.expiry_knock_in && p.has_ki && lr[l] < (R)p.knock_in);
+      cash[l] += terminal_value(p, (double)lr[l], hit, bad);
+    }
+    if (live && l < n && !isfinite(cash[l])) bad = 1;
+  }
 }
 
 // ------------------------------------------------------------------ host plan
@@ -595,9 +811,13 @@ inline void stage(std::vector<unsigned char>& blob, const std::vector<T>& v, siz
   if (!v.empty()) std::memcpy(blob.data() + *offset, v.data(), v.size() * sizeof(T));
 }
 
-inline AcPlanDev upload(kg_context* ctx, AcHostPlan& h) {
-  std::vector<unsigned char> blob;
+struct AcOffsets {
   size_t o[12];
+};
+
+inline AcOffsets stage_plan(std::vector<unsigned char>& blob, const AcHostPlan& h) {
+  AcOffsets r;
+  size_t* o = r.o;
   stage(blob, h.steps, &o[0]);
   stage(blob, h.step_event, &o[1]);
   stage(blob, h.step_observe, &o[2]);
@@ -610,8 +830,12 @@ inline AcPlanDev upload(kg_context* ctx, AcHostPlan& h) {
   stage(blob, h.fill32, &o[9]);
   stage(blob, h.events32, &o[10]);
   stage(blob, h.intervals32, &o[11]);
-  unsigned char* d = (unsigned char*)ctx->scratch.reserve(blob.size());
-  upload_async(ctx, d, blob.data(), blob.size());
+  return r;
+}
+
+// The plan of `h` with its table pointers into device copy `d` of the blob.
+inline AcPlanDev bind_plan(const AcHostPlan& h, unsigned char* d, const AcOffsets& off) {
+  const size_t* o = off.o;
   AcPlanDev p = h.dev;
   p.steps = (const AcStepD*)(d + o[0]);
   p.step_event = (const int32_t*)(d + o[1]);
@@ -626,6 +850,24 @@ inline AcPlanDev upload(kg_context* ctx, AcHostPlan& h) {
   p.events32 = (const float*)(d + o[10]);
   p.intervals32 = (const float*)(d + o[11]);
   return p;
+}
+
+inline AcPlanDev upload(kg_context* ctx, AcHostPlan& h) {
+  std::vector<unsigned char> blob;
+  const AcOffsets off = stage_plan(blob, h);
+  unsigned char* d = (unsigned char*)ctx->scratch.reserve(blob.size());
+  upload_async(ctx, d, blob.data(), blob.size());
+  return bind_plan(h, d, off);
+}
+
+// Whether two sampled plans share the step grid the lane loops walk.
+inline bool same_grid(const AcHostPlan& a, const AcHostPlan& b) {
+  if (a.dev.n_steps != b.dev.n_steps || a.dev.bridge != b.dev.bridge || a.step_event != b.step_event ||
+      a.step_observe != b.step_observe || a.intervals.size() != b.intervals.size())
+    return false;
+  for (size_t k = 0; k < a.intervals.size(); ++k)
+    if (a.intervals[k].end != b.intervals[k].end) return false;
+  return true;
 }
 
 }  // namespace kg

@@ -236,7 +236,11 @@ struct Moments {
 
 __device__ inline Moments moments_zero() { return Moments{0.0, 0.0, 0.0, 0.0}; }
 
+// The moment updates round every product and sum on its own, as the
+// reference's Rust does (no contraction into FMAs): the result is then the
+// same in every kernel that inlines them.
 __device__ inline void moments_add(Moments& m, double value, bool variance) {
+#pragma clang fp contract(off)
   double a = fabs(value);
   if (a > m.scale) {
     double ratio = m.scale / a;
@@ -252,6 +256,7 @@ __device__ inline void moments_add(Moments& m, double value, bool variance) {
 }
 
 __device__ inline Moments moments_merge(const Moments& a, const Moments& b) {
+#pragma clang fp contract(off)
   if (a.count == 0.0) return b;
   if (b.count == 0.0) return a;
   double sc = fmax(a.scale, b.scale);
@@ -339,6 +344,7 @@ __device__ inline void block_merge(Moments (&m)[N], Moments* smem) {
 template <int N, int K>
 __device__ inline void block_moments(const double (&v)[N][K], const int (&n)[N], Moments (&out)[N],
                                      double* smem) {
+#pragma clang fp contract(off)
   const int lane = threadIdx.x & (WAVE - 1);
   const int wave = threadIdx.x / WAVE;
   const int waves = blockDim.x / WAVE;

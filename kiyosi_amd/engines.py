@@ -13,7 +13,7 @@ import ctypes as C
 import os
 
 from . import _abi, _native
-from .errors import InvalidInputError
+from .errors import InvalidInputError, UnsupportedOperationError
 from .instruments import EuropeanOption, _Autocallable
 
 _SAMPLING = {"pseudo_random": _abi.PSEUDO_RANDOM, "sobol": _abi.SOBOL}
@@ -100,6 +100,18 @@ class _MonteCarloEngine:
         estimate = self.price_with_statistics(instrument, context)
         return estimate.price, estimate.seed
 
+    def _scenarios(self, entry, terms, scenarios):
+        """Prices of `scenarios` (pricing contexts) from one shared-draw batch
+        entry point, or None when the batch does not apply to them."""
+        markets = (_abi.BsmMarket * len(scenarios))(*[ctx.bsm_market() for ctx in scenarios])
+        out = (_abi.Estimate * len(scenarios))()
+        ctx = _native.context(self.device)
+        status = entry(ctx.handle, C.byref(self._settings()), C.byref(terms), markets, len(scenarios), out)
+        if status == _abi.UNSUPPORTED_OPERATION:
+            return None
+        ctx.check(status)
+        return [(o.price, o.seed if o.has_seed else None) for o in out]
+
     def price(self, instrument, context) -> float:
         return self.price_with_statistics(instrument, context).price
 
@@ -140,27 +152,15 @@ class MonteCarloVanillaEngine(_MonteCarloEngine):
         engines/monte_carlo.rs:1334-1409). Returns a PricingResult whose
         values equal numerical_greeks(engine, ...) with the same seed."""
         from . import greeks
-        from .instruments import EuropeanOption
         if not isinstance(option, EuropeanOption):
             return greeks.numerical_greeks(self, option, context, request, settings)
-        settings = settings or greeks.NumericalShiftSettings()
         engine = self._resolved()
-        scenarios = greeks.plan(option, context, request, settings)
-        if len(scenarios) > _abi.MAX_SCENARIOS:
-            return greeks.numerical_greeks(engine, option, context, request, settings)
-        markets = (_abi.BsmMarket * len(scenarios))(*[ctx.bsm_market() for ctx in scenarios])
-        out = (_abi.Estimate * len(scenarios))()
-        ctx = _native.context(engine.device)
-        status = _native.lib().kg_vanilla_scenarios(ctx.handle, C.byref(engine._settings()),
-                                                    C.byref(option._terms()), markets, len(scenarios), out)
-        if status == _abi.UNSUPPORTED_OPERATION:  # peak-scaled payoffs
-            return greeks.numerical_greeks(engine, option, context, request, settings)
-        ctx.check(status)
-        prices = {greeks.key(scn): out[i].price for i, scn in enumerate(scenarios)}
-        values, missing = greeks.complete(option, context, request, settings, out[0].price,
-                                          lambda scn: prices[greeks.key(scn)])
-        seed = engine.seed if any(out[i].has_seed for i in range(len(scenarios))) else None
-        return greeks.PricingResult(out[0].price, greeks.normalize_request(request), values, missing, seed)
+
+        def batch(scenarios):
+            if len(scenarios) > _abi.MAX_SCENARIOS:
+                return None
+            return engine._scenarios(_native.lib().kg_vanilla_scenarios, option._terms(), scenarios)
+        return _batched_greeks(engine, option, context, request, settings, lambda scenarios: [scenarios], batch)
 
     def american_report(self, option, context, *, training_path_count=None, basis="quadratic",
                         epochs=4, batch=16384, learning_rate=1e-3):
@@ -195,6 +195,37 @@ class MonteCarloAutocallableEngine(_MonteCarloEngine):
         if not isinstance(instrument, _Autocallable):
             raise TypeError("MonteCarloAutocallableEngine prices autocallable notes")
         return _native.lib().kg_autocallable_price, instrument._terms(), context.bsm_market()
+
+    def _resolved(self):
+        return self if self.seed is not None else self.with_resolved_seed()
+
+    def price_with_greeks(self, note, context, request=None, settings=None):
+        """Price and Greeks of an autocallable note by central differences on
+        shared draws (price_with_greeks_with_settings,
+        engines/monte_carlo/autocallable.rs:919-999): the stencil's scenarios
+        at each valuation time are priced as lanes of one path set (spot_prices
+        :174-197, market_prices :199-241). Each price equals pricing its
+        scenario alone with the same seed, so the result equals
+        numerical_greeks(engine, ...)."""
+        from . import greeks
+        if context.model_parameters.volatility == 0.0:
+            raise UnsupportedOperationError(
+                "zero volatility selects the deterministic engine, which is not part of the GPU engine")
+        engine = self._resolved()
+        terms = note._terms()
+
+        def groups(scenarios):
+            by_time = {}
+            for scn in scenarios:
+                by_time.setdefault(greeks.key(scn)[4], []).append(scn)
+            out = []
+            for same in by_time.values():
+                out += [same[i:i + _abi.MAX_AUTOCALL_SCENARIOS]
+                        for i in range(0, len(same), _abi.MAX_AUTOCALL_SCENARIOS)]
+            return out
+        return _batched_greeks(engine, note, context, request, settings, groups,
+                               lambda scenarios: engine._scenarios(_native.lib().kg_autocallable_scenarios,
+                                                                   terms, scenarios))
 
 
 class MonteCarloBarrierEngine(_MonteCarloEngine):
@@ -242,6 +273,27 @@ _BOOK_KINDS = {
     "MonteCarloAutocallableEngine": _abi.BOOK_AUTOCALL,
     "MonteCarloAccumulatorEngine": _abi.BOOK_ACCUMULATOR,
 }
+
+
+def _batched_greeks(engine, option, context, request, settings, groups, batch):
+    """Plans the stencil, prices its scenarios group by group through
+    `batch` (falling back to one call per scenario where a batch does not
+    apply) and completes the Greeks from those prices."""
+    from . import greeks
+    settings = settings or greeks.NumericalShiftSettings()
+    scenarios = greeks.plan(option, context, request, settings)
+    prices, seed = {}, None
+    for group in groups(scenarios):
+        results = batch(group)
+        if results is None:
+            results = [engine.price_with_seed(option, scn) for scn in group]
+        for scn, (price, used) in zip(group, results):
+            prices[greeks.key(scn)] = price
+            seed = used if used is not None else seed
+    p0 = prices[greeks.key(context)]
+    values, unavailable = greeks.complete(option, context, request, settings, p0,
+                                          lambda scn: prices[greeks.key(scn)])
+    return greeks.PricingResult(p0, greeks.normalize_request(request), values, unavailable, seed)
 
 
 class Book:

@@ -54,13 +54,17 @@ class NumericalShiftSettings:
 
 
 class PricingResult:
-    """Price, requested Greeks and the seed that reproduces them."""
+    """Price, requested Greeks and the seed that reproduces them. A requested
+    Greek without a value is "stencil_unavailable" when its stencil could not
+    be formed (other bump sizes may recover it) and "undefined" when the
+    derivative does not exist at this valuation (PricingResult::status,
+    risk/analytics.rs:195-207)."""
 
     def __init__(self, price, requested, values, unavailable, seed):
         self.price = price
         self.requested = frozenset(requested)
         self._values = dict(values)
-        self._unavailable = frozenset(unavailable)
+        self._unavailable = frozenset(unavailable) - set(self._values)
         self.seed = seed
 
     def get(self, greek):
@@ -72,18 +76,33 @@ class PricingResult:
         return self._values[greek]
 
     def status(self, greek):
-        if greek in self._values:
-            return "available"
         if greek not in self.requested:
             return "not_requested"
-        return "stencil_unavailable"
+        if greek in self._values:
+            return "available"
+        return "stencil_unavailable" if greek in self._unavailable else "undefined"
 
     def __eq__(self, other):
         return (isinstance(other, PricingResult) and self.price == other.price and self._values == other._values
-                and self.seed == other.seed)
+                and self._unavailable == other._unavailable and self.seed == other.seed)
 
     def __repr__(self):
         return f"PricingResult(price={self.price!r}, greeks={self._values!r}, seed={self.seed!r})"
+
+
+def _at_spot_discontinuity(option, context, radius):
+    hook = getattr(option, "at_spot_discontinuity", None)
+    return bool(hook(context, radius)) if hook else False
+
+
+def _all_greeks_unavailable(option, context):
+    """OptionContract::all_greeks_unavailable's default (instruments/vanilla.rs:161-166)."""
+    return context.model_parameters.volatility == 0.0 and _at_spot_discontinuity(option, context, 0.0)
+
+
+def _time_stencil_available(option, context, before, after):
+    hook = getattr(option, "time_stencil_available", None)
+    return bool(hook(context, before, after)) if hook else True
 
 
 def normalize_request(request):
@@ -126,7 +145,8 @@ def key(context):
 def time_stencil(option, context, settings):
     """(before, after) valuation times in microseconds, or None
     (numerical.rs:212-273 for flat curves, whose calendar variance time has
-    no transitions)."""
+    no transitions). Automatic bumps halve until the contract accepts the
+    stencil (no observation inside it)."""
     time = to_micros(context.valuation_time)
     start = getattr(option, "valuation_start", None)
     lower = to_micros(start) if start is not None else _FAR_PAST
@@ -143,12 +163,54 @@ def time_stencil(option, context, settings):
         shift = min(shift, before_width, after_width)
     before = time - min(shift, before_width)
     after = time + min(shift, after_width)
-    return (before, after) if before != after else None
+    while settings.time_shift is None and shift > 1 and not _time_stencil_available(option, context, before, after):
+        shift //= 2
+        before = time - min(shift, before_width)
+        after = time + min(shift, after_width)
+    if before != after and _time_stencil_available(option, context, before, after):
+        return before, after
+    return None
+
+
+def expiry_greeks(option, context, request, settings, p0, price):
+    """Greeks at expiry from terminal-payoff stencils (numerical.rs:28-112):
+    vega, volga, rho and dividend rho are zero; vanna and zomma are zero where
+    the spot stencil exists; delta, gamma and speed come from terminal values
+    at bumped spots; theta, charm and color stay undefined."""
+    values, unavailable = {}, set()
+    if not request:
+        return values, unavailable
+    for g in ("vega", "volga", "rho", "dividend_rho"):
+        if g in request:
+            values[g] = 0.0
+    if not _at_spot_discontinuity(option, context, 0.0):
+        unavailable |= {"delta", "gamma", "speed", "vanna", "zomma"}
+    spot = context.spot_price
+    h = symmetric_shift(spot, settings.spot_shift_at(spot))
+    if h <= 0.0 or spot <= h or _at_spot_discontinuity(option, context, h):
+        return values, unavailable
+    for g in ("vanna", "zomma"):
+        if g in request:
+            values[g] = 0.0
+    if request & {"delta", "gamma", "speed"}:
+        up, down = price(context.replace(spot_price=spot + h)), price(context.replace(spot_price=spot - h))
+        if "delta" in request:
+            values["delta"] = (up - down) / h / 2.0
+        if "gamma" in request:
+            values["gamma"] = ((up - p0) - (p0 - down)) / h / h
+        two_h = 2.0 * h
+        if ("speed" in request and math.isfinite(two_h) and spot > two_h and symmetric_shift(spot, two_h) == two_h
+                and not _at_spot_discontinuity(option, context, two_h)):
+            up2 = price(context.replace(spot_price=spot + two_h))
+            down2 = price(context.replace(spot_price=spot - two_h))
+            values["speed"] = ((up2 - up) - (up - down) + (down - down2)) / h / h / two_h
+    return values, unavailable
 
 
 def complete(option, context, request, settings, p0, price):
-    """Greeks of `request` from the stencil prices; `price(context)` prices
-    one scenario. Returns ({greek: value}, {greek not computable})."""
+    """Greeks of `request` from the stencil prices (complete_greeks_with_pair,
+    numerical.rs:331-640); `price(context)` prices one scenario. Returns
+    ({greek: value}, {greeks whose stencil is unavailable})."""
     request = normalize_request(request)
     model = context.model_parameters
     if not isinstance(model, BlackScholesMertonParameters):
@@ -156,15 +218,17 @@ def complete(option, context, request, settings, p0, price):
     time = to_micros(context.valuation_time)
     values = {}
     if time == to_micros(option.expiry_time):
-        from .errors import UnsupportedOperationError
-        raise UnsupportedOperationError("Greeks at expiry use the terminal stencil, which this engine does not provide")
+        return expiry_greeks(option, context, request, settings, p0, price)
+    if not request or _all_greeks_unavailable(option, context):
+        return values, set()
     spot, vol = context.spot_price, model.volatility
     h = symmetric_shift(spot, settings.spot_shift_at(spot))
     v = symmetric_shift(vol, settings.volatility_shift_at(vol)) if vol > 0.0 else 0.0
-    spot_ok = any(g in request for g in _SPOT) and h > 0.0 and spot > h
+    spot_ok = (any(g in request for g in _SPOT) and h > 0.0 and spot > h
+               and not _at_spot_discontinuity(option, context, h))
     two_h = 2.0 * h
     speed_ok = ("speed" in request and spot_ok and math.isfinite(two_h) and spot > two_h
-                and symmetric_shift(spot, two_h) == two_h)
+                and symmetric_shift(spot, two_h) == two_h and not _at_spot_discontinuity(option, context, two_h))
 
     def at(s=None, sigma=None, valuation=None):
         return price(scenario(context, spot=s, volatility=sigma, valuation=valuation))
@@ -240,7 +304,11 @@ def complete(option, context, request, settings, p0, price):
                     values["charm"] = (late[0] - early[0]) / years
                 if "color" in request:
                     values["color"] = (late[1] - early[1]) / years
-    return values, request - set(values)
+    spot_undefined = _at_spot_discontinuity(option, context, 0.0)
+    unavailable = {g for g in GREEKS
+                   if not ((g in _SPOT and spot_undefined)
+                           or (g in ("vega", "volga", "vanna", "zomma") and vol == 0.0))}
+    return values, unavailable
 
 
 def plan(option, context, request, settings):
@@ -255,17 +323,21 @@ def plan(option, context, request, settings):
 
 
 def numerical_greeks(engine, option, context, request=None, settings=None):
-    """Greeks by pricing every stencil scenario with `engine.price` on the
-    engine's resolved seed (risk/numerical.rs:164-210)."""
+    """Greeks by pricing every stencil scenario with `engine.price_with_seed`
+    on the engine's resolved seed (risk/numerical.rs:164-210); the result
+    carries the seed when any scenario sampled."""
     settings = settings or NumericalShiftSettings()
     engine = engine if engine.seed is not None else engine.with_resolved_seed()
-    cache = {}
+    cache, seed = {}, [None]
 
     def price(ctx):
         k = key(ctx)
         if k not in cache:
-            cache[k] = engine.price(option, ctx)
+            value, used = engine.price_with_seed(option, ctx)
+            cache[k] = value
+            if used is not None:
+                seed[0] = used
         return cache[k]
     p0 = price(context)
-    values, missing = complete(option, context, request, settings, p0, price)
-    return PricingResult(p0, normalize_request(request), values, missing, engine.seed)
+    values, unavailable = complete(option, context, request, settings, p0, price)
+    return PricingResult(p0, normalize_request(request), values, unavailable, seed[0])

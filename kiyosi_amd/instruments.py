@@ -11,8 +11,8 @@ import datetime as _dt
 import math
 
 from . import _abi
-from .errors import InvalidInputError
-from .market import TradingCalendar, clock_micros, to_micros
+from .errors import InvalidInputError, MissingHistoryError
+from .market import MICROS_PER_DAY, TradingCalendar, clock_micros, to_micros
 
 
 def _positive(name, value):
@@ -45,6 +45,13 @@ class EuropeanOption:
     def _terms(self):
         return _abi.EuropeanTerms(1 if self.option_type == "call" else -1, 0, self.strike,
                                   to_micros(self.effective_time), to_micros(self.expiry_time))
+
+    def at_spot_discontinuity(self, context, radius):
+        """The terminal kink at the strike (instruments/vanilla.rs:201-206)."""
+        spot = context.spot_price
+        return ((to_micros(context.valuation_time) == to_micros(self.expiry_time)
+                 and spot - radius <= self.strike <= spot + radius)
+                or context.deterministic_crosses(self.expiry_time, self.strike, radius))
 
 
 class KnockInObservation:
@@ -175,6 +182,120 @@ class _Autocallable:
             expiry_time=to_micros(self.expiry_time), events=C.cast(array, C.POINTER(_abi.AutocallEvent)))
         terms._keep = array
         return terms
+
+    # Greek-stencil hooks of OptionContract for autocallable notes
+    # (instruments/autocallable.rs:1426-1583).
+    @property
+    def valuation_start(self):
+        return self.effective_time
+
+    def _period_coupon(self):
+        return self._coupon() if self._coupon_kind == _abi.COUPON_PHOENIX else 0.0
+
+    def _daily_clock(self):
+        ki = self.knock_in_observation
+        if self.knock_in_level is None or ki is None or ki.kind() != "every_trading_day":
+            return None
+        return clock_micros(ki.observation_clock_time())
+
+    def _knock_in_affects_payoff(self):
+        """knock_in_affects_payoff (instruments/autocallable.rs:1075-1080)."""
+        if self.knock_in_level is None:
+            return False
+        maturity = self._coupon() if self._coupon_kind == _abi.COUPON_SNOWBALL else 0.0
+        if self._redemption_kind == _abi.REDEMPTION_FIXED_COUPON:
+            return self.minimum_coupon_rate != maturity
+        return self.lower_strike != self.upper_strike or maturity != 0.0
+
+    def _has_daily_observation(self, start, end):
+        """A daily knock-in observation in [start, end] (:1148-1159)."""
+        clock = self._daily_clock()
+        if clock is None:
+            return False
+        start = max(start, to_micros(self.effective_time))
+        end = min(end, to_micros(self.expiry_time))
+        for day in range(start // MICROS_PER_DAY, end // MICROS_PER_DAY + 1):
+            at = day * MICROS_PER_DAY + clock
+            if start <= at <= end and self.calendar.is_trading_day_number(day):
+                return True
+        return False
+
+    def _observes_knock_in(self, time):
+        clock = self._daily_clock()
+        if clock is not None:
+            return (to_micros(self.effective_time) <= time <= to_micros(self.expiry_time)
+                    and time % MICROS_PER_DAY == clock
+                    and self.calendar.is_trading_day_number(time // MICROS_PER_DAY))
+        return (self.knock_in_level is not None and self.knock_in_observation is not None
+                and self.knock_in_observation.kind() == "at_expiry" and time == to_micros(self.expiry_time))
+
+    def _prior(self, time):
+        """prior_with_work_check (:1320-1369): checks the stated barrier state
+        against the observations before `time`."""
+        outs = [e[0] for e in self._events() if e[1]]
+        had_out = bool(outs) and outs[0] < time
+        effective = to_micros(self.effective_time)
+        had_in = (self._daily_clock() is not None and time > effective
+                  and self._has_daily_observation(effective, min(time - 1, to_micros(self.expiry_time))))
+        if self.barrier_state is None and (had_out or had_in):
+            raise MissingHistoryError("prior autocallable state is required")
+        if self.barrier_state == "knocked_out" and not had_out:
+            raise InvalidInputError("knock-out predates observations")
+        if self.barrier_state == "knocked_in" and not had_in:
+            raise InvalidInputError("knock-in predates monitoring")
+        return self.barrier_state == "knocked_in"
+
+    def time_stencil_available(self, context, before, after):
+        """No observation inside a time stencil [before, after] (:1556-1583)."""
+        try:
+            self._prior(before)
+        except (InvalidInputError, MissingHistoryError):
+            return False
+        if self.barrier_state == "knocked_out":
+            return True
+        if any(before <= e[0] <= after for e in self._events()):
+            return False
+        return not (self._daily_clock() is not None and self.barrier_state != "knocked_in"
+                    and self._knock_in_affects_payoff() and self._has_daily_observation(before, after))
+
+    def at_spot_discontinuity(self, context, radius):
+        """Whether a spot interval of `radius` crosses a level observed at the
+        valuation or a deterministic one (:1438-1554)."""
+        if self.barrier_state == "knocked_out":
+            return False
+        time = to_micros(context.valuation_time)
+        until = context.deterministic_until(self.expiry_time)
+        for e in self._events():
+            if e[0] <= time or e[0] > until:
+                continue
+            if (e[1] and context.forward_crosses(e[0], e[3], radius)) or (
+                    self._period_coupon() != 0.0 and context.forward_crosses(e[0], e[4] if e[2] else 0.0, radius)):
+                return True
+        if until > time and self._knock_in_affects_payoff():
+            from .errors import UnsupportedOperationError
+            raise UnsupportedOperationError("deterministic knock-in paths (zero volatility) are not supported")
+        spot = context.spot_price
+
+        def crosses(level):
+            return spot - radius <= level <= spot + radius
+        now = [e for e in self._events() if e[0] == time]
+        if now:
+            e = now[0]
+            if self._period_coupon() != 0.0 and e[2] and e[4] > 0.0 and crosses(e[4]):
+                return True
+            if e[1] and crosses(e[3]):
+                return True
+            if e[1] and spot > e[3]:
+                return False
+        if self.knock_in_level is not None and time == to_micros(self.expiry_time):
+            knocked = self.barrier_state == "knocked_in" or (self._observes_knock_in(time)
+                                                             and spot < self.knock_in_level)
+            if knocked and self._redemption_kind == _abi.REDEMPTION_DOWNSIDE and (
+                    crosses(self.upper_strike) or crosses(self.lower_strike)):
+                return True
+        return (self.barrier_state != "knocked_in" and self._knock_in_affects_payoff()
+                and self._observes_knock_in(time) and self.knock_in_level is not None
+                and crosses(self.knock_in_level))
 
 
 class SnowballOption(_Autocallable):

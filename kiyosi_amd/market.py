@@ -63,6 +63,10 @@ class TradingCalendar:
     def is_trading_day(self, date) -> bool:
         return self.kind == self.ALL_DAYS or date.weekday() < 5
 
+    def is_trading_day_number(self, day: int) -> bool:
+        """Trading-day test of day `day` since 1970-01-01 (a Thursday)."""
+        return self.kind == self.ALL_DAYS or (day + 3) % 7 < 5
+
     def add_trading_days(self, start, days: int):
         """The date `days` trading days after `start` (a datetime keeps its clock)."""
         current = start
@@ -119,6 +123,48 @@ class PricingContext:
             raise UnsupportedOperationError("this engine needs Black-Scholes-Merton parameters")
         return _abi.BsmMarket(self.spot_price, model.risk_free_rate, model.dividend_yield,
                               model.volatility, to_micros(self.valuation_time))
+
+    # Deterministic-forward queries of PricingContext (market/parameters.rs:
+    # 386-400, 460-484) for flat BSM curves, which quote no discount factors,
+    # under Actual/365 Fixed variance time, which starts at the valuation.
+    def deterministic_until(self, end) -> int:
+        """End of the period without variance: `end` at zero volatility,
+        else the valuation time (variance_start, time/convention.rs:119-135)."""
+        end = to_micros(end)
+        if self.model_parameters.volatility == 0.0:
+            return end
+        return min(to_micros(self.valuation_time), end)
+
+    def forward_crosses(self, time, level, radius) -> bool:
+        """Whether the forward at `time` of spots in [spot - radius, spot +
+        radius] reaches `level` from both sides."""
+        model = self.model_parameters
+        carry = (model.risk_free_rate - model.dividend_yield) * years_until(self.valuation_time, time)
+
+        def compare(spot):
+            if spot > 0.0 and level > 0.0:
+                x = log_ratio(spot, level) + carry
+            else:
+                x = spot - level
+            return None if math.isnan(x) else (x > 0.0) - (x < 0.0)
+        low, high = compare(max(self.spot_price - radius, 0.0)), compare(self.spot_price + radius)
+        return low is not None and high is not None and low <= 0 and high >= 0
+
+    def deterministic_crosses(self, time, level, radius) -> bool:
+        time = to_micros(time)
+        return (time >= to_micros(self.valuation_time) and self.deterministic_until(time) == time
+                and self.forward_crosses(time, level, radius))
+
+
+def log_ratio(numerator, denominator):
+    """ln(numerator / denominator), accurate near 1 (numerics.rs:80-92)."""
+    relative = (numerator - denominator) / denominator
+    if abs(relative) < 0.5:
+        return math.log1p(relative)
+    ratio = numerator / denominator
+    if ratio != 0.0 and math.isfinite(ratio) and abs(ratio) >= 2.2250738585072014e-308:
+        return math.log(ratio)
+    return math.log(numerator) - math.log(denominator)
 
 
 class LocalVolatilitySurface:

@@ -61,9 +61,10 @@ class OraclePricer:
     def with_resolved_seed(self):
         return self
 
-    def price(self, opt, ctx):
+    def price_with_seed(self, opt, ctx):
         s = _abi.Settings(self.paths, self.seed, 1, _abi.PSEUDO_RANDOM, int(self.antithetic), _abi.FP64, 0, 0)
-        return self.oracle.vanilla_price(s, opt._terms(), ctx.bsm_market(), workers=16).price
+        out = self.oracle.vanilla_price(s, opt._terms(), ctx.bsm_market(), workers=16)
+        return out.price, (out.seed if out.has_seed else None)
 
 
 def test_symmetric_shift_and_default_bumps():
@@ -87,10 +88,76 @@ def test_time_stencil_is_centred_one_day_and_shrinks_near_expiry():
     assert after - to_micros(late) == (to_micros(END) - to_micros(late)) // 8 == to_micros(late) - before
 
 
+def test_expiry_greeks_use_terminal_stencils():
+    """At expiry: vega, volga, rho and dividend rho are zero; delta and gamma
+    come from intrinsic values at bumped spots; at the strike the kink leaves
+    the spot Greeks undefined; theta stays undefined."""
+    ctx = context(spot=110.0, valuation=END)
+    opt = option("call", 100.0)
+
+    def intrinsic(c):
+        return max(c.spot_price - 100.0, 0.0)
+    values, unavailable = greeks.complete(opt, ctx, None, greeks.NumericalShiftSettings(), 10.0, intrinsic)
+    assert values["vega"] == values["rho"] == values["volga"] == values["dividend_rho"] == 0.0
+    assert values["delta"] == pytest.approx(1.0) and values["gamma"] == pytest.approx(0.0, abs=1e-9)
+    assert values["vanna"] == 0.0 and "theta" not in values
+    res = greeks.PricingResult(10.0, greeks.GREEKS, values, unavailable, None)
+    assert res.status("theta") == "undefined"
+    at_strike, unavailable = greeks.complete(opt, context(spot=100.0, valuation=END), None,
+                                             greeks.NumericalShiftSettings(), 0.0, intrinsic)
+    assert "delta" not in at_strike and "delta" not in unavailable  # undefined at the kink
+    assert at_strike["vega"] == 0.0
+
+
 def test_full_stencil_has_twenty_one_scenarios():
     scenarios = greeks.plan(option(), context(), None, greeks.NumericalShiftSettings())
     assert len(scenarios) == 21 and greeks.key(scenarios[0]) == greeks.key(context())
     assert len(greeks.plan(option(), context(), "delta", greeks.NumericalShiftSettings())) == 3
+
+
+def snowball_at(valuation, clock=dt.time(15, 0), state="no_event"):
+    from kiyosi_amd.instruments import KnockInObservation, SnowballObservation, SnowballOption
+    from kiyosi_amd.market import TradingCalendar
+    cal = TradingCalendar.weekdays()
+    dates = [cal.add_trading_days(START, 21 * k) for k in range(1, 13)]
+    return SnowballOption(maturity_coupon_rate=0.15, initial_spot=100.0, knock_in_level=75.0,
+                          upper_strike=100.0, lower_strike=0.0,
+                          knock_in_observation=KnockInObservation.every_trading_day(clock),
+                          effective_time=START, expiry_time=dates[-1],
+                          autocall_observations=[SnowballObservation(d, 100.0, 0.15) for d in dates],
+                          barrier_state=state, calendar=cal)
+
+
+def test_autocallable_time_stencil_halves_until_no_observation_lies_inside():
+    """Monday 12:00 with daily knock-in at 15:00: one day, 12 h, 6 h and 3 h
+    windows all hold a 15:00 observation (inclusive); 1.5 h does not
+    (numerical.rs:258-265, instruments/autocallable.rs:1556-1583)."""
+    t = dt.datetime(2025, 1, 6, 12, 0)
+    note = snowball_at(t)
+    before, after = greeks.time_stencil(note, context(valuation=t), greeks.NumericalShiftSettings())
+    assert after - to_micros(t) == to_micros(t) - before == 5_400_000_000
+    # An explicit bump is not shortened, and a stencil holding an observation is unavailable.
+    fixed = greeks.NumericalShiftSettings(time_shift=dt.timedelta(hours=6))
+    assert greeks.time_stencil(note, context(valuation=t), fixed) is None
+    # Knock-in history that cannot hold before the stencil's start makes it unavailable.
+    assert not snowball_at(t, state=None).time_stencil_available(context(valuation=t), to_micros(t) - 1,
+                                                                 to_micros(t) + 1)
+
+
+def test_autocallable_spot_discontinuities():
+    """An observation at the valuation whose knock-out level lies within the
+    spot stencil, and a daily knock-in observed now within it
+    (instruments/autocallable.rs:1518-1554)."""
+    from kiyosi_amd.market import TradingCalendar
+    obs_day = TradingCalendar.weekdays().add_trading_days(START, 21)
+    note = snowball_at(obs_day, clock=dt.time(0, 0))
+    near = context(spot=100.005, valuation=obs_day)
+    assert note.at_spot_discontinuity(near, 0.01) and not note.at_spot_discontinuity(near, 0.001)
+    assert not note.at_spot_discontinuity(context(spot=101.0, valuation=obs_day), 0.01)
+    day = dt.datetime(2025, 1, 7)
+    assert note.at_spot_discontinuity(context(spot=75.004, valuation=day), 0.01)
+    assert not note.at_spot_discontinuity(context(spot=75.004, valuation=day + dt.timedelta(hours=1)), 0.01)
+    assert not snowball_at(day, state="knocked_out").at_spot_discontinuity(context(spot=75.0, valuation=day), 1.0)
 
 
 @pytest.mark.parametrize("kind", ["call", "put"])
