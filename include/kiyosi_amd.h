@@ -269,8 +269,8 @@ kg_status kg_heston_price(kg_context* ctx, const kg_settings* settings,
  * cumulative noise against the barrier on each observation date (knock-ins
  * with more than 8 fixings left are conditioned on the hit with a lognormal
  * price, :320-333); continuous monitoring draws the terminal value and
- * weights it by the Brownian-bridge survival (:402-440). Values only: the
- * likelihood-ratio Greeks of :459-472 are not computed. */
+ * weights it by the Brownian-bridge survival (:402-440). Delta and gamma:
+ * kg_barrier_price_with_greeks below. */
 enum { KG_DOWN_AND_OUT = 0, KG_DOWN_AND_IN = 1, KG_UP_AND_OUT = 2, KG_UP_AND_IN = 3 };
 enum { KG_OBSERVE_CONTINUOUS = 0, KG_OBSERVE_SCHEDULED = 1 };
 enum { KG_REBATE_AT_EXPIRY = 0, KG_REBATE_AT_HIT = 1 };
@@ -293,6 +293,90 @@ typedef struct kg_barrier_terms {
 kg_status kg_barrier_price(kg_context* ctx, const kg_settings* settings,
                            const kg_barrier_terms* option, const kg_bsm_market* market,
                            kg_estimate* out);
+
+/* ---- Spot Greeks of the barrier and digital engines: the likelihood-ratio
+ * delta and gamma that MonteCarloBarrierEngine (barrier.rs:194-205, 440-472)
+ * and MonteCarloDigitalEngine (digital.rs:116-136) compute inside the path
+ * closure, on the draws of the price. `request` selects them; `spot_radius`
+ * is the symmetric spot shift of the caller's stencil
+ * (symmetric_shift(spot, spot_shift_at(spot)), barrier.rs:186-193): a
+ * monitored barrier level within it makes the spot Greeks unavailable
+ * (at_spot_discontinuity, instruments/monitoring.rs:430-445). Greeks are
+ * never computed at expiry. `available` reports which were set; a barrier
+ * resolved to its vanilla payoff reports the vanilla's analytic delta and
+ * gamma, other resolved contracts zero (barrier.rs:206-229). */
+enum { KG_GREEK_DELTA = 1, KG_GREEK_GAMMA = 2 };
+
+typedef struct kg_spot_greeks {
+  int32_t request;   /* in: KG_GREEK_* bits */
+  int32_t available; /* out: KG_GREEK_* bits of the values set */
+  double spot_radius;
+  double delta;
+  double gamma;
+} kg_spot_greeks;
+
+/* Single barrier with its spot Greeks:
+ * MonteCarloBarrierEngine::price_with_greeks_with_settings (barrier.rs:618-640).
+ * out->price is bit-identical to kg_barrier_price on the same inputs. */
+kg_status kg_barrier_price_with_greeks(kg_context* ctx, const kg_settings* settings,
+                                       const kg_barrier_terms* option, const kg_bsm_market* market,
+                                       kg_spot_greeks* greeks, kg_estimate* out);
+
+/* ---- Double barrier options: MonteCarloBarrierEngine for
+ * DoubleBarrierOption (barrier.rs:14-18, 160-541, 648). The contract is its
+ * lower (down-type) barrier plus the upper level
+ * (instruments/double_barrier.rs:119-144). Scheduled fixings knock on
+ * either level; continuous monitoring weights the terminal draw by the
+ * image/spectral survival series of double_barrier.rs:6-112. Continuous
+ * at-hit knock-out rebates need the analytic first-exit value and return
+ * KG_UNSUPPORTED_OPERATION, as for single barriers. */
+enum { KG_DOUBLE_KNOCK_OUT = 0, KG_DOUBLE_KNOCK_IN = 1 };
+
+typedef struct kg_double_barrier_terms {
+  int32_t option_type;   /* 1 call, -1 put */
+  int32_t knock_type;    /* KG_DOUBLE_KNOCK_* */
+  int32_t observation;   /* KG_OBSERVE_* */
+  int32_t rebate_timing; /* KG_REBATE_*; at-hit knock-in rebates are invalid */
+  double strike;
+  double lower_barrier;  /* 0 < lower < upper */
+  double upper_barrier;
+  double rebate;
+  int64_t effective_time;
+  int64_t expiry_time;
+  int32_t n_observations; /* scheduled fixings, strictly increasing */
+  int32_t reserved;
+  const int64_t* observation_times;
+} kg_double_barrier_terms;
+
+kg_status kg_double_barrier_price(kg_context* ctx, const kg_settings* settings,
+                                  const kg_double_barrier_terms* option, const kg_bsm_market* market,
+                                  kg_estimate* out);
+kg_status kg_double_barrier_price_with_greeks(kg_context* ctx, const kg_settings* settings,
+                                              const kg_double_barrier_terms* option,
+                                              const kg_bsm_market* market, kg_spot_greeks* greeks,
+                                              kg_estimate* out);
+
+/* ---- European digitals: MonteCarloDigitalEngine (engines/monte_carlo/
+ * digital.rs:55-150) for DigitalOption (instruments/vanilla.rs:236-366).
+ * Cash-or-nothing pays cash_payout, asset-or-nothing the terminal spot,
+ * when the terminal value is strictly in the money in standardised
+ * coordinates. The valuation may precede the effective time. */
+enum { KG_CASH_OR_NOTHING = 0, KG_ASSET_OR_NOTHING = 1 };
+
+typedef struct kg_digital_terms {
+  int32_t option_type; /* 1 call, -1 put */
+  int32_t payout_kind; /* KG_CASH_OR_NOTHING or KG_ASSET_OR_NOTHING */
+  double strike;
+  double cash_payout;  /* nonnegative; ignored for asset settlement */
+  int64_t effective_time;
+  int64_t expiry_time;
+} kg_digital_terms;
+
+kg_status kg_digital_price(kg_context* ctx, const kg_settings* settings, const kg_digital_terms* option,
+                           const kg_bsm_market* market, kg_estimate* out);
+kg_status kg_digital_price_with_greeks(kg_context* ctx, const kg_settings* settings,
+                                       const kg_digital_terms* option, const kg_bsm_market* market,
+                                       kg_spot_greeks* greeks, kg_estimate* out);
 
 /* ---- Average-price options: MonteCarloAsianEngine
  * (engines/monte_carlo/asian.rs:374-1257). Geometric averages sample the
@@ -353,7 +437,9 @@ enum {
   KG_BOOK_BARRIER = 0,
   KG_BOOK_ASIAN = 1,
   KG_BOOK_AUTOCALL = 2,
-  KG_BOOK_ACCUMULATOR = 3
+  KG_BOOK_ACCUMULATOR = 3,
+  KG_BOOK_DOUBLE_BARRIER = 4,
+  KG_BOOK_DIGITAL = 5
 };
 
 typedef struct kg_book_contract {
@@ -361,7 +447,8 @@ typedef struct kg_book_contract {
   int32_t reserved;
   kg_settings settings;
   kg_bsm_market market;
-  const void* terms; /* kg_barrier_terms, kg_asian_terms, kg_autocall_terms or kg_accumulator_terms */
+  const void* terms; /* kg_barrier_terms, kg_asian_terms, kg_autocall_terms, kg_accumulator_terms,
+                        kg_double_barrier_terms or kg_digital_terms, by kind */
 } kg_book_contract;
 
 typedef struct kg_book_tuning {

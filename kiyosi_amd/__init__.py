@@ -11,6 +11,7 @@ from .engines import (
     MonteCarloAsianEngine,
     MonteCarloAutocallableEngine,
     MonteCarloBarrierEngine,
+    MonteCarloDigitalEngine,
     MonteCarloEstimate,
     MonteCarloVanillaEngine,
 )
@@ -30,6 +31,8 @@ from .instruments import (
     BinarySnowballOption,
     GeometricAveragePriceOption,
     CouponObservation,
+    DigitalOption,
+    DoubleBarrierOption,
     EuropeanOption,
     KnockInObservation,
     KnockOutObservation,

@@ -150,7 +150,10 @@ DOWN_AND_OUT, DOWN_AND_IN, UP_AND_OUT, UP_AND_IN = 0, 1, 2, 3
 OBSERVE_CONTINUOUS, OBSERVE_SCHEDULED = 0, 1
 REBATE_AT_EXPIRY, REBATE_AT_HIT = 0, 1
 AVERAGE_ARITHMETIC, AVERAGE_GEOMETRIC = 0, 1
-BOOK_BARRIER, BOOK_ASIAN, BOOK_AUTOCALL, BOOK_ACCUMULATOR = 0, 1, 2, 3
+BOOK_BARRIER, BOOK_ASIAN, BOOK_AUTOCALL, BOOK_ACCUMULATOR, BOOK_DOUBLE_BARRIER, BOOK_DIGITAL = 0, 1, 2, 3, 4, 5
+GREEK_DELTA, GREEK_GAMMA = 1, 2
+DOUBLE_KNOCK_OUT, DOUBLE_KNOCK_IN = 0, 1
+CASH_OR_NOTHING, ASSET_OR_NOTHING = 0, 1
 
 
 class BarrierTerms(C.Structure):
@@ -167,6 +170,45 @@ class BarrierTerms(C.Structure):
         ("n_observations", C.c_int32),
         ("reserved", C.c_int32),
         ("observation_times", C.POINTER(C.c_int64)),
+    ]
+
+
+class SpotGreeks(C.Structure):
+    _fields_ = [
+        ("request", C.c_int32),
+        ("available", C.c_int32),
+        ("spot_radius", C.c_double),
+        ("delta", C.c_double),
+        ("gamma", C.c_double),
+    ]
+
+
+class DoubleBarrierTerms(C.Structure):
+    _fields_ = [
+        ("option_type", C.c_int32),
+        ("knock_type", C.c_int32),
+        ("observation", C.c_int32),
+        ("rebate_timing", C.c_int32),
+        ("strike", C.c_double),
+        ("lower_barrier", C.c_double),
+        ("upper_barrier", C.c_double),
+        ("rebate", C.c_double),
+        ("effective_time", C.c_int64),
+        ("expiry_time", C.c_int64),
+        ("n_observations", C.c_int32),
+        ("reserved", C.c_int32),
+        ("observation_times", C.POINTER(C.c_int64)),
+    ]
+
+
+class DigitalTerms(C.Structure):
+    _fields_ = [
+        ("option_type", C.c_int32),
+        ("payout_kind", C.c_int32),
+        ("strike", C.c_double),
+        ("cash_payout", C.c_double),
+        ("effective_time", C.c_int64),
+        ("expiry_time", C.c_int64),
     ]
 
 

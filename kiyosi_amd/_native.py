@@ -39,6 +39,16 @@ SIGNATURES = {
                                   P(_abi.Estimate)]),
     "kg_barrier_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.BarrierTerms), P(_abi.BsmMarket),
                                    P(_abi.Estimate)]),
+    "kg_barrier_price_with_greeks": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.BarrierTerms),
+                                               P(_abi.BsmMarket), P(_abi.SpotGreeks), P(_abi.Estimate)]),
+    "kg_double_barrier_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.DoubleBarrierTerms),
+                                          P(_abi.BsmMarket), P(_abi.Estimate)]),
+    "kg_double_barrier_price_with_greeks": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.DoubleBarrierTerms),
+                                                      P(_abi.BsmMarket), P(_abi.SpotGreeks), P(_abi.Estimate)]),
+    "kg_digital_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.DigitalTerms), P(_abi.BsmMarket),
+                                   P(_abi.Estimate)]),
+    "kg_digital_price_with_greeks": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.DigitalTerms),
+                                               P(_abi.BsmMarket), P(_abi.SpotGreeks), P(_abi.Estimate)]),
     "kg_asian_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AsianTerms), P(_abi.BsmMarket),
                                  P(_abi.Estimate)]),
     "kg_accumulator_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AccumulatorTerms), P(_abi.BsmMarket),

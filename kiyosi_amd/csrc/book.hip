@@ -1,4 +1,6 @@
-// Barrier, Asian and accumulator engines and the heterogeneous book.
+// Barrier (single and double), digital, Asian and accumulator engines, the
+// heterogeneous book, and the spot-Greek kernel of the barrier and digital
+// engines.
 //
 // Every contract becomes a BookItem: a plan built on the host (restating the
 // reference engine's own plan building, cited per builder below) plus its
@@ -21,7 +23,7 @@
 
 namespace kg {
 
-enum : int32_t { IK_BARRIER = 0, IK_GEO_ASIAN = 1, IK_ARITH_ASIAN = 2, IK_AUTOCALL = 3, IK_ACCUM = 4 };
+enum : int32_t { IK_BARRIER = 0, IK_GEO_ASIAN = 1, IK_ARITH_ASIAN = 2, IK_AUTOCALL = 3, IK_ACCUM = 4, IK_DIGITAL = 5 };
 
 struct BookItem {
   int32_t kind, dims;
@@ -37,6 +39,7 @@ struct BookItem {
     ArithAsianDev a;
     AcPlanDev ac;
     AccumDev acc;
+    DigitalDev dg;
   } u;
 };
 
@@ -73,7 +76,9 @@ __device__ __attribute__((always_inline)) void sample_body(const BookItem& it, i
   auto run = [&](bool on, float sign) -> double {
     switch (it.kind) {
       case IK_BARRIER:
-        return barrier_path<FP32>(it.u.b, dr, on, sign, bad);
+        return barrier_path<FP32, false>(it.u.b, dr, on, sign, bad);
+      case IK_DIGITAL:
+        return digital_path<FP32, false>(it.u.dg, dr, on, sign, bad);
       case IK_GEO_ASIAN:
         return geo_asian<FP32>(it.u.g, dr, on, sign, bad);
       case IK_ARITH_ASIAN:
@@ -114,6 +119,27 @@ __device__ __attribute__((always_inline)) void book_sample(const BookItem& it, i
     sample_fp64<SOBOL>(it, block, j, pos, neg, live_out, paired_out, bad);
 }
 
+// Moments of one block's 256 sample values staged in LDS (pos, neg and the
+// live/paired bits of each sample), formed by the first wave: lane l folds
+// samples 4l .. 4l+3 in sample order, then the wave merges. Every kernel that
+// prices these contracts uses this one order, so their results agree bit
+// for bit. `pairs` receives the antithetic pair means.
+__device__ inline void fold_block(const double* pos, const double* neg, const unsigned char* state, Moments* out,
+                                  Moments* pairs) {
+  Moments m = moments_zero(), p = moments_zero();
+  for (int k = 0; k < 4; ++k) {
+    const int j = threadIdx.x * 4 + k;
+    const unsigned char st = state[j];
+    if (st & 1) moments_add(m, pos[j], true);
+    if (st & 2) {
+      moments_add(m, neg[j], true);
+      moments_add(p, midpoint(pos[j], neg[j]), true);
+    }
+  }
+  *out = wave_merge(m);
+  *pairs = wave_merge(p);
+}
+
 template <bool FP32, int SPT>
 __global__ void __launch_bounds__(256)
     book_kernel(const BookItem* __restrict__ items, const int2* __restrict__ tiles, int n_tiles,
@@ -146,19 +172,8 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
     if (threadIdx.x < WAVE) {
-      // Lane l folds samples 4l .. 4l+3 in order, then the wave merges.
-      Moments m = moments_zero(), p = moments_zero();
-      for (int k = 0; k < 4; ++k) {
-        const int j = threadIdx.x * 4 + k;
-        const unsigned char st = s_state[j];
-        if (st & 1) moments_add(m, s_pos[j], true);
-        if (st & 2) {
-          moments_add(m, s_neg[j], true);
-          moments_add(p, midpoint(s_pos[j], s_neg[j]), true);
-        }
-      }
-      m = wave_merge(m);
-      p = wave_merge(p);
+      Moments m, p;
+      fold_block(s_pos, s_neg, s_state, &m, &p);
       if (threadIdx.x == 0) {
         partials[(it.first_block + tb.y) * 2 + 0] = m;
         partials[(it.first_block + tb.y) * 2 + 1] = p;
@@ -169,11 +184,12 @@ __global__ void __launch_bounds__(256)
   if (bad) raise_flag(flags, FLAG_NONFINITE);
 }
 
-// One workgroup per contract: its block partials in order (see merge_partials).
-__global__ void __launch_bounds__(256)
+// One workgroup per contract: its block partials in order, with the thread
+// layout of merge_partials, so a book result equals the single-contract one.
+__global__ void __launch_bounds__(MERGE_THREADS)
     merge_book(const BookItem* __restrict__ items, const Moments* __restrict__ partials,
                Moments* __restrict__ out) {
-  __shared__ Moments smem[4];
+  __shared__ Moments smem[MERGE_THREADS / WAVE];
   const BookItem& it = items[blockIdx.x];
   const int64_t blocks = (it.samples + BLOCK_SAMPLES - 1) / BLOCK_SAMPLES;
   const int64_t per = (blocks + blockDim.x - 1) / blockDim.x;
@@ -191,6 +207,76 @@ __global__ void book_reset(int* counter, uint32_t* flags) {
   *flags = 0u;
 }
 
+// Value and likelihood-ratio delta/gamma of one barrier or digital contract
+// (barrier.rs:440-472, digital.rs:116-136): one sample per thread, one
+// 256-sample block per workgroup, four partials per block in the order of
+// sample_paths::<3> (monte_carlo.rs:1777-1796): the three path means, then
+// the antithetic pair means of the value.
+template <bool FP32, bool SOBOL>
+__global__ void __launch_bounds__(256)
+    spot_greek_kernel(const BookItem* __restrict__ item, Moments* __restrict__ partials,
+                      uint32_t* __restrict__ flags) {
+  const BookItem& it = *item;
+  const int64_t block = blockIdx.x;
+  const int j = threadIdx.x;
+  const int64_t n = block * BLOCK_SAMPLES + j;
+  const bool live = n < it.samples;
+  const bool paired = it.antithetic && live && n < it.paths / 2;
+  Draws<SOBOL, FP32> dr;
+  dr.dims = it.dims;
+  dr.direct = true;
+  dr.dirs = it.dirs;
+  dr.shifts = it.shifts;
+  dr.point = (uint64_t)n;
+  if (!SOBOL) {
+    const uint64_t first = (uint64_t)j * (uint64_t)it.dims;
+    if (FP32) {
+      dr.g32.init(it.seed, (uint64_t)block);
+      dr.g32.advance((first >> 1) * 4);
+      dr.has32 = false;
+      if (first & 1) {
+        float e;
+        f32::box_muller_pair(dr.g32, &e, &dr.spare32);
+        dr.has32 = true;
+      }
+    } else {
+      dr.s.start(it.seed, (uint64_t)block, first);
+    }
+  }
+  const Draws<SOBOL, FP32> origin = dr;
+  uint32_t bad = 0;
+  auto run = [&](bool on, float sign, double* out) {
+    out[0] = it.kind == IK_DIGITAL ? digital_path<FP32, true>(it.u.dg, dr, on, sign, bad, out + 1)
+                                   : barrier_path<FP32, true>(it.u.b, dr, on, sign, bad, out + 1);
+  };
+  double pos[3], neg[3] = {0.0, 0.0, 0.0};
+  run(live, 1.0f, pos);
+  if (it.antithetic && __ballot(paired) != 0) {
+    dr = origin;
+    run(paired, -1.0f, neg);
+  }
+  if (bad) raise_flag(flags, FLAG_NONFINITE);
+  // Each output is folded in the book kernel's order (fold_block), so the
+  // price equals kg_barrier_price / kg_digital_price bit for bit.
+  __shared__ double s_pos[BLOCK_SAMPLES], s_neg[BLOCK_SAMPLES];
+  __shared__ unsigned char s_state[BLOCK_SAMPLES];
+  s_state[j] = (unsigned char)(live | (paired << 1));
+  for (int k = 0; k < 3; ++k) {
+    s_pos[j] = pos[k];
+    s_neg[j] = neg[k];
+    __syncthreads();
+    if (threadIdx.x < WAVE) {
+      Moments m, p;
+      fold_block(s_pos, s_neg, s_state, &m, &p);
+      if (threadIdx.x == 0) {
+        partials[block * 4 + k] = m;
+        if (k == 0) partials[block * 4 + 3] = p;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ host
 // Result of planning one contract: either a known value or a sampled item.
 enum class Estimator { PathMeans, Estimate };
@@ -199,6 +285,10 @@ struct Planned {
   double value = 0.0;
   Estimator kind = Estimator::Estimate;
   double control_expected = 0.0;  // arithmetic Asian control mean
+  // Spot Greeks requested and kept after the engine's filters, and the
+  // values of a contract resolved before sampling.
+  int32_t greeks = 0;
+  double known_delta = 0.0, known_gamma = 0.0;
   BookItem item{};
   Sampler sampler{};
   // Tables staged into the shared blob: (blob offset, byte offset of the
@@ -247,14 +337,68 @@ static double vanilla_bsm(double S, double K, double r, double q, double sigma, 
   return sign * (S * std::exp(-q * T) * cdf(sign * d1) - K * std::exp(-r * T) * cdf(sign * d2));
 }
 
+// A barrier contract as MonteCarloBarrierEngine::native reads it
+// (barrier.rs:170-178): a single barrier, or a double barrier as its lower
+// (down-type) barrier and the upper level.
+struct BarrierContract {
+  int32_t option_type, barrier_type, observation, rebate_timing;
+  double strike, barrier, rebate;
+  bool has_upper;
+  double upper;
+  int64_t effective_time, expiry_time;
+  int32_t n_observations;
+  const int64_t* observation_times;
+};
+static BarrierContract contract_of(const kg_barrier_terms& o) {
+  if (o.barrier_type < KG_DOWN_AND_OUT || o.barrier_type > KG_UP_AND_IN)
+    fail(KG_INVALID_INPUT, "unknown barrier type");
+  return {o.option_type,    o.barrier_type, o.observation, o.rebate_timing,  o.strike,
+          o.barrier,        o.rebate,       false,         0.0,              o.effective_time,
+          o.expiry_time,    o.n_observations, o.observation_times};
+}
+// DoubleBarrierOption::new (instruments/double_barrier.rs:119-144).
+static BarrierContract contract_of(const kg_double_barrier_terms& o) {
+  if (o.knock_type != KG_DOUBLE_KNOCK_OUT && o.knock_type != KG_DOUBLE_KNOCK_IN)
+    fail(KG_INVALID_INPUT, "unknown double-barrier type");
+  if (!(o.lower_barrier > 0.0) || !std::isfinite(o.lower_barrier) || !std::isfinite(o.upper_barrier) ||
+      !(o.upper_barrier > o.lower_barrier))
+    fail(KG_INVALID_INPUT, "double barriers need finite levels with 0 < lower < upper");
+  if (o.knock_type == KG_DOUBLE_KNOCK_IN && o.rebate_timing == KG_REBATE_AT_HIT)
+    fail(KG_INVALID_INPUT, "knock-in rebates are paid at expiry");
+  return {o.option_type,
+          o.knock_type == KG_DOUBLE_KNOCK_IN ? KG_DOWN_AND_IN : KG_DOWN_AND_OUT,
+          o.observation,
+          o.rebate_timing,
+          o.strike,
+          o.lower_barrier,
+          o.rebate,
+          true,
+          o.upper_barrier,
+          o.effective_time,
+          o.expiry_time,
+          o.n_observations,
+          o.observation_times};
+}
+
+// Delta and gamma of the Black-Scholes vanilla, the Greeks of a barrier
+// resolved to its vanilla payoff (barrier.rs:206-220).
+static void vanilla_spot_greeks(double S, double K, double r, double q, double sigma, double T, double sign,
+                                double* delta, double* gamma) {
+  const double w = sigma * std::sqrt(T);
+  const double d1 = (log_ratio(S, K) + (r - q) * T) / w + 0.5 * w;
+  const double pdf = std::exp(-0.5 * d1 * d1) * 0.3989422804014327;
+  *delta = sign * std::exp(-q * T) * 0.5 * std::erfc(-sign * d1 * 0.7071067811865476);
+  *gamma = std::exp(-q * T) * pdf / (S * w);
+}
+
 // MonteCarloBarrierEngine::native plan (engines/monte_carlo/barrier.rs:160-290)
-// and resolved_price_with_state (engines/analytic/barrier.rs:32-80).
-static Planned plan_barrier(const kg_barrier_terms& o, const kg_bsm_market& m, Blob& blob) {
+// for flat rates and no prior touch. `request` (KG_GREEK_* bits) and
+// `radius` drive the spot-Greek filter of :186-205.
+static Planned plan_barrier(const BarrierContract& o, const kg_bsm_market& m, Blob& blob, int32_t request = 0,
+                            double radius = 0.0) {
   Planned pl;
   pl.kind = Estimator::PathMeans;
   if (o.option_type != 1 && o.option_type != -1) fail(KG_INVALID_INPUT, "option_type must be 1 or -1");
-  if (o.barrier_type < KG_DOWN_AND_OUT || o.barrier_type > KG_UP_AND_IN)
-    fail(KG_INVALID_INPUT, "unknown barrier type");
   if (!(o.strike > 0.0) || !(o.barrier > 0.0) || !(o.rebate >= 0.0) || !std::isfinite(o.rebate))
     fail(KG_INVALID_INPUT, "strike and barrier must be positive and the rebate nonnegative");
   if (o.observation == KG_OBSERVE_SCHEDULED && (o.n_observations < 1 || !o.observation_times))
@@ -262,6 +406,7 @@ static Planned plan_barrier(const kg_barrier_terms& o, const kg_bsm_market& m, B
   for (int i = 1; i < o.n_observations; ++i)
     if (o.observation_times[i] <= o.observation_times[i - 1])
       fail(KG_INVALID_INPUT, "observation times must be strictly increasing");
+  if (!(radius >= 0.0) || !std::isfinite(radius)) fail(KG_INVALID_INPUT, "spot_radius must be nonnegative");
   const int64_t val = m.valuation_time, expiry = o.expiry_time;
   if (val < o.effective_time || val > expiry) fail(KG_INVALID_INPUT, "valuation outside contract life");
   check_market(m);
@@ -271,33 +416,41 @@ static Planned plan_barrier(const kg_barrier_terms& o, const kg_bsm_market& m, B
   const bool at_hit = o.rebate_timing == KG_REBATE_AT_HIT;
   const double sign = o.option_type, r = m.rate, q = m.dividend, sigma = m.volatility, S = m.spot;
   const double T = (double)(expiry - val) / (365.0 * 86400e6);
+  // Monitoring state (instruments/monitoring.rs:293-313).
   bool observed_now = continuous;
   for (int i = 0; i < o.n_observations && !continuous; ++i) observed_now |= o.observation_times[i] == val;
-  const bool touched = observed_now && (up ? S >= o.barrier : S <= o.barrier);
-  const bool finished = val == expiry;
+  const bool touched =
+      observed_now && ((up ? S >= o.barrier : S <= o.barrier) || (o.has_upper && S >= o.upper));
+  const bool finished =
+      val == expiry || (!continuous && o.observation_times[o.n_observations - 1] <= val);
   const bool last_is_expiry =
       continuous || (o.n_observations > 0 && o.observation_times[o.n_observations - 1] == expiry);
+  // payoff_requires_touch (monitoring.rs:314-321; double_barrier.rs:216-224)
   const bool requires_touch =
-      last_is_expiry && (up ? (sign > 0 && o.strike >= o.barrier) : (sign < 0 && o.strike <= o.barrier));
+      o.has_upper ? last_is_expiry && (sign > 0 ? o.strike >= o.upper : o.strike <= o.barrier)
+                  : last_is_expiry && (up ? (sign > 0 && o.strike >= o.barrier) : (sign < 0 && o.strike <= o.barrier));
+  // Spot Greeks: never at expiry, nor when the stencil crosses a monitored
+  // level (spot_discontinuity, monitoring.rs:430-445).
+  auto crosses = [&](double level) { return S - radius <= level && level <= S + radius; };
+  const bool discontinuity = observed_now && (crosses(o.barrier) || (o.has_upper && crosses(o.upper)));
+  pl.greeks = (val != expiry && !discontinuity) ? request & (KG_GREEK_DELTA | KG_GREEK_GAMMA) : 0;
   const double cash = o.rebate == 0.0 ? 0.0 : o.rebate * std::exp(-r * T);
   const bool fixed = o.rebate == 0.0 && requires_touch;
-  if (touched && !knock_in) {
+  // resolved_price_with_state (engines/analytic/barrier.rs:32-80).
+  if ((touched && !knock_in) || (!touched && finished && knock_in) || (fixed && !knock_in)) {
     pl.known = true;
-    pl.value = at_hit ? o.rebate : cash;
-    return pl;
-  }
-  if (!touched && finished && knock_in) {
-    pl.known = true;
-    pl.value = cash;
-    return pl;
-  }
-  if (fixed && !knock_in) {
-    pl.known = true;
+    pl.value = (touched && !knock_in) ? (at_hit ? o.rebate : cash) : (!touched && finished && knock_in) ? cash : 0.0;
     return pl;
   }
   if (touched || finished || fixed) {
     pl.known = true;
     pl.value = vanilla_bsm(S, o.strike, r, q, sigma, T, sign);
+    if (pl.greeks) {
+      double d, g;
+      vanilla_spot_greeks(S, o.strike, r, q, sigma, T, sign, &d, &g);
+      pl.known_delta = (pl.greeks & KG_GREEK_DELTA) ? d : 0.0;
+      pl.known_gamma = (pl.greeks & KG_GREEK_GAMMA) ? g : 0.0;
+    }
     return pl;
   }
   if (continuous && !knock_in && at_hit && o.rebate != 0.0)
@@ -327,11 +480,13 @@ static Planned plan_barrier(const kg_barrier_terms& o, const kg_bsm_market& m, B
     s.discount = -r * elapsed;
     s.observe = observe;
     s.lower = observe ? mean_distance(date, o.barrier) : 0.0;
+    s.upper = observe && o.has_upper ? mean_distance(date, o.upper) : 0.0;
     s.remaining = years(date, expiry);
     steps.push_back(s);
     steps32.push_back((float)s.mean);
     steps32.push_back((float)s.width);
     steps32.push_back((float)s.lower);
+    steps32.push_back((float)s.upper);
     width_sum += s.width;
     prev = date;
   }
@@ -341,6 +496,8 @@ static Planned plan_barrier(const kg_barrier_terms& o, const kg_bsm_market& m, B
   b.knock_in = knock_in;
   b.continuous = continuous;
   b.at_hit = at_hit;
+  b.has_upper = o.has_upper;
+  b.request = pl.greeks;
   b.bounded = std::fabs(std::log(S)) + 10.0 * width_sum + std::fabs(drift * T) < 600.0;
   b.condition_on_hit = knock_in && !continuous && steps.size() > 8 && b.bounded;
   if (b.condition_on_hit)
@@ -356,6 +513,8 @@ static Planned plan_barrier(const kg_barrier_terms& o, const kg_bsm_market& m, B
   b.hvr = hvr;
   b.moneyness = mean_distance(expiry, o.strike);
   b.barrier_abs = std::fabs(log_ratio(o.barrier, S));
+  b.initial = -log_ratio(o.barrier, S);
+  b.band = o.has_upper ? log_ratio(o.upper, o.barrier) : 0.0;
   b.terminal_barrier = mean_distance(expiry, o.barrier);
   b.log_discount = -r * T;
   pl.item.kind = IK_BARRIER;
@@ -364,6 +523,51 @@ static Planned plan_barrier(const kg_barrier_terms& o, const kg_bsm_market& m, B
   const size_t o1 = blob.add(steps), o2 = blob.add(steps32);
   pl.stage(o1, pl.item.u.b.steps);
   pl.stage(o2, pl.item.u.b.steps32);
+  return pl;
+}
+
+// MonteCarloDigitalEngine::native plan (engines/monte_carlo/digital.rs:55-108).
+static Planned plan_digital(const kg_digital_terms& o, const kg_bsm_market& m, Blob&, int32_t request = 0) {
+  Planned pl;
+  pl.kind = Estimator::PathMeans;
+  if (o.option_type != 1 && o.option_type != -1) fail(KG_INVALID_INPUT, "option_type must be 1 or -1");
+  if (o.payout_kind != KG_CASH_OR_NOTHING && o.payout_kind != KG_ASSET_OR_NOTHING)
+    fail(KG_INVALID_INPUT, "unknown digital payout kind");
+  if (!(o.strike > 0.0) || !std::isfinite(o.strike)) fail(KG_INVALID_INPUT, "strike must be positive and finite");
+  if (o.expiry_time < o.effective_time) fail(KG_INVALID_INPUT, "expiry precedes the effective time");
+  const bool asset = o.payout_kind == KG_ASSET_OR_NOTHING;
+  if (!asset && (!(o.cash_payout >= 0.0) || !std::isfinite(o.cash_payout)))
+    fail(KG_INVALID_INPUT, "payout must be finite and nonnegative");
+  // No valuation start (instruments/vanilla.rs:375-377): only expiry bounds it.
+  if (m.valuation_time > o.expiry_time) fail(KG_INVALID_INPUT, "valuation outside contract life");
+  check_market(m);
+  const bool call = o.option_type > 0;
+  const double S = m.spot, sigma = m.volatility, r = m.rate, q = m.dividend;
+  const double time = (double)(o.expiry_time - m.valuation_time) / (365.0 * 86400e6);
+  if (time == 0.0) {  // Payoff::digital (engines.rs:212-227); the expiry stencil gives the Greeks
+    pl.known = true;
+    pl.value = (call ? S - o.strike : o.strike - S) > 0.0 ? (asset ? S : o.cash_payout) : 0.0;
+    return pl;
+  }
+  pl.greeks = request & (KG_GREEK_DELTA | KG_GREEK_GAMMA);
+  const double root = std::sqrt(time), width = sigma * root, carry = (r - q) * time;
+  const double lm = log_ratio(S, o.strike) + carry;
+  const double standardized = width < std::numeric_limits<double>::min() ? (lm / sigma) / root : lm / width;
+  DigitalDev d{};
+  d.call = call;
+  d.asset = asset;
+  d.request = pl.greeks;
+  d.d2 = standardized - 0.5 * width;
+  d.amount = asset ? S : o.cash_payout;
+  d.drift = asset ? (-q - 0.5 * sigma * sigma) * time : -r * time;
+  if (!std::isfinite(width) || !std::isfinite(carry) || !std::isfinite(d.drift)) unstable();
+  d.width = width;
+  d.log_spot = std::log(S);
+  d.log_width = std::log(sigma) + std::log(root);
+  d.log_amount = std::log(d.amount);
+  pl.item.kind = IK_DIGITAL;
+  pl.item.dims = 1;
+  pl.item.u.dg = d;
   return pl;
 }
 
@@ -671,7 +875,13 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
     validate_settings(&c.settings, 1);
     switch (c.kind) {
       case KG_BOOK_BARRIER:
-        plans[i] = plan_barrier(*(const kg_barrier_terms*)c.terms, c.market, blob);
+        plans[i] = plan_barrier(contract_of(*(const kg_barrier_terms*)c.terms), c.market, blob);
+        break;
+      case KG_BOOK_DOUBLE_BARRIER:
+        plans[i] = plan_barrier(contract_of(*(const kg_double_barrier_terms*)c.terms), c.market, blob);
+        break;
+      case KG_BOOK_DIGITAL:
+        plans[i] = plan_digital(*(const kg_digital_terms*)c.terms, c.market, blob);
         break;
       case KG_BOOK_ASIAN:
         plans[i] = plan_asian(*(const kg_asian_terms*)c.terms, c.market, blob);
@@ -764,7 +974,7 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
       hipGraphExec_t exec;
       KG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       run_book_once(ctx, fp32, spt, wpc, d_items, d_tiles, (int)tiles.size(), d_counter, d_parts, flags);
-      hipLaunchKernelGGL(merge_book, dim3(items.size()), dim3(256), 0, ctx->stream, d_items, d_parts, d_out);
+      hipLaunchKernelGGL(merge_book, dim3(items.size()), dim3(MERGE_THREADS), 0, ctx->stream, d_items, d_parts, d_out);
       KG_HIP(hipStreamEndCapture(ctx->stream, &graph));
       KG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
       timing_begin(ctx);
@@ -781,7 +991,7 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
       for (int k = 0; k <= replays; ++k) {
         timing_begin(ctx);
         run_book_once(ctx, fp32, spt, wpc, d_items, d_tiles, (int)tiles.size(), d_counter, d_parts, flags);
-        hipLaunchKernelGGL(merge_book, dim3(items.size()), dim3(256), 0, ctx->stream, d_items, d_parts, d_out);
+        hipLaunchKernelGGL(merge_book, dim3(items.size()), dim3(MERGE_THREADS), 0, ctx->stream, d_items, d_parts, d_out);
         KG_HIP(hipGetLastError());
         device_ms = timing_end(ctx);
       }
@@ -816,7 +1026,7 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
       KG_HIP(hipMemcpyAsync(d_items, &it, sizeof(BookItem), hipMemcpyHostToDevice, ctx->stream));
       KG_HIP(hipMemcpyAsync(d_tiles, t2.data(), t2.size() * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
       run_book_once(ctx, fp32, spt, wpc, d_items, d_tiles, (int)t2.size(), d_counter, d_parts, flags);
-      hipLaunchKernelGGL(merge_book, dim3(1), dim3(256), 0, ctx->stream, d_items, d_parts, d_out);
+      hipLaunchKernelGGL(merge_book, dim3(1), dim3(MERGE_THREADS), 0, ctx->stream, d_items, d_parts, d_out);
       check_flags(ctx);
       KG_HIP(hipMemcpyAsync(&results[2 * k], d_out, 2 * sizeof(Moments), hipMemcpyDeviceToHost, ctx->stream));
       KG_HIP(hipStreamSynchronize(ctx->stream));
@@ -869,6 +1079,86 @@ static void price_single(kg_context* ctx, int kind, const kg_settings* settings,
   price_book(ctx, &c, 1, settings->precision, &t, out);
 }
 
+// Prices one barrier or digital contract with its likelihood-ratio delta and
+// gamma: plan as in the book, then one launch of spot_greek_kernel and a
+// block-ordered merge of its four partial sets.
+static void price_spot_greeks(kg_context* ctx, const kg_settings* settings, Planned p, Blob& blob,
+                              kg_spot_greeks* greeks, kg_estimate* out) {
+  *out = kg_estimate{};
+  greeks->available = 0;
+  greeks->delta = greeks->gamma = 0.0;
+  if (p.known) {
+    fill_known(p.value, settings->sampling == KG_SOBOL, out);
+    greeks->available = p.greeks;
+    greeks->delta = p.known_delta;
+    greeks->gamma = p.known_gamma;
+    return;
+  }
+  const Sampler s = validate_settings(settings, (uint64_t)p.item.dims);
+  p.item.samples = s.samples();
+  p.item.paths = s.path_count;
+  p.item.seed = s.seed;
+  p.item.antithetic = s.antithetic;
+  p.item.sobol = s.sobol;
+  p.item.first_block = 0;
+  size_t dirs_off = 0, shifts_off = 0;
+  if (s.sobol) {
+    std::vector<uint64_t> dirs, shifts;
+    sobol_tables(ctx, (uint32_t)p.item.dims, s.seed, dirs, shifts);
+    dirs_off = blob.add(dirs);
+    shifts_off = blob.add(shifts);
+  }
+  const size_t item_off = (blob.bytes.size() + 255) & ~size_t(255);
+  blob.bytes.resize(item_off + sizeof(BookItem));
+  unsigned char* d = (unsigned char*)ctx->scratch.reserve(blob.bytes.size());
+  for (auto& f : p.fixups) {
+    const void* at = d + f.first;
+    std::memcpy((unsigned char*)&p.item + f.second, &at, sizeof(at));
+  }
+  if (s.sobol) {
+    p.item.dirs = (const uint64_t*)(d + dirs_off);
+    p.item.shifts = (const uint64_t*)(d + shifts_off);
+  }
+  check_tables(p.item);
+  std::memcpy(blob.bytes.data() + item_off, &p.item, sizeof(BookItem));
+  upload_async(ctx, d, blob.bytes.data(), blob.bytes.size());
+  const BookItem* d_item = (const BookItem*)(d + item_off);
+  const int64_t blocks = s.blocks();
+  if (blocks > (int64_t(1) << 31) - 1) fail(KG_INVALID_INPUT, "too many sample blocks");
+  Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 4 * blocks);
+  uint32_t* flags = reset_flags(ctx);
+  timing_begin(ctx);
+  auto launch = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(BLOCK_SAMPLES), 0, ctx->stream, d_item, parts, flags);
+  };
+  if (s.sobol && s.fp32)
+    launch(spot_greek_kernel<true, true>);
+  else if (s.sobol)
+    launch(spot_greek_kernel<false, true>);
+  else if (s.fp32)
+    launch(spot_greek_kernel<true, false>);
+  else
+    launch(spot_greek_kernel<false, false>);
+  KG_HIP(hipGetLastError());
+  Moments r[4];
+  finish_sampling(ctx, parts, blocks, 4, r, &out->device_ms);
+  fill_estimate(s, r[0], r[3], 0.0, false, true, out);
+  out->path_steps = (double)s.path_count * (double)p.item.dims;
+  greeks->available = p.greeks;
+  if (p.greeks & KG_GREEK_DELTA) greeks->delta = scaled_value(r[1].scale, r[1].mean, 0.0);
+  if (p.greeks & KG_GREEK_GAMMA) greeks->gamma = scaled_value(r[2].scale, r[2].mean, 0.0);
+  if (!std::isfinite(greeks->delta) || !std::isfinite(greeks->gamma)) unstable();
+}
+
+template <class Terms>
+static void barrier_with_greeks(kg_context* ctx, const kg_settings* settings, const Terms* option,
+                                const kg_bsm_market* market, kg_spot_greeks* greeks, kg_estimate* out) {
+  if (!settings || !option || !market || !greeks || !out) fail(KG_INVALID_INPUT, "arguments must not be null");
+  Blob blob;
+  Planned p = plan_barrier(contract_of(*option), *market, blob, greeks->request, greeks->spot_radius);
+  price_spot_greeks(ctx, settings, std::move(p), blob, greeks, out);
+}
+
 }  // namespace kg
 
 using namespace kg;
@@ -878,6 +1168,41 @@ extern "C" {
 kg_status kg_barrier_price(kg_context* ctx, const kg_settings* settings, const kg_barrier_terms* option,
                            const kg_bsm_market* market, kg_estimate* out) {
   return guarded(ctx, [&] { price_single(ctx, KG_BOOK_BARRIER, settings, option, market, out); });
+}
+
+kg_status kg_barrier_price_with_greeks(kg_context* ctx, const kg_settings* settings,
+                                       const kg_barrier_terms* option, const kg_bsm_market* market,
+                                       kg_spot_greeks* greeks, kg_estimate* out) {
+  return guarded(ctx, [&] { barrier_with_greeks(ctx, settings, option, market, greeks, out); });
+}
+
+kg_status kg_double_barrier_price(kg_context* ctx, const kg_settings* settings,
+                                  const kg_double_barrier_terms* option, const kg_bsm_market* market,
+                                  kg_estimate* out) {
+  return guarded(ctx, [&] { price_single(ctx, KG_BOOK_DOUBLE_BARRIER, settings, option, market, out); });
+}
+
+kg_status kg_double_barrier_price_with_greeks(kg_context* ctx, const kg_settings* settings,
+                                              const kg_double_barrier_terms* option,
+                                              const kg_bsm_market* market, kg_spot_greeks* greeks,
+                                              kg_estimate* out) {
+  return guarded(ctx, [&] { barrier_with_greeks(ctx, settings, option, market, greeks, out); });
+}
+
+kg_status kg_digital_price(kg_context* ctx, const kg_settings* settings, const kg_digital_terms* option,
+                           const kg_bsm_market* market, kg_estimate* out) {
+  return guarded(ctx, [&] { price_single(ctx, KG_BOOK_DIGITAL, settings, option, market, out); });
+}
+
+kg_status kg_digital_price_with_greeks(kg_context* ctx, const kg_settings* settings,
+                                       const kg_digital_terms* option, const kg_bsm_market* market,
+                                       kg_spot_greeks* greeks, kg_estimate* out) {
+  return guarded(ctx, [&] {
+    if (!settings || !option || !market || !greeks || !out) fail(KG_INVALID_INPUT, "arguments must not be null");
+    Blob blob;
+    Planned p = plan_digital(*option, *market, blob, greeks->request);
+    price_spot_greeks(ctx, settings, std::move(p), blob, greeks, out);
+  });
 }
 
 kg_status kg_asian_price(kg_context* ctx, const kg_settings* settings, const kg_asian_terms* option,

@@ -3,7 +3,9 @@
 // persistent kernel (book.hip) runs every contract type.
 //
 //   barrier_path   MonteCarloBarrierEngine sample closure
-//                  (engines/monte_carlo/barrier.rs:291-477), values only
+//                  (engines/monte_carlo/barrier.rs:291-477), single and
+//                  double barriers, optionally with likelihood-ratio Greeks
+//   digital_path   MonteCarloDigitalEngine (engines/monte_carlo/digital.rs)
 //   geo_asian      GeometricParameters::geometric (asian.rs:741-784)
 //   arith_asian    ArithmeticGrid::path + ArithmeticParameters::arithmetic
 //                  with the geometric control residual (asian.rs:640-712,
@@ -69,21 +71,126 @@ __device__ inline double d_lognormal_price(double spot, double strike, double lo
 }
 
 // ------------------------------------------------------------------ barrier
+// double_barrier::survival (engines/monte_carlo/double_barrier.rs:6-112): the
+// probability that the Brownian bridge from `initial` to `terminal` (log
+// distances above the lower level) stays inside (0, band), and its first two
+// derivatives in `initial` with the terminal value held fixed. Image series
+// for small variance, spectral series otherwise; a series that does not
+// converge raises `bad`, the reference's unstable().
+__device__ inline void d_double_survival(double initial, double terminal, double band, double deviation,
+                                         bool derivatives, double out[3], uint32_t& bad) {
+  out[0] = out[1] = out[2] = 0.0;
+  if (terminal <= 0.0 || terminal >= band) return;
+  double x = initial / band, y = terminal / band;
+  const double ratio = deviation / band;
+  const double variance = ratio * ratio;
+  if (variance == 0.0) {
+    out[0] = 1.0;
+    return;
+  }
+  if (!isfinite(variance) || variance <= 0.0) {
+    bad = 1;
+    return;
+  }
+  const bool reflected = x + y > 1.0;
+  if (reflected) {
+    x = 1.0 - x;
+    y = 1.0 - y;
+  }
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  bool converged = false;
+  if (variance < 0.25) {
+    const double inverse_variance = isnormal(variance) ? 1.0 / variance : INFINITY;
+    s0 = -expm1(-2.0 * (x / ratio) * (y / ratio));
+    // One image term: k shifts the band, `direct` picks the unreflected image.
+    auto add = [&](double k, bool direct) {
+      const double exponent = direct ? -2.0 * (k / ratio) * ((y - x + k) / ratio)
+                                     : -2.0 * ((x + k) / ratio) * ((y + k) / ratio);
+      const double sign = direct ? 1.0 : -1.0;
+      const double weight = exp(exponent);
+      if (k != 0.0) s0 += sign * weight;
+      if (derivatives) {
+        const double numerator = direct ? 2.0 * k : -2.0 * (y + k);
+        if (numerator != 0.0) {
+          const double slope = numerator * inverse_variance;
+          const double first = weight * slope;
+          const double second = first * slope;
+          if (isnormal(weight) && isnormal(first) && isnormal(second)) {
+            s1 += sign * first;
+            s2 += sign * second;
+          } else {
+            const double log_slope = log(fabs(numerator)) - 2.0 * log(ratio);
+            s1 += sign * copysign(exp(exponent + log_slope), numerator);
+            s2 += sign * exp(exponent + 2.0 * log_slope);
+          }
+        }
+      }
+      return weight;
+    };
+    add(0.0, false);
+    for (int n = 1; n <= 64; ++n) {
+      const double k = n;
+      const double tail = add(k, true) + add(k, false) + add(-k, true) + add(-k, false);
+      if (n >= 2 && tail < 1e-18) {
+        converged = true;
+        break;
+      }
+    }
+  } else {
+    const double distance = y - x;
+    const double log_scale = log(2.0 * 2.5066282746310002 * ratio) + 0.5 * distance * distance / variance;
+    const double score = -distance / variance;
+    for (int n = 1; n <= 256; ++n) {
+      const double frequency = (double)n * M_PI;
+      const double exponent = log_scale - 0.5 * frequency * frequency * variance;
+      const double weight = exp(exponent);
+      const double sy = sin(frequency * y);
+      const double product = sin(frequency * x) * sy;
+      s0 += weight * product;
+      if (derivatives) {
+        const double first = frequency * cos(frequency * x) * sy;
+        s1 += weight * (first + score * product);
+        s2 += weight * (2.0 * score * first + (score * score + 1.0 / variance - frequency * frequency) * product);
+      }
+      if (n >= 4 && exponent + 4.0 * log(frequency) < -40.0) {
+        converged = true;
+        break;
+      }
+    }
+  }
+  if (!converged || !(s0 >= -1e-13 && s0 <= 1.0 + 1e-13)) {
+    bad = 1;
+    return;
+  }
+  out[0] = fmin(fmax(s0, 0.0), 1.0);
+  if (derivatives) {
+    out[1] = (reflected ? -s1 : s1) / band;
+    out[2] = (s2 / band) / band;
+  }
+}
+
 struct BarrierStep {
-  double mean, width, discount, lower, remaining;
+  double mean, width, discount, lower, upper, remaining;
   int32_t observe;
   int32_t pad;
 };
+// A single barrier, or a double barrier as its lower (down-type) level plus
+// `upper` (has_upper): instruments/double_barrier.rs:119-144.
 struct BarrierDev {
-  int32_t n_steps, up, knock_in, continuous, at_hit, condition_on_hit, bounded, pad;
+  int32_t n_steps, up, knock_in, continuous, at_hit, condition_on_hit, bounded, has_upper;
+  int32_t request, pad;  // KG_GREEK_* bits of the spot-Greek kernel; 0 in the book
   double sign, strike, rebate, spot, q, sigma, hvr, moneyness, barrier_abs, terminal_barrier;
-  double log_discount;
+  double log_discount, initial, band;
   const BarrierStep* steps;
-  const float* steps32;  // mean, width, lower per step
+  const float* steps32;  // mean, width, lower, upper per step
 };
 
-template <bool FP32, class D>
-__device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign, uint32_t& bad) {
+// MonteCarloBarrierEngine's sample closure (barrier.rs:291-477). With LR the
+// likelihood-ratio delta and gamma of the path go to lr[0], lr[1]
+// (:440-472); otherwise only the value is formed.
+template <bool FP32, bool LR, class D>
+__device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign, uint32_t& bad,
+                               double* lr = nullptr) {
   using R = typename std::conditional<FP32, float, double>::type;
   R noise = 0, terminal = 0;
   bool touched = false, has_cond = false, active = live;
@@ -92,14 +199,15 @@ __device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign
   // hit; the FP64 values are formed after the loop, once for all lanes,
   // instead of once per lane inside it.
   int hit = -1;
-  R hit_noise = 0, hit_terminal = 0;
+  R hit_noise = 0, hit_terminal = 0, z0 = 0;
   for (int i = 0; i < b.n_steps; ++i) {
     if (__ballot(active) == 0) break;
     if (!active) continue;
     const BarrierStep& s = b.steps[i];
-    const R width = FP32 ? (R)b.steps32[3 * i + 1] : (R)s.width;
-    const R mean = FP32 ? (R)b.steps32[3 * i] : (R)s.mean;
+    const R width = FP32 ? (R)b.steps32[4 * i + 1] : (R)s.width;
+    const R mean = FP32 ? (R)b.steps32[4 * i] : (R)s.mean;
     const R z = (R)sign * dr.template normal<R>(i);
+    if (LR && i == 0) z0 = z;
     noise = FP32 ? (R)fmaf((float)width, (float)z, (float)noise) : (R)fma((double)width, (double)z, (double)noise);
     terminal = mean + noise;
     if (!b.bounded) {
@@ -107,8 +215,15 @@ __device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign
       if (!isfinite(sp) || sp <= 0.0) bad = 1;
     }
     if (s.observe) {
-      const R lower = FP32 ? (R)b.steps32[3 * i + 2] : (R)s.lower;
-      if (b.up ? noise >= -lower : noise <= -lower) {
+      const R lower = FP32 ? (R)b.steps32[4 * i + 2] : (R)s.lower;
+      bool crossed;
+      if (b.has_upper) {
+        const R upper = FP32 ? (R)b.steps32[4 * i + 3] : (R)s.upper;
+        crossed = noise <= -lower || noise >= -upper;
+      } else {
+        crossed = b.up ? noise >= -lower : noise <= -lower;
+      }
+      if (crossed) {
         touched = true;
         if (b.condition_on_hit && b.n_steps - i > 8) {
           hit = i;
@@ -134,7 +249,9 @@ __device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign
   } else if (hit >= 0) {
     knocked_out = discounted_product(b.rebate, 1.0, b.at_hit ? b.steps[hit].discount : b.log_discount);
   }
+  if (LR) lr[0] = lr[1] = 0.0;
   if (!live) return 0.0;
+  const bool want_delta = LR && (b.request & 1), want_gamma = LR && (b.request & 2);
   const double asset = discounted_product(b.spot, 1.0, (double)terminal);
   const double distance = b.moneyness + (double)noise;
   const double payoff = fabs(distance) < 0.5 ? b.strike * expm1(distance) : asset - b.strike;
@@ -142,9 +259,25 @@ __device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign
   auto weighted = [&](double amount, double lw) {
     return discounted_product(fabs(amount), amount < 0.0 ? -1.0 : 1.0, b.log_discount + lw);
   };
-  double value;
+  double value, first = 0.0, second = 0.0;
   if (has_cond) {
     value = conditioned;
+  } else if (b.continuous && b.has_upper) {
+    double sv[3];
+    d_double_survival(b.initial, b.terminal_barrier + (double)noise, b.band, b.steps[0].width,
+                      want_delta || want_gamma, sv, bad);
+    const double rebate = (!b.knock_in && b.at_hit) ? 0.0 : b.rebate;
+    value = weighted(intrinsic, b.knock_in ? log1p(-sv[0]) : log(sv[0])) +
+            weighted(rebate, b.knock_in ? log(sv[0]) : log1p(-sv[0]));
+    if (want_delta || want_gamma) {
+      const double difference = (b.knock_in ? -1.0 : 1.0) * (intrinsic - rebate);
+      auto derivative = [&](double w) {
+        return discounted_product(fabs(difference), (difference < 0.0 ? -1.0 : 1.0) * (w < 0.0 ? -1.0 : 1.0),
+                                  b.log_discount + log(fabs(w)));
+      };
+      first = derivative(sv[1]);
+      if (want_gamma) second = derivative(sv[2]);
+    }
   } else if (b.continuous) {
     const double w0 = b.steps[0].width;
     const double dtb = b.terminal_barrier + (double)noise;
@@ -155,13 +288,59 @@ __device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign
     const double rebate = (!b.knock_in && b.at_hit) ? 0.0 : b.rebate;
     value = weighted(intrinsic, b.knock_in ? exponent : survival) +
             weighted(rebate, b.knock_in ? survival : exponent);
+    if ((want_delta || want_gamma) && slope > 0.0 && exponent > -INFINITY) {
+      const double difference = (b.knock_in ? -1.0 : 1.0) * (intrinsic - rebate);
+      first = weighted(b.up ? -difference : difference, exponent + log(slope));
+      if (want_gamma) second = weighted(-difference, exponent + 2.0 * log(slope));
+    }
   } else if (touched && !b.knock_in) {
     value = knocked_out;
   } else {
     value = weighted(touched == (bool)b.knock_in ? intrinsic : b.rebate, 0.0);
   }
   if (!isfinite(value)) bad = 1;
+  if (LR) {
+    const double width = b.steps[0].width, z = (double)z0, score = z / width;
+    if (want_delta) lr[0] = (value * score + first) / b.spot;
+    if (want_gamma) {
+      const double second_score = ((z * z - 1.0) / width - z) / width;
+      lr[1] = ((value * second_score + first * (2.0 * score - 1.0) + second) / b.spot) / b.spot;
+    }
+    if (!isfinite(lr[0]) || !isfinite(lr[1]) || !isfinite(first) || !isfinite(second)) bad = 1;
+  }
   return value;
+}
+
+// ------------------------------------------------------------------ digital
+// MonteCarloDigitalEngine's path function (engines/monte_carlo/digital.rs
+// :110-140): strict in-the-money test in standardised coordinates, payoff
+// amount * exp(exponent), likelihood-ratio delta and gamma in log space.
+struct DigitalDev {
+  int32_t call, asset, request, pad;
+  double d2, drift, width, amount, log_spot, log_width, log_amount;
+};
+
+template <bool FP32, bool LR, class D>
+__device__ double digital_path(const DigitalDev& g, D& dr, bool live, float sign, uint32_t& bad,
+                               double* lr = nullptr) {
+  if (LR) lr[0] = lr[1] = 0.0;
+  if (!live) return 0.0;
+  const double normal = (double)sign * (FP32 ? (double)dr.template normal<float>(0) : dr.template normal<double>(0));
+  const double distance = g.d2 + normal;
+  if (!(g.call ? distance > 0.0 : distance < 0.0)) return 0.0;
+  const double exponent = g.drift + (g.asset ? g.width * normal : 0.0);
+  const double payoff = discounted_product(g.amount, 1.0, exponent);
+  if (!isfinite(payoff)) bad = 1;
+  if (LR) {
+    const double log_payoff = g.log_amount + exponent;
+    if ((g.request & 1) && normal != 0.0)
+      lr[0] = copysign(exp(log_payoff + log(fabs(normal)) - g.log_spot - g.log_width), normal);
+    const double score = fma(-g.width, normal, normal * normal - 1.0);
+    if ((g.request & 2) && score != 0.0)
+      lr[1] = copysign(exp(log_payoff + log(fabs(score)) - 2.0 * g.log_spot - 2.0 * g.log_width), score);
+    if (!isfinite(lr[0]) || !isfinite(lr[1])) bad = 1;
+  }
+  return payoff;
 }
 
 // ------------------------------------------------------------------ Asians

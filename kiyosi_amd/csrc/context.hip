@@ -15,8 +15,6 @@ namespace kg {
 // order; the runs are then merged as a fixed tree. The result depends only
 // on the partials, not on scheduling, so a seed replays exactly
 // (monte_carlo.rs:1800-1809 merges the same partials sequentially).
-constexpr int MERGE_THREADS = 1024;
-
 __global__ void __launch_bounds__(MERGE_THREADS) merge_partials(const Moments* __restrict__ partials,
                                                                 int64_t blocks, int slots,
                                                                 Moments* __restrict__ out) {

@@ -14,6 +14,9 @@ namespace kg {
 
 constexpr int BLOCK_SAMPLES = 256;  // monte_carlo.rs:20
 constexpr int WAVE = 64;
+// Threads of every kernel that merges per-block partials in block order; one
+// layout everywhere keeps the merge tree, and so the result bits, the same.
+constexpr int MERGE_THREADS = 1024;
 
 // ------------------------------------------------------------------ PCG32
 struct Pcg32 {

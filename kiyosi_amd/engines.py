@@ -228,14 +228,78 @@ class MonteCarloAutocallableEngine(_MonteCarloEngine):
                                                                    terms, scenarios))
 
 
-class MonteCarloBarrierEngine(_MonteCarloEngine):
-    """Single-barrier Monte Carlo (engines/monte_carlo/barrier.rs)."""
+class _SpotGreekEngine(_MonteCarloEngine):
+    """Engines whose Greeks are the likelihood-ratio delta and gamma computed
+    on the draws of the price (engines.rs:138-162)."""
+
+    def _greek_entry(self, instrument):
+        raise NotImplementedError
+
+    def price_with_greeks(self, option, context, request=None, settings=None):
+        """Price with native delta and gamma from one GPU launch; other Greeks
+        are unsupported, and at expiry the terminal-payoff stencil supplies
+        delta and gamma. The price equals ``price`` with the same seed."""
+        from . import greeks
+        settings = settings or greeks.NumericalShiftSettings()
+        requested = greeks.normalize_request(request)
+        engine = self._resolved()
+        fn, terms = engine._greek_entry(option)
+        spot = context.spot_price
+        radius = greeks.symmetric_shift(spot, settings.spot_shift_at(spot))
+        bits = (_abi.GREEK_DELTA if "delta" in requested else 0) | (_abi.GREEK_GAMMA if "gamma" in requested else 0)
+        spot_greeks = _abi.SpotGreeks(bits, 0, radius, 0.0, 0.0)
+        out = _abi.Estimate()
+        ctx = _native.context(engine.device)
+        ctx.check(fn(ctx.handle, C.byref(engine._settings()), C.byref(terms), C.byref(context.bsm_market()),
+                     C.byref(spot_greeks), C.byref(out)))
+        values = {}
+        if spot_greeks.available & _abi.GREEK_DELTA:
+            values["delta"] = spot_greeks.delta
+        if spot_greeks.available & _abi.GREEK_GAMMA:
+            values["gamma"] = spot_greeks.gamma
+        return greeks.native_spot_result(option, context, requested, settings, out.price, values,
+                                         out.seed if out.has_seed else None,
+                                         lambda scn: engine.price(option, scn))
+
+    def _resolved(self):
+        return self if self.seed is not None else self.with_resolved_seed()
+
+
+class MonteCarloBarrierEngine(_SpotGreekEngine):
+    """Single- and double-barrier Monte Carlo (engines/monte_carlo/barrier.rs)."""
 
     def _entry(self, instrument, context):
-        from .instruments import BarrierOption
+        from .instruments import BarrierOption, DoubleBarrierOption
+        if isinstance(instrument, DoubleBarrierOption):
+            return _native.lib().kg_double_barrier_price, instrument._terms(), context.bsm_market()
         if not isinstance(instrument, BarrierOption):
-            raise TypeError("MonteCarloBarrierEngine prices BarrierOption")
+            raise TypeError("MonteCarloBarrierEngine prices BarrierOption and DoubleBarrierOption")
         return _native.lib().kg_barrier_price, instrument._terms(), context.bsm_market()
+
+    def _greek_entry(self, instrument):
+        from .instruments import BarrierOption, DoubleBarrierOption
+        if isinstance(instrument, DoubleBarrierOption):
+            return _native.lib().kg_double_barrier_price_with_greeks, instrument._terms()
+        if not isinstance(instrument, BarrierOption):
+            raise TypeError("MonteCarloBarrierEngine prices BarrierOption and DoubleBarrierOption")
+        return _native.lib().kg_barrier_price_with_greeks, instrument._terms()
+
+
+class MonteCarloDigitalEngine(_SpotGreekEngine):
+    """Cash-or-nothing and asset-or-nothing Monte Carlo
+    (engines/monte_carlo/digital.rs)."""
+
+    def _entry(self, instrument, context):
+        from .instruments import DigitalOption
+        if not isinstance(instrument, DigitalOption):
+            raise TypeError("MonteCarloDigitalEngine prices DigitalOption")
+        return _native.lib().kg_digital_price, instrument._terms(), context.bsm_market()
+
+    def _greek_entry(self, instrument):
+        from .instruments import DigitalOption
+        if not isinstance(instrument, DigitalOption):
+            raise TypeError("MonteCarloDigitalEngine prices DigitalOption")
+        return _native.lib().kg_digital_price_with_greeks, instrument._terms()
 
 
 class MonteCarloAsianEngine(_MonteCarloEngine):
@@ -269,6 +333,7 @@ class MonteCarloAccumulatorEngine(_MonteCarloEngine):
 
 _BOOK_KINDS = {
     "MonteCarloBarrierEngine": _abi.BOOK_BARRIER,
+    "MonteCarloDigitalEngine": _abi.BOOK_DIGITAL,
     "MonteCarloAsianEngine": _abi.BOOK_ASIAN,
     "MonteCarloAutocallableEngine": _abi.BOOK_AUTOCALL,
     "MonteCarloAccumulatorEngine": _abi.BOOK_ACCUMULATOR,
@@ -310,6 +375,8 @@ class Book:
             if kind is None:
                 raise TypeError(f"{type(engine).__name__} cannot join a book")
             engine = engine if engine.seed is not None else engine.with_resolved_seed()
+            if kind == _abi.BOOK_BARRIER and type(instrument).__name__ == "DoubleBarrierOption":
+                kind = _abi.BOOK_DOUBLE_BARRIER
             if kind == _abi.BOOK_ASIAN:
                 terms = instrument._terms(engine.step_count)
             else:

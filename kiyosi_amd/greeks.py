@@ -60,10 +60,11 @@ class PricingResult:
     derivative does not exist at this valuation (PricingResult::status,
     risk/analytics.rs:195-207)."""
 
-    def __init__(self, price, requested, values, unavailable, seed):
+    def __init__(self, price, requested, values, unavailable, seed, supported=GREEKS):
         self.price = price
         self.requested = frozenset(requested)
-        self._values = dict(values)
+        self.supported = frozenset(supported)
+        self._values = {g: v for g, v in values.items() if g in self.supported}
         self._unavailable = frozenset(unavailable) - set(self._values)
         self.seed = seed
 
@@ -80,6 +81,8 @@ class PricingResult:
             return "not_requested"
         if greek in self._values:
             return "available"
+        if greek not in self.supported:
+            return "unsupported"
         return "stencil_unavailable" if greek in self._unavailable else "undefined"
 
     def __eq__(self, other):
@@ -309,6 +312,23 @@ def complete(option, context, request, settings, p0, price):
                    if not ((g in _SPOT and spot_undefined)
                            or (g in ("vega", "volga", "vanna", "zomma") and vol == 0.0))}
     return values, unavailable
+
+
+def native_spot_result(option, context, request, settings, price, values, seed, reprice):
+    """Completes an engine's native delta and gamma (engines.rs:138-162):
+    only delta and gamma are supported; off expiry a missing one is
+    "stencil_unavailable" unless the spot itself sits on a discontinuity;
+    at expiry the terminal-payoff stencil supplies them
+    (numerical.rs:28-112). `reprice(context)` prices one scenario."""
+    request = normalize_request(request)
+    supported = {"delta", "gamma"}
+    values = {g: v for g, v in values.items() if g in request}
+    if to_micros(context.valuation_time) != to_micros(option.expiry_time):
+        unavailable = set() if _at_spot_discontinuity(option, context, 0.0) else supported
+        return PricingResult(price, request, values, unavailable & request, seed, supported)
+    extra, unavailable = expiry_greeks(option, context, request & supported, settings, price, reprice)
+    values.update(extra)
+    return PricingResult(price, request, values, unavailable & request, seed, supported)
 
 
 def plan(option, context, request, settings):

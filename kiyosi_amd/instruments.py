@@ -394,7 +394,62 @@ _BARRIER_TYPES = {"down_and_out": _abi.DOWN_AND_OUT, "down_and_in": _abi.DOWN_AN
                   "up_and_out": _abi.UP_AND_OUT, "up_and_in": _abi.UP_AND_IN}
 
 
-class BarrierOption:
+class _Monitored:
+    """Monitoring state shared by single and double barriers
+    (instruments/monitoring.rs:272-321, 430-445), for contracts without prior
+    touches and Black-Scholes contexts with positive volatility, where no
+    deterministic interval exists."""
+
+    _upper = None
+
+    def _times(self):
+        return None if self.observation_schedule is None else [to_micros(t) for t in self.observation_schedule]
+
+    def _monitored(self, time):
+        times = self._times()
+        return (to_micros(self.effective_time) <= time <= to_micros(self.expiry_time)
+                and (times is None or time in times))
+
+    def _levels(self):
+        return [self.barrier_level] if self._upper is None else [self.barrier_level, self._upper]
+
+    def _state(self, context):
+        """(touched, finished) at the valuation (Monitoring::state)."""
+        time, spot = to_micros(context.valuation_time), context.spot_price
+        if self._upper is None:
+            hit = spot >= self.barrier_level if self.barrier_type.startswith("up") else spot <= self.barrier_level
+        else:
+            hit = spot <= self.barrier_level or spot >= self._upper
+        times = self._times()
+        finished = time == to_micros(self.expiry_time) or (times is not None and times[-1] <= time)
+        return self._monitored(time) and hit, finished
+
+    def _knock_in(self):
+        return self.barrier_type.endswith("_in")
+
+    def at_spot_discontinuity(self, context, radius):
+        """A monitored level within the spot stencil, or the vanilla kink at
+        expiry while the payoff is live (spot_discontinuity and
+        live_payoff_discontinuity, monitoring.rs:322-340, 430-445)."""
+        spot, time = context.spot_price, to_micros(context.valuation_time)
+        if self._monitored(time) and any(spot - radius <= level <= spot + radius for level in self._levels()):
+            return True
+        vanilla = EuropeanOption(option_type=self.option_type, strike=self.strike,
+                                 effective_time=self.effective_time, expiry_time=self.expiry_time)
+        touched, finished = self._state(context)
+        if self._knock_in():
+            return (touched or not finished) and vanilla.at_spot_discontinuity(context, radius)
+        return not touched and vanilla.at_spot_discontinuity(context, radius)
+
+    def _schedule(self):
+        times = self.observation_schedule
+        if times is None:
+            return None, 0
+        arr = (C.c_int64 * max(len(times), 1))(*[to_micros(t) for t in times])
+        return arr, len(times)
+
+
+class BarrierOption(_Monitored):
     """Single barrier call or put (instruments/barrier.rs); continuous monitoring
     without an observation schedule, scheduled fixings otherwise."""
 
@@ -421,19 +476,107 @@ class BarrierOption:
         self.calendar = calendar
 
     def _terms(self):
-        times = self.observation_schedule
-        arr, n = None, 0
-        if times is not None:
-            arr = (C.c_int64 * max(len(times), 1))(*[to_micros(t) for t in times])
-            n = len(times)
+        arr, n = self._schedule()
         terms = _abi.BarrierTerms(
             1 if self.option_type == "call" else -1, _BARRIER_TYPES[self.barrier_type],
-            _abi.OBSERVE_CONTINUOUS if times is None else _abi.OBSERVE_SCHEDULED,
+            _abi.OBSERVE_CONTINUOUS if arr is None else _abi.OBSERVE_SCHEDULED,
             _abi.REBATE_AT_HIT if self.rebate_timing == "at_hit" else _abi.REBATE_AT_EXPIRY,
             self.strike, self.barrier_level, self.rebate, to_micros(self.effective_time),
             to_micros(self.expiry_time), n, 0, C.cast(arr, C.POINTER(C.c_int64)) if arr is not None else None)
         terms._keep = arr
         return terms
+
+
+class DoubleBarrierOption(_Monitored):
+    """Call or put knocked in or out by either of two levels
+    (instruments/double_barrier.rs:74-144); equality touches. Continuous
+    monitoring without an observation schedule, scheduled fixings otherwise."""
+
+    def __init__(self, *, option_type, strike, effective_time, expiry_time, lower_barrier, upper_barrier,
+                 barrier_type, rebate=0.0, rebate_timing="at_expiry", observation_schedule=None,
+                 touch_state=None, calendar=None):
+        if option_type not in ("call", "put"):
+            raise InvalidInputError("option_type must be 'call' or 'put'")
+        if barrier_type not in ("knock_in", "knock_out"):
+            raise InvalidInputError("barrier_type must be 'knock_in' or 'knock_out'")
+        if rebate_timing not in ("at_hit", "at_expiry"):
+            raise InvalidInputError("rebate_timing must be 'at_hit' or 'at_expiry'")
+        lower, upper = _positive("lower_barrier", lower_barrier), _finite("upper_barrier", upper_barrier)
+        if not lower < upper:
+            raise InvalidInputError("upper_barrier must exceed lower_barrier")
+        if barrier_type == "knock_in" and rebate_timing == "at_hit":
+            raise InvalidInputError("knock-in rebates are paid at expiry")
+        if touch_state == "touched":
+            raise InvalidInputError("touched barriers are resolved by the CPU engine")
+        self.option_type = option_type
+        self.strike = _positive("strike", strike)
+        self.barrier_level = lower
+        self._upper = upper
+        self.lower_barrier, self.upper_barrier = lower, upper
+        # The lower barrier carries the contract (double_barrier.rs:124-135).
+        self.barrier_type = "down_and_in" if barrier_type == "knock_in" else "down_and_out"
+        self.double_barrier_type = barrier_type
+        self.rebate = _finite("rebate", rebate)
+        if self.rebate < 0.0:
+            raise InvalidInputError("rebate must be nonnegative")
+        self.rebate_timing = rebate_timing
+        self.effective_time = effective_time
+        self.expiry_time = expiry_time
+        self.observation_schedule = None if observation_schedule is None else list(observation_schedule)
+        self.calendar = calendar
+
+    def _terms(self):
+        arr, n = self._schedule()
+        terms = _abi.DoubleBarrierTerms(
+            1 if self.option_type == "call" else -1,
+            _abi.DOUBLE_KNOCK_IN if self.double_barrier_type == "knock_in" else _abi.DOUBLE_KNOCK_OUT,
+            _abi.OBSERVE_CONTINUOUS if arr is None else _abi.OBSERVE_SCHEDULED,
+            _abi.REBATE_AT_HIT if self.rebate_timing == "at_hit" else _abi.REBATE_AT_EXPIRY,
+            self.strike, self.lower_barrier, self.upper_barrier, self.rebate, to_micros(self.effective_time),
+            to_micros(self.expiry_time), n, 0, C.cast(arr, C.POINTER(C.c_int64)) if arr is not None else None)
+        terms._keep = arr
+        return terms
+
+
+class DigitalOption:
+    """European cash-or-nothing or asset-or-nothing option
+    (instruments/vanilla.rs:236-366): pays `payout` (cash) or the terminal
+    spot (asset) when strictly in the money."""
+
+    def __init__(self, *, option_type, strike, effective_time, expiry_time, payout=None):
+        if option_type not in ("call", "put"):
+            raise InvalidInputError("option_type must be 'call' or 'put'")
+        if payout is not None:
+            payout = _finite("payout", payout)
+            if payout < 0.0:
+                raise InvalidInputError("payout must be nonnegative")
+        self.option_type = option_type
+        self.strike = _positive("strike", strike)
+        self.effective_time = effective_time
+        self.expiry_time = expiry_time
+        if to_micros(expiry_time) < to_micros(effective_time):
+            raise InvalidInputError("expiry precedes the effective time")
+        self.payout = payout
+
+    @classmethod
+    def cash_or_nothing(cls, option_type, strike, payout, effective_time, expiry_time):
+        return cls(option_type=option_type, strike=strike, payout=payout, effective_time=effective_time,
+                   expiry_time=expiry_time)
+
+    @classmethod
+    def asset_or_nothing(cls, option_type, strike, effective_time, expiry_time):
+        return cls(option_type=option_type, strike=strike, effective_time=effective_time, expiry_time=expiry_time)
+
+    def _terms(self):
+        return _abi.DigitalTerms(1 if self.option_type == "call" else -1,
+                                 _abi.ASSET_OR_NOTHING if self.payout is None else _abi.CASH_OR_NOTHING,
+                                 self.strike, float(self.payout or 0.0), to_micros(self.effective_time),
+                                 to_micros(self.expiry_time))
+
+    def at_spot_discontinuity(self, context, radius):
+        """The terminal jump at the strike (instruments/vanilla.rs:378-380)."""
+        return EuropeanOption(option_type=self.option_type, strike=self.strike, effective_time=self.effective_time,
+                              expiry_time=self.expiry_time).at_spot_discontinuity(context, radius)
 
 
 class _AveragePrice:

@@ -172,6 +172,51 @@ int oracle_barrier_price(const kg_settings* s, const kg_barrier_terms* o, const 
   });
 }
 
+void write_greeks(const SpotGreeks& g, kg_spot_greeks* out) {
+  out->available = g.available;
+  out->delta = g.delta;
+  out->gamma = g.gamma;
+}
+
+int oracle_barrier_price_with_greeks(const kg_settings* s, const kg_barrier_terms* o, const kg_bsm_market* m,
+                                     int workers, kg_spot_greeks* greeks, kg_estimate* out) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    SpotGreeks g;
+    write(barrier_native(st, barrier_spec(*o), *m, context(), greeks->request, greeks->spot_radius, &g), out);
+    write_greeks(g, greeks);
+  });
+}
+
+int oracle_double_barrier_price(const kg_settings* s, const kg_double_barrier_terms* o, const kg_bsm_market* m,
+                                int workers, kg_spot_greeks* greeks, kg_estimate* out) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    if (!(o->lower_barrier > 0.0) || !(o->upper_barrier > o->lower_barrier))
+      throw Failure(INVALID_INPUT, "double barriers need 0 < lower < upper");
+    SpotGreeks g;
+    write(barrier_native(st, barrier_spec(*o), *m, context(), greeks ? greeks->request : 0,
+                         greeks ? greeks->spot_radius : 0.0, &g),
+          out);
+    if (greeks) write_greeks(g, greeks);
+  });
+}
+
+int oracle_digital_price(const kg_settings* s, const kg_digital_terms* o, const kg_bsm_market* m, int workers,
+                         kg_spot_greeks* greeks, kg_estimate* out) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    SpotGreeks g;
+    write(digital_native(st, *o, *m, context(), greeks ? greeks->request : 0, &g), out);
+    if (greeks) write_greeks(g, greeks);
+  });
+}
+
+int oracle_double_survival(double initial, double terminal, double band, double deviation, int derivatives,
+                           int branch, double* out) {
+  return guarded([&] { double_survival(initial, terminal, band, deviation, derivatives != 0, out, branch); });
+}
+
 int oracle_asian_price(const kg_settings* s, const kg_asian_terms* o, const kg_bsm_market* m,
                        int workers, kg_estimate* out) {
   return guarded([&] {

@@ -1,6 +1,7 @@
 // Test infrastructure, not product code: FP64 CPU restatements of kiyosi's
-// barrier, Asian and accumulator Monte Carlo estimators (value only), the
-// checker of the HIP book kernel.
+// barrier (single and double, with likelihood-ratio delta/gamma), digital,
+// Asian and accumulator Monte Carlo estimators, the checker of the HIP book
+// and spot-Greek kernels.
 #pragma once
 
 #include "../include/kiyosi_amd.h"
@@ -125,8 +126,147 @@ inline Estimate known_estimate(const Settings& st, double value) {
 }
 
 // ---------------------------------------------------------------- barrier
-inline Estimate barrier_price(const Settings& st, const kg_barrier_terms& o, const kg_bsm_market& m,
-                              const Context& cx) {
+// double_barrier::survival (engines/monte_carlo/double_barrier.rs:6-112):
+// probability that a Brownian bridge from `initial` to `terminal` (log
+// distances above the lower level) stays inside (0, band), with its first
+// and second derivatives in `initial` at a fixed terminal value. Image
+// series for small variance, spectral series otherwise. `branch` forces the
+// image (0) or spectral (1) series for tests; -1 chooses as the reference.
+inline void double_survival(double initial, double terminal, double band, double deviation,
+                            bool derivatives, double out[3], int branch = -1) {
+  out[0] = out[1] = out[2] = 0.0;
+  if (terminal <= 0.0 || terminal >= band) return;
+  double x = initial / band, y = terminal / band;
+  const double ratio = deviation / band;
+  const double variance = ratio * ratio;
+  if (variance == 0.0) {
+    out[0] = 1.0;
+    return;
+  }
+  if (!std::isfinite(variance) || variance <= 0.0) unstable();
+  const bool reflected = x + y > 1.0;
+  if (reflected) {
+    x = 1.0 - x;
+    y = 1.0 - y;
+  }
+  double sum[3] = {0.0, 0.0, 0.0};
+  bool converged = false;
+  const bool image = branch < 0 ? variance < 0.25 : branch == 0;
+  if (image) {
+    const double inverse_variance = std::isnormal(variance) ? 1.0 / variance : INFINITY;
+    sum[0] = -std::expm1(-2.0 * (x / ratio) * (y / ratio));
+    auto add = [&](double k, bool direct) {
+      const double exponent = direct ? -2.0 * (k / ratio) * ((y - x + k) / ratio)
+                                     : -2.0 * ((x + k) / ratio) * ((y + k) / ratio);
+      const double sign = direct ? 1.0 : -1.0;
+      const double weight = std::exp(exponent);
+      if (k != 0.0) sum[0] += sign * weight;
+      if (derivatives) {
+        const double numerator = direct ? 2.0 * k : -2.0 * (y + k);
+        if (numerator != 0.0) {
+          const double slope = numerator * inverse_variance;
+          const double first = weight * slope;
+          const double second = first * slope;
+          if (std::isnormal(weight) && std::isnormal(first) && std::isnormal(second)) {
+            sum[1] += sign * first;
+            sum[2] += sign * second;
+          } else {
+            const double log_slope = std::log(std::fabs(numerator)) - 2.0 * std::log(ratio);
+            sum[1] += sign * std::copysign(std::exp(exponent + log_slope), numerator);
+            sum[2] += sign * std::exp(exponent + 2.0 * log_slope);
+          }
+        }
+      }
+      return weight;
+    };
+    add(0.0, false);
+    for (int n = 1; n <= 64; ++n) {
+      const double k = n;
+      const double tail = add(k, true) + add(k, false) + add(-k, true) + add(-k, false);
+      if (n >= 2 && tail < 1e-18) {
+        converged = true;
+        break;
+      }
+    }
+  } else {
+    const double distance = y - x;
+    const double log_scale =
+        std::log(2.0 * std::sqrt(2.0 * M_PI) * ratio) + 0.5 * distance * distance / variance;
+    for (int n = 1; n <= 256; ++n) {
+      const double frequency = (double)n * M_PI;
+      const double exponent = log_scale - 0.5 * frequency * frequency * variance;
+      const double weight = std::exp(exponent);
+      const double sy = std::sin(frequency * y);
+      const double product = std::sin(frequency * x) * sy;
+      sum[0] += weight * product;
+      if (derivatives) {
+        const double first = frequency * std::cos(frequency * x) * sy;
+        const double score = -distance / variance;
+        sum[1] += weight * (first + score * product);
+        sum[2] += weight * (2.0 * score * first +
+                            (score * score + 1.0 / variance - frequency * frequency) * product);
+      }
+      if (n >= 4 && exponent + 4.0 * std::log(frequency) < -40.0) {
+        converged = true;
+        break;
+      }
+    }
+  }
+  if (!converged || !(sum[0] >= -1e-13 && sum[0] <= 1.0 + 1e-13)) unstable();
+  out[0] = std::min(std::max(sum[0], 0.0), 1.0);
+  if (derivatives) {
+    out[1] = (reflected ? -sum[1] : sum[1]) / band;
+    out[2] = (sum[2] / band) / band;
+  }
+}
+
+// One barrier contract as MonteCarloBarrierEngine::native sees it
+// (barrier.rs:160-178): a single barrier, or a double barrier as its lower
+// (down-type) barrier plus the upper level.
+struct BarrierSpec {
+  int option_type, barrier_type, observation, rebate_timing;
+  double strike, barrier, rebate;
+  bool has_upper;
+  double upper;
+  int64_t effective_time, expiry_time;
+  int n_observations;
+  const int64_t* observation_times;
+};
+inline BarrierSpec barrier_spec(const kg_barrier_terms& o) {
+  return {o.option_type, o.barrier_type, o.observation, o.rebate_timing, o.strike, o.barrier, o.rebate, false,
+          0.0, o.effective_time, o.expiry_time, o.n_observations, o.observation_times};
+}
+inline BarrierSpec barrier_spec(const kg_double_barrier_terms& o) {
+  return {o.option_type, o.knock_type == KG_DOUBLE_KNOCK_IN ? KG_DOWN_AND_IN : KG_DOWN_AND_OUT, o.observation,
+          o.rebate_timing, o.strike, o.lower_barrier, o.rebate, true, o.upper_barrier, o.effective_time,
+          o.expiry_time, o.n_observations, o.observation_times};
+}
+
+// Delta and gamma of the Black-Scholes vanilla (AnalyticEuropeanEngine's
+// closed form), for barriers resolved to their vanilla payoff.
+inline void vanilla_spot_greeks(double S, double K, double r, double q, double sigma, double T, double sign,
+                                double* delta, double* gamma) {
+  const double w = sigma * std::sqrt(T);
+  const double d1 = (log_ratio(S, K) + (r - q) * T) / w + 0.5 * w;
+  const double pdf = std::exp(-0.5 * d1 * d1) / std::sqrt(2.0 * M_PI);
+  *delta = sign * std::exp(-q * T) * normal_cdf(sign * d1);
+  *gamma = std::exp(-q * T) * pdf / (S * w);
+}
+
+// Result of the barrier and digital engines' native pricing: the estimate
+// and the likelihood-ratio delta/gamma (`available` bits say which).
+struct SpotGreeks {
+  int available = 0;
+  double delta = 0.0, gamma = 0.0;
+};
+
+// MonteCarloBarrierEngine::native (barrier.rs:160-541) for flat BSM rates and
+// no prior touch. `request` holds KG_GREEK_* bits; `radius` is the symmetric
+// spot shift whose interval, when it crosses a monitored level, makes the
+// spot Greeks unavailable (at_spot_discontinuity, instruments/monitoring.rs
+// :430-445).
+inline Estimate barrier_native(const Settings& st, const BarrierSpec& o, const kg_bsm_market& m, const Context& cx,
+                               int request, double radius, SpotGreeks* greeks) {
   const int64_t val = m.valuation_time, expiry = o.expiry_time;
   if (val < o.effective_time || val > expiry)
     throw Failure(INVALID_INPUT, "valuation outside contract life");
@@ -138,27 +278,52 @@ inline Estimate barrier_price(const Settings& st, const kg_barrier_terms& o, con
   const double sign = o.option_type;
   const double r = m.rate, q = m.dividend, sigma = m.volatility, S = m.spot;
   const double T = years_until(val, expiry);
-  // resolved_price_with_state (engines/analytic/barrier.rs:32-80) for a note
-  // without earlier touches: touched now, expired, or a knock-out whose
-  // payoff needs the touch.
+  // Monitoring state (instruments/monitoring.rs:293-321) for a contract
+  // without earlier touches.
   bool observed_now = continuous;
   for (int i = 0; i < o.n_observations && !continuous; ++i) observed_now |= o.observation_times[i] == val;
-  const bool touched = observed_now && (up ? S >= o.barrier : S <= o.barrier);
-  const bool finished = val == expiry;
-  bool last_is_expiry = continuous || (o.n_observations > 0 && o.observation_times[o.n_observations - 1] == expiry);
-  const bool requires_touch = last_is_expiry && (up ? (sign > 0 && o.strike >= o.barrier)
-                                                    : (sign < 0 && o.strike <= o.barrier));
+  const bool touched = observed_now && ((up ? S >= o.barrier : S <= o.barrier) || (o.has_upper && S >= o.upper));
+  const bool finished = val == expiry || (!continuous && o.n_observations > 0 &&
+                                          o.observation_times[o.n_observations - 1] <= val);
+  const bool last_is_expiry =
+      continuous || (o.n_observations > 0 && o.observation_times[o.n_observations - 1] == expiry);
+  bool requires_touch;
+  if (o.has_upper)  // DoubleBarrierOption::payoff_requires_touch (instruments/double_barrier.rs:216-224)
+    requires_touch = last_is_expiry && (sign > 0 ? o.strike >= o.upper : o.strike <= o.barrier);
+  else
+    requires_touch = last_is_expiry && (up ? (sign > 0 && o.strike >= o.barrier) : (sign < 0 && o.strike <= o.barrier));
+  // The request keeps delta and gamma off expiry and off a spot stencil
+  // that crosses a monitored level (barrier.rs:194-205).
+  auto crosses = [&](double level) { return S - radius <= level && level <= S + radius; };
+  const bool discontinuity = observed_now && (crosses(o.barrier) || (o.has_upper && crosses(o.upper)));
+  int req = (val != expiry && !discontinuity) ? request & (KG_GREEK_DELTA | KG_GREEK_GAMMA) : 0;
+  const bool delta = req & KG_GREEK_DELTA, gamma = req & KG_GREEK_GAMMA;
+  if (greeks) *greeks = SpotGreeks{};
   auto cash = [&] { return o.rebate == 0.0 ? 0.0 : o.rebate * std::exp(-r * T); };
   auto vanilla = [&] {
     if (T == 0.0) return std::max(sign * (S - o.strike), 0.0);
     return lognormal_price(S, o.strike, log_ratio(S, o.strike) + (r - q) * T, -q * T, -r * T, sigma,
                            std::sqrt(T), sign);
   };
-  if (touched && !knock_in) return known_estimate(st, at_hit ? o.rebate : cash());
-  if (!touched && finished && knock_in) return known_estimate(st, cash());
+  // resolved_price_with_state (engines/analytic/barrier.rs:32-80) and the
+  // Greeks of a resolved contract (barrier.rs:206-229).
+  auto resolved = [&](double value, bool vanilla_payoff) {
+    if (greeks && req) {
+      greeks->available = req;
+      if (vanilla_payoff) {  // volatility_affects_price: the vanilla's analytic delta/gamma
+        double d, g;
+        vanilla_spot_greeks(S, o.strike, r, q, sigma, T, sign, &d, &g);
+        greeks->delta = delta ? d : 0.0;
+        greeks->gamma = gamma ? g : 0.0;
+      }
+    }
+    return known_estimate(st, value);
+  };
+  if (touched && !knock_in) return resolved(at_hit ? o.rebate : cash(), false);
+  if (!touched && finished && knock_in) return resolved(cash(), false);
   const bool fixed = o.rebate == 0.0 && requires_touch;
-  if (fixed && !knock_in) return known_estimate(st, 0.0);
-  if (touched || finished || fixed) return known_estimate(st, vanilla());
+  if (fixed && !knock_in) return resolved(0.0, false);
+  if (touched || finished || fixed) return resolved(vanilla(), true);
   if (continuous && !knock_in && at_hit && o.rebate != 0.0)
     throw Failure(UNSUPPORTED, "continuous at-hit rebates need the analytic hit value");
   // Plan (barrier.rs:228-290).
@@ -168,7 +333,7 @@ inline Estimate barrier_price(const Settings& st, const kg_barrier_terms& o, con
     return log_ratio(S, level) + carry * e - hvr * e;
   };
   struct Step {
-    double mean, width, discount, lower;
+    double mean, width, discount, lower, upper;
     bool observe;
     int64_t date;
   };
@@ -188,33 +353,38 @@ inline Estimate barrier_price(const Settings& st, const kg_barrier_terms& o, con
     double elapsed = years_until(val, date);
     double mean = drift * elapsed;
     if (width <= 0.0 || !std::isfinite(width) || !std::isfinite(mean)) unstable();
-    steps.push_back({mean, width, -r * elapsed, observe ? mean_distance(date, o.barrier) : 0.0, observe, date});
+    steps.push_back({mean, width, -r * elapsed, observe ? mean_distance(date, o.barrier) : 0.0,
+                     observe && o.has_upper ? mean_distance(date, o.upper) : 0.0, observe, date});
     width_sum += width;
     prev = date;
   }
   const double moneyness = mean_distance(expiry, o.strike);
   const double barrier = log_ratio(o.barrier, S);
   const double terminal_barrier = mean_distance(expiry, o.barrier);
+  const double band = o.has_upper ? log_ratio(o.upper, o.barrier) : 0.0;
   const double log_discount = -r * T;
   const bool bounded = std::fabs(std::log(S)) + 10.0 * width_sum + std::fabs(drift * T) < 600.0;
   const bool condition_on_hit = knock_in && !continuous && steps.size() > 8 && bounded;
   auto weighted = [&](double amount, double lw) {
     return scaled_value(std::fabs(amount), amount < 0 ? -1.0 : 1.0, log_discount + lw);
   };
-  Moments total[1], pairs[1];
-  path_samples_lazy<1>(
+  Moments total[3], pairs[3];
+  path_samples_lazy<3>(
       st, steps.size(), true, cx,
       [&](LazyPath& normals, double* out) {
         double terminal = 0.0, noise = 0.0, knocked_out = 0.0;
         bool touched_p = false, has_cond = false;
         double conditioned = 0.0;
+        const double first_normal = normals.normal();
         for (size_t index = 0; index < steps.size(); ++index) {
           const Step& s = steps[index];
-          double z = normals.normal();
+          double z = index == 0 ? first_normal : normals.normal();
           noise = std::fma(s.width, z, noise);
           terminal = s.mean + noise;
           if (!bounded && scaled_value(S, 1.0, terminal) <= 0.0) unstable();
-          if (s.observe && (up ? noise >= -s.lower : noise <= -s.lower)) {
+          const bool hit = o.has_upper ? (noise <= -s.lower || noise >= -s.upper)
+                                       : (up ? noise >= -s.lower : noise <= -s.lower);
+          if (s.observe && hit) {
             touched_p = true;
             if (condition_on_hit && steps.size() - index > 8) {
               double asset = scaled_value(S, 1.0, terminal);
@@ -235,9 +405,22 @@ inline Estimate barrier_price(const Settings& st, const kg_barrier_terms& o, con
         double distance = moneyness + noise;
         double payoff = std::fabs(distance) < 0.5 ? o.strike * std::expm1(distance) : asset - o.strike;
         double intrinsic = std::max(sign * payoff, 0.0);
-        double value;
+        double value, first = 0.0, second = 0.0;
         if (has_cond) {
           value = conditioned;
+        } else if (continuous && o.has_upper) {
+          double sv[3];
+          double_survival(-barrier, terminal_barrier + noise, band, steps[0].width, delta || gamma, sv);
+          double rebate = (!knock_in && at_hit) ? 0.0 : o.rebate;
+          value = weighted(intrinsic, knock_in ? std::log1p(-sv[0]) : std::log(sv[0])) +
+                  weighted(rebate, knock_in ? std::log(sv[0]) : std::log1p(-sv[0]));
+          const double difference = (knock_in ? -1.0 : 1.0) * (intrinsic - rebate);
+          auto derivative = [&](double w) {
+            return scaled_value(std::fabs(difference), (difference < 0 ? -1.0 : 1.0) * (w < 0 ? -1.0 : 1.0),
+                                log_discount + std::log(std::fabs(w)));
+          };
+          if (delta || gamma) first = derivative(sv[1]);
+          if (gamma) second = derivative(sv[2]);
         } else if (continuous) {
           double w0 = steps[0].width;
           double dtb = terminal_barrier + noise;
@@ -248,16 +431,102 @@ inline Estimate barrier_price(const Settings& st, const kg_barrier_terms& o, con
           double rebate = (!knock_in && at_hit) ? 0.0 : o.rebate;
           value = weighted(intrinsic, knock_in ? exponent : survival) +
                   weighted(rebate, knock_in ? survival : exponent);
+          if ((delta || gamma) && slope > 0.0 && exponent > -INFINITY) {
+            const double difference = (knock_in ? -1.0 : 1.0) * (intrinsic - rebate);
+            first = weighted(up ? -difference : difference, exponent + std::log(slope));
+            if (gamma) second = weighted(-difference, exponent + 2.0 * std::log(slope));
+          }
         } else if (touched_p && !knock_in) {
           value = knocked_out;
         } else {
           value = weighted(touched_p == knock_in ? intrinsic : o.rebate, 0.0);
         }
-        if (!std::isfinite(value)) unstable();
+        // Likelihood-ratio delta and gamma (barrier.rs:459-472).
+        const double width = steps[0].width, z = first_normal, score = z / width;
+        const double d = delta ? (value * score + first) / S : 0.0;
+        double g = 0.0;
+        if (gamma) {
+          const double second_score = ((z * z - 1.0) / width - z) / width;
+          g = ((value * second_score + first * (2.0 * score - 1.0) + second) / S) / S;
+        }
+        if (!std::isfinite(value) || !std::isfinite(d) || !std::isfinite(g)) unstable();
         out[0] = value;
+        out[1] = d;
+        out[2] = g;
       },
       total, pairs);
   Means mm = path_mean(st, total[0], pairs[0]);
+  if (greeks) {
+    greeks->available = req;
+    greeks->delta = delta ? scaled_value(total[1].scale, total[1].mean, 0.0) : 0.0;
+    greeks->gamma = gamma ? scaled_value(total[2].scale, total[2].mean, 0.0) : 0.0;
+  }
+  return from_sampling(st, mm.mean, mm.has_se, mm.se);
+}
+
+inline Estimate barrier_price(const Settings& st, const kg_barrier_terms& o, const kg_bsm_market& m,
+                              const Context& cx) {
+  return barrier_native(st, barrier_spec(o), m, cx, 0, 0.0, nullptr);
+}
+
+// ---------------------------------------------------------------- digital
+// MonteCarloDigitalEngine::native (engines/monte_carlo/digital.rs:55-150):
+// one terminal normal, strict in-the-money test in standardised
+// coordinates, likelihood-ratio delta and gamma.
+inline Estimate digital_native(const Settings& st, const kg_digital_terms& o, const kg_bsm_market& m,
+                               const Context& cx, int request, SpotGreeks* greeks) {
+  if (m.valuation_time > o.expiry_time) throw Failure(INVALID_INPUT, "valuation outside contract life");
+  if (m.volatility == 0.0) throw Failure(UNSUPPORTED, "zero volatility is deterministic");
+  const bool call = o.option_type > 0, asset = o.payout_kind == KG_ASSET_OR_NOTHING;
+  const double S = m.spot, sigma = m.volatility, r = m.rate, q = m.dividend;
+  if (greeks) *greeks = SpotGreeks{};
+  const double time = years_until(m.valuation_time, o.expiry_time);
+  const int req = request & (KG_GREEK_DELTA | KG_GREEK_GAMMA);
+  if (time == 0.0) {  // Payoff::digital value at expiry (engines.rs:212-227)
+    // No native Greeks at expiry: the expiry stencil supplies them.
+    const double value = (call ? S - o.strike : o.strike - S) > 0.0 ? (asset ? S : o.cash_payout) : 0.0;
+    return known_estimate(st, value);
+  }
+  const double root = std::sqrt(time), width = sigma * root, carry = (r - q) * time;
+  const double d2 = standardize(log_ratio(S, o.strike) + carry, sigma, root) - 0.5 * width;
+  const double amount = asset ? S : o.cash_payout;
+  const double drift = asset ? (-q - 0.5 * sigma * sigma) * time : -r * time;
+  if (!std::isfinite(width) || !std::isfinite(carry) || !std::isfinite(drift)) unstable();
+  const double log_spot = std::log(S), log_width = std::log(sigma) + std::log(root);
+  const bool want_delta = req & KG_GREEK_DELTA, want_gamma = req & KG_GREEK_GAMMA;
+  Moments total[3], pairs[3];
+  path_samples<3>(
+      st, 1, true, cx,
+      [&](const double* normals, double* out) {
+        const double normal = normals[0];
+        const double distance = d2 + normal;
+        out[0] = out[1] = out[2] = 0.0;
+        if (!(call ? distance > 0.0 : distance < 0.0)) return;
+        const double exponent = drift + (asset ? width * normal : 0.0);
+        const double payoff = scaled_value(amount, 1.0, exponent);
+        const double log_payoff = std::log(amount) + exponent;
+        const double d = want_delta && normal != 0.0
+                             ? std::copysign(std::exp(log_payoff + std::log(std::fabs(normal)) - log_spot - log_width),
+                                             normal)
+                             : 0.0;
+        const double score = std::fma(-width, normal, normal * normal - 1.0);
+        const double g = want_gamma && score != 0.0
+                             ? std::copysign(std::exp(log_payoff + std::log(std::fabs(score)) - 2.0 * log_spot -
+                                                      2.0 * log_width),
+                                             score)
+                             : 0.0;
+        if (!std::isfinite(d) || !std::isfinite(g)) unstable();
+        out[0] = payoff;
+        out[1] = d;
+        out[2] = g;
+      },
+      total, pairs);
+  Means mm = path_mean(st, total[0], pairs[0]);
+  if (greeks) {
+    greeks->available = req;
+    greeks->delta = want_delta ? scaled_value(total[1].scale, total[1].mean, 0.0) : 0.0;
+    greeks->gamma = want_gamma ? scaled_value(total[2].scale, total[2].mean, 0.0) : 0.0;
+  }
   return from_sampling(st, mm.mean, mm.has_se, mm.se);
 }
 

@@ -155,6 +155,36 @@ def barrier_price(settings, terms, market, workers: int = 1):
     return out
 
 
+def barrier_price_with_greeks(settings, terms, market, request=3, spot_radius=0.0, workers: int = 1):
+    """(estimate, kg_spot_greeks) of MonteCarloBarrierEngine::native with delta/gamma."""
+    out, greeks = _abi.Estimate(), _abi.SpotGreeks(request, 0, spot_radius, 0.0, 0.0)
+    _check(lib().oracle_barrier_price_with_greeks(C.byref(settings), C.byref(terms), C.byref(market), workers,
+                                                  C.byref(greeks), C.byref(out)))
+    return out, greeks
+
+
+def double_barrier_price(settings, terms, market, request=0, spot_radius=0.0, workers: int = 1):
+    out, greeks = _abi.Estimate(), _abi.SpotGreeks(request, 0, spot_radius, 0.0, 0.0)
+    _check(lib().oracle_double_barrier_price(C.byref(settings), C.byref(terms), C.byref(market), workers,
+                                             C.byref(greeks), C.byref(out)))
+    return out, greeks
+
+
+def digital_price(settings, terms, market, request=0, workers: int = 1):
+    out, greeks = _abi.Estimate(), _abi.SpotGreeks(request, 0, 0.0, 0.0, 0.0)
+    _check(lib().oracle_digital_price(C.byref(settings), C.byref(terms), C.byref(market), workers,
+                                      C.byref(greeks), C.byref(out)))
+    return out, greeks
+
+
+def double_survival(initial, terminal, band, deviation, derivatives=True, branch=-1):
+    """[survival, d/d initial, d2/d initial2] of the double-barrier bridge series."""
+    out = (C.c_double * 3)()
+    _check(lib().oracle_double_survival(C.c_double(initial), C.c_double(terminal), C.c_double(band),
+                                        C.c_double(deviation), int(derivatives), branch, out))
+    return list(out)
+
+
 def asian_price(settings, terms, market, workers: int = 1):
     out = _abi.Estimate()
     _check(lib().oracle_asian_price(C.byref(settings), C.byref(terms), C.byref(market), workers, C.byref(out)))

@@ -4,8 +4,9 @@ Reads kiyosi's benchmark calibration and reference files (data, not code):
   benchmarks/quantlib/calibration/monte_carlo.json          per-seed prices
   benchmarks/quantlib/calibration/american_monte_carlo.json per-seed LSMC prices
   benchmarks/quantlib/references/*.json                     analytic / FD values
-and records the two golden values of python/tests/test_monte_carlo.py
-(:46-52 and :761). Run once where the kiyosi sources are available:
+and records the golden values of python/tests/test_monte_carlo.py (:46-52
+and :761) and of crates/kiyosi/tests/double_barrier_monte_carlo.rs
+(:121-157). Run once where the kiyosi sources are available:
 
     python tests/golden/extract_fixtures.py /path/to/kiyosi
 """
@@ -94,6 +95,19 @@ def main() -> None:
         "se_constants": {
             "plain": 13.831466750894657, "pair": 7.286842354,
             "source": "crates/kiyosi/tests/monte_carlo.rs:800-805; python/tests/test_monte_carlo.py:648",
+        },
+        "double_barrier_knock_out_greeks": {
+            "inputs": {"strike": 100.0, "lower": 80.0, "upper": 120.0, "spot": 100.0, "rate": 0.04,
+                       "dividend": 0.01, "sigma": 0.2, "valuation": "2025-01-01", "expiry": "2026-01-01",
+                       "monitoring": "continuous", "sampling": "sobol", "path_count": 32768,
+                       "seeds": [17, 42, 103]},
+            "call": {"price": 1.0838814592583166, "delta": -0.009437716341942348,
+                     "gamma": -0.006614552070516311},
+            "put": {"price": 1.5760435147766145, "delta": -0.031725540191196444,
+                    "gamma": -0.008683048910995694},
+            "tolerance": 0.003,
+            "crossover_sigmas": [0.20273255202675663, 0.20273255608140772, 0.3040988310811233],
+            "source": "crates/kiyosi/tests/double_barrier_monte_carlo.rs:121-157, 232-276",
         },
     }, indent=1))
     print("fixtures written to", OUT)
