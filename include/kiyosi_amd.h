@@ -546,6 +546,10 @@ kg_status kg_sobol_points(kg_context* ctx, uint32_t dims, uint64_t seed, uint64_
                           uint64_t count, uint64_t* out);
 kg_status kg_pcg_normals(kg_context* ctx, uint64_t seed, uint64_t stream, uint64_t first,
                          uint64_t count, double* out);
+/* The single-precision inverse normal of the FP32-step kernels applied to
+ * (bits[i] + 1/2) / 2^32; no reference counterpart (the reference maps Sobol
+ * bits through Cephes ndtri in FP64, engines/monte_carlo/sobol.rs:249-346). */
+kg_status kg_fp32_normals_from_bits(kg_context* ctx, const uint32_t* bits, uint64_t count, float* out);
 
 /* ---- MLP regressor diagnostics (no reference counterpart: they expose the
  * kernels of KG_BASIS_MLP so their gradients and Adam steps can be checked

@@ -26,6 +26,7 @@ SIGNATURES = {
     "kg_last_error": (C.c_char_p, [C.c_void_p]),
     "kg_pcg32_outputs": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_uint32)]),
     "kg_pcg_normals": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_double)]),
+    "kg_fp32_normals_from_bits": (C.c_int, [C.c_void_p, P(C.c_uint32), C.c_uint64, P(C.c_float)]),
     "kg_sobol_points": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_uint64)]),
     "kg_vanilla_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.EuropeanTerms), P(_abi.BsmMarket),
                                    P(_abi.Estimate)]),

@@ -64,8 +64,7 @@ struct AcPlanDev {
   const double* fill;    // bridge fill diffusions per step
   // FP32 copies of the per-step tables
   const float* steps32;     // drift, diffusion pairs
-  const float* weight32;
-  const float* fill32;
+  const float* wf32;        // bridge fill weight, diffusion pairs (one scalar load per step)
   const float* events32;    // knock_out, coupon_barrier pairs
   const float* intervals32; // drift, diffusion pairs
 };
@@ -87,8 +86,8 @@ template <>
 struct Tab<float> {
   static __device__ inline float drift(const AcPlanDev& p, int i) { return p.steps32[2 * i]; }
   static __device__ inline float diff(const AcPlanDev& p, int i) { return p.steps32[2 * i + 1]; }
-  static __device__ inline float w(const AcPlanDev& p, int i) { return p.weight32[i]; }
-  static __device__ inline float f(const AcPlanDev& p, int i) { return p.fill32[i]; }
+  static __device__ inline float w(const AcPlanDev& p, int i) { return p.wf32[2 * i]; }
+  static __device__ inline float f(const AcPlanDev& p, int i) { return p.wf32[2 * i + 1]; }
   static __device__ inline float ko(const AcPlanDev& p, int e) { return p.events32[2 * e]; }
   static __device__ inline float cb(const AcPlanDev& p, int e) { return p.events32[2 * e + 1]; }
   static __device__ inline float idrift(const AcPlanDev& p, int k) { return p.intervals32[2 * k]; }
@@ -211,25 +210,53 @@ __device__ double bridge_path(const AcPlanDev& p, Draws<SOBOL, FP32>& dr, bool l
       filling = monitoring;
       if (!filling) dr.skip(end - start);
     }
-    R minimum = (R)INFINITY;
-    for (int i = start; i < end; ++i) {
-      if (__ballot(filling) == 0) break;
-      if (filling && minimum < level) {
-        dr.skip(end - i);
-        filling = false;
+    // below: the path went under the knock-in level in this interval (the
+    // reference's `minimum < level`).
+    bool below = false;
+    if constexpr (SOBOL) {
+      // Sobol draws are stateless, so every lane steps the whole interval and
+      // nothing in the loop branches: a lane that is not filling resets lr to
+      // the target below and never reads `below`, and a lane that went under
+      // the level (where the reference stops filling) keeps below set. The
+      // lanes still filling compute exactly what the reference computes.
+      if (wave_any(filling)) {
+        const float* wf = FP32 ? p.wf32 + 2 * start : nullptr;
+        for (int i = start; i < end; ++i) {
+          const R z = (R)sign * dr.template normal<R>(i + 1);
+          if constexpr (FP32) {
+            // lr + w (target - lr) + f z, in three roundings
+            const float w = wf[0], f = wf[1];
+            wf += 2;
+            lr = fmaf(w, target - lr, fmaf(f, z, lr));
+          } else {
+            const double w = Tab<double>::w(p, i), f = Tab<double>::f(p, i);
+            lr += fma(w, target - lr, f * z);
+          }
+          below = below || lr < level;
+        }
       }
-      if (filling) {
-        R z = (R)sign * dr.template normal<R>(i + 1);
-        R w = Tab<R>::w(p, i), f = Tab<R>::f(p, i);
-        lr += FP32 ? fmaf((float)w, (float)(target - lr), (float)(f * z))
-                   : fma((double)w, (double)(target - lr), (double)(f * z));
-        minimum = fmin(minimum, lr);
+    } else {
+      R minimum = (R)INFINITY;
+      for (int i = start; i < end; ++i) {
+        if (__ballot(filling) == 0) break;
+        if (filling && minimum < level) {
+          dr.skip(end - i);
+          filling = false;
+        }
+        if (filling) {
+          R z = (R)sign * dr.template normal<R>(i + 1);
+          R w = Tab<R>::w(p, i), f = Tab<R>::f(p, i);
+          lr += FP32 ? fmaf((float)w, (float)(target - lr), (float)(f * z))
+                     : fma((double)w, (double)(target - lr), (double)(f * z));
+          minimum = fmin(minimum, lr);
+        }
       }
+      below = minimum < level;
     }
     lr = target;
     start = end + 1;
     if (active) {
-      if (monitoring && (minimum < level || (p.step_observe[end] && lr < level))) {
+      if (monitoring && (below || (p.step_observe[end] && lr < level))) {
         monitoring = false;
         ki = true;
       }
@@ -534,7 +561,7 @@ struct AcHostPlan {
   std::vector<AcEventD> events;
   std::vector<AcIntervalD> intervals;
   std::vector<double> weight, fill;
-  std::vector<float> steps32, weight32, fill32, events32, intervals32;
+  std::vector<float> steps32, weight32, fill32, wf32, events32, intervals32;
 };
 
 struct NoteView {
@@ -792,6 +819,10 @@ inline AcHostPlan build_plan(const kg_autocall_terms& t, const kg_bsm_market& m)
   }
   for (double v : h.weight) h.weight32.push_back((float)v);
   for (double v : h.fill) h.fill32.push_back((float)v);
+  for (size_t i = 0; i < h.weight32.size(); ++i) {
+    h.wf32.push_back(h.weight32[i]);
+    h.wf32.push_back(h.fill32[i]);
+  }
   for (const AcEventD& e : h.events) {
     h.events32.push_back((float)e.knock_out);
     h.events32.push_back((float)e.coupon_barrier);
@@ -812,7 +843,7 @@ inline void stage(std::vector<unsigned char>& blob, const std::vector<T>& v, siz
 }
 
 struct AcOffsets {
-  size_t o[12];
+  size_t o[11];
 };
 
 inline AcOffsets stage_plan(std::vector<unsigned char>& blob, const AcHostPlan& h) {
@@ -826,10 +857,9 @@ inline AcOffsets stage_plan(std::vector<unsigned char>& blob, const AcHostPlan& 
   stage(blob, h.weight, &o[5]);
   stage(blob, h.fill, &o[6]);
   stage(blob, h.steps32, &o[7]);
-  stage(blob, h.weight32, &o[8]);
-  stage(blob, h.fill32, &o[9]);
-  stage(blob, h.events32, &o[10]);
-  stage(blob, h.intervals32, &o[11]);
+  stage(blob, h.wf32, &o[8]);
+  stage(blob, h.events32, &o[9]);
+  stage(blob, h.intervals32, &o[10]);
   return r;
 }
 
@@ -845,10 +875,9 @@ inline AcPlanDev bind_plan(const AcHostPlan& h, unsigned char* d, const AcOffset
   p.weight = (const double*)(d + o[5]);
   p.fill = (const double*)(d + o[6]);
   p.steps32 = (const float*)(d + o[7]);
-  p.weight32 = (const float*)(d + o[8]);
-  p.fill32 = (const float*)(d + o[9]);
-  p.events32 = (const float*)(d + o[10]);
-  p.intervals32 = (const float*)(d + o[11]);
+  p.wf32 = (const float*)(d + o[8]);
+  p.events32 = (const float*)(d + o[9]);
+  p.intervals32 = (const float*)(d + o[10]);
   return p;
 }
 

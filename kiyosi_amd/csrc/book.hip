@@ -752,8 +752,7 @@ static Planned plan_autocall(const kg_autocall_terms& t, const kg_bsm_market& m,
   pl.stage(blob.add(h.weight), p.weight);
   pl.stage(blob.add(h.fill), p.fill);
   pl.stage(blob.add(h.steps32), p.steps32);
-  pl.stage(blob.add(h.weight32), p.weight32);
-  pl.stage(blob.add(h.fill32), p.fill32);
+  pl.stage(blob.add(h.wf32), p.wf32);
   pl.stage(blob.add(h.events32), p.events32);
   pl.stage(blob.add(h.intervals32), p.intervals32);
   return pl;
@@ -842,7 +841,7 @@ static void check_tables(const BookItem& it) {
     case IK_AUTOCALL: {
       const AcPlanDev& p = it.u.ac;
       ok = ok && p.steps && p.step_event && p.step_observe && p.events && p.intervals && p.weight &&
-           p.fill && p.steps32 && p.weight32 && p.fill32 && p.events32 && p.intervals32;
+           p.fill && p.steps32 && p.wf32 && p.events32 && p.intervals32;
       break;
     }
     default:

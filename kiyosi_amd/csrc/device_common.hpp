@@ -18,6 +18,11 @@ constexpr int WAVE = 64;
 // layout everywhere keeps the merge tree, and so the result bits, the same.
 constexpr int MERGE_THREADS = 1024;
 
+// True when `p` holds in some active lane of the wave. The builtin takes the
+// predicate as a lane mask, so no VGPR copy of it is made (HIP's __ballot
+// takes an int and materialises one).
+__device__ inline bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
 // ------------------------------------------------------------------ PCG32
 struct Pcg32 {
   uint64_t state, inc;

@@ -51,15 +51,21 @@ def main():
         np.linspace(0, 2**32 - 1, 2_000_001).astype(np.uint64),
         (2**32 - 1 - np.arange(0, 1 << 16, dtype=np.uint64)),
     ]))
-    p32 = ((bits.astype(np.float64) + 0.5) * 2.0 ** -32).astype(np.float32)
-    q32 = ((2.0**32 - 1 - bits.astype(np.float64) + 0.5) * 2.0 ** -32).astype(np.float32)
-    x = (p32 - q32).astype(np.float32)
-    w = (-np.log(np.float32(4) * p32 * q32)).astype(np.float32)
+    # As the kernel evaluates it: ps = min(p, 1 - p), l = log2(4 ps (1 - ps)),
+    # central variable -ln2 l - 2.5 and tail variable sqrt(ln2) sqrt(-l) - 3.
+    k = np.minimum(bits, 2**32 - 1 - bits).astype(np.float64)
+    ps = ((k + 0.5) * 2.0 ** -32).astype(np.float32)
+    ax = (1.0 - 2.0 * ps.astype(np.float64)).astype(np.float32)
+    four = (4.0 - 4.0 * ps.astype(np.float64)).astype(np.float32)
+    l = np.log2((ps * four).astype(np.float32)).astype(np.float32)
+    u = (-np.log(2.0) * l.astype(np.float64) - 2.5).astype(np.float32)
+    t = (np.sqrt(np.log(2.0)) * np.sqrt(-l.astype(np.float32)).astype(np.float64) - 3.0).astype(np.float32)
     c32 = central.astype(np.float32)
     t32 = tail.astype(np.float32)
-    pc = np.polyval(c32, (w - np.float32(2.5)).astype(np.float32)).astype(np.float32)
-    pt = np.polyval(t32, (np.sqrt(w) - np.float32(3)).astype(np.float32)).astype(np.float32)
-    z = np.where(w < W_SPLIT, pc, pt).astype(np.float32) * x
+    pc = np.polyval(c32, u).astype(np.float32)
+    pt = np.polyval(t32, t).astype(np.float32)
+    sign = np.where(bits >= 2**31, np.float32(1), np.float32(-1))
+    z = np.where(l <= np.float32(-W_SPLIT / np.log(2.0)), pt, pc).astype(np.float32) * ax * sign
     exact = ndtri((bits.astype(np.float64) + 0.5) * 2.0 ** -32)
     err = np.abs(z.astype(np.float64) - exact)
     rel = err / np.maximum(np.abs(exact), 1e-3)

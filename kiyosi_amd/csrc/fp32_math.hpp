@@ -44,7 +44,11 @@ __device__ inline float expm1(float x) {
 }
 
 // Inverse normal CDF of the midpoint of a 32-bit uniform:
-// p = (bits + 1/2) / 2^32.
+// p = (bits + 1/2) / 2^32. With l = log2(4 p (1 - p)) <= 0, the central
+// polynomial runs in u = -ln2 l - 5/2 and the tail one (w = -ln2 l >= 5) in
+// u = sqrt(ln2) sqrt(-l) - 3, each a single fma from l. The tail is a
+// wave-uniform branch taken only when some lane of the wave needs it (about
+// one draw in 290), so the common path carries no exec-mask bookkeeping.
 __device__ inline float normal_from_bits(uint32_t bits) {
   constexpr float CENTRAL[9] = {3.293115824e-08f,  4.844033015e-07f, -4.897925464e-06f,
                                 -6.199391464e-06f, 3.087891673e-04f, -1.773068332e-03f,
@@ -52,22 +56,24 @@ __device__ inline float normal_from_bits(uint32_t bits) {
   constexpr float TAIL[9] = {1.557544238e-06f,  -2.612999815e-04f, 1.683062175e-03f,
                              -4.751312081e-03f, 8.147120476e-03f,  -1.092399564e-02f,
                              1.335252635e-02f,  1.416593313e+00f,  4.006433964e+00f};
+  constexpr float SQRT_LN2 = 0.832554611157697756f;
+  constexpr float TAIL_L = -7.21347520444482f;  // log2 value where w = 5
   const bool upper = (bits >> 31) != 0;
   const uint32_t k = upper ? ~bits : bits;            // distance to the nearer end
   const float ps = fmaf((float)k, 0x1p-32f, 0x1p-33f);  // min(p, 1 - p) in (0, 1/2]
   const float ax = fmaf(-2.0f, ps, 1.0f);              // |2p - 1|
-  const float w = -log(ps * fmaf(-4.0f, ps, 4.0f));    // -ln(4 p (1 - p))
-  float r;
-  if (w < 5.0f) {
-    const float u = w - 2.5f;
-    r = CENTRAL[0];
+  const float l = __builtin_amdgcn_logf(ps * fmaf(-4.0f, ps, 4.0f));
+  const float u = fmaf(-LN2, l, -2.5f);
+  float r = CENTRAL[0];
 #pragma unroll
-    for (int i = 1; i < 9; ++i) r = fmaf(r, u, CENTRAL[i]);
-  } else {
-    const float u = sqrt(w) - 3.0f;
-    r = TAIL[0];
+  for (int i = 1; i < 9; ++i) r = fmaf(r, u, CENTRAL[i]);
+  const bool tail = l <= TAIL_L;
+  if (wave_any(tail)) {
+    const float t = fmaf(SQRT_LN2, sqrt(-l), -3.0f);
+    float q = TAIL[0];
 #pragma unroll
-    for (int i = 1; i < 9; ++i) r = fmaf(r, u, TAIL[i]);
+    for (int i = 1; i < 9; ++i) q = fmaf(q, t, TAIL[i]);
+    r = tail ? q : r;
   }
   return upper ? ax * r : -ax * r;
 }

@@ -1,7 +1,9 @@
 // Generator diagnostics: PCG32 outputs, block-stream normals and scrambled
 // Sobol points computed on the GPU with per-thread skip-ahead, for bit-level
 // comparison with the reference streams (monte_carlo.rs:2224-2277,
-// 1678-1703; engines/monte_carlo/sobol.rs:136-243).
+// 1678-1703; engines/monte_carlo/sobol.rs:136-243), and the single-precision
+// inverse normal of the FP32-step kernels on given 32-bit inputs.
+#include "fp32_math.hpp"
 #include "host_common.hpp"
 
 namespace kg {
@@ -25,6 +27,13 @@ __global__ void pcg_normals_kernel(uint64_t seed, uint64_t stream, uint64_t firs
   NormalStream s;
   s.start(seed, stream, first + i);
   out[i] = s.next();
+}
+
+// The FP32-step kernels' inverse normal of (bits + 1/2) / 2^32.
+__global__ void fp32_normals_kernel(const uint32_t* __restrict__ bits, uint64_t count, float* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  out[i] = f32::normal_from_bits(bits[i]);
 }
 
 __global__ void sobol_points_kernel(const uint64_t* __restrict__ dirs,
@@ -65,6 +74,22 @@ kg_status kg_pcg32_outputs(kg_context* ctx, uint64_t seed, uint64_t stream, uint
                        seed, stream, first, count, d);
     KG_HIP(hipGetLastError());
     KG_HIP(hipMemcpyAsync(out, d, count * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    KG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+kg_status kg_fp32_normals_from_bits(kg_context* ctx, const uint32_t* bits, uint64_t count, float* out) {
+  return guarded(ctx, [&] {
+    check_count(count);
+    if (count == 0) return;
+    if (!bits || !out) fail(KG_INVALID_INPUT, "bits and out must not be null");
+    uint32_t* d_in = (uint32_t*)ctx->scratch.reserve(count * (sizeof(uint32_t) + sizeof(float)));
+    float* d_out = (float*)(d_in + count);
+    KG_HIP(hipMemcpyAsync(d_in, bits, count * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(fp32_normals_kernel, dim3((count + 255) / 256), dim3(256), 0, ctx->stream, d_in, count,
+                       d_out);
+    KG_HIP(hipGetLastError());
+    KG_HIP(hipMemcpyAsync(out, d_out, count * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     KG_HIP(hipStreamSynchronize(ctx->stream));
   });
 }

@@ -59,6 +59,31 @@ def test_sobol_points_are_bit_exact(gpu, oracle):
         np.testing.assert_array_equal(out, oracle.sobol_points(dims, seed, first, count))
 
 
+def test_fp32_inverse_normal_meets_its_stated_bound(gpu):
+    """The FP32-step kernels' inverse normal of (bits + 1/2) / 2^32 stays within
+    3e-6 of SciPy's double-precision ndtri over every region: both ends, the
+    centre, the central/tail switch near w = 5 and a dense random sample. The
+    fitted form's bound is 2.3e-6 (kiyosi_amd/csrc/fit_inverse_normal.py);
+    3e-6 leaves room for the device log2."""
+    from scipy.special import ndtri
+    rng = np.random.default_rng(5)
+    edge = np.arange(0, 1 << 14, dtype=np.uint64)
+    switch = np.uint64(int(2**32 * 0.00169))  # 4 p (1 - p) = e^-5 near here
+    bits = np.unique(np.concatenate([
+        edge, (1 << 32) - 1 - edge, (1 << 31) + np.arange(-4096, 4096, dtype=np.int64).astype(np.uint64),
+        switch + np.arange(-50000, 50000, dtype=np.int64).astype(np.uint64),
+        (1 << 32) - 1 - switch - np.arange(-50000, 50000, dtype=np.int64).astype(np.uint64),
+        rng.integers(0, 1 << 32, size=1 << 20, dtype=np.uint64)])).astype(np.uint32)
+    out = np.zeros(bits.size, dtype=np.float32)
+    gpu.check(_native.lib().kg_fp32_normals_from_bits(gpu.handle, bits.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                                       bits.size, out.ctypes.data_as(C.POINTER(C.c_float))))
+    exact = ndtri((bits.astype(np.float64) + 0.5) * 2.0**-32)
+    err = np.abs(out.astype(np.float64) - exact)
+    assert err.max() < 3e-6, (bits[err.argmax()], out[err.argmax()], exact[err.argmax()])
+    # Odd symmetry of the midpoint rule: bits and ~bits give opposite normals.
+    assert np.array_equal(out[:edge.size], -out[-edge.size:][::-1])
+
+
 @pytest.mark.parametrize("paths,antithetic,sampling", [
     (1, False, _abi.PSEUDO_RANDOM), (255, False, _abi.PSEUDO_RANDOM), (257, True, _abi.PSEUDO_RANDOM),
     (4097, True, _abi.PSEUDO_RANDOM), (65536, False, _abi.PSEUDO_RANDOM), (1, False, _abi.SOBOL),
