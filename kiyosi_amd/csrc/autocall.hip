@@ -24,12 +24,13 @@ namespace kg {
 // The draw source of sample j of `block`: the block's Sobol LDS tables (or
 // direct points for long paths), or the block's PCG stream advanced to
 // normal j * dims.
-template <bool SOBOL, bool FP32>
-__device__ inline Draws<SOBOL, FP32> setup_draws(int dims, uint64_t seed, int64_t block, int j, int64_t n,
-                                                 const uint64_t* __restrict__ dirs,
-                                                 const uint64_t* __restrict__ shifts, int sobol_lds,
-                                                 unsigned char* lds) {
-  Draws<SOBOL, FP32> dr;
+template <bool SOBOL, bool FP32, bool LDS>
+__device__ inline Draws<SOBOL, FP32, LDS ? 0 : 1> setup_draws(int dims, uint64_t seed, int64_t block, int j,
+                                                              int64_t n, const uint64_t* __restrict__ dirs,
+                                                              const uint64_t* __restrict__ shifts,
+                                                              unsigned char* lds) {
+  Draws<SOBOL, FP32, LDS ? 0 : 1> dr;
+  const bool sobol_lds = LDS;
   dr.dims = dims;
   dr.direct = !sobol_lds;
   dr.dirs = dirs;
@@ -63,21 +64,23 @@ __device__ inline Draws<SOBOL, FP32> setup_draws(int dims, uint64_t seed, int64_
   return dr;
 }
 
-template <bool SOBOL, bool FP32>
+// LDS: Sobol points come from the block's LDS tables (paths short enough for
+// them); otherwise from global directions.
+template <bool SOBOL, bool FP32, bool LDS>
 __global__ void __launch_bounds__(256)
     autocall_kernel(AcPlanDev p, uint64_t seed, int64_t samples, int64_t paths, int antithetic,
                     const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shifts,
-                    int sobol_lds, Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
+                    Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ Moments smem[2 * 4];
   const int64_t block = blockIdx.x;
   const int j = threadIdx.x;
   const int64_t n = block * BLOCK_SAMPLES + j;
   const bool live = n < samples;
-  Draws<SOBOL, FP32> dr = setup_draws<SOBOL, FP32>(p.n_steps, seed, block, j, n, dirs, shifts, sobol_lds, lds);
+  auto dr = setup_draws<SOBOL, FP32, LDS>(p.n_steps, seed, block, j, n, dirs, shifts, lds);
   uint32_t bad = 0;
   const bool paired = antithetic && live && n < paths / 2;
-  const Draws<SOBOL, FP32> origin = dr;
+  const auto origin = dr;
   double pos = p.bridge ? bridge_path<SOBOL, FP32>(p, dr, live, 1.0f, bad)
                         : step_path<SOBOL, FP32>(p, dr, live, 1.0f, bad);
   double neg = 0.0;
@@ -105,10 +108,10 @@ __global__ void __launch_bounds__(256)
 // Scenario lanes on shared draws: lane l's total and pair moments of the
 // block go to partials[block * slots + slot_base + 2l] and [... + 1],
 // formed exactly as the single-plan kernel forms its own.
-template <int L, bool SOBOL, bool FP32>
+template <int L, bool SOBOL, bool FP32, bool LDS>
 __global__ void __launch_bounds__(256)
     autocall_lanes_kernel(const AcLanes* __restrict__ lanes, uint64_t seed, int64_t samples, int64_t paths, int antithetic,
-                          const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shifts, int sobol_lds,
+                          const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shifts,
                           Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ Moments smem[2 * 4];
@@ -117,10 +120,10 @@ __global__ void __launch_bounds__(256)
   const int64_t n = block * BLOCK_SAMPLES + j;
   const bool live = n < samples;
   const AcLanes& P = *lanes;
-  Draws<SOBOL, FP32> dr = setup_draws<SOBOL, FP32>(P.n_steps, seed, block, j, n, dirs, shifts, sobol_lds, lds);
+  auto dr = setup_draws<SOBOL, FP32, LDS>(P.n_steps, seed, block, j, n, dirs, shifts, lds);
   uint32_t bad = 0;
   const bool paired = antithetic && live && n < paths / 2;
-  const Draws<SOBOL, FP32> origin = dr;
+  const auto origin = dr;
   double pos[L], neg[L];
   if (P.bridge)
     bridge_lanes<L, SOBOL, FP32>(P, dr, live, 1.0f, bad, pos);
@@ -159,23 +162,24 @@ template <bool SOBOL, bool FP32, int L, class Plan>
 static void launch(kg_context* ctx, const Plan& p, int dims, const Sampler& s, const uint64_t* dirs,
                    const uint64_t* shifts, Moments* parts, uint32_t* flags) {
   size_t lds = 0;
-  int sobol_lds = 0;
+  bool sobol_lds = false;
   if (SOBOL) {
     lds = sobol_lds_bytes(dims, FP32);
     sobol_lds = lds <= SOBOL_LDS_LIMIT;
     if (!sobol_lds) lds = 0;
   }
-  void (*kernel)(Plan, uint64_t, int64_t, int64_t, int, const uint64_t*, const uint64_t*, int, Moments*,
-                 uint32_t*);
+  using Kernel = void (*)(Plan, uint64_t, int64_t, int64_t, int, const uint64_t*, const uint64_t*, Moments*,
+                          uint32_t*);
+  Kernel kernel;
   if constexpr (std::is_same<Plan, const AcLanes*>::value)
-    kernel = autocall_lanes_kernel<L, SOBOL, FP32>;
+    kernel = sobol_lds ? autocall_lanes_kernel<L, SOBOL, FP32, true> : autocall_lanes_kernel<L, SOBOL, FP32, false>;
   else
-    kernel = autocall_kernel<SOBOL, FP32>;
+    kernel = sobol_lds ? autocall_kernel<SOBOL, FP32, true> : autocall_kernel<SOBOL, FP32, false>;
   if (lds > 64 * 1024)
     KG_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
   hipLaunchKernelGGL(kernel, dim3(s.blocks()), dim3(256), lds, ctx->stream, p, s.seed, s.samples(),
-                     s.path_count, (int)s.antithetic, dirs, shifts, sobol_lds, parts, flags);
+                     s.path_count, (int)s.antithetic, dirs, shifts, parts, flags);
   KG_HIP(hipGetLastError());
 }
 

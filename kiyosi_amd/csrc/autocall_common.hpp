@@ -120,8 +120,10 @@ __device__ inline double terminal_value(const Plan& p, double lr, bool ki, uint3
   return discounted_product(1.0, amount, p.final_log_discount);
 }
 
-// Source of the normals of one sample.
-template <bool SOBOL, bool FP32>
+// Source of the normals of one sample. DIRECT fixes at compile time whether
+// Sobol points come from global directions (1) or the block's LDS tables (0);
+// -1 leaves the choice to the `direct` field.
+template <bool SOBOL, bool FP32, int DIRECT = -1>
 struct Draws {
   // Sobol: LDS tables of this block and the lane's Gray-code digits
   const void* lo;
@@ -141,19 +143,20 @@ struct Draws {
 
   __device__ inline double normal64(int d) {
     if (SOBOL) {
-      uint64_t bits = direct ? sobol_point_direct(dirs, shifts, dims, point, d)
-                             : sobol_lds_bits64((const uint64_t*)lo, (const uint64_t*)hi, d, glo, ghi);
+      uint64_t bits = is_direct() ? sobol_point_direct(dirs, shifts, dims, point, d)
+                                  : sobol_lds_bits64((const uint64_t*)lo, (const uint64_t*)hi, d, glo, ghi);
       return normal_from_bits(bits);
     }
     return s.next();
   }
+  __device__ inline bool is_direct() const { return DIRECT < 0 ? direct : DIRECT != 0; }
+  // Top 32 of the 52 bits of Sobol dimension d of this sample's point.
+  __device__ inline uint32_t bits32(int d) {
+    return is_direct() ? (uint32_t)(sobol_point_direct(dirs, shifts, dims, point, d) >> 20)
+                       : sobol_lds_bits32((const uint32_t*)lo, (const uint32_t*)hi, d, glo, ghi);
+  }
   __device__ inline float normal32(int d) {
-    if (SOBOL) {
-      uint32_t bits =
-          direct ? (uint32_t)(sobol_point_direct(dirs, shifts, dims, point, d) >> 20)
-                 : sobol_lds_bits32((const uint32_t*)lo, (const uint32_t*)hi, d, glo, ghi);
-      return f32::normal_from_bits(bits);
-    }
+    if (SOBOL) return f32::normal_from_bits(bits32(d));
     if (has32) {
       has32 = false;
       return spare32;
@@ -187,8 +190,8 @@ struct Draws {
 
 // Bridge path (bridge_payoffs::<1, true>, autocallable.rs:628-736). `sign`
 // is -1 for the antithetic leg. Returns the discounted cash of the path.
-template <bool SOBOL, bool FP32>
-__device__ double bridge_path(const AcPlanDev& p, Draws<SOBOL, FP32>& dr, bool live, float sign,
+template <bool SOBOL, bool FP32, class D>
+__device__ double bridge_path(const AcPlanDev& p, D& dr, bool live, float sign,
                               uint32_t& bad) {
   using R = typename std::conditional<FP32, float, double>::type;
   double cash = p.coupon;
@@ -220,15 +223,31 @@ __device__ double bridge_path(const AcPlanDev& p, Draws<SOBOL, FP32>& dr, bool l
       // the level (where the reference stops filling) keeps below set. The
       // lanes still filling compute exactly what the reference computes.
       if (wave_any(filling)) {
+        // FP32: the next step's point bits (LDS) and fill weights (scalar
+        // load) are read one step ahead, so both overlap the current step's
+        // arithmetic. The last reads repeat dimension and step `end`, which
+        // exist: `end` is a step of the path.
         const float* wf = FP32 ? p.wf32 + 2 * start : nullptr;
+        uint32_t next = 0;
+        float next_w = 0.0f, next_f = 0.0f;
+        if constexpr (FP32) {
+          next = dr.bits32(start + 1);
+          next_w = wf[0];
+          next_f = wf[1];
+        }
         for (int i = start; i < end; ++i) {
-          const R z = (R)sign * dr.template normal<R>(i + 1);
           if constexpr (FP32) {
-            // lr + w (target - lr) + f z, in three roundings
-            const float w = wf[0], f = wf[1];
+            const uint32_t bits = next;
+            const float w = next_w, f = next_f;
+            next = dr.bits32(i + 2 < end ? i + 2 : end);
             wf += 2;
+            next_w = wf[0];
+            next_f = wf[1];
+            const float z = sign * f32::normal_from_bits(bits);
+            // lr + w (target - lr) + f z, in three roundings
             lr = fmaf(w, target - lr, fmaf(f, z, lr));
           } else {
+            const R z = (R)sign * dr.template normal<R>(i + 1);
             const double w = Tab<double>::w(p, i), f = Tab<double>::f(p, i);
             lr += fma(w, target - lr, f * z);
           }
@@ -281,8 +300,8 @@ __device__ double bridge_path(const AcPlanDev& p, Draws<SOBOL, FP32>& dr, bool l
 }
 
 // Daily or event-only path (payoff, autocallable.rs:823-862).
-template <bool SOBOL, bool FP32>
-__device__ double step_path(const AcPlanDev& p, Draws<SOBOL, FP32>& dr, bool live, float sign,
+template <bool SOBOL, bool FP32, class D>
+__device__ double step_path(const AcPlanDev& p, D& dr, bool live, float sign,
                             uint32_t& bad) {
   using R = typename std::conditional<FP32, float, double>::type;
   double cash = p.coupon;
@@ -392,8 +411,8 @@ struct LaneTab<float> {
 // they occupy one register each rather than one lane mask per lane.
 __device__ inline bool bit(uint32_t mask, int l) { return (mask >> l) & 1u; }
 
-template <int L, bool SOBOL, bool FP32>
-__device__ void bridge_lanes(const AcLanes& P, Draws<SOBOL, FP32>& dr, bool live, float sign, uint32_t& bad,
+template <int L, bool SOBOL, bool FP32, class D>
+__device__ void bridge_lanes(const AcLanes& P, D& dr, bool live, float sign, uint32_t& bad,
                              double (&cash)[L]) {
   using R = typename std::conditional<FP32, float, double>::type;
   using T = LaneTab<R>;
@@ -480,8 +499,8 @@ __device__ void bridge_lanes(const AcLanes& P, Draws<SOBOL, FP32>& dr, bool live
   }
 }
 
-template <int L, bool SOBOL, bool FP32>
-__device__ void step_lanes(const AcLanes& P, Draws<SOBOL, FP32>& dr, bool live, float sign, uint32_t& bad,
+template <int L, bool SOBOL, bool FP32, class D>
+__device__ void step_lanes(const AcLanes& P, D& dr, bool live, float sign, uint32_t& bad,
                            double (&cash)[L]) {
   using R = typename std::conditional<FP32, float, double>::type;
   using T = LaneTab<R>;

@@ -51,7 +51,7 @@ __device__ __attribute__((always_inline)) void sample_body(const BookItem& it, i
   const int64_t n = block * BLOCK_SAMPLES + j;
   const bool live = n < it.samples;
   const bool paired = it.antithetic && live && n < it.paths / 2;
-  Draws<SOBOL, FP32> dr;
+  Draws<SOBOL, FP32, 1> dr;
   dr.dims = it.dims;
   dr.direct = true;
   dr.dirs = it.dirs;
@@ -72,7 +72,7 @@ __device__ __attribute__((always_inline)) void sample_body(const BookItem& it, i
       dr.s.start(it.seed, (uint64_t)block, first);
     }
   }
-  const Draws<SOBOL, FP32> origin = dr;
+  const auto origin = dr;
   auto run = [&](bool on, float sign) -> double {
     switch (it.kind) {
       case IK_BARRIER:
@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(256)
   const int64_t n = block * BLOCK_SAMPLES + j;
   const bool live = n < it.samples;
   const bool paired = it.antithetic && live && n < it.paths / 2;
-  Draws<SOBOL, FP32> dr;
+  Draws<SOBOL, FP32, 1> dr;
   dr.dims = it.dims;
   dr.direct = true;
   dr.dirs = it.dirs;
@@ -243,7 +243,7 @@ __global__ void __launch_bounds__(256)
       dr.s.start(it.seed, (uint64_t)block, first);
     }
   }
-  const Draws<SOBOL, FP32> origin = dr;
+  const auto origin = dr;
   uint32_t bad = 0;
   auto run = [&](bool on, float sign, double* out) {
     out[0] = it.kind == IK_DIGITAL ? digital_path<FP32, true>(it.u.dg, dr, on, sign, bad, out + 1)
