@@ -222,6 +222,20 @@ kg_status kg_autocallable_scenarios(kg_context* ctx, const kg_settings* settings
                                     const kg_autocall_terms* note, const kg_bsm_market* markets,
                                     int32_t n, kg_estimate* out);
 
+/* ---- Autocallable portfolio: MonteCarloAutocallableEngine::price_many and
+ * price_many_on_shared_grid (engines/monte_carlo/autocallable/portfolio.rs
+ * :187-318; Python binding crates/kiyosi-python/src/engines/monte_carlo.rs
+ * :699-746). Every note is priced with the one resolved seed of `settings`,
+ * and out[i] equals kg_autocallable_price for notes[i] bit for bit, as the
+ * reference's batch equals its scalar prices
+ * (crates/kiyosi/tests/autocallable_monte_carlo.rs:52-99). Notes whose
+ * plans share the step grid (steps, events, knock-in observations and
+ * bridge) run as lanes on one path set; the others run alone. At most
+ * 1,048,576 notes (binding_support::validate_contract_batch_size). */
+kg_status kg_autocallable_price_many(kg_context* ctx, const kg_settings* settings,
+                                     const kg_autocall_terms* notes, size_t n,
+                                     const kg_bsm_market* market, kg_estimate* out);
+
 /* ---- Heston path stepping. kiyosi has Heston only in its Fourier and FD
  * engines (market/heston.rs:58-227, engines/analytic/heston.rs,
  * engines/finite_difference/heston.rs); every reference MC engine rejects

@@ -17,6 +17,8 @@
 // has a bridge and payoff (:823-862) otherwise. Control flow is uniform per
 // wave: every lane walks the same steps with per-lane predicates, so plan
 // tables are read once per wave.
+#include <unordered_map>
+
 #include "autocall_common.hpp"
 
 namespace kg {
@@ -279,6 +281,95 @@ static void bind_lanes(AcLanes& P, unsigned char* d, const LaneOffsets& off) {
 
 using namespace kg;
 
+namespace kg {
+
+// One sampled note on its own: the single-plan kernel and the block merge.
+static void price_note(kg_context* ctx, const Sampler& s, AcHostPlan& h, kg_estimate* out) {
+  AcPlanDev p = upload(ctx, h);
+  const int64_t blocks = s.blocks();
+  Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 2 * blocks);
+  uint32_t* flags = reset_flags(ctx);
+  if (s.sobol) sobol_setup(ctx, (uint32_t)p.n_steps, s.seed);
+  timing_begin(ctx);
+  launch_any(ctx, p, p.n_steps, s, parts, flags);
+  Moments r[2];
+  *out = kg_estimate{};
+  finish_sampling(ctx, parts, blocks, 2, r, &out->device_ms);
+  out->path_steps = (double)s.path_count * (double)p.n_steps;
+  fill_estimate(s, r[0], r[1], 0.0, false, true, out);
+}
+
+// Up to AC_MAX_SCENARIOS sampled plans on one step grid (same_grid) as lanes
+// on the same draws, in launches of AC_MAX_LANES lanes; the lane sets follow
+// their tables in one upload. Lane l's estimate equals price_note of plan l
+// bit for bit.
+static void price_lane_set(kg_context* ctx, const Sampler& s, const std::vector<AcHostPlan*>& plans,
+                           const std::vector<kg_estimate*>& outs) {
+  const int m = (int)plans.size();
+  const int dims = plans[0]->dev.n_steps;
+  const int slots = 2 * m;
+  std::vector<unsigned char> blob;
+  std::vector<AcLanes> groups;
+  std::vector<LaneOffsets> offs;
+  for (int g = 0; g < m; g += AC_MAX_LANES) {
+    std::vector<const AcHostPlan*> L(plans.begin() + g, plans.begin() + std::min(m, g + AC_MAX_LANES));
+    offs.emplace_back();
+    groups.push_back(stage_lanes(L, blob, offs.back()));
+    groups.back().slot_base = 2 * g;
+    groups.back().slots = slots;
+  }
+  const size_t at = (blob.size() + 255) & ~size_t(255);
+  blob.resize(at + sizeof(AcLanes) * groups.size());
+  unsigned char* d = (unsigned char*)ctx->scratch.reserve(blob.size());
+  for (size_t g = 0; g < groups.size(); ++g) {
+    bind_lanes(groups[g], d, offs[g]);
+    std::memcpy(blob.data() + at + g * sizeof(AcLanes), &groups[g], sizeof(AcLanes));
+  }
+  upload_async(ctx, d, blob.data(), blob.size());
+  const int64_t blocks = s.blocks();
+  Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * slots * blocks);
+  uint32_t* flags = reset_flags(ctx);
+  if (s.sobol) sobol_setup(ctx, (uint32_t)dims, s.seed);
+  timing_begin(ctx);
+  for (size_t g = 0; g < groups.size(); ++g) {
+    const AcLanes* d_lanes = (const AcLanes*)(d + at + g * sizeof(AcLanes));
+    if (groups[g].n <= 2)
+      launch_any<2>(ctx, d_lanes, dims, s, parts, flags);
+    else
+      launch_any<4>(ctx, d_lanes, dims, s, parts, flags);
+  }
+  Moments r[2 * AC_MAX_SCENARIOS];
+  double ms = 0.0;
+  finish_sampling(ctx, parts, blocks, slots, r, &ms);
+  for (int l = 0; l < m; ++l) {
+    kg_estimate* o = outs[l];
+    *o = kg_estimate{};
+    o->device_ms = ms;
+    o->path_steps = (double)s.path_count * (double)dims;
+    fill_estimate(s, r[2 * l], r[2 * l + 1], 0.0, false, true, o);
+  }
+}
+
+// Hash of the step grid that same_grid compares.
+static uint64_t grid_hash(const AcHostPlan& h) {
+  uint64_t x = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) {
+    x ^= v;
+    x *= 1099511628211ull;
+  };
+  mix((uint64_t)h.dev.n_steps);
+  mix((uint64_t)h.dev.bridge);
+  for (int32_t e : h.step_event) mix((uint64_t)(uint32_t)e);
+  for (uint8_t o : h.step_observe) mix(o);
+  for (const AcIntervalD& b : h.intervals) mix((uint64_t)(uint32_t)b.end);
+  return x;
+}
+
+constexpr size_t BATCH_NOTES = size_t(1) << 20;  // binding_support::validate_contract_batch_size
+constexpr size_t PLAN_CHUNK = 1024;              // notes planned and held at once
+
+}  // namespace kg
+
 extern "C" kg_status kg_autocallable_price(kg_context* ctx, const kg_settings* settings,
                                            const kg_autocall_terms* note,
                                            const kg_bsm_market* market, kg_estimate* out) {
@@ -291,17 +382,7 @@ extern "C" kg_status kg_autocallable_price(kg_context* ctx, const kg_settings* s
       return;
     }
     Sampler s = validate_settings(settings, (uint64_t)h.dev.n_steps);
-    AcPlanDev p = upload(ctx, h);
-    const int64_t blocks = s.blocks();
-    Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 2 * blocks);
-    uint32_t* flags = reset_flags(ctx);
-    if (s.sobol) sobol_setup(ctx, (uint32_t)p.n_steps, s.seed);
-    timing_begin(ctx);
-    launch_any(ctx, p, p.n_steps, s, parts, flags);
-    Moments r[2];
-    finish_sampling(ctx, parts, blocks, 2, r, &out->device_ms);
-    out->path_steps = (double)s.path_count * (double)p.n_steps;
-    fill_estimate(s, r[0], r[1], 0.0, false, true, out);
+    price_note(ctx, s, h, out);
   });
 }
 
@@ -317,65 +398,76 @@ extern "C" kg_status kg_autocallable_scenarios(kg_context* ctx, const kg_setting
       out[i] = kg_estimate{};
     }
     // Scenarios with a known value take it; the others become lanes.
-    std::vector<int> lane_of;
-    for (int i = 0; i < n; ++i) {
-      if (h[i].fixed) continue;
-      if (!lane_of.empty() && !same_grid(h[lane_of[0]], h[i]))
-        fail(KG_UNSUPPORTED_OPERATION, "the scenarios do not share the path grid; price them one at a time");
-      lane_of.push_back(i);
-    }
+    std::vector<AcHostPlan*> lanes;
+    std::vector<kg_estimate*> outs;
     const bool sobol = settings && settings->sampling == KG_SOBOL;
-    if (lane_of.empty()) {
-      for (int i = 0; i < n; ++i) fill_known(h[i].fixed_value, sobol, &out[i]);
-      return;
+    for (int i = 0; i < n; ++i) {
+      if (h[i].fixed) {
+        fill_known(h[i].fixed_value, sobol, &out[i]);
+        continue;
+      }
+      if (!lanes.empty() && !same_grid(*lanes[0], h[i]))
+        fail(KG_UNSUPPORTED_OPERATION, "the scenarios do not share the path grid; price them one at a time");
+      lanes.push_back(&h[i]);
+      outs.push_back(&out[i]);
     }
-    const int dims = h[lane_of[0]].dev.n_steps;
-    Sampler s = validate_settings(settings, (uint64_t)dims);
-    // Launch groups of at most AC_MAX_LANES lanes; the lane sets follow
-    // their tables in one upload.
-    const int m = (int)lane_of.size();
-    const int slots = 2 * m;
-    std::vector<unsigned char> blob;
-    std::vector<AcLanes> groups;
-    std::vector<LaneOffsets> offs;
-    for (int g = 0; g < m; g += AC_MAX_LANES) {
-      std::vector<const AcHostPlan*> L;
-      for (int l = g; l < std::min(m, g + AC_MAX_LANES); ++l) L.push_back(&h[lane_of[l]]);
-      offs.emplace_back();
-      groups.push_back(stage_lanes(L, blob, offs.back()));
-      groups.back().slot_base = 2 * g;
-      groups.back().slots = slots;
-    }
-    const size_t at = (blob.size() + 255) & ~size_t(255);
-    blob.resize(at + sizeof(AcLanes) * groups.size());
-    unsigned char* d = (unsigned char*)ctx->scratch.reserve(blob.size());
-    for (size_t g = 0; g < groups.size(); ++g) {
-      bind_lanes(groups[g], d, offs[g]);
-      std::memcpy(blob.data() + at + g * sizeof(AcLanes), &groups[g], sizeof(AcLanes));
-    }
-    upload_async(ctx, d, blob.data(), blob.size());
-    const int64_t blocks = s.blocks();
-    Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * slots * blocks);
-    uint32_t* flags = reset_flags(ctx);
-    if (s.sobol) sobol_setup(ctx, (uint32_t)dims, s.seed);
-    timing_begin(ctx);
-    for (size_t g = 0; g < groups.size(); ++g) {
-      const AcLanes* d_lanes = (const AcLanes*)(d + at + g * sizeof(AcLanes));
-      if (groups[g].n <= 2)
-        launch_any<2>(ctx, d_lanes, dims, s, parts, flags);
-      else
-        launch_any<4>(ctx, d_lanes, dims, s, parts, flags);
-    }
-    Moments r[2 * AC_MAX_SCENARIOS];
-    double ms = 0.0;
-    finish_sampling(ctx, parts, blocks, slots, r, &ms);
-    for (int i = 0; i < n; ++i)
-      if (h[i].fixed) fill_known(h[i].fixed_value, sobol, &out[i]);
-    for (int l = 0; l < m; ++l) {
-      kg_estimate* o = &out[lane_of[l]];
-      o->device_ms = ms;
-      o->path_steps = (double)s.path_count * (double)dims;
-      fill_estimate(s, r[2 * l], r[2 * l + 1], 0.0, false, true, o);
+    if (lanes.empty()) return;
+    Sampler s = validate_settings(settings, (uint64_t)lanes[0]->dev.n_steps);
+    price_lane_set(ctx, s, lanes, outs);
+  });
+}
+
+extern "C" kg_status kg_autocallable_price_many(kg_context* ctx, const kg_settings* settings,
+                                                const kg_autocall_terms* notes, size_t n,
+                                                const kg_bsm_market* market, kg_estimate* out) {
+  return guarded(ctx, [&] {
+    if (n == 0) return;
+    if (!notes || !market || !out || !settings) fail(KG_INVALID_INPUT, "arguments must not be null");
+    if (n > BATCH_NOTES) fail(KG_INVALID_INPUT, "a batch holds at most 1,048,576 notes");
+    const bool sobol = settings->sampling == KG_SOBOL;
+    for (size_t c0 = 0; c0 < n; c0 += PLAN_CHUNK) {
+      const size_t c1 = std::min(n, c0 + PLAN_CHUNK);
+      std::vector<AcHostPlan> h(c1 - c0);
+      // Groups of sampled notes with one step grid, in order of first note.
+      std::vector<std::vector<size_t>> groups;
+      std::unordered_map<uint64_t, std::vector<size_t>> by_hash;
+      for (size_t i = c0; i < c1; ++i) {
+        AcHostPlan& p = h[i - c0];
+        p = build_plan(notes[i], *market);
+        out[i] = kg_estimate{};
+        if (p.fixed) {
+          fill_known(p.fixed_value, sobol, &out[i]);
+          continue;
+        }
+        std::vector<size_t>& candidates = by_hash[grid_hash(p)];
+        size_t g = groups.size();
+        for (size_t k : candidates)
+          if (same_grid(h[groups[k][0] - c0], p)) {
+            g = k;
+            break;
+          }
+        if (g == groups.size()) {
+          candidates.push_back(g);
+          groups.emplace_back();
+        }
+        groups[g].push_back(i);
+      }
+      for (const std::vector<size_t>& g : groups) {
+        const Sampler s = validate_settings(settings, (uint64_t)h[g[0] - c0].dev.n_steps);
+        if (g.size() == 1) {
+          price_note(ctx, s, h[g[0] - c0], &out[g[0]]);
+          continue;
+        }
+        for (size_t k = 0; k < g.size(); k += AC_MAX_SCENARIOS) {
+          std::vector<AcHostPlan*> lanes;
+          std::vector<kg_estimate*> outs;
+          for (size_t l = k; l < std::min(g.size(), k + AC_MAX_SCENARIOS); ++l) {
+            lanes.push_back(&h[g[l] - c0]);
+            outs.push_back(&out[g[l]]);
+          }
+          price_lane_set(ctx, s, lanes, outs);
+        }
+      }
     }
   });
 }

@@ -120,6 +120,17 @@ __device__ inline double terminal_value(const Plan& p, double lr, bool ki, uint3
   return discounted_product(1.0, amount, p.final_log_discount);
 }
 
+// One daily bridge fill step lr + w (target - lr) + f z. FP64 rounds as
+// the reference writes it (lr += w.mul_add(target - lr, f * z),
+// autocallable.rs:678); FP32 takes three roundings. Every kernel that fills
+// a bridge uses these, so lane and single-note prices agree bit for bit.
+__device__ inline double fill_step(double lr, double target, double w, double f, double z) {
+  return lr + fma(w, target - lr, f * z);
+}
+__device__ inline float fill_step(float lr, float target, float w, float f, float z) {
+  return fmaf(w, target - lr, fmaf(f, z, lr));
+}
+
 // Source of the normals of one sample. DIRECT fixes at compile time whether
 // Sobol points come from global directions (1) or the block's LDS tables (0);
 // -1 leaves the choice to the `direct` field.
@@ -244,12 +255,10 @@ __device__ double bridge_path(const AcPlanDev& p, D& dr, bool live, float sign,
             next_w = wf[0];
             next_f = wf[1];
             const float z = sign * f32::normal_from_bits(bits);
-            // lr + w (target - lr) + f z, in three roundings
-            lr = fmaf(w, target - lr, fmaf(f, z, lr));
+            lr = fill_step(lr, target, w, f, z);
           } else {
             const R z = (R)sign * dr.template normal<R>(i + 1);
-            const double w = Tab<double>::w(p, i), f = Tab<double>::f(p, i);
-            lr += fma(w, target - lr, f * z);
+            lr = fill_step(lr, target, Tab<double>::w(p, i), Tab<double>::f(p, i), z);
           }
           below = below || lr < level;
         }
@@ -263,10 +272,8 @@ __device__ double bridge_path(const AcPlanDev& p, D& dr, bool live, float sign,
           filling = false;
         }
         if (filling) {
-          R z = (R)sign * dr.template normal<R>(i + 1);
-          R w = Tab<R>::w(p, i), f = Tab<R>::f(p, i);
-          lr += FP32 ? fmaf((float)w, (float)(target - lr), (float)(f * z))
-                     : fma((double)w, (double)(target - lr), (double)(f * z));
+          const R z = (R)sign * dr.template normal<R>(i + 1);
+          lr = fill_step(lr, target, Tab<R>::w(p, i), Tab<R>::f(p, i), z);
           minimum = fmin(minimum, lr);
         }
       }
@@ -458,9 +465,7 @@ __device__ void bridge_lanes(const AcLanes& P, D& dr, bool live, float sign, uin
 #pragma unroll
         for (int l = 0; l < L; ++l)
           if (bit(filling, l)) {
-            const R w = T::w(P, i, l), f = T::f(P, i, l);
-            lr[l] += FP32 ? fmaf((float)w, (float)(target[l] - lr[l]), (float)(f * zf))
-                          : fma((double)w, (double)(target[l] - lr[l]), (double)(f * zf));
+            lr[l] = fill_step(lr[l], target[l], T::w(P, i, l), T::f(P, i, l), zf);
             minimum[l] = fmin(minimum[l], lr[l]);
           }
       }

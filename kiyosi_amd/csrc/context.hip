@@ -74,7 +74,9 @@ void upload_async(kg_context* ctx, void* dst, const void* src, size_t n) {
   if (n == 0) return;
   const size_t at = PINNED_RESULT_BYTES + ((ctx->staging_used + 255) & ~size_t(255));
   if (at + n > ctx->pinned_bytes) {
-    // Growing the buffer synchronises, so the staged bytes restart at zero.
+    // The staged bytes restart at zero, so every copy still reading them
+    // must finish first; the buffer grows if one upload does not fit.
+    KG_HIP(hipStreamSynchronize(ctx->stream));
     reserve_pinned(ctx, 2 * (PINNED_RESULT_BYTES + n));
     ctx->staging_used = 0;
     upload_async(ctx, dst, src, n);

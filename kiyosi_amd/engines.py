@@ -199,6 +199,30 @@ class MonteCarloAutocallableEngine(_MonteCarloEngine):
     def _resolved(self):
         return self if self.seed is not None else self.with_resolved_seed()
 
+    def price_many(self, instruments, context, shared_grid=False):
+        """Prices of notes of one product type on one resolved seed
+        (price_many / price_many_on_shared_grid, crates/kiyosi-python/src/
+        engines/monte_carlo.rs:699-746): each equals ``price`` of that note
+        with the same seed, bit for bit. Notes sharing a step grid run as
+        lanes of one GPU path set; `shared_grid` selects the same prices."""
+        notes = list(instruments)
+        if len(notes) > 1 << 20:
+            raise InvalidInputError("a batch holds at most 1,048,576 notes")
+        if any(not isinstance(n, _Autocallable) for n in notes):
+            raise TypeError("price_many prices autocallable notes")
+        if len({type(n) for n in notes}) > 1:
+            raise UnsupportedOperationError("Monte Carlo price_many requires one autocallable product type")
+        if not notes:
+            return ()
+        engine = self._resolved()
+        terms = [n._terms() for n in notes]
+        array = (_abi.AutocallTerms * len(notes))(*terms)
+        out = (_abi.Estimate * len(notes))()
+        ctx = _native.context(engine.device)
+        ctx.check(_native.lib().kg_autocallable_price_many(ctx.handle, C.byref(engine._settings()), array,
+                                                           len(notes), C.byref(context.bsm_market()), out))
+        return tuple(o.price for o in out)
+
     def price_with_greeks(self, note, context, request=None, settings=None):
         """Price and Greeks of an autocallable note by central differences on
         shared draws (price_with_greeks_with_settings,

@@ -1,0 +1,106 @@
+"""GPU parity of the autocallable portfolio batch (price_many,
+engines/monte_carlo/autocallable/portfolio.rs:187-318): every note of a
+batch is priced on one resolved seed and equals its scalar price bit for
+bit, as the reference's own test asserts
+(crates/kiyosi/tests/autocallable_monte_carlo.rs:52-99), for notes that
+share a step grid (lanes), notes on other grids, repeated notes, notes with
+a known value and mixed batches across several lane launches."""
+
+import datetime as dt
+
+import pytest
+
+from kiyosi_amd import workloads
+from kiyosi_amd.engines import MonteCarloAutocallableEngine
+from kiyosi_amd.errors import UnsupportedOperationError
+from kiyosi_amd.instruments import KnockInObservation, SnowballObservation, SnowballOption
+from kiyosi_amd.market import BlackScholesMertonParameters, PricingContext, TradingCalendar
+
+from test_gpu_autocall import phoenix
+
+pytestmark = pytest.mark.gpu
+V = workloads.VALUATION
+
+
+def context(spot=100.0):
+    return PricingContext(model_parameters=BlackScholesMertonParameters(risk_free_rate=0.02, dividend_yield=0.0,
+                                                                        volatility=0.25),
+                          spot_price=spot, valuation_time=V)
+
+
+def snowball(coupon=0.15, knock_in=75.0, at_expiry=False, dates=None, first_level=100.0, barrier_state=None):
+    cal = TradingCalendar.weekdays()
+    dates = dates or [cal.add_trading_days(V, 21 * k) for k in range(1, 13)]
+    levels = [first_level] + [100.0 - 0.5 * k for k in range(1, len(dates))]
+    return SnowballOption(
+        maturity_coupon_rate=coupon, initial_spot=100.0, knock_in_level=knock_in, upper_strike=100.0,
+        lower_strike=0.0,
+        knock_in_observation=(KnockInObservation.at_expiry() if at_expiry
+                              else KnockInObservation.every_trading_day(dt.time(0, 0))),
+        effective_time=V, expiry_time=dates[-1],
+        autocall_observations=[SnowballObservation(d, lv, coupon) for d, lv in zip(dates, levels)],
+        barrier_state=barrier_state, calendar=cal)
+
+
+FIRST = snowball()
+SECOND = snowball(coupon=0.09, knock_in=90.0)
+EARLY = snowball(dates=[TradingCalendar.weekdays().add_trading_days(V, d) for d in (22, 43, 62)])
+OTHER_GRID = snowball(coupon=0.09, knock_in=90.0, at_expiry=True)
+
+BATCHES = [
+    ("same grid", [FIRST, SECOND, FIRST]),
+    ("early notes", [FIRST] + [EARLY] * 32),
+    ("mixed grids", [FIRST, OTHER_GRID, SECOND]),
+    ("41 mixed", [[FIRST, OTHER_GRID, SECOND][i % 3] for i in range(41)]),
+    ("coupon ladder", [snowball(coupon=0.01 * k) for k in range(1, 21)]),
+]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("sampling,antithetic,paths", [("pseudo_random", True, 2048), ("sobol", False, 4096),
+                                                       ("pseudo_random", False, 3001)])
+@pytest.mark.parametrize("name,notes", BATCHES, ids=[b[0] for b in BATCHES])
+def test_batch_prices_equal_scalar_prices_bit_for_bit(gpu, name, notes, sampling, antithetic, paths, precision):
+    engine = MonteCarloAutocallableEngine(path_count=paths, seed=42, sampling=sampling, antithetic=antithetic,
+                                          precision=precision)
+    got = engine.price_many(notes, context())
+    assert len(got) == len(notes)
+    for price, note in zip(got, notes):
+        assert price == engine.price(note, context())
+    assert engine.price_many(notes, context(), shared_grid=True) == got
+
+
+def test_phoenix_batch_and_known_values(gpu):
+    engine = MonteCarloAutocallableEngine(path_count=4096, seed=7, sampling="sobol")
+    notes = [phoenix(), phoenix(ki=70.0), phoenix(gap=5), phoenix()]
+    got = engine.price_many(notes, context())
+    assert got == tuple(engine.price(n, context()) for n in notes)
+    assert got[0] == got[3]
+
+
+def test_notes_with_a_known_value_take_it_inside_a_batch(gpu):
+    # Valued on the first observation date at spot 105: the note whose level
+    # there is 100 knocks out on the host, the one at 110 is sampled.
+    first_date = TradingCalendar.weekdays().add_trading_days(V, 21)
+    at = PricingContext(model_parameters=BlackScholesMertonParameters(risk_free_rate=0.02, dividend_yield=0.0,
+                                                                      volatility=0.25),
+                        spot_price=105.0, valuation_time=first_date)
+    out_now = snowball(barrier_state="no_event")
+    alive = snowball(first_level=110.0, barrier_state="no_event")
+    engine = MonteCarloAutocallableEngine(path_count=4096, seed=11)
+    batch = engine.price_many([out_now, alive, out_now], at)
+    assert batch == (engine.price(out_now, at), engine.price(alive, at), engine.price(out_now, at))
+    assert engine.price_with_statistics(out_now, at).seed is None
+
+
+def test_unseeded_batch_uses_one_seed(gpu):
+    engine = MonteCarloAutocallableEngine(path_count=2048)
+    first, second = engine.price_many([FIRST, FIRST], context())
+    assert first == second
+
+
+def test_batches_need_one_product_type(gpu):
+    engine = MonteCarloAutocallableEngine(path_count=1024, seed=1)
+    with pytest.raises(UnsupportedOperationError):
+        engine.price_many([FIRST, phoenix()], context())
+    assert engine.price_many([], context()) == ()
