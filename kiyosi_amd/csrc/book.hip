@@ -76,7 +76,7 @@ __device__ __attribute__((always_inline)) void sample_body(const BookItem& it, i
   auto run = [&](bool on, float sign) -> double {
     switch (it.kind) {
       case IK_BARRIER:
-        return barrier_path<FP32, false>(it.u.b, dr, on, sign, bad);
+        return barrier_path<FP32, false, false>(it.u.b, dr, on, sign, bad);
       case IK_DIGITAL:
         return digital_path<FP32, false>(it.u.dg, dr, on, sign, bad);
       case IK_GEO_ASIAN:
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(256)
   uint32_t bad = 0;
   auto run = [&](bool on, float sign, double* out) {
     out[0] = it.kind == IK_DIGITAL ? digital_path<FP32, true>(it.u.dg, dr, on, sign, bad, out + 1)
-                                   : barrier_path<FP32, true>(it.u.b, dr, on, sign, bad, out + 1);
+                                   : barrier_path<FP32, true, true>(it.u.b, dr, on, sign, bad, out + 1);
   };
   double pos[3], neg[3] = {0.0, 0.0, 0.0};
   run(live, 1.0f, pos);
@@ -289,6 +289,9 @@ struct Planned {
   // values of a contract resolved before sampling.
   int32_t greeks = 0;
   double known_delta = 0.0, known_gamma = 0.0;
+  // Priced by the spot-Greek kernel instead of the persistent kernel
+  // (continuously monitored double barriers).
+  bool separate = false;
   BookItem item{};
   Sampler sampler{};
   // Tables staged into the shared blob: (blob offset, byte offset of the
@@ -855,6 +858,9 @@ struct BookRun {
   std::vector<int> sampled;  // indices into plans with sampling work
 };
 
+static void price_spot_greeks(kg_context* ctx, const kg_settings* settings, Planned p, Blob& blob,
+                              kg_spot_greeks* greeks, kg_estimate* out);
+
 static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_t n,
                        int32_t precision, kg_book_tuning* tuning, kg_estimate* out) {
   if (n == 0) return;
@@ -896,6 +902,13 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
     }
     Planned& p = plans[i];
     if (p.known) continue;
+    // Continuously monitored double barriers need the survival series, which
+    // the persistent kernel leaves out; they run on the spot-Greek kernel
+    // after the book, as kg_double_barrier_price runs them.
+    if (p.item.kind == IK_BARRIER && p.item.u.b.has_upper && p.item.u.b.continuous) {
+      p.separate = true;
+      continue;
+    }
     p.sampler = validate_settings(&c.settings, (uint64_t)p.item.dims);
     const Sampler& s = p.sampler;
     p.item.samples = s.samples();
@@ -914,7 +927,7 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
   }
   std::vector<int> sampled;
   for (size_t i = 0; i < n; ++i)
-    if (!plans[i].known) sampled.push_back((int)i);
+    if (!plans[i].known && !plans[i].separate) sampled.push_back((int)i);
   std::vector<Moments> results(2 * sampled.size());
   double device_ms = 0.0;
   if (!sampled.empty()) {
@@ -1040,6 +1053,17 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
     out[i] = kg_estimate{};
     if (p.known) {
       fill_known(p.value, cs.sampling == KG_SOBOL, &out[i]);
+      continue;
+    }
+    if (p.separate) {
+      kg_settings own = cs;
+      own.precision = precision;
+      Blob own_blob;
+      kg_spot_greeks none{};
+      price_spot_greeks(ctx, &own,
+                        plan_barrier(contract_of(*(const kg_double_barrier_terms*)contracts[i].terms),
+                                     contracts[i].market, own_blob),
+                        own_blob, &none, &out[i]);
       continue;
     }
     const Moments& m = results[2 * k];

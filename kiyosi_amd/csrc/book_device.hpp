@@ -129,8 +129,13 @@ __device__ inline void d_double_survival(double initial, double terminal, double
     };
     add(0.0, false);
     for (int n = 1; n <= 64; ++n) {
-      const double k = n;
-      const double tail = add(k, true) + add(k, false) + add(-k, true) + add(-k, false);
+      // The four images of n in the reference's order, (k, direct), (k,
+      // reflected), (-k, direct), (-k, reflected), summed left to right; one
+      // at a time, so that their exponentials do not overlap in registers
+      // (the book kernel inlines this beside every contract type).
+      double tail = 0.0;
+#pragma unroll 1
+      for (int t = 0; t < 4; ++t) tail += add(t < 2 ? (double)n : -(double)n, (t & 1) == 0);
       if (n >= 2 && tail < 1e-18) {
         converged = true;
         break;
@@ -187,8 +192,12 @@ struct BarrierDev {
 
 // MonteCarloBarrierEngine's sample closure (barrier.rs:291-477). With LR the
 // likelihood-ratio delta and gamma of the path go to lr[0], lr[1]
-// (:440-472); otherwise only the value is formed.
-template <bool FP32, bool LR, class D>
+// (:440-472); otherwise only the value is formed. SURVIVAL compiles the
+// double-barrier survival series for continuously monitored double barriers;
+// the persistent book kernel leaves it out (its FP64 temporaries would raise
+// the register count of every contract type there) and never receives such
+// contracts.
+template <bool FP32, bool LR, bool SURVIVAL, class D>
 __device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign, uint32_t& bad,
                                double* lr = nullptr) {
   using R = typename std::conditional<FP32, float, double>::type;
@@ -262,7 +271,7 @@ __device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign
   double value, first = 0.0, second = 0.0;
   if (has_cond) {
     value = conditioned;
-  } else if (b.continuous && b.has_upper) {
+  } else if (SURVIVAL && b.continuous && b.has_upper) {
     double sv[3];
     d_double_survival(b.initial, b.terminal_barrier + (double)noise, b.band, b.steps[0].width,
                       want_delta || want_gamma, sv, bad);

@@ -1,0 +1,54 @@
+"""Register and scratch budgets of the hot kernels, read from the gfx950
+assembly that hipcc emits for them (no GPU needed).
+
+The persistent book kernel inlines every contract type's path code, so a
+heavy helper added to one type raises the register count of all of them: at
+most 256 VGPRs keeps two waves per SIMD, and the FP32 book and the FP32
+snowball kernel must not spill to scratch. A regression here halves the
+book's throughput before any timing shows it."""
+
+import pathlib
+import re
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+CSRC = ROOT / "kiyosi_amd" / "csrc"
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+
+def kernel_resources(source, tmp_path):
+    asm = tmp_path / (source + ".s")
+    flags = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+             "-packed-fp32-ops", "-I", str(ROOT / "include"), "--cuda-device-only", "-S"]
+    subprocess.run([HIPCC, *flags, str(CSRC / source), "-o", str(asm)], check=True, capture_output=True)
+    text = asm.read_text()
+    out = {}
+    for block in text.split("  - .agpr_count:")[1:]:
+        name = re.search(r"\.name:\s+(\S+)", block).group(1)
+        vgpr = int(re.search(r"\.vgpr_count:\s+(\d+)", block).group(1))
+        private = int(re.search(r"\.private_segment_fixed_size:\s+(\d+)", block).group(1))
+        out[name] = (vgpr, private)
+    return out
+
+
+@pytest.mark.skipif(not pathlib.Path(HIPCC).exists(), reason="hipcc is not installed")
+def test_book_kernels_keep_two_waves_per_simd(tmp_path):
+    res = kernel_resources("book.hip", tmp_path)
+    book = {k: v for k, v in res.items() if "book_kernel" in k}
+    assert len(book) == 6
+    for name, (vgpr, private) in book.items():
+        assert vgpr <= 256, (name, vgpr)
+        if "book_kernelILb1E" in name:  # FP32
+            assert private == 0, (name, private)
+
+
+@pytest.mark.skipif(not pathlib.Path(HIPCC).exists(), reason="hipcc is not installed")
+def test_snowball_sobol_fp32_kernel_has_no_scratch(tmp_path):
+    res = kernel_resources("autocall.hip", tmp_path)
+    name = next(k for k in res if k.startswith("_ZN2kg15autocall_kernelILb1ELb1ELb1E"))
+    vgpr, private = res[name]
+    assert private == 0
+    assert vgpr <= 128, vgpr  # four or more waves per SIMD
