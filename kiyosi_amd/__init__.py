@@ -21,6 +21,8 @@ from .errors import (
     InvalidInputError,
     KiyosiError,
     MissingHistoryError,
+    NonConvergenceError,
+    RootNotBracketedError,
     UnsupportedOperationError,
 )
 from .instruments import (
@@ -47,6 +49,15 @@ from .market import (
     LocalVolatilitySurface,
     PricingContext,
     TradingCalendar,
+)
+
+from .solvers import (
+    ImpliedCouponSettings,
+    ImpliedVolatilitySettings,
+    SnowballCouponQuote,
+    implied_phoenix_coupon,
+    implied_snowball_coupon,
+    implied_volatility,
 )
 
 __version__ = "0.1.0"

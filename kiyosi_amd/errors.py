@@ -30,6 +30,14 @@ class MissingHistoryError(KiyosiError):
     """Prior barrier history is required but unknown."""
 
 
+class RootNotBracketedError(KiyosiError):
+    """Error::RootNotBracketed: a solver's bounds do not bracket the target."""
+
+
+class NonConvergenceError(KiyosiError, ArithmeticError):
+    """Error::NonConvergence: a solver or series stopped without converging."""
+
+
 class DeviceError(OSError):
     """A HIP runtime failure on the GPU."""
 

@@ -427,6 +427,25 @@ class _Monitored:
     def _knock_in(self):
         return self.barrier_type.endswith("_in")
 
+    def _payoff_requires_touch(self):
+        """payoff_requires_touch (monitoring.rs:314-321; double_barrier.rs:216-224)."""
+        times = self._times()
+        ends = times is None or times[-1] == to_micros(self.expiry_time)
+        call = self.option_type == "call"
+        if self._upper is not None:
+            return ends and (self.strike >= self._upper if call else self.strike <= self.barrier_level)
+        if self.barrier_type.startswith("up"):
+            return ends and call and self.strike >= self.barrier_level
+        return ends and not call and self.strike <= self.barrier_level
+
+    def volatility_affects_price(self, context):
+        """OptionContract::volatility_affects_price (instruments/barrier.rs:283-292,
+        double_barrier.rs:226-235)."""
+        touched, finished = self._state(context)
+        knock_in = self._knock_in()
+        return not ((touched and not knock_in) or (not touched and finished and knock_in)
+                    or (not knock_in and self._payoff_requires_touch() and self.rebate == 0.0))
+
     def at_spot_discontinuity(self, context, radius):
         """A monitored level within the spot stencil, or the vanilla kink at
         expiry while the payoff is live (spot_discontinuity and
