@@ -415,6 +415,17 @@ typedef struct kg_asian_terms {
 kg_status kg_asian_price(kg_context* ctx, const kg_settings* settings, const kg_asian_terms* option,
                          const kg_bsm_market* market, kg_estimate* out);
 
+/* Strike batch of an arithmetic average: MonteCarloAsianEngine::price_strikes
+ * (engines/monte_carlo/asian.rs:197-345). prices[i] equals the price of
+ * kg_asian_price for the option with strike strikes[i] and the same
+ * settings, bit for bit, as the reference's batch equals its scalar prices
+ * (crates/kiyosi/tests/asian_monte_carlo.rs:144-151). Distinct strikes are
+ * priced once; up to 32 of them (the reference's STRIKE_TILE) share each
+ * path of one launch. Strikes must be positive and finite. */
+kg_status kg_asian_price_strikes(kg_context* ctx, const kg_settings* settings, const kg_asian_terms* option,
+                                 const kg_bsm_market* market, const double* strikes, size_t n,
+                                 double* prices);
+
 /* ---- Accumulators and decumulators: MonteCarloAccumulatorEngine
  * (engines/monte_carlo/accumulator.rs:91-218; instrument
  * instruments/accumulator.rs:401-526). One step per trading-day fixing at

@@ -54,6 +54,8 @@ SIGNATURES = {
                                                P(_abi.BsmMarket), P(_abi.SpotGreeks), P(_abi.Estimate)]),
     "kg_asian_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AsianTerms), P(_abi.BsmMarket),
                                  P(_abi.Estimate)]),
+    "kg_asian_price_strikes": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AsianTerms), P(_abi.BsmMarket),
+                                         P(C.c_double), C.c_size_t, P(C.c_double)]),
     "kg_accumulator_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AccumulatorTerms), P(_abi.BsmMarket),
                                        P(_abi.Estimate)]),
     "kg_american_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AmericanTerms), P(_abi.AmericanModel),

@@ -277,6 +277,76 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Arithmetic Asian strike lanes (MonteCarloAsianEngine::price_strikes,
+// asian.rs:197-345): one path per sample, shared by up to STRIKE_LANES strike
+// payoffs; lane l's total and pair moments of the block go to
+// partials[block * 2n + 2l] and [... + 1], each folded in the book's sample
+// order, so a lane equals pricing its strike alone bit for bit.
+constexpr int STRIKE_LANES = 32;  // asian.rs:12 (STRIKE_TILE)
+
+template <bool FP32, bool SOBOL>
+__global__ void __launch_bounds__(256)
+    asian_strikes_kernel(const BookItem* __restrict__ item, const ArithAsianDev* __restrict__ lanes, int n_lanes,
+                         Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
+  __shared__ double s_pos[BLOCK_SAMPLES], s_neg[BLOCK_SAMPLES];
+  __shared__ unsigned char s_state[BLOCK_SAMPLES];
+  const BookItem& it = *item;
+  const int64_t block = blockIdx.x;
+  const int j = threadIdx.x;
+  const int64_t n = block * BLOCK_SAMPLES + j;
+  const bool live = n < it.samples;
+  const bool paired = it.antithetic && live && n < it.paths / 2;
+  Draws<SOBOL, FP32, 1> dr;
+  dr.dims = it.dims;
+  dr.direct = true;
+  dr.dirs = it.dirs;
+  dr.shifts = it.shifts;
+  dr.point = (uint64_t)n;
+  if (!SOBOL) {
+    const uint64_t first = (uint64_t)j * (uint64_t)it.dims;
+    if (FP32) {
+      dr.g32.init(it.seed, (uint64_t)block);
+      dr.g32.advance((first >> 1) * 4);
+      dr.has32 = false;
+      if (first & 1) {
+        float e;
+        f32::box_muller_pair(dr.g32, &e, &dr.spare32);
+        dr.has32 = true;
+      }
+    } else {
+      dr.s.start(it.seed, (uint64_t)block, first);
+    }
+  }
+  const auto origin = dr;
+  // The lanes share the path grid; the control's geometric sum is formed for
+  // all of them (it changes no other part of the path state).
+  ArithAsianDev grid = lanes[0];
+  grid.control = 1;
+  const ArithPath pos = arith_path<FP32>(grid, dr, live, 1.0f);
+  ArithPath neg{0.0, -INFINITY, 0.0};
+  if (it.antithetic && __ballot(paired) != 0) {
+    dr = origin;
+    neg = arith_path<FP32>(grid, dr, paired, -1.0f);
+  }
+  uint32_t bad = 0;
+  s_state[j] = (unsigned char)(live | (paired << 1));
+  for (int l = 0; l < n_lanes; ++l) {
+    s_pos[j] = live ? arith_payoff(lanes[l], pos, bad) : 0.0;
+    s_neg[j] = paired ? arith_payoff(lanes[l], neg, bad) : 0.0;
+    __syncthreads();
+    if (threadIdx.x < WAVE) {
+      Moments m, pm;
+      fold_block(s_pos, s_neg, s_state, &m, &pm);
+      if (threadIdx.x == 0) {
+        partials[block * 2 * n_lanes + 2 * l] = m;
+        partials[block * 2 * n_lanes + 2 * l + 1] = pm;
+      }
+    }
+    __syncthreads();
+  }
+  if (bad) raise_flag(flags, FLAG_NONFINITE);
+}
+
 // ------------------------------------------------------------------ host
 // Result of planning one contract: either a known value or a sampled item.
 enum class Estimator { PathMeans, Estimate };
@@ -1173,6 +1243,129 @@ static void price_spot_greeks(kg_context* ctx, const kg_settings* settings, Plan
   if (!std::isfinite(greeks->delta) || !std::isfinite(greeks->gamma)) unstable();
 }
 
+// MonteCarloAsianEngine::price_strikes (asian.rs:197-345) for an arithmetic
+// average: the distinct strikes (quote_distinct_strikes, engines.rs:38-75)
+// are planned one by one; those that sample share each path as lanes of
+// asian_strikes_kernel, STRIKE_LANES per launch. A lane whose controlled
+// price is negative is priced alone, as the reference does, which resamples
+// it without the control (asian.rs:744-752).
+static void asian_price_strikes(kg_context* ctx, const kg_settings* settings, const kg_asian_terms* option,
+                                const kg_bsm_market* market, const double* strikes, size_t n, double* prices) {
+  if (!settings || !option || !market) fail(KG_INVALID_INPUT, "arguments must not be null");
+  if (n == 0) return;
+  if (!strikes || !prices) fail(KG_INVALID_INPUT, "arguments must not be null");
+  if (n > (size_t(1) << 30) / 24) fail(KG_INVALID_INPUT, "Monte Carlo strike batch workspace exceeds 1 GiB");
+  if (option->averaging != KG_AVERAGE_ARITHMETIC)
+    fail(KG_UNSUPPORTED_OPERATION, "strike batches price arithmetic averages");
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(strikes[i]) || !(strikes[i] > 0.0)) fail(KG_INVALID_INPUT, "strikes must be positive and finite");
+  std::vector<size_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return strikes[a] < strikes[b]; });
+  std::vector<double> unique;
+  std::vector<size_t> slot(n);
+  for (size_t k = 0; k < n; ++k) {
+    const double v = strikes[order[k]];
+    if (unique.empty() || unique.back() != v) unique.push_back(v);
+    slot[order[k]] = unique.size() - 1;
+  }
+  std::vector<double> value(unique.size());
+  std::vector<Planned> plans(unique.size());
+  std::vector<size_t> sampled;
+  Blob blob;  // every plan's tables (arithmetic plans stage none)
+  for (size_t u = 0; u < unique.size(); ++u) {
+    kg_asian_terms t = *option;
+    t.strike = unique[u];
+    plans[u] = plan_asian(t, *market, blob);
+    if (plans[u].known)
+      value[u] = plans[u].value;
+    else
+      sampled.push_back(u);
+  }
+  if (!sampled.empty()) {
+    const Planned& p0 = plans[sampled[0]];
+    const ArithAsianDev& a0 = p0.item.u.a;
+    for (size_t u : sampled) {
+      const ArithAsianDev& a = plans[u].item.u.a;
+      if (plans[u].item.kind != IK_ARITH_ASIAN || a.n != a0.n || a.spot != a0.spot || a.weight != a0.weight ||
+          a.log_discount != a0.log_discount || a.drift != a0.drift || a.width != a0.width || a.sign != a0.sign ||
+          a.interval_forward != a0.interval_forward)
+        fail(KG_COMPUTATION_FAILURE, "strike lanes do not share their path grid");
+    }
+    const Sampler s = validate_settings(settings, (uint64_t)p0.item.dims);
+    BookItem item = p0.item;
+    item.samples = s.samples();
+    item.paths = s.path_count;
+    item.seed = s.seed;
+    item.antithetic = s.antithetic;
+    item.sobol = s.sobol;
+    item.first_block = 0;
+    size_t dirs_off = 0, shifts_off = 0;
+    if (s.sobol) {
+      std::vector<uint64_t> dirs, shifts;
+      sobol_tables(ctx, (uint32_t)item.dims, s.seed, dirs, shifts);
+      dirs_off = blob.add(dirs);
+      shifts_off = blob.add(shifts);
+    }
+    std::vector<ArithAsianDev> lanes;
+    for (size_t u : sampled) lanes.push_back(plans[u].item.u.a);
+    const size_t lanes_off = blob.add(lanes);
+    const size_t item_off = (blob.bytes.size() + 255) & ~size_t(255);
+    blob.bytes.resize(item_off + sizeof(BookItem));
+    unsigned char* d = (unsigned char*)ctx->scratch.reserve(blob.bytes.size());
+    if (s.sobol) {
+      item.dirs = (const uint64_t*)(d + dirs_off);
+      item.shifts = (const uint64_t*)(d + shifts_off);
+    }
+    check_tables(item);
+    std::memcpy(blob.bytes.data() + item_off, &item, sizeof(BookItem));
+    upload_async(ctx, d, blob.bytes.data(), blob.bytes.size());
+    const BookItem* d_item = (const BookItem*)(d + item_off);
+    const ArithAsianDev* d_lanes = (const ArithAsianDev*)(d + lanes_off);
+    const int64_t blocks = s.blocks();
+    if (blocks > (int64_t(1) << 31) - 1) fail(KG_INVALID_INPUT, "too many sample blocks");
+    Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 2 * STRIKE_LANES * blocks);
+    std::vector<size_t> redo;
+    for (size_t g = 0; g < sampled.size(); g += STRIKE_LANES) {
+      const int m = (int)std::min<size_t>(STRIKE_LANES, sampled.size() - g);
+      uint32_t* flags = reset_flags(ctx);
+      timing_begin(ctx);
+      auto launch = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(BLOCK_SAMPLES), 0, ctx->stream, d_item, d_lanes + g,
+                           m, parts, flags);
+      };
+      if (s.sobol && s.fp32)
+        launch(asian_strikes_kernel<true, true>);
+      else if (s.sobol)
+        launch(asian_strikes_kernel<false, true>);
+      else if (s.fp32)
+        launch(asian_strikes_kernel<true, false>);
+      else
+        launch(asian_strikes_kernel<false, false>);
+      KG_HIP(hipGetLastError());
+      Moments r[2 * STRIKE_LANES];
+      double ms = 0.0;
+      finish_sampling(ctx, parts, blocks, 2 * m, r, &ms);
+      for (int l = 0; l < m; ++l) {
+        const size_t u = sampled[g + l];
+        value[u] = scaled_value(r[2 * l].scale, r[2 * l].mean, 0.0) + plans[u].control_expected;
+        if (plans[u].item.u.a.control && value[u] < 0.0) redo.push_back(u);
+      }
+    }
+    for (size_t u : redo) {
+      kg_asian_terms t = *option;
+      t.strike = unique[u];
+      kg_estimate e{};
+      price_single(ctx, KG_BOOK_ASIAN, settings, &t, market, &e);
+      value[u] = e.price;
+    }
+  }
+  for (size_t i = 0; i < n; ++i) {
+    if (!std::isfinite(value[slot[i]])) unstable();
+    prices[i] = value[slot[i]];
+  }
+}
+
 template <class Terms>
 static void barrier_with_greeks(kg_context* ctx, const kg_settings* settings, const Terms* option,
                                 const kg_bsm_market* market, kg_spot_greeks* greeks, kg_estimate* out) {
@@ -1226,6 +1419,11 @@ kg_status kg_digital_price_with_greeks(kg_context* ctx, const kg_settings* setti
     Planned p = plan_digital(*option, *market, blob, greeks->request);
     price_spot_greeks(ctx, settings, std::move(p), blob, greeks, out);
   });
+}
+
+kg_status kg_asian_price_strikes(kg_context* ctx, const kg_settings* settings, const kg_asian_terms* option,
+                                 const kg_bsm_market* market, const double* strikes, size_t n, double* prices) {
+  return guarded(ctx, [&] { asian_price_strikes(ctx, settings, option, market, strikes, n, prices); });
 }
 
 kg_status kg_asian_price(kg_context* ctx, const kg_settings* settings, const kg_asian_terms* option,

@@ -394,8 +394,16 @@ __device__ inline double geometric_values_d(const ArithAsianDev& p, double lf, d
   return p.sign > 0 ? d_positive_difference(la, lc) : d_positive_difference(lc, la);
 }
 
+// State of one arithmetic-average path after its loop: the relative sum of
+// expm1 growths, the log-sum once growth passed e^600, and the geometric
+// sum of the control variate. Nothing in it depends on the strike.
+struct ArithPath {
+  double rel, log_sum, gg;
+};
+
+// ArithmeticGrid::path (asian.rs:640-712): the strike-independent loop.
 template <bool FP32, class D>
-__device__ double arith_asian(const ArithAsianDev& p, D& dr, bool live, float sign, uint32_t& bad) {
+__device__ ArithPath arith_path(const ArithAsianDev& p, D& dr, bool live, float sign) {
   using R = typename std::conditional<FP32, float, double>::type;
   const int dims = p.n - 1;
   R growth = 0;
@@ -416,7 +424,15 @@ __device__ double arith_asian(const ArithAsianDev& p, D& dr, bool live, float si
       rel += fraction * (FP32 ? (double)f32::expm1((float)growth) : expm1((double)growth));
     }
   }
-  if (!live) return 0.0;
+  return ArithPath{rel, log_sum, gg};
+}
+
+// ArithmeticParameters::arithmetic with the geometric control residual
+// (asian.rs:1100-1256, 1016-1061): the discounted payoff of one path state
+// for the strike terms of `p`.
+__device__ inline double arith_payoff(const ArithAsianDev& p, const ArithPath& a, uint32_t& bad) {
+  const double inv_n = 1.0 / (double)p.n;
+  const double rel = a.rel, log_sum = a.log_sum, gg = a.gg;
   const double half = 0.5 * inv_n, rest = 1.0 - half;
   const double log_spot = log(p.spot), log_weight = log(p.weight);
   const double log_rest = log_sum == -INFINITY ? log(rest) + log1p(rel / rest) : log_sum;
@@ -454,6 +470,13 @@ __device__ double arith_asian(const ArithAsianDev& p, D& dr, bool live, float si
   const double v = price - residual;
   if (!isfinite(v)) bad = 1;
   return v;
+}
+
+template <bool FP32, class D>
+__device__ double arith_asian(const ArithAsianDev& p, D& dr, bool live, float sign, uint32_t& bad) {
+  const ArithPath a = arith_path<FP32>(p, dr, live, sign);
+  if (!live) return 0.0;
+  return arith_payoff(p, a, bad);
 }
 
 // ------------------------------------------------------------------ accumulator

@@ -152,7 +152,10 @@ class MonteCarloVanillaEngine(_MonteCarloEngine):
         engines/monte_carlo.rs:1334-1409). Returns a PricingResult whose
         values equal numerical_greeks(engine, ...) with the same seed."""
         from . import greeks
-        if not isinstance(option, EuropeanOption):
+        from .instruments import AmericanOption
+        # American options (a subclass of EuropeanOption here) price every
+        # stencil scenario with the exercise-grid estimator.
+        if not isinstance(option, EuropeanOption) or isinstance(option, AmericanOption):
             return greeks.numerical_greeks(self, option, context, request, settings)
         engine = self._resolved()
 
@@ -180,6 +183,28 @@ class MonteCarloVanillaEngine(_MonteCarloEngine):
                                                   C.byref(option._american_terms(engine.step_count)),
                                                   C.byref(model), C.byref(options), C.byref(report)))
         return report
+
+    def price_strikes(self, instrument, context, *, strikes):
+        """American prices at each strike on one resolved seed, with a separate
+        exercise rule fitted per distinct strike (MonteCarloVanillaEngine::
+        price_strikes, american.rs:677-709): each equals ``price`` of that
+        strike's option. Distinct strikes are priced once each on the GPU."""
+        import copy
+        import math
+        from .instruments import AmericanOption
+        if not isinstance(instrument, AmericanOption):
+            raise TypeError("price_strikes takes an AmericanOption")
+        values = [float(k) for k in strikes]
+        for k in values:
+            if not (math.isfinite(k) and k > 0.0):
+                raise InvalidInputError("strikes must be positive and finite")
+        engine = self._resolved()
+        quotes = {}
+        for k in sorted(set(values)):
+            option = copy.copy(instrument)
+            option.strike = k
+            quotes[k] = engine.price(option, context)
+        return tuple(quotes[k] for k in values)
 
     def price_american_out_of_sample(self, option, context, *, training_path_count, basis="quadratic"):
         """Fits the exercise rule on training paths (seed ^ 0x9e3779b97f4a7c15) and
@@ -343,6 +368,29 @@ class MonteCarloAsianEngine(_MonteCarloEngine):
         if not isinstance(instrument, _AveragePrice):
             raise TypeError("MonteCarloAsianEngine prices average-price options")
         return _native.lib().kg_asian_price, instrument._terms(self.step_count), context.bsm_market()
+
+    def price_strikes(self, instrument, context, *, strikes):
+        """Prices of the arithmetic-average option at each strike on one
+        resolved seed (MonteCarloAsianEngine::price_strikes, asian.rs:197-345):
+        each equals ``price`` of that strike's option bit for bit; distinct
+        strikes share the GPU paths as lanes."""
+        from .instruments import ArithmeticAveragePriceOption
+        if not isinstance(instrument, ArithmeticAveragePriceOption):
+            raise TypeError("price_strikes takes an ArithmeticAveragePriceOption")
+        values = [float(k) for k in strikes]
+        if not values:
+            return ()
+        engine = self._resolved()
+        array = (C.c_double * len(values))(*values)
+        out = (C.c_double * len(values))()
+        ctx = _native.context(engine.device)
+        ctx.check(_native.lib().kg_asian_price_strikes(ctx.handle, C.byref(engine._settings()),
+                                                       C.byref(instrument._terms(engine.step_count)),
+                                                       C.byref(context.bsm_market()), array, len(values), out))
+        return tuple(out)
+
+    def _resolved(self):
+        return self if self.seed is not None else self.with_resolved_seed()
 
 
 class MonteCarloAccumulatorEngine(_MonteCarloEngine):
