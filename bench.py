@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the config-2 FP32 sampling kernel: FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0 from the two counter passes of `bench.py --profile-pass`,
 # scaled on a known 256 MiB read/write (profiles/hbm_traffic_fetch_write_counters.md).
-C2_KERNEL_HBM_BYTES = 961_200
+C2_KERNEL_HBM_BYTES = 923_410
 
 
 def log(msg):
