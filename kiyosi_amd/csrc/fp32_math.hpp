@@ -7,7 +7,8 @@
 //     (monte_carlo.rs:1852-1894) maps onto them directly;
 //   * an inverse normal of the form x * P(w), w = -ln(4 p (1 - p)), after
 //     M. Giles (GPU Computing Gems Jade Edition, 2011), with coefficients
-//     fitted by kiyosi_amd/csrc/fit_inverse_normal.py (max abs error 2.3e-6
+//     fitted by kiyosi_amd/csrc/fit_inverse_normal.py (degree 7 central,
+//     degree 8 tail; max abs error 7.3e-7 in the central region and 2.3e-6
 //     over all 32-bit inputs, including both tails).
 #pragma once
 
@@ -45,14 +46,15 @@ __device__ inline float expm1(float x) {
 
 // Inverse normal CDF of the midpoint of a 32-bit uniform:
 // p = (bits + 1/2) / 2^32. With l = log2(4 p (1 - p)) <= 0, the central
-// polynomial runs in u = -ln2 l - 5/2 and the tail one (w = -ln2 l >= 5) in
-// u = sqrt(ln2) sqrt(-l) - 3, each a single fma from l. The tail is a
+// polynomial runs in l itself (its constant term is sqrt(pi/2), the slope of
+// the inverse normal at p = 1/2) and the tail one (w = -ln2 l >= 5) in
+// t = sqrt(ln2) sqrt(-l) - 3. The tail is a
 // wave-uniform branch taken only when some lane of the wave needs it (about
 // one draw in 290), so the common path carries no exec-mask bookkeeping.
 __device__ inline float normal_from_bits(uint32_t bits) {
-  constexpr float CENTRAL[9] = {3.293115824e-08f,  4.844033015e-07f, -4.897925464e-06f,
-                                -6.199391464e-06f, 3.087891673e-04f, -1.773068332e-03f,
-                                -5.907721352e-03f, 3.488027155e-01f, 2.123313427e+00f};
+  constexpr float CENTRAL[8] = {-3.723791053e-08f, -1.437705009e-06f, -1.994798913e-05f,
+                                -6.944068446e-05f, 1.055057975e-03f,  7.811283227e-03f,
+                                -2.274445444e-01f, 1.253313541e+00f};
   constexpr float TAIL[9] = {1.557544238e-06f,  -2.612999815e-04f, 1.683062175e-03f,
                              -4.751312081e-03f, 8.147120476e-03f,  -1.092399564e-02f,
                              1.335252635e-02f,  1.416593313e+00f,  4.006433964e+00f};
@@ -63,10 +65,9 @@ __device__ inline float normal_from_bits(uint32_t bits) {
   const float ps = fmaf((float)k, 0x1p-32f, 0x1p-33f);  // min(p, 1 - p) in (0, 1/2]
   const float ax = fmaf(-2.0f, ps, 1.0f);              // |2p - 1|
   const float l = __builtin_amdgcn_logf(ps * fmaf(-4.0f, ps, 4.0f));
-  const float u = fmaf(-LN2, l, -2.5f);
   float r = CENTRAL[0];
 #pragma unroll
-  for (int i = 1; i < 9; ++i) r = fmaf(r, u, CENTRAL[i]);
+  for (int i = 1; i < 8; ++i) r = fmaf(r, l, CENTRAL[i]);
   const bool tail = l <= TAIL_L;
   if (wave_any(tail)) {
     const float t = fmaf(SQRT_LN2, sqrt(-l), -3.0f);
