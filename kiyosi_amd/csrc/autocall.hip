@@ -253,7 +253,7 @@ static AcLanes stage_lanes(const std::vector<const AcHostPlan*>& L, std::vector<
   stage(blob, interleave(L, &AcHostPlan::events, 1), &o[6]);
   stage(blob, intervals, &o[7]);
   stage(blob, interleave(L, &AcHostPlan::steps32, 2), &o[8]);
-  stage(blob, interleave(L, &AcHostPlan::weight32, 1), &o[9]);
+  stage(blob, interleave(L, &AcHostPlan::keep32, 1), &o[9]);
   stage(blob, interleave(L, &AcHostPlan::fill32, 1), &o[10]);
   stage(blob, interleave(L, &AcHostPlan::events32, 2), &o[11]);
   stage(blob, interleave(L, &AcHostPlan::intervals32, 2), &o[12]);
@@ -271,7 +271,7 @@ static void bind_lanes(AcLanes& P, unsigned char* d, const LaneOffsets& off) {
   P.events = (const AcEventD*)(d + o[6]);
   P.intervals = (const AcStepD*)(d + o[7]);
   P.steps32 = (const float*)(d + o[8]);
-  P.weight32 = (const float*)(d + o[9]);
+  P.keep32 = (const float*)(d + o[9]);
   P.fill32 = (const float*)(d + o[10]);
   P.events32 = (const float*)(d + o[11]);
   P.intervals32 = (const float*)(d + o[12]);

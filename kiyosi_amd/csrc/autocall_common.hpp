@@ -64,7 +64,7 @@ struct AcPlanDev {
   const double* fill;    // bridge fill diffusions per step
   // FP32 copies of the per-step tables
   const float* steps32;     // drift, diffusion pairs
-  const float* wf32;        // bridge fill weight, diffusion pairs (one scalar load per step)
+  const float* wf32;        // bridge fill 1 - weight, -diffusion pairs (one scalar load per step)
   const float* events32;    // knock_out, coupon_barrier pairs
   const float* intervals32; // drift, diffusion pairs
 };
@@ -75,7 +75,7 @@ template <>
 struct Tab<double> {
   static __device__ inline double drift(const AcPlanDev& p, int i) { return p.steps[i].drift; }
   static __device__ inline double diff(const AcPlanDev& p, int i) { return p.steps[i].diffusion; }
-  static __device__ inline double w(const AcPlanDev& p, int i) { return p.weight[i]; }
+  static __device__ inline double weight(const AcPlanDev& p, int i) { return p.weight[i]; }
   static __device__ inline double f(const AcPlanDev& p, int i) { return p.fill[i]; }
   static __device__ inline double ko(const AcPlanDev& p, int e) { return p.events[e].knock_out; }
   static __device__ inline double cb(const AcPlanDev& p, int e) { return p.events[e].coupon_barrier; }
@@ -86,7 +86,7 @@ template <>
 struct Tab<float> {
   static __device__ inline float drift(const AcPlanDev& p, int i) { return p.steps32[2 * i]; }
   static __device__ inline float diff(const AcPlanDev& p, int i) { return p.steps32[2 * i + 1]; }
-  static __device__ inline float w(const AcPlanDev& p, int i) { return p.wf32[2 * i]; }
+  static __device__ inline float weight(const AcPlanDev& p, int i) { return p.wf32[2 * i]; }
   static __device__ inline float f(const AcPlanDev& p, int i) { return p.wf32[2 * i + 1]; }
   static __device__ inline float ko(const AcPlanDev& p, int e) { return p.events32[2 * e]; }
   static __device__ inline float cb(const AcPlanDev& p, int e) { return p.events32[2 * e + 1]; }
@@ -120,16 +120,46 @@ __device__ inline double terminal_value(const Plan& p, double lr, bool ki, uint3
   return discounted_product(1.0, amount, p.final_log_discount);
 }
 
-// One daily bridge fill step lr + w (target - lr) + f z. FP64 rounds as
-// the reference writes it (lr += w.mul_add(target - lr, f * z),
-// autocallable.rs:678); FP32 takes three roundings. Every kernel that fills
+// One interval of daily bridge fill steps lr <- lr + w (target - lr) + f z
+// and the knock-in test `minimum < level` over them. FP64 rounds as the
+// reference writes it (lr += w.mul_add(target - lr, f * z),
+// autocallable.rs:678) and tracks the minimum. FP32 steps the gap
+// g = target - lr instead, g <- (1 - w) g - f z: two roundings and two
+// instructions a step, and lr < level becomes g > target - level. The
+// tables hold the coefficients each form takes: w and f for FP64,
+// 1 - w and -f (each rounded once from FP64) for FP32. Every kernel that fills
 // a bridge uses these, so lane and single-note prices agree bit for bit.
-__device__ inline double fill_step(double lr, double target, double w, double f, double z) {
-  return lr + fma(w, target - lr, f * z);
-}
-__device__ inline float fill_step(float lr, float target, float w, float f, float z) {
-  return fmaf(w, target - lr, fmaf(f, z, lr));
-}
+template <typename R>
+struct FillTrack;
+template <>
+struct FillTrack<double> {
+  double lr, lowest, level;
+  __device__ inline void start(double lr0, double target, double lvl) {
+    lr = lr0;
+    lowest = INFINITY;
+    level = lvl;
+  }
+  __device__ inline void step(double target, double w, double f, double z) {
+    lr = lr + fma(w, target - lr, f * z);
+    lowest = fmin(lowest, lr);
+  }
+  __device__ inline bool below() const { return lowest < level; }
+};
+template <>
+struct FillTrack<float> {
+  float gap, limit;
+  bool under;
+  __device__ inline void start(float lr0, float target, float lvl) {
+    gap = target - lr0;
+    limit = target - lvl;
+    under = false;
+  }
+  __device__ inline void step(float, float keep, float neg_f, float z) {
+    gap = fmaf(keep, gap, neg_f * z);
+    under = under || gap > limit;
+  }
+  __device__ inline bool below() const { return under; }
+};
 
 // Source of the normals of one sample. DIRECT fixes at compile time whether
 // Sobol points come from global directions (1) or the block's LDS tables (0);
@@ -226,59 +256,63 @@ __device__ double bridge_path(const AcPlanDev& p, D& dr, bool live, float sign,
     }
     // below: the path went under the knock-in level in this interval (the
     // reference's `minimum < level`).
-    bool below = false;
+    FillTrack<R> track;
+    track.start(lr, target, level);
     if constexpr (SOBOL) {
       // Sobol draws are stateless, so every lane steps the whole interval and
       // nothing in the loop branches: a lane that is not filling resets lr to
-      // the target below and never reads `below`, and a lane that went under
-      // the level (where the reference stops filling) keeps below set. The
-      // lanes still filling compute exactly what the reference computes.
+      // the target below and never reads the track, and a lane that went
+      // under the level (where the reference stops filling) stays below.
+      // The lanes still filling compute exactly what the reference computes.
       if (wave_any(filling)) {
-        // FP32: the next step's point bits (LDS) and fill weights (scalar
-        // load) are read one step ahead, so both overlap the current step's
-        // arithmetic. The last reads repeat dimension and step `end`, which
-        // exist: `end` is a step of the path.
-        const float* wf = FP32 ? p.wf32 + 2 * start : nullptr;
-        uint32_t next = 0;
-        float next_w = 0.0f, next_f = 0.0f;
+        // FP32: the next step's point bits (LDS) and fill coefficients
+        // (scalar load) are read one step ahead, so both overlap the current
+        // step's arithmetic. The last coefficient read is step `end`, which
+        // exists (`end` is a step of the path); the last point read is
+        // dimension end + 1 <= dims, which the LDS tables' slack holds and the
+        // direct form clamps to `end`. The loop runs in pairs of steps so that
+        // consecutive LDS reads share one address and differ by an immediate
+        // offset (the compiler does not unroll a loop holding wave votes).
         if constexpr (FP32) {
-          next = dr.bits32(start + 1);
-          next_w = wf[0];
-          next_f = wf[1];
-        }
-        for (int i = start; i < end; ++i) {
-          if constexpr (FP32) {
+          const float* wf = p.wf32 + 2 * start;
+          uint32_t next = dr.bits32(start + 1);
+          float next_w = wf[0], next_f = wf[1];
+          auto fill_one = [&](int i) {
             const uint32_t bits = next;
             const float w = next_w, f = next_f;
-            next = dr.bits32(i + 2 < end ? i + 2 : end);
+            next = dr.bits32(dr.is_direct() && i + 2 > end ? end : i + 2);
             wf += 2;
             next_w = wf[0];
             next_f = wf[1];
-            const float z = sign * f32::normal_from_bits(bits);
-            lr = fill_step(lr, target, w, f, z);
-          } else {
-            const R z = (R)sign * dr.template normal<R>(i + 1);
-            lr = fill_step(lr, target, Tab<double>::w(p, i), Tab<double>::f(p, i), z);
+            track.step(target, w, f, f32::normal_from_bits(bits, sign));
+          };
+          int i = start;
+          for (; i + 1 < end; i += 2) {
+            fill_one(i);
+            fill_one(i + 1);
           }
-          below = below || lr < level;
+          if (i < end) fill_one(i);
+        } else {
+          for (int i = start; i < end; ++i) {
+            const R z = (R)sign * dr.template normal<R>(i + 1);
+            track.step(target, Tab<double>::weight(p, i), Tab<double>::f(p, i), z);
+          }
         }
       }
     } else {
-      R minimum = (R)INFINITY;
       for (int i = start; i < end; ++i) {
         if (__ballot(filling) == 0) break;
-        if (filling && minimum < level) {
+        if (filling && track.below()) {
           dr.skip(end - i);
           filling = false;
         }
         if (filling) {
           const R z = (R)sign * dr.template normal<R>(i + 1);
-          lr = fill_step(lr, target, Tab<R>::w(p, i), Tab<R>::f(p, i), z);
-          minimum = fmin(minimum, lr);
+          track.step(target, Tab<R>::weight(p, i), Tab<R>::f(p, i), z);
         }
       }
-      below = minimum < level;
     }
+    const bool below = track.below();
     lr = target;
     start = end + 1;
     if (active) {
@@ -379,8 +413,8 @@ struct AcLanes {
   const AcEventD* events;       // [event][lane]
   const AcStepD* intervals;     // [interval][lane] drift, diffusion
   const float* steps32;         // [step][lane] drift, diffusion
-  const float* weight32;        // [step][lane]
-  const float* fill32;          // [step][lane]
+  const float* keep32;          // [step][lane] 1 - weight
+  const float* fill32;          // [step][lane] -diffusion
   const float* events32;        // [event][lane] knock_out, coupon_barrier
   const float* intervals32;     // [interval][lane] drift, diffusion
 };
@@ -391,7 +425,7 @@ template <>
 struct LaneTab<double> {
   static __device__ inline double drift(const AcLanes& P, int i, int l) { return P.steps[i * P.n + l].drift; }
   static __device__ inline double diff(const AcLanes& P, int i, int l) { return P.steps[i * P.n + l].diffusion; }
-  static __device__ inline double w(const AcLanes& P, int i, int l) { return P.weight[i * P.n + l]; }
+  static __device__ inline double weight(const AcLanes& P, int i, int l) { return P.weight[i * P.n + l]; }
   static __device__ inline double f(const AcLanes& P, int i, int l) { return P.fill[i * P.n + l]; }
   static __device__ inline double ko(const AcLanes& P, int e, int l) { return P.events[e * P.n + l].knock_out; }
   static __device__ inline double cb(const AcLanes& P, int e, int l) { return P.events[e * P.n + l].coupon_barrier; }
@@ -404,7 +438,7 @@ template <>
 struct LaneTab<float> {
   static __device__ inline float drift(const AcLanes& P, int i, int l) { return P.steps32[2 * (i * P.n + l)]; }
   static __device__ inline float diff(const AcLanes& P, int i, int l) { return P.steps32[2 * (i * P.n + l) + 1]; }
-  static __device__ inline float w(const AcLanes& P, int i, int l) { return P.weight32[i * P.n + l]; }
+  static __device__ inline float weight(const AcLanes& P, int i, int l) { return P.keep32[i * P.n + l]; }
   static __device__ inline float f(const AcLanes& P, int i, int l) { return P.fill32[i * P.n + l]; }
   static __device__ inline float ko(const AcLanes& P, int e, int l) { return P.events32[2 * (e * P.n + l)]; }
   static __device__ inline float cb(const AcLanes& P, int e, int l) { return P.events32[2 * (e * P.n + l) + 1]; }
@@ -425,7 +459,8 @@ __device__ void bridge_lanes(const AcLanes& P, D& dr, bool live, float sign, uin
   using T = LaneTab<R>;
   const int n = P.n;
   uint32_t monitoring = 0, ki = 0, active = 0;
-  R lr[L], target[L], minimum[L];
+  R lr[L], target[L];
+  FillTrack<R> track[L];
 #pragma unroll
   for (int l = 0; l < L; ++l) {
     const bool on = l < n;
@@ -444,10 +479,10 @@ __device__ void bridge_lanes(const AcLanes& P, D& dr, bool live, float sign, uin
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       target[l] = lr[l];
-      minimum[l] = (R)INFINITY;
       if (bit(active, l))
         target[l] = lr[l] + (FP32 ? fmaf((float)T::idiff(P, k, l), (float)z, (float)T::idrift(P, k, l))
                                   : fma((double)T::idiff(P, k, l), (double)z, (double)T::idrift(P, k, l)));
+      track[l].start(lr[l], target[l], (R)P.s[l].knock_in);
     }
     // monitoring implies active: a knock-out ends both.
     uint32_t filling = monitoring;
@@ -457,17 +492,14 @@ __device__ void bridge_lanes(const AcLanes& P, D& dr, bool live, float sign, uin
       if (filling) {
 #pragma unroll
         for (int l = 0; l < L; ++l)
-          if (bit(filling, l) && minimum[l] < (R)P.s[l].knock_in) filling &= ~(1u << l);
+          if (bit(filling, l) && track[l].below()) filling &= ~(1u << l);
         if (!filling) dr.skip(end - i);
       }
       if (filling) {
         const R zf = (R)sign * dr.template normal<R>(i + 1);
 #pragma unroll
         for (int l = 0; l < L; ++l)
-          if (bit(filling, l)) {
-            lr[l] = fill_step(lr[l], target[l], T::w(P, i, l), T::f(P, i, l), zf);
-            minimum[l] = fmin(minimum[l], lr[l]);
-          }
+          if (bit(filling, l)) track[l].step(target[l], T::weight(P, i, l), T::f(P, i, l), zf);
       }
     }
     start = end + 1;
@@ -478,7 +510,7 @@ __device__ void bridge_lanes(const AcLanes& P, D& dr, bool live, float sign, uin
       lr[l] = target[l];
       if (!bit(active, l)) continue;
       const R level = (R)P.s[l].knock_in;
-      if (bit(monitoring, l) && (minimum[l] < level || (observe && lr[l] < level))) {
+      if (bit(monitoring, l) && (track[l].below() || (observe && lr[l] < level))) {
         monitoring &= ~(1u << l);
         ki |= 1u << l;
       }
@@ -585,7 +617,7 @@ struct AcHostPlan {
   std::vector<AcEventD> events;
   std::vector<AcIntervalD> intervals;
   std::vector<double> weight, fill;
-  std::vector<float> steps32, weight32, fill32, wf32, events32, intervals32;
+  std::vector<float> steps32, keep32, fill32, wf32, events32, intervals32;
 };
 
 struct NoteView {
@@ -841,10 +873,10 @@ inline AcHostPlan build_plan(const kg_autocall_terms& t, const kg_bsm_market& m)
     h.steps32.push_back((float)s.drift);
     h.steps32.push_back((float)s.diffusion);
   }
-  for (double v : h.weight) h.weight32.push_back((float)v);
-  for (double v : h.fill) h.fill32.push_back((float)v);
-  for (size_t i = 0; i < h.weight32.size(); ++i) {
-    h.wf32.push_back(h.weight32[i]);
+  for (double v : h.weight) h.keep32.push_back((float)(1.0 - v));
+  for (double v : h.fill) h.fill32.push_back((float)-v);
+  for (size_t i = 0; i < h.keep32.size(); ++i) {
+    h.wf32.push_back(h.keep32[i]);
     h.wf32.push_back(h.fill32[i]);
   }
   for (const AcEventD& e : h.events) {

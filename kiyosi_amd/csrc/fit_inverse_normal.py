@@ -65,11 +65,15 @@ def main():
         np.linspace(0, 2**32 - 1, 2_000_001).astype(np.uint64),
         (2**32 - 1 - np.arange(0, 1 << 16, dtype=np.uint64)),
     ]))
-    # As the kernel evaluates it: ps = min(p, 1 - p), l = log2(4 ps (1 - ps)),
-    # central variable l and tail variable sqrt(ln2) sqrt(-l) - 3, Horner in
-    # float32 with one rounding per step as fmaf gives.
-    k = np.minimum(bits, 2**32 - 1 - bits).astype(np.float64)
-    ps = ((k + 0.5) * 2.0 ** -32).astype(np.float32)
+    # As the kernel evaluates it: s = bits read as int32,
+    # q = fl(fl(s) * -2^-32 - 2^-33) (one fma), ps = |q| = min(p, 1 - p),
+    # l = log2(4 ps (1 - ps)), central variable l and tail variable
+    # sqrt(ln2) sqrt(-l) - 3, Horner in float32 with one rounding per step as
+    # fmaf gives; the sign of the normal is the sign of q.
+    s = bits.astype(np.int64)
+    s = np.where(s >= 2**31, s - 2**32, s)
+    q = (s.astype(np.float32).astype(np.float64) * -2.0 ** -32 - 2.0 ** -33).astype(np.float32)
+    ps = np.abs(q)
     ax = (1.0 - 2.0 * ps.astype(np.float64)).astype(np.float32)
     four = (4.0 - 4.0 * ps.astype(np.float64)).astype(np.float32)
     l = np.log2((ps * four).astype(np.float32)).astype(np.float32)
@@ -78,8 +82,7 @@ def main():
     t32 = tail.astype(np.float32)
     pc = horner(c32, l)
     pt = horner(t32, t)
-    sign = np.where(bits >= 2**31, np.float32(1), np.float32(-1))
-    z = np.where(l <= np.float32(-W_SPLIT / np.log(2.0)), pt, pc).astype(np.float32) * ax * sign
+    z = np.copysign(np.where(l <= np.float32(-W_SPLIT / np.log(2.0)), pt, pc).astype(np.float32) * ax, q)
     exact = ndtri((bits.astype(np.float64) + 0.5) * 2.0 ** -32)
     err = np.abs(z.astype(np.float64) - exact)
     rel = err / np.maximum(np.abs(exact), 1e-3)

@@ -45,13 +45,18 @@ __device__ inline float expm1(float x) {
 }
 
 // Inverse normal CDF of the midpoint of a 32-bit uniform:
-// p = (bits + 1/2) / 2^32. With l = log2(4 p (1 - p)) <= 0, the central
-// polynomial runs in l itself (its constant term is sqrt(pi/2), the slope of
-// the inverse normal at p = 1/2) and the tail one (w = -ln2 l >= 5) in
-// t = sqrt(ln2) sqrt(-l) - 3. The tail is a
+// p = (bits + 1/2) / 2^32. Read as a signed integer s, the bits give
+// q = -(s + 1/2) / 2^32 with |q| = min(p, 1 - p) and the sign of the normal
+// (s = -(k + 1) for p > 1/2, with k = ~bits the distance to the top end), so
+// one conversion and one fma carry both. With l = log2(4 |q| (1 - |q|)) <= 0,
+// the central polynomial runs in l itself (its constant term is sqrt(pi/2),
+// the slope of the inverse normal at p = 1/2) and the tail one
+// (w = -ln2 l >= 5) in t = sqrt(ln2) sqrt(-l) - 3. The tail is a
 // wave-uniform branch taken only when some lane of the wave needs it (about
 // one draw in 290), so the common path carries no exec-mask bookkeeping.
-__device__ inline float normal_from_bits(uint32_t bits) {
+// `sign` (+1 or -1) multiplies the result through q, which costs nothing
+// once it is a constant.
+__device__ inline float normal_from_bits(uint32_t bits, float sign = 1.0f) {
   constexpr float CENTRAL[8] = {-3.723791053e-08f, -1.437705009e-06f, -1.994798913e-05f,
                                 -6.944068446e-05f, 1.055057975e-03f,  7.811283227e-03f,
                                 -2.274445444e-01f, 1.253313541e+00f};
@@ -60,10 +65,9 @@ __device__ inline float normal_from_bits(uint32_t bits) {
                              1.335252635e-02f,  1.416593313e+00f,  4.006433964e+00f};
   constexpr float SQRT_LN2 = 0.832554611157697756f;
   constexpr float TAIL_L = -7.21347520444482f;  // log2 value where w = 5
-  const bool upper = (bits >> 31) != 0;
-  const uint32_t k = upper ? ~bits : bits;            // distance to the nearer end
-  const float ps = fmaf((float)k, 0x1p-32f, 0x1p-33f);  // min(p, 1 - p) in (0, 1/2]
-  const float ax = fmaf(-2.0f, ps, 1.0f);              // |2p - 1|
+  const float q = fmaf((float)(int32_t)bits, -0x1p-32f * sign, -0x1p-33f * sign);
+  const float ps = fabsf(q);                       // min(p, 1 - p) in (0, 1/2]
+  const float ax = fmaf(-2.0f, ps, 1.0f);          // |2p - 1|
   const float l = __builtin_amdgcn_logf(ps * fmaf(-4.0f, ps, 4.0f));
   float r = CENTRAL[0];
 #pragma unroll
@@ -71,12 +75,12 @@ __device__ inline float normal_from_bits(uint32_t bits) {
   const bool tail = l <= TAIL_L;
   if (wave_any(tail)) {
     const float t = fmaf(SQRT_LN2, sqrt(-l), -3.0f);
-    float q = TAIL[0];
+    float v = TAIL[0];
 #pragma unroll
-    for (int i = 1; i < 9; ++i) q = fmaf(q, t, TAIL[i]);
-    r = tail ? q : r;
+    for (int i = 1; i < 9; ++i) v = fmaf(v, t, TAIL[i]);
+    r = tail ? v : r;
   }
-  return upper ? ax * r : -ax * r;
+  return copysignf(ax * r, q);
 }
 
 // Uniform midpoint in (0, 1) from the high PCG word of a 52-bit uniform.

@@ -18,8 +18,11 @@ namespace kg {
 
 constexpr size_t SOBOL_LDS_LIMIT = 64 * 1024;
 
+// Both tables plus one dimension of slack after `hi`, so that a read one
+// dimension past the path (a loop's last look-ahead, whose value goes
+// unused) stays inside the allocation.
 inline size_t sobol_lds_bytes(int dims, bool fp32) {
-  return (size_t)dims * 32 * (fp32 ? sizeof(uint32_t) : sizeof(uint64_t));
+  return ((size_t)dims * 32 + 16) * (fp32 ? sizeof(uint32_t) : sizeof(uint64_t));
 }
 
 template <typename W, int SHIFT>
