@@ -23,40 +23,29 @@
 
 namespace kg {
 
-// The draw source of sample j of `block`: the block's Sobol LDS tables (or
-// direct points for long paths), or the block's PCG stream advanced to
-// normal j * dims.
-template <bool SOBOL, bool FP32, bool LDS>
-__device__ inline Draws<SOBOL, FP32, LDS ? 0 : 1> setup_draws(int dims, uint64_t seed, int64_t block, int j,
-                                                              int64_t n, const uint64_t* __restrict__ dirs,
-                                                              const uint64_t* __restrict__ shifts,
-                                                              unsigned char* lds) {
-  Draws<SOBOL, FP32, LDS ? 0 : 1> dr;
-  const bool sobol_lds = LDS;
-  dr.dims = dims;
-  dr.direct = !sobol_lds;
-  dr.dirs = dirs;
-  dr.shifts = shifts;
-  dr.point = (uint64_t)n;
+// The draw source of sample j of `block`, from a prototype that holds the
+// Sobol tables (if any): the lane's Gray-code digits of the block's LDS
+// tables, its point index for direct Sobol points, or the block's PCG stream
+// advanced to normal j * dims.
+template <bool SOBOL, bool FP32, class D>
+__device__ inline D sample_draws(D dr, uint64_t seed, int64_t block, int j) {
+  dr.point = (uint64_t)(block * BLOCK_SAMPLES + j);
   if (SOBOL) {
-    if (sobol_lds) {
+    if (!dr.is_direct()) {
       const uint32_t g = (uint32_t)(j ^ (j >> 1));
       dr.glo = g & 15u;
       dr.ghi = g >> 4;
-      if (FP32)
-        sobol_build_lds32(dirs, shifts, dims, (uint64_t)block * BLOCK_SAMPLES, lds, &dr.lo, &dr.hi);
-      else
-        sobol_build_lds64(dirs, shifts, dims, (uint64_t)block * BLOCK_SAMPLES, lds, &dr.lo, &dr.hi);
     }
   } else {
-    const uint64_t first = (uint64_t)j * (uint64_t)dims;
+    const uint64_t first = (uint64_t)j * (uint64_t)dr.dims;
     if (FP32) {
       dr.g32.init(seed, (uint64_t)block);
       dr.g32.advance((first >> 1) * 4);
       dr.has32 = false;
       if (first & 1) {
-        float e;
-        f32::box_muller_pair(dr.g32, &e, &dr.spare32);
+        float e, spare;
+        f32::box_muller_pair(dr.g32, &e, &spare);
+        dr.spare32 = spare;
         dr.has32 = true;
       }
     } else {
@@ -66,8 +55,33 @@ __device__ inline Draws<SOBOL, FP32, LDS ? 0 : 1> setup_draws(int dims, uint64_t
   return dr;
 }
 
-// LDS: Sobol points come from the block's LDS tables (paths short enough for
-// them); otherwise from global directions.
+// Builds the block's Sobol LDS tables (all threads of the block take part)
+// and returns the draw source of this thread's sample.
+template <bool SOBOL, bool FP32, bool LDS>
+__device__ inline Draws<SOBOL, FP32, LDS ? 0 : 1> setup_draws(int dims, uint64_t seed, int64_t block, int j,
+                                                              const uint64_t* __restrict__ dirs,
+                                                              const uint64_t* __restrict__ shifts,
+                                                              unsigned char* lds) {
+  Draws<SOBOL, FP32, LDS ? 0 : 1> dr;
+  dr.dims = dims;
+  dr.direct = !LDS;
+  dr.dirs = dirs;
+  dr.shifts = shifts;
+  if (SOBOL && LDS) {
+    const void* lo = nullptr;
+    const void* hi = nullptr;
+    if (FP32)
+      sobol_build_lds32(dirs, shifts, dims, (uint64_t)block * BLOCK_SAMPLES, lds, &lo, &hi);
+    else
+      sobol_build_lds64(dirs, shifts, dims, (uint64_t)block * BLOCK_SAMPLES, lds, &lo, &hi);
+    dr.lo = lo;
+    dr.hi = hi;
+  }
+  return sample_draws<SOBOL, FP32>(dr, seed, block, j);
+}
+
+// One thread per sample, whole paths in one pass: plans without a bridge
+// (step_path) and bridge plans too large for the two-pass kernels.
 template <bool SOBOL, bool FP32, bool LDS>
 __global__ void __launch_bounds__(256)
     autocall_kernel(AcPlanDev p, uint64_t seed, int64_t samples, int64_t paths, int antithetic,
@@ -79,7 +93,7 @@ __global__ void __launch_bounds__(256)
   const int j = threadIdx.x;
   const int64_t n = block * BLOCK_SAMPLES + j;
   const bool live = n < samples;
-  auto dr = setup_draws<SOBOL, FP32, LDS>(p.n_steps, seed, block, j, n, dirs, shifts, lds);
+  auto dr = setup_draws<SOBOL, FP32, LDS>(p.n_steps, seed, block, j, dirs, shifts, lds);
   uint32_t bad = 0;
   const bool paired = antithetic && live && n < paths / 2;
   const auto origin = dr;
@@ -107,6 +121,225 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Bridge plans in two passes (bridge_events, bridge_knock_in): the daily
+// fill runs only for the legs that reach expiry still monitoring the
+// knock-in, gathered per sample block. The launches:
+//   bridge_setup_kernel   - the interval rows (interval_row) and the Sobol
+//                           table rows of the intervals' first dimensions
+//                           without a base point, once; and (Sobol) every
+//                           sample block's base point, one word per
+//                           dimension;
+//   bridge_events_kernel  - pass one per sample and leg; writes each leg's
+//                           value (the final value, or the cash before the
+//                           expiry payoff of a leg that waits on its fill)
+//                           and the waiting legs (sample << 1 | leg) of each
+//                           wave into the wave's own queue region of 128
+//                           items, with their count;
+//   bridge_fill_kernel    - pass two, one wave per sample block over its
+//                           four regions; adds the expiry payoff;
+//   bridge_moments_kernel - the per-block moments of the leg values, merged
+//                           exactly as autocall_kernel merges them.
+template <bool FP32>
+using SobolWord = typename std::conditional<FP32, uint32_t, uint64_t>::type;
+constexpr int FILL_THREADS = 256;
+constexpr int FILL_WAVES = FILL_THREADS / 64;
+constexpr int REGION = 128;  // queue items per pass-one wave: 64 samples, two legs
+
+template <bool SOBOL, bool FP32, bool LDS>
+__global__ void __launch_bounds__(256)
+    bridge_setup_kernel(AcPlanDev p, const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shifts,
+                        int64_t blocks, void* __restrict__ rows_out, void* __restrict__ table_out,
+                        SobolWord<FP32>* __restrict__ base_rows) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  using W = SobolWord<FP32>;
+  constexpr int SHIFT = FP32 ? 20 : 0;
+  const int dims = p.n_steps;
+  if (blockIdx.x == 0) {
+    IntervalRow<R>* rows = (IntervalRow<R>*)rows_out;
+    for (int k = threadIdx.x; k < p.n_intervals; k += 256) {
+      rows[k] = interval_row<R>(p, k);
+      if (SOBOL && LDS) {
+        W* lo = (W*)table_out;
+        sobol_table_rows<W, SHIFT>(dirs, dims, rows[k].first, 0ull, lo + k * 16,
+                                   lo + ((size_t)p.n_intervals + k) * 16);
+      }
+    }
+    return;
+  }
+  if (!(SOBOL && LDS)) return;
+  const int64_t i = (int64_t)(blockIdx.x - 1) * 256 + threadIdx.x;
+  if (i >= blocks * dims) return;
+  const int64_t b = i / dims;
+  const int d = (int)(i - b * dims);
+  base_rows[i] = (W)(sobol_block_base(dirs, shifts, dims, (uint64_t)b * BLOCK_SAMPLES, d) >> SHIFT);
+}
+
+template <bool SOBOL, bool FP32, bool LDS>
+__global__ void __launch_bounds__(256)
+    bridge_events_kernel(AcPlanDev p, uint64_t seed, int64_t samples, int64_t paths, int antithetic,
+                         const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shifts,
+                         double* __restrict__ values, const void* __restrict__ rows_in,
+                         const void* __restrict__ table_in, const void* __restrict__ base_rows,
+                         uint32_t* __restrict__ queue, uint32_t* __restrict__ counts, uint32_t* __restrict__ flags) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  using W = SobolWord<FP32>;
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int64_t block = blockIdx.x;
+  const int j = threadIdx.x;
+  const int64_t n = block * BLOCK_SAMPLES + j;
+  const bool live = n < samples;
+  const int dims = p.n_steps;
+  // LDS: the interval rows, then (Sobol) the table rows of the intervals'
+  // first dimensions (row k holds the first dimension of interval k) with
+  // this block's base point folded into `hi`. All reads are independent.
+  IntervalRow<R>* rows = (IntervalRow<R>*)lds;
+  for (int k = j; k < p.n_intervals; k += BLOCK_SAMPLES) rows[k] = ((const IntervalRow<R>*)rows_in)[k];
+  Draws<SOBOL, FP32, LDS ? 0 : 1> dr;
+  dr.dims = dims;
+  dr.direct = !LDS;
+  dr.dirs = dirs;
+  dr.shifts = shifts;
+  if (SOBOL && LDS) {
+    W* lo = (W*)(rows + p.n_intervals);
+    W* hi = lo + (size_t)p.n_intervals * 16;
+    const W* base = (const W*)base_rows + block * dims;
+    const W* table = (const W*)table_in;
+    for (int e = j; e < p.n_intervals * 32; e += BLOCK_SAMPLES) {
+      const int k = e >> 4;  // lo rows, then hi rows
+      W v = table[e];
+      if (k >= p.n_intervals) v ^= base[((const IntervalRow<R>*)rows_in)[k - p.n_intervals].first];
+      lo[e] = v;
+    }
+    dr.lo = lo;
+    dr.hi = hi;
+  }
+  __syncthreads();
+  dr = sample_draws<SOBOL, FP32>(dr, seed, block, j);
+  const auto origin = dr;
+  uint32_t bad = 0;
+  const bool paired = antithetic && live && n < paths / 2;
+  const BridgeEvents<R> up = bridge_events<SOBOL, FP32, SOBOL && LDS>(p, rows, dr, live, 1.0f);
+  BridgeEvents<R> down{};
+  const bool two_legs = antithetic && __ballot(paired) != 0;
+  if (two_legs) {
+    dr = origin;
+    down = bridge_events<SOBOL, FP32, SOBOL && LDS>(p, rows, dr, paired, -1.0f);
+  }
+  const bool want_up = live && up.need_fill;
+  const bool want_down = two_legs && paired && down.need_fill;
+  if (live) values[n] = want_up ? up.cash : bridge_finish(p, up, false, true, bad);
+  if (two_legs && paired) values[samples + n] = want_down ? down.cash : bridge_finish(p, down, false, true, bad);
+  // This wave's region: its leg-0 items in sample order, then its leg-1
+  // items; the count packs both numbers.
+  const uint64_t m_up = __ballot(want_up), m_down = __ballot(want_down);
+  const int lane = j & 63;
+  const uint64_t lower = (1ull << lane) - 1ull;
+  uint32_t* region = queue + (size_t)(block * FILL_WAVES + (j >> 6)) * REGION;
+  if (want_up) region[__popcll(m_up & lower)] = (uint32_t)(n << 1);
+  if (want_down) region[__popcll(m_up) + __popcll(m_down & lower)] = (uint32_t)(n << 1 | 1);
+  if (lane == 0) counts[block * FILL_WAVES + (j >> 6)] = (uint32_t)(__popcll(m_up) + __popcll(m_down));
+  if (bad) raise_flag(flags, FLAG_NONFINITE);
+}
+
+// Pass two over the queue, one wave per sample block: the wave copies its
+// block's base row into LDS once (so every lane's point is two table reads
+// and one broadcast read of the row, XORed) and fills the block's queued
+// legs 64 at a time.
+template <bool SOBOL, bool FP32, bool LDS>
+__global__ void __launch_bounds__(FILL_THREADS)
+    bridge_fill_kernel(AcPlanDev p, uint64_t seed, int64_t samples, int64_t blocks,
+                       const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shifts,
+                       const void* __restrict__ base_rows, const uint32_t* __restrict__ queue,
+                       const uint32_t* __restrict__ counts, double* __restrict__ values,
+                       uint32_t* __restrict__ flags) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  using W = SobolWord<FP32>;
+  constexpr int SHIFT = FP32 ? 20 : 0;
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int dims = p.n_steps;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t block = (int64_t)blockIdx.x * FILL_WAVES + wave;
+  // LDS tables without a base point (block-independent), then each wave's
+  // base row of dims + 1 words (the last one for the fill loop's final
+  // look-ahead, whose value goes unused).
+  Draws<SOBOL, FP32, LDS ? 2 : 1> proto;
+  proto.dims = dims;
+  proto.direct = !LDS;
+  proto.dirs = dirs;
+  proto.shifts = shifts;
+  proto.base = nullptr;
+  W* row = nullptr;
+  if (SOBOL && LDS) {
+    W* lo = (W*)lds;
+    W* hi = lo + (size_t)dims * 16;
+    for (int d = threadIdx.x; d < dims; d += FILL_THREADS)
+      sobol_table_rows<W, SHIFT>(dirs, dims, d, 0ull, lo + d * 16, hi + d * 16);
+    row = hi + (size_t)dims * 16 + 16 + (size_t)wave * (dims + 1);
+    if (block < blocks)
+      for (int d = lane; d <= dims; d += 64) row[d] = ((const W*)base_rows)[block * dims + d];
+    __syncthreads();
+    proto.lo = lo;
+    proto.hi = hi;
+  }
+  if (block >= blocks) return;
+  // The block's four regions, concatenated.
+  uint32_t edge[FILL_WAVES + 1];
+  edge[0] = 0;
+#pragma unroll
+  for (int w = 0; w < FILL_WAVES; ++w) edge[w + 1] = edge[w] + counts[block * FILL_WAVES + w];
+  uint32_t bad = 0;
+  for (uint32_t c = lane; c < edge[FILL_WAVES]; c += 64) {
+    int w = 0;
+#pragma unroll
+    for (int u = 1; u < FILL_WAVES; ++u) w += c >= edge[u];
+    const uint32_t item = queue[(size_t)(block * FILL_WAVES + w) * REGION + (c - edge[w])];
+    const int64_t n = item >> 1;
+    const int leg = item & 1;
+    auto dr = sample_draws<SOBOL, FP32>(proto, seed, block, (int)(n - block * BLOCK_SAMPLES));
+    if constexpr (SOBOL && LDS) dr.base = row;
+    R lr;
+    const bool ki = bridge_knock_in<SOBOL, FP32>(p, dr, leg ? -1.0f : 1.0f, lr);
+    double& value = values[leg * samples + n];
+    const double cash = value + terminal_value(p, (double)lr, ki, bad);
+    if (!isfinite(cash)) bad = 1;
+    value = cash;
+  }
+  if (bad) raise_flag(flags, FLAG_NONFINITE);
+}
+
+__global__ void __launch_bounds__(256)
+    bridge_moments_kernel(const double* __restrict__ values, int64_t samples, int64_t paths, int antithetic,
+                          Moments* __restrict__ partials) {
+  __shared__ Moments smem[2 * 4];
+  const int64_t block = blockIdx.x;
+  const int64_t n = block * BLOCK_SAMPLES + threadIdx.x;
+  const bool live = n < samples;
+  const bool paired = antithetic && live && n < paths / 2;
+  Moments m[2] = {moments_zero(), moments_zero()};
+  if (live) {
+    const double pos = values[n];
+    moments_add(m[0], pos, true);
+    if (paired) {
+      const double neg = values[samples + n];
+      moments_add(m[0], neg, true);
+      moments_add(m[1], midpoint(pos, neg), true);
+    }
+  }
+  // Without antithetic pairs the pair set is empty everywhere, and merging
+  // empty moments gives empty moments: only the first set needs the tree.
+  if (antithetic) {
+    block_merge<2>(m, smem);
+  } else {
+    Moments one[1] = {m[0]};
+    block_merge<1>(one, smem);
+    m[0] = one[0];
+  }
+  if (threadIdx.x == 0) {
+    partials[block * 2 + 0] = m[0];
+    partials[block * 2 + 1] = m[1];
+  }
+}
+
 // Scenario lanes on shared draws: lane l's total and pair moments of the
 // block go to partials[block * slots + slot_base + 2l] and [... + 1],
 // formed exactly as the single-plan kernel forms its own.
@@ -122,7 +355,7 @@ __global__ void __launch_bounds__(256)
   const int64_t n = block * BLOCK_SAMPLES + j;
   const bool live = n < samples;
   const AcLanes& P = *lanes;
-  auto dr = setup_draws<SOBOL, FP32, LDS>(P.n_steps, seed, block, j, n, dirs, shifts, lds);
+  auto dr = setup_draws<SOBOL, FP32, LDS>(P.n_steps, seed, block, j, dirs, shifts, lds);
   uint32_t bad = 0;
   const bool paired = antithetic && live && n < paths / 2;
   const auto origin = dr;
@@ -160,6 +393,63 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+static int cu_count(kg_context* ctx) {
+  int cus = 0;
+  KG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  return std::max(cus, 1);
+}
+
+// The two-pass launches of a bridge plan (bridge_setup_kernel,
+// bridge_events_kernel, bridge_fill_kernel, bridge_moments_kernel).
+template <bool SOBOL, bool FP32, bool LDS>
+static void launch_bridge(kg_context* ctx, const AcPlanDev& p, int dims, const Sampler& s, const uint64_t* dirs,
+                          const uint64_t* shifts, Moments* parts, uint32_t* flags) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  using W = SobolWord<FP32>;
+  const int64_t samples = s.samples(), blocks = s.blocks();
+  const int legs = s.antithetic ? 2 : 1;
+  const size_t regions = (size_t)blocks * FILL_WAVES;
+  double* values = (double*)ctx->bridge_values.reserve(sizeof(double) * (size_t)samples * legs);
+  // Per pass-one wave: a region of REGION items and its count.
+  uint32_t* queue = (uint32_t*)ctx->bridge_queue.reserve(sizeof(uint32_t) * regions * (REGION + 1));
+  uint32_t* counts = queue + regions * REGION;
+  // The interval rows and the first-dimension table rows (lo, then hi), then
+  // (Sobol) one base word per dimension per sample block plus one: the fill
+  // kernel copies dims + 1 words of a row.
+  const size_t rows_bytes = ((size_t)p.n_intervals * sizeof(IntervalRow<R>) + 255) & ~(size_t)255;
+  const size_t table_bytes = ((size_t)p.n_intervals * 32 * sizeof(W) + 255) & ~(size_t)255;
+  const size_t base_words = SOBOL && LDS ? (size_t)blocks * dims + 1 : 0;
+  unsigned char* setup =
+      (unsigned char*)ctx->bridge_base.reserve(rows_bytes + table_bytes + sizeof(W) * std::max<size_t>(base_words, 1));
+  void* rows = setup;
+  void* table = setup + rows_bytes;
+  W* base = (W*)(setup + rows_bytes + table_bytes);
+  size_t events_lds = (size_t)p.n_intervals * sizeof(IntervalRow<R>), fill_lds = 0;
+  int64_t setup_blocks = 1;
+  if (SOBOL && LDS) {
+    events_lds += (size_t)p.n_intervals * 32 * sizeof(W);
+    fill_lds = sobol_lds_bytes(dims, FP32) + (size_t)FILL_WAVES * (dims + 1) * sizeof(W);
+    setup_blocks += (blocks * dims + 255) / 256;
+  }
+  hipLaunchKernelGGL((bridge_setup_kernel<SOBOL, FP32, LDS>), dim3((unsigned)setup_blocks), dim3(256), 0, ctx->stream,
+                     p, dirs, shifts, blocks, rows, table, base);
+  KG_HIP(hipGetLastError());
+  hipLaunchKernelGGL((bridge_events_kernel<SOBOL, FP32, LDS>), dim3(blocks), dim3(256), events_lds, ctx->stream, p,
+                     s.seed, samples, s.path_count, (int)s.antithetic, dirs, shifts, values, rows, table, base, queue,
+                     counts, flags);
+  KG_HIP(hipGetLastError());
+  if (fill_lds > 64 * 1024)
+    KG_HIP(hipFuncSetAttribute((const void*)bridge_fill_kernel<SOBOL, FP32, LDS>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)fill_lds));
+  hipLaunchKernelGGL((bridge_fill_kernel<SOBOL, FP32, LDS>), dim3((unsigned)((blocks + FILL_WAVES - 1) / FILL_WAVES)),
+                     dim3(FILL_THREADS), fill_lds, ctx->stream, p, s.seed, samples, blocks, dirs, shifts, base, queue,
+                     counts, values, flags);
+  KG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(bridge_moments_kernel, dim3(blocks), dim3(256), 0, ctx->stream, values, samples, s.path_count,
+                     (int)s.antithetic, parts);
+  KG_HIP(hipGetLastError());
+}
+
 template <bool SOBOL, bool FP32, int L, class Plan>
 static void launch(kg_context* ctx, const Plan& p, int dims, const Sampler& s, const uint64_t* dirs,
                    const uint64_t* shifts, Moments* parts, uint32_t* flags) {
@@ -169,6 +459,16 @@ static void launch(kg_context* ctx, const Plan& p, int dims, const Sampler& s, c
     lds = sobol_lds_bytes(dims, FP32);
     sobol_lds = lds <= SOBOL_LDS_LIMIT;
     if (!sobol_lds) lds = 0;
+  }
+  if constexpr (std::is_same<Plan, AcPlanDev>::value) {
+    // Queue items carry the sample index in 31 bits.
+    if (p.bridge && s.samples() < ((int64_t)1 << 31)) {
+      if (sobol_lds)
+        launch_bridge<SOBOL, FP32, true>(ctx, p, dims, s, dirs, shifts, parts, flags);
+      else
+        launch_bridge<SOBOL, FP32, false>(ctx, p, dims, s, dirs, shifts, parts, flags);
+      return;
+    }
   }
   using Kernel = void (*)(Plan, uint64_t, int64_t, int64_t, int, const uint64_t*, const uint64_t*, Moments*,
                           uint32_t*);

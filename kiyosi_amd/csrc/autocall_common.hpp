@@ -161,14 +161,17 @@ struct FillTrack<float> {
   __device__ inline bool below() const { return under; }
 };
 
-// Source of the normals of one sample. DIRECT fixes at compile time whether
-// Sobol points come from global directions (1) or the block's LDS tables (0);
-// -1 leaves the choice to the `direct` field.
+// Source of the normals of one sample. DIRECT fixes at compile time where
+// Sobol points come from: global directions (1), the block's LDS tables (0),
+// or block-independent LDS tables XOR the sample block's base point, a row of
+// global memory (2; the tables' `hi` then leaves the base out); -1 leaves
+// the choice between 0 and 1 to the `direct` field.
 template <bool SOBOL, bool FP32, int DIRECT = -1>
 struct Draws {
-  // Sobol: LDS tables of this block and the lane's Gray-code digits
+  // Sobol: LDS tables and the lane's Gray-code digits
   const void* lo;
   const void* hi;
+  const void* base;  // DIRECT 2: the sample block's base point, one word per dimension
   uint32_t glo, ghi;
   int dims;
   // Sobol fallback for long paths: point index and global directions
@@ -186,13 +189,16 @@ struct Draws {
     if (SOBOL) {
       uint64_t bits = is_direct() ? sobol_point_direct(dirs, shifts, dims, point, d)
                                   : sobol_lds_bits64((const uint64_t*)lo, (const uint64_t*)hi, d, glo, ghi);
+      if constexpr (DIRECT == 2) bits ^= ((const uint64_t*)base)[d];
       return normal_from_bits(bits);
     }
     return s.next();
   }
-  __device__ inline bool is_direct() const { return DIRECT < 0 ? direct : DIRECT != 0; }
+  __device__ inline bool is_direct() const { return DIRECT < 0 ? direct : DIRECT == 1; }
   // Top 32 of the 52 bits of Sobol dimension d of this sample's point.
   __device__ inline uint32_t bits32(int d) {
+    if constexpr (DIRECT == 2)
+      return sobol_lds_bits32((const uint32_t*)lo, (const uint32_t*)hi, d, glo, ghi) ^ ((const uint32_t*)base)[d];
     return is_direct() ? (uint32_t)(sobol_point_direct(dirs, shifts, dims, point, d) >> 20)
                        : sobol_lds_bits32((const uint32_t*)lo, (const uint32_t*)hi, d, glo, ghi);
   }
@@ -229,6 +235,88 @@ struct Draws {
   }
 };
 
+// Target of bridge interval k (the interval's end point), drawn from the
+// interval's first dimension (bridge_payoffs::<1, true>,
+// autocallable.rs:640-660); `draw` is that dimension's index in `dr`.
+template <bool FP32, class D>
+__device__ inline typename std::conditional<FP32, float, double>::type interval_target(
+    const AcPlanDev& p, D& dr, int k, int draw, typename std::conditional<FP32, float, double>::type lr,
+    float sign) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  const R z = (R)sign * dr.template normal<R>(draw);
+  return lr + (FP32 ? fmaf((float)Tab<R>::idiff(p, k), (float)z, (float)Tab<R>::idrift(p, k))
+                    : fma((double)Tab<R>::idiff(p, k), (double)z, (double)Tab<R>::idrift(p, k)));
+}
+
+// The daily fill of one bridge interval, steps start..end-1 drawing
+// dimensions start+1..end (autocallable.rs:662-690); returns whether a
+// filling lane went under the knock-in level (the reference's
+// `minimum < level`). Lanes that are not filling step along and ignore the
+// result.
+template <bool SOBOL, bool FP32, class D>
+__device__ inline bool interval_fill(const AcPlanDev& p, D& dr, int start, int end,
+                                     typename std::conditional<FP32, float, double>::type lr,
+                                     typename std::conditional<FP32, float, double>::type target,
+                                     typename std::conditional<FP32, float, double>::type level,
+                                     bool filling, float sign) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  FillTrack<R> track;
+  track.start(lr, target, level);
+  if constexpr (SOBOL) {
+    // Sobol draws are stateless, so every lane steps the whole interval and
+    // nothing in the loop branches: a lane that went under the level (where
+    // the reference stops filling) stays below, and the lanes still filling
+    // compute exactly what the reference computes.
+    if (!wave_any(filling)) return false;
+    if constexpr (FP32) {
+      // The next step's point bits (LDS) and fill coefficients (scalar load)
+      // are read one step ahead, so both overlap the current step's
+      // arithmetic. The last coefficient read is step `end`, which exists
+      // (`end` is a step of the path); the last point read is dimension
+      // end + 1 <= dims, which the LDS tables' slack holds and the direct
+      // form clamps to `end`. The loop runs in pairs of steps so that
+      // consecutive LDS reads share one address and differ by an immediate
+      // offset (the compiler does not unroll a loop holding wave votes).
+      const float* wf = p.wf32 + 2 * start;
+      uint32_t next = dr.bits32(start + 1);
+      float next_w = wf[0], next_f = wf[1];
+      auto fill_one = [&](int i) {
+        const uint32_t bits = next;
+        const float w = next_w, f = next_f;
+        next = dr.bits32(dr.is_direct() && i + 2 > end ? end : i + 2);
+        wf += 2;
+        next_w = wf[0];
+        next_f = wf[1];
+        track.step(target, w, f, f32::normal_from_bits(bits, sign));
+      };
+      int i = start;
+      for (; i + 1 < end; i += 2) {
+        fill_one(i);
+        fill_one(i + 1);
+      }
+      if (i < end) fill_one(i);
+    } else {
+      for (int i = start; i < end; ++i) {
+        const R z = (R)sign * dr.template normal<R>(i + 1);
+        track.step(target, Tab<double>::weight(p, i), Tab<double>::f(p, i), z);
+      }
+    }
+  } else {
+    for (int i = start; i < end; ++i) {
+      if (__ballot(filling) == 0) break;
+      if (filling && track.below()) {
+        dr.skip(end - i);
+        filling = false;
+      }
+      if (filling) {
+        const R z = (R)sign * dr.template normal<R>(i + 1);
+        track.step(target, Tab<R>::weight(p, i), Tab<R>::f(p, i), z);
+      }
+    }
+  }
+  return track.below();
+}
+
 // Bridge path (bridge_payoffs::<1, true>, autocallable.rs:628-736). `sign`
 // is -1 for the antithetic leg. Returns the discounted cash of the path.
 template <bool SOBOL, bool FP32, class D>
@@ -248,71 +336,11 @@ __device__ double bridge_path(const AcPlanDev& p, D& dr, bool live, float sign,
     R target = lr;
     bool filling = false;
     if (active) {
-      R z = (R)sign * dr.template normal<R>(start);
-      target = lr + (FP32 ? fmaf((float)Tab<R>::idiff(p, k), (float)z, (float)Tab<R>::idrift(p, k))
-                          : fma((double)Tab<R>::idiff(p, k), (double)z, (double)Tab<R>::idrift(p, k)));
+      target = interval_target<FP32>(p, dr, k, start, lr, sign);
       filling = monitoring;
       if (!filling) dr.skip(end - start);
     }
-    // below: the path went under the knock-in level in this interval (the
-    // reference's `minimum < level`).
-    FillTrack<R> track;
-    track.start(lr, target, level);
-    if constexpr (SOBOL) {
-      // Sobol draws are stateless, so every lane steps the whole interval and
-      // nothing in the loop branches: a lane that is not filling resets lr to
-      // the target below and never reads the track, and a lane that went
-      // under the level (where the reference stops filling) stays below.
-      // The lanes still filling compute exactly what the reference computes.
-      if (wave_any(filling)) {
-        // FP32: the next step's point bits (LDS) and fill coefficients
-        // (scalar load) are read one step ahead, so both overlap the current
-        // step's arithmetic. The last coefficient read is step `end`, which
-        // exists (`end` is a step of the path); the last point read is
-        // dimension end + 1 <= dims, which the LDS tables' slack holds and the
-        // direct form clamps to `end`. The loop runs in pairs of steps so that
-        // consecutive LDS reads share one address and differ by an immediate
-        // offset (the compiler does not unroll a loop holding wave votes).
-        if constexpr (FP32) {
-          const float* wf = p.wf32 + 2 * start;
-          uint32_t next = dr.bits32(start + 1);
-          float next_w = wf[0], next_f = wf[1];
-          auto fill_one = [&](int i) {
-            const uint32_t bits = next;
-            const float w = next_w, f = next_f;
-            next = dr.bits32(dr.is_direct() && i + 2 > end ? end : i + 2);
-            wf += 2;
-            next_w = wf[0];
-            next_f = wf[1];
-            track.step(target, w, f, f32::normal_from_bits(bits, sign));
-          };
-          int i = start;
-          for (; i + 1 < end; i += 2) {
-            fill_one(i);
-            fill_one(i + 1);
-          }
-          if (i < end) fill_one(i);
-        } else {
-          for (int i = start; i < end; ++i) {
-            const R z = (R)sign * dr.template normal<R>(i + 1);
-            track.step(target, Tab<double>::weight(p, i), Tab<double>::f(p, i), z);
-          }
-        }
-      }
-    } else {
-      for (int i = start; i < end; ++i) {
-        if (__ballot(filling) == 0) break;
-        if (filling && track.below()) {
-          dr.skip(end - i);
-          filling = false;
-        }
-        if (filling) {
-          const R z = (R)sign * dr.template normal<R>(i + 1);
-          track.step(target, Tab<R>::weight(p, i), Tab<R>::f(p, i), z);
-        }
-      }
-    }
-    const bool below = track.below();
+    const bool below = interval_fill<SOBOL, FP32>(p, dr, start, end, lr, target, level, filling, sign);
     lr = target;
     start = end + 1;
     if (active) {
@@ -335,6 +363,158 @@ __device__ double bridge_path(const AcPlanDev& p, D& dr, bool live, float sign,
   if (active) {
     const bool hit = ki || (p.expiry_knock_in && p.has_ki && lr < level);
     cash += terminal_value(p, (double)lr, hit, bad);
+  }
+  if (live && !isfinite(cash)) bad = 1;
+  return cash;
+}
+
+// The same bridge path in two passes, for kernels that gather the paths
+// still waiting on their knock-in. A path's daily fill decides only whether
+// it knocks in, and knocking in changes only the payoff at expiry (event
+// coupons and knock-outs read the interval end points alone), so the fill of
+// a path that knocks out, or that is already knocked in, changes nothing.
+// Pass one (bridge_events) walks the interval end points, events and
+// observation-date knock-ins; pass two (bridge_knock_in) fills the paths
+// that reach expiry still monitoring, until they knock in; the expiry payoff
+// is added last, as bridge_path adds it. The sum is bridge_path's, bit for
+// bit.
+template <typename R>
+struct BridgeEvents {
+  double cash;     // coupon and event payments, expiry payoff excluded
+  R lr;            // last end point drawn
+  bool active;     // alive at expiry
+  bool ki;         // knocked in without the daily fill
+  bool need_fill;  // the expiry payoff waits on the daily fill
+};
+
+// What pass one reads of interval k, gathered per block in LDS so that the
+// interval loop waits on no global load: the end point's drift and
+// diffusion, its observation flag and event (knock-out and coupon levels,
+// discounted payments; the rare non-normal payments fall back to the event
+// in global memory).
+template <typename R>
+struct IntervalRow {
+  double pay_coupon, pay_out, pay_both;
+  R drift, diffusion, knock_out, coupon_barrier;
+  int32_t first, end, event, observe, has_knock_out, pad;  // first: the interval's first dimension
+};
+
+template <typename R>
+__device__ inline IntervalRow<R> interval_row(const AcPlanDev& p, int k) {
+  IntervalRow<R> r;
+  r.first = k == 0 ? 0 : p.intervals[k - 1].end + 1;
+  r.pad = 0;
+  r.end = p.intervals[k].end;
+  r.drift = Tab<R>::idrift(p, k);
+  r.diffusion = Tab<R>::idiff(p, k);
+  r.observe = p.step_observe[r.end];
+  r.event = p.step_event[r.end];
+  if (r.event >= 0) {
+    const AcEventD& ev = p.events[r.event];
+    r.pay_coupon = ev.pay_coupon;
+    r.pay_out = ev.pay_out;
+    r.pay_both = ev.pay_both;
+    r.knock_out = Tab<R>::ko(p, r.event);
+    r.coupon_barrier = Tab<R>::cb(p, r.event);
+    r.has_knock_out = ev.has_knock_out;
+  } else {
+    r.pay_coupon = r.pay_out = r.pay_both = 0.0;
+    r.knock_out = r.coupon_barrier = 0;
+    r.has_knock_out = 0;
+  }
+  return r;
+}
+
+// Pass one over the rows of `rows` (interval_row of every interval). ROWS:
+// the end point of interval k is dimension k of `dr` (tables that hold the
+// intervals' first dimensions only); otherwise dimension `start` as in
+// bridge_path. The arithmetic is bridge_path's, step for step.
+template <bool SOBOL, bool FP32, bool ROWS, class D>
+__device__ BridgeEvents<typename std::conditional<FP32, float, double>::type> bridge_events(
+    const AcPlanDev& p, const IntervalRow<typename std::conditional<FP32, float, double>::type>* rows, D& dr,
+    bool live, float sign) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  BridgeEvents<R> out;
+  double cash = p.coupon;
+  bool monitoring = !p.knocked_in && p.has_ki;
+  const R level = (R)p.knock_in;
+  bool ki = p.knocked_in;
+  bool active = live;
+  R lr = 0;
+  int start = 0;
+  for (int k = 0; k < p.n_intervals; ++k) {
+    if (__ballot(active) == 0) break;
+    // The whole row is read up front and combined with non-short-circuit
+    // `&` and `|`, so that no per-lane branch guards a read.
+    const IntervalRow<R> row = rows[k];
+    const int end = row.end;
+    // Every lane steps: a lane that is not active (knocked out, or not a
+    // sample) never has its lr, draws or flags read again, so only cash and
+    // the knock-out itself need the `active` guard, through selects.
+    const R z = (R)sign * dr.template normal<R>(ROWS ? k : start);
+    lr = lr + (FP32 ? fmaf((float)row.diffusion, (float)z, (float)row.drift)
+                    : fma((double)row.diffusion, (double)z, (double)row.drift));
+    dr.skip(end - start);
+    const bool observed_in = monitoring & (row.observe != 0) & (lr < level);
+    ki = ki | observed_in;
+    monitoring = monitoring & !observed_in;
+    if (row.event >= 0) {
+      const bool hit = active & (row.has_knock_out != 0) & (lr >= row.knock_out);
+      const bool due = lr >= row.coupon_barrier;
+      const double v = hit ? (due ? row.pay_both : row.pay_out) : (due ? row.pay_coupon : 0.0);
+      double pay = v;
+      const bool odd = active & !(v == 0.0 || isnormal(v));
+      if (wave_any(odd)) {
+        if (odd) pay = event_payment(p.events[row.event], due, hit);
+      }
+      cash = active ? cash + pay : cash;
+      active = active & !hit;
+      monitoring = monitoring & !hit;
+    }
+    start = end + 1;
+  }
+  out.cash = cash;
+  out.lr = lr;
+  out.active = active;
+  out.ki = ki;
+  // An expiry knock-in below the level settles the question too.
+  out.need_fill = active && monitoring && !(p.expiry_knock_in && lr < level);
+  return out;
+}
+
+// Pass two, for a path that bridge_events left monitoring at expiry (so it
+// never knocks out): draws the same end points again and fills until the
+// path knocks in. Returns whether it does; `lr_end` is the last end point.
+template <bool SOBOL, bool FP32, class D>
+__device__ bool bridge_knock_in(const AcPlanDev& p, D& dr, float sign,
+                                typename std::conditional<FP32, float, double>::type& lr_end) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  const R level = (R)p.knock_in;
+  bool monitoring = true;
+  R lr = 0;
+  int start = 0;
+  for (int k = 0; k < p.n_intervals; ++k) {
+    const int end = p.intervals[k].end;
+    const R target = interval_target<FP32>(p, dr, k, start, lr, sign);
+    const bool filling = monitoring;
+    if (!filling) dr.skip(end - start);
+    if (interval_fill<SOBOL, FP32>(p, dr, start, end, lr, target, level, filling, sign) && filling)
+      monitoring = false;
+    lr = target;
+    start = end + 1;
+  }
+  lr_end = lr;
+  return !monitoring;
+}
+
+// The expiry payoff added to a pass-one result (bridge_path's last lines).
+template <typename R>
+__device__ double bridge_finish(const AcPlanDev& p, const BridgeEvents<R>& ev, bool late_ki, bool live,
+                                uint32_t& bad) {
+  double cash = ev.cash;
+  if (ev.active) {
+    const bool hit = ev.ki || late_ki || (p.expiry_knock_in && p.has_ki && ev.lr < (R)p.knock_in);
+    cash += terminal_value(p, (double)ev.lr, hit, bad);
   }
   if (live && !isfinite(cash)) bad = 1;
   return cash;

@@ -22,6 +22,8 @@ constexpr int MERGE_THREADS = 1024;
 // predicate as a lane mask, so no VGPR copy of it is made (HIP's __ballot
 // takes an int and materialises one).
 __device__ inline bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+// True when p holds on every active lane of the wave.
+__device__ inline bool wave_all(bool p) { return __builtin_amdgcn_ballot_w64(!p) == 0; }
 
 // ------------------------------------------------------------------ PCG32
 struct Pcg32 {
@@ -247,9 +249,8 @@ __device__ inline Moments moments_zero() { return Moments{0.0, 0.0, 0.0, 0.0}; }
 // The moment updates round every product and sum on its own, as the
 // reference's Rust does (no contraction into FMAs): the result is then the
 // same in every kernel that inlines them.
-__device__ inline void moments_add(Moments& m, double value, bool variance) {
+__device__ inline void moments_add_general(Moments& m, double value, double a, bool variance) {
 #pragma clang fp contract(off)
-  double a = fabs(value);
   if (a > m.scale) {
     double ratio = m.scale / a;
     m.mean *= ratio;
@@ -263,16 +264,48 @@ __device__ inline void moments_add(Moments& m, double value, bool variance) {
   if (variance) m.m2 += delta * (v - m.mean);
 }
 
+__device__ inline void moments_add(Moments& m, double value, bool variance) {
+#pragma clang fp contract(off)
+  double a = fabs(value);
+  // The first finite nonzero value into empty moments: the general update
+  // gives exactly {1, |value|, +-1, 0} (ratio 0, v = value / |value| = +-1
+  // exactly, mean v, M2 v * (v - v) = 0), here without its divisions.
+  const bool first = m.count == 0.0 && m.scale == 0.0 && a > 0.0 && a <= 1.7976931348623157e308;
+  if (first) {
+    m = Moments{1.0, a, value > 0.0 ? 1.0 : -1.0, 0.0};
+  } else {
+    moments_add_general(m, value, a, variance);
+  }
+}
+
 __device__ inline Moments moments_merge(const Moments& a, const Moments& b) {
 #pragma clang fp contract(off)
   if (a.count == 0.0) return b;
   if (b.count == 0.0) return a;
   double sc = fmax(a.scale, b.scale);
-  double left = sc == 0.0 ? 0.0 : a.scale / sc;
-  double right = sc == 0.0 ? 0.0 : b.scale / sc;
+  // left = a.scale / sc and right = b.scale / sc. For a finite nonzero sc
+  // the operand that equals sc gives exactly 1, so one quotient serves; and
+  // equal counts give wb = b.count / n = 1/2 exactly. The results are the
+  // plain quotients, bit for bit.
+  double left, right;
+  if (sc == 0.0) {
+    left = right = 0.0;
+  } else if (sc <= 1.7976931348623157e308) {
+    const bool a_top = a.scale == sc;
+    const double q = (a_top ? b.scale : a.scale) / sc;
+    left = a_top ? 1.0 : q;
+    right = a_top ? q : 1.0;
+  } else {
+    left = a.scale / sc;
+    right = b.scale / sc;
+  }
   double delta = b.mean * right - a.mean * left;
   double n = a.count + b.count;
-  double wb = b.count / n;
+  double wb;
+  if (wave_all(a.count == b.count))
+    wb = 0.5;
+  else
+    wb = b.count / n;
   Moments r;
   r.mean = a.mean * left + delta * wb;
   r.m2 = a.m2 * left * left + b.m2 * right * right + delta * delta * (a.count * wb);
@@ -326,13 +359,16 @@ __device__ inline void block_merge(Moments (&m)[N], Moments* smem) {
   const int lane = threadIdx.x & (WAVE - 1);
   const int wave = threadIdx.x / WAVE;
   const int waves = blockDim.x / WAVE;
+#pragma unroll
   for (int i = 0; i < N; ++i) m[i] = wave_merge(m[i]);
   if (lane == 0)
+#pragma unroll
     for (int i = 0; i < N; ++i) smem[i * waves + wave] = m[i];
   __syncthreads();
   if (wave == 0 && waves > 1) {
     int lanes = 1;
     while (lanes < waves) lanes <<= 1;
+#pragma unroll
     for (int i = 0; i < N; ++i) {
       Moments acc = lane < waves ? smem[i * waves + lane] : moments_zero();
       acc = wave_merge(acc, lanes);

@@ -8,8 +8,8 @@
 //   * an inverse normal of the form x * P(w), w = -ln(4 p (1 - p)), after
 //     M. Giles (GPU Computing Gems Jade Edition, 2011), with coefficients
 //     fitted by kiyosi_amd/csrc/fit_inverse_normal.py (degree 7 central,
-//     degree 8 tail; max abs error 7.3e-7 in the central region and 2.3e-6
-//     over all 32-bit inputs, including both tails).
+//     degree 8 tail; max abs error 1.2e-6 over all 32-bit inputs, including
+//     both tails).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -51,20 +51,21 @@ __device__ inline float expm1(float x) {
 // one conversion and one fma carry both. With l = log2(4 |q| (1 - |q|)) <= 0,
 // the central polynomial runs in l itself (its constant term is sqrt(pi/2),
 // the slope of the inverse normal at p = 1/2) and the tail one
-// (w = -ln2 l >= 5) in t = sqrt(ln2) sqrt(-l) - 3. The tail is a
-// wave-uniform branch taken only when some lane of the wave needs it (about
-// one draw in 290), so the common path carries no exec-mask bookkeeping.
+// (w = -ln2 l >= 6) in t = sqrt(ln2) sqrt(-l) - 3. The tail is a
+// wave-uniform branch taken only when some lane of the wave needs it (one
+// draw in 800, so about one wave step in 13), and the common path carries no
+// exec-mask bookkeeping.
 // `sign` (+1 or -1) multiplies the result through q, which costs nothing
 // once it is a constant.
 __device__ inline float normal_from_bits(uint32_t bits, float sign = 1.0f) {
-  constexpr float CENTRAL[8] = {-3.723791053e-08f, -1.437705009e-06f, -1.994798913e-05f,
-                                -6.944068446e-05f, 1.055057975e-03f,  7.811283227e-03f,
-                                -2.274445444e-01f, 1.253313541e+00f};
-  constexpr float TAIL[9] = {1.557544238e-06f,  -2.612999815e-04f, 1.683062175e-03f,
-                             -4.751312081e-03f, 8.147120476e-03f,  -1.092399564e-02f,
-                             1.335252635e-02f,  1.416593313e+00f,  4.006433964e+00f};
+  constexpr float CENTRAL[8] = {-3.670217907e-08f, -1.419892442e-06f, -1.972815153e-05f,
+                                -6.814792869e-05f, 1.058840542e-03f,  7.816414349e-03f,
+                                -2.274419963e-01f, 1.253313780e+00f};
+  constexpr float TAIL[9] = {5.152326776e-05f,  -4.903596127e-04f, 2.017864492e-03f,
+                             -4.844579846e-03f, 7.992940024e-03f,  -1.083856169e-02f,
+                             1.336772740e-02f,  1.416583896e+00f,  4.006433964e+00f};
   constexpr float SQRT_LN2 = 0.832554611157697756f;
-  constexpr float TAIL_L = -7.21347520444482f;  // log2 value where w = 5
+  constexpr float TAIL_L = -8.65617024533378f;  // log2 value where w = 6
   const float q = fmaf((float)(int32_t)bits, -0x1p-32f * sign, -0x1p-33f * sign);
   const float ps = fabsf(q);                       // min(p, 1 - p) in (0, 1/2]
   const float ax = fmaf(-2.0f, ps, 1.0f);          // |2p - 1|

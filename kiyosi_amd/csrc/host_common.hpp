@@ -77,6 +77,9 @@ struct kg_context {
   std::mutex mu;
   std::string error;
   kg::DeviceBuffer partials, scratch, flags, results;
+  // Bridge notes in two passes: per-leg values, the fill queue (count first)
+  // and the sample blocks' Sobol base points.
+  kg::DeviceBuffer bridge_values, bridge_queue, bridge_base;
   kg::SobolSetup sobol;
   std::vector<uint32_t> joe_kuo;  // direction-number records, loaded once
   // Book engine: device buffers and the autotuned tile shape per precision.

@@ -25,6 +25,49 @@ inline size_t sobol_lds_bytes(int dims, bool fp32) {
   return ((size_t)dims * 32 + 16) * (fp32 ? sizeof(uint32_t) : sizeof(uint64_t));
 }
 
+// Dimension d of the point at index `first`, a multiple of 256: shift_d
+// XOR the directions of the set bits of gray(first).
+// The set bits are taken four at a time, so that their loads are in flight
+// together rather than one after another.
+__device__ inline uint64_t sobol_block_base(const uint64_t* __restrict__ dirs,
+                                            const uint64_t* __restrict__ shifts, int dims, uint64_t first,
+                                            int d) {
+  uint64_t base = shifts[d];
+  uint64_t g = first ^ (first >> 1);
+  while (g) {
+    const uint64_t g1 = g & (g - 1), g2 = g1 & (g1 - 1), g3 = g2 & (g2 - 1);
+    const uint64_t a = dirs[(uint64_t)__builtin_ctzll(g) * dims + d];
+    const uint64_t b = g1 ? dirs[(uint64_t)__builtin_ctzll(g1) * dims + d] : 0;
+    const uint64_t c = g2 ? dirs[(uint64_t)__builtin_ctzll(g2) * dims + d] : 0;
+    const uint64_t e = g3 ? dirs[(uint64_t)__builtin_ctzll(g3) * dims + d] : 0;
+    base ^= a ^ b ^ c ^ e;
+    g = g3 & (g3 - 1);
+  }
+  return base;
+}
+
+// Row d of both tables: lo[h] = XOR of directions 0..3 selected by h,
+// hi[h] = base XOR directions 4..7 selected by h; stored >> SHIFT.
+template <typename W, int SHIFT>
+__device__ inline void sobol_table_rows(const uint64_t* __restrict__ dirs, int dims, int d, uint64_t base,
+                                        W* lo, W* hi) {
+  uint64_t v[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = dirs[(uint64_t)k * dims + d];
+#pragma unroll
+  for (int h = 0; h < 16; ++h) {
+    uint64_t a = 0, b = base;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (h & (1 << k)) {
+        a ^= v[k];
+        b ^= v[4 + k];
+      }
+    lo[h] = (W)(a >> SHIFT);
+    hi[h] = (W)(b >> SHIFT);
+  }
+}
+
 template <typename W, int SHIFT>
 __device__ inline void sobol_build_lds(const uint64_t* __restrict__ dirs,
                                        const uint64_t* __restrict__ shifts, int dims,
@@ -32,30 +75,9 @@ __device__ inline void sobol_build_lds(const uint64_t* __restrict__ dirs,
                                        const void** hi_out) {
   W* lo = (W*)lds;
   W* hi = lo + (size_t)dims * 16;
-  const uint64_t g0 = first ^ (first >> 1);
-  for (int d = threadIdx.x; d < dims; d += blockDim.x) {
-    uint64_t base = shifts[d];
-    uint64_t g = g0;
-    while (g) {
-      base ^= dirs[(uint64_t)__builtin_ctzll(g) * dims + d];
-      g &= g - 1;
-    }
-    uint64_t v[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = dirs[(uint64_t)k * dims + d];
-#pragma unroll
-    for (int h = 0; h < 16; ++h) {
-      uint64_t a = 0, b = base;
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (h & (1 << k)) {
-          a ^= v[k];
-          b ^= v[4 + k];
-        }
-      lo[d * 16 + h] = (W)(a >> SHIFT);
-      hi[d * 16 + h] = (W)(b >> SHIFT);
-    }
-  }
+  for (int d = threadIdx.x; d < dims; d += blockDim.x)
+    sobol_table_rows<W, SHIFT>(dirs, dims, d, sobol_block_base(dirs, shifts, dims, first, d), lo + d * 16,
+                               hi + d * 16);
   __syncthreads();
   *lo_out = lo;
   *hi_out = hi;

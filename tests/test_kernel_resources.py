@@ -46,9 +46,15 @@ def test_book_kernels_keep_two_waves_per_simd(tmp_path):
 
 
 @pytest.mark.skipif(not pathlib.Path(HIPCC).exists(), reason="hipcc is not installed")
-def test_snowball_sobol_fp32_kernel_has_no_scratch(tmp_path):
+def test_snowball_sobol_fp32_kernels_have_no_scratch(tmp_path):
+    """The snowball's FP32 Sobol kernels: the two bridge passes and the
+    single-pass kernel of plans without a bridge."""
     res = kernel_resources("autocall.hip", tmp_path)
-    name = next(k for k in res if k.startswith("_ZN2kg15autocall_kernelILb1ELb1ELb1E"))
-    vgpr, private = res[name]
-    assert private == 0
-    assert vgpr <= 128, vgpr  # four or more waves per SIMD
+    prefixes = ("_ZN2kg15autocall_kernelILb1ELb1ELb1E", "_ZN2kg20bridge_events_kernelILb1ELb1ELb1E",
+                "_ZN2kg18bridge_fill_kernelILb1ELb1ELb1E")
+    for prefix in prefixes:
+        name = next(k for k in res if k.startswith(prefix))
+        vgpr, private = res[name]
+        assert private == 0, name
+        assert vgpr <= 128, (name, vgpr)  # four or more waves per SIMD
+    assert all(private == 0 for name, (_, private) in res.items() if "bridge" in name)
