@@ -123,55 +123,38 @@ __global__ void __launch_bounds__(256)
 
 // Bridge plans in two passes (bridge_events, bridge_knock_in): the daily
 // fill runs only for the legs that reach expiry still monitoring the
-// knock-in, gathered per sample block. The launches:
-//   bridge_setup_kernel   - the interval rows (interval_row) and the Sobol
-//                           table rows of the intervals' first dimensions
-//                           without a base point, once; and (Sobol) every
-//                           sample block's base point, one word per
-//                           dimension;
-//   bridge_events_kernel  - pass one per sample and leg; writes each leg's
-//                           value (the final value, or the cash before the
-//                           expiry payoff of a leg that waits on its fill)
-//                           and the waiting legs (sample << 1 | leg) of each
-//                           wave into the wave's own queue region of 128
-//                           items, with their count;
-//   bridge_fill_kernel    - pass two, one wave per sample block over its
-//                           four regions; adds the expiry payoff;
-//   bridge_moments_kernel - the per-block moments of the leg values, merged
-//                           exactly as autocall_kernel merges them.
+// knock-in, gathered per sample block. The host builds the interval rows
+// (interval_row's fields) and the Sobol table rows of the intervals' first
+// dimensions without a base point. The launches:
+//   sobol_base_rows_kernel - (Sobol) every sample block's base point, one
+//                            word per dimension; kept across calls with the
+//                            same dimensions, seed and block count;
+//   bridge_events_kernel   - pass one per sample and leg; writes each leg's
+//                            value (the final value, or the cash before the
+//                            expiry payoff of a leg that waits on its fill)
+//                            and the waiting legs (sample << 1 | leg) of each
+//                            wave into the wave's own queue region of 128
+//                            items, with their count;
+//   bridge_fill_kernel     - one wave per sample block: pass two over the
+//                            block's four regions (adding the expiry
+//                            payoff), then the block's moments, merged
+//                            exactly as autocall_kernel merges them.
 template <bool FP32>
 using SobolWord = typename std::conditional<FP32, uint32_t, uint64_t>::type;
 constexpr int FILL_THREADS = 256;
 constexpr int FILL_WAVES = FILL_THREADS / 64;
 constexpr int REGION = 128;  // queue items per pass-one wave: 64 samples, two legs
 
-template <bool SOBOL, bool FP32, bool LDS>
+template <bool FP32>
 __global__ void __launch_bounds__(256)
-    bridge_setup_kernel(AcPlanDev p, const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shifts,
-                        int64_t blocks, void* __restrict__ rows_out, void* __restrict__ table_out,
-                        SobolWord<FP32>* __restrict__ base_rows) {
-  using R = typename std::conditional<FP32, float, double>::type;
-  using W = SobolWord<FP32>;
+    sobol_base_rows_kernel(const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shifts, int dims,
+                           int64_t blocks, SobolWord<FP32>* __restrict__ rows) {
   constexpr int SHIFT = FP32 ? 20 : 0;
-  const int dims = p.n_steps;
-  if (blockIdx.x == 0) {
-    IntervalRow<R>* rows = (IntervalRow<R>*)rows_out;
-    for (int k = threadIdx.x; k < p.n_intervals; k += 256) {
-      rows[k] = interval_row<R>(p, k);
-      if (SOBOL && LDS) {
-        W* lo = (W*)table_out;
-        sobol_table_rows<W, SHIFT>(dirs, dims, rows[k].first, 0ull, lo + k * 16,
-                                   lo + ((size_t)p.n_intervals + k) * 16);
-      }
-    }
-    return;
-  }
-  if (!(SOBOL && LDS)) return;
-  const int64_t i = (int64_t)(blockIdx.x - 1) * 256 + threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= blocks * dims) return;
   const int64_t b = i / dims;
   const int d = (int)(i - b * dims);
-  base_rows[i] = (W)(sobol_block_base(dirs, shifts, dims, (uint64_t)b * BLOCK_SAMPLES, d) >> SHIFT);
+  rows[i] = (SobolWord<FP32>)(sobol_block_base(dirs, shifts, dims, (uint64_t)b * BLOCK_SAMPLES, d) >> SHIFT);
 }
 
 template <bool SOBOL, bool FP32, bool LDS>
@@ -191,9 +174,10 @@ __global__ void __launch_bounds__(256)
   const int dims = p.n_steps;
   // LDS: the interval rows, then (Sobol) the table rows of the intervals'
   // first dimensions (row k holds the first dimension of interval k) with
-  // this block's base point folded into `hi`. All reads are independent.
+  // this block's base point folded into `hi`.
   IntervalRow<R>* rows = (IntervalRow<R>*)lds;
-  for (int k = j; k < p.n_intervals; k += BLOCK_SAMPLES) rows[k] = ((const IntervalRow<R>*)rows_in)[k];
+  const IntervalRow<R>* rows_g = (const IntervalRow<R>*)rows_in;
+  for (int k = j; k < p.n_intervals; k += BLOCK_SAMPLES) rows[k] = rows_g[k];
   Draws<SOBOL, FP32, LDS ? 0 : 1> dr;
   dr.dims = dims;
   dr.direct = !LDS;
@@ -207,7 +191,7 @@ __global__ void __launch_bounds__(256)
     for (int e = j; e < p.n_intervals * 32; e += BLOCK_SAMPLES) {
       const int k = e >> 4;  // lo rows, then hi rows
       W v = table[e];
-      if (k >= p.n_intervals) v ^= base[((const IntervalRow<R>*)rows_in)[k - p.n_intervals].first];
+      if (k >= p.n_intervals) v ^= base[rows_g[k - p.n_intervals].first];
       lo[e] = v;
     }
     dr.lo = lo;
@@ -230,7 +214,7 @@ __global__ void __launch_bounds__(256)
   if (live) values[n] = want_up ? up.cash : bridge_finish(p, up, false, true, bad);
   if (two_legs && paired) values[samples + n] = want_down ? down.cash : bridge_finish(p, down, false, true, bad);
   // This wave's region: its leg-0 items in sample order, then its leg-1
-  // items; the count packs both numbers.
+  // items; the count covers both.
   const uint64_t m_up = __ballot(want_up), m_down = __ballot(want_down);
   const int lane = j & 63;
   const uint64_t lower = (1ull << lane) - 1ull;
@@ -241,17 +225,38 @@ __global__ void __launch_bounds__(256)
   if (bad) raise_flag(flags, FLAG_NONFINITE);
 }
 
+// One butterfly level of wave_merge on four sets at once (lane l holds the
+// values of lanes l of four waves), so the four merge chains interleave.
+__device__ inline void wave_merge4(Moments (&m)[4], bool full) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  for (int mask = 1; mask < WAVE; mask <<= 1) {
+    Moments o[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) o[w] = moments_shfl_xor(m[w], mask);
+    if (full) {
+#pragma unroll
+      for (int w = 0; w < 4; ++w) m[w] = (lane & mask) ? moments_merge_full(o[w], m[w]) : moments_merge_full(m[w], o[w]);
+    } else {
+#pragma unroll
+      for (int w = 0; w < 4; ++w) m[w] = (lane & mask) ? moments_merge(o[w], m[w]) : moments_merge(m[w], o[w]);
+    }
+  }
+}
+
 // Pass two over the queue, one wave per sample block: the wave copies its
 // block's base row into LDS once (so every lane's point is two table reads
 // and one broadcast read of the row, XORed) and fills the block's queued
-// legs 64 at a time.
+// legs 64 at a time; then it forms the block's moments from the 256 leg
+// values as block_merge over 256 threads forms them: lane l holds samples
+// l, 64 + l, 128 + l and 192 + l, merges each set across the wave
+// (wave_merge), then the four results ((0, 1), (2, 3)).
 template <bool SOBOL, bool FP32, bool LDS>
 __global__ void __launch_bounds__(FILL_THREADS)
-    bridge_fill_kernel(AcPlanDev p, uint64_t seed, int64_t samples, int64_t blocks,
+    bridge_fill_kernel(AcPlanDev p, uint64_t seed, int64_t samples, int64_t paths, int antithetic, int64_t blocks,
                        const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shifts,
                        const void* __restrict__ base_rows, const uint32_t* __restrict__ queue,
                        const uint32_t* __restrict__ counts, double* __restrict__ values,
-                       uint32_t* __restrict__ flags) {
+                       Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
   using R = typename std::conditional<FP32, float, double>::type;
   using W = SobolWord<FP32>;
   constexpr int SHIFT = FP32 ? 20 : 0;
@@ -305,38 +310,114 @@ __global__ void __launch_bounds__(FILL_THREADS)
     value = cash;
   }
   if (bad) raise_flag(flags, FLAG_NONFINITE);
-}
-
-__global__ void __launch_bounds__(256)
-    bridge_moments_kernel(const double* __restrict__ values, int64_t samples, int64_t paths, int antithetic,
-                          Moments* __restrict__ partials) {
-  __shared__ Moments smem[2 * 4];
-  const int64_t block = blockIdx.x;
-  const int64_t n = block * BLOCK_SAMPLES + threadIdx.x;
-  const bool live = n < samples;
-  const bool paired = antithetic && live && n < paths / 2;
-  Moments m[2] = {moments_zero(), moments_zero()};
-  if (live) {
-    const double pos = values[n];
-    moments_add(m[0], pos, true);
-    if (paired) {
-      const double neg = values[samples + n];
-      moments_add(m[0], neg, true);
-      moments_add(m[1], midpoint(pos, neg), true);
+  // The lanes' stores above are read back by other lanes below.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  // Set 0 (every leg value), then set 1 (the pair midpoints): one set's
+  // registers at a time.
+  Moments m[4];
+  bool full = true;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const int64_t n = block * BLOCK_SAMPLES + w * 64 + lane;
+    const bool live = n < samples;
+    full = full && live;
+    m[w] = moments_zero();
+    if (live) {
+      moments_add(m[w], values[n], true);
+      if (antithetic && n < paths / 2) moments_add(m[w], values[samples + n], true);
     }
   }
-  // Without antithetic pairs the pair set is empty everywhere, and merging
-  // empty moments gives empty moments: only the first set needs the tree.
-  if (antithetic) {
-    block_merge<2>(m, smem);
-  } else {
-    Moments one[1] = {m[0]};
-    block_merge<1>(one, smem);
-    m[0] = one[0];
+  // Without an empty set anywhere in the block the merges need no empty
+  // checks (moments_merge_full).
+  wave_merge4(m, wave_all(full));
+  if (lane == 0) partials[block * 2] = moments_merge(moments_merge(m[0], m[1]), moments_merge(m[2], m[3]));
+  // Merging empty sets gives empty sets: without antithetic pairs set 1 is
+  // empty.
+  full = true;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const int64_t n = block * BLOCK_SAMPLES + w * 64 + lane;
+    const bool paired = antithetic && n < samples && n < paths / 2;
+    full = full && paired;
+    m[w] = moments_zero();
+    if (paired) moments_add(m[w], midpoint(values[n], values[samples + n]), true);
   }
-  if (threadIdx.x == 0) {
-    partials[block * 2 + 0] = m[0];
-    partials[block * 2 + 1] = m[1];
+  if (antithetic) wave_merge4(m, wave_all(full));
+  if (lane == 0) partials[block * 2 + 1] = moments_merge(moments_merge(m[0], m[1]), moments_merge(m[2], m[3]));
+}
+
+// The cached base rows of `ctx` for (dims, seed, blocks, precision), built
+// on a miss.
+template <bool FP32>
+static const SobolWord<FP32>* bridge_base_rows(kg_context* ctx, int dims, uint64_t seed, int64_t blocks,
+                                               const uint64_t* dirs, const uint64_t* shifts) {
+  BridgeBaseKey& key = ctx->bridge_base_key;
+  if (key.valid && key.dims == dims && key.seed == seed && key.blocks == blocks && key.fp32 == FP32)
+    return (const SobolWord<FP32>*)ctx->bridge_base.ptr;
+  key.valid = false;
+  // One word per dimension per sample block, plus one: the fill kernel
+  // copies dims + 1 words of a row.
+  SobolWord<FP32>* base =
+      (SobolWord<FP32>*)ctx->bridge_base.reserve(sizeof(SobolWord<FP32>) * ((size_t)blocks * dims + 1));
+  const int64_t cells = blocks * dims;
+  hipLaunchKernelGGL(sobol_base_rows_kernel<FP32>, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, ctx->stream,
+                     dirs, shifts, dims, blocks, base);
+  KG_HIP(hipGetLastError());
+  key = BridgeBaseKey{dims, seed, blocks, FP32, true};
+  return base;
+}
+
+// Interval rows (and, for LDS Sobol, the first-dimension table rows) of a
+// bridge plan, built on the host.
+template <bool SOBOL, bool FP32, bool LDS>
+static void bridge_rows(kg_context* ctx, const AcHostPlan& h, int dims, uint64_t seed,
+                        std::vector<unsigned char>& blob, size_t* table_offset) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  using W = SobolWord<FP32>;
+  const size_t n = h.intervals.size();
+  std::vector<IntervalRow<R>> rows(n);
+  for (size_t k = 0; k < n; ++k) {
+    IntervalRow<R>& r = rows[k];
+    r = IntervalRow<R>{};
+    r.first = k == 0 ? 0 : h.intervals[k - 1].end + 1;
+    r.end = h.intervals[k].end;
+    r.drift = FP32 ? (R)h.intervals32[2 * k] : (R)h.intervals[k].drift;
+    r.diffusion = FP32 ? (R)h.intervals32[2 * k + 1] : (R)h.intervals[k].diffusion;
+    r.observe = h.step_observe[r.end];
+    r.event = h.step_event[r.end];
+    if (r.event >= 0) {
+      const AcEventD& ev = h.events[r.event];
+      r.pay_coupon = ev.pay_coupon;
+      r.pay_out = ev.pay_out;
+      r.pay_both = ev.pay_both;
+      r.knock_out = FP32 ? (R)h.events32[2 * r.event] : (R)ev.knock_out;
+      r.coupon_barrier = FP32 ? (R)h.events32[2 * r.event + 1] : (R)ev.coupon_barrier;
+      r.has_knock_out = ev.has_knock_out;
+    }
+  }
+  blob.assign((const unsigned char*)rows.data(), (const unsigned char*)(rows.data() + n));
+  *table_offset = (blob.size() + 255) & ~(size_t)255;
+  if (SOBOL && LDS) {
+    constexpr int SHIFT = FP32 ? 20 : 0;
+    const SobolSetup& so = sobol_setup(ctx, (uint32_t)dims, seed);
+    std::vector<W> table(n * 32);
+    for (size_t k = 0; k < n; ++k) {
+      const int d = rows[k].first;
+      for (int hix = 0; hix < 16; ++hix) {
+        uint64_t a = 0, b = 0;
+        for (int bit = 0; bit < 4; ++bit)
+          if (hix & (1 << bit)) {
+            a ^= so.directions[(size_t)bit * dims + d];
+            b ^= so.directions[(size_t)(4 + bit) * dims + d];
+          }
+        table[k * 16 + hix] = (W)(a >> SHIFT);
+        table[(n + k) * 16 + hix] = (W)(b >> SHIFT);
+      }
+    }
+    blob.resize(*table_offset);
+    blob.insert(blob.end(), (const unsigned char*)table.data(), (const unsigned char*)(table.data() + table.size()));
   }
 }
 
@@ -399,13 +480,13 @@ static int cu_count(kg_context* ctx) {
   return std::max(cus, 1);
 }
 
-// The two-pass launches of a bridge plan (bridge_setup_kernel,
-// bridge_events_kernel, bridge_fill_kernel, bridge_moments_kernel).
+// The two-pass launches of a bridge plan (see bridge_events_kernel).
 template <bool SOBOL, bool FP32, bool LDS>
-static void launch_bridge(kg_context* ctx, const AcPlanDev& p, int dims, const Sampler& s, const uint64_t* dirs,
-                          const uint64_t* shifts, Moments* parts, uint32_t* flags) {
+static void launch_bridge(kg_context* ctx, const AcHostPlan& h, const AcPlanDev& p, const Sampler& s,
+                          const uint64_t* dirs, const uint64_t* shifts, Moments* parts, uint32_t* flags) {
   using R = typename std::conditional<FP32, float, double>::type;
   using W = SobolWord<FP32>;
+  const int dims = p.n_steps;
   const int64_t samples = s.samples(), blocks = s.blocks();
   const int legs = s.antithetic ? 2 : 1;
   const size_t regions = (size_t)blocks * FILL_WAVES;
@@ -413,41 +494,37 @@ static void launch_bridge(kg_context* ctx, const AcPlanDev& p, int dims, const S
   // Per pass-one wave: a region of REGION items and its count.
   uint32_t* queue = (uint32_t*)ctx->bridge_queue.reserve(sizeof(uint32_t) * regions * (REGION + 1));
   uint32_t* counts = queue + regions * REGION;
-  // The interval rows and the first-dimension table rows (lo, then hi), then
-  // (Sobol) one base word per dimension per sample block plus one: the fill
-  // kernel copies dims + 1 words of a row.
-  const size_t rows_bytes = ((size_t)p.n_intervals * sizeof(IntervalRow<R>) + 255) & ~(size_t)255;
-  const size_t table_bytes = ((size_t)p.n_intervals * 32 * sizeof(W) + 255) & ~(size_t)255;
-  const size_t base_words = SOBOL && LDS ? (size_t)blocks * dims + 1 : 0;
-  unsigned char* setup =
-      (unsigned char*)ctx->bridge_base.reserve(rows_bytes + table_bytes + sizeof(W) * std::max<size_t>(base_words, 1));
-  void* rows = setup;
-  void* table = setup + rows_bytes;
-  W* base = (W*)(setup + rows_bytes + table_bytes);
+  std::vector<unsigned char> blob;
+  size_t table_offset = 0;
+  bridge_rows<SOBOL, FP32, LDS>(ctx, h, dims, s.seed, blob, &table_offset);
+  unsigned char* d_rows = (unsigned char*)ctx->bridge_rows.reserve(std::max<size_t>(blob.size(), 1));
+  upload_async(ctx, d_rows, blob.data(), blob.size());
+  const W* base = SOBOL && LDS ? bridge_base_rows<FP32>(ctx, dims, s.seed, blocks, dirs, shifts) : nullptr;
   size_t events_lds = (size_t)p.n_intervals * sizeof(IntervalRow<R>), fill_lds = 0;
-  int64_t setup_blocks = 1;
   if (SOBOL && LDS) {
     events_lds += (size_t)p.n_intervals * 32 * sizeof(W);
     fill_lds = sobol_lds_bytes(dims, FP32) + (size_t)FILL_WAVES * (dims + 1) * sizeof(W);
-    setup_blocks += (blocks * dims + 255) / 256;
   }
-  hipLaunchKernelGGL((bridge_setup_kernel<SOBOL, FP32, LDS>), dim3((unsigned)setup_blocks), dim3(256), 0, ctx->stream,
-                     p, dirs, shifts, blocks, rows, table, base);
-  KG_HIP(hipGetLastError());
   hipLaunchKernelGGL((bridge_events_kernel<SOBOL, FP32, LDS>), dim3(blocks), dim3(256), events_lds, ctx->stream, p,
-                     s.seed, samples, s.path_count, (int)s.antithetic, dirs, shifts, values, rows, table, base, queue,
-                     counts, flags);
+                     s.seed, samples, s.path_count, (int)s.antithetic, dirs, shifts, values, d_rows,
+                     d_rows + table_offset, base, queue, counts, flags);
   KG_HIP(hipGetLastError());
   if (fill_lds > 64 * 1024)
     KG_HIP(hipFuncSetAttribute((const void*)bridge_fill_kernel<SOBOL, FP32, LDS>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)fill_lds));
   hipLaunchKernelGGL((bridge_fill_kernel<SOBOL, FP32, LDS>), dim3((unsigned)((blocks + FILL_WAVES - 1) / FILL_WAVES)),
-                     dim3(FILL_THREADS), fill_lds, ctx->stream, p, s.seed, samples, blocks, dirs, shifts, base, queue,
-                     counts, values, flags);
+                     dim3(FILL_THREADS), fill_lds, ctx->stream, p, s.seed, samples, s.path_count, (int)s.antithetic,
+                     blocks, dirs, shifts, base, queue, counts, values, parts, flags);
   KG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(bridge_moments_kernel, dim3(blocks), dim3(256), 0, ctx->stream, values, samples, s.path_count,
-                     (int)s.antithetic, parts);
-  KG_HIP(hipGetLastError());
+}
+
+template <bool SOBOL, bool FP32>
+static void launch_bridge_any(kg_context* ctx, const AcHostPlan& h, const AcPlanDev& p, const Sampler& s,
+                              const uint64_t* dirs, const uint64_t* shifts, Moments* parts, uint32_t* flags) {
+  if (SOBOL && sobol_lds_bytes(p.n_steps, FP32) <= SOBOL_LDS_LIMIT)
+    launch_bridge<SOBOL, FP32, true>(ctx, h, p, s, dirs, shifts, parts, flags);
+  else
+    launch_bridge<SOBOL, FP32, false>(ctx, h, p, s, dirs, shifts, parts, flags);
 }
 
 template <bool SOBOL, bool FP32, int L, class Plan>
@@ -459,16 +536,6 @@ static void launch(kg_context* ctx, const Plan& p, int dims, const Sampler& s, c
     lds = sobol_lds_bytes(dims, FP32);
     sobol_lds = lds <= SOBOL_LDS_LIMIT;
     if (!sobol_lds) lds = 0;
-  }
-  if constexpr (std::is_same<Plan, AcPlanDev>::value) {
-    // Queue items carry the sample index in 31 bits.
-    if (p.bridge && s.samples() < ((int64_t)1 << 31)) {
-      if (sobol_lds)
-        launch_bridge<SOBOL, FP32, true>(ctx, p, dims, s, dirs, shifts, parts, flags);
-      else
-        launch_bridge<SOBOL, FP32, false>(ctx, p, dims, s, dirs, shifts, parts, flags);
-      return;
-    }
   }
   using Kernel = void (*)(Plan, uint64_t, int64_t, int64_t, int, const uint64_t*, const uint64_t*, Moments*,
                           uint32_t*);
@@ -589,9 +656,28 @@ static void price_note(kg_context* ctx, const Sampler& s, AcHostPlan& h, kg_esti
   const int64_t blocks = s.blocks();
   Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 2 * blocks);
   uint32_t* flags = reset_flags(ctx);
-  if (s.sobol) sobol_setup(ctx, (uint32_t)p.n_steps, s.seed);
+  const uint64_t* dirs = nullptr;
+  const uint64_t* shifts = nullptr;
+  if (s.sobol) {
+    const SobolSetup& so = sobol_setup(ctx, (uint32_t)p.n_steps, s.seed);
+    dirs = (const uint64_t*)so.d_directions.ptr;
+    shifts = (const uint64_t*)so.d_shifts.ptr;
+  }
   timing_begin(ctx);
-  launch_any(ctx, p, p.n_steps, s, parts, flags);
+  // Bridge plans run in two passes (queue items carry the sample index in
+  // 31 bits); other plans, and larger ones, in one.
+  if (p.bridge && s.samples() < ((int64_t)1 << 31)) {
+    if (s.sobol && s.fp32)
+      launch_bridge_any<true, true>(ctx, h, p, s, dirs, shifts, parts, flags);
+    else if (s.sobol)
+      launch_bridge_any<true, false>(ctx, h, p, s, dirs, shifts, parts, flags);
+    else if (s.fp32)
+      launch_bridge_any<false, true>(ctx, h, p, s, dirs, shifts, parts, flags);
+    else
+      launch_bridge_any<false, false>(ctx, h, p, s, dirs, shifts, parts, flags);
+  } else {
+    launch_any(ctx, p, p.n_steps, s, parts, flags);
+  }
   Moments r[2];
   *out = kg_estimate{};
   finish_sampling(ctx, parts, blocks, 2, r, &out->device_ms);

@@ -387,11 +387,12 @@ struct BridgeEvents {
   bool need_fill;  // the expiry payoff waits on the daily fill
 };
 
-// What pass one reads of interval k, gathered per block in LDS so that the
-// interval loop waits on no global load: the end point's drift and
-// diffusion, its observation flag and event (knock-out and coupon levels,
-// discounted payments; the rare non-normal payments fall back to the event
-// in global memory).
+// What pass one reads of interval k (built on the host from the plan's
+// tables and gathered per block in LDS, so that the interval loop waits on
+// no global load): the end point's drift and diffusion (the plan's interval
+// row), its observation flag and event (knock-out and coupon levels as the
+// plan's precision holds them, discounted payments; the rare non-normal
+// payments fall back to the event in global memory).
 template <typename R>
 struct IntervalRow {
   double pay_coupon, pay_out, pay_both;
@@ -399,33 +400,7 @@ struct IntervalRow {
   int32_t first, end, event, observe, has_knock_out, pad;  // first: the interval's first dimension
 };
 
-template <typename R>
-__device__ inline IntervalRow<R> interval_row(const AcPlanDev& p, int k) {
-  IntervalRow<R> r;
-  r.first = k == 0 ? 0 : p.intervals[k - 1].end + 1;
-  r.pad = 0;
-  r.end = p.intervals[k].end;
-  r.drift = Tab<R>::idrift(p, k);
-  r.diffusion = Tab<R>::idiff(p, k);
-  r.observe = p.step_observe[r.end];
-  r.event = p.step_event[r.end];
-  if (r.event >= 0) {
-    const AcEventD& ev = p.events[r.event];
-    r.pay_coupon = ev.pay_coupon;
-    r.pay_out = ev.pay_out;
-    r.pay_both = ev.pay_both;
-    r.knock_out = Tab<R>::ko(p, r.event);
-    r.coupon_barrier = Tab<R>::cb(p, r.event);
-    r.has_knock_out = ev.has_knock_out;
-  } else {
-    r.pay_coupon = r.pay_out = r.pay_both = 0.0;
-    r.knock_out = r.coupon_barrier = 0;
-    r.has_knock_out = 0;
-  }
-  return r;
-}
-
-// Pass one over the rows of `rows` (interval_row of every interval). ROWS:
+// Pass one over the rows of `rows` (one IntervalRow per interval). ROWS:
 // the end point of interval k is dimension k of `dr` (tables that hold the
 // intervals' first dimensions only); otherwise dimension `start` as in
 // bridge_path. The arithmetic is bridge_path's, step for step.

@@ -314,6 +314,30 @@ __device__ inline Moments moments_merge(const Moments& a, const Moments& b) {
   return r;
 }
 
+// moments_merge for operands that both hold values (count > 0): the same
+// arithmetic, bit for bit, without the empty-operand branches, so that
+// independent merges can interleave.
+__device__ inline Moments moments_merge_full(const Moments& a, const Moments& b) {
+#pragma clang fp contract(off)
+  const double sc = fmax(a.scale, b.scale);
+  // As in moments_merge: one quotient, of the operand below sc; x / x is 1
+  // for a finite nonzero sc and NaN for an infinite or NaN one.
+  const bool a_top = a.scale == sc;
+  const double q = (a_top ? b.scale : a.scale) / sc;
+  const double top = sc <= 1.7976931348623157e308 ? 1.0 : __builtin_nan("");
+  const double left = sc == 0.0 ? 0.0 : (a_top ? top : q);
+  const double right = sc == 0.0 ? 0.0 : (a_top ? q : top);
+  const double delta = b.mean * right - a.mean * left;
+  const double n = a.count + b.count;
+  const double wb = b.count / n;
+  Moments r;
+  r.mean = a.mean * left + delta * wb;
+  r.m2 = a.m2 * left * left + b.m2 * right * right + delta * delta * (a.count * wb);
+  r.count = n;
+  r.scale = sc;
+  return r;
+}
+
 __device__ inline Moments moments_shfl_xor(const Moments& m, int mask) {
   Moments r;
   r.count = __shfl_xor(m.count, mask, WAVE);

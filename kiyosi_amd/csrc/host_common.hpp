@@ -59,6 +59,14 @@ struct DeviceBuffer {
 
 // Host-built scrambled Sobol directions for (dims, seed), cached like the
 // reference's request cache (engines/monte_carlo/sobol.rs:12-69).
+struct BridgeBaseKey {
+  int dims = 0;
+  uint64_t seed = 0;
+  int64_t blocks = 0;
+  bool fp32 = false;
+  bool valid = false;
+};
+
 struct SobolSetup {
   uint32_t dims = 0;
   uint64_t seed = 0;
@@ -77,9 +85,11 @@ struct kg_context {
   std::mutex mu;
   std::string error;
   kg::DeviceBuffer partials, scratch, flags, results;
-  // Bridge notes in two passes: per-leg values, the fill queue (count first)
-  // and the sample blocks' Sobol base points.
-  kg::DeviceBuffer bridge_values, bridge_queue, bridge_base;
+  // Bridge notes in two passes: per-leg values, the fill queue, the plan's
+  // interval rows, and the sample blocks' Sobol base points (kept for the
+  // key's dimensions, seed, block count and precision).
+  kg::DeviceBuffer bridge_values, bridge_queue, bridge_rows, bridge_base;
+  kg::BridgeBaseKey bridge_base_key;
   kg::SobolSetup sobol;
   std::vector<uint32_t> joe_kuo;  // direction-number records, loaded once
   // Book engine: device buffers and the autotuned tile shape per precision.
