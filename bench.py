@@ -10,10 +10,12 @@ paths for the speed ratio and the price error, stated in units of the path
 standard error.
 
 Prints one JSON line. ``value`` is path-steps per second of the FP32-step
-engine; ``roofline`` compares the sampling kernel's algorithmic FLOP rate (9
-FLOP per path-step: Sobol inverse normal 3, bridge fill 4, knock-in compare
-1, event/terminal share 1) with the FP32 vector peak, using the kernel
-duration from HIP events on the engine's stream; ``cpu_baseline`` times the
+engine; ``roofline`` compares the sampling kernels' algorithmic FLOP rate (9
+FLOP per nominal path-step: Sobol inverse normal 3, bridge fill 4, knock-in
+compare 1, event/terminal share 1, counted over all 2^20 x 252 path-steps
+although the kernels fill only the paths whose payoff depends on it) with
+the FP32 vector peak, using the kernels' duration from HIP events on the
+engine's stream; ``cpu_baseline`` times the
 FP64 CPU restatement of the reference engine on the host cores. ``configs``
 adds the other workloads: the European call (config 1), the Heston Asian
 (config 3), the 4,096-contract book (config 4) and the local-volatility
@@ -33,10 +35,12 @@ sys.path.insert(0, str(ROOT))
 FLOP_PER_PATH_STEP = 9.0
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the config-2 FP32 sampling kernel: FETCH_SIZE x 2.0 +
-# WRITE_SIZE x 1.0 from the two counter passes of `bench.py --profile-pass`,
-# scaled on a known 256 MiB read/write (profiles/hbm_traffic_fetch_write_counters.md).
-C2_KERNEL_HBM_BYTES = 923_410
+# HBM bytes per config-2 FP32 pricing of its two sampling kernels
+# (bridge_events_kernel 13,438,765 + bridge_fill_kernel 16,685,122; the
+# Sobol base rows are kept across calls): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+# from the two counter passes of `bench.py --profile-pass`, scaled on a known
+# 256 MiB read/write (profiles/hbm_traffic_fetch_write_counters.md).
+C2_KERNEL_HBM_BYTES = 30_123_887
 
 
 def log(msg):
@@ -274,7 +278,9 @@ def main():
         "contracts_per_sec": args.steps / wall32,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": C2_KERNEL_HBM_BYTES,
-                     "kernel": "autocall_kernel<sobol, fp32>", "kernel_ms": dev32},
+                     "kernel": "bridge_events_kernel + bridge_fill_kernel <sobol, fp32>", "kernel_ms": dev32,
+                     "basis": "9 FLOP per nominal path-step (2^20 x 252), the reference algorithm's count; "
+                              "the kernels skip daily fills that cannot change the price"},
     }
     if not args.headline_only:
         configs = {}
