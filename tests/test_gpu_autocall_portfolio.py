@@ -47,8 +47,16 @@ SECOND = snowball(coupon=0.09, knock_in=90.0)
 EARLY = snowball(dates=[TradingCalendar.weekdays().add_trading_days(V, d) for d in (22, 43, 62)])
 OTHER_GRID = snowball(coupon=0.09, knock_in=90.0, at_expiry=True)
 
+# 30 monthly knock-outs over 630 trading days: the Sobol tables no longer fit
+# in LDS, so the scalar price takes the two-pass kernels' direct-point form
+# while the batch keeps one pass.
+LONG_DATES = [TradingCalendar.weekdays().add_trading_days(V, 21 * k) for k in range(1, 31)]
+LONG = snowball(dates=LONG_DATES, knock_in=80.0)
+LONG_SECOND = snowball(dates=LONG_DATES, coupon=0.11, knock_in=85.0)
+
 BATCHES = [
     ("same grid", [FIRST, SECOND, FIRST]),
+    ("long notes", [LONG, LONG_SECOND]),
     ("early notes", [FIRST] + [EARLY] * 32),
     ("mixed grids", [FIRST, OTHER_GRID, SECOND]),
     ("41 mixed", [[FIRST, OTHER_GRID, SECOND][i % 3] for i in range(41)]),
