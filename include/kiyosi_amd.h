@@ -415,6 +415,17 @@ typedef struct kg_asian_terms {
 kg_status kg_asian_price(kg_context* ctx, const kg_settings* settings, const kg_asian_terms* option,
                          const kg_bsm_market* market, kg_estimate* out);
 
+/* MonteCarloAsianEngine::price_with_greeks_with_settings (asian.rs:885-898
+ * through engines.rs:138-162): the price with the engine's native delta and
+ * gamma (greeks->request, KG_GREEK_* bits), the per-path conditional
+ * estimators of GeometricParameters::geometric (asian.rs:741-784) and
+ * ArithmeticParameters::arithmetic with its geometric control
+ * (asian.rs:1063-1098, 1233-1255). greeks->available holds the Greeks
+ * returned; at expiry (a known price) none are, and the caller's expiry
+ * stencil supplies them. The price equals kg_asian_price's bit for bit. */
+kg_status kg_asian_price_with_greeks(kg_context* ctx, const kg_settings* settings, const kg_asian_terms* option,
+                                     const kg_bsm_market* market, kg_spot_greeks* greeks, kg_estimate* out);
+
 /* Strike batch of an arithmetic average: MonteCarloAsianEngine::price_strikes
  * (engines/monte_carlo/asian.rs:197-345). prices[i] equals the price of
  * kg_asian_price for the option with strike strikes[i] and the same

@@ -52,6 +52,8 @@ SIGNATURES = {
                                    P(_abi.Estimate)]),
     "kg_digital_price_with_greeks": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.DigitalTerms),
                                                P(_abi.BsmMarket), P(_abi.SpotGreeks), P(_abi.Estimate)]),
+    "kg_asian_price_with_greeks": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AsianTerms),
+                                             P(_abi.BsmMarket), P(_abi.SpotGreeks), P(_abi.Estimate)]),
     "kg_asian_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AsianTerms), P(_abi.BsmMarket),
                                  P(_abi.Estimate)]),
     "kg_asian_price_strikes": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AsianTerms), P(_abi.BsmMarket),

@@ -246,8 +246,22 @@ __global__ void __launch_bounds__(256)
   const auto origin = dr;
   uint32_t bad = 0;
   auto run = [&](bool on, float sign, double* out) {
-    out[0] = it.kind == IK_DIGITAL ? digital_path<FP32, true>(it.u.dg, dr, on, sign, bad, out + 1)
-                                   : barrier_path<FP32, true, true>(it.u.b, dr, on, sign, bad, out + 1);
+    switch (it.kind) {
+      case IK_DIGITAL:
+        out[0] = digital_path<FP32, true>(it.u.dg, dr, on, sign, bad, out + 1);
+        break;
+      case IK_GEO_ASIAN:
+        out[0] = geo_asian<FP32, true>(it.u.g, dr, on, sign, bad, out + 1);
+        break;
+      case IK_ARITH_ASIAN: {
+        const ArithPath a = arith_path<FP32>(it.u.a, dr, on, sign);
+        out[0] = out[1] = out[2] = 0.0;
+        if (on) out[0] = arith_payoff_greeks(it.u.a, a, bad, out + 1);
+        break;
+      }
+      default:
+        out[0] = barrier_path<FP32, true, true>(it.u.b, dr, on, sign, bad, out + 1);
+    }
   };
   double pos[3], neg[3] = {0.0, 0.0, 0.0};
   run(live, 1.0f, pos);
@@ -355,6 +369,7 @@ struct Planned {
   double value = 0.0;
   Estimator kind = Estimator::Estimate;
   double control_expected = 0.0;  // arithmetic Asian control mean
+  double control_delta = 0.0, control_gamma = 0.0;  // and its delta and gamma
   // Spot Greeks requested and kept after the engine's filters, and the
   // values of a contract resolved before sampling.
   int32_t greeks = 0;
@@ -646,8 +661,9 @@ static Planned plan_digital(const kg_digital_terms& o, const kg_bsm_market& m, B
 
 // MonteCarloAsianEngine::prepare (engines/monte_carlo/asian.rs:374-551) and
 // geometric_control (:1016-1061) for flat rates and a fresh average.
-static Planned plan_asian(const kg_asian_terms& o, const kg_bsm_market& m, Blob&) {
+static Planned plan_asian(const kg_asian_terms& o, const kg_bsm_market& m, Blob&, int32_t request = 0) {
   Planned pl;
+  const int32_t spot_greeks = request & (KG_GREEK_DELTA | KG_GREEK_GAMMA);
   pl.kind = Estimator::PathMeans;
   if (o.option_type != 1 && o.option_type != -1) fail(KG_INVALID_INPUT, "option_type must be 1 or -1");
   if (o.step_count < 2 || o.step_count > 10000)
@@ -689,6 +705,11 @@ static Planned plan_asian(const kg_asian_terms& o, const kg_bsm_market& m, Blob&
     g.log_strike = std::log(o.strike);
     g.sign = sign;
     g.log_discount = log_discount;
+    g.log_spot = std::log(m.spot);
+    g.log_weight = std::log(weight);
+    g.log_base_width = std::log(sigma) + 0.5 * std::log(vt);
+    g.request = spot_greeks;
+    pl.greeks = spot_greeks;
     if (!std::isfinite(g.drift) || !std::isfinite(g.width)) unstable();
     pl.item.kind = IK_GEO_ASIAN;
     pl.item.dims = 1;
@@ -706,6 +727,9 @@ static Planned plan_asian(const kg_asian_terms& o, const kg_bsm_market& m, Blob&
   a.interval_forward = carry * dt;
   a.drift = drift_rate * dt;
   a.width = sigma * std::sqrt(dt);
+  a.log_width = std::log(sigma) + 0.5 * std::log(dt);
+  a.request = spot_greeks;
+  pl.greeks = spot_greeks;
   a.sign = sign;
   if (a.width < 1e-5) fail(KG_UNSUPPORTED_OPERATION, "arithmetic widths below 1e-5 are not supported");
   const double n = o.step_count, fw = 1.0 - 0.5 / n;
@@ -742,6 +766,10 @@ static Planned plan_asian(const kg_asian_terms& o, const kg_bsm_market& m, Blob&
       const double la = lf + log_discount + lcdf(sign * d1);
       const double lc = ls + log_discount + lcdf(sign * d2);
       pl.control_expected = sign > 0 ? pdiff(la, lc) : pdiff(lc, la);
+      // geometric_values' delta and gamma (asian.rs:1084-1096)
+      pl.control_delta = sign * std::exp(la - std::log(m.spot));
+      pl.control_gamma = std::exp(lf + log_discount - 2.0 * std::log(m.spot) - std::log(w) - 0.5 * d1 * d1 -
+                                  0.5 * std::log(2.0 * M_PI));
     }
   }
   pl.item.kind = IK_ARITH_ASIAN;
@@ -1235,11 +1263,33 @@ static void price_spot_greeks(kg_context* ctx, const kg_settings* settings, Plan
   KG_HIP(hipGetLastError());
   Moments r[4];
   finish_sampling(ctx, parts, blocks, 4, r, &out->device_ms);
+  const bool control = p.item.kind == IK_ARITH_ASIAN && p.item.u.a.control;
+  if (control && scaled_value(r[0].scale, r[0].mean, 0.0) + p.control_expected < 0.0) {
+    // A controlled price below zero is resampled without the control on the
+    // same seed (asian.rs:618-630).
+    p.item.u.a.control = 0;
+    p.control_expected = p.control_delta = p.control_gamma = 0.0;
+    kg_settings same = *settings;
+    same.seed = s.seed;
+    same.has_seed = 1;
+    Blob fresh = blob;
+    fresh.bytes.resize(item_off);
+    price_spot_greeks(ctx, &same, std::move(p), fresh, greeks, out);
+    return;
+  }
   fill_estimate(s, r[0], r[3], 0.0, false, true, out);
   out->path_steps = (double)s.path_count * (double)p.item.dims;
+  if (control) {  // the control's exact means (asian.rs:612-616)
+    out->price += p.control_expected;
+    if (out->has_standard_error) {
+      const double hw = 1.959963984540054 * out->standard_error;
+      out->ci_lower = out->price - hw;
+      out->ci_upper = out->price + hw;
+    }
+  }
   greeks->available = p.greeks;
-  if (p.greeks & KG_GREEK_DELTA) greeks->delta = scaled_value(r[1].scale, r[1].mean, 0.0);
-  if (p.greeks & KG_GREEK_GAMMA) greeks->gamma = scaled_value(r[2].scale, r[2].mean, 0.0);
+  if (p.greeks & KG_GREEK_DELTA) greeks->delta = scaled_value(r[1].scale, r[1].mean, 0.0) + p.control_delta;
+  if (p.greeks & KG_GREEK_GAMMA) greeks->gamma = scaled_value(r[2].scale, r[2].mean, 0.0) + p.control_gamma;
   if (!std::isfinite(greeks->delta) || !std::isfinite(greeks->gamma)) unstable();
 }
 
@@ -1429,6 +1479,16 @@ kg_status kg_asian_price_strikes(kg_context* ctx, const kg_settings* settings, c
 kg_status kg_asian_price(kg_context* ctx, const kg_settings* settings, const kg_asian_terms* option,
                          const kg_bsm_market* market, kg_estimate* out) {
   return guarded(ctx, [&] { price_single(ctx, KG_BOOK_ASIAN, settings, option, market, out); });
+}
+
+kg_status kg_asian_price_with_greeks(kg_context* ctx, const kg_settings* settings, const kg_asian_terms* option,
+                                     const kg_bsm_market* market, kg_spot_greeks* greeks, kg_estimate* out) {
+  return guarded(ctx, [&] {
+    if (!settings || !option || !market || !greeks || !out) fail(KG_INVALID_INPUT, "arguments must not be null");
+    Blob blob;
+    Planned p = plan_asian(*option, *market, blob, greeks->request);
+    price_spot_greeks(ctx, settings, std::move(p), blob, greeks, out);
+  });
 }
 
 kg_status kg_accumulator_price(kg_context* ctx, const kg_settings* settings,

@@ -355,10 +355,16 @@ __device__ double digital_path(const DigitalDev& g, D& dr, bool live, float sign
 // ------------------------------------------------------------------ Asians
 struct GeoAsianDev {
   double im, drift, width, log_width, log_strike, sign, log_discount;
+  double log_spot, log_weight, log_base_width;  // for the likelihood-ratio Greeks
+  int32_t request, pad;                         // KG_GREEK_* bits
 };
 
-template <bool FP32, class D>
-__device__ double geo_asian(const GeoAsianDev& g, D& dr, bool live, float sign, uint32_t& bad) {
+// GeometricParameters::geometric (asian.rs:741-784). LR: also the per-path
+// delta and gamma, lr[0] and lr[1].
+template <bool FP32, bool LR = false, class D>
+__device__ double geo_asian(const GeoAsianDev& g, D& dr, bool live, float sign, uint32_t& bad,
+                            double* lr = nullptr) {
+  if (LR) lr[0] = lr[1] = 0.0;
   if (!live) return 0.0;
   const double z = (double)sign * (double)(FP32 ? (double)dr.template normal<float>(0)
                                                  : dr.template normal<double>(0));
@@ -376,6 +382,15 @@ __device__ double geo_asian(const GeoAsianDev& g, D& dr, bool live, float sign, 
     lp = g.log_strike + log(-expm1(distance));
   const double v = exp(lp + g.log_discount);
   if (!isfinite(v)) bad = 1;
+  if (LR) {
+    const double log_asset = g.log_strike + distance + g.log_discount + g.log_weight;
+    if (g.request & 1) lr[0] = g.sign * exp(log_asset - g.log_spot);
+    const double numerator = z - exp(g.log_base_width);
+    if ((g.request & 2) && numerator != 0.0)
+      lr[1] = g.sign * copysign(exp(log_asset - 2.0 * g.log_spot + log(fabs(numerator)) - g.log_base_width),
+                                numerator);
+    if (!isfinite(lr[0]) || !isfinite(lr[1])) bad = 1;
+  }
   return v;
 }
 
@@ -383,6 +398,8 @@ struct ArithAsianDev {
   int32_t n, control;
   double spot, strike, weight, log_discount, drift, width, sign, interval_forward;
   double c_log_strike, c_las, c_fw;
+  double log_width;  // ln(sigma) + ln(dt) / 2, for the gamma
+  int32_t request, pad;
 };
 
 __device__ inline double geometric_values_d(const ArithAsianDev& p, double lf, double ls,
@@ -469,6 +486,73 @@ __device__ inline double arith_payoff(const ArithAsianDev& p, const ArithPath& a
   }
   const double v = price - residual;
   if (!isfinite(v)) bad = 1;
+  return v;
+}
+
+// The same payoff with its conditional delta and gamma (asian.rs:1233-1255,
+// and geometric_values' for the control residual, :1063-1098) in lr[0] and
+// lr[1]: the last interval is integrated in closed form, so both are
+// smooth functions of the path.
+__device__ inline double arith_payoff_greeks(const ArithAsianDev& p, const ArithPath& a, uint32_t& bad,
+                                             double* lr) {
+  constexpr double HALF_LN_2PI = 0.9189385332046727;
+  const double inv_n = 1.0 / (double)p.n;
+  const double rel = a.rel, log_sum = a.log_sum, gg = a.gg;
+  const double half = 0.5 * inv_n, rest = 1.0 - half;
+  const double log_spot = log(p.spot), log_weight = log(p.weight);
+  const double log_rest = log_sum == -INFINITY ? log(rest) + log1p(rel / rest) : log_sum;
+  const double log_q = log_spot + log_weight + log_rest + p.interval_forward;
+  const double log_c = log_spot + log_weight + log(half);
+  const bool has_extra = log_sum == -INFINITY;
+  const double extra = has_extra ? rest * expm1(p.interval_forward) + rel * exp(p.interval_forward) : 0.0;
+  const bool want_delta = p.request & 1, want_gamma = p.request & 2;
+  double res[3] = {0.0, 0.0, 0.0};
+  if (p.control) {
+    const double w = p.c_fw * p.width;
+    const double lf = p.c_las + gg + p.c_fw * p.drift + 0.5 * w * w;
+    const double d1 = (lf - p.c_log_strike) / w + 0.5 * w;
+    const double d2 = d1 - w;
+    const double la = lf + p.log_discount + d_log_normal_cdf(p.sign * d1);
+    const double lc = p.c_log_strike + p.log_discount + d_log_normal_cdf(p.sign * d2);
+    res[0] = p.sign > 0 ? d_positive_difference(la, lc) : d_positive_difference(lc, la);
+    if (want_delta) res[1] = p.sign * exp(la - log_spot);
+    if (want_gamma) res[2] = exp(lf + p.log_discount - 2.0 * log_spot - log(w) - 0.5 * d1 * d1 - HALF_LN_2PI);
+  }
+  const double log_strike = log(p.strike);
+  double price, delta = 0.0, gamma = 0.0;
+  if (log_c >= log_strike) {
+    if (p.sign < 0) {
+      price = 0.0;
+    } else {
+      price = exp(log_q + p.log_discount) +
+              d_positive_difference(log_c + p.log_discount, log_strike + p.log_discount);
+      if (want_delta) delta = exp(log_q + p.log_discount - log_spot) + exp(log_c + p.log_discount - log_spot);
+    }
+  } else {
+    const double log_b = log_strike + log(-expm1(log_c - log_strike));
+    double forward = log_q - log_b;
+    const double baseline = p.weight * ((p.spot - p.strike) / p.spot);
+    if (has_extra) {
+      const double excess = baseline + p.weight * extra;
+      const double bb = exp(log_b - log_spot);
+      if (bb > 0.0 && fabs(excess / bb) < 0.5) forward = log1p(excess / bb);
+    }
+    const double center = forward / p.width;
+    const double d1 = center + 0.5 * p.width, d2 = center - 0.5 * p.width;
+    const double n1 = d_log_normal_cdf(p.sign * d1), n2 = d_log_normal_cdf(p.sign * d2);
+    const double la = log_q + p.log_discount + n1;
+    const double lc = log_b + p.log_discount + n2;
+    price = p.sign > 0 ? d_positive_difference(la, lc) : d_positive_difference(lc, la);
+    if (want_delta)
+      delta = p.sign * (exp(log_c + p.log_discount - log_spot + n2) + exp(log_q + p.log_discount - log_spot + n1));
+    if (want_gamma)
+      gamma = exp(p.log_discount + 2.0 * log(p.strike) - 2.0 * log_spot - log_b - p.log_width - 0.5 * d2 * d2 -
+                  HALF_LN_2PI);
+  }
+  const double v = price - res[0];
+  lr[0] = delta - res[1];
+  lr[1] = gamma - res[2];
+  if (!isfinite(v) || !isfinite(lr[0]) || !isfinite(lr[1])) bad = 1;
   return v;
 }
 

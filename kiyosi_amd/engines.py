@@ -351,8 +351,11 @@ class MonteCarloDigitalEngine(_SpotGreekEngine):
         return _native.lib().kg_digital_price_with_greeks, instrument._terms()
 
 
-class MonteCarloAsianEngine(_MonteCarloEngine):
-    """Average-price Monte Carlo (engines/monte_carlo/asian.rs)."""
+class MonteCarloAsianEngine(_SpotGreekEngine):
+    """Average-price Monte Carlo (engines/monte_carlo/asian.rs). Greeks are
+    the engine's native delta and gamma (asian.rs:885-898): the conditional
+    estimators of the geometric and arithmetic payoffs on the price's
+    draws."""
 
     def __init__(self, *, step_count=50, **kwargs):
         if not 2 <= step_count <= 10_000:
@@ -368,6 +371,12 @@ class MonteCarloAsianEngine(_MonteCarloEngine):
         if not isinstance(instrument, _AveragePrice):
             raise TypeError("MonteCarloAsianEngine prices average-price options")
         return _native.lib().kg_asian_price, instrument._terms(self.step_count), context.bsm_market()
+
+    def _greek_entry(self, instrument):
+        from .instruments import _AveragePrice
+        if not isinstance(instrument, _AveragePrice):
+            raise TypeError("MonteCarloAsianEngine prices average-price options")
+        return _native.lib().kg_asian_price_with_greeks, instrument._terms(self.step_count)
 
     def price_strikes(self, instrument, context, *, strikes):
         """Prices of the arithmetic-average option at each strike on one

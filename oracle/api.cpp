@@ -225,6 +225,16 @@ int oracle_asian_price(const kg_settings* s, const kg_asian_terms* o, const kg_b
   });
 }
 
+int oracle_asian_price_with_greeks(const kg_settings* s, const kg_asian_terms* o, const kg_bsm_market* m,
+                                   int workers, kg_spot_greeks* greeks, kg_estimate* out) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    SpotGreeks g;
+    write(asian_native(st, *o, *m, context(), greeks->request, &g), out);
+    write_greeks(g, greeks);
+  });
+}
+
 int oracle_accumulator_price(const kg_settings* s, const kg_accumulator_terms* o,
                              const kg_bsm_market* m, int workers, kg_estimate* out) {
   return guarded([&] {

@@ -537,16 +537,29 @@ struct AsianArith {
   double interval_forward;
 };
 
-inline double geometric_values(const AsianArith& p, double lf, double ls, double width) {
+
+// geometric_values (asian.rs:1063-1098): price, delta and gamma of the
+// geometric control for log forward lf, log strike ls and width.
+inline void geometric_values(const AsianArith& p, double lf, double ls, double width, double* out) {
   double d1 = (lf - ls) / width + 0.5 * width;
   double d2 = d1 - width;
   double la = lf + p.log_discount + log_normal_cdf(p.sign * d1);
   double lc = ls + p.log_discount + log_normal_cdf(p.sign * d2);
-  return p.sign > 0 ? positive_difference(la, lc) : positive_difference(lc, la);
+  out[0] = p.sign > 0 ? positive_difference(la, lc) : positive_difference(lc, la);
+  const double log_spot = std::log(p.spot);
+  out[1] = p.sign * std::exp(la - log_spot);
+  out[2] = std::exp(lf + p.log_discount - 2.0 * log_spot - std::log(width) - 0.5 * d1 * d1 -
+                    0.5 * std::log(2.0 * M_PI));
 }
 
-inline Estimate asian_price(const Settings& st, const kg_asian_terms& o, const kg_bsm_market& m,
-                            const Context& cx) {
+// MonteCarloAsianEngine::native (asian.rs:347-372, prepare :374-560) for
+// flat BSM rates, no seasoning and (arithmetic) no forward start. `request`
+// holds KG_GREEK_* bits; delta and gamma are the path means of the engine's
+// conditional estimators (GeometricParameters::geometric :741-784,
+// ArithmeticParameters::arithmetic :1100-1256), plus the geometric
+// control's exact values when the control is used.
+inline Estimate asian_native(const Settings& st, const kg_asian_terms& o, const kg_bsm_market& m,
+                             const Context& cx, int request, SpotGreeks* greeks) {
   const int64_t val = m.valuation_time;
   if (val < o.effective_time || val > o.expiry_time)
     throw Failure(INVALID_INPUT, "valuation must be within contract life");
@@ -554,6 +567,7 @@ inline Estimate asian_price(const Settings& st, const kg_asian_terms& o, const k
   if (o.step_count < 2 || o.step_count > 10000) throw Failure(INVALID_INPUT, "step_count out of range");
   const double tau = years_until(val, o.expiry_time);
   const double sign = o.option_type;
+  // At expiry the price is known and the native estimator has no Greeks.
   if (tau == 0.0) return known_estimate(st, std::max(sign * (m.spot - o.strike), 0.0));
   if (m.volatility == 0.0) throw Failure(UNSUPPORTED, "zero volatility is deterministic");
   const double period = years_until(o.averaging_start_time, o.expiry_time);
@@ -566,25 +580,27 @@ inline Estimate asian_price(const Settings& st, const kg_asian_terms& o, const k
   const double carry = r - q;
   const double drift_rate = carry - 0.5 * sigma * sigma;
   const double log_discount = -r * tau;
-  Moments total[1], pairs[1];
+  const int req = request & (KG_GREEK_DELTA | KG_GREEK_GAMMA);
+  const bool want_delta = req & KG_GREEK_DELTA, want_gamma = req & KG_GREEK_GAMMA;
+  Moments total[3], pairs[3];
+  double expected[3] = {0.0, 0.0, 0.0};
   if (o.averaging == KG_AVERAGE_GEOMETRIC) {
     const double vt = lead + future / 3.0;
     const double im = weight * log_ratio(m.spot, o.strike);
     const double drift = weight * drift_rate * (lead + future / 2.0);
     const double width = sigma * (weight * std::sqrt(vt));
     const double log_width = std::log(sigma) + std::log(weight) + 0.5 * std::log(vt);
-    const double log_strike = std::log(o.strike);
-    path_samples<1>(
+    const double log_base_width = std::log(sigma) + 0.5 * std::log(vt);
+    const double log_strike = std::log(o.strike), log_spot = std::log(m.spot), log_weight = std::log(weight);
+    path_samples<3>(
         st, 1, true, cx,
         [&](const double* n, double* out) {
           // GeometricParameters::geometric (asian.rs:741-784)
+          out[0] = out[1] = out[2] = 0.0;
           double distance = im + drift + width * n[0];
           double center = (im + drift) / width;
           double standardized = center + n[0];
-          if (sign * standardized <= 0.0) {
-            out[0] = 0.0;
-            return;
-          }
+          if (sign * standardized <= 0.0) return;
           double lp;
           if (std::fabs(distance) < 1e-5 && std::isfinite(standardized))
             lp = log_strike + log_width + std::log(std::fabs(standardized)) +
@@ -594,123 +610,153 @@ inline Estimate asian_price(const Settings& st, const kg_asian_terms& o, const k
           else
             lp = log_strike + std::log(-std::expm1(distance));
           out[0] = std::exp(lp + log_discount);
-          if (!std::isfinite(out[0])) unstable();
+          const double log_asset = log_strike + distance + log_discount + log_weight;
+          if (want_delta) out[1] = sign * std::exp(log_asset - log_spot);
+          const double numerator = n[0] - std::exp(log_base_width);
+          if (want_gamma && numerator != 0.0)
+            out[2] = sign * std::copysign(std::exp(log_asset - 2.0 * log_spot + std::log(std::fabs(numerator)) -
+                                                   log_base_width),
+                                          numerator);
+          if (!std::isfinite(out[0]) || !std::isfinite(out[1]) || !std::isfinite(out[2])) unstable();
         },
         total, pairs);
-    Means mm = path_mean(st, total[0], pairs[0]);
-    return from_sampling(st, mm.mean, mm.has_se, mm.se);
-  }
-  if (lead > 0.0) throw Failure(UNSUPPORTED, "forward-start arithmetic averages are not restated");
-  AsianArith p;
-  p.spot = m.spot;
-  p.strike = o.strike;
-  p.weight = weight;
-  p.log_discount = log_discount;
-  p.n = o.step_count;
-  const double dt = future / o.step_count;
-  p.interval_forward = carry * dt;
-  p.drift = drift_rate * dt;
-  p.width = sigma * std::sqrt(dt);
-  p.log_width = std::log(sigma) + 0.5 * std::log(dt);
-  p.sign = sign;
-  const size_t dims = (size_t)o.step_count - 1;
-  // geometric_control (asian.rs:1016-1061)
-  bool control = false;
-  double c_log_strike = 0, c_las = 0, c_fw = 0, c_expected = 0;
-  {
-    const double n = o.step_count;
-    const double fw = 1.0 - 0.5 / n;
-    const double las = std::log(p.spot) + std::log(p.weight);
-    const double ls = std::log(p.strike);
-    if (!(p.width < 1e-5 || p.width > 0.8 || std::fabs(p.log_discount) > 50.0 ||
-          std::fabs(std::log(p.spot)) > 200.0 || std::fabs(las) > 200.0 || !std::isfinite(ls))) {
-      double mean = 0.0 + fw * p.drift;
-      for (int s = 1; s < o.step_count; ++s) mean += (fw - (double)s / n) * p.drift;
-      double variance = 0.0 * 0.0 + (p.width * p.width) * (n / 3.0 - 1.0 / (12.0 * n));
-      double w = std::sqrt(variance);
-      double lf = las + mean + 0.5 * variance;
-      if (w >= 0.02 && w <= 0.8 && std::fabs(lf - ls) <= 2.0 * w) {
-        control = true;
-        c_log_strike = ls;
-        c_las = las;
-        c_fw = fw;
-        c_expected = geometric_values(p, lf, ls, w);
+  } else {
+    if (lead > 0.0) throw Failure(UNSUPPORTED, "forward-start arithmetic averages are not restated");
+    AsianArith p;
+    p.spot = m.spot;
+    p.strike = o.strike;
+    p.weight = weight;
+    p.log_discount = log_discount;
+    p.n = o.step_count;
+    const double dt = future / o.step_count;
+    p.interval_forward = carry * dt;
+    p.drift = drift_rate * dt;
+    p.width = sigma * std::sqrt(dt);
+    p.log_width = std::log(sigma) + 0.5 * std::log(dt);
+    p.sign = sign;
+    const size_t dims = (size_t)o.step_count - 1;
+    // geometric_control (asian.rs:1016-1061)
+    bool control = false;
+    double c_log_strike = 0, c_las = 0, c_fw = 0;
+    {
+      const double n = o.step_count;
+      const double fw = 1.0 - 0.5 / n;
+      const double las = std::log(p.spot) + std::log(p.weight);
+      const double ls = std::log(p.strike);
+      if (!(p.width < 1e-5 || p.width > 0.8 || std::fabs(p.log_discount) > 50.0 ||
+            std::fabs(std::log(p.spot)) > 200.0 || std::fabs(las) > 200.0 || !std::isfinite(ls))) {
+        double mean = 0.0 + fw * p.drift;
+        for (int s = 1; s < o.step_count; ++s) mean += (fw - (double)s / n) * p.drift;
+        double variance = 0.0 * 0.0 + (p.width * p.width) * (n / 3.0 - 1.0 / (12.0 * n));
+        double w = std::sqrt(variance);
+        double lf = las + mean + 0.5 * variance;
+        if (w >= 0.02 && w <= 0.8 && std::fabs(lf - ls) <= 2.0 * w) {
+          control = true;
+          c_log_strike = ls;
+          c_las = las;
+          c_fw = fw;
+          geometric_values(p, lf, ls, w, expected);
+        }
       }
     }
+    auto run = [&](bool use_control) {
+      path_samples<3>(
+          st, dims, true, cx,
+          [&](const double* normals, double* out) {
+            // ArithmeticGrid::path (asian.rs:640-712), flat rates, no lead
+            double growth = 0.0, rel = 0.0, log_sum = -INFINITY, gg = 0.0;
+            for (size_t i = 0; i < dims; ++i) {
+              growth += p.drift + p.width * normals[i];
+              double fraction = ((int)i + 2 == p.n ? 0.5 : 1.0) / (double)p.n;
+              if (use_control) gg += fraction * growth;
+              if (growth > 600.0 || log_sum != -INFINITY) {
+                if (log_sum == -INFINITY) log_sum = std::log(1.0 - 0.5 / (double)p.n + rel);
+                log_sum = log_add(log_sum, std::log(fraction) + growth);
+              } else {
+                rel += fraction * std::expm1(growth);
+              }
+            }
+            const double half = 0.5 / (double)p.n, rest = 1.0 - half;
+            const double log_spot = std::log(p.spot);
+            const double log_rest = log_sum == -INFINITY ? std::log(rest) + std::log1p(rel / rest) : log_sum;
+            const double log_q = log_spot + std::log(p.weight) + 0.0 + log_rest + p.interval_forward;
+            const double log_c = log_spot + std::log(p.weight) + 0.0 + std::log(half);
+            const double log_accrued = log_add(-INFINITY, log_c);
+            const bool has_extra = log_sum == -INFINITY;
+            const double extra = has_extra ? std::expm1(0.0) + std::exp(0.0) * (rest * std::expm1(p.interval_forward) +
+                                                                                rel * std::exp(p.interval_forward))
+                                           : 0.0;
+            // ArithmeticParameters::arithmetic (asian.rs:1100-1256)
+            double residual[3] = {0.0, 0.0, 0.0};
+            if (use_control) {
+              double width = c_fw * p.width;
+              double lf = c_las + 0.0 + gg + c_fw * p.drift + 0.5 * width * width;
+              geometric_values(p, lf, c_log_strike, width, residual);
+              if (!want_delta) residual[1] = 0.0;
+              if (!want_gamma) residual[2] = 0.0;
+            }
+            const double log_strike = std::log(p.strike);
+            double price, delta = 0.0, gamma = 0.0;
+            if (log_accrued >= log_strike) {
+              if (p.sign < 0) {
+                price = 0.0;
+              } else {
+                double c = positive_difference(log_accrued + p.log_discount, log_strike + p.log_discount);
+                price = std::exp(log_q + p.log_discount) + c;
+                if (want_delta)
+                  delta = std::exp(log_q + p.log_discount - log_spot) + std::exp(log_c + p.log_discount - log_spot);
+              }
+            } else {
+              const double log_b = log_strike + std::log(-std::expm1(log_accrued - log_strike));
+              double forward = log_q - log_b;
+              const double baseline = 0.0 + p.weight * ((p.spot - p.strike) / p.spot);
+              if (has_extra) {
+                double excess = baseline + p.weight * extra;
+                double b = std::exp(log_b - log_spot);
+                if (b > 0.0 && std::fabs(excess / b) < 0.5) forward = std::log1p(excess / b);
+              }
+              if (!(p.width >= 1e-5)) throw Failure(UNSUPPORTED, "arithmetic widths below 1e-5");
+              double center = forward / p.width;
+              double d1 = center + 0.5 * p.width, d2 = center - 0.5 * p.width;
+              double la = log_q + p.log_discount + log_normal_cdf(p.sign * d1);
+              double lc = log_b + p.log_discount + log_normal_cdf(p.sign * d2);
+              price = p.sign > 0 ? positive_difference(la, lc) : positive_difference(lc, la);
+              if (want_delta)
+                delta = p.sign * (std::exp(log_c + p.log_discount - log_spot + log_normal_cdf(p.sign * d2)) +
+                                  std::exp(log_q + p.log_discount - log_spot + log_normal_cdf(p.sign * d1)));
+              if (want_gamma) {
+                const double k = p.strike;  // adjusted strike without seasoning
+                gamma = std::exp(p.log_discount + 2.0 * std::log(k) - 2.0 * log_spot - log_b - p.log_width -
+                                 0.5 * d2 * d2 - 0.5 * std::log(2.0 * M_PI));
+              }
+            }
+            out[0] = price - residual[0];
+            out[1] = delta - residual[1];
+            out[2] = gamma - residual[2];
+            if (!std::isfinite(out[0]) || !std::isfinite(out[1]) || !std::isfinite(out[2])) unstable();
+          },
+          total, pairs);
+    };
+    run(control);
+    if (!control) expected[0] = expected[1] = expected[2] = 0.0;
+    Means mm = path_mean(st, total[0], pairs[0]);
+    if (mm.mean + expected[0] < 0.0) {  // asian.rs:618-630: resample without the control
+      run(false);
+      expected[0] = expected[1] = expected[2] = 0.0;
+    }
   }
-  auto run = [&](bool use_control) {
-    path_samples<1>(
-        st, dims, true, cx,
-        [&](const double* normals, double* out) {
-          // ArithmeticGrid::path (asian.rs:640-712), flat rates, no lead
-          double growth = 0.0, rel = 0.0, log_sum = -INFINITY, gg = 0.0;
-          for (size_t i = 0; i < dims; ++i) {
-            growth += p.drift + p.width * normals[i];
-            double fraction = ((int)i + 2 == p.n ? 0.5 : 1.0) / (double)p.n;
-            if (use_control) gg += fraction * growth;
-            if (growth > 600.0 || log_sum != -INFINITY) {
-              if (log_sum == -INFINITY) log_sum = std::log(1.0 - 0.5 / (double)p.n + rel);
-              log_sum = log_add(log_sum, std::log(fraction) + growth);
-            } else {
-              rel += fraction * std::expm1(growth);
-            }
-          }
-          const double half = 0.5 / (double)p.n, rest = 1.0 - half;
-          const double log_spot = std::log(p.spot);
-          const double log_rest = log_sum == -INFINITY ? std::log(rest) + std::log1p(rel / rest) : log_sum;
-          const double log_q = log_spot + std::log(p.weight) + 0.0 + log_rest + p.interval_forward;
-          const double log_c = log_spot + std::log(p.weight) + 0.0 + std::log(half);
-          const double log_accrued = log_add(-INFINITY, log_c);
-          const bool has_extra = log_sum == -INFINITY;
-          const double extra = has_extra ? std::expm1(0.0) + std::exp(0.0) * (rest * std::expm1(p.interval_forward) +
-                                                                              rel * std::exp(p.interval_forward))
-                                         : 0.0;
-          // ArithmeticParameters::arithmetic (asian.rs:1100-1256)
-          double residual = 0.0;
-          if (use_control) {
-            double width = c_fw * p.width;
-            double lf = c_las + 0.0 + gg + c_fw * p.drift + 0.5 * width * width;
-            residual = geometric_values(p, lf, c_log_strike, width);
-          }
-          const double log_strike = std::log(p.strike);
-          double price;
-          if (log_accrued >= log_strike) {
-            if (p.sign < 0) {
-              price = 0.0;
-            } else {
-              double c = positive_difference(log_accrued + p.log_discount, log_strike + p.log_discount);
-              price = std::exp(log_q + p.log_discount) + c;
-            }
-          } else {
-            const double log_b = log_strike + std::log(-std::expm1(log_accrued - log_strike));
-            double forward = log_q - log_b;
-            const double baseline = 0.0 + p.weight * ((p.spot - p.strike) / p.spot);
-            if (has_extra) {
-              double excess = baseline + p.weight * extra;
-              double b = std::exp(log_b - log_spot);
-              if (b > 0.0 && std::fabs(excess / b) < 0.5) forward = std::log1p(excess / b);
-            }
-            if (!(p.width >= 1e-5)) throw Failure(UNSUPPORTED, "arithmetic widths below 1e-5");
-            double center = forward / p.width;
-            double d1 = center + 0.5 * p.width, d2 = center - 0.5 * p.width;
-            double la = log_q + p.log_discount + log_normal_cdf(p.sign * d1);
-            double lc = log_b + p.log_discount + log_normal_cdf(p.sign * d2);
-            price = p.sign > 0 ? positive_difference(la, lc) : positive_difference(lc, la);
-          }
-          out[0] = price - residual;
-          if (!std::isfinite(out[0])) unstable();
-        },
-        total, pairs);
-  };
-  run(control);
   Means mm = path_mean(st, total[0], pairs[0]);
-  double price = mm.mean + (control ? c_expected : 0.0);
-  if (price < 0.0) {  // asian.rs:744-752: resample without the control
-    run(false);
-    mm = path_mean(st, total[0], pairs[0]);
-    price = mm.mean;
+  if (greeks) {
+    greeks->available = req;
+    greeks->delta = want_delta ? scaled_value(total[1].scale, total[1].mean, 0.0) + expected[1] : 0.0;
+    greeks->gamma = want_gamma ? scaled_value(total[2].scale, total[2].mean, 0.0) + expected[2] : 0.0;
   }
-  return from_sampling(st, price, mm.has_se, mm.se);
+  return from_sampling(st, mm.mean + expected[0], mm.has_se, mm.se);
+}
+
+inline Estimate asian_price(const Settings& st, const kg_asian_terms& o, const kg_bsm_market& m,
+                            const Context& cx) {
+  return asian_native(st, o, m, cx, 0, nullptr);
 }
 
 // ---------------------------------------------------------------- accumulator

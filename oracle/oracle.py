@@ -191,6 +191,14 @@ def asian_price(settings, terms, market, workers: int = 1):
     return out
 
 
+def asian_price_with_greeks(settings, terms, market, request=3, workers: int = 1):
+    """(estimate, kg_spot_greeks) of MonteCarloAsianEngine::native with delta/gamma."""
+    out, greeks = _abi.Estimate(), _abi.SpotGreeks(request, 0, 0.0, 0.0, 0.0)
+    _check(lib().oracle_asian_price_with_greeks(C.byref(settings), C.byref(terms), C.byref(market), workers,
+                                                C.byref(greeks), C.byref(out)))
+    return out, greeks
+
+
 def accumulator_price(settings, terms, market, workers: int = 1):
     out = _abi.Estimate()
     _check(lib().oracle_accumulator_price(C.byref(settings), C.byref(terms), C.byref(market), workers,

@@ -157,3 +157,57 @@ def test_accumulator_matches_a_brute_force_daily_simulation(oracle):
     val[alive] = (q * (S - 97.0) * math.exp(-0.05 * times[-1]))[alive]
     brute, brute_se = val.mean(), val.std() / math.sqrt(P)
     assert abs(est.price - brute) < 4 * math.hypot(est.standard_error, brute_se)
+
+
+def geometric_closed_form(S, K, r, q, sig, T, kind):
+    """Continuously sampled geometric average from 0 to T: log G is normal
+    with mean log S + (r - q - sig^2/2) T/2 and variance sig^2 T/3."""
+    w = sig * math.sqrt(T / 3)
+    lf = math.log(S) + (r - q - 0.5 * sig * sig) * T / 2 + 0.5 * w * w
+    d1 = (lf - math.log(K)) / w + 0.5 * w
+    disc, F = math.exp(-r * T), math.exp(lf)
+    price = disc * kind * (F * norm.cdf(kind * d1) - K * norm.cdf(kind * (d1 - w)))
+    return price, disc * kind * F / S * norm.cdf(kind * d1), disc * F / S**2 * norm.pdf(d1) / w
+
+
+@pytest.mark.parametrize("kind", [1, -1])
+def test_geometric_asian_native_greeks(oracle, kind):
+    """Delta is the per-path derivative of the sampled payoff, so it equals a
+    common-random-number difference of prices; gamma (a likelihood-ratio
+    form) meets the closed form's gamma within the sampling error."""
+    t = _abi.AsianTerms(kind, _abi.AVERAGE_GEOMETRIC, 100, 0, 100.0, 0.0, us(START), us(START), us(END))
+    s = settings(400_000, antithetic=True)
+    mk = lambda spot: _abi.BsmMarket(spot, 0.04, 0.01, 0.2, us(START))
+    est, g = oracle.asian_price_with_greeks(s, t, mk(100.0), workers=8)
+    assert g.available == 3
+    assert est.price == oracle.asian_price(s, t, mk(100.0), workers=8).price
+    h = 0.01
+    fd = (oracle.asian_price(s, t, mk(100 + h), workers=8).price
+          - oracle.asian_price(s, t, mk(100 - h), workers=8).price) / (2 * h)
+    assert abs(g.delta - fd) < 1e-4
+    _, delta, gamma = geometric_closed_form(100.0, 100.0, 0.04, 0.01, 0.2, 1.0, kind)
+    assert abs(g.delta - delta) < 3e-3
+    assert g.gamma == pytest.approx(gamma, rel=0.02)
+    only_gamma = oracle.asian_price_with_greeks(s, t, mk(100.0), request=2, workers=8)[1]
+    assert only_gamma.available == 2 and only_gamma.gamma == g.gamma and only_gamma.delta == 0.0
+
+
+@pytest.mark.parametrize("kind", [1, -1])
+def test_arithmetic_asian_native_greeks_are_derivatives_of_the_conditional_price(oracle, kind):
+    """Each path's arithmetic estimate is a smooth Black-type price of the
+    last interval, and its delta and gamma are that price's first and second
+    spot derivatives (with the geometric control's likewise), so both equal
+    common-random-number differences of prices on the same draws."""
+    t = _abi.AsianTerms(kind, _abi.AVERAGE_ARITHMETIC, 50, 0, 100.0, 0.0, us(START), us(START), us(END))
+    s = settings(20_000, antithetic=True)
+    mk = lambda spot: _abi.BsmMarket(spot, 0.04, 0.01, 0.2, us(START))
+    price = lambda spot: oracle.asian_price(s, t, mk(spot), workers=8).price
+    est, g = oracle.asian_price_with_greeks(s, t, mk(100.0), workers=8)
+    assert g.available == 3 and est.price == price(100.0)
+    h = 0.01
+    assert g.delta == pytest.approx((price(100 + h) - price(100 - h)) / (2 * h), rel=1e-6)
+    h = 0.05
+    assert g.gamma == pytest.approx((price(100 + h) - 2 * est.price + price(100 - h)) / h**2, rel=1e-4)
+    _, delta, gamma = geometric_closed_form(100.0, 100.0, 0.04, 0.01, 0.2, 1.0, kind)
+    assert 0 < kind * g.delta < kind * delta + 0.05 and 0.8 * gamma < g.gamma < 1.3 * gamma
+

@@ -154,3 +154,38 @@ def test_fp32_book_within_half_a_standard_error(gpu):
                        if p.path_standard_error > 0])
     assert np.mean(errors <= 0.5) >= 0.99, np.sort(errors)[-5:]
     assert errors.max() < 1.0
+
+
+@pytest.mark.parametrize("averaging", [_abi.AVERAGE_GEOMETRIC, _abi.AVERAGE_ARITHMETIC])
+@pytest.mark.parametrize("kind", [1, -1])
+@pytest.mark.parametrize("paths,antithetic,sampling", [(4096, False, _abi.PSEUDO_RANDOM),
+                                                      (3001, True, _abi.PSEUDO_RANDOM),
+                                                      (2048, False, _abi.SOBOL)])
+def test_asian_native_greeks_match_the_oracle(gpu, oracle, averaging, kind, paths, antithetic, sampling):
+    t = _abi.AsianTerms(kind, averaging, 64, 0, 100.0, 0.0, us(START), us(START), us(END))
+    s = settings(paths, antithetic=antithetic, sampling=sampling)
+    m = market(rate=0.04, dividend=0.01)
+    for request in (3, 1, 2):
+        out, g = _abi.Estimate(), _abi.SpotGreeks(request, 0, 0.0, 0.0, 0.0)
+        gpu.check(_native.lib().kg_asian_price_with_greeks(gpu.handle, C.byref(s), C.byref(t), C.byref(m),
+                                                           C.byref(g), C.byref(out)))
+        ref, rg = oracle.asian_price_with_greeks(s, t, m, request=request, workers=8)
+        assert out.price == price_one(gpu, "kg_asian_price", s, t, m).price
+        assert out.price == pytest.approx(ref.price, rel=1e-10)
+        assert g.available == rg.available == request
+        assert g.delta == pytest.approx(rg.delta, rel=1e-10, abs=1e-15)
+        assert g.gamma == pytest.approx(rg.gamma, rel=1e-10, abs=1e-15)
+
+
+def test_asian_engine_price_with_greeks(gpu, oracle):
+    import kiyosi_amd as k
+    opt = k.ArithmeticAveragePriceOption(option_type="put", strike=100, effective_time=START,
+                                         averaging_start_time=START, expiry_time=END)
+    ctx = k.PricingContext(model_parameters=k.BlackScholesMertonParameters(
+        risk_free_rate=0.04, dividend_yield=0.01, volatility=0.2), spot_price=100, valuation_time=START)
+    engine = k.MonteCarloAsianEngine(path_count=8192, seed=11, step_count=40)
+    res = engine.price_with_greeks(opt, ctx, ["delta", "gamma"])
+    assert res.price == engine.price(opt, ctx)
+    _, rg = oracle.asian_price_with_greeks(engine._settings(), opt._terms(40), ctx.bsm_market(), workers=8)
+    assert res.get("delta") == pytest.approx(rg.delta, rel=1e-10)
+    assert res.get("gamma") == pytest.approx(rg.gamma, rel=1e-10)
