@@ -482,8 +482,9 @@ static int cu_count(kg_context* ctx) {
 
 // The two-pass launches of a bridge plan (see bridge_events_kernel).
 template <bool SOBOL, bool FP32, bool LDS>
-static void launch_bridge(kg_context* ctx, const AcHostPlan& h, const AcPlanDev& p, const Sampler& s,
-                          const uint64_t* dirs, const uint64_t* shifts, Moments* parts, uint32_t* flags) {
+static void launch_bridge(kg_context* ctx, const AcPlanDev& p, const Sampler& s, const uint64_t* dirs,
+                          const uint64_t* shifts, Moments* parts, uint32_t* flags, const unsigned char* d_rows,
+                          size_t table_offset) {
   using R = typename std::conditional<FP32, float, double>::type;
   using W = SobolWord<FP32>;
   const int dims = p.n_steps;
@@ -494,11 +495,6 @@ static void launch_bridge(kg_context* ctx, const AcHostPlan& h, const AcPlanDev&
   // Per pass-one wave: a region of REGION items and its count.
   uint32_t* queue = (uint32_t*)ctx->bridge_queue.reserve(sizeof(uint32_t) * regions * (REGION + 1));
   uint32_t* counts = queue + regions * REGION;
-  std::vector<unsigned char> blob;
-  size_t table_offset = 0;
-  bridge_rows<SOBOL, FP32, LDS>(ctx, h, dims, s.seed, blob, &table_offset);
-  unsigned char* d_rows = (unsigned char*)ctx->bridge_rows.reserve(std::max<size_t>(blob.size(), 1));
-  upload_async(ctx, d_rows, blob.data(), blob.size());
   const W* base = SOBOL && LDS ? bridge_base_rows<FP32>(ctx, dims, s.seed, blocks, dirs, shifts) : nullptr;
   size_t events_lds = (size_t)p.n_intervals * sizeof(IntervalRow<R>), fill_lds = 0;
   if (SOBOL && LDS) {
@@ -519,12 +515,24 @@ static void launch_bridge(kg_context* ctx, const AcHostPlan& h, const AcPlanDev&
 }
 
 template <bool SOBOL, bool FP32>
-static void launch_bridge_any(kg_context* ctx, const AcHostPlan& h, const AcPlanDev& p, const Sampler& s,
-                              const uint64_t* dirs, const uint64_t* shifts, Moments* parts, uint32_t* flags) {
+static void launch_bridge_any(kg_context* ctx, const AcPlanDev& p, const Sampler& s, const uint64_t* dirs,
+                              const uint64_t* shifts, Moments* parts, uint32_t* flags, const unsigned char* d_rows,
+                              size_t table_offset) {
   if (SOBOL && sobol_lds_bytes(p.n_steps, FP32) <= SOBOL_LDS_LIMIT)
-    launch_bridge<SOBOL, FP32, true>(ctx, h, p, s, dirs, shifts, parts, flags);
+    launch_bridge<SOBOL, FP32, true>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
   else
-    launch_bridge<SOBOL, FP32, false>(ctx, h, p, s, dirs, shifts, parts, flags);
+    launch_bridge<SOBOL, FP32, false>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
+}
+
+// The interval rows of launch_bridge_any's kernels (same table choice).
+template <bool SOBOL, bool FP32>
+static void bridge_rows_any(kg_context* ctx, const AcHostPlan& h, uint64_t seed, std::vector<unsigned char>& blob,
+                            size_t* table_offset) {
+  const int dims = h.dev.n_steps;
+  if (SOBOL && sobol_lds_bytes(dims, FP32) <= SOBOL_LDS_LIMIT)
+    bridge_rows<SOBOL, FP32, true>(ctx, h, dims, seed, blob, table_offset);
+  else
+    bridge_rows<SOBOL, FP32, false>(ctx, h, dims, seed, blob, table_offset);
 }
 
 template <bool SOBOL, bool FP32, int L, class Plan>
@@ -652,7 +660,31 @@ namespace kg {
 
 // One sampled note on its own: the single-plan kernel and the block merge.
 static void price_note(kg_context* ctx, const Sampler& s, AcHostPlan& h, kg_estimate* out) {
-  AcPlanDev p = upload(ctx, h);
+  // Bridge plans run in two passes (queue items carry the sample index in
+  // 31 bits); other plans, and larger ones, in one. The two-pass interval
+  // rows follow the plan's tables in the same upload.
+  const bool two_pass = h.dev.bridge && s.samples() < ((int64_t)1 << 31);
+  std::vector<unsigned char> blob;
+  const AcOffsets off = stage_plan(blob, h);
+  size_t rows_at = 0, table_offset = 0;
+  if (two_pass) {
+    std::vector<unsigned char> rows;
+    if (s.sobol && s.fp32)
+      bridge_rows_any<true, true>(ctx, h, s.seed, rows, &table_offset);
+    else if (s.sobol)
+      bridge_rows_any<true, false>(ctx, h, s.seed, rows, &table_offset);
+    else if (s.fp32)
+      bridge_rows_any<false, true>(ctx, h, s.seed, rows, &table_offset);
+    else
+      bridge_rows_any<false, false>(ctx, h, s.seed, rows, &table_offset);
+    rows_at = (blob.size() + 255) & ~(size_t)255;
+    blob.resize(rows_at);
+    blob.insert(blob.end(), rows.begin(), rows.end());
+  }
+  unsigned char* d = (unsigned char*)ctx->scratch.reserve(std::max<size_t>(blob.size(), 1));
+  upload_async(ctx, d, blob.data(), blob.size());
+  const AcPlanDev p = bind_plan(h, d, off);
+  const unsigned char* d_rows = d + rows_at;
   const int64_t blocks = s.blocks();
   Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 2 * blocks);
   uint32_t* flags = reset_flags(ctx);
@@ -664,17 +696,15 @@ static void price_note(kg_context* ctx, const Sampler& s, AcHostPlan& h, kg_esti
     shifts = (const uint64_t*)so.d_shifts.ptr;
   }
   timing_begin(ctx);
-  // Bridge plans run in two passes (queue items carry the sample index in
-  // 31 bits); other plans, and larger ones, in one.
-  if (p.bridge && s.samples() < ((int64_t)1 << 31)) {
+  if (two_pass) {
     if (s.sobol && s.fp32)
-      launch_bridge_any<true, true>(ctx, h, p, s, dirs, shifts, parts, flags);
+      launch_bridge_any<true, true>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
     else if (s.sobol)
-      launch_bridge_any<true, false>(ctx, h, p, s, dirs, shifts, parts, flags);
+      launch_bridge_any<true, false>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
     else if (s.fp32)
-      launch_bridge_any<false, true>(ctx, h, p, s, dirs, shifts, parts, flags);
+      launch_bridge_any<false, true>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
     else
-      launch_bridge_any<false, false>(ctx, h, p, s, dirs, shifts, parts, flags);
+      launch_bridge_any<false, false>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
   } else {
     launch_any(ctx, p, p.n_steps, s, parts, flags);
   }

@@ -14,12 +14,16 @@ namespace kg {
 // one workgroup per slot. Thread t folds a contiguous run of blocks in block
 // order; the runs are then merged as a fixed tree. The result depends only
 // on the partials, not on scheduling, so a seed replays exactly
-// (monte_carlo.rs:1800-1809 merges the same partials sequentially).
+// (monte_carlo.rs:1800-1809 merges the same partials sequentially). With
+// `flags`, workgroup 0 also copies the call's flag word behind the slots, so
+// one copy brings back both.
 __global__ void __launch_bounds__(MERGE_THREADS) merge_partials(const Moments* __restrict__ partials,
                                                                 int64_t blocks, int slots,
-                                                                Moments* __restrict__ out) {
+                                                                Moments* __restrict__ out,
+                                                                const uint32_t* __restrict__ flags) {
   __shared__ Moments smem[MERGE_THREADS / WAVE];
   const int s = blockIdx.x;
+  if (flags != nullptr && s == 0 && threadIdx.x == 0) *(uint32_t*)(out + slots) = *flags;
   const int64_t per = (blocks + blockDim.x - 1) / blockDim.x;
   const int64_t begin = per * threadIdx.x;
   const int64_t end = begin + per < blocks ? begin + per : blocks;
@@ -32,7 +36,7 @@ void merge_and_fetch(kg_context* ctx, const Moments* partials, int64_t blocks, i
                      Moments* host_out) {
   Moments* d_out = (Moments*)ctx->results.reserve(sizeof(Moments) * slots);
   hipLaunchKernelGGL(merge_partials, dim3(slots), dim3(MERGE_THREADS), 0, ctx->stream, partials, blocks, slots,
-                     d_out);
+                     d_out, (const uint32_t*)nullptr);
   KG_HIP(hipGetLastError());
   KG_HIP(hipMemcpyAsync(host_out, d_out, sizeof(Moments) * slots, hipMemcpyDeviceToHost,
                         ctx->stream));
@@ -92,16 +96,15 @@ void finish_sampling(kg_context* ctx, const Moments* partials, int64_t blocks, i
                      double* device_ms) {
   KG_HIP(hipEventRecord(ctx->stop, ctx->stream));
   reserve_pinned(ctx, PINNED_RESULT_BYTES);
-  Moments* d_out = (Moments*)ctx->results.reserve(sizeof(Moments) * slots);
-  hipLaunchKernelGGL(merge_partials, dim3(slots), dim3(MERGE_THREADS), 0, ctx->stream, partials, blocks, slots,
-                     d_out);
-  KG_HIP(hipGetLastError());
-  unsigned char* pin = (unsigned char*)ctx->pinned;
   const size_t result_bytes = sizeof(Moments) * slots;
   if (result_bytes + sizeof(uint32_t) > PINNED_RESULT_BYTES) fail(KG_INVALID_INPUT, "too many merged outputs");
-  KG_HIP(hipMemcpyAsync(pin, d_out, result_bytes, hipMemcpyDeviceToHost, ctx->stream));
-  KG_HIP(hipMemcpyAsync(pin + result_bytes, ctx->flags.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                        ctx->stream));
+  // The merged slots, then the flag word (written by merge_partials).
+  Moments* d_out = (Moments*)ctx->results.reserve(result_bytes + sizeof(uint32_t));
+  hipLaunchKernelGGL(merge_partials, dim3(slots), dim3(MERGE_THREADS), 0, ctx->stream, partials, blocks, slots,
+                     d_out, (const uint32_t*)ctx->flags.ptr);
+  KG_HIP(hipGetLastError());
+  unsigned char* pin = (unsigned char*)ctx->pinned;
+  KG_HIP(hipMemcpyAsync(pin, d_out, result_bytes + sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
   KG_HIP(hipStreamSynchronize(ctx->stream));
   uint32_t h;
   std::memcpy(&h, pin + result_bytes, sizeof(h));

@@ -88,7 +88,7 @@ struct kg_context {
   // Bridge notes in two passes: per-leg values, the fill queue, the plan's
   // interval rows, and the sample blocks' Sobol base points (kept for the
   // key's dimensions, seed, block count and precision).
-  kg::DeviceBuffer bridge_values, bridge_queue, bridge_rows, bridge_base;
+  kg::DeviceBuffer bridge_values, bridge_queue, bridge_base;
   kg::BridgeBaseKey bridge_base_key;
   kg::SobolSetup sobol;
   std::vector<uint32_t> joe_kuo;  // direction-number records, loaded once
