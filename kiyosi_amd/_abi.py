@@ -8,6 +8,9 @@ INTERRUPTED, DEVICE_ERROR, MISSING_HISTORY = 4, 5, 6
 PSEUDO_RANDOM, SOBOL = 0, 1
 FP64, FP32_STEP = 0, 1
 
+STATUS_AVAILABLE, STATUS_UNSUPPORTED_SAMPLING, STATUS_INSUFFICIENT_OBSERVATIONS = 0, 1, 2
+STATUS_INCOMPLETE_ANTITHETIC_PAIR, STATUS_ZERO_OBSERVED_VARIANCE = 3, 4
+
 UNCERTAINTY_NAMES = {
     0: "available",
     1: "unsupported_sampling",

@@ -403,75 +403,26 @@ __device__ inline void block_merge(Moments (&m)[N], Moments* smem) {
 }
 
 // Moments of the values a block's threads hold (at most K per thread and
-// output; n[i] of them valid), formed from FP64 sums instead of a Welford
-// merge tree: the block's largest magnitude is the scale, the scaled values
-// and their squares are summed in a fixed shuffle order and then in wave
-// order, and count, mean and M2 come out in thread 0. For kernels whose
-// threads hold only a few values, where merge trees would cost more than the
-// sampling. `smem` needs 3 * N * (blockDim.x / 64) doubles.
+// output; n[i] of them valid): each thread adds its values in order with the
+// reference's scale-normalised Welford update (Moments::add,
+// monte_carlo.rs:2163-2190), and the block's threads are then merged with
+// the Chan tree of block_merge (Moments::merge, :2192-2222) in thread order.
+// M2 is thus formed from deviations, never from raw sums of squares, so it
+// keeps its relative precision for payoffs whose spread is tiny next to
+// their level. The result is in thread 0; `smem` needs N * (blockDim.x / 64)
+// entries.
 template <int N, int K>
 __device__ inline void block_moments(const double (&v)[N][K], const int (&n)[N], Moments (&out)[N],
-                                     double* smem) {
-#pragma clang fp contract(off)
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = threadIdx.x / WAVE;
-  const int waves = blockDim.x / WAVE;
+                                     Moments* smem) {
+#pragma unroll
   for (int i = 0; i < N; ++i) {
-    double a = 0.0;
+    Moments m = moments_zero();
+#pragma unroll
     for (int k = 0; k < K; ++k)
-      if (k < n[i]) a = fmax(a, fabs(v[i][k]));
-    for (int o = WAVE / 2; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o, WAVE));
-    if (lane == 0) smem[i * waves + wave] = a;
+      if (k < n[i]) moments_add(m, v[i][k], true);
+    out[i] = m;
   }
-  __syncthreads();
-  double scale[N];
-  for (int i = 0; i < N; ++i) {
-    double a = 0.0;
-    for (int w = 0; w < waves; ++w) a = fmax(a, smem[i * waves + w]);
-    scale[i] = a;
-  }
-  __syncthreads();
-  for (int i = 0; i < N; ++i) {
-    // Dividing keeps full precision when 1 / scale would be subnormal.
-    const bool divide = scale[i] > 1e300;
-    const double inv = scale[i] > 0.0 ? 1.0 / scale[i] : 0.0;
-    double s1 = 0.0, s2 = 0.0, c = 0.0;
-    for (int k = 0; k < K; ++k)
-      if (k < n[i]) {
-        const double x = divide ? v[i][k] / scale[i] : v[i][k] * inv;
-        s1 += x;
-        s2 = fma(x, x, s2);
-        c += 1.0;
-      }
-    for (int o = WAVE / 2; o > 0; o >>= 1) {
-      s1 += __shfl_down(s1, o, WAVE);
-      s2 += __shfl_down(s2, o, WAVE);
-      c += __shfl_down(c, o, WAVE);
-    }
-    if (lane == 0) {
-      smem[3 * (i * waves + wave)] = s1;
-      smem[3 * (i * waves + wave) + 1] = s2;
-      smem[3 * (i * waves + wave) + 2] = c;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < N; ++i) {
-      double s1 = 0.0, s2 = 0.0, c = 0.0;
-      for (int w = 0; w < waves; ++w) {
-        s1 += smem[3 * (i * waves + w)];
-        s2 += smem[3 * (i * waves + w) + 1];
-        c += smem[3 * (i * waves + w) + 2];
-      }
-      if (c == 0.0) {
-        out[i] = moments_zero();
-      } else {
-        const double mean = s1 / c;
-        out[i] = Moments{c, scale[i], mean, fmax(s2 - s1 * mean, 0.0)};
-      }
-    }
-  }
-  __syncthreads();
+  block_merge<N>(out, smem);
 }
 
 // f64::midpoint for finite inputs well inside the exponent range.

@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(128)
     european_kernel(EuroParams p, uint64_t seed, int64_t samples, int64_t paths, int antithetic,
                     int sobol, const uint64_t* __restrict__ dirs, uint64_t shift,
                     Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
-  __shared__ double smem[3 * 2 * 2];
+  __shared__ Moments smem[2 * 2];
   const int64_t block = blockIdx.x;
   const int j = 2 * threadIdx.x;
   const int64_t n0 = block * BLOCK_SAMPLES + j;
@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(128)
     european_scenarios_kernel(const EuroParams* __restrict__ params, int n_scen, uint64_t seed, int64_t samples,
                               int64_t paths, int antithetic, int sobol, const uint64_t* __restrict__ dirs,
                               uint64_t shift, Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
-  __shared__ double smem[3 * 1 * 2];
+  __shared__ Moments smem[1 * 2];
   const int64_t block = blockIdx.x;
   const int j = 2 * threadIdx.x;
   const int64_t n0 = block * BLOCK_SAMPLES + j;

@@ -60,6 +60,7 @@ void write(const Estimate& e, kg_estimate* out) {
   out->uncertainty_status = e.status;
   out->has_seed = e.has_seed;
   out->seed = e.seed;
+  out->path_standard_error = e.path_standard_error;
 }
 }  // namespace
 

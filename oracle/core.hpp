@@ -561,6 +561,9 @@ struct Estimate {
   int status = INSUFFICIENT_OBSERVATIONS;
   uint64_t seed = 0;
   int has_seed = 0;
+  // Diagnostic beside the reference's fields: the standard error of the
+  // per-path values as if they were independent (both antithetic legs).
+  double path_standard_error = 0.0;
 };
 
 // sampling_standard_error (monte_carlo.rs:1426-1452).
@@ -581,6 +584,10 @@ inline Estimate estimate(const Settings& st, const Moments& sample, const Moment
   e.price = scaled_value(sample.scale, sample.mean, log_discount);
   e.has_seed = has_seed;
   e.seed = seed;
+  if (sample.count >= 2) {
+    double dev = std::sqrt(std::max(sample.m2, 0.0) / (double)(sample.count - 1) / (double)sample.count);
+    e.path_standard_error = discounted_product(sample.scale, dev, log_discount);
+  }
   bool has_se = false;
   double se = 0.0;
   if (st.sampling == SOBOL) {

@@ -151,3 +151,79 @@ def test_python_engine_interface(gpu):
     assert unseeded.seed is not None
     with pytest.raises(k.InvalidInputError):
         k.MonteCarloVanillaEngine(path_count=1000, sampling="sobol")
+
+
+def scenarios(ctx, s, o, markets):
+    arr = (_abi.BsmMarket * len(markets))(*markets)
+    out = (_abi.Estimate * len(markets))()
+    ctx.check(_native.lib().kg_vanilla_scenarios(ctx.handle, C.byref(s), C.byref(o), arr, len(markets), out))
+    return list(out)
+
+
+# Deep in-the-money calls at tiny volatility: the path values spread by about
+# 50 * sigma around 50, so the block M2 must come from deviations (a raw
+# sum-of-squares form loses 63 % of it at sigma = 1e-8). Each value carries
+# about 50 * 2^-53 of device-vs-glibc rounding; the standard error then agrees
+# to roughly (rounding / spread) / sqrt(paths), below 1e-9 for every case here.
+@pytest.mark.parametrize("sigma", [1e-4, 1e-6, 1e-8])
+def test_standard_error_of_a_low_variance_payoff_matches_the_oracle(gpu, oracle, sigma):
+    o, m = option(strike=50.0), market(sigma=sigma)
+    for paths, seed in ((65536, 1729), (1000, 42), (300_001, 7)):
+        s = settings(paths, seed)
+        got, ref = gpu_price(gpu, s, o, m), oracle.vanilla_price(s, o, m, workers=8)
+        assert got.price == pytest.approx(ref.price, rel=1e-12)
+        assert got.uncertainty_status == ref.uncertainty_status == _abi.STATUS_AVAILABLE
+        assert got.standard_error == pytest.approx(ref.standard_error, rel=1e-9)
+        assert got.path_standard_error == pytest.approx(ref.path_standard_error, rel=1e-9)
+        assert got.path_standard_error == pytest.approx(got.standard_error, rel=1e-15)
+        assert got.ci_upper - got.ci_lower == pytest.approx(ref.ci_upper - ref.ci_lower, rel=1e-9)
+
+
+def test_antithetic_pair_standard_error_at_small_volatility(gpu, oracle):
+    """Antithetic pair means cancel the first-order term, so the pair spread is
+    the second-order one, 50 * sigma^2 / 2: at sigma = 1e-4 it is 2.5e-7 and the
+    pair SE agrees to 1e-9; at sigma = 1e-6 it is 2.5e-11 against about 5e-15
+    of per-value rounding, which bounds the agreement near 1e-5."""
+    o = option(strike=50.0)
+    for sigma, rel in ((1e-4, 1e-9), (1e-6, 1e-5)):
+        m = market(sigma=sigma)
+        s = settings(131072, 1729, antithetic=True)
+        got, ref = gpu_price(gpu, s, o, m), oracle.vanilla_price(s, o, m, workers=8)
+        assert got.uncertainty_status == ref.uncertainty_status == _abi.STATUS_AVAILABLE
+        assert got.standard_error == pytest.approx(ref.standard_error, rel=rel)
+
+
+def test_zero_variance_payoff_reports_zero_observed_variance(gpu, oracle):
+    """A call struck far above any path (sigma 1e-4, strike 200) pays zero on
+    every path: the status is ZERO_OBSERVED_VARIANCE with no standard error, as
+    in the reference (monte_carlo.rs:938-960), for plain and antithetic runs."""
+    o, m = option(strike=200.0), market(sigma=1e-4)
+    for antithetic in (False, True):
+        s = settings(65536, 1729, antithetic=antithetic)
+        got, ref = gpu_price(gpu, s, o, m), oracle.vanilla_price(s, o, m, workers=8)
+        assert ref.uncertainty_status == _abi.STATUS_ZERO_OBSERVED_VARIANCE
+        assert got.uncertainty_status == ref.uncertainty_status
+        assert got.has_standard_error == ref.has_standard_error == 0
+        assert got.price == ref.price == 0.0
+
+
+@pytest.mark.parametrize("antithetic", [False, True])
+def test_scenario_lanes_match_the_oracle_at_small_volatility(gpu, oracle, antithetic):
+    """kg_vanilla_scenarios: each lane's price equals the oracle's single
+    pricing of that market on the same seed, including the low-variance and
+    zero-variance lanes; without antithetic pairs the lane's per-path standard
+    error is the reference's standard error and is checked too (with pairs the
+    reference keeps no variance of the legs, monte_carlo.rs:1774)."""
+    o = option(strike=50.0)
+    markets = [market(sigma=1e-4), market(sigma=1e-6), market(sigma=1e-8), market(sigma=0.2),
+               market(spot=20.0, sigma=1e-4)]
+    s = settings(65536, 1729, antithetic=antithetic)
+    lanes = scenarios(gpu, s, o, markets)
+    for lane, m in zip(lanes, markets):
+        ref = oracle.vanilla_price(s, o, m, workers=8)
+        assert lane.price == pytest.approx(ref.price, rel=1e-11, abs=0.0)
+        if not antithetic:
+            assert lane.path_standard_error == pytest.approx(ref.standard_error, rel=1e-9, abs=0.0)
+        single = gpu_price(gpu, s, o, m)
+        assert lane.price == single.price
+        assert lane.path_standard_error == single.path_standard_error
