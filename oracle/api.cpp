@@ -315,6 +315,28 @@ int oracle_lv_lsmc(const kg_settings* s, const kg_american_terms* o, const kg_am
   });
 }
 
+// European payoff under local volatility on the LSMC engine's log-Euler
+// stepping (diagnostic of the scheme; kind 0 vanilla, 1 cash digital).
+int oracle_lv_european(const kg_settings* s, const kg_american_terms* o, const kg_american_model* m, int kind,
+                       double cash, int workers, kg_estimate* out) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    write(lv_european(st, lv_setup(*o, *m), kind, cash), out);
+  });
+}
+
+// Terminal spot and variance estimates of the Heston scheme (diagnostic).
+int oracle_heston_terminal(const kg_settings* s, const kg_heston_terms* o, const kg_heston_market* m,
+                           int workers, kg_estimate* out) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    Estimate e[2];
+    heston_terminal(st, *o, *m, context(), e);
+    write(e[0], &out[0]);
+    write(e[1], &out[1]);
+  });
+}
+
 // Plan of an autocallable note as the oracle builds it, for plan parity tests:
 // per step (drift, diffusion, discount, event, observe), and bridge flag.
 int oracle_autocallable_plan(const kg_autocall_terms* t, const kg_bsm_market* m, double* steps,

@@ -48,15 +48,21 @@ inline HestonPlan heston_plan(const kg_heston_terms& o, const kg_heston_market& 
   return p;
 }
 
+// One full-truncation Euler step of log-spot x and variance v on the normal
+// pair (z[0] drives the variance, z[1] the independent part of the spot).
+inline void heston_step(const HestonPlan& p, const double* z, double& x, double& v) {
+  double vp = std::max(v, 0.0);
+  double sq = std::sqrt(vp) * p.sdt;
+  double zs = std::fma(p.rho, z[0], p.rho_c * z[1]);
+  x += std::fma(sq, zs, std::fma(-p.hdt, vp, p.a));
+  v += std::fma(p.xi * sq, z[0], std::fma(-p.kdt, vp, p.kth));
+}
+
 // One path over 2*steps normals; returns the discounted payoff.
 inline double heston_path(const HestonPlan& p, const double* z) {
   double x = 0.0, v = p.v0, sum = 0.0;
   for (int k = 0; k < p.steps; ++k) {
-    double vp = std::max(v, 0.0);
-    double sq = std::sqrt(vp) * p.sdt;
-    double zs = std::fma(p.rho, z[2 * k], p.rho_c * z[2 * k + 1]);
-    x += std::fma(sq, zs, std::fma(-p.hdt, vp, p.a));
-    v += std::fma(p.xi * sq, z[2 * k], std::fma(-p.kdt, vp, p.kth));
+    heston_step(p, z + 2 * k, x, v);
     if (p.payoff == KG_HESTON_ASIAN_ARITHMETIC)
       sum += std::exp(x);
     else if (p.payoff == KG_HESTON_ASIAN_GEOMETRIC)
@@ -83,6 +89,24 @@ inline Estimate heston_price(const Settings& st, const kg_heston_terms& o, const
       st, 2 * (size_t)p.steps, true, cx,
       [&](const double* z, double* out) { out[0] = heston_path(p, z); }, total, pairs);
   return estimate(st, total[0], pairs[0], 0.0, false, true, st.seed);
+}
+
+// Diagnostic of the scheme: estimates of E[S_T] and E[v_T] over the same
+// paths heston_price draws (the terminal spot and variance of each path).
+inline void heston_terminal(const Settings& st, const kg_heston_terms& o, const kg_heston_market& m,
+                            const Context& cx, Estimate out[2]) {
+  HestonPlan p = heston_plan(o, m);
+  Moments total[2], pairs[2];
+  path_samples<2>(
+      st, 2 * (size_t)p.steps, true, cx,
+      [&](const double* z, double* res) {
+        double x = 0.0, v = p.v0;
+        for (int k = 0; k < p.steps; ++k) heston_step(p, z + 2 * k, x, v);
+        res[0] = p.spot * std::exp(x);
+        res[1] = v;
+      },
+      total, pairs);
+  for (int i = 0; i < 2; ++i) out[i] = estimate(st, total[i], pairs[i], 0.0, false, true, st.seed);
 }
 
 }  // namespace oracle

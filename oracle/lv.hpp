@@ -296,4 +296,22 @@ inline double lv_price(const Settings& st, const LvSetup& L, int basis, const st
   return std::max(scaled_value(total[0].scale, total[0].mean, 0.0), payoff(L.spot));
 }
 
+// Diagnostic of the log-Euler local-volatility scheme (lv_step, the stepping
+// of lv_lsmc and of the GPU engine) on a European payoff: kind 0 pays
+// max(sign (S_T - K), 0), kind 1 pays `cash` when sign (S_T - K) > 0.
+inline Estimate lv_european(const Settings& st, const LvSetup& L, int kind, double cash) {
+  Moments total[1], pairs[1];
+  path_samples_skipping<1>(
+      st, L.steps, true, Context{},
+      [&](SkippingPath& draws, double* out) {
+        double x = 0.0;
+        for (int step = 0; step < L.steps; ++step) x = lv_step(L, step, x, draws.normal());
+        const double d = L.sign * (L.spot * std::exp(x) - L.strike);
+        const double pay = kind == 0 ? std::max(d, 0.0) : (d > 0.0 ? cash : 0.0);
+        out[0] = scaled_value(1.0, pay, -L.rate * L.time);
+      },
+      total, pairs);
+  return estimate(st, total[0], pairs[0], 0.0, false, true, st.seed);
+}
+
 }  // namespace oracle

@@ -61,6 +61,10 @@ def lib():
                                               P(C.c_double)]
         handle.oracle_lv_lsmc.argtypes = [P(_abi.Settings), P(_abi.AmericanTerms), P(_abi.AmericanModel), C.c_int,
                                           C.c_int64, C.c_int, P(C.c_double), P(C.c_double)]
+        handle.oracle_lv_european.argtypes = [P(_abi.Settings), P(_abi.AmericanTerms), P(_abi.AmericanModel),
+                                              C.c_int, C.c_double, C.c_int, P(_abi.Estimate)]
+        handle.oracle_heston_terminal.argtypes = [
+            P(_abi.Settings), P(_abi.HestonTerms), P(_abi.HestonMarket), C.c_int, P(_abi.Estimate)]
         handle.oracle_autocallable_plan.argtypes = [
             P(_abi.AutocallTerms), P(_abi.BsmMarket), P(C.c_double), C.c_size_t,
             P(C.c_size_t), P(C.c_int)]
@@ -236,3 +240,19 @@ def lv_lsmc(settings, terms, model, basis, training_paths=0, workers: int = 1):
     _check(lib().oracle_lv_lsmc(C.byref(settings), C.byref(terms), C.byref(model), basis, training_paths,
                                 workers, C.byref(a), C.byref(b)))
     return a.value, b.value
+
+
+def lv_european(settings, terms, model, kind=0, cash=1.0, workers: int = 1):
+    """European payoff (kind 0 vanilla, 1 cash digital paying `cash`) under
+    local volatility on the LSMC engine's log-Euler stepping."""
+    out = _abi.Estimate()
+    _check(lib().oracle_lv_european(C.byref(settings), C.byref(terms), C.byref(model), kind, cash, workers,
+                                    C.byref(out)))
+    return out
+
+
+def heston_terminal(settings, terms, market, workers: int = 1):
+    """Estimates of E[S_T] and E[v_T] on the Heston scheme's paths."""
+    out = (_abi.Estimate * 2)()
+    _check(lib().oracle_heston_terminal(C.byref(settings), C.byref(terms), C.byref(market), workers, out))
+    return out[0], out[1]

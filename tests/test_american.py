@@ -3,6 +3,7 @@ LSMC prices and its finite-difference references (CPU); the GPU engine
 checked against the oracle and the FD judge."""
 
 import ctypes as C
+import math
 import pytest
 
 from conftest import END, START, fd_richardson, golden, us
@@ -52,6 +53,49 @@ def test_local_vol_lsmc_with_a_flat_surface_is_the_reference_lsmc(oracle):
     lv, _ = oracle.lv_lsmc(settings(65536), terms, american_model(ctx), _abi.BASIS_QUADRATIC, workers=8)
     ref = golden("calibration_american.json")["runs"][0]["samples"][0]
     assert lv == pytest.approx(ref["price"], rel=1e-9)
+
+
+def two_by_two_model():
+    """The reference's fd_local_vol surface (cases/fd_local_vol.json): sigma 0.30
+    and 0.15 at spots 80 and 120 on the valuation date, 0.35 and 0.20 a year on."""
+    from kiyosi_amd.market import LocalVolatilityParameters, PricingContext
+    two = LocalVolatilitySurface([START, END], [80.0, 120.0], [[0.30, 0.15], [0.35, 0.20]])
+    ctx = PricingContext(model_parameters=LocalVolatilityParameters(risk_free_rate=0.05, dividend_yield=0.02,
+                                                                    surface=two),
+                         spot_price=100.0, valuation_time=START)
+    return american_model(ctx)
+
+
+def test_local_vol_stepping_converges_to_the_reference_fd_prices(oracle):
+    """The LSMC engine's log-Euler local-volatility steps (the oracle's lv_step,
+    which the GPU engine matches at 1e-8) price a European call and a cash
+    digital under the 2x2 surface; Richardson over 64 and 128 steps meets the
+    reference's FD prices (references/fd_local_vol.json) within 3 standard
+    errors of the extrapolated estimate. Reading the surface with time and
+    spot swapped prices the call near 15.6, a sigma for sigma^2 in the drift
+    moves it by about 5, and a surface read at the wrong time row by more than
+    1: each fails by many standard errors."""
+    refs = {c["id"]: c["value"] for c in golden("references.json")["fd_local_vol"]}
+    model = two_by_two_model()
+    s = settings(1 << 20, seed=1729, antithetic=True)
+    for kind, cash, key in ((0, 1.0, "fd-local-vol-european-call"), (1, 10.0, "fd-local-vol-cash-digital-call")):
+        coarse, fine = (oracle.lv_european(s, _abi.AmericanTerms(1, n, 100.0, us(START), us(END)), model, kind, cash,
+                                           workers=8) for n in (64, 128))
+        extrapolated = 2 * fine.price - coarse.price
+        se = (4 * fine.standard_error**2 + coarse.standard_error**2) ** 0.5
+        assert abs(extrapolated - refs[key]) < 3 * se, (key, extrapolated, refs[key], se)
+        assert abs(fine.price - refs[key]) < 4 * fine.standard_error + 0.01
+
+
+def test_local_vol_stepping_keeps_the_spot_martingale(oracle):
+    """Each log-Euler step is Gaussian in log-spot given its start, with drift
+    (r - q - sigma^2/2) dt, so E[S_T] = S0 e^{(r-q)T} exactly: a call struck at
+    1e-9 is worth S0 e^{-qT} - K e^{-rT} within its standard error."""
+    est = oracle.lv_european(settings(1 << 19, seed=5), _abi.AmericanTerms(1, 50, 1e-9, us(START), us(END)),
+                             two_by_two_model(), 0, 1.0, workers=8)
+    exact = 100 * math.exp(-0.02) - 1e-9 * math.exp(-0.05)
+    assert abs(est.price - exact) < 4 * est.standard_error
+    assert est.standard_error < 0.05
 
 
 def test_out_of_sample_is_below_in_sample_and_near_fd(oracle):
