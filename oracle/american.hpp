@@ -1,6 +1,7 @@
 // Test infrastructure, not product code: FP64 CPU restatement of kiyosi's
-// least-squares Monte Carlo for American options (one strike, BSM, the
-// forward path matrix), the checker of the HIP LSMC kernels.
+// least-squares Monte Carlo for American options (one strike, BSM; forward,
+// reverse and anchored paths; pseudorandom or Sobol draws), the checker of
+// the HIP LSMC kernels.
 // Citations are to crates/kiyosi/src/engines/monte_carlo/american.rs.
 #pragma once
 
@@ -101,12 +102,71 @@ inline SpotsAt spots_at(const AmericanSetup& a, int step) {
   return s;
 }
 
-// american_scenarios::<1, 1> (:955-1455) in forward mode with the full path
-// matrix (the normalised case; windows and replay change no value). Returns
-// the in-sample price; `policy` receives the per-date coefficients
-// (valid flag + 3 values per date).
+// Paths of american_scenarios (:1184-1219): the normalised case stores one
+// Brownian coordinate per sample (the antithetic leg is its negative), in
+// forward mode as the running sum of the path's normals (ExercisePaths::fill,
+// :365-447) or, when samples x intervals exceeds 2^25 path values
+// (PREFERRED_PATH_WORKSPACE / 8, :22, :1218-1219), in reverse mode: the
+// terminal coordinate first and each earlier one from the Brownian bridge
+// (reverse_row, :278-316), step-major within a block. The anchored case
+// (spot or drift too large for exp of a coordinate, :1149-1157) stores each
+// leg's spot, stepped forward by exp(diffusion z + drift) (advance and
+// evolve_spot, :355-362, :546-552).
+constexpr size_t REVERSE_PATH_VALUES = (size_t(256) << 20) / sizeof(double);
+
+inline bool american_normalized(double spot, double drift, double diffusion, int steps) {
+  return std::fabs(std::log(spot)) + (std::fabs(drift) + 10.0 * std::fabs(diffusion)) * steps < 600.0;
+}
+
+inline bool american_reverse(const Settings& st, double spot, double drift, double diffusion, int steps) {
+  return american_normalized(spot, drift, diffusion, steps) && st.sample_count() * (size_t)steps > REVERSE_PATH_VALUES;
+}
+
+// reverse_row (:278-316) for one block: moves `coords` (the block's samples)
+// from the coordinate after step + 2 intervals to the one after step + 1,
+// or draws the terminal coordinate when step is the last.
+inline void reverse_row(const Settings& st, const SobolSequence* seq, size_t block, int intervals, int step,
+                        std::vector<double>& coords, NormalDraws& draws) {
+  const int elapsed = step + 1;
+  const int dimension = intervals - elapsed;
+  double weight, deviation;
+  if (dimension == 0) {
+    weight = 0.0;
+    deviation = std::sqrt((double)elapsed);
+  } else {
+    const double fraction = (double)elapsed / (double)(elapsed + 1);
+    weight = fraction;
+    deviation = std::sqrt(fraction);
+  }
+  const size_t begin = block * BLOCK;
+  if (st.sampling == SOBOL) {
+    // dimension_normals (sobol.rs:155-174): the points begin.. of one
+    // dimension in Gray-code order.
+    uint64_t bits = seq->shifts[dimension];
+    uint64_t gray = begin ^ (begin >> 1);
+    while (gray) {
+      bits ^= seq->directions[(size_t)__builtin_ctzll(gray) * seq->dims + dimension];
+      gray &= gray - 1;
+    }
+    for (size_t i = 0; i < coords.size(); ++i) {
+      const double normal = normal_from_bits(bits);
+      bits ^= seq->directions[(size_t)__builtin_ctzll(begin + i + 1) * seq->dims + dimension];
+      coords[i] = std::fma(weight, coords[i], deviation * normal);
+    }
+  } else {
+    for (double& c : coords) c = std::fma(weight, c, deviation * draws.normal());
+  }
+}
+
+// american_scenarios::<1, 1> (:955-1455) for one strike under BSM, in the
+// mode the reference picks: forward with the full path matrix (windows and
+// replay change no value), reverse, or anchored. Returns the in-sample price;
+// `policy` receives the per-date coefficients (valid flag + 3 values per
+// date).
+// `mode` -1 picks the reference's mode; 0 and 1 force forward and reverse
+// paths (a diagnostic for the tests: the reference decides by size alone).
 inline double american_lsmc(const Settings& st, const kg_european_terms& o, const kg_bsm_market& m,
-                            int steps, const Context& cx, std::vector<double>* policy) {
+                            int steps, const Context& cx, std::vector<double>* policy, int mode = -1) {
   const double sign = o.option_type;
   const double immediate = std::max(sign * (m.spot - o.strike), 0.0);
   AmericanSetup a = american_setup(o, m, steps);
@@ -115,45 +175,79 @@ inline double american_lsmc(const Settings& st, const kg_european_terms& o, cons
   if (m.volatility == 0.0) throw Failure(UNSUPPORTED, "zero volatility is deterministic");
   if (o.option_type > 0 && m.dividend <= 0.0 && m.rate >= 0.0)
     throw Failure(UNSUPPORTED, "calls without dividends are priced as Europeans");
-  bool normalized = std::fabs(std::log(m.spot)) + (std::fabs(a.drift) + 10.0 * std::fabs(a.diffusion)) * steps < 600.0;
-  if (!normalized) throw Failure(UNSUPPORTED, "anchored spot paths are not restated");
+  const bool normalized = american_normalized(m.spot, a.drift, a.diffusion, steps);
+  const bool reverse =
+      mode < 0 ? american_reverse(st, m.spot, a.drift, a.diffusion, steps) : (normalized && mode == 1);
   const size_t paths = st.path_count;
   const size_t samples = st.sample_count();
   const size_t blocks = st.blocks();
   SobolSequence seq;
   if (st.sampling == SOBOL) seq = sobol_new(steps, st.seed, cx.table, cx.table_len);
-  // Coordinates: per block, [step][lane] with lanes = samples of the block.
+  auto paired_in = [&](size_t begin, size_t end) -> size_t {
+    if (!st.antithetic) return 0;
+    const size_t top = std::min(end, paths / 2);
+    return top > begin ? top - begin : 0;
+  };
+  // Coordinates per block: forward [step][lane]; reverse [lane], one row;
+  // anchored [step][slot] with a paired sample's two legs side by side.
   std::vector<std::vector<double>> coords(blocks);
+  std::vector<NormalDraws> reverse_draws(blocks);
   std::vector<std::vector<double>> spots(blocks), cash(blocks);
   auto payoff = [&](double s) { return std::max(sign * (s - a.strike), 0.0); };
+  auto advance = [&](double& value, double normal) {
+    value *= std::exp(std::fma(a.diffusion, normal, a.drift));
+    if (!std::isfinite(value) || value <= 0.0) unstable();
+    return value;
+  };
   SpotsAt terminal = spots_at(a, steps - 1);
   run_blocks(blocks, st.workers, [&](size_t b) {
     size_t begin = b * BLOCK, end = std::min(begin + BLOCK, samples);
     size_t count = end - begin;
-    coords[b].assign(count * steps, 0.0);
-    SobolCursor cursor;
-    NormalDraws d;
-    d.gen = Pcg32(st.seed, b);
-    if (st.sampling == SOBOL) {
-      cursor = SobolCursor(&seq, begin);
-      d.cursor = &cursor;
-    }
-    for (size_t lane = 0; lane < count; ++lane) {
-      double w = 0.0;
-      for (int k = 0; k < steps; ++k) {
-        w += d.normal();
-        coords[b][k * count + lane] = w;
+    const size_t paired = paired_in(begin, end);
+    if (reverse) {
+      coords[b].assign(count, 0.0);
+      reverse_draws[b].gen = Pcg32(st.seed, b);
+      reverse_row(st, &seq, b, steps, steps - 1, coords[b], reverse_draws[b]);
+    } else {
+      const size_t width = normalized ? count : count + paired;
+      coords[b].assign(width * steps, 0.0);
+      SobolCursor cursor;
+      NormalDraws d;
+      d.gen = Pcg32(st.seed, b);
+      if (st.sampling == SOBOL) {
+        cursor = SobolCursor(&seq, begin);
+        d.cursor = &cursor;
       }
-      if (d.cursor) cursor.next_path();
+      size_t slot = 0;
+      for (size_t lane = 0; lane < count; ++lane) {
+        if (normalized) {
+          double w = 0.0;
+          for (int k = 0; k < steps; ++k) {
+            w += d.normal();
+            coords[b][k * width + lane] = w;
+          }
+        } else {
+          const bool two = lane < paired;
+          double pos = m.spot, neg = m.spot;
+          for (int k = 0; k < steps; ++k) {
+            const double normal = d.normal();
+            coords[b][k * width + slot] = advance(pos, normal);
+            if (two) coords[b][k * width + slot + 1] = advance(neg, -normal);
+          }
+          slot += two ? 2 : 1;
+        }
+        if (d.cursor) cursor.next_path();
+      }
     }
-    // visit_row: paired samples visit +W then -W.
-    size_t paired = st.antithetic ? (std::min(end, paths / 2) > begin ? std::min(end, paths / 2) - begin : 0) : 0;
-    for (size_t lane = 0; lane < count; ++lane) {
-      double w = coords[b][(steps - 1) * count + lane];
-      double s = terminal(w);
+    // visit_row: paired samples visit +W then -W; anchored rows hold both legs.
+    const size_t row = reverse ? 0 : (size_t)(steps - 1);
+    const size_t width = reverse || normalized ? count : count + paired;
+    for (size_t lane = 0; lane < width; ++lane) {
+      double w = coords[b][row * width + lane];
+      double s = normalized ? terminal(w) : w;
       spots[b].push_back(s);
       cash[b].push_back(payoff(s));
-      if (lane < paired) {
+      if (normalized && lane < paired) {
         double s2 = terminal(-w);
         spots[b].push_back(s2);
         cash[b].push_back(payoff(s2));
@@ -178,15 +272,16 @@ inline double american_lsmc(const Settings& st, const kg_european_terms& o, cons
     run_blocks(blocks, st.workers, [&](size_t b) {
       size_t begin = b * BLOCK, end = std::min(begin + BLOCK, samples);
       size_t count = end - begin;
-      size_t paired = st.antithetic ? (std::min(end, paths / 2) > begin ? std::min(end, paths / 2) - begin : 0) : 0;
+      const size_t paired = paired_in(begin, end);
+      if (reverse) reverse_row(st, &seq, b, steps, step, coords[b], reverse_draws[b]);
       double mat[3][4] = {};
       size_t state = 0;
-      auto visit = [&](double w) {
+      auto visit = [&](double v) {
         double& s = spots[b][state];
         double& c = cash[b][state];
         ++state;
         apply(s, c);
-        s = at(w);
+        s = normalized ? at(v) : v;
         c *= a.discount;
         if (!std::isfinite(c)) unstable();
         if (payoff(s) > 0.0) {
@@ -202,10 +297,16 @@ inline double american_lsmc(const Settings& st, const kg_european_terms& o, cons
           mat[2][3] += sq * t;
         }
       };
-      for (size_t lane = 0; lane < count; ++lane) {
-        double w = coords[b][step * count + lane];
-        visit(w);
-        if (lane < paired) visit(-w);
+      if (normalized) {
+        const size_t row = reverse ? 0 : (size_t)step;
+        for (size_t lane = 0; lane < count; ++lane) {
+          double w = coords[b][row * count + lane];
+          visit(w);
+          if (lane < paired) visit(-w);
+        }
+      } else {
+        const size_t width = count + paired;
+        for (size_t slot = 0; slot < width; ++slot) visit(coords[b][(size_t)step * width + slot]);
       }
       for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 4; ++c) parts[b][r * 4 + c] = mat[r][c];
@@ -238,6 +339,42 @@ inline double american_lsmc(const Settings& st, const kg_european_terms& o, cons
       mean.add(c, false);
     }
   return std::max(scaled_value(mean.scale, mean.mean, a.log_discount), immediate);
+}
+
+// Diagnostic of the path generation: the normalised Brownian coordinates
+// after 1..steps intervals for every sample, out[step * samples + sample],
+// drawn in forward or reverse mode exactly as american_lsmc draws them.
+inline void american_coordinates(const Settings& st, int steps, bool reverse, const Context& cx, double* out) {
+  const size_t samples = st.sample_count();
+  const size_t blocks = st.blocks();
+  SobolSequence seq;
+  if (st.sampling == SOBOL) seq = sobol_new(steps, st.seed, cx.table, cx.table_len);
+  run_blocks(blocks, st.workers, [&](size_t b) {
+    const size_t begin = b * BLOCK, count = std::min(begin + BLOCK, samples) - begin;
+    NormalDraws d;
+    d.gen = Pcg32(st.seed, b);
+    if (reverse) {
+      std::vector<double> coords(count, 0.0);
+      for (int step = steps - 1; step >= 0; --step) {
+        reverse_row(st, &seq, b, steps, step, coords, d);
+        for (size_t i = 0; i < count; ++i) out[(size_t)step * samples + begin + i] = coords[i];
+      }
+      return;
+    }
+    SobolCursor cursor;
+    if (st.sampling == SOBOL) {
+      cursor = SobolCursor(&seq, begin);
+      d.cursor = &cursor;
+    }
+    for (size_t i = 0; i < count; ++i) {
+      double w = 0.0;
+      for (int k = 0; k < steps; ++k) {
+        w += d.normal();
+        out[(size_t)k * samples + begin + i] = w;
+      }
+      if (d.cursor) cursor.next_path();
+    }
+  });
 }
 
 // price_american_out_of_sample (:574-637): fit on training paths with seed

@@ -247,12 +247,20 @@ int oracle_accumulator_price(const kg_settings* s, const kg_accumulator_terms* o
 // In-sample LSMC price (american_scenarios) and the fitted policy
 // (steps-1 rows of: valid, c0, c1, c2).
 int oracle_american_lsmc(const kg_settings* s, const kg_european_terms* o, const kg_bsm_market* m,
-                         int steps, int workers, double* price, double* policy) {
+                         int steps, int workers, int mode, double* price, double* policy) {
   return guarded([&] {
     Settings st = to_settings(s, workers);
     std::vector<double> pol;
-    *price = american_lsmc(st, *o, *m, steps, context(), &pol);
+    *price = american_lsmc(st, *o, *m, steps, context(), &pol, mode);
     if (policy) std::copy(pol.begin(), pol.end(), policy);
+  });
+}
+
+// Brownian coordinates of the LSMC paths, [step][sample] (diagnostic).
+int oracle_american_coordinates(const kg_settings* s, int steps, int reverse, int workers, double* out) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    american_coordinates(st, steps, reverse != 0, context(), out);
   });
 }
 

@@ -53,7 +53,8 @@ def lib():
             getattr(handle, f"oracle_{name}_price").argtypes = [
                 P(_abi.Settings), P(terms), P(_abi.BsmMarket), C.c_int, P(_abi.Estimate)]
         handle.oracle_american_lsmc.argtypes = [P(_abi.Settings), P(_abi.EuropeanTerms), P(_abi.BsmMarket),
-                                                 C.c_int, C.c_int, P(C.c_double), P(C.c_double)]
+                                                 C.c_int, C.c_int, C.c_int, P(C.c_double), P(C.c_double)]
+        handle.oracle_american_coordinates.argtypes = [P(_abi.Settings), C.c_int, C.c_int, C.c_int, P(C.c_double)]
         handle.oracle_american_out_of_sample.argtypes = [P(_abi.Settings), C.c_int64, P(_abi.EuropeanTerms),
                                                          P(_abi.BsmMarket), C.c_int, C.c_int, P(C.c_double)]
         handle.oracle_fd_american.argtypes = [P(_abi.LocalVolSurface), C.c_int64, C.c_double, C.c_double,
@@ -210,13 +211,24 @@ def accumulator_price(settings, terms, market, workers: int = 1):
     return out
 
 
-def american_lsmc(settings, terms, market, steps, workers: int = 1):
-    """In-sample LSMC price and the fitted policy, shape (steps - 1, 4)."""
+def american_lsmc(settings, terms, market, steps, workers: int = 1, mode: int = -1):
+    """In-sample LSMC price and the fitted policy, shape (steps - 1, 4).
+    mode -1 picks forward or reverse paths as the reference does; 0 and 1
+    force them."""
     price = C.c_double()
     policy = np.zeros((steps - 1, 4))
-    _check(lib().oracle_american_lsmc(C.byref(settings), C.byref(terms), C.byref(market), steps, workers,
+    _check(lib().oracle_american_lsmc(C.byref(settings), C.byref(terms), C.byref(market), steps, workers, mode,
                                       C.byref(price), _ptr(policy, C.c_double)))
     return price.value, policy
+
+
+def american_coordinates(settings, steps, reverse, workers: int = 1):
+    """Normalised Brownian coordinates of the LSMC paths, shape (steps, samples)."""
+    samples = (settings.path_count + 1) // 2 if settings.antithetic else settings.path_count
+    out = np.zeros((steps, samples))
+    _check(lib().oracle_american_coordinates(C.byref(settings), steps, int(reverse), workers,
+                                             _ptr(out, C.c_double)))
+    return out
 
 
 def american_out_of_sample(settings, training_paths, terms, market, steps, workers: int = 1):

@@ -4,6 +4,8 @@ checked against the oracle and the FD judge."""
 
 import ctypes as C
 import math
+
+import numpy as np
 import pytest
 
 from conftest import END, START, fd_richardson, golden, us
@@ -19,8 +21,8 @@ def put(strike=100.0):
     return _abi.EuropeanTerms(-1, 0, strike, us(START), us(END))
 
 
-def settings(paths, seed=1009, antithetic=False, precision=_abi.FP64, large=False):
-    return _abi.Settings(paths, seed, 1, _abi.PSEUDO_RANDOM, int(antithetic), precision, 0, int(large))
+def settings(paths, seed=1009, antithetic=False, precision=_abi.FP64, large=False, sampling=_abi.PSEUDO_RANDOM):
+    return _abi.Settings(paths, seed, 1, sampling, int(antithetic), precision, 0, int(large))
 
 
 def test_lsmc_calibration_vectors(oracle):
@@ -96,6 +98,66 @@ def test_local_vol_stepping_keeps_the_spot_martingale(oracle):
     exact = 100 * math.exp(-0.02) - 1e-9 * math.exp(-0.05)
     assert abs(est.price - exact) < 4 * est.standard_error
     assert est.standard_error < 0.05
+
+
+@pytest.mark.parametrize("sampling", [_abi.PSEUDO_RANDOM, _abi.SOBOL])
+@pytest.mark.parametrize("reverse", [False, True])
+def test_lsmc_paths_have_brownian_covariance(oracle, sampling, reverse):
+    """The normalised coordinates after k intervals, drawn forward (running
+    sums) or in reverse (terminal first, then the Brownian bridge back in
+    time, american.rs:278-316), have E[W_k] = 0, E[W_k W_j] = min(k, j) and
+    uncorrelated unit increments. A wrong bridge weight, a deviation without
+    its square root or an off-by-one in the elapsed count breaks the
+    variances by several percent."""
+    steps = 20
+    w = oracle.american_coordinates(settings(1 << 16, seed=42, sampling=sampling), steps, reverse, workers=8)
+    n = w.shape[1]
+    k = np.arange(1, steps + 1)
+    for j in (0, 9, steps - 1):
+        prod = w * w[j]
+        se = prod.std(axis=1) / np.sqrt(n)
+        assert np.all(np.abs(prod.mean(axis=1) - np.minimum(k, j + 1)) < 5 * se + 1e-12), j
+    inc = np.diff(np.vstack([np.zeros(n), w]), axis=0)
+    assert np.all(np.abs(inc.mean(axis=1)) < 5 / np.sqrt(n))
+    lag = inc[1:] * inc[:-1]
+    assert np.all(np.abs(lag.mean(axis=1)) < 5 * lag.std(axis=1) / np.sqrt(n))
+
+
+def test_reverse_paths_reference_case(oracle):
+    """The reference's reverse-mode test (tests/american_monte_carlo.rs:187-208):
+    1,000,000 paths x 150 intervals (above 2^25 path values, so the engine
+    draws in reverse), seed 42, within 0.04 of the lattice price; here the
+    reference's FD price of the same put (references/fd_bsm.json) stands in
+    for its 2000-step binomial price."""
+    s = settings(1_000_000, seed=42)
+    auto, _ = oracle.american_lsmc(s, put(), bsm(), 150, workers=8)
+    forced, _ = oracle.american_lsmc(s, put(), bsm(), 150, workers=8, mode=1)
+    assert auto == forced
+    fd = {c["id"]: c["value"] for c in golden("references.json")["fd_bsm"]}["fd-bsm-american-put"]
+    assert abs(auto - fd) < 0.04
+
+
+@pytest.mark.parametrize("paths", [1 << 18, 1 << 20])
+def test_sobol_lsmc_near_the_fd_price(oracle, paths):
+    """Sobol draws in forward mode (2^18 x 50) and in reverse mode (2^20 x 50,
+    dimension intervals - k for the coordinate after k intervals) price the
+    reference's BSM put near its FD value."""
+    fd = {c["id"]: c["value"] for c in golden("references.json")["fd_bsm"]}["fd-bsm-american-put"]
+    price, _ = oracle.american_lsmc(settings(paths, seed=1729, sampling=_abi.SOBOL), put(), bsm(), 50, workers=8)
+    assert abs(price - fd) < 0.04
+
+
+def test_anchored_paths_are_scale_invariant(oracle):
+    """Spot and strike at 1e255: ln S0 plus ten deviations over 50 steps passes
+    600, so the reference steps each leg's spot directly (anchored paths,
+    american.rs:1149-1157, 355-362). The put is homogeneous of degree one in
+    (spot, strike), so on the same draws it is 1e253 times the price at 100,
+    up to rounding."""
+    for antithetic in (False, True):
+        s = settings(1 << 16, seed=7, antithetic=antithetic)
+        small, _ = oracle.american_lsmc(s, put(), bsm(), 50, workers=8)
+        big, _ = oracle.american_lsmc(s, put(1e255), bsm(1e255), 50, workers=8)
+        assert big / 1e253 == pytest.approx(small, rel=1e-11)
 
 
 def test_out_of_sample_is_below_in_sample_and_near_fd(oracle):
