@@ -3,8 +3,10 @@
 // price_american_out_of_sample :574-637, solve_quadratic :1693-1749).
 //
 // Data in HBM for one pricing (P = samples, S = exercise dates):
-//   coordinates  [S][P] (BSM: Brownian coordinate W_k per sample, the
-//                antithetic leg is -W_k; local vol: log-spot per path slot)
+//   coordinates  BSM: Brownian coordinate W_k per sample, the antithetic
+//                leg is -W_k, [S][P] in forward mode and [P] (the current
+//                date only) in reverse mode; local vol: log-spot [S][slots];
+//                anchored BSM: spot [S][slots] in FP64
 //   spot, cash   [slots] FP64, slots = P or 2P with antithetic pairs
 //   partials     [blocks][entries] per-block normal equations
 //   policy       [S-1][8] FP64: valid flag and continuation coefficients
@@ -60,10 +62,10 @@ __device__ inline void contributions(double x, double t, double* e) {
   }
 }
 
-template <bool FP32, bool LV, int BASIS>
+template <bool FP32, int KIND, int BASIS>
 __global__ void __launch_bounds__(256)
     lsmc_backward(LsmcModel m, int step, const double* __restrict__ pol, int have, int64_t samples,
-                  int64_t paths, int antithetic, const void* __restrict__ coords,
+                  int64_t paths, int antithetic, uint64_t seed, SobolRef so, void* __restrict__ coords,
                   double* __restrict__ spot, double* __restrict__ cash, double* __restrict__ partials,
                   uint32_t* __restrict__ flags) {
   constexpr int E = Entries<BASIS>::N;
@@ -77,6 +79,19 @@ __global__ void __launch_bounds__(256)
   double acc[E];
   for (int e = 0; e < E; ++e) acc[e] = 0.0;
   uint32_t bad = 0;
+  // Reverse mode: this date's coordinate from the next date's, in place.
+  double w_rev = 0.0;
+  if (KIND == PATHS_BROWNIAN && m.reverse && n < samples) {
+    const int count = (int)(samples - block * BLOCK_SAMPLES < BLOCK_SAMPLES ? samples - block * BLOCK_SAMPLES
+                                                                            : BLOCK_SAMPLES);
+    const int row = m.steps - 1 - step;
+    const double prev = FP32 ? (double)((const float*)coords)[n] : ((const double*)coords)[n];
+    w_rev = bridge_back<FP32>(prev, reverse_normal<FP32>(seed, block, j, count, row, n, so), step, m.steps);
+    if (FP32)
+      ((float*)coords)[n] = (float)w_rev;
+    else
+      ((double*)coords)[n] = w_rev;
+  }
   for (int leg = 0; leg < 2; ++leg) {
     double e[E];
     for (int k = 0; k < E; ++k) e[k] = 0.0;
@@ -94,13 +109,16 @@ __global__ void __launch_bounds__(256)
         }
       }
       double s;
-      if (LV) {
+      if (KIND == PATHS_LOCAL_VOL) {
         const double x = FP32 ? (double)((const float*)coords)[(size_t)step * slots + slot]
                               : ((const double*)coords)[(size_t)step * slots + slot];
         s = m.spot * (FP32 ? (double)f32::exp((float)x) : exp(x));
+      } else if (KIND == PATHS_ANCHORED) {
+        s = ((const double*)coords)[(size_t)step * slots + slot];
       } else {
-        const double w = FP32 ? (double)((const float*)coords)[(size_t)step * samples + n]
-                              : ((const double*)coords)[(size_t)step * samples + n];
+        const double w = m.reverse ? w_rev
+                         : FP32    ? (double)((const float*)coords)[(size_t)step * samples + n]
+                                   : ((const double*)coords)[(size_t)step * samples + n];
         s = FP32 ? bsm_spot32(m, step, (float)(leg ? -w : w)) : bsm_spot(m, step, leg ? -w : w);
       }
       c *= m.discount;
@@ -307,7 +325,8 @@ __global__ void __launch_bounds__(256)
 template <bool FP32, bool LV, int BASIS>
 __global__ void __launch_bounds__(256)
     lsmc_price(LsmcModel m, const double* __restrict__ policy, uint64_t seed, int64_t samples,
-               int64_t paths, int antithetic, Moments* __restrict__ parts, uint32_t* __restrict__ flags) {
+               int64_t paths, int antithetic, SobolRef so, Moments* __restrict__ parts,
+               uint32_t* __restrict__ flags) {
   __shared__ Moments smem[2 * 4];
   __shared__ double lv_lds[LV ? 3 * LV_LDS_NODES : 1];
   const int64_t block = blockIdx.x;
@@ -316,8 +335,8 @@ __global__ void __launch_bounds__(256)
   const bool live = n < samples;
   const bool paired = antithetic && live && n < paths / 2;
   uint32_t bad = 0;
-  Normals<FP32> z;
-  if (live) z.start(seed, (uint64_t)block, (uint64_t)j * (uint64_t)m.steps);
+  PathDraws<FP32> z;
+  if (live) z.start(seed, (uint64_t)block, j, m.steps, n, so);
   LvRows rows{};
   rows.row = -1;
   double current[2] = {m.spot, m.spot}, x[2] = {0.0, 0.0}, values[2] = {0.0, 0.0};
@@ -372,14 +391,25 @@ __global__ void __launch_bounds__(256)
 }
 
 // ------------------------------------------------------------------ host
-template <bool FP32, bool LV, int BASIS>
+static SobolRef sobol_ref(kg_context* ctx, const Sampler& s, int steps) {
+  if (!s.sobol) return SobolRef{nullptr, nullptr, 0};
+  const SobolSetup& so = sobol_setup(ctx, (uint32_t)steps, s.seed);
+  return SobolRef{(const uint64_t*)so.d_directions.ptr, (const uint64_t*)so.d_shifts.ptr, steps};
+}
+
+template <bool FP32, int KIND, int BASIS>
 static double run_lsmc(kg_context* ctx, const LsmcModel& m, const Sampler& s, double* d_policy,
                        int32_t* dates_fitted, double* immediate_out) {
   const int64_t samples = s.samples(), blocks = s.blocks();
   const int legs = s.antithetic ? 2 : 1;
   const size_t word = FP32 ? 4 : 8;
+  const bool reverse = KIND == PATHS_BROWNIAN && m.reverse;
+  // Stored values: one coordinate per sample in reverse mode, else one per
+  // date and sample (Brownian) or per date and slot (local vol, anchored).
+  const size_t values = reverse ? (size_t)samples
+                        : (size_t)m.steps * (size_t)(KIND == PATHS_BROWNIAN ? samples : samples * legs);
   // Rounded up so the FP64 arrays after the coordinates stay aligned.
-  const size_t coord_bytes = ((size_t)m.steps * (size_t)(LV ? samples * legs : samples) * word + 255) & ~size_t(255);
+  const size_t coord_bytes = (values * word + 255) & ~size_t(255);
   constexpr int E = Entries<BASIS>::N;
   unsigned char* base = (unsigned char*)ctx->scratch.reserve(coord_bytes + 2 * (size_t)samples * legs * 8 +
                                                              (size_t)blocks * E * 8 + 1024);
@@ -387,17 +417,22 @@ static double run_lsmc(kg_context* ctx, const LsmcModel& m, const Sampler& s, do
   double* spot = (double*)(base + coord_bytes);
   double* cash = spot + samples * legs;
   double* partials = cash + samples * legs;
+  const SobolRef so = sobol_ref(ctx, s, m.steps);
   uint32_t* flags = reset_flags(ctx);
-  hipLaunchKernelGGL((lsmc_generate<FP32, LV>), dim3(blocks), dim3(256), 0, ctx->stream, m, s.seed, samples,
-                     s.path_count, (int)s.antithetic, coords, spot, cash);
+  if (reverse)
+    hipLaunchKernelGGL((lsmc_reverse_start<FP32>), dim3(blocks), dim3(256), 0, ctx->stream, m, s.seed, samples,
+                       s.path_count, (int)s.antithetic, so, coords, spot, cash);
+  else
+    hipLaunchKernelGGL((lsmc_generate<FP32, KIND>), dim3(blocks), dim3(256), 0, ctx->stream, m, s.seed, samples,
+                       s.path_count, (int)s.antithetic, so, coords, spot, cash, flags);
   KG_HIP(hipGetLastError());
   const size_t lds = BASIS == KG_BASIS_QUADRATIC ? (size_t)E * 512 * 8 : 0;
   int have = 0;
   int fitted = 0;
   for (int step = m.steps - 2; step >= 0; --step) {
-    hipLaunchKernelGGL((lsmc_backward<FP32, LV, BASIS>), dim3(blocks), dim3(256), lds, ctx->stream, m, step,
-                       d_policy + (size_t)(step + 1) * 8, have, samples, s.path_count, (int)s.antithetic,
-                       coords, spot, cash, partials, flags);
+    hipLaunchKernelGGL((lsmc_backward<FP32, KIND, BASIS>), dim3(blocks), dim3(256), lds, ctx->stream, m, step,
+                       d_policy + (size_t)(step + 1) * 8, have, samples, s.path_count, (int)s.antithetic, s.seed,
+                       so, coords, spot, cash, partials, flags);
     KG_HIP(hipGetLastError());
     hipLaunchKernelGGL((lsmc_solve<BASIS>), dim3(1), dim3(256), 0, ctx->stream, partials, blocks,
                        d_policy + (size_t)step * 8, flags);
@@ -421,23 +456,25 @@ static void run_price(kg_context* ctx, const LsmcModel& m, const Sampler& s, con
                       Moments out[2]) {
   const int64_t blocks = s.blocks();
   Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 2 * blocks);
+  const SobolRef so = sobol_ref(ctx, s, m.steps);
   uint32_t* flags = reset_flags(ctx);
   hipLaunchKernelGGL((lsmc_price<FP32, LV, BASIS>), dim3(blocks), dim3(256), 0, ctx->stream, m, d_policy, s.seed,
-                     s.samples(), s.path_count, (int)s.antithetic, parts, flags);
+                     s.samples(), s.path_count, (int)s.antithetic, so, parts, flags);
   KG_HIP(hipGetLastError());
   check_flags(ctx);
   merge_and_fetch(ctx, parts, blocks, 2, out);
 }
 
-template <bool FP32, bool LV, int BASIS>
+template <bool FP32, int KIND, int BASIS>
 static void american(kg_context* ctx, const LsmcModel& m, const Sampler& pricing, const Sampler& training,
                      bool oos, double immediate, kg_lsmc_report* out) {
+  constexpr bool LV = KIND == PATHS_LOCAL_VOL;
   double* policy = (double*)ctx->policy.reserve((size_t)m.steps * 8 * sizeof(double));
   hipLaunchKernelGGL(clear_doubles, dim3(1), dim3(256), 0, ctx->stream, policy, m.steps * 8);
   KG_HIP(hipGetLastError());
   int32_t fitted = 0;
   timing_begin(ctx);
-  const double in_sample = run_lsmc<FP32, LV, BASIS>(ctx, m, oos ? training : pricing, policy, &fitted, &immediate);
+  const double in_sample = run_lsmc<FP32, KIND, BASIS>(ctx, m, oos ? training : pricing, policy, &fitted, &immediate);
   out->training_ms = timing_end(ctx);
   out->in_sample_price = in_sample;
   out->dates_fitted = fitted;
@@ -456,6 +493,22 @@ static void american(kg_context* ctx, const LsmcModel& m, const Sampler& pricing
   }
 }
 
+template <int BASIS>
+static void american_basis(kg_context* ctx, const LsmcHost& h, kg_lsmc_report* out) {
+  const LsmcModel& m = h.model;
+  // Anchored spots can leave the FP32 range, so those paths step in FP64.
+  const bool fp32 = h.pricing.fp32 && !m.anchored;
+#define KG_AMERICAN(F, K) american<F, K, BASIS>(ctx, m, h.pricing, h.training, h.oos, h.immediate, out)
+  if (m.lv) {
+    fp32 ? KG_AMERICAN(true, PATHS_LOCAL_VOL) : KG_AMERICAN(false, PATHS_LOCAL_VOL);
+  } else if (m.anchored) {
+    KG_AMERICAN(false, PATHS_ANCHORED);
+  } else {
+    fp32 ? KG_AMERICAN(true, PATHS_BROWNIAN) : KG_AMERICAN(false, PATHS_BROWNIAN);
+  }
+#undef KG_AMERICAN
+}
+
 }  // namespace kg
 
 using namespace kg;
@@ -471,25 +524,13 @@ extern "C" kg_status kg_american_price(kg_context* ctx, const kg_settings* setti
       out->price = out->in_sample_price = h.known_value;
       return;
     }
-    const bool fp32 = h.pricing.fp32;
-    const bool lv = h.model.lv != 0;
-    const int basis = options->basis;
-    if (basis == KG_BASIS_MLP) {
+    if (options->basis == KG_BASIS_MLP) {
       american_mlp(ctx, h, *options, out);
       return;
     }
-#define KG_AMERICAN(F, L, B) american<F, L, B>(ctx, h.model, h.pricing, h.training, h.oos, h.immediate, out)
-    if (basis == KG_BASIS_QUADRATIC) {
-      if (fp32)
-        lv ? KG_AMERICAN(true, true, KG_BASIS_QUADRATIC) : KG_AMERICAN(true, false, KG_BASIS_QUADRATIC);
-      else
-        lv ? KG_AMERICAN(false, true, KG_BASIS_QUADRATIC) : KG_AMERICAN(false, false, KG_BASIS_QUADRATIC);
-    } else {
-      if (fp32)
-        lv ? KG_AMERICAN(true, true, KG_BASIS_POLY4) : KG_AMERICAN(true, false, KG_BASIS_POLY4);
-      else
-        lv ? KG_AMERICAN(false, true, KG_BASIS_POLY4) : KG_AMERICAN(false, false, KG_BASIS_POLY4);
-    }
-#undef KG_AMERICAN
+    if (options->basis == KG_BASIS_QUADRATIC)
+      american_basis<KG_BASIS_QUADRATIC>(ctx, h, out);
+    else
+      american_basis<KG_BASIS_POLY4>(ctx, h, out);
   });
 }

@@ -9,8 +9,29 @@
 
 namespace kg {
 
+// How a sample's path is stored (american.rs:1149-1219): Brownian
+// coordinates (normalised BSM, the antithetic leg negated), log-spot per
+// path slot under local volatility, or the spot itself per path slot when
+// ln S0 and the path's drift and diffusion could overflow exp of a
+// coordinate (anchored BSM paths).
+enum PathKind : int32_t { PATHS_BROWNIAN = 0, PATHS_LOCAL_VOL = 1, PATHS_ANCHORED = 2 };
+
+// Paths above this many stored values are drawn in reverse (american.rs:22,
+// 1218-1219): PREFERRED_PATH_WORKSPACE of 256 MiB in FP64 values.
+constexpr double REVERSE_PATH_VALUES = 33554432.0;
+
+// Scrambled Sobol directions of the sampler, [column * dims + dim], or
+// dirs == nullptr for pseudorandom draws.
+struct SobolRef {
+  const uint64_t* dirs;
+  const uint64_t* shifts;
+  int32_t dims;
+};
+
 struct LsmcModel {
   int32_t lv, steps;
+  int32_t anchored;  // PATHS_ANCHORED: spots stored per slot
+  int32_t reverse;   // Brownian coordinates drawn terminal-first, one row kept
   double spot, strike, sign, rate, dividend, sigma, time, dt;
   double drift, diffusion, discount, log_discount;
   double year0;  // years from the surface's first row to valuation
@@ -49,8 +70,8 @@ inline LsmcHost lsmc_plan(kg_context* ctx, const kg_settings& settings, const kg
   if (!(mk.spot > 0.0) || !std::isfinite(mk.spot)) fail(KG_INVALID_INPUT, "spot must be positive");
   if (mk.valuation_time < o.effective_time || mk.valuation_time > o.expiry_time)
     fail(KG_INVALID_INPUT, "valuation outside contract life");
-  if (settings.sampling == KG_SOBOL)
-    fail(KG_UNSUPPORTED_OPERATION, "the GPU LSMC engine draws pseudorandom normals");
+  if (settings.sampling == KG_SOBOL && opt.basis == KG_BASIS_MLP)
+    fail(KG_UNSUPPORTED_OPERATION, "the MLP regressor trains on pseudorandom paths");
   const double sign = o.option_type;
   h.immediate = std::max(sign * (mk.spot - o.strike), 0.0);
   const double time = (double)(o.expiry_time - mk.valuation_time) / (365.0 * 86400e6);
@@ -95,7 +116,14 @@ inline LsmcHost lsmc_plan(kg_context* ctx, const kg_settings& settings, const kg
   if (!lv) {
     const bool normalized =
         std::fabs(std::log(mk.spot)) + (std::fabs(m.drift) + 10.0 * std::fabs(m.diffusion)) * S < 600.0;
-    if (!normalized) fail(KG_UNSUPPORTED_OPERATION, "anchored spot paths are not supported on the GPU");
+    if (!normalized && opt.basis == KG_BASIS_MLP)
+      fail(KG_UNSUPPORTED_OPERATION, "the MLP regressor needs normalised (non-anchored) paths");
+    m.anchored = !normalized;
+    // The in-sample paths decide the mode, as in american_scenarios; the
+    // MLP regressor (no reference counterpart) always draws forward.
+    const Sampler& in_sample = h.oos ? h.training : h.pricing;
+    m.reverse = normalized && opt.basis != KG_BASIS_MLP &&
+                (double)in_sample.samples() * (double)S > REVERSE_PATH_VALUES;
     const double lg = log_ratio(o.strike, mk.spot);
     const double cutoff = sign * lg - 32.0 * std::numeric_limits<double>::epsilon() * (1.0 + std::fabs(lg));
     for (int k = 0; k < S; ++k) {

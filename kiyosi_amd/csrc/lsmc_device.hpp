@@ -5,6 +5,7 @@
 
 #include "fp32_math.hpp"
 #include "lsmc.hpp"
+#include "sobol_device.hpp"
 
 namespace kg {
 
@@ -160,10 +161,84 @@ struct Normals {
   }
 };
 
-template <bool FP32, bool LV>
+// The normals of one forward path: the sample's run of its block's PCG
+// stream (normal j * steps onwards for sample j), or dimensions 0, 1, ..
+// of Sobol point n (the reference's cursor, engines/monte_carlo/sobol.rs).
+template <bool FP32>
+struct PathDraws {
+  Normals<FP32> pcg;
+  SobolRef so;
+  uint64_t point;
+  int dim;
+  __device__ inline void start(uint64_t seed, uint64_t block, int j, int steps, int64_t n, const SobolRef& s) {
+    so = s;
+    point = (uint64_t)n;
+    dim = 0;
+    if (!so.dirs) pcg.start(seed, block, (uint64_t)j * (uint64_t)steps);
+  }
+  __device__ inline double next() {
+    if (!so.dirs) return pcg.next();
+    const uint64_t bits = sobol_point_direct(so.dirs, so.shifts, so.dims, point, dim++);
+    return FP32 ? (double)f32::normal_from_bits((uint32_t)(bits >> 20)) : normal_from_bits(bits);
+  }
+};
+
+// Normal `row` of a block's reverse-mode draws for lane j
+// (ExercisePaths::reverse_row, american.rs:278-316): pseudorandom draws take
+// the block's stream step-major, normal row * count + j of Pcg32(seed,
+// block) with `count` samples in the block; Sobol draws take dimension
+// `row` of point n. Each lane forms its own Box-Muller pair and keeps the
+// half its index selects.
+template <bool FP32>
+__device__ inline double reverse_normal(uint64_t seed, int64_t block, int j, int count, int row, int64_t n,
+                                        const SobolRef& so) {
+  if (so.dirs) {
+    const uint64_t bits = sobol_point_direct(so.dirs, so.shifts, so.dims, (uint64_t)n, row);
+    return FP32 ? (double)f32::normal_from_bits((uint32_t)(bits >> 20)) : normal_from_bits(bits);
+  }
+  const uint64_t k = (uint64_t)row * (uint64_t)count + (uint64_t)j;
+  Pcg32 g;
+  g.init(seed, (uint64_t)block);
+  g.advance((k >> 1) * 4);
+  if (FP32) {
+    float c, s;
+    f32::box_muller_pair(g, &c, &s);
+    return (double)((k & 1) ? s : c);
+  }
+  double c, s;
+  box_muller_pair(g, &c, &s);
+  return (k & 1) ? s : c;
+}
+
+// One reverse-mode row: the coordinate after step + 1 intervals from the
+// one after step + 2 (W_m | W_(m+1) has mean m/(m+1) W_(m+1) and variance
+// m/(m+1) in units of one interval), or the terminal coordinate
+// sqrt(steps) z at the last step.
+template <bool FP32>
+__device__ inline double bridge_back(double w, double z, int step, int steps) {
+  const int elapsed = step + 1;
+  double weight, deviation;
+  if (elapsed == steps) {
+    weight = 0.0;
+    deviation = sqrt((double)elapsed);
+  } else {
+    const double fraction = (double)elapsed / (double)(elapsed + 1);
+    weight = fraction;
+    deviation = sqrt(fraction);
+  }
+  if (FP32) return (double)fmaf((float)weight, (float)w, (float)deviation * (float)z);
+  return fma(weight, w, deviation * z);
+}
+
+// Path generation in forward mode: Brownian coordinates [step][sample],
+// local-vol log-spots [step][slot], or anchored spots [step][slot] (FP64),
+// then each slot's terminal spot and payoff.
+template <bool FP32, int KIND>
 __global__ void __launch_bounds__(256)
-    lsmc_generate(LsmcModel m, uint64_t seed, int64_t samples, int64_t paths, int antithetic,
-                  void* __restrict__ coords, double* __restrict__ spot, double* __restrict__ cash) {
+    lsmc_generate(LsmcModel m, uint64_t seed, int64_t samples, int64_t paths, int antithetic, SobolRef so,
+                  void* __restrict__ coords, double* __restrict__ spot, double* __restrict__ cash,
+                  uint32_t* __restrict__ flags) {
+  constexpr bool LV = KIND == PATHS_LOCAL_VOL;
   __shared__ double lv_lds[LV ? 3 * LV_LDS_NODES : 1];
   const int64_t block = blockIdx.x;
   const int j = threadIdx.x;
@@ -172,16 +247,30 @@ __global__ void __launch_bounds__(256)
   const bool paired = antithetic && live && n < paths / 2;
   const int legs = antithetic ? 2 : 1;
   const int64_t slots = samples * legs;
-  Normals<FP32> z;
-  if (live) z.start(seed, (uint64_t)block, (uint64_t)j * (uint64_t)m.steps);
+  PathDraws<FP32> z;
+  if (live) z.start(seed, (uint64_t)block, j, m.steps, n, so);
   LvRows rows{};
   rows.row = -1;
   int cp = -1, cn = -1;  // spot brackets of the two legs
   double w = 0.0, xp = 0.0, xn = 0.0;
+  double ap = m.spot, an = m.spot;  // anchored spots of the two legs
+  bool bad = false;
   for (int k = 0; k < m.steps; ++k) {
     if (LV) lv_stage(m, k, lv_lds, rows);  // block-uniform
     if (!live) continue;
     const double e = z.next();
+    if (KIND == PATHS_ANCHORED) {
+      // advance + evolve_spot (american.rs:355-362, 546-552)
+      ap *= exp(fma(m.diffusion, e, m.drift));
+      bad = bad || !isfinite(ap) || ap <= 0.0;
+      ((double*)coords)[(size_t)k * slots + n * legs] = ap;
+      if (paired) {
+        an *= exp(fma(m.diffusion, -e, m.drift));
+        bad = bad || !isfinite(an) || an <= 0.0;
+        ((double*)coords)[(size_t)k * slots + n * legs + 1] = an;
+      }
+      continue;
+    }
     if (LV) {
       xp = lv_step<FP32>(m, rows, xp, e, cp);
       if (paired) xn = lv_step<FP32>(m, rows, xn, -e, cn);
@@ -200,13 +289,44 @@ __global__ void __launch_bounds__(256)
         ((double*)coords)[(size_t)k * samples + n] = w;
     }
   }
+  if (bad) raise_flag(flags, FLAG_NONFINITE);
   if (!live) return;
   for (int leg = 0; leg < (paired ? 2 : 1); ++leg) {
     double s;
-    if (LV)
+    if (KIND == PATHS_ANCHORED)
+      s = leg ? an : ap;
+    else if (LV)
       s = m.spot * exp(leg ? xn : xp);
     else
       s = FP32 ? bsm_spot32(m, m.steps - 1, (float)(leg ? -w : w)) : bsm_spot(m, m.steps - 1, leg ? -w : w);
+    spot[n * legs + leg] = s;
+    cash[n * legs + leg] = payoff(m, s);
+  }
+}
+
+// Reverse mode, first row: each sample's terminal Brownian coordinate
+// (one value per sample, FP32 or FP64) and its slots' terminal spot and
+// payoff.
+template <bool FP32>
+__global__ void __launch_bounds__(256)
+    lsmc_reverse_start(LsmcModel m, uint64_t seed, int64_t samples, int64_t paths, int antithetic, SobolRef so,
+                       void* __restrict__ coords, double* __restrict__ spot, double* __restrict__ cash) {
+  const int64_t block = blockIdx.x;
+  const int j = threadIdx.x;
+  const int64_t n = block * BLOCK_SAMPLES + j;
+  if (n >= samples) return;
+  const int count = (int)(samples - block * BLOCK_SAMPLES < BLOCK_SAMPLES ? samples - block * BLOCK_SAMPLES
+                                                                          : BLOCK_SAMPLES);
+  const int last = m.steps - 1;
+  const double w = bridge_back<FP32>(0.0, reverse_normal<FP32>(seed, block, j, count, 0, n, so), last, m.steps);
+  if (FP32)
+    ((float*)coords)[n] = (float)w;
+  else
+    ((double*)coords)[n] = w;
+  const bool paired = antithetic && n < paths / 2;
+  const int legs = antithetic ? 2 : 1;
+  for (int leg = 0; leg < (paired ? 2 : 1); ++leg) {
+    const double s = FP32 ? bsm_spot32(m, last, (float)(leg ? -w : w)) : bsm_spot(m, last, leg ? -w : w);
     spot[n * legs + leg] = s;
     cash[n * legs + leg] = payoff(m, s);
   }

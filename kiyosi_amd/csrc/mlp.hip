@@ -769,8 +769,9 @@ static void run(kg_context* ctx, LsmcHost& h, const kg_lsmc_options& opt, kg_lsm
   uint32_t* flags = reset_flags(ctx);
 
   timing_begin(ctx);
-  hipLaunchKernelGGL((lsmc_generate<FP32, LV>), dim3(blocks), dim3(256), 0, ctx->stream, m, tr.seed, samples,
-                     tr.path_count, (int)tr.antithetic, coords, spot, cash);
+  hipLaunchKernelGGL((lsmc_generate<FP32, LV ? PATHS_LOCAL_VOL : PATHS_BROWNIAN>), dim3(blocks), dim3(256), 0,
+                     ctx->stream, m, tr.seed, samples, tr.path_count, (int)tr.antithetic, SobolRef{nullptr, nullptr, 0},
+                     coords, spot, cash, flags);
   KG_HIP(hipGetLastError());
   std::vector<int32_t> fitted(S, 0);
   double loss_sum = 0.0;

@@ -251,3 +251,64 @@ def test_config5_poly4_lower_bound_near_the_fd_judge(gpu, oracle):
     assert rep.price <= fd + 3 * rep.standard_error
     assert abs(rep.price - fd) < rep.standard_error + 0.02
     assert rep.price >= quad.price - 2 * rep.standard_error
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("paths,antithetic", [(1 << 20, False), (1_400_001, True)])
+def test_gpu_reverse_paths_match_the_oracle(gpu, oracle, paths, antithetic):
+    """Above 2^25 path values the engine draws the Brownian coordinates in
+    reverse as the reference does (terminal first, the bridge back one date
+    per regression): 2^20 x 50 (the size of the local-vol American workload)
+    and 700,001 antithetic samples x 50 (a last block of 97 samples, an odd
+    stream offset per row), FP64 against the oracle at 1e-9."""
+    s = settings(paths, seed=1729, antithetic=antithetic, large=True)
+    got = gpu_american(gpu, s, am_terms(), bsm_model())
+    ref, _ = oracle.american_lsmc(s, put(), bsm(), 50, workers=16)
+    forward, _ = oracle.american_lsmc(s, put(), bsm(), 50, workers=16, mode=0)
+    assert ref != forward  # the reverse draws really differ from the forward ones
+    assert got.price == pytest.approx(ref, rel=1e-9)
+
+
+@pytest.mark.gpu
+def test_gpu_reverse_paths_fp32_within_half_a_standard_error(gpu):
+    """FP32 bridge and spots against the FP64 reverse-mode engine on the same
+    draws, judged by the out-of-sample standard error of a 2^20-path pricing."""
+    plain = gpu_american(gpu, settings(1 << 20, seed=5, large=True), am_terms(), bsm_model(), training=1 << 20)
+    fast = gpu_american(gpu, settings(1 << 20, seed=5, precision=_abi.FP32_STEP, large=True), am_terms(), bsm_model(),
+                        training=1 << 20)
+    assert abs(fast.price - plain.price) <= 0.5 * plain.standard_error
+    assert abs(fast.in_sample_price - plain.in_sample_price) <= 0.5 * plain.standard_error
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("paths,antithetic", [(1 << 12, False), (1 << 16, True), (1 << 20, False)])
+def test_gpu_sobol_lsmc_matches_the_oracle(gpu, oracle, paths, antithetic):
+    """Sobol draws: forward (2^12, 2^16 antithetic) and reverse (2^20 x 50,
+    dimension 50 - k for the coordinate after k intervals), FP64 at 1e-9."""
+    s = settings(paths, seed=42, antithetic=antithetic, large=True, sampling=_abi.SOBOL)
+    got = gpu_american(gpu, s, am_terms(), bsm_model())
+    ref, _ = oracle.american_lsmc(s, put(), bsm(), 50, workers=16)
+    assert got.price == pytest.approx(ref, rel=1e-9)
+
+
+@pytest.mark.gpu
+def test_gpu_sobol_out_of_sample_matches_the_oracle(gpu, oracle):
+    s = settings(1 << 14, seed=5, sampling=_abi.SOBOL)
+    got = gpu_american(gpu, s, am_terms(), bsm_model(), training=1 << 15)
+    ref = oracle.american_out_of_sample(s, 1 << 15, put(), bsm(), 50, workers=16)
+    assert got.price == pytest.approx(ref, rel=1e-9)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32_STEP])
+@pytest.mark.parametrize("antithetic", [False, True])
+def test_gpu_anchored_paths_match_the_oracle(gpu, oracle, precision, antithetic):
+    """Spot and strike at 1e255 take the anchored paths (each leg's spot
+    stepped directly, in FP64 whatever the precision asked: such spots leave
+    the FP32 range); GPU against the oracle at 1e-9."""
+    s = settings(40_001, seed=11, antithetic=antithetic, precision=precision)
+    terms = _abi.AmericanTerms(-1, 50, 1e255, us(START), us(END))
+    model = _abi.AmericanModel(1e255, 0.05, 0.02, 0.2, us(START), None)
+    got = gpu_american(gpu, s, terms, model)
+    ref, _ = oracle.american_lsmc(s, put(1e255), bsm(1e255), 50, workers=16)
+    assert got.price == pytest.approx(ref, rel=1e-9)
