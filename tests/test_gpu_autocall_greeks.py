@@ -53,7 +53,8 @@ CASES = [
 
 @pytest.mark.parametrize("precision", [_abi.FP64, _abi.FP32_STEP], ids=["fp64", "fp32"])
 @pytest.mark.parametrize("name,make,sampling,antithetic,paths", CASES, ids=[c[0] for c in CASES])
-def test_each_lane_equals_its_scenario_priced_alone(gpu, name, make, sampling, antithetic, paths, precision):
+def test_each_lane_equals_its_scenario_priced_alone(gpu, oracle, name, make, sampling, antithetic, paths,
+                                                   precision):
     note = make()
     scenarios = same_time_scenarios(note, context())
     assert 9 <= len(scenarios) <= _abi.MAX_AUTOCALL_SCENARIOS  # spans several launch groups
@@ -64,6 +65,9 @@ def test_each_lane_equals_its_scenario_priced_alone(gpu, name, make, sampling, a
         alone = gpu_price(gpu, s, note, scn)
         for f in FIELDS:
             assert getattr(out[i], f) == getattr(alone, f), (name, i, f)
+        if precision == _abi.FP64:
+            ref = oracle.autocallable_price(s, note._terms(), scn.bsm_market(), workers=8)
+            assert out[i].price == pytest.approx(ref.price, rel=1e-10, abs=1e-13), (name, i)
 
 
 def snowball_with_history():
@@ -78,7 +82,7 @@ def snowball_with_history():
                           barrier_state="no_event", calendar=cal), dates
 
 
-def test_scenarios_with_a_known_value_take_it(gpu):
+def test_scenarios_with_a_known_value_take_it(gpu, oracle):
     """On a knock-out date, scenarios at or above the level settle now; the
     others are sampled lanes."""
     note, dates = snowball_with_history()
@@ -92,6 +96,8 @@ def test_scenarios_with_a_known_value_take_it(gpu):
         alone = gpu_price(gpu, s, note, c)
         for f in FIELDS:
             assert getattr(out[i], f) == getattr(alone, f), (i, f)
+        ref = oracle.autocallable_price(s, note._terms(), c.bsm_market(), workers=8)
+        assert out[i].price == pytest.approx(ref.price, rel=1e-10, abs=1e-13), i
     assert out[1].has_seed == 0 and out[0].has_seed == 1
 
 

@@ -189,3 +189,38 @@ def test_asian_engine_price_with_greeks(gpu, oracle):
     _, rg = oracle.asian_price_with_greeks(engine._settings(), opt._terms(40), ctx.bsm_market(), workers=8)
     assert res.get("delta") == pytest.approx(rg.delta, rel=1e-10)
     assert res.get("gamma") == pytest.approx(rg.gamma, rel=1e-10)
+
+
+@pytest.fixture(scope="module")
+def full_book(gpu):
+    """The 4,096-contract book at its full path counts (2^14..2^18 per
+    contract), priced in one persistent launch in FP64 and in FP32 steps."""
+    entries = workloads.book_4096()
+    book = Book(entries)
+    plain, _ = book.price("fp64")
+    fast, tuning = book.price("fp32")
+    return entries, plain, fast, tuning
+
+
+def test_config4_full_book_fp32_within_half_a_standard_error(full_book):
+    entries, plain, fast, tuning = full_book
+    assert len(plain) == len(fast) == 4096
+    errors = np.array([abs(f.price - p.price) / p.path_standard_error for f, p in zip(fast, plain)
+                       if p.path_standard_error > 0])
+    assert errors.size > 4000
+    assert errors.max() <= 0.5, np.sort(errors)[-5:]
+
+
+def test_config4_sampled_contracts_at_full_path_counts_match_the_oracle(full_book, oracle):
+    """64 contracts drawn with a fixed seed from the full book: each FP64
+    price from the one persistent launch equals the oracle's price of that
+    contract alone, at its full path count, to 1e-10."""
+    entries, plain, _, _ = full_book
+    pick = np.random.default_rng(20251204).choice(len(entries), size=64, replace=False)
+    kinds = set()
+    for i in sorted(pick):
+        engine, instrument, context = entries[i]
+        ref = oracle_entry(oracle, engine, instrument, context)
+        assert plain[i].price == pytest.approx(ref.price, rel=1e-10, abs=1e-12), (i, type(instrument).__name__)
+        kinds.add(type(instrument).__name__)
+    assert len(kinds) >= 5

@@ -49,7 +49,9 @@ STRIKES = [60.0 + 2.0 * i for i in range(40)] + [100.0, 80.0, 100.0]
 @pytest.mark.parametrize("sampling,antithetic,paths", [("pseudo_random", False, 4096),
                                                        ("pseudo_random", True, 3001),
                                                        ("sobol", False, 2048)])
-def test_asian_strike_batch_equals_scalar_prices(gpu, kind, sampling, antithetic, paths, precision):
+def test_asian_strike_batch_equals_scalar_prices(gpu, oracle, kind, sampling, antithetic, paths, precision):
+    """Each strike lane equals that strike priced alone on the GPU bit for bit,
+    and the oracle's price of that strike on the same seed at 1e-10 (FP64)."""
     engine = MonteCarloAsianEngine(path_count=paths, seed=42, sampling=sampling, antithetic=antithetic,
                                    step_count=16, precision=precision)
     template = asian(kind)
@@ -57,10 +59,14 @@ def test_asian_strike_batch_equals_scalar_prices(gpu, kind, sampling, antithetic
     assert len(got) == len(STRIKES)
     for strike, price in zip(STRIKES, got):
         assert price == alone(engine, template, strike, context()), strike
+        if precision == "fp64":
+            ref = oracle.asian_price(engine._settings(), asian(kind, strike)._terms(16), context().bsm_market(),
+                                     workers=8)
+            assert price == pytest.approx(ref.price, rel=1e-10, abs=1e-12), strike
     assert got[40] == got[42]
 
 
-def test_asian_strike_batch_edge_cases(gpu):
+def test_asian_strike_batch_edge_cases(gpu, oracle):
     engine = MonteCarloAsianEngine(path_count=1024, seed=7, step_count=8)
     assert engine.price_strikes(asian(), context(), strikes=[]) == ()
     with pytest.raises(k.InvalidInputError):
@@ -72,9 +78,12 @@ def test_asian_strike_batch_edge_cases(gpu):
     got = engine.price_strikes(asian("put"), low_vol, strikes=strikes)
     assert got == tuple(alone(engine, asian("put"), s, low_vol) for s in strikes)
     assert all(p >= 0.0 for p in got)
+    for strike, price in zip(strikes, got):
+        ref = oracle.asian_price(engine._settings(), asian("put", strike)._terms(8), low_vol.bsm_market(), workers=8)
+        assert price == pytest.approx(ref.price, rel=1e-10, abs=1e-12), strike
 
 
-def test_american_strike_batch_equals_scalar_prices(gpu):
+def test_american_strike_batch_equals_scalar_prices(gpu, oracle):
     engine = MonteCarloVanillaEngine(path_count=1 << 14, seed=42, step_count=16, antithetic=True)
     template = AmericanOption(option_type="put", strike=100.0, effective_time=START, expiry_time=END)
     strikes = [90.0, 110.0, 100.0, 90.0]
@@ -83,6 +92,13 @@ def test_american_strike_batch_equals_scalar_prices(gpu):
                                   context()) for s in strikes)
     assert got == expected
     assert got[0] == got[3] and got[0] < got[2] < got[1]
+    import ctypes as C
+    from kiyosi_amd import _abi
+    from conftest import us
+    for strike, price in zip(strikes, got):
+        ref, _ = oracle.american_lsmc(engine._settings(), _abi.EuropeanTerms(-1, 0, strike, us(START), us(END)),
+                                      context().bsm_market(), 16, workers=8)
+        assert price == pytest.approx(ref, rel=1e-9), strike
 
 
 def test_american_greeks_come_from_american_prices(gpu):
