@@ -480,6 +480,19 @@ static int cu_count(kg_context* ctx) {
   return std::max(cus, 1);
 }
 
+// Dynamic LDS of bridge_events_kernel: the plan's interval rows, and with
+// Sobol tables in LDS 32 table words per interval.
+static size_t bridge_events_lds(int n_intervals, int dims, bool sobol, bool fp32) {
+  size_t bytes = (size_t)n_intervals * (fp32 ? sizeof(IntervalRow<float>) : sizeof(IntervalRow<double>));
+  if (sobol && sobol_lds_bytes(dims, fp32) <= SOBOL_LDS_LIMIT)
+    bytes += (size_t)n_intervals * 32 * (fp32 ? sizeof(uint32_t) : sizeof(uint64_t));
+  return bytes;
+}
+
+// Largest bridge_events_kernel LDS taken without an opt-in; plans with more
+// interval rows (many knock-out events after the lockout) price in one pass.
+constexpr size_t BRIDGE_EVENTS_LDS_LIMIT = 64 * 1024;
+
 // The two-pass launches of a bridge plan (see bridge_events_kernel).
 template <bool SOBOL, bool FP32, bool LDS>
 static void launch_bridge(kg_context* ctx, const AcPlanDev& p, const Sampler& s, const uint64_t* dirs,
@@ -501,6 +514,7 @@ static void launch_bridge(kg_context* ctx, const AcPlanDev& p, const Sampler& s,
     events_lds += (size_t)p.n_intervals * 32 * sizeof(W);
     fill_lds = sobol_lds_bytes(dims, FP32) + (size_t)FILL_WAVES * (dims + 1) * sizeof(W);
   }
+  if (events_lds > BRIDGE_EVENTS_LDS_LIMIT) fail(KG_DEVICE_ERROR, "bridge plan exceeds its pass-one LDS");
   hipLaunchKernelGGL((bridge_events_kernel<SOBOL, FP32, LDS>), dim3(blocks), dim3(256), events_lds, ctx->stream, p,
                      s.seed, samples, s.path_count, (int)s.antithetic, dirs, shifts, values, d_rows,
                      d_rows + table_offset, base, queue, counts, flags);
@@ -661,9 +675,12 @@ namespace kg {
 // One sampled note on its own: the single-plan kernel and the block merge.
 static void price_note(kg_context* ctx, const Sampler& s, AcHostPlan& h, kg_estimate* out) {
   // Bridge plans run in two passes (queue items carry the sample index in
-  // 31 bits); other plans, and larger ones, in one. The two-pass interval
-  // rows follow the plan's tables in the same upload.
-  const bool two_pass = h.dev.bridge && s.samples() < ((int64_t)1 << 31);
+  // 31 bits) while pass one's interval rows fit its LDS; other plans, and
+  // larger ones, in one. The two-pass interval rows follow the plan's tables
+  // in the same upload.
+  const bool two_pass = h.dev.bridge && s.samples() < ((int64_t)1 << 31) &&
+                        bridge_events_lds(h.dev.n_intervals, h.dev.n_steps, s.sobol, s.fp32) <=
+                            BRIDGE_EVENTS_LDS_LIMIT;
   std::vector<unsigned char> blob;
   const AcOffsets off = stage_plan(blob, h);
   size_t rows_at = 0, table_offset = 0;

@@ -100,3 +100,34 @@ def test_python_engine_interface(gpu):
     knocked_out.barrier_state = "knocked_out"
     with pytest.raises(k.InvalidInputError):
         engine.price(knocked_out, context)
+
+
+def daily_autocall(days, lockout=20):
+    """A snowball that knocks out on any trading day after a lockout of
+    `lockout` days, with daily knock-in: a bridge plan (the lockout leaves a
+    gap of more than eight days) with one interval per knock-out day."""
+    from kiyosi_amd.instruments import SnowballObservation, SnowballOption
+    cal = TradingCalendar.weekdays()
+    dates = [cal.add_trading_days(V, d) for d in range(lockout, days + 1)]
+    return SnowballOption(
+        maturity_coupon_rate=0.15, initial_spot=100.0, knock_in_level=75.0, upper_strike=100.0, lower_strike=0.0,
+        knock_in_observation=KnockInObservation.every_trading_day(dt.time(0, 0)), effective_time=V,
+        expiry_time=dates[-1], autocall_observations=[SnowballObservation(d, 103.0, 0.15) for d in dates],
+        calendar=cal)
+
+
+@pytest.mark.parametrize("days,sampling", [(1000, _abi.PSEUDO_RANDOM), (252, _abi.SOBOL)])
+def test_daily_autocall_bridge_notes_beyond_the_pass_one_lds(gpu, oracle, days, sampling):
+    """Bridge notes whose interval rows exceed pass one's 64 KiB of LDS (981
+    knock-out days for pseudorandom FP64 rows of 80 bytes; 233 for Sobol,
+    whose rows add 256 bytes of table words) price in one pass and match the
+    oracle at 1e-10."""
+    note = daily_autocall(days)
+    context = PricingContext(model_parameters=BlackScholesMertonParameters(
+        risk_free_rate=0.02, dividend_yield=0.0, volatility=0.25), spot_price=100.0, valuation_time=V)
+    plan = oracle.autocallable_plan(note._terms(), context.bsm_market(), cap=4000)
+    assert plan[1]  # a bridge plan
+    s = settings(4096, sampling=sampling)
+    got = gpu_price(gpu, s, note, context)
+    ref = oracle.autocallable_price(s, note._terms(), context.bsm_market(), workers=8)
+    assert got.price == pytest.approx(ref.price, rel=1e-10)

@@ -224,3 +224,44 @@ def test_config4_sampled_contracts_at_full_path_counts_match_the_oracle(full_boo
         assert plain[i].price == pytest.approx(ref.price, rel=1e-10, abs=1e-12), (i, type(instrument).__name__)
         kinds.add(type(instrument).__name__)
     assert len(kinds) >= 5
+
+
+@pytest.mark.parametrize("sampling", [_abi.PSEUDO_RANDOM, _abi.SOBOL])
+def test_asian_native_greeks_redo_without_the_control(gpu, oracle, sampling):
+    """Far out-of-the-money arithmetic puts at sigma 0.45 (the strikes whose
+    controlled price turns negative in the strike-batch tests): the native
+    Greeks entry resamples without the geometric control, as the reference
+    does (asian.rs:618-630); price, delta and gamma equal the oracle's."""
+    m = market(rate=0.04, dividend=0.01, sigma=0.45)
+    s = settings(1024, sampling=sampling)
+    for strike in (40.0, 45.0, 50.0):
+        t = _abi.AsianTerms(-1, _abi.AVERAGE_ARITHMETIC, 8, 0, strike, 0.0, us(START), us(START), us(END))
+        out, g = _abi.Estimate(), _abi.SpotGreeks(3, 0, 0.0, 0.0, 0.0)
+        gpu.check(_native.lib().kg_asian_price_with_greeks(gpu.handle, C.byref(s), C.byref(t), C.byref(m),
+                                                           C.byref(g), C.byref(out)))
+        ref, rg = oracle.asian_price_with_greeks(s, t, m, request=3, workers=8)
+        assert out.price >= 0.0
+        assert out.price == pytest.approx(ref.price, rel=1e-10, abs=1e-14), strike
+        assert g.delta == pytest.approx(rg.delta, rel=1e-10, abs=1e-14), strike
+        assert g.gamma == pytest.approx(rg.gamma, rel=1e-10, abs=1e-14), strike
+
+
+@pytest.mark.parametrize("averaging", [_abi.AVERAGE_GEOMETRIC, _abi.AVERAGE_ARITHMETIC])
+@pytest.mark.parametrize("sampling", [_abi.PSEUDO_RANDOM, _abi.SOBOL])
+def test_asian_native_greeks_fp32_within_half_a_standard_error(gpu, averaging, sampling):
+    """The FP32-step spot-Greek kernels against the FP64 ones on the same
+    draws: price within half a standard error of the FP64 run's per-path
+    spread, delta and gamma within 1e-3 of their FP64 values."""
+    t = _abi.AsianTerms(1, averaging, 64, 0, 100.0, 0.0, us(START), us(START), us(END))
+    m = market(rate=0.04, dividend=0.01)
+    res = {}
+    for precision in (_abi.FP64, _abi.FP32_STEP):
+        s = settings(1 << 16, sampling=sampling, precision=precision)
+        out, g = _abi.Estimate(), _abi.SpotGreeks(3, 0, 0.0, 0.0, 0.0)
+        gpu.check(_native.lib().kg_asian_price_with_greeks(gpu.handle, C.byref(s), C.byref(t), C.byref(m),
+                                                           C.byref(g), C.byref(out)))
+        res[precision] = (out, g)
+    (plain, gp), (fast, gf) = res[_abi.FP64], res[_abi.FP32_STEP]
+    assert abs(fast.price - plain.price) <= 0.5 * plain.path_standard_error
+    assert gf.delta == pytest.approx(gp.delta, abs=1e-3)
+    assert gf.gamma == pytest.approx(gp.gamma, abs=1e-3)
