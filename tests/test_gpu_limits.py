@@ -130,8 +130,13 @@ def test_american_engine_limits(gpu):
     assert lib.kg_american_price(gpu.handle, C.byref(s), C.byref(terms), C.byref(model), C.byref(opts),
                                  C.byref(report)) == _abi.INVALID_INPUT
     terms.step_count = 50
-    sob = settings(sampling=_abi.SOBOL)
+    # Sobol paths price with the polynomial bases; the MLP regressor trains on
+    # pseudorandom paths only.
+    sob = settings(paths=4096, sampling=_abi.SOBOL)
     assert lib.kg_american_price(gpu.handle, C.byref(sob), C.byref(terms), C.byref(model), C.byref(opts),
+                                 C.byref(report)) == _abi.OK
+    mlp_default = _abi.LsmcOptions(_abi.BASIS_MLP, 0, 0, 4, 4096, 1e-3)
+    assert lib.kg_american_price(gpu.handle, C.byref(sob), C.byref(terms), C.byref(model), C.byref(mlp_default),
                                  C.byref(report)) == _abi.UNSUPPORTED_OPERATION
     mlp = _abi.LsmcOptions(_abi.BASIS_MLP, 0, 0, 4, 16, 1e-3)
     assert lib.kg_american_price(gpu.handle, C.byref(s), C.byref(terms), C.byref(model), C.byref(mlp),
