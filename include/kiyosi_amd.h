@@ -132,6 +132,19 @@ kg_status kg_vanilla_price(kg_context* ctx, const kg_settings* settings,
                            const kg_european_terms* option, const kg_bsm_market* market,
                            kg_estimate* out);
 
+/* ---- kg_vanilla_price without device allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc). Same kernels and results, bit for
+ * bit; the per-block partials, the flag word and the merged moments live in
+ * one fixed device array per GPU (about 2 MiB, held until the process ends)
+ * instead of the context's buffers. At most 8,388,608 samples (32,768 blocks
+ * of 256); larger requests return KG_INVALID_INPUT. Every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU (several
+ * contexts or threads) race; serialise them. */
+kg_status kg_vanilla_price_noalloc(kg_context* ctx, const kg_settings* settings,
+                                   const kg_european_terms* option, const kg_bsm_market* market,
+                                   kg_estimate* out);
+
 /* ---- Shared-draw scenarios of a European option: the batched Greek stencil
  * of MonteCarloVanillaEngine::price_with_greeks_with_settings
  * (monte_carlo.rs:1334-1409, european_greek_prices :1143-1162). Prices the

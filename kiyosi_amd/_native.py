@@ -30,6 +30,8 @@ SIGNATURES = {
     "kg_sobol_points": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_uint64)]),
     "kg_vanilla_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.EuropeanTerms), P(_abi.BsmMarket),
                                    P(_abi.Estimate)]),
+    "kg_vanilla_price_noalloc": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.EuropeanTerms),
+                                           P(_abi.BsmMarket), P(_abi.Estimate)]),
     "kg_vanilla_scenarios": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.EuropeanTerms), P(_abi.BsmMarket),
                                        C.c_int32, P(_abi.Estimate)]),
     "kg_autocallable_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AutocallTerms), P(_abi.BsmMarket),

@@ -116,6 +116,35 @@ void finish_sampling(kg_context* ctx, const Moments* partials, int64_t blocks, i
   *device_ms = ms;
 }
 
+void finish_sampling_into(kg_context* ctx, const Moments* partials, int64_t blocks, int slots, Moments* d_out,
+                          const uint32_t* d_flags, Moments* host_out, double* device_ms) {
+  KG_HIP(hipEventRecord(ctx->stop, ctx->stream));
+  hipLaunchKernelGGL(merge_partials, dim3(slots), dim3(MERGE_THREADS), 0, ctx->stream, partials, blocks, slots,
+                     d_out, d_flags);
+  KG_HIP(hipGetLastError());
+  // Merged slots and flag word into a fixed host buffer: pageable memory,
+  // copied before the synchronisation returns.
+  constexpr int MAX_SLOTS = 4;
+  if (slots > MAX_SLOTS) fail(KG_INVALID_INPUT, "too many merged outputs");
+  unsigned char host[MAX_SLOTS * sizeof(Moments) + sizeof(uint32_t)];
+  const size_t result_bytes = sizeof(Moments) * slots;
+  KG_HIP(hipMemcpyAsync(host, d_out, result_bytes + sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP(hipStreamSynchronize(ctx->stream));
+  uint32_t h;
+  std::memcpy(&h, host + result_bytes, sizeof(h));
+  if (h & FLAG_CANCELLED) fail(KG_INTERRUPTED, "operation interrupted");
+  if (h & FLAG_NONFINITE) unstable();
+  std::memcpy(host_out, host, result_bytes);
+  float ms = 0.f;
+  KG_HIP(hipEventElapsedTime(&ms, ctx->start, ctx->stop));
+  *device_ms = ms;
+}
+
+void clear_flag_word(kg_context* ctx, uint32_t* word) {
+  hipLaunchKernelGGL(clear_word, dim3(1), dim3(1), 0, ctx->stream, word);
+  KG_HIP(hipGetLastError());
+}
+
 void timing_begin(kg_context* ctx) { KG_HIP(hipEventRecord(ctx->start, ctx->stream)); }
 double timing_end(kg_context* ctx) {
   KG_HIP(hipEventRecord(ctx->stop, ctx->stream));
@@ -123,6 +152,40 @@ double timing_end(kg_context* ctx) {
   float ms = 0.f;
   KG_HIP(hipEventElapsedTime(&ms, ctx->start, ctx->stop));
   return ms;
+}
+
+// Linear matrix scrambling and the digital shift of one dimension
+// (sobol.rs:100-133): Pcg32(seed, dimension) draws the shift, then one
+// lower-triangular row per bit with a unit diagonal; column c of the result
+// goes to out[c * stride].
+static void scramble_dimension(const uint64_t (&raw)[52], uint64_t seed, uint32_t d, uint64_t* out,
+                               size_t stride, uint64_t* shift) {
+  constexpr int B = 52;
+  constexpr uint64_t MASK = (1ULL << B) - 1;
+  Pcg32 g;
+  g.init(seed, d);
+  auto word = [&g]() {
+    uint64_t hi = g.next();
+    return (hi << 32) | (uint64_t)g.next();
+  };
+  *shift = word() & MASK;
+  uint64_t rows[B];
+  for (int row = 0; row < B; ++row) {
+    const int bit = B - 1 - row;
+    rows[row] = (word() & (MASK << (bit + 1))) | (1ULL << bit);
+  }
+  for (int c = 0; c < B; ++c) {
+    uint64_t v = 0;
+    for (int row = 0; row < B; ++row)
+      v |= (uint64_t)(__builtin_popcountll(raw[c] & rows[row]) & 1) << (B - 1 - row);
+    out[(size_t)c * stride] = v;
+  }
+}
+
+void sobol_dimension_zero(uint64_t seed, uint64_t (&dirs)[52], uint64_t* shift) {
+  uint64_t raw[52];
+  for (int c = 0; c < 52; ++c) raw[c] = 1ULL << (51 - c);
+  scramble_dimension(raw, seed, 0, dirs, 1, shift);
 }
 
 // SobolSequence::new (engines/monte_carlo/sobol.rs:85-134): Joe-Kuo
@@ -152,24 +215,7 @@ static void build_sobol(const std::vector<uint32_t>& table, uint32_t dims, uint6
         raw[c] = v;
       }
     }
-    Pcg32 g;
-    g.init(seed, d);
-    auto word = [&g]() {
-      uint64_t hi = g.next();
-      return (hi << 32) | (uint64_t)g.next();
-    };
-    shifts[d] = word() & MASK;
-    uint64_t rows[B];
-    for (int row = 0; row < B; ++row) {
-      const int bit = B - 1 - row;
-      rows[row] = (word() & (MASK << (bit + 1))) | (1ULL << bit);
-    }
-    for (int c = 0; c < B; ++c) {
-      uint64_t v = 0;
-      for (int row = 0; row < B; ++row)
-        v |= (uint64_t)(__builtin_popcountll(raw[c] & rows[row]) & 1) << (B - 1 - row);
-      dirs[(size_t)c * dims + d] = v;
-    }
+    scramble_dimension(raw, seed, d, &dirs[d], dims, &shifts[d]);
   }
 }
 
@@ -205,6 +251,8 @@ extern "C" {
 
 const char* kg_version(void) { return "kiyosi_amd 0.1.0 (gfx950)"; }
 
+static std::mutex g_setup_mu;  // per-device one-time set-up across contexts
+
 kg_status kg_context_create(int32_t device, kg_context** out) {
   if (!out) return KG_INVALID_INPUT;
   *out = nullptr;
@@ -216,7 +264,14 @@ kg_status kg_context_create(int32_t device, kg_context** out) {
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
       hipEventCreate(&ctx->start) != hipSuccess || hipEventCreate(&ctx->stop) != hipSuccess) {
-    delete ctx;
+    kg_context_destroy(ctx);
+    return KG_DEVICE_ERROR;
+  }
+  try {
+    std::lock_guard<std::mutex> lock(g_setup_mu);
+    kg::noalloc_setup(device);
+  } catch (const kg::Error&) {
+    kg_context_destroy(ctx);
     return KG_DEVICE_ERROR;
   }
   *out = ctx;

@@ -233,6 +233,39 @@ static bool plan_european(const kg_european_terms& o, const kg_bsm_market& m, Eu
 
 static bool needs_peak_scaling(const EuroParams& p) { return p.sign > 0.0 && p.diffusion > 25.0; }
 
+// ------------------------------------------------------------------ no-allocation entry
+// Device memory of kg_vanilla_price_noalloc, one copy per GPU, held until the
+// process ends: the flag word, the merged Moments, Sobol dimension 0 (52
+// direction words and the shift) and the per-block partials.
+constexpr int64_t NOALLOC_MAX_BLOCKS = 32768;  // 8,388,608 samples
+constexpr size_t NOALLOC_FLAGS = 0, NOALLOC_RESULTS = 256, NOALLOC_SOBOL = 512, NOALLOC_PARTIALS = 1024;
+constexpr size_t NOALLOC_BYTES = NOALLOC_PARTIALS + (size_t)NOALLOC_MAX_BLOCKS * 2 * sizeof(Moments);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static void* g_scratch[NOALLOC_MAX_DEVICES] = {};
+
+// Called by kg_context_create with `device` current: the device address of
+// this GPU's copy of the array (the array's name is a host-side stand-in).
+void noalloc_setup(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES || g_scratch[device]) return;
+  void* ptr = nullptr;
+  KG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  g_scratch[device] = ptr;
+}
+
+struct SobolWords {
+  uint64_t dirs[52];
+  uint64_t shift;
+};
+
+// Stores Sobol dimension 0, passed by value, into device memory.
+__global__ void store_sobol_words(SobolWords w, uint64_t* __restrict__ out) {
+  const int t = threadIdx.x;
+  if (t < 52) out[t] = w.dirs[t];
+  if (t == 52) out[52] = w.shift;
+}
+
 }  // namespace kg
 
 using namespace kg;
@@ -332,5 +365,69 @@ extern "C" kg_status kg_vanilla_scenarios(kg_context* ctx, const kg_settings* se
       out[i].device_ms = ms;
       out[i].path_steps = (double)s.path_count;
     }
+  });
+}
+
+extern "C" kg_status kg_vanilla_price_noalloc(kg_context* ctx, const kg_settings* settings,
+                                              const kg_european_terms* option, const kg_bsm_market* market,
+                                              kg_estimate* out) {
+  return guarded(ctx, [&] {
+    if (!option || !market || !out) fail(KG_INVALID_INPUT, "arguments must not be null");
+    Sampler s = validate_settings(settings, 1);
+    if (s.blocks() > NOALLOC_MAX_BLOCKS)
+      fail(KG_INVALID_INPUT, "kg_vanilla_price_noalloc takes at most 8388608 samples (32768 blocks of 256)");
+    unsigned char* base = ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES
+                              ? (unsigned char*)g_scratch[ctx->device]
+                              : nullptr;
+    if (!base) fail(KG_DEVICE_ERROR, "the no-allocation workspace of this device is not set up");
+    uint32_t* flags = (uint32_t*)(base + NOALLOC_FLAGS);
+    Moments* results = (Moments*)(base + NOALLOC_RESULTS);
+    uint64_t* sobol = (uint64_t*)(base + NOALLOC_SOBOL);
+    Moments* parts = (Moments*)(base + NOALLOC_PARTIALS);
+    const kg_european_terms& o = *option;
+    const kg_bsm_market& m = *market;
+    EuroParams p;
+    double known = 0.0;
+    *out = kg_estimate{};
+    if (plan_european(o, m, &p, &known)) {
+      fill_known(known, settings->sampling == KG_SOBOL, out);
+      return;
+    }
+    uint64_t shift = 0;
+    if (s.sobol) {
+      SobolWords w;
+      sobol_dimension_zero(s.seed, w.dirs, &w.shift);
+      shift = w.shift;
+      hipLaunchKernelGGL(store_sobol_words, dim3(1), dim3(64), 0, ctx->stream, w, sobol);
+      KG_HIP(hipGetLastError());
+    }
+    double log_scale = p.log_scale;
+    double ms = 0.0;
+    Moments r[2];
+    auto pass = [&](const EuroParams& q) {
+      clear_flag_word(ctx, flags);
+      timing_begin(ctx);
+      if (s.fp32)
+        hipLaunchKernelGGL(european_kernel<true>, dim3(s.blocks()), dim3(128), 0, ctx->stream, q, s.seed,
+                           s.samples(), s.path_count, (int)s.antithetic, (int)s.sobol, sobol, shift, parts, flags);
+      else
+        hipLaunchKernelGGL(european_kernel<false>, dim3(s.blocks()), dim3(128), 0, ctx->stream, q, s.seed,
+                           s.samples(), s.path_count, (int)s.antithetic, (int)s.sobol, sobol, shift, parts, flags);
+      KG_HIP(hipGetLastError());
+      double t = 0.0;
+      finish_sampling_into(ctx, parts, s.blocks(), 2, results, flags, r, &t);
+      ms += t;
+    };
+    if (needs_peak_scaling(p)) {
+      p.mode = 1;
+      pass(p);
+      log_scale = r[0].scale - 746.0;
+      p.mode = 2;
+      p.peak_shift = log_scale;
+    }
+    pass(p);
+    out->device_ms = ms;
+    out->path_steps = (double)s.path_count;
+    fill_estimate(s, r[0], r[1], log_scale, m.valuation_time == o.expiry_time, true, out);
   });
 }

@@ -282,6 +282,19 @@ const SobolSetup& sobol_setup(kg_context* ctx, uint32_t dims, uint64_t seed);
 // shifts on the host without touching the cache.
 void sobol_tables(kg_context* ctx, uint32_t dims, uint64_t seed, std::vector<uint64_t>& dirs,
                   std::vector<uint64_t>& shifts);
+// Scrambled directions (52 columns) and shift of Sobol dimension 0 for a
+// seed, on the host without allocating.
+void sobol_dimension_zero(uint64_t seed, uint64_t (&dirs)[52], uint64_t* shift);
+// finish_sampling with caller-given device memory: merges the partials into
+// d_out (slots Moments, then the flag word copied from d_flags), reads them
+// back and raises on a flag; allocates nothing. At most 4 slots.
+void finish_sampling_into(kg_context* ctx, const Moments* partials, int64_t blocks, int slots, Moments* d_out,
+                          const uint32_t* d_flags, Moments* host_out, double* device_ms);
+// Per-device set-up of kg_vanilla_price_noalloc's workspace (european.hip),
+// with the device current; once per device.
+void noalloc_setup(int device);
+// Clears a caller-given device flag word with a kernel on the context stream.
+void clear_flag_word(kg_context* ctx, uint32_t* word);
 // Event timing on the context stream.
 void timing_begin(kg_context* ctx);
 double timing_end(kg_context* ctx);
