@@ -399,26 +399,31 @@ static void bridge_rows(kg_context* ctx, const AcHostPlan& h, int dims, uint64_t
   }
   blob.assign((const unsigned char*)rows.data(), (const unsigned char*)(rows.data() + n));
   *table_offset = (blob.size() + 255) & ~(size_t)255;
-  if (SOBOL && LDS) {
-    constexpr int SHIFT = FP32 ? 20 : 0;
-    const SobolSetup& so = sobol_setup(ctx, (uint32_t)dims, seed);
-    std::vector<W> table(n * 32);
-    for (size_t k = 0; k < n; ++k) {
-      const int d = rows[k].first;
-      for (int hix = 0; hix < 16; ++hix) {
-        uint64_t a = 0, b = 0;
-        for (int bit = 0; bit < 4; ++bit)
-          if (hix & (1 << bit)) {
-            a ^= so.directions[(size_t)bit * dims + d];
-            b ^= so.directions[(size_t)(4 + bit) * dims + d];
-          }
-        table[k * 16 + hix] = (W)(a >> SHIFT);
-        table[(n + k) * 16 + hix] = (W)(b >> SHIFT);
-      }
+  // Room for the first-dimension Sobol tables, which bridge_table_kernel
+  // fills on the device from the scrambled directions.
+  if (SOBOL && LDS) blob.resize(*table_offset + n * 32 * sizeof(W), 0);
+}
+
+// The Sobol table rows of each interval's first dimension d = rows[k].first
+// (lo[k][h] = directions 0..3 selected by h, hi[k][h] = directions 4..7),
+// stored >> SHIFT: one thread per (interval, h).
+template <bool FP32>
+__global__ void __launch_bounds__(256)
+    bridge_table_kernel(const IntervalRow<typename std::conditional<FP32, float, double>::type>* __restrict__ rows,
+                        int n, const uint64_t* __restrict__ dirs, int dims, SobolWord<FP32>* __restrict__ table) {
+  constexpr int SHIFT = FP32 ? 20 : 0;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * 16) return;
+  const int k = t >> 4, hix = t & 15;
+  const int d = rows[k].first;
+  uint64_t a = 0, b = 0;
+  for (int bit = 0; bit < 4; ++bit)
+    if (hix & (1 << bit)) {
+      a ^= dirs[(size_t)bit * dims + d];
+      b ^= dirs[(size_t)(4 + bit) * dims + d];
     }
-    blob.resize(*table_offset);
-    blob.insert(blob.end(), (const unsigned char*)table.data(), (const unsigned char*)(table.data() + table.size()));
-  }
+  table[k * 16 + hix] = (SobolWord<FP32>)(a >> SHIFT);
+  table[(n + k) * 16 + hix] = (SobolWord<FP32>)(b >> SHIFT);
 }
 
 // Scenario lanes on shared draws: lane l's total and pair moments of the
@@ -515,6 +520,10 @@ static void launch_bridge(kg_context* ctx, const AcPlanDev& p, const Sampler& s,
     fill_lds = sobol_lds_bytes(dims, FP32) + (size_t)FILL_WAVES * (dims + 1) * sizeof(W);
   }
   if (events_lds > BRIDGE_EVENTS_LDS_LIMIT) fail(KG_DEVICE_ERROR, "bridge plan exceeds its pass-one LDS");
+  if (SOBOL && LDS)
+    hipLaunchKernelGGL(bridge_table_kernel<FP32>, dim3((unsigned)((p.n_intervals * 16 + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (const IntervalRow<R>*)d_rows, p.n_intervals, dirs, dims,
+                       (W*)const_cast<unsigned char*>(d_rows + table_offset));
   hipLaunchKernelGGL((bridge_events_kernel<SOBOL, FP32, LDS>), dim3(blocks), dim3(256), events_lds, ctx->stream, p,
                      s.seed, samples, s.path_count, (int)s.antithetic, dirs, shifts, values, d_rows,
                      d_rows + table_offset, base, queue, counts, flags);

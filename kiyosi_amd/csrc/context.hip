@@ -188,34 +188,87 @@ void sobol_dimension_zero(uint64_t seed, uint64_t (&dirs)[52], uint64_t* shift) 
   scramble_dimension(raw, seed, 0, dirs, 1, shift);
 }
 
-// SobolSequence::new (engines/monte_carlo/sobol.rs:85-134): Joe-Kuo
-// recurrence to 52-bit direction integers, then linear matrix scrambling and
-// a digital shift drawn from Pcg32(seed, dimension).
+// The unscrambled 52-bit direction integers of dimensions [first, dims) from
+// the Joe-Kuo records (sobol.rs:85-99): dimension 0 is the identity,
+// others follow the primitive-polynomial recurrence from their initial
+// values. raw[(d - first) * 52 + c]; `pos` is the record offset of `first`.
+static void raw_directions(const std::vector<uint32_t>& table, uint32_t first, uint32_t dims, size_t* pos,
+                           std::vector<uint64_t>& raw) {
+  constexpr int B = 52;
+  raw.assign((size_t)(dims - first) * B, 0);
+  for (uint32_t d = first; d < dims; ++d) {
+    uint64_t* r = &raw[(size_t)(d - first) * B];
+    if (d == 0) {
+      for (int c = 0; c < B; ++c) r[c] = 1ULL << (B - 1 - c);
+      continue;
+    }
+    if (*pos + 2 > table.size()) fail(KG_INVALID_INPUT, "Sobol direction table is truncated");
+    const int degree = (int)table[*pos];
+    const uint32_t poly = table[*pos + 1];
+    for (int c = 0; c < degree && c < B; ++c) r[c] = (uint64_t)table[*pos + 2 + c] << (B - 1 - c);
+    *pos += (size_t)degree + 2;
+    for (int c = degree; c < B; ++c) {
+      uint64_t v = r[c - degree] ^ (r[c - degree] >> degree);
+      for (int k = 1; k < degree; ++k)
+        if ((poly >> (degree - 1 - k)) & 1u) v ^= r[c - k];
+      r[c] = v;
+    }
+  }
+}
+
+// SobolSequence::new (engines/monte_carlo/sobol.rs:85-134) on the host:
+// raw directions, then linear matrix scrambling and a digital shift drawn
+// from Pcg32(seed, dimension).
 static void build_sobol(const std::vector<uint32_t>& table, uint32_t dims, uint64_t seed,
                         std::vector<uint64_t>& dirs, std::vector<uint64_t>& shifts) {
   constexpr int B = 52;
-  constexpr uint64_t MASK = (1ULL << B) - 1;
   dirs.assign((size_t)dims * B, 0);
   shifts.assign(dims, 0);
+  std::vector<uint64_t> raw;
   size_t pos = 0;
+  raw_directions(table, 0, dims, &pos, raw);
   for (uint32_t d = 0; d < dims; ++d) {
-    uint64_t raw[B] = {};
-    if (d == 0) {
-      for (int c = 0; c < B; ++c) raw[c] = 1ULL << (B - 1 - c);
+    uint64_t r[B];
+    for (int c = 0; c < B; ++c) r[c] = raw[(size_t)d * B + c];
+    scramble_dimension(r, seed, d, &dirs[d], dims, &shifts[d]);
+  }
+}
+
+// The same scrambling on the device, one wave per dimension: lane r draws
+// the scramble row of bit 51 - r (PCG words 1 + r of Pcg32(seed, d), i.e.
+// outputs 2 + 2r and 3 + 2r, reached by skip-ahead), lane 52 the shift (word
+// 0); then lane c forms scrambled column c as the parities of raw column c
+// against the 52 rows.
+__global__ void __launch_bounds__(256) sobol_scramble_kernel(const uint64_t* __restrict__ raw, uint32_t dims,
+                                                             uint64_t seed, uint64_t* __restrict__ dirs,
+                                                             uint64_t* __restrict__ shifts) {
+  constexpr int B = 52;
+  constexpr uint64_t MASK = (1ULL << B) - 1;
+  __shared__ uint64_t rows[4][B];
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  const uint32_t d = blockIdx.x * 4 + wave;
+  const bool live = d < dims;
+  if (live && lane <= B) {
+    Pcg32 g;
+    g.init(seed, d);
+    const int word = lane == B ? 0 : 1 + lane;
+    g.advance(2 * (uint64_t)word);
+    const uint64_t hi = g.next();
+    const uint64_t w = (hi << 32) | (uint64_t)g.next();
+    if (lane == B) {
+      shifts[d] = w & MASK;
     } else {
-      if (pos + 2 > table.size()) fail(KG_INVALID_INPUT, "Sobol direction table is truncated");
-      const int degree = (int)table[pos];
-      const uint32_t poly = table[pos + 1];
-      for (int c = 0; c < degree && c < B; ++c) raw[c] = (uint64_t)table[pos + 2 + c] << (B - 1 - c);
-      pos += (size_t)degree + 2;
-      for (int c = degree; c < B; ++c) {
-        uint64_t v = raw[c - degree] ^ (raw[c - degree] >> degree);
-        for (int k = 1; k < degree; ++k)
-          if ((poly >> (degree - 1 - k)) & 1u) v ^= raw[c - k];
-        raw[c] = v;
-      }
+      const int bit = B - 1 - lane;
+      rows[wave][lane] = (w & (MASK << (bit + 1))) | (1ULL << bit);
     }
-    scramble_dimension(raw, seed, d, &dirs[d], dims, &shifts[d]);
+  }
+  __syncthreads();
+  if (live && lane < B) {
+    const uint64_t col = raw[(size_t)d * B + lane];
+    uint64_t v = 0;
+    for (int row = 0; row < B; ++row)
+      v |= (uint64_t)(__builtin_popcountll(col & rows[wave][row]) & 1) << (B - 1 - row);
+    dirs[(size_t)lane * dims + d] = v;
   }
 }
 
@@ -225,20 +278,31 @@ void sobol_tables(kg_context* ctx, uint32_t dims, uint64_t seed, std::vector<uin
   build_sobol(ctx->joe_kuo, dims, seed, dirs, shifts);
 }
 
+// The raw directions of the first `dims` dimensions on the device, kept
+// across calls and seeds (they do not depend on the seed); grown on demand.
+static const uint64_t* raw_on_device(kg_context* ctx, uint32_t dims) {
+  if (ctx->sobol_raw_dims >= dims) return (const uint64_t*)ctx->sobol_raw.ptr;
+  if (ctx->joe_kuo.empty()) ctx->joe_kuo.assign(kg_joe_kuo_words, kg_joe_kuo_end);
+  std::vector<uint64_t> raw;
+  size_t pos = 0;
+  raw_directions(ctx->joe_kuo, 0, dims, &pos, raw);
+  KG_HIP(hipStreamSynchronize(ctx->stream));  // no kernel may still read the old buffer
+  void* d = ctx->sobol_raw.reserve(raw.size() * sizeof(uint64_t));
+  KG_HIP(hipMemcpy(d, raw.data(), raw.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  ctx->sobol_raw_dims = dims;
+  return (const uint64_t*)d;
+}
+
 const SobolSetup& sobol_setup(kg_context* ctx, uint32_t dims, uint64_t seed) {
   SobolSetup& s = ctx->sobol;
   if (s.valid && s.dims == dims && s.seed == seed) return s;
-  if (ctx->joe_kuo.empty())
-    ctx->joe_kuo.assign(kg_joe_kuo_words, kg_joe_kuo_end);
   s.valid = false;
-  build_sobol(ctx->joe_kuo, dims, seed, s.directions, s.shifts);
-  void* dd = s.d_directions.reserve(s.directions.size() * sizeof(uint64_t));
-  void* ds = s.d_shifts.reserve(s.shifts.size() * sizeof(uint64_t));
-  KG_HIP(hipMemcpyAsync(dd, s.directions.data(), s.directions.size() * sizeof(uint64_t),
-                        hipMemcpyHostToDevice, ctx->stream));
-  KG_HIP(hipMemcpyAsync(ds, s.shifts.data(), s.shifts.size() * sizeof(uint64_t),
-                        hipMemcpyHostToDevice, ctx->stream));
-  KG_HIP(hipStreamSynchronize(ctx->stream));
+  const uint64_t* raw = raw_on_device(ctx, dims);
+  uint64_t* dd = (uint64_t*)s.d_directions.reserve((size_t)dims * 52 * sizeof(uint64_t));
+  uint64_t* ds = (uint64_t*)s.d_shifts.reserve((size_t)dims * sizeof(uint64_t));
+  hipLaunchKernelGGL(sobol_scramble_kernel, dim3((dims + 3) / 4), dim3(256), 0, ctx->stream, raw, dims, seed, dd,
+                     ds);
+  KG_HIP(hipGetLastError());
   s.dims = dims;
   s.seed = seed;
   s.valid = true;
@@ -290,6 +354,7 @@ void kg_context_destroy(kg_context* ctx) {
     ctx->results.release();
     ctx->sobol.d_directions.release();
     ctx->sobol.d_shifts.release();
+    ctx->sobol_raw.release();
     for (kg::DeviceBuffer* b : {&ctx->book_blob, &ctx->book_items, &ctx->book_tiles, &ctx->book_parts,
                                 &ctx->book_out, &ctx->book_counter, &ctx->policy, &ctx->lsmc_tables,
                                 &ctx->mlp_state})

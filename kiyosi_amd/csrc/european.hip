@@ -72,10 +72,11 @@ __device__ inline uint64_t sobol_dim0(const uint64_t* __restrict__ dirs, uint64_
 // 4 * (j/2)).
 template <bool FP32>
 __device__ inline void euro_normals(uint64_t seed, int64_t block, int j, int64_t n0, int sobol,
-                                    const uint64_t* __restrict__ dirs, uint64_t shift, double z[2], float zf[2]) {
+                                    const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shift,
+                                    double z[2], float zf[2]) {
   if (sobol) {
     for (int s = 0; s < 2; ++s) {
-      uint64_t bits = sobol_dim0(dirs, shift, (uint64_t)(n0 + s));
+      uint64_t bits = sobol_dim0(dirs, *shift, (uint64_t)(n0 + s));
       if (FP32)
         zf[s] = f32::normal_from_bits((uint32_t)(bits >> 20));
       else
@@ -95,7 +96,7 @@ __device__ inline void euro_normals(uint64_t seed, int64_t block, int j, int64_t
 template <bool FP32>
 __global__ void __launch_bounds__(128)
     european_kernel(EuroParams p, uint64_t seed, int64_t samples, int64_t paths, int antithetic,
-                    int sobol, const uint64_t* __restrict__ dirs, uint64_t shift,
+                    int sobol, const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shift,
                     Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
   __shared__ Moments smem[2 * 2];
   const int64_t block = blockIdx.x;
@@ -139,7 +140,8 @@ template <bool FP32>
 __global__ void __launch_bounds__(128)
     european_scenarios_kernel(const EuroParams* __restrict__ params, int n_scen, uint64_t seed, int64_t samples,
                               int64_t paths, int antithetic, int sobol, const uint64_t* __restrict__ dirs,
-                              uint64_t shift, Moments* __restrict__ partials, uint32_t* __restrict__ flags) {
+                              const uint64_t* __restrict__ shift, Moments* __restrict__ partials,
+                              uint32_t* __restrict__ flags) {
   __shared__ Moments smem[1 * 2];
   const int64_t block = blockIdx.x;
   const int j = 2 * threadIdx.x;
@@ -175,11 +177,11 @@ static void european_pass(kg_context* ctx, const Sampler& s, const EuroParams& p
   Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 2 * blocks);
   uint32_t* flags = reset_flags(ctx);
   const uint64_t* dirs = nullptr;
-  uint64_t shift = 0;
+  const uint64_t* shift = nullptr;
   if (s.sobol) {
     const SobolSetup& so = sobol_setup(ctx, 1, s.seed);
     dirs = (const uint64_t*)so.d_directions.ptr;
-    shift = so.shifts[0];
+    shift = (const uint64_t*)so.d_shifts.ptr;
   }
   timing_begin(ctx);
   if (s.fp32)
@@ -331,11 +333,11 @@ extern "C" kg_status kg_vanilla_scenarios(kg_context* ctx, const kg_settings* se
       upload_async(ctx, d_params, params.data(), params.size() * sizeof(EuroParams));
       Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * (size_t)sc * blocks);
       const uint64_t* dirs = nullptr;
-      uint64_t shift = 0;
+      const uint64_t* shift = nullptr;
       if (s.sobol) {
         const SobolSetup& so = sobol_setup(ctx, 1, s.seed);
         dirs = (const uint64_t*)so.d_directions.ptr;
-        shift = so.shifts[0];
+        shift = (const uint64_t*)so.d_shifts.ptr;
       }
       uint32_t* flags = reset_flags(ctx);
       timing_begin(ctx);
@@ -393,11 +395,10 @@ extern "C" kg_status kg_vanilla_price_noalloc(kg_context* ctx, const kg_settings
       fill_known(known, settings->sampling == KG_SOBOL, out);
       return;
     }
-    uint64_t shift = 0;
+    const uint64_t* shift = sobol + 52;
     if (s.sobol) {
       SobolWords w;
       sobol_dimension_zero(s.seed, w.dirs, &w.shift);
-      shift = w.shift;
       hipLaunchKernelGGL(store_sobol_words, dim3(1), dim3(64), 0, ctx->stream, w, sobol);
       KG_HIP(hipGetLastError());
     }

@@ -67,12 +67,12 @@ struct BridgeBaseKey {
   bool valid = false;
 };
 
+// Scrambled directions [column * dims + dim] (52 columns) and shifts [dim]
+// of the last (dims, seed), built on the device on the context's stream.
 struct SobolSetup {
   uint32_t dims = 0;
   uint64_t seed = 0;
   bool valid = false;
-  std::vector<uint64_t> directions;  // [column * dims + dim], 52 columns
-  std::vector<uint64_t> shifts;      // [dim]
   DeviceBuffer d_directions, d_shifts;
 };
 
@@ -92,6 +92,8 @@ struct kg_context {
   kg::BridgeBaseKey bridge_base_key;
   kg::SobolSetup sobol;
   std::vector<uint32_t> joe_kuo;  // direction-number records, loaded once
+  kg::DeviceBuffer sobol_raw;     // unscrambled directions [dim][52], seed-independent
+  uint32_t sobol_raw_dims = 0;
   // Book engine: device buffers and the autotuned tile shape per precision.
   kg::DeviceBuffer book_blob, book_items, book_tiles, book_parts, book_out, book_counter;
   // American engine: exercise policy rows, model tables and MLP state.
@@ -276,7 +278,7 @@ void merge_and_fetch(kg_context* ctx, const Moments* partials, int64_t blocks, i
 uint32_t* reset_flags(kg_context* ctx);
 // Reads the flag word back (synchronising the stream) and throws on a raised flag.
 void check_flags(kg_context* ctx);
-// Builds (or reuses) the scrambled Sobol directions on the host and device.
+// Builds (or reuses) the scrambled Sobol directions on the device.
 const SobolSetup& sobol_setup(kg_context* ctx, uint32_t dims, uint64_t seed);
 // Builds scrambled Sobol directions ([column * dims + dim], 52 columns) and
 // shifts on the host without touching the cache.
