@@ -117,6 +117,27 @@ const char* kg_last_error(const kg_context* ctx);
 /* Version string of the library. */
 const char* kg_version(void);
 
+/* ---- Cooperative cancellation: CancellationToken (crates/kiyosi/src/
+ * cancellation.rs:33-68), whose flag the reference engines check per block,
+ * per 64 paths and per 1,024 steps (engines/monte_carlo.rs:1749-1751,
+ * 1813-1828). A token is one word of pinned, device-visible host memory
+ * owned by the library; kg_cancel raises it from any thread and it stays
+ * raised until kg_cancel_reset (a cancelled request returns no partial
+ * result). A context bound to a token checks it when a call starts and
+ * between the launches of multi-launch calls (LSMC dates, MLP training
+ * steps, scenario groups, book replays and autotuning), and the persistent
+ * book kernel reads it between tiles; a call that sees it raised returns
+ * KG_INTERRUPTED (Error::Io with ErrorKind::Interrupted) with every kernel
+ * ended normally, and the context stays usable. */
+typedef struct kg_cancel_token kg_cancel_token;
+kg_status kg_cancel_token_create(kg_cancel_token** out);
+void kg_cancel_token_destroy(kg_cancel_token* token);
+void kg_cancel(kg_cancel_token* token);
+void kg_cancel_reset(kg_cancel_token* token);
+int32_t kg_cancel_requested(const kg_cancel_token* token);
+/* Binds `token` (NULL unbinds) to the context's later calls. */
+kg_status kg_context_set_cancel_token(kg_context* ctx, kg_cancel_token* token);
+
 /* ---- European vanilla: MonteCarloVanillaEngine::price_with_statistics
  * (monte_carlo.rs:1307-1320), sample (:1093-1141), EuropeanPayoff
  * (:1165-1252). option_type: 1 call, -1 put. */

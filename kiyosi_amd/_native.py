@@ -21,6 +21,12 @@ _contexts = {}
 P = C.POINTER
 SIGNATURES = {
     "kg_version": (C.c_char_p, []),
+    "kg_cancel_token_create": (C.c_int, [P(C.c_void_p)]),
+    "kg_cancel_token_destroy": (None, [C.c_void_p]),
+    "kg_cancel": (None, [C.c_void_p]),
+    "kg_cancel_reset": (None, [C.c_void_p]),
+    "kg_cancel_requested": (C.c_int32, [C.c_void_p]),
+    "kg_context_set_cancel_token": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kg_context_create": (C.c_int, [C.c_int32, P(C.c_void_p)]),
     "kg_context_destroy": (None, [C.c_void_p]),
     "kg_last_error": (C.c_char_p, [C.c_void_p]),
@@ -91,8 +97,48 @@ def lib():
         return _lib
 
 
+# Calls expected to run longer than about one signal-polling period (10 ms
+# of device time) run on a worker thread; see Context.call.
+LONG_CALL_PATHS = 1 << 22
+SIGNAL_POLL_S = 0.01
+
+
+def run_interruptible(run, cancel, poll=SIGNAL_POLL_S):
+    """Runs `run()` on a worker thread while the calling (main) thread waits in
+    steps of `poll` seconds, so that Python handles signals between steps; on
+    KeyboardInterrupt it calls `cancel()`, waits for the worker to end and
+    re-raises, as kiyosi's Python binding does for every request
+    (crates/kiyosi-python/src/cancellation.rs:5-47)."""
+    done = threading.Event()
+    result = {}
+
+    def worker():
+        try:
+            result["value"] = run()
+        except BaseException as error:  # re-raised on the caller's thread
+            result["error"] = error
+        finally:
+            done.set()
+
+    thread = threading.Thread(target=worker, name="kiyosi-request", daemon=True)
+    thread.start()
+    try:
+        while not done.wait(poll):
+            pass
+    except KeyboardInterrupt:
+        cancel()
+        done.wait()
+        thread.join()
+        raise
+    thread.join()
+    if "error" in result:
+        raise result["error"]
+    return result["value"]
+
+
 class Context:
-    """One HIP device context (stream, workspace, Sobol cache)."""
+    """One HIP device context (stream, workspace, Sobol cache) with its own
+    cancellation token."""
 
     def __init__(self, device: int = 0):
         handle = C.c_void_p()
@@ -101,15 +147,43 @@ class Context:
             raise_for_status(status, f"cannot create a context on HIP device {device}")
         self.handle = handle
         self.device = device
+        token = C.c_void_p()
+        status = lib().kg_cancel_token_create(C.byref(token))
+        if status:
+            raise_for_status(status, "cannot create a cancellation token")
+        self.token = token
+        lib().kg_context_set_cancel_token(self.handle, self.token)
 
     def check(self, status: int) -> None:
         if status:
             raise_for_status(status, lib().kg_last_error(self.handle).decode())
 
+    def cancel(self) -> None:
+        """Raises the token: the running call (and later ones until the next
+        call starts) return InterruptedError."""
+        lib().kg_cancel(self.token)
+
+    def call(self, fn, *args, interruptible=False):
+        """Runs fn(handle, *args) for one request and raises on a failed status.
+        The token is lowered first (each request starts uncancelled). Long
+        requests (`interruptible`) run on a worker thread while the main
+        thread polls signals every 10 ms, and a KeyboardInterrupt cancels
+        them; shorter ones run on the calling thread, where a signal is
+        handled as soon as they return."""
+        lib().kg_cancel_reset(self.token)
+        if interruptible and threading.current_thread() is threading.main_thread():
+            status = run_interruptible(lambda: fn(self.handle, *args), self.cancel)
+        else:
+            status = fn(self.handle, *args)
+        self.check(status)
+
     def __del__(self):
         if getattr(self, "handle", None) and _lib is not None:
             _lib.kg_context_destroy(self.handle)
             self.handle = None
+        if getattr(self, "token", None) and _lib is not None:
+            _lib.kg_cancel_token_destroy(self.token)
+            self.token = None
 
 
 def context(device: int = 0) -> Context:

@@ -144,13 +144,23 @@ template <bool FP32, int SPT>
 __global__ void __launch_bounds__(256)
     book_kernel(const BookItem* __restrict__ items, const int2* __restrict__ tiles, int n_tiles,
                 int* __restrict__ counter, Moments* __restrict__ partials,
-                uint32_t* __restrict__ flags) {
+                uint32_t* __restrict__ flags, const volatile int32_t* cancel) {
   __shared__ int s_tile;
   __shared__ double s_pos[BLOCK_SAMPLES], s_neg[BLOCK_SAMPLES];
   __shared__ unsigned char s_state[BLOCK_SAMPLES];
   uint32_t bad = 0;
-  for (;;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+  for (int taken = 0;; ++taken) {
+    if (threadIdx.x == 0) {
+      // Every eighth tile a workgroup takes, thread 0 reads the cancellation
+      // word (host memory) first; a raised word ends the workgroup's loop
+      // like an exhausted queue.
+      if (cancel && (taken & 7) == 0 && *cancel != 0) {
+        s_tile = n_tiles;
+        raise_flag(flags, FLAG_CANCELLED);
+      } else {
+        s_tile = atomicAdd(counter, 1);
+      }
+    }
     __syncthreads();
     // The tile index is the same in every lane; saying so lets the contract's
     // fields and step tables be read with scalar loads.
@@ -868,20 +878,21 @@ static int cu_count(kg_context* ctx) {
 template <bool FP32>
 static void launch_book(kg_context* ctx, int spt, int wpc, const BookItem* items, const int2* tiles,
                         int n_tiles, int* counter, Moments* parts, uint32_t* flags) {
+  const int32_t* cancel = cancel_word(ctx);
   const int threads = BLOCK_SAMPLES / spt;
   const int grid = std::max(1, std::min(n_tiles, cu_count(ctx) * wpc));
   switch (spt) {
     case 1:
       hipLaunchKernelGGL((book_kernel<FP32, 1>), dim3(grid), dim3(threads), 0, ctx->stream, items, tiles,
-                         n_tiles, counter, parts, flags);
+                         n_tiles, counter, parts, flags, cancel);
       break;
     case 2:
       hipLaunchKernelGGL((book_kernel<FP32, 2>), dim3(grid), dim3(threads), 0, ctx->stream, items, tiles,
-                         n_tiles, counter, parts, flags);
+                         n_tiles, counter, parts, flags, cancel);
       break;
     default:
       hipLaunchKernelGGL((book_kernel<FP32, 4>), dim3(grid), dim3(threads), 0, ctx->stream, items, tiles,
-                         n_tiles, counter, parts, flags);
+                         n_tiles, counter, parts, flags, cancel);
       break;
   }
   KG_HIP(hipGetLastError());
@@ -914,6 +925,7 @@ static void autotune(kg_context* ctx, bool fp32, const BookItem* items,
   int best_spt = 1, best_wpc = 4;
   for (int spt : {1, 2, 4})
     for (int wpc : {2, 4, 8}) {
+      check_cancel(ctx);
       run_book_once(ctx, fp32, spt, wpc, items, d_sample, (int)sample.size(), counter, parts, flags);
       timing_begin(ctx);
       run_book_once(ctx, fp32, spt, wpc, items, d_sample, (int)sample.size(), counter, parts, flags);
@@ -1080,25 +1092,33 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
     const bool use_graph = tuning && tuning->use_graph;
     const int replays = tuning ? std::max(0, tuning->replays) : 0;
     if (use_graph) {
-      hipGraph_t graph;
-      hipGraphExec_t exec;
+      // Destroys the graph on every exit, a cancelled run's included.
+      struct GraphGuard {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        ~GraphGuard() {
+          if (exec) (void)hipGraphExecDestroy(exec);
+          if (graph) (void)hipGraphDestroy(graph);
+        }
+      } guard;
       KG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       run_book_once(ctx, fp32, spt, wpc, d_items, d_tiles, (int)tiles.size(), d_counter, d_parts, flags);
       hipLaunchKernelGGL(merge_book, dim3(items.size()), dim3(MERGE_THREADS), 0, ctx->stream, d_items, d_parts, d_out);
-      KG_HIP(hipStreamEndCapture(ctx->stream, &graph));
-      KG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      KG_HIP(hipStreamEndCapture(ctx->stream, &guard.graph));
+      KG_HIP(hipGraphInstantiate(&guard.exec, guard.graph, nullptr, nullptr, 0));
+      hipGraphExec_t exec = guard.exec;
       timing_begin(ctx);
       KG_HIP(hipGraphLaunch(exec, ctx->stream));
       device_ms = timing_end(ctx);
+      check_flags(ctx);  // a cancelled first run ends the call before its replays
       if (replays) {
         timing_begin(ctx);
         for (int k = 0; k < replays; ++k) KG_HIP(hipGraphLaunch(exec, ctx->stream));
         device_ms = timing_end(ctx) / replays;
       }
-      KG_HIP(hipGraphExecDestroy(exec));
-      KG_HIP(hipGraphDestroy(graph));
     } else {
       for (int k = 0; k <= replays; ++k) {
+        if (k) check_flags(ctx);
         timing_begin(ctx);
         run_book_once(ctx, fp32, spt, wpc, d_items, d_tiles, (int)tiles.size(), d_counter, d_parts, flags);
         hipLaunchKernelGGL(merge_book, dim3(items.size()), dim3(MERGE_THREADS), 0, ctx->stream, d_items, d_parts, d_out);

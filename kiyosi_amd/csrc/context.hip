@@ -369,4 +369,54 @@ void kg_context_destroy(kg_context* ctx) {
 }
 
 const char* kg_last_error(const kg_context* ctx) { return ctx ? ctx->error.c_str() : ""; }
+
+kg_status kg_cancel_token_create(kg_cancel_token** out) {
+  if (!out) return KG_INVALID_INPUT;
+  *out = nullptr;
+  kg_cancel_token* t = new (std::nothrow) kg_cancel_token();
+  if (!t) return KG_COMPUTATION_FAILURE;
+  void* host = nullptr;
+  // Coherent, so a kernel reading the word sees a raise made while it runs.
+  if (hipHostMalloc(&host, 256, hipHostMallocMapped | hipHostMallocPortable | hipHostMallocCoherent) !=
+      hipSuccess) {
+    delete t;
+    return KG_DEVICE_ERROR;
+  }
+  void* dev = nullptr;
+  if (hipHostGetDevicePointer(&dev, host, 0) != hipSuccess) {
+    (void)hipHostFree(host);
+    delete t;
+    return KG_DEVICE_ERROR;
+  }
+  t->host = (volatile int32_t*)host;
+  *t->host = 0;
+  t->device = (int32_t*)dev;
+  *out = t;
+  return KG_OK;
+}
+
+void kg_cancel_token_destroy(kg_cancel_token* token) {
+  if (!token) return;
+  if (token->host) (void)hipHostFree((void*)token->host);
+  delete token;
+}
+
+void kg_cancel(kg_cancel_token* token) {
+  if (token) __atomic_store_n(token->host, 1, __ATOMIC_RELEASE);
+}
+
+void kg_cancel_reset(kg_cancel_token* token) {
+  if (token) __atomic_store_n(token->host, 0, __ATOMIC_RELEASE);
+}
+
+int32_t kg_cancel_requested(const kg_cancel_token* token) {
+  return token ? __atomic_load_n(token->host, __ATOMIC_ACQUIRE) : 0;
+}
+
+kg_status kg_context_set_cancel_token(kg_context* ctx, kg_cancel_token* token) {
+  if (!ctx) return KG_INVALID_INPUT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->cancel = token;
+  return KG_OK;
+}
 }

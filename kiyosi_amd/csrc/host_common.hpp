@@ -78,8 +78,15 @@ struct SobolSetup {
 
 }  // namespace kg
 
+// A cancellation token: one int32 in pinned, device-visible host memory.
+struct kg_cancel_token {
+  volatile int32_t* host = nullptr;
+  int32_t* device = nullptr;
+};
+
 struct kg_context {
   int device = 0;
+  kg_cancel_token* cancel = nullptr;  // bound token, or none
   hipStream_t stream = nullptr;
   hipEvent_t start = nullptr, stop = nullptr;
   std::mutex mu;
@@ -111,11 +118,21 @@ namespace kg {
 
 // Runs `body` with the context's device current and its lock held,
 // translating exceptions into status codes and messages.
+// The bound token's state; the device word the persistent kernels poll.
+inline bool cancel_requested(const kg_context* ctx) { return ctx->cancel && *ctx->cancel->host != 0; }
+inline const int32_t* cancel_word(const kg_context* ctx) { return ctx->cancel ? ctx->cancel->device : nullptr; }
+// Raises KG_INTERRUPTED when the bound token is raised: called when a call
+// starts and between the launches of multi-launch calls.
+inline void check_cancel(const kg_context* ctx) {
+  if (cancel_requested(ctx)) fail(KG_INTERRUPTED, "operation interrupted");
+}
+
 template <class F>
 kg_status guarded(kg_context* ctx, F&& body) {
   if (!ctx) return KG_INVALID_INPUT;
   std::lock_guard<std::mutex> lock(ctx->mu);
   try {
+    check_cancel(ctx);
     KG_HIP(hipSetDevice(ctx->device));
     ctx->staging_used = 0;  // the previous call ended with a synchronisation
     body();

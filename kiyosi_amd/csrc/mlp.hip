@@ -779,6 +779,7 @@ static void run(kg_context* ctx, LsmcHost& h, const kg_lsmc_options& opt, kg_lsm
   bool first = true;
   int adam_step = 0;
   for (int step = S - 2; step >= 0; --step) {
+    check_cancel(ctx);  // the loop synchronises once per date
     hipLaunchKernelGGL((mlp_mark<FP32, LV>), dim3(blocks), dim3(256), 0, ctx->stream, m, step, samples,
                        tr.path_count, (int)tr.antithetic, coords, spot, cash, counts, flags);
     KG_HIP(hipGetLastError());

@@ -92,8 +92,8 @@ class _MonteCarloEngine:
         ctx = _native.context(engine.device)
         out = _abi.Estimate()
         fn, terms, market = engine._entry(instrument, context)
-        ctx.check(fn(ctx.handle, C.byref(engine._settings()), C.byref(terms), C.byref(market),
-                     C.byref(out)))
+        ctx.call(fn, C.byref(engine._settings()), C.byref(terms), C.byref(market), C.byref(out),
+                 interruptible=engine.path_count > _native.LONG_CALL_PATHS)
         return MonteCarloEstimate(out)
 
     def price_with_seed(self, instrument, context):
@@ -179,9 +179,9 @@ class MonteCarloVanillaEngine(_MonteCarloEngine):
                                    int(training_path_count or 0), epochs, batch, learning_rate)
         report = _abi.LsmcReport()
         model = american_model(context)
-        ctx.check(_native.lib().kg_american_price(ctx.handle, C.byref(engine._settings()),
-                                                  C.byref(option._american_terms(engine.step_count)),
-                                                  C.byref(model), C.byref(options), C.byref(report)))
+        ctx.call(_native.lib().kg_american_price, C.byref(engine._settings()),
+                 C.byref(option._american_terms(engine.step_count)), C.byref(model), C.byref(options),
+                 C.byref(report), interruptible=True)
         return report
 
     def price_strikes(self, instrument, context, *, strikes):
@@ -474,6 +474,6 @@ class Book:
         n = len(self.entries)
         out = (_abi.Estimate * max(n, 1))()
         tuning = _abi.BookTuning(samples_per_thread, workgroups_per_cu, int(use_graph), replays, 0.0, 0.0)
-        ctx.check(_native.lib().kg_price_book(ctx.handle, self._array, n, _PRECISION[precision],
-                                              C.byref(tuning), out))
+        ctx.call(_native.lib().kg_price_book, self._array, n, _PRECISION[precision], C.byref(tuning), out,
+                 interruptible=True)
         return [MonteCarloEstimate(out[i]) for i in range(n)], tuning
