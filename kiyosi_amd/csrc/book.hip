@@ -670,7 +670,9 @@ static Planned plan_digital(const kg_digital_terms& o, const kg_bsm_market& m, B
 }
 
 // MonteCarloAsianEngine::prepare (engines/monte_carlo/asian.rs:374-551) and
-// geometric_control (:1016-1061) for flat rates and a fresh average.
+// geometric_control (:1016-1061) for flat rates: seasoned averages (a
+// realized average over the elapsed part of the window), forward starts and
+// single fixings included.
 static Planned plan_asian(const kg_asian_terms& o, const kg_bsm_market& m, Blob&, int32_t request = 0) {
   Planned pl;
   const int32_t spot_greeks = request & (KG_GREEK_DELTA | KG_GREEK_GAMMA);
@@ -684,32 +686,68 @@ static Planned plan_asian(const kg_asian_terms& o, const kg_bsm_market& m, Blob&
     fail(KG_INVALID_INPUT, "valuation must be within contract life");
   if (o.averaging_start_time < o.effective_time || o.averaging_start_time > o.expiry_time)
     fail(KG_INVALID_INPUT, "averaging must start within the contract life");
+  // AverageTerms::validate (instruments/asian.rs:105-124)
   if (val > o.averaging_start_time && !o.has_realized)
     fail(KG_MISSING_HISTORY, "prior realized average is required");
-  if (val > o.averaging_start_time)
-    fail(KG_UNSUPPORTED_OPERATION, "seasoned averages are priced by the CPU engine");
+  if (val <= o.averaging_start_time && o.has_realized)
+    fail(KG_INVALID_INPUT, "realized average must match the elapsed averaging period");
+  if (o.has_realized && (!(o.realized_average > 0.0) || !std::isfinite(o.realized_average)))
+    fail(KG_INVALID_INPUT, "realized average must be finite and positive");
   const double sign = o.option_type;
   auto years = [](int64_t a, int64_t b) { return (double)(b - a) / (365.0 * 86400e6); };
   const double tau = years(val, o.expiry_time);
-  if (tau == 0.0) {
+  if (tau == 0.0) {  // AverageTerms::settlement (:74-79)
     pl.known = true;
-    pl.value = std::max(sign * (m.spot - o.strike), 0.0);
+    pl.value = std::max(sign * ((o.has_realized ? o.realized_average : m.spot) - o.strike), 0.0);
     return pl;
   }
   check_market(m);
   const double period = years(o.averaging_start_time, o.expiry_time);
-  if (period == 0.0) fail(KG_UNSUPPORTED_OPERATION, "single-fixing averages use the vanilla engine");
-  const double lead = std::max(years(val, o.averaging_start_time), 0.0);
-  const double future = years(std::max(val, o.averaging_start_time), o.expiry_time);
-  const double weight = future / period;
+  const bool single = period == 0.0;
+  const bool geometric = o.averaging == KG_AVERAGE_GEOMETRIC;
   const double sigma = m.volatility, r = m.rate, q = m.dividend;
   const double carry = r - q, drift_rate = carry - 0.5 * sigma * sigma;
   const double log_discount = -r * tau;
-  if (o.averaging == KG_AVERAGE_GEOMETRIC) {
-    const double vt = lead + future / 3.0;
+  const double elapsed = years(o.averaging_start_time, val);
+  const double realized = o.has_realized ? o.realized_average : 0.0;
+  if (!geometric && !single) {
+    // adjusted_strike (instruments/asian.rs:221-233); at or below zero the
+    // call is exercised on every path and priced in closed form
+    // (TurnbullWakemanArithmeticAveragePriceEngine, engines/analytic/asian.rs
+    // :279-292), the put is worthless; delta = exp(forward_log) for calls.
+    const double adjusted = elapsed > 0.0 ? o.strike + elapsed / tau * (o.strike - realized) : o.strike;
+    if (adjusted <= 0.0) {
+      const double x = carry * tau;  // exprel (engines/analytic/asian.rs:21-23)
+      const double m1 = x == 0.0 ? 1.0 : std::expm1(x) / x;
+      if (!std::isfinite(m1) || m1 <= 0.0) fail(KG_COMPUTATION_FAILURE, "invalid arithmetic moment");
+      const double scale = tau / period;
+      const double forward_log = std::log(m1) + std::log(scale) - r * tau;
+      pl.known = true;
+      if (sign < 0.0) {
+        pl.value = 0.0;
+      } else {
+        const double accrued = std::max(realized * (elapsed / period) - o.strike, 0.0);
+        pl.value = h_scaled_cash(accrued, -r * tau) + h_scaled_cash(m.spot, forward_log);
+      }
+      if (!std::isfinite(pl.value)) unstable();
+      pl.greeks = spot_greeks;
+      pl.known_delta = sign > 0.0 ? std::exp(forward_log) : 0.0;
+      pl.known_gamma = 0.0;
+      return pl;
+    }
+  }
+  const double lead = single ? 0.0 : std::max(years(val, o.averaging_start_time), 0.0);
+  const double future = single ? tau : years(std::max(val, o.averaging_start_time), o.expiry_time);
+  const double weight = single ? 1.0 : future / period;
+  const double prior_weight = val > o.averaging_start_time ? elapsed / period : 0.0;
+  if (geometric || single) {
+    // GeometricParameters (asian.rs:452-484): a single fixing is the
+    // terminal spot (variance time tau, forward moneyness, drift -sigma^2 tau / 2).
+    const double vt = single ? tau : lead + future / 3.0;
     GeoAsianDev g{};
-    g.im = weight * log_ratio(m.spot, o.strike);
-    g.drift = weight * drift_rate * (lead + future / 2.0);
+    g.im = (single ? log_ratio(m.spot, o.strike) + (-q * tau + r * tau) : weight * log_ratio(m.spot, o.strike)) +
+           (val > o.averaging_start_time ? prior_weight * log_ratio(realized, o.strike) : 0.0);
+    g.drift = single ? -0.5 * sigma * sigma * tau : weight * drift_rate * (lead + future / 2.0);
     g.width = sigma * (weight * std::sqrt(vt));
     g.log_width = std::log(sigma) + std::log(weight) + 0.5 * std::log(vt);
     g.log_strike = std::log(o.strike);
@@ -726,7 +764,6 @@ static Planned plan_asian(const kg_asian_terms& o, const kg_bsm_market& m, Blob&
     pl.item.u.g = g;
     return pl;
   }
-  if (lead > 0.0) fail(KG_UNSUPPORTED_OPERATION, "forward-start arithmetic averages are not supported");
   ArithAsianDev a{};
   a.n = o.step_count;
   a.spot = m.spot;
@@ -739,16 +776,25 @@ static Planned plan_asian(const kg_asian_terms& o, const kg_bsm_market& m, Blob&
   a.width = sigma * std::sqrt(dt);
   a.log_width = std::log(sigma) + 0.5 * std::log(dt);
   a.request = spot_greeks;
+  a.lead = lead > 0.0;
+  a.lead_drift = drift_rate * lead;
+  a.lead_width = sigma * std::sqrt(lead);
+  a.log_history = realized > 0.0 && prior_weight > 0.0 ? std::log(realized) + std::log(prior_weight) : -INFINITY;
+  a.prior_baseline = prior_weight * ((realized - o.strike) / m.spot);
+  a.adjusted_strike = std::fma(-prior_weight, realized, o.strike);
   pl.greeks = spot_greeks;
   a.sign = sign;
+  if (!std::isfinite(a.drift) || !std::isfinite(a.width) || !std::isfinite(a.lead_drift) ||
+      !std::isfinite(a.lead_width) || !std::isfinite(a.interval_forward))
+    unstable();
   if (a.width < 1e-5) fail(KG_UNSUPPORTED_OPERATION, "arithmetic widths below 1e-5 are not supported");
   const double n = o.step_count, fw = 1.0 - 0.5 / n;
-  const double las = std::log(a.spot) + std::log(a.weight), ls = std::log(a.strike);
+  const double las = std::log(a.spot) + std::log(a.weight), ls = std::log(a.adjusted_strike);
   if (!(a.width > 0.8 || std::fabs(a.log_discount) > 50.0 || std::fabs(std::log(a.spot)) > 200.0 ||
         std::fabs(las) > 200.0 || !std::isfinite(ls))) {
-    double mean = fw * a.drift;
+    double mean = a.lead_drift + fw * a.drift;
     for (int s = 1; s < o.step_count; ++s) mean += (fw - (double)s / n) * a.drift;
-    const double variance = (a.width * a.width) * (n / 3.0 - 1.0 / (12.0 * n));
+    const double variance = a.lead_width * a.lead_width + (a.width * a.width) * (n / 3.0 - 1.0 / (12.0 * n));
     const double w = std::sqrt(variance);
     const double lf = las + mean + 0.5 * variance;
     if (w >= 0.02 && w <= 0.8 && std::fabs(lf - ls) <= 2.0 * w) {
@@ -783,7 +829,7 @@ static Planned plan_asian(const kg_asian_terms& o, const kg_bsm_market& m, Blob&
     }
   }
   pl.item.kind = IK_ARITH_ASIAN;
-  pl.item.dims = o.step_count - 1;
+  pl.item.dims = o.step_count - 1 + a.lead;
   pl.item.u.a = a;
   return pl;
 }
@@ -1359,7 +1405,8 @@ static void asian_price_strikes(kg_context* ctx, const kg_settings* settings, co
       const ArithAsianDev& a = plans[u].item.u.a;
       if (plans[u].item.kind != IK_ARITH_ASIAN || a.n != a0.n || a.spot != a0.spot || a.weight != a0.weight ||
           a.log_discount != a0.log_discount || a.drift != a0.drift || a.width != a0.width || a.sign != a0.sign ||
-          a.interval_forward != a0.interval_forward)
+          a.interval_forward != a0.interval_forward || a.lead != a0.lead || a.lead_drift != a0.lead_drift ||
+          a.lead_width != a0.lead_width)
         fail(KG_COMPUTATION_FAILURE, "strike lanes do not share their path grid");
     }
     const Sampler s = validate_settings(settings, (uint64_t)p0.item.dims);

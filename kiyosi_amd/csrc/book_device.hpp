@@ -399,7 +399,13 @@ struct ArithAsianDev {
   double spot, strike, weight, log_discount, drift, width, sign, interval_forward;
   double c_log_strike, c_las, c_fw;
   double log_width;  // ln(sigma) + ln(dt) / 2, for the gamma
-  int32_t request, pad;
+  int32_t request;
+  int32_t lead;  // a forward start: dimension 0 steps from valuation to the averaging start
+  double lead_drift, lead_width;
+  // Seasoning: ln(prior_weight * realized) (-inf when none), the realized
+  // part of the baseline, prior_weight * (realized - strike) / spot, and the
+  // adjusted strike strike - prior_weight * realized.
+  double log_history, prior_baseline, adjusted_strike;
 };
 
 __device__ inline double geometric_values_d(const ArithAsianDev& p, double lf, double ls,
@@ -415,22 +421,28 @@ __device__ inline double geometric_values_d(const ArithAsianDev& p, double lf, d
 // expm1 growths, the log-sum once growth passed e^600, and the geometric
 // sum of the control variate. Nothing in it depends on the strike.
 struct ArithPath {
-  double rel, log_sum, gg;
+  double rel, log_sum, gg, lead_growth;
 };
 
-// ArithmeticGrid::path (asian.rs:640-712): the strike-independent loop.
+// ArithmeticGrid::path (asian.rs:640-712): the strike-independent loop; a
+// forward start takes its lead step from dimension 0.
 template <bool FP32, class D>
 __device__ ArithPath arith_path(const ArithAsianDev& p, D& dr, bool live, float sign) {
   using R = typename std::conditional<FP32, float, double>::type;
   const int dims = p.n - 1;
   R growth = 0;
-  double rel = 0.0, log_sum = -INFINITY, gg = 0.0;
+  double rel = 0.0, log_sum = -INFINITY, gg = 0.0, lead_growth = 0.0;
   const R drift = (R)p.drift, width = (R)p.width;
   const double inv_n = 1.0 / (double)p.n;
+  if (p.lead && live) {
+    const R z = (R)sign * dr.template normal<R>(0);
+    lead_growth = FP32 ? (double)((float)p.lead_drift + (float)p.lead_width * (float)z)
+                       : p.lead_drift + p.lead_width * (double)z;
+  }
   for (int i = 0; i < dims; ++i) {
     if (__ballot(live) == 0) break;
     if (!live) continue;
-    const R z = (R)sign * dr.template normal<R>(i);
+    const R z = (R)sign * dr.template normal<R>(i + p.lead);
     growth += drift + width * z;
     const double fraction = (i + 2 == p.n ? 0.5 : 1.0) * inv_n;
     if (p.control) gg += fraction * (double)growth;
@@ -441,7 +453,7 @@ __device__ ArithPath arith_path(const ArithAsianDev& p, D& dr, bool live, float 
       rel += fraction * (FP32 ? (double)f32::expm1((float)growth) : expm1((double)growth));
     }
   }
-  return ArithPath{rel, log_sum, gg};
+  return ArithPath{rel, log_sum, gg, lead_growth};
 }
 
 // ArithmeticParameters::arithmetic with the geometric control residual
@@ -449,30 +461,32 @@ __device__ ArithPath arith_path(const ArithAsianDev& p, D& dr, bool live, float 
 // for the strike terms of `p`.
 __device__ inline double arith_payoff(const ArithAsianDev& p, const ArithPath& a, uint32_t& bad) {
   const double inv_n = 1.0 / (double)p.n;
-  const double rel = a.rel, log_sum = a.log_sum, gg = a.gg;
+  const double rel = a.rel, log_sum = a.log_sum, gg = a.gg, lg = a.lead_growth;
   const double half = 0.5 * inv_n, rest = 1.0 - half;
   const double log_spot = log(p.spot), log_weight = log(p.weight);
   const double log_rest = log_sum == -INFINITY ? log(rest) + log1p(rel / rest) : log_sum;
-  const double log_q = log_spot + log_weight + log_rest + p.interval_forward;
-  const double log_c = log_spot + log_weight + log(half);
+  const double log_q = log_spot + log_weight + lg + log_rest + p.interval_forward;
+  const double log_c = log_spot + log_weight + lg + log(half);
+  const double log_accrued = d_log_add(p.log_history, log_c);
   const bool has_extra = log_sum == -INFINITY;
-  const double extra = has_extra ? rest * expm1(p.interval_forward) + rel * exp(p.interval_forward) : 0.0;
+  const double extra =
+      has_extra ? expm1(lg) + exp(lg) * (rest * expm1(p.interval_forward) + rel * exp(p.interval_forward)) : 0.0;
   double residual = 0.0;
   if (p.control) {
     const double w = p.c_fw * p.width;
-    const double lf = p.c_las + gg + p.c_fw * p.drift + 0.5 * w * w;
+    const double lf = p.c_las + lg + gg + p.c_fw * p.drift + 0.5 * w * w;
     residual = geometric_values_d(p, lf, p.c_log_strike, w);
   }
   const double log_strike = log(p.strike);
   double price;
-  if (log_c >= log_strike) {
+  if (log_accrued >= log_strike) {
     price = p.sign < 0 ? 0.0
                        : exp(log_q + p.log_discount) +
-                             d_positive_difference(log_c + p.log_discount, log_strike + p.log_discount);
+                             d_positive_difference(log_accrued + p.log_discount, log_strike + p.log_discount);
   } else {
-    const double log_b = log_strike + log(-expm1(log_c - log_strike));
+    const double log_b = log_strike + log(-expm1(log_accrued - log_strike));
     double forward = log_q - log_b;
-    const double baseline = p.weight * ((p.spot - p.strike) / p.spot);
+    const double baseline = p.prior_baseline + p.weight * ((p.spot - p.strike) / p.spot);
     if (has_extra) {
       const double excess = baseline + p.weight * extra;
       const double bb = exp(log_b - log_spot);
@@ -497,19 +511,21 @@ __device__ inline double arith_payoff_greeks(const ArithAsianDev& p, const Arith
                                              double* lr) {
   constexpr double HALF_LN_2PI = 0.9189385332046727;
   const double inv_n = 1.0 / (double)p.n;
-  const double rel = a.rel, log_sum = a.log_sum, gg = a.gg;
+  const double rel = a.rel, log_sum = a.log_sum, gg = a.gg, lg = a.lead_growth;
   const double half = 0.5 * inv_n, rest = 1.0 - half;
   const double log_spot = log(p.spot), log_weight = log(p.weight);
   const double log_rest = log_sum == -INFINITY ? log(rest) + log1p(rel / rest) : log_sum;
-  const double log_q = log_spot + log_weight + log_rest + p.interval_forward;
-  const double log_c = log_spot + log_weight + log(half);
+  const double log_q = log_spot + log_weight + lg + log_rest + p.interval_forward;
+  const double log_c = log_spot + log_weight + lg + log(half);
+  const double log_accrued = d_log_add(p.log_history, log_c);
   const bool has_extra = log_sum == -INFINITY;
-  const double extra = has_extra ? rest * expm1(p.interval_forward) + rel * exp(p.interval_forward) : 0.0;
+  const double extra =
+      has_extra ? expm1(lg) + exp(lg) * (rest * expm1(p.interval_forward) + rel * exp(p.interval_forward)) : 0.0;
   const bool want_delta = p.request & 1, want_gamma = p.request & 2;
   double res[3] = {0.0, 0.0, 0.0};
   if (p.control) {
     const double w = p.c_fw * p.width;
-    const double lf = p.c_las + gg + p.c_fw * p.drift + 0.5 * w * w;
+    const double lf = p.c_las + lg + gg + p.c_fw * p.drift + 0.5 * w * w;
     const double d1 = (lf - p.c_log_strike) / w + 0.5 * w;
     const double d2 = d1 - w;
     const double la = lf + p.log_discount + d_log_normal_cdf(p.sign * d1);
@@ -520,18 +536,18 @@ __device__ inline double arith_payoff_greeks(const ArithAsianDev& p, const Arith
   }
   const double log_strike = log(p.strike);
   double price, delta = 0.0, gamma = 0.0;
-  if (log_c >= log_strike) {
+  if (log_accrued >= log_strike) {
     if (p.sign < 0) {
       price = 0.0;
     } else {
       price = exp(log_q + p.log_discount) +
-              d_positive_difference(log_c + p.log_discount, log_strike + p.log_discount);
+              d_positive_difference(log_accrued + p.log_discount, log_strike + p.log_discount);
       if (want_delta) delta = exp(log_q + p.log_discount - log_spot) + exp(log_c + p.log_discount - log_spot);
     }
   } else {
-    const double log_b = log_strike + log(-expm1(log_c - log_strike));
+    const double log_b = log_strike + log(-expm1(log_accrued - log_strike));
     double forward = log_q - log_b;
-    const double baseline = p.weight * ((p.spot - p.strike) / p.spot);
+    const double baseline = p.prior_baseline + p.weight * ((p.spot - p.strike) / p.spot);
     if (has_extra) {
       const double excess = baseline + p.weight * extra;
       const double bb = exp(log_b - log_spot);
@@ -546,8 +562,8 @@ __device__ inline double arith_payoff_greeks(const ArithAsianDev& p, const Arith
     if (want_delta)
       delta = p.sign * (exp(log_c + p.log_discount - log_spot + n2) + exp(log_q + p.log_discount - log_spot + n1));
     if (want_gamma)
-      gamma = exp(p.log_discount + 2.0 * log(p.strike) - 2.0 * log_spot - log_b - p.log_width - 0.5 * d2 * d2 -
-                  HALF_LN_2PI);
+      gamma = exp(p.log_discount + 2.0 * log(p.adjusted_strike) - 2.0 * log_spot - log_b - p.log_width -
+                  0.5 * d2 * d2 - HALF_LN_2PI);
   }
   const double v = price - res[0];
   lr[0] = delta - res[1];

@@ -211,6 +211,14 @@ inline double scaled_value(double scale, double weight, double log_discount) {
   if (!std::isfinite(r)) unstable();
   return r;
 }
+// scaled_cash (numerics.rs:175-186): amount * exp(exponent) without
+// spurious overflow or underflow.
+inline double h_scaled_cash(double amount, double exponent) {
+  if (amount == 0.0) return 0.0;
+  const double discount = std::exp(exponent), value = amount * discount;
+  if (std::isnormal(discount) && std::isnormal(value)) return value;
+  return std::exp(std::log(amount) + exponent);
+}
 inline double log_ratio(double num, double den) {  // numerics.rs:80-92
   double rel = (num - den) / den;
   if (std::fabs(rel) < 0.5) return std::log1p(rel);

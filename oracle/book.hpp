@@ -553,7 +553,7 @@ inline void geometric_values(const AsianArith& p, double lf, double ls, double w
 }
 
 // MonteCarloAsianEngine::native (asian.rs:347-372, prepare :374-560) for
-// flat BSM rates, no seasoning and (arithmetic) no forward start. `request`
+// flat BSM rates: seasoned averages, forward starts and single fixings. `request`
 // holds KG_GREEK_* bits; delta and gamma are the path means of the engine's
 // conditional estimators (GeometricParameters::geometric :741-784,
 // ArithmeticParameters::arithmetic :1100-1256), plus the geometric
@@ -563,31 +563,65 @@ inline Estimate asian_native(const Settings& st, const kg_asian_terms& o, const 
   const int64_t val = m.valuation_time;
   if (val < o.effective_time || val > o.expiry_time)
     throw Failure(INVALID_INPUT, "valuation must be within contract life");
-  if (val > o.averaging_start_time) throw Failure(UNSUPPORTED, "seasoned averages are not restated");
   if (o.step_count < 2 || o.step_count > 10000) throw Failure(INVALID_INPUT, "step_count out of range");
+  // AverageTerms::validate (instruments/asian.rs:105-124)
+  if (val > o.averaging_start_time && !o.has_realized)
+    throw Failure(MISSING_HISTORY, "prior realized average is required");
+  if (val <= o.averaging_start_time && o.has_realized)
+    throw Failure(INVALID_INPUT, "realized average must match the elapsed averaging period");
+  const double realized = o.has_realized ? o.realized_average : 0.0;
   const double tau = years_until(val, o.expiry_time);
   const double sign = o.option_type;
-  // At expiry the price is known and the native estimator has no Greeks.
-  if (tau == 0.0) return known_estimate(st, std::max(sign * (m.spot - o.strike), 0.0));
+  const int req = request & (KG_GREEK_DELTA | KG_GREEK_GAMMA);
+  const bool want_delta = req & KG_GREEK_DELTA, want_gamma = req & KG_GREEK_GAMMA;
+  if (greeks) *greeks = SpotGreeks{};
+  // At expiry the price is the settlement (:74-79) and the native estimator
+  // has no Greeks.
+  if (tau == 0.0) return known_estimate(st, std::max(sign * ((o.has_realized ? realized : m.spot) - o.strike), 0.0));
   if (m.volatility == 0.0) throw Failure(UNSUPPORTED, "zero volatility is deterministic");
   const double period = years_until(o.averaging_start_time, o.expiry_time);
   const bool single = period == 0.0;
-  if (single) throw Failure(UNSUPPORTED, "single-fixing averages are not restated");
-  const double lead = std::max(years_until(val, o.averaging_start_time), 0.0);
-  const double future = years_until(std::max(val, o.averaging_start_time), o.expiry_time);
-  const double weight = future / period;
+  const bool geometric_kind = o.averaging == KG_AVERAGE_GEOMETRIC;
   const double sigma = m.volatility, r = m.rate, q = m.dividend;
   const double carry = r - q;
   const double drift_rate = carry - 0.5 * sigma * sigma;
   const double log_discount = -r * tau;
-  const int req = request & (KG_GREEK_DELTA | KG_GREEK_GAMMA);
-  const bool want_delta = req & KG_GREEK_DELTA, want_gamma = req & KG_GREEK_GAMMA;
+  const double elapsed = years_until(o.averaging_start_time, val);
+  if (!geometric_kind && !single) {
+    // asian.rs:399-414: a non-positive adjusted strike (instruments/asian.rs
+    // :221-233) prices the certain exercise in closed form, Turnbull-Wakeman
+    // under flat rates (engines/analytic/asian.rs:279-292).
+    const double adjusted = elapsed > 0.0 ? o.strike + elapsed / tau * (o.strike - realized) : o.strike;
+    if (adjusted <= 0.0) {
+      const double x = carry * tau;
+      const double m1 = x == 0.0 ? 1.0 : std::expm1(x) / x;
+      const double log_m1 = std::log(m1) + std::log(tau / period) - r * tau;
+      double value = 0.0;
+      if (sign > 0.0)
+        value = scaled_cash(std::max(realized * (elapsed / period) - o.strike, 0.0), -r * tau) +
+                scaled_cash(m.spot, log_m1);
+      if (greeks) {
+        greeks->available = req;
+        greeks->delta = want_delta && sign > 0.0 ? std::exp(log_m1) : 0.0;
+        greeks->gamma = 0.0;
+      }
+      return known_estimate(st, value);
+    }
+  }
+  const double lead = single ? 0.0 : std::max(years_until(val, o.averaging_start_time), 0.0);
+  const double future = single ? tau : years_until(std::max(val, o.averaging_start_time), o.expiry_time);
+  const double weight = single ? 1.0 : future / period;
+  const double prior_weight = val > o.averaging_start_time ? elapsed / period : 0.0;
   Moments total[3], pairs[3];
   double expected[3] = {0.0, 0.0, 0.0};
-  if (o.averaging == KG_AVERAGE_GEOMETRIC) {
-    const double vt = lead + future / 3.0;
-    const double im = weight * log_ratio(m.spot, o.strike);
-    const double drift = weight * drift_rate * (lead + future / 2.0);
+  if (geometric_kind || single) {
+    // GeometricParameters (asian.rs:452-484); a single fixing is the
+    // terminal spot.
+    const double vt = single ? tau : lead + future / 3.0;
+    const double im = (single ? log_ratio(m.spot, o.strike) + (-q * tau + r * tau)
+                              : weight * log_ratio(m.spot, o.strike)) +
+                      (val > o.averaging_start_time ? prior_weight * log_ratio(realized, o.strike) : 0.0);
+    const double drift = single ? -0.5 * sigma * sigma * tau : weight * drift_rate * (lead + future / 2.0);
     const double width = sigma * (weight * std::sqrt(vt));
     const double log_width = std::log(sigma) + std::log(weight) + 0.5 * std::log(vt);
     const double log_base_width = std::log(sigma) + 0.5 * std::log(vt);
@@ -621,7 +655,6 @@ inline Estimate asian_native(const Settings& st, const kg_asian_terms& o, const 
         },
         total, pairs);
   } else {
-    if (lead > 0.0) throw Failure(UNSUPPORTED, "forward-start arithmetic averages are not restated");
     AsianArith p;
     p.spot = m.spot;
     p.strike = o.strike;
@@ -634,7 +667,12 @@ inline Estimate asian_native(const Settings& st, const kg_asian_terms& o, const 
     p.width = sigma * std::sqrt(dt);
     p.log_width = std::log(sigma) + 0.5 * std::log(dt);
     p.sign = sign;
-    const size_t dims = (size_t)o.step_count - 1;
+    const bool has_lead = lead > 0.0;
+    const double lead_drift = drift_rate * lead, lead_width = sigma * std::sqrt(lead);
+    const double log_history =
+        realized > 0.0 && prior_weight > 0.0 ? std::log(realized) + std::log(prior_weight) : -INFINITY;
+    const double adjusted_strike = std::fma(-prior_weight, realized, p.strike);
+    const size_t dims = (size_t)o.step_count - 1 + (has_lead ? 1 : 0);
     // geometric_control (asian.rs:1016-1061)
     bool control = false;
     double c_log_strike = 0, c_las = 0, c_fw = 0;
@@ -642,12 +680,12 @@ inline Estimate asian_native(const Settings& st, const kg_asian_terms& o, const 
       const double n = o.step_count;
       const double fw = 1.0 - 0.5 / n;
       const double las = std::log(p.spot) + std::log(p.weight);
-      const double ls = std::log(p.strike);
+      const double ls = std::log(adjusted_strike);
       if (!(p.width < 1e-5 || p.width > 0.8 || std::fabs(p.log_discount) > 50.0 ||
             std::fabs(std::log(p.spot)) > 200.0 || std::fabs(las) > 200.0 || !std::isfinite(ls))) {
-        double mean = 0.0 + fw * p.drift;
+        double mean = lead_drift + fw * p.drift;
         for (int s = 1; s < o.step_count; ++s) mean += (fw - (double)s / n) * p.drift;
-        double variance = 0.0 * 0.0 + (p.width * p.width) * (n / 3.0 - 1.0 / (12.0 * n));
+        double variance = lead_width * lead_width + (p.width * p.width) * (n / 3.0 - 1.0 / (12.0 * n));
         double w = std::sqrt(variance);
         double lf = las + mean + 0.5 * variance;
         if (w >= 0.02 && w <= 0.8 && std::fabs(lf - ls) <= 2.0 * w) {
@@ -663,10 +701,13 @@ inline Estimate asian_native(const Settings& st, const kg_asian_terms& o, const 
       path_samples<3>(
           st, dims, true, cx,
           [&](const double* normals, double* out) {
-            // ArithmeticGrid::path (asian.rs:640-712), flat rates, no lead
+            // ArithmeticGrid::path (asian.rs:640-712), flat rates; a forward
+            // start's lead step takes normal 0.
+            const size_t first = has_lead ? 1 : 0;
+            const double lg = lead_drift + lead_width * (has_lead ? normals[0] : 0.0);
             double growth = 0.0, rel = 0.0, log_sum = -INFINITY, gg = 0.0;
-            for (size_t i = 0; i < dims; ++i) {
-              growth += p.drift + p.width * normals[i];
+            for (size_t i = 0; i + first < dims; ++i) {
+              growth += p.drift + p.width * normals[i + first];
               double fraction = ((int)i + 2 == p.n ? 0.5 : 1.0) / (double)p.n;
               if (use_control) gg += fraction * growth;
               if (growth > 600.0 || log_sum != -INFINITY) {
@@ -679,18 +720,18 @@ inline Estimate asian_native(const Settings& st, const kg_asian_terms& o, const 
             const double half = 0.5 / (double)p.n, rest = 1.0 - half;
             const double log_spot = std::log(p.spot);
             const double log_rest = log_sum == -INFINITY ? std::log(rest) + std::log1p(rel / rest) : log_sum;
-            const double log_q = log_spot + std::log(p.weight) + 0.0 + log_rest + p.interval_forward;
-            const double log_c = log_spot + std::log(p.weight) + 0.0 + std::log(half);
-            const double log_accrued = log_add(-INFINITY, log_c);
+            const double log_q = log_spot + std::log(p.weight) + lg + log_rest + p.interval_forward;
+            const double log_c = log_spot + std::log(p.weight) + lg + std::log(half);
+            const double log_accrued = log_add(log_history, log_c);
             const bool has_extra = log_sum == -INFINITY;
-            const double extra = has_extra ? std::expm1(0.0) + std::exp(0.0) * (rest * std::expm1(p.interval_forward) +
-                                                                                rel * std::exp(p.interval_forward))
+            const double extra = has_extra ? std::expm1(lg) + std::exp(lg) * (rest * std::expm1(p.interval_forward) +
+                                                                              rel * std::exp(p.interval_forward))
                                            : 0.0;
             // ArithmeticParameters::arithmetic (asian.rs:1100-1256)
             double residual[3] = {0.0, 0.0, 0.0};
             if (use_control) {
               double width = c_fw * p.width;
-              double lf = c_las + 0.0 + gg + c_fw * p.drift + 0.5 * width * width;
+              double lf = c_las + lg + gg + c_fw * p.drift + 0.5 * width * width;
               geometric_values(p, lf, c_log_strike, width, residual);
               if (!want_delta) residual[1] = 0.0;
               if (!want_gamma) residual[2] = 0.0;
@@ -709,7 +750,8 @@ inline Estimate asian_native(const Settings& st, const kg_asian_terms& o, const 
             } else {
               const double log_b = log_strike + std::log(-std::expm1(log_accrued - log_strike));
               double forward = log_q - log_b;
-              const double baseline = 0.0 + p.weight * ((p.spot - p.strike) / p.spot);
+              const double baseline = prior_weight * ((realized - p.strike) / p.spot) +
+                                      p.weight * ((p.spot - p.strike) / p.spot);
               if (has_extra) {
                 double excess = baseline + p.weight * extra;
                 double b = std::exp(log_b - log_spot);
@@ -725,7 +767,7 @@ inline Estimate asian_native(const Settings& st, const kg_asian_terms& o, const 
                 delta = p.sign * (std::exp(log_c + p.log_discount - log_spot + log_normal_cdf(p.sign * d2)) +
                                   std::exp(log_q + p.log_discount - log_spot + log_normal_cdf(p.sign * d1)));
               if (want_gamma) {
-                const double k = p.strike;  // adjusted strike without seasoning
+                const double k = adjusted_strike;
                 gamma = std::exp(p.log_discount + 2.0 * std::log(k) - 2.0 * log_spot - log_b - p.log_width -
                                  0.5 * d2 * d2 - 0.5 * std::log(2.0 * M_PI));
               }

@@ -28,7 +28,14 @@ struct Failure : std::runtime_error {
   int code;
   Failure(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
-constexpr int INVALID_INPUT = 1, UNSUPPORTED = 2, COMPUTATION_FAILURE = 3;
+constexpr int INVALID_INPUT = 1, UNSUPPORTED = 2, COMPUTATION_FAILURE = 3, MISSING_HISTORY = 6;
+// scaled_cash (numerics.rs:175-186)
+inline double scaled_cash(double amount, double exponent) {
+  if (amount == 0.0) return 0.0;
+  const double discount = std::exp(exponent), value = amount * discount;
+  if (std::isnormal(discount) && std::isnormal(value)) return value;
+  return std::exp(std::log(amount) + exponent);
+}
 [[noreturn]] inline void unstable() {  // monte_carlo.rs:2119-2121
   throw Failure(COMPUTATION_FAILURE,
                 "Monte Carlo produced a non-finite parameter, path or result");

@@ -211,3 +211,124 @@ def test_arithmetic_asian_native_greeks_are_derivatives_of_the_conditional_price
     _, delta, gamma = geometric_closed_form(100.0, 100.0, 0.04, 0.01, 0.2, 1.0, kind)
     assert 0 < kind * g.delta < kind * delta + 0.05 and 0.8 * gamma < g.gamma < 1.3 * gamma
 
+
+
+# ---------------------------------------------------------------- Asian carve-outs
+SEASON = START + dt.timedelta(days=120)
+
+
+def asian_terms(kind, averaging, n, strike, realized=None, start=START, averaging_start=START, end=END):
+    return _abi.AsianTerms(kind, averaging, n, int(realized is not None), strike, realized or 0.0, us(start),
+                           us(averaging_start), us(end))
+
+
+def market_at(valuation, spot=100.0, rate=0.04, dividend=0.01, sigma=0.25):
+    return _abi.BsmMarket(spot, rate, dividend, sigma, us(valuation))
+
+
+@pytest.mark.parametrize("kind", [1, -1])
+@pytest.mark.parametrize("realized", [80.0, 104.0])
+def test_seasoned_arithmetic_asian_is_a_scaled_fresh_average(oracle, kind, realized):
+    """With prior weight p = 120/365 and future weight w = 245/365, the payoff
+    (p R + w F - K)+ is w (F - K*)+ with K* = (K - p R) / w, F the average
+    over the remaining window: on the same draws the seasoned price, delta and
+    gamma are w times those of a fresh average from the valuation date struck
+    at K* (asian.rs:426-430, 687-694, 1156-1158, 1246)."""
+    n = 40
+    p, w = 120 / 365, 245 / 365
+    seasoned = asian_terms(kind, _abi.AVERAGE_ARITHMETIC, n, 100.0, realized)
+    fresh = asian_terms(kind, _abi.AVERAGE_ARITHMETIC, n, (100.0 - p * realized) / w, start=SEASON,
+                        averaging_start=SEASON)
+    m = market_at(SEASON)
+    s = settings(20_000, seed=3)
+    a, ga = oracle.asian_price_with_greeks(s, seasoned, m, workers=8)
+    b, gb = oracle.asian_price_with_greeks(s, fresh, m, workers=8)
+    assert a.price == pytest.approx(w * b.price, rel=1e-11, abs=1e-13)
+    assert ga.delta == pytest.approx(w * gb.delta, rel=1e-10, abs=1e-13)
+    assert ga.gamma == pytest.approx(w * gb.gamma, rel=1e-9, abs=1e-13)
+
+
+@pytest.mark.parametrize("kind", [1, -1])
+def test_seasoned_geometric_asian_matches_its_closed_form(oracle, kind):
+    """ln of the seasoned geometric average is p ln R + w ln G with ln G normal
+    (mean ln S + (r - q - s^2/2) f/2, variance s^2 f/3 over the remaining f
+    years): a Black formula in that lognormal."""
+    r, q, sig, R, K = 0.04, 0.01, 0.25, 96.0, 100.0
+    p, f = 120 / 365, 245 / 365
+    w = f / (p + f)
+    mean = p * math.log(R) + w * (math.log(100.0) + (r - q - 0.5 * sig * sig) * f / 2)
+    var = (w * sig) ** 2 * f / 3
+    d1 = (mean - math.log(K) + var) / math.sqrt(var)
+    d2 = d1 - math.sqrt(var)
+    fwd = math.exp(mean + 0.5 * var)
+    exact = math.exp(-r * f) * (kind * fwd * norm.cdf(kind * d1) - kind * K * norm.cdf(kind * d2))
+    est = oracle.asian_price(settings(400_000), asian_terms(kind, _abi.AVERAGE_GEOMETRIC, 10, K, R),
+                             market_at(SEASON, rate=r, dividend=q, sigma=sig), workers=8)
+    assert abs(est.price - exact) < 4 * est.standard_error
+
+
+@pytest.mark.parametrize("averaging", [_abi.AVERAGE_ARITHMETIC, _abi.AVERAGE_GEOMETRIC])
+@pytest.mark.parametrize("kind", [1, -1])
+def test_single_fixing_average_is_the_black_scholes_price(oracle, averaging, kind):
+    """An averaging window of zero length fixes once at expiry (asian.rs:399,
+    452-471): the estimator samples the terminal lognormal."""
+    r, q, sig = 0.04, 0.01, 0.25
+    est = oracle.asian_price(settings(400_000), asian_terms(kind, averaging, 20, 100.0, averaging_start=END),
+                             market_at(START, rate=r, dividend=q, sigma=sig), workers=8)
+    d1 = (math.log(100 / 100) + (r - q + 0.5 * sig * sig)) / sig
+    exact = kind * (100 * math.exp(-q) * norm.cdf(kind * d1) - 100 * math.exp(-r) * norm.cdf(kind * (d1 - sig)))
+    assert abs(est.price - exact) < 4 * est.standard_error
+
+
+@pytest.mark.parametrize("kind", [1, -1])
+def test_forward_start_arithmetic_asian_matches_a_brute_force_simulation(oracle, kind):
+    """Averaging from 91 days after valuation: the estimator's lead step
+    (dimension 0, asian.rs:644-645) then the trapezoid window, against an
+    independent simulation of the same trapezoid average."""
+    n, r, q, sig = 40, 0.04, 0.01, 0.25
+    start = START + dt.timedelta(days=91)
+    lead, window = 91 / 365, 274 / 365
+    est = oracle.asian_price(settings(200_000), asian_terms(kind, _abi.AVERAGE_ARITHMETIC, n, 100.0,
+                                                            averaging_start=start),
+                             market_at(START, rate=r, dividend=q, sigma=sig), workers=8)
+    rng = np.random.default_rng(29)
+    P, h = 400_000, window / n
+    x0 = (r - q - 0.5 * sig * sig) * lead + sig * math.sqrt(lead) * rng.standard_normal(P)
+    inc = (r - q - 0.5 * sig * sig) * h + sig * math.sqrt(h) * rng.standard_normal((P, n))
+    S = 100 * np.exp(x0[:, None] + np.cumsum(inc, axis=1))
+    S0 = 100 * np.exp(x0)
+    A = (0.5 * S0 + S[:, :-1].sum(1) + 0.5 * S[:, -1]) / n
+    v = np.maximum(kind * (A - 100), 0) * math.exp(-r)
+    assert abs(est.price - v.mean()) < 4 * math.hypot(est.standard_error, v.std() / math.sqrt(P))
+
+
+def test_seasoned_arithmetic_with_a_non_positive_adjusted_strike(oracle):
+    """Realized average 400 over the first 120 days makes the adjusted strike
+    negative: every call path exercises and the put is worthless
+    (asian.rs:399-414); the call's closed form equals a brute-force mean of
+    the discounted p R + w A - K over simulated windows."""
+    r, q, sig, R = 0.04, 0.01, 0.25, 400.0
+    p, f = 120 / 365, 245 / 365
+    w = f / (p + f)
+    m = market_at(SEASON, rate=r, dividend=q, sigma=sig)
+    call, g = oracle.asian_price_with_greeks(settings(4096), asian_terms(1, _abi.AVERAGE_ARITHMETIC, 50, 100.0, R),
+                                             m, workers=8)
+    put = oracle.asian_price(settings(4096), asian_terms(-1, _abi.AVERAGE_ARITHMETIC, 50, 100.0, R), m, workers=8)
+    assert put.price == 0.0 and call.has_seed == 0
+    rng = np.random.default_rng(7)
+    P, n = 200_000, 245
+    h = f / n
+    lg = np.cumsum((r - q - 0.5 * sig * sig) * h + sig * math.sqrt(h) * rng.standard_normal((P, n)), axis=1)
+    S = 100 * np.exp(lg)
+    A = (50.0 + S[:, :-1].sum(1) + 0.5 * S[:, -1]) / n
+    v = (p * R + w * A - 100.0) * math.exp(-r * f)
+    assert abs(call.price - v.mean()) < 4 * v.std() / math.sqrt(P) + 1e-3
+    assert g.delta == pytest.approx(w * math.exp(-r * f) * math.expm1((r - q) * f) / ((r - q) * f), rel=1e-12)
+
+
+def test_seasoning_needs_the_realized_average(oracle):
+    with pytest.raises(Exception, match="realized"):
+        oracle.asian_price(settings(1024), asian_terms(1, _abi.AVERAGE_ARITHMETIC, 10, 100.0), market_at(SEASON))
+    with pytest.raises(Exception, match="realized"):
+        oracle.asian_price(settings(1024), asian_terms(1, _abi.AVERAGE_ARITHMETIC, 10, 100.0, 99.0),
+                           market_at(START))

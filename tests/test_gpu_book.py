@@ -265,3 +265,90 @@ def test_asian_native_greeks_fp32_within_half_a_standard_error(gpu, averaging, s
     assert abs(fast.price - plain.price) <= 0.5 * plain.path_standard_error
     assert gf.delta == pytest.approx(gp.delta, abs=1e-3)
     assert gf.gamma == pytest.approx(gp.gamma, abs=1e-3)
+
+
+SEASON = START + dt.timedelta(days=120)
+ASIAN_CASES = [
+    # (name, kind, averaging, n, strike, realized, averaging_start, valuation)
+    ("seasoned arithmetic call", 1, _abi.AVERAGE_ARITHMETIC, 40, 100.0, 92.0, START, SEASON),
+    ("seasoned arithmetic put", -1, _abi.AVERAGE_ARITHMETIC, 40, 100.0, 104.0, START, SEASON),
+    ("seasoned geometric call", 1, _abi.AVERAGE_GEOMETRIC, 40, 100.0, 97.0, START, SEASON),
+    ("non-positive adjusted strike", 1, _abi.AVERAGE_ARITHMETIC, 40, 100.0, 400.0, START, SEASON),
+    ("forward-start arithmetic call", 1, _abi.AVERAGE_ARITHMETIC, 40, 100.0, None,
+     START + dt.timedelta(days=91), START),
+    ("forward-start arithmetic put", -1, _abi.AVERAGE_ARITHMETIC, 33, 95.0, None,
+     START + dt.timedelta(days=91), START),
+    ("forward-start geometric", 1, _abi.AVERAGE_GEOMETRIC, 40, 100.0, None, START + dt.timedelta(days=91), START),
+    ("single-fixing arithmetic", 1, _abi.AVERAGE_ARITHMETIC, 40, 100.0, None, END, START),
+    ("single-fixing geometric put", -1, _abi.AVERAGE_GEOMETRIC, 40, 100.0, None, END, START),
+]
+
+
+@pytest.mark.parametrize("paths,antithetic,sampling", [(4096, False, _abi.PSEUDO_RANDOM),
+                                                      (3001, True, _abi.PSEUDO_RANDOM),
+                                                      (2048, False, _abi.SOBOL)])
+@pytest.mark.parametrize("name,kind,averaging,n,strike,realized,avg_start,valuation", ASIAN_CASES,
+                         ids=[c[0] for c in ASIAN_CASES])
+def test_asian_carve_outs_match_the_oracle(gpu, oracle, name, kind, averaging, n, strike, realized, avg_start,
+                                           valuation, paths, antithetic, sampling):
+    """Seasoned averages, a non-positive adjusted strike, forward-start and
+    single-fixing averages: price, delta and gamma of the GPU engine equal
+    the oracle's at 1e-10 (asian.rs:374-560)."""
+    t = _abi.AsianTerms(kind, averaging, n, int(realized is not None), strike, realized or 0.0, us(START),
+                        us(avg_start), us(END))
+    m = _abi.BsmMarket(100.0, 0.04, 0.01, 0.25, us(valuation))
+    s = settings(paths, antithetic=antithetic, sampling=sampling)
+    got, ref = price_one(gpu, "kg_asian_price", s, t, m), oracle.asian_price(s, t, m, workers=8)
+    assert got.price == pytest.approx(ref.price, rel=1e-10, abs=1e-13), name
+    assert got.uncertainty_status == ref.uncertainty_status
+    out, g = _abi.Estimate(), _abi.SpotGreeks(3, 0, 0.0, 0.0, 0.0)
+    gpu.check(_native.lib().kg_asian_price_with_greeks(gpu.handle, C.byref(s), C.byref(t), C.byref(m),
+                                                       C.byref(g), C.byref(out)))
+    ref_est, rg = oracle.asian_price_with_greeks(s, t, m, request=3, workers=8)
+    assert out.price == got.price
+    assert g.available == rg.available
+    assert g.delta == pytest.approx(rg.delta, rel=1e-10, abs=1e-13), name
+    assert g.gamma == pytest.approx(rg.gamma, rel=1e-10, abs=1e-13), name
+
+
+@pytest.mark.parametrize("realized,avg_start,valuation", [(92.0, START, SEASON),
+                                                          (None, START + dt.timedelta(days=91), START)])
+def test_asian_strike_lanes_for_seasoned_and_forward_start_averages(gpu, oracle, realized, avg_start, valuation):
+    strikes = [80.0, 95.0, 100.0, 110.0, 125.0]
+    t = _abi.AsianTerms(1, _abi.AVERAGE_ARITHMETIC, 30, int(realized is not None), 100.0, realized or 0.0,
+                        us(START), us(avg_start), us(END))
+    m = _abi.BsmMarket(100.0, 0.04, 0.01, 0.25, us(valuation))
+    s = settings(4096)
+    arr = (C.c_double * len(strikes))(*strikes)
+    out = (C.c_double * len(strikes))()
+    gpu.check(_native.lib().kg_asian_price_strikes(gpu.handle, C.byref(s), C.byref(t), C.byref(m), arr,
+                                                   len(strikes), out))
+    for k, price in zip(strikes, out):
+        t.strike = k
+        assert price == pytest.approx(oracle.asian_price(s, t, m, workers=8).price, rel=1e-10, abs=1e-13), k
+
+
+def test_book_prices_seasoned_and_forward_start_averages(gpu, oracle):
+    from kiyosi_amd.engines import Book, MonteCarloAsianEngine
+    from kiyosi_amd.instruments import ArithmeticAveragePriceOption, GeometricAveragePriceOption
+    from kiyosi_amd.market import BlackScholesMertonParameters, PricingContext
+    ctx_season = PricingContext(model_parameters=BlackScholesMertonParameters(
+        risk_free_rate=0.04, dividend_yield=0.01, volatility=0.25), spot_price=100.0, valuation_time=SEASON)
+    ctx_start = PricingContext(model_parameters=BlackScholesMertonParameters(
+        risk_free_rate=0.04, dividend_yield=0.01, volatility=0.25), spot_price=100.0, valuation_time=START)
+    engine = MonteCarloAsianEngine(path_count=4096, seed=77, step_count=30)
+    entries = [
+        (engine, ArithmeticAveragePriceOption(option_type="call", strike=100.0, effective_time=START,
+                                              averaging_start_time=START, expiry_time=END, realized_average=92.0),
+         ctx_season),
+        (engine, GeometricAveragePriceOption(option_type="put", strike=100.0, effective_time=START,
+                                             averaging_start_time=START, expiry_time=END, realized_average=101.0),
+         ctx_season),
+        (engine, ArithmeticAveragePriceOption(option_type="call", strike=100.0, effective_time=START,
+                                              averaging_start_time=START + dt.timedelta(days=91), expiry_time=END),
+         ctx_start),
+    ]
+    estimates, _ = Book(entries).price("fp64")
+    for (eng, ins, ctx), got in zip(entries, estimates):
+        ref = oracle.asian_price(eng._settings(), ins._terms(eng.step_count), ctx.bsm_market(), workers=8)
+        assert got.price == pytest.approx(ref.price, rel=1e-10, abs=1e-13)
