@@ -473,10 +473,21 @@ kg_status kg_asian_price_strikes(kg_context* ctx, const kg_settings* settings, c
 
 /* ---- Accumulators and decumulators: MonteCarloAccumulatorEngine
  * (engines/monte_carlo/accumulator.rs:91-218; instrument
- * instruments/accumulator.rs:401-526). One step per trading-day fixing at
- * the observation clock: knock-out settles the accumulated quantity at the
- * knock-out fixing, otherwise the daily quantity (times the acceleration
- * factor beyond the strike) accrues and settles at expiry. */
+ * instruments/accumulator.rs:319-526). Steps at the trading-day fixings at
+ * the observation clock from max(valuation, effective time), the explicit
+ * knock-out times and expiry: knock-out settles the accumulated quantity at
+ * the knock-out fixing, otherwise the daily quantity (times the acceleration
+ * factor beyond the strike) accrues and settles at expiry. With an explicit
+ * knock-out schedule and unit acceleration the purchases between knock-out
+ * dates are fixed and grouped onto those dates (fixed_purchase_events,
+ * :401-454). Valuation lies in [effective time, expiry]: the reference's
+ * accumulator keeps the default valuation start (vanilla.rs:141-143,
+ * monitoring.rs:470-481). The history follows validate_pricing (:319-358):
+ * after accumulation or knock-out observations began, the settlement state
+ * and the accumulated quantity must be known (KG_MISSING_HISTORY
+ * otherwise). */
+enum { KG_SETTLED_UNKNOWN = -1, KG_SETTLED_NO = 0, KG_SETTLED_YES = 1 };
+
 typedef struct kg_accumulator_terms {
   int32_t option_type; /* 1 accumulator (call), -1 decumulator (put) */
   int32_t calendar;    /* KG_CALENDAR_* */
@@ -484,10 +495,15 @@ typedef struct kg_accumulator_terms {
   double knock_out_level;
   double daily_quantity;
   double acceleration_factor;
-  double accumulated_quantity; /* quantity accrued before valuation */
+  double accumulated_quantity; /* quantity accrued before valuation, if has_accumulated_quantity */
   int64_t observation_clock_micros;
   int64_t effective_time;
   int64_t expiry_time;
+  int32_t has_accumulated_quantity;
+  int32_t knock_out_settled;        /* KG_SETTLED_* */
+  int32_t has_knock_out_schedule;   /* 0: daily knock-out at the observation clock */
+  int32_t n_knock_out_observations; /* explicit knock-out times (the schedule may be empty) */
+  const int64_t* knock_out_observation_times; /* strictly increasing trading days in the contract life */
 } kg_accumulator_terms;
 
 kg_status kg_accumulator_price(kg_context* ctx, const kg_settings* settings,

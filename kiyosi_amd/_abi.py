@@ -8,6 +8,8 @@ INTERRUPTED, DEVICE_ERROR, MISSING_HISTORY = 4, 5, 6
 PSEUDO_RANDOM, SOBOL = 0, 1
 FP64, FP32_STEP = 0, 1
 
+SETTLED_UNKNOWN, SETTLED_NO, SETTLED_YES = -1, 0, 1
+
 STATUS_AVAILABLE, STATUS_UNSUPPORTED_SAMPLING, STATUS_INSUFFICIENT_OBSERVATIONS = 0, 1, 2
 STATUS_INCOMPLETE_ANTITHETIC_PAIR, STATUS_ZERO_OBSERVED_VARIANCE = 3, 4
 
@@ -241,6 +243,11 @@ class AccumulatorTerms(C.Structure):
         ("observation_clock_micros", C.c_int64),
         ("effective_time", C.c_int64),
         ("expiry_time", C.c_int64),
+        ("has_accumulated_quantity", C.c_int32),
+        ("knock_out_settled", C.c_int32),
+        ("has_knock_out_schedule", C.c_int32),
+        ("n_knock_out_observations", C.c_int32),
+        ("knock_out_observation_times", C.POINTER(C.c_int64)),
     ]
 
 

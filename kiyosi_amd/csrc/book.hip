@@ -834,58 +834,219 @@ static Planned plan_asian(const kg_asian_terms& o, const kg_bsm_market& m, Blob&
   return pl;
 }
 
-// MonteCarloAccumulatorEngine::sample plan (engines/monte_carlo/accumulator.rs:91-180),
-// daily accrual without fixed-purchase grouping.
+// The accumulator's schedules and state (instruments/accumulator.rs).
+struct AccumTerms {
+  const kg_accumulator_terms& o;
+  static constexpr int64_t DAY = 86400LL * 1000000LL;
+  bool trading(int64_t t) const {  // TradingCalendar::is_trading_day of t's date
+    const int64_t d = floor_div64(t, DAY);
+    return o.calendar == KG_CALENDAR_ALL_DAYS || ((d % 7) + 7 + 3) % 7 < 5;
+  }
+  int64_t at_clock(int64_t day) const { return day * DAY + o.observation_clock_micros; }
+  bool on_clock(int64_t t) const { return t - floor_div64(t, DAY) * DAY == o.observation_clock_micros; }
+  bool explicit_ko() const { return o.has_knock_out_schedule != 0; }
+  // ObservationSchedule::daily_times (time/schedule.rs:126-151)
+  std::vector<int64_t> daily_times(int64_t start, int64_t end, bool include_start) const {
+    std::vector<int64_t> out;
+    for (int64_t d = floor_div64(start, DAY);; ++d) {
+      const int64_t t = at_clock(d);
+      if ((t > start || (include_start && t == start)) && t <= end && trading(t)) out.push_back(t);
+      if (d == floor_div64(end, DAY)) break;
+    }
+    return out;
+  }
+  // ObservationSchedule::any_daily (:153-172) over [start, end].
+  template <class P>
+  bool any_daily(int64_t start, int64_t end, P pred) const {
+    for (int64_t d = floor_div64(start, DAY);; ++d) {
+      const int64_t t = at_clock(d);
+      if (t > end) break;
+      if (t >= start && trading(t) && pred(t)) return true;
+      if (d == floor_div64(end, DAY)) break;
+    }
+    return false;
+  }
+  template <class P>
+  bool any_accumulation(int64_t start, int64_t end, P pred) const {  // :494-507
+    return any_daily(std::max(start, o.effective_time), std::min(end, o.expiry_time), pred);
+  }
+  bool observes_knock_out(int64_t t) const {  // :512-517
+    if (!explicit_ko()) return on_clock(t);
+    return std::binary_search(o.knock_out_observation_times, o.knock_out_observation_times + o.n_knock_out_observations,
+                              t);
+  }
+  bool accumulates_at(int64_t t) const {  // :521-526
+    return o.effective_time <= t && t <= o.expiry_time && on_clock(t) && trading(t);
+  }
+  bool is_knock_out(double spot) const { return o.option_type > 0 ? spot >= o.knock_out_level : spot <= o.knock_out_level; }
+  double unit_payoff(double spot) const { return o.option_type > 0 ? spot - o.strike : o.strike - spot; }
+  double purchase(double spot) const {  // :508-511
+    const double distance = spot - o.strike;
+    const bool accelerated = o.option_type > 0 ? distance < 0.0 : distance > 0.0;
+    return accelerated ? o.daily_quantity * o.acceleration_factor : o.daily_quantity;
+  }
+  bool accrues(int64_t val) const {  // :462-476
+    if (o.knock_out_settled == KG_SETTLED_YES || o.daily_quantity == 0.0 || val >= o.expiry_time) return false;
+    return any_accumulation(val, o.expiry_time, [&](int64_t t) {
+      return t > val && (o.acceleration_factor != 0.0 || !is_knock_out(o.strike) || !observes_knock_out(t));
+    });
+  }
+};
+
+// MonteCarloAccumulatorEngine::sample plan (engines/monte_carlo/accumulator.rs:91-180):
+// history validation (validate_pricing, instruments/accumulator.rs:319-358),
+// the valuation-date state (initial, :528-568), the fixing dates (future
+// observation times :360-390, or the fixed purchase events of an explicit
+// knock-out schedule, :401-454) and the per-step transition tables.
 static Planned plan_accumulator(const kg_accumulator_terms& o, const kg_bsm_market& m, Blob& blob) {
   Planned pl;
   if (o.option_type != 1 && o.option_type != -1) fail(KG_INVALID_INPUT, "option_type must be 1 or -1");
-  if (!(o.strike > 0.0) || !(o.knock_out_level > 0.0) || !(o.daily_quantity >= 0.0) ||
-      !(o.acceleration_factor >= 0.0))
-    fail(KG_INVALID_INPUT, "accumulator terms must be positive");
-  const int64_t val = m.valuation_time;
-  if (val < o.effective_time || val >= o.expiry_time)
-    fail(KG_UNSUPPORTED_OPERATION, "valuation must lie inside the accumulation period");
+  if (!(o.strike > 0.0) || !std::isfinite(o.strike)) fail(KG_INVALID_INPUT, "strike must be positive and finite");
+  if (!(o.knock_out_level > 0.0) || !std::isfinite(o.knock_out_level))
+    fail(KG_INVALID_INPUT, "knock-out level must be positive and finite");
+  if (!(o.daily_quantity >= 0.0) || !std::isfinite(o.daily_quantity) || !(o.acceleration_factor >= 0.0) ||
+      !std::isfinite(o.acceleration_factor))
+    fail(KG_INVALID_INPUT, "quantities must be finite and nonnegative");
+  if (o.has_accumulated_quantity && (!(o.accumulated_quantity >= 0.0) || !std::isfinite(o.accumulated_quantity)))
+    fail(KG_INVALID_INPUT, "accumulated_quantity must be finite and nonnegative");
+  if (o.expiry_time < o.effective_time) fail(KG_INVALID_INPUT, "expiry precedes the effective time");
+  if (o.knock_out_settled < KG_SETTLED_UNKNOWN || o.knock_out_settled > KG_SETTLED_YES)
+    fail(KG_INVALID_INPUT, "unknown knock-out settlement state");
+  if (o.has_knock_out_schedule != 0 && o.has_knock_out_schedule != 1)
+    fail(KG_INVALID_INPUT, "has_knock_out_schedule must be 0 or 1");
+  if (o.n_knock_out_observations < 0 || (o.n_knock_out_observations > 0 && !o.knock_out_observation_times) ||
+      (!o.has_knock_out_schedule && o.n_knock_out_observations != 0))
+    fail(KG_INVALID_INPUT, "knock-out observation times are missing or unexpected");
+  const AccumTerms a{o};
+  for (int i = 0; i < o.n_knock_out_observations; ++i) {
+    const int64_t t = o.knock_out_observation_times[i];
+    if (i && t <= o.knock_out_observation_times[i - 1])
+      fail(KG_INVALID_INPUT, "knock-out observation times must be strictly increasing");
+    if (t < o.effective_time || t > o.expiry_time)
+      fail(KG_INVALID_INPUT, "knock-out observation times must lie within the contract life");
+    if (!a.trading(t)) fail(KG_INVALID_INPUT, "knock-out observation times must be trading days");
+  }
   check_market(m);
+  const int64_t val = m.valuation_time;
+  if (val < o.effective_time || val > o.expiry_time)  // validate_life (monitoring.rs:470-481)
+    fail(KG_INVALID_INPUT, "valuation outside contract life");
+  // validate_pricing (:319-358)
+  const bool had_accumulation = a.any_accumulation(o.effective_time, val, [&](int64_t t) { return t < val; });
+  bool had_knock_out = had_accumulation;
+  if (a.explicit_ko()) {
+    had_knock_out = false;
+    for (int i = 0; i < o.n_knock_out_observations; ++i) had_knock_out |= o.knock_out_observation_times[i] < val;
+  }
+  if (had_knock_out && o.knock_out_settled == KG_SETTLED_UNKNOWN)
+    fail(KG_MISSING_HISTORY, "prior knock-out settlement state is required");
+  if ((had_accumulation || had_knock_out) && o.knock_out_settled != KG_SETTLED_YES && !o.has_accumulated_quantity)
+    fail(KG_MISSING_HISTORY, "prior accumulated quantity is required");
+  if (o.knock_out_settled == KG_SETTLED_YES) {
+    if (!had_knock_out) fail(KG_INVALID_INPUT, "knock-out settlement predates knock-out observations");
+  } else if (o.has_accumulated_quantity && o.accumulated_quantity > 0.0 && !had_accumulation) {
+    fail(KG_INVALID_INPUT, "accumulated_quantity predates accumulation observations");
+  }
   const double S = m.spot;
+  // initial (:528-568)
+  auto known = [&](double value) {
+    if (!std::isfinite(value)) unstable();
+    pl.known = true;
+    pl.value = value;
+    return pl;
+  };
+  if (o.knock_out_settled == KG_SETTLED_YES) return known(0.0);
+  double quantity = o.has_accumulated_quantity ? o.accumulated_quantity : 0.0;
+  if (a.observes_knock_out(val) && a.is_knock_out(S) && a.trading(val)) return known(quantity * a.unit_payoff(S));
+  if (a.accumulates_at(val)) quantity += a.purchase(S);
+  const bool accrues = a.accrues(val);
+  if (quantity == 0.0 && !accrues) return known(0.0);
+  if (val == o.expiry_time || (!accrues && m.rate == 0.0 && m.dividend == 0.0))
+    return known(quantity * a.unit_payoff(S));
+  if (!std::isfinite(quantity)) unstable();
+  // fixed_purchase_events (:401-454)
+  struct Event {
+    int64_t time;
+    double before, at;
+  };
+  std::vector<Event> events;
+  bool fixed = false;
+  if (a.explicit_ko() && !(o.acceleration_factor != 1.0 && o.daily_quantity != 0.0)) {
+    // Valuation is at or after the effective time, so fixings start after it.
+    std::vector<int64_t> fixings;
+    if (o.daily_quantity != 0.0) fixings = a.daily_times(val, o.expiry_time, false);
+    if ((double)fixings.size() * o.daily_quantity <= std::numeric_limits<double>::max()) {
+      const int64_t* ko = o.knock_out_observation_times;
+      const int nk = o.n_knock_out_observations;
+      const int64_t* first = std::upper_bound(ko, ko + nk, val);
+      std::vector<int64_t> times(first, ko + nk);
+      if (times.empty() || times.back() != o.expiry_time) times.push_back(o.expiry_time);
+      size_t previous = 0, grouped = 0;
+      for (int64_t t : times) {
+        const size_t before =
+            previous + (size_t)(std::lower_bound(fixings.begin() + previous, fixings.end(), t) - (fixings.begin() + previous));
+        const bool at = before < fixings.size() && fixings[before] == t;
+        grouped += before - previous;
+        events.push_back(Event{t, (double)(before - previous) * o.daily_quantity, at ? o.daily_quantity : 0.0});
+        previous = before + (at ? 1 : 0);
+      }
+      fixed = grouped >= 2 || o.daily_quantity == 0.0;
+    }
+  }
+  std::vector<int64_t> dates;
+  if (fixed) {
+    for (const Event& e : events) dates.push_back(e.time);
+  } else {
+    // future_observation_times (:360-390)
+    dates = a.daily_times(val, o.expiry_time, false);
+    for (int i = 0; i < o.n_knock_out_observations; ++i)
+      if (o.knock_out_observation_times[i] > val) dates.push_back(o.knock_out_observation_times[i]);
+    dates.push_back(o.expiry_time);
+    std::sort(dates.begin(), dates.end());
+    dates.erase(std::unique(dates.begin(), dates.end()), dates.end());
+  }
+  if (dates.empty()) unstable();
   std::vector<double> steps;
   std::vector<float> steps32;
+  std::vector<uint8_t> flags;
   int64_t prev = val;
-  const int64_t day_ms = 86400LL * 1000000LL;
-  for (int64_t d = floor_div64(val, day_ms); d <= floor_div64(o.expiry_time, day_ms); ++d) {
-    const int64_t at = d * day_ms + o.observation_clock_micros;
-    const bool trading = o.calendar == KG_CALENDAR_ALL_DAYS || ((d % 7) + 7 + 3) % 7 < 5;
-    if (at <= val || at > o.expiry_time || !trading) continue;
+  for (size_t k = 0; k < dates.size(); ++k) {
+    const int64_t at = dates[k];
     double drift, diff, disc;
     transition(m.rate, m.dividend, m.volatility, years(prev, at), years(val, at), &drift, &diff, &disc);
     steps.push_back(drift);
     steps.push_back(diff);
     steps.push_back(disc);
+    steps.push_back(fixed ? events[k].before : 0.0);
+    steps.push_back(fixed ? events[k].at : 0.0);
     steps32.push_back((float)drift);
     steps32.push_back((float)diff);
+    flags.push_back((uint8_t)((a.observes_knock_out(at) ? ACC_OBSERVE : 0) |
+                              (!fixed && a.accumulates_at(at) ? ACC_ACCUMULATE : 0)));
     prev = at;
   }
-  if (steps.empty()) fail(KG_UNSUPPORTED_OPERATION, "no fixing remains");
-  AccumDev a{};
-  a.n_steps = (int32_t)(steps.size() / 3);
-  a.sign = o.option_type;
-  a.strike_log = log_ratio(o.strike, S);
-  a.ko_log = log_ratio(o.knock_out_level, S);
-  a.strike = o.strike;
-  a.spot = S;
-  a.daily_qty = o.daily_quantity;
-  a.accel = o.acceleration_factor;
-  a.acc_qty = o.accumulated_quantity;
+  AccumDev d{};
+  d.n_steps = (int32_t)dates.size();
+  d.fixed = fixed;
+  d.sign = o.option_type;
+  d.strike_log = log_ratio(o.strike, S);
+  d.ko_log = log_ratio(o.knock_out_level, S);
+  d.strike = o.strike;
+  d.spot = S;
+  d.daily_qty = o.daily_quantity;
+  d.accel = o.acceleration_factor;
+  d.acc_qty = quantity;
   const double lo = std::log(std::numeric_limits<double>::min()) + 1.0;
   const double hi = std::log(std::numeric_limits<double>::max()) - 1.0;
-  a.safe_lo = std::max(lo, lo - std::log(S));
-  a.safe_hi = std::min(hi, hi - std::log(S));
-  a.terminal_discount = steps[steps.size() - 1];
+  d.safe_lo = std::max(lo, lo - std::log(S));
+  d.safe_hi = std::min(hi, hi - std::log(S));
+  d.terminal_discount = steps[5 * (dates.size() - 1) + 2];
   pl.item.kind = IK_ACCUM;
-  pl.item.dims = a.n_steps;
-  pl.item.u.acc = a;
-  const size_t o1 = blob.add(steps), o2 = blob.add(steps32);
+  pl.item.dims = d.n_steps;
+  pl.item.u.acc = d;
+  const size_t o1 = blob.add(steps), o2 = blob.add(steps32), o3 = blob.add(flags);
   pl.stage(o1, pl.item.u.acc.steps);
   pl.stage(o2, pl.item.u.acc.steps32);
+  pl.stage(o3, pl.item.u.acc.flags);
   return pl;
 }
 
@@ -995,7 +1156,7 @@ static void check_tables(const BookItem& it) {
       ok = ok && it.u.b.steps && it.u.b.steps32;
       break;
     case IK_ACCUM:
-      ok = ok && it.u.acc.steps && it.u.acc.steps32;
+      ok = ok && it.u.acc.steps && it.u.acc.steps32 && it.u.acc.flags;
       break;
     case IK_AUTOCALL: {
       const AcPlanDev& p = it.u.ac;

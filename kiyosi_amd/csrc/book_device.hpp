@@ -580,12 +580,17 @@ __device__ double arith_asian(const ArithAsianDev& p, D& dr, bool live, float si
 }
 
 // ------------------------------------------------------------------ accumulator
+// Step flags of the accumulator path.
+enum : uint8_t { ACC_OBSERVE = 1, ACC_ACCUMULATE = 2 };
+
 struct AccumDev {
-  int32_t n_steps, pad;
+  int32_t n_steps;
+  int32_t fixed;  // purchases grouped onto knock-out dates (before / at each step)
   double sign, strike_log, ko_log, strike, spot, daily_qty, accel, acc_qty, safe_lo, safe_hi;
   double terminal_discount;
-  const double* steps;  // drift, diffusion, discount per step
-  const float* steps32; // drift, diffusion per step
+  const double* steps;   // drift, diffusion, discount, before, at per step
+  const float* steps32;  // drift, diffusion per step
+  const uint8_t* flags;  // ACC_* per step
 };
 
 template <bool FP32, class D>
@@ -613,22 +618,28 @@ __device__ double accum_path(const AccumDev& a, D& dr, bool live, float sign, ui
     if (!active) continue;
     const R z = (R)sign * dr.template normal<R>(i);
     lr += FP32 ? (R)fmaf(a.steps32[2 * i + 1], (float)z, a.steps32[2 * i])
-               : (R)fma(a.steps[3 * i + 1], (double)z, a.steps[3 * i]);
+               : (R)fma(a.steps[5 * i + 1], (double)z, a.steps[5 * i]);
     if (!((double)lr >= a.safe_lo && (double)lr <= a.safe_hi)) {
       const double sp = discounted_product(a.spot, 1.0, (double)lr);
       if (!isfinite(sp) || sp <= 0.0) bad = 1;
     }
-    if (sg * lr >= sg * ko) {
+    const uint8_t f = a.flags[i];
+    if (a.fixed) quantity += a.steps[5 * i + 3];
+    if ((f & ACC_OBSERVE) && sg * lr >= sg * ko) {
       ko_step = i;
       ko_lr = lr;
       ko_quantity = quantity;
       active = false;
       continue;
     }
-    quantity += a.daily_qty * (sg * lr < sg * sk ? a.accel : 1.0);
+    if (a.fixed)
+      quantity += a.steps[5 * i + 4];
+    else if (f & ACC_ACCUMULATE)
+      quantity += a.daily_qty * (sg * lr < sg * sk ? a.accel : 1.0);
+    if (!isfinite(quantity)) bad = 1;
   }
   if (ko_step >= 0)
-    value = discounted_product(ko_quantity, unit((double)ko_lr), a.steps[3 * ko_step + 2]);
+    value = discounted_product(ko_quantity, unit((double)ko_lr), a.steps[5 * ko_step + 2]);
   else if (active)
     value = discounted_product(quantity, unit((double)lr), a.terminal_discount);
   if (live && !isfinite(value)) bad = 1;

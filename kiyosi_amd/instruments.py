@@ -631,11 +631,14 @@ class GeometricAveragePriceOption(_AveragePrice):
 
 class Accumulator:
     """Accumulator (call) or decumulator (put) with daily accrual and knock-out
-    (instruments/accumulator.rs)."""
+    (instruments/accumulator.rs). ``knock_out_observation_schedule`` lists
+    explicit knock-out times (None: daily at the observation clock);
+    ``accumulated_quantity`` and ``knock_out_settled`` are the history before
+    valuation (None: unknown)."""
 
     def __init__(self, *, strike, knock_out_level, daily_quantity, acceleration_factor, effective_time,
                  expiry_time, observation_clock_time=_dt.time(0, 0), accumulated_quantity=None,
-                 option_type="call", calendar=None):
+                 option_type="call", calendar=None, knock_out_observation_schedule=None, knock_out_settled=None):
         if option_type not in ("call", "put"):
             raise InvalidInputError("option_type must be 'call' or 'put'")
         self.strike = _positive("strike", strike)
@@ -645,16 +648,28 @@ class Accumulator:
         self.effective_time = effective_time
         self.expiry_time = expiry_time
         self.observation_clock_time = observation_clock_time
-        self.accumulated_quantity = float(accumulated_quantity or 0.0)
+        self.accumulated_quantity = None if accumulated_quantity is None else float(accumulated_quantity)
         self.option_type = option_type
         self.calendar = calendar if calendar is not None else TradingCalendar.weekdays()
+        self.knock_out_observation_schedule = (None if knock_out_observation_schedule is None
+                                               else list(knock_out_observation_schedule))
+        if knock_out_settled not in (None, True, False):
+            raise InvalidInputError("knock_out_settled must be None, True or False")
+        self.knock_out_settled = knock_out_settled
 
     def _terms(self):
-        return _abi.AccumulatorTerms(
+        times = self.knock_out_observation_schedule
+        arr = (C.c_int64 * len(times))(*[to_micros(t) for t in times]) if times else None
+        settled = {None: _abi.SETTLED_UNKNOWN, False: _abi.SETTLED_NO, True: _abi.SETTLED_YES}
+        t = _abi.AccumulatorTerms(
             1 if self.option_type == "call" else -1, self.calendar.kind, self.strike, self.knock_out_level,
-            self.daily_quantity, self.acceleration_factor, self.accumulated_quantity,
+            self.daily_quantity, self.acceleration_factor, self.accumulated_quantity or 0.0,
             clock_micros(self.observation_clock_time), to_micros(self.effective_time),
-            to_micros(self.expiry_time))
+            to_micros(self.expiry_time), int(self.accumulated_quantity is not None),
+            settled[self.knock_out_settled], int(times is not None), len(times) if times else 0,
+            C.cast(arr, C.POINTER(C.c_int64)) if arr else None)
+        t._keep = arr
+        return t
 
 
 class AmericanOption(EuropeanOption):

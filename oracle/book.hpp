@@ -802,35 +802,160 @@ inline Estimate asian_price(const Settings& st, const kg_asian_terms& o, const k
 }
 
 // ---------------------------------------------------------------- accumulator
+// Accumulator schedule and history (instruments/accumulator.rs:319-568),
+// restated with plain scans over the calendar days of the contract life.
+struct AccumulatorSchedule {
+  const kg_accumulator_terms& o;
+  int64_t clock_at(int64_t day) const { return day * MICROS_PER_DAY + o.observation_clock_micros; }
+  bool at_clock(int64_t t) const { return t - day_of(t) * MICROS_PER_DAY == o.observation_clock_micros; }
+  bool trading(int64_t t) const { return is_trading_day(o.calendar, day_of(t)); }
+  // Every accumulation fixing of the life (accumulation_schedule, :309-317).
+  std::vector<int64_t> fixings() const {
+    std::vector<int64_t> out;
+    for (int64_t d = day_of(o.effective_time); d <= day_of(o.expiry_time); ++d) {
+      const int64_t t = clock_at(d);
+      if (t >= o.effective_time && t <= o.expiry_time && is_trading_day(o.calendar, d)) out.push_back(t);
+    }
+    return out;
+  }
+  bool explicit_schedule() const { return o.has_knock_out_schedule != 0; }
+  bool observes(int64_t t) const {  // observes_knock_out (:512-517)
+    if (!explicit_schedule()) return at_clock(t);
+    for (int i = 0; i < o.n_knock_out_observations; ++i)
+      if (o.knock_out_observation_times[i] == t) return true;
+    return false;
+  }
+  bool accumulates(int64_t t) const {  // accumulates_at (:521-526)
+    return o.effective_time <= t && t <= o.expiry_time && at_clock(t) && trading(t);
+  }
+  bool knocked(double spot) const { return o.option_type > 0 ? spot >= o.knock_out_level : spot <= o.knock_out_level; }
+  double unit(double spot) const { return o.option_type > 0 ? spot - o.strike : o.strike - spot; }
+  double purchase(double spot) const {  // purchase (:508-511)
+    const bool accelerated = o.option_type > 0 ? spot - o.strike < 0.0 : spot - o.strike > 0.0;
+    return accelerated ? o.daily_quantity * o.acceleration_factor : o.daily_quantity;
+  }
+};
+
 inline Estimate accumulator_price(const Settings& st, const kg_accumulator_terms& o,
                                   const kg_bsm_market& m, const Context& cx) {
-  const int64_t val = m.valuation_time;
-  if (val < o.effective_time || val >= o.expiry_time)
-    throw Failure(UNSUPPORTED, "valuation must lie inside the accumulation period");
+  if (o.option_type != 1 && o.option_type != -1) throw Failure(INVALID_INPUT, "option type");
+  for (double v : {o.strike, o.knock_out_level})
+    if (!(v > 0.0) || !std::isfinite(v)) throw Failure(INVALID_INPUT, "strike and level must be positive");
+  for (double v : {o.daily_quantity, o.acceleration_factor, o.has_accumulated_quantity ? o.accumulated_quantity : 0.0})
+    if (!(v >= 0.0) || !std::isfinite(v)) throw Failure(INVALID_INPUT, "quantities must be finite and nonnegative");
+  if (o.expiry_time < o.effective_time) throw Failure(INVALID_INPUT, "expiry precedes effective date");
+  const AccumulatorSchedule a{o};
+  for (int i = 0; i < o.n_knock_out_observations; ++i) {
+    const int64_t t = o.knock_out_observation_times[i];
+    if ((i > 0 && t <= o.knock_out_observation_times[i - 1]) || t < o.effective_time || t > o.expiry_time ||
+        !a.trading(t))
+      throw Failure(INVALID_INPUT, "knock-out times");
+  }
   if (m.volatility == 0.0) throw Failure(UNSUPPORTED, "zero volatility is deterministic");
-  const double sign = o.option_type;
+  const int64_t val = m.valuation_time;
+  if (val < o.effective_time || val > o.expiry_time) throw Failure(INVALID_INPUT, "valuation outside contract life");
+  const std::vector<int64_t> all = a.fixings();
+  // validate_pricing (:319-358)
+  bool had_accumulation = false, had_knock_out = false;
+  for (int64_t t : all) had_accumulation |= t < val;
+  if (a.explicit_schedule()) {
+    for (int i = 0; i < o.n_knock_out_observations; ++i) had_knock_out |= o.knock_out_observation_times[i] < val;
+  } else {
+    had_knock_out = had_accumulation;
+  }
+  if (had_knock_out && o.knock_out_settled == KG_SETTLED_UNKNOWN)
+    throw Failure(MISSING_HISTORY, "prior knock-out settlement state is required");
+  if ((had_accumulation || had_knock_out) && o.knock_out_settled != KG_SETTLED_YES && !o.has_accumulated_quantity)
+    throw Failure(MISSING_HISTORY, "prior accumulated quantity is required");
+  if (o.knock_out_settled == KG_SETTLED_YES && !had_knock_out)
+    throw Failure(INVALID_INPUT, "settlement predates knock-out observations");
+  if (o.knock_out_settled != KG_SETTLED_YES && o.has_accumulated_quantity && o.accumulated_quantity > 0.0 &&
+      !had_accumulation)
+    throw Failure(INVALID_INPUT, "accumulated quantity predates accumulation");
+  // initial (:528-568)
+  if (o.knock_out_settled == KG_SETTLED_YES) return known_estimate(st, 0.0);
   const double S = m.spot;
-  const double log_spot = std::log(S);
-  const double lo = std::log(std::numeric_limits<double>::min()) + 1.0;
-  const double hi = std::log(std::numeric_limits<double>::max()) - 1.0;
-  const double safe_lo = std::max(lo, lo - log_spot), safe_hi = std::min(hi, hi - log_spot);
-  const double strike = log_ratio(o.strike, S), knock_out = log_ratio(o.knock_out_level, S);
-  // Daily fixings after valuation at the observation clock (accumulator.rs:150-175).
+  double quantity = o.has_accumulated_quantity ? o.accumulated_quantity : 0.0;
+  if (a.observes(val) && a.knocked(S) && a.trading(val)) return known_estimate(st, quantity * a.unit(S));
+  if (a.accumulates(val)) quantity += a.purchase(S);
+  bool accrues = false;  // accrues (:462-476)
+  if (o.daily_quantity != 0.0 && val < o.expiry_time)
+    for (int64_t t : all)
+      accrues |= t > val && (o.acceleration_factor != 0.0 || !a.knocked(o.strike) || !a.observes(t));
+  if (quantity == 0.0 && !accrues) return known_estimate(st, 0.0);
+  if (val == o.expiry_time || (!accrues && m.rate == 0.0 && m.dividend == 0.0))
+    return known_estimate(st, quantity * a.unit(S));
+  if (!std::isfinite(quantity)) unstable();
+  // Simulated dates and their purchases (future_observation_times :360-390,
+  // fixed_purchase_events :401-454).
+  std::vector<int64_t> future;
+  for (int64_t t : all)
+    if (t > val) future.push_back(t);
+  std::vector<int64_t> knock_out_dates;
+  for (int i = 0; i < o.n_knock_out_observations; ++i)
+    if (o.knock_out_observation_times[i] > val) knock_out_dates.push_back(o.knock_out_observation_times[i]);
+  struct Date {
+    int64_t time;
+    double before = 0.0, at = 0.0;
+    bool observe = false, accumulate = false;
+  };
+  std::vector<Date> dates;
+  bool fixed = false;
+  if (a.explicit_schedule() && (o.acceleration_factor == 1.0 || o.daily_quantity == 0.0)) {
+    std::vector<int64_t> events = knock_out_dates;
+    if (events.empty() || events.back() != o.expiry_time) events.push_back(o.expiry_time);
+    const std::vector<int64_t> purchases = o.daily_quantity == 0.0 ? std::vector<int64_t>{} : future;
+    int64_t previous = val;
+    size_t grouped = 0;
+    for (int64_t e : events) {
+      Date d;
+      d.time = e;
+      size_t strictly_between = 0;
+      for (int64_t t : purchases) {
+        if (t > previous && t < e) ++strictly_between;
+        if (t == e) d.at = o.daily_quantity;
+      }
+      grouped += strictly_between;
+      d.before = (double)strictly_between * o.daily_quantity;
+      dates.push_back(d);
+      previous = e;
+    }
+    fixed = (grouped >= 2 || o.daily_quantity == 0.0) &&
+            std::isfinite((double)purchases.size() * o.daily_quantity);
+  }
+  if (!fixed) {
+    dates.clear();
+    std::vector<int64_t> times = future;
+    times.insert(times.end(), knock_out_dates.begin(), knock_out_dates.end());
+    times.push_back(o.expiry_time);
+    std::sort(times.begin(), times.end());
+    times.erase(std::unique(times.begin(), times.end()), times.end());
+    for (int64_t t : times) {
+      Date d;
+      d.time = t;
+      d.accumulate = a.accumulates(t);
+      dates.push_back(d);
+    }
+  }
   struct Step {
     double drift, diffusion, discount;
   };
   std::vector<Step> steps;
   int64_t prev = val;
-  for (int64_t d = day_of(val); d <= day_of(o.expiry_time); ++d) {
-    int64_t at = d * MICROS_PER_DAY + o.observation_clock_micros;
-    if (at <= val || at > o.expiry_time || !is_trading_day(o.calendar, d)) continue;
+  for (Date& d : dates) {
+    d.observe = a.observes(d.time);
     Step s;
-    transition(m.rate, m.dividend, m.volatility, years_until(prev, at), years_until(val, at), &s.drift,
+    transition(m.rate, m.dividend, m.volatility, years_until(prev, d.time), years_until(val, d.time), &s.drift,
                &s.diffusion, &s.discount);
     steps.push_back(s);
-    prev = at;
+    prev = d.time;
   }
-  if (steps.empty()) throw Failure(UNSUPPORTED, "no fixing remains");
+  const double sign = o.option_type;
+  const double log_spot = std::log(S);
+  const double lo = std::log(std::numeric_limits<double>::min()) + 1.0;
+  const double hi = std::log(std::numeric_limits<double>::max()) - 1.0;
+  const double safe_lo = std::max(lo, lo - log_spot), safe_hi = std::min(hi, hi - log_spot);
+  const double strike = log_ratio(o.strike, S), knock_out = log_ratio(o.knock_out_level, S);
   const double terminal_discount = steps.back().discount;
   auto unit_payoff = [&](double spot, double lr) {
     double distance = lr - strike;
@@ -842,22 +967,29 @@ inline Estimate accumulator_price(const Settings& st, const kg_accumulator_terms
     if (s <= 0.0) unstable();
     return s;
   };
+  // MonteCarloAccumulatorEngine::sample's path (engines/monte_carlo/accumulator.rs:181-214).
   Moments total[1], pairs[1];
   path_samples_lazy<1>(
       st, steps.size(), true, cx,
       [&](LazyPath& normals, double* out) {
-        double lr = 0.0, quantity = o.accumulated_quantity;
-        for (const Step& s : steps) {
+        double lr = 0.0, held = quantity;
+        for (size_t i = 0; i < steps.size(); ++i) {
+          const Step& s = steps[i];
+          const Date& d = dates[i];
           lr += std::fma(s.diffusion, normals.normal(), s.drift);
           if (!(lr >= safe_lo && lr <= safe_hi)) spot_at(lr);
-          if (sign * lr >= sign * knock_out) {
-            out[0] = scaled_value(quantity, unit_payoff(spot_at(lr), lr), s.discount);
+          held += d.before;
+          if (d.observe && sign * lr >= sign * knock_out) {
+            out[0] = scaled_value(held, unit_payoff(spot_at(lr), lr), s.discount);
             return;
           }
-          quantity += o.daily_quantity * (sign * lr < sign * strike ? o.acceleration_factor : 1.0);
-          if (!std::isfinite(quantity)) unstable();
+          if (fixed)
+            held += d.at;
+          else if (d.accumulate)
+            held += o.daily_quantity * (sign * lr < sign * strike ? o.acceleration_factor : 1.0);
+          if (!std::isfinite(held)) unstable();
         }
-        out[0] = scaled_value(quantity, unit_payoff(spot_at(lr), lr), terminal_discount);
+        out[0] = scaled_value(held, unit_payoff(spot_at(lr), lr), terminal_discount);
       },
       total, pairs);
   return estimate(st, total[0], pairs[0], 0.0, false, true, st.seed);
