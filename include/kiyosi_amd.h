@@ -35,7 +35,8 @@ typedef enum kg_status {
   KG_COMPUTATION_FAILURE = 3,   /* Error::ComputationFailure (non-finite path or result) */
   KG_INTERRUPTED = 4,           /* Error::Io(Interrupted): the cancel flag was raised */
   KG_DEVICE_ERROR = 5,          /* Error::Io: a HIP runtime failure */
-  KG_MISSING_HISTORY = 6        /* Error::MissingHistory */
+  KG_MISSING_HISTORY = 6,       /* Error::MissingHistory */
+  KG_NON_CONVERGENCE = 7        /* Error::NonConvergence (a series or quadrature did not converge) */
 } kg_status;
 
 /* MonteCarloSampling (monte_carlo.rs:76-82). */
@@ -317,11 +318,18 @@ kg_status kg_heston_price(kg_context* ctx, const kg_settings* settings,
  * cumulative noise against the barrier on each observation date (knock-ins
  * with more than 8 fixings left are conditioned on the hit with a lognormal
  * price, :320-333); continuous monitoring draws the terminal value and
- * weights it by the Brownian-bridge survival (:402-440). Delta and gamma:
- * kg_barrier_price_with_greeks below. */
+ * weights it by the Brownian-bridge survival (:402-440). A continuous
+ * knock-out whose rebate is paid at the hit carries no rebate on its paths
+ * and adds the analytic value of the hit payment (:486-531). touch_state is
+ * the history before valuation (BarrierTouchState, instruments/barrier.rs:
+ * 38-47; monitoring.rs:277-313): once monitoring began it must be known
+ * (KG_MISSING_HISTORY), a touch before monitoring is invalid, and a prior
+ * touch settles the contract (its at-hit rebate already paid). Delta and
+ * gamma: kg_barrier_price_with_greeks below. */
 enum { KG_DOWN_AND_OUT = 0, KG_DOWN_AND_IN = 1, KG_UP_AND_OUT = 2, KG_UP_AND_IN = 3 };
 enum { KG_OBSERVE_CONTINUOUS = 0, KG_OBSERVE_SCHEDULED = 1 };
 enum { KG_REBATE_AT_EXPIRY = 0, KG_REBATE_AT_HIT = 1 };
+enum { KG_TOUCH_UNKNOWN = -1, KG_TOUCH_UNTOUCHED = 0, KG_TOUCH_TOUCHED = 1 };
 
 typedef struct kg_barrier_terms {
   int32_t option_type;  /* 1 call, -1 put */
@@ -334,7 +342,7 @@ typedef struct kg_barrier_terms {
   int64_t effective_time;
   int64_t expiry_time;
   int32_t n_observations; /* scheduled fixings, strictly increasing */
-  int32_t reserved;
+  int32_t touch_state;    /* KG_TOUCH_*: observations strictly before valuation */
   const int64_t* observation_times;
 } kg_barrier_terms;
 
@@ -375,9 +383,10 @@ kg_status kg_barrier_price_with_greeks(kg_context* ctx, const kg_settings* setti
  * lower (down-type) barrier plus the upper level
  * (instruments/double_barrier.rs:119-144). Scheduled fixings knock on
  * either level; continuous monitoring weights the terminal draw by the
- * image/spectral survival series of double_barrier.rs:6-112. Continuous
- * at-hit knock-out rebates need the analytic first-exit value and return
- * KG_UNSUPPORTED_OPERATION, as for single barriers. */
+ * image/spectral survival series of double_barrier.rs:6-112. A continuous
+ * knock-out with an at-hit rebate adds the analytic first-exit value of the
+ * rebate (engines/analytic/double_barrier.rs:423-520); touch_state as for
+ * single barriers. */
 enum { KG_DOUBLE_KNOCK_OUT = 0, KG_DOUBLE_KNOCK_IN = 1 };
 
 typedef struct kg_double_barrier_terms {
@@ -392,7 +401,7 @@ typedef struct kg_double_barrier_terms {
   int64_t effective_time;
   int64_t expiry_time;
   int32_t n_observations; /* scheduled fixings, strictly increasing */
-  int32_t reserved;
+  int32_t touch_state;    /* KG_TOUCH_* */
   const int64_t* observation_times;
 } kg_double_barrier_terms;
 

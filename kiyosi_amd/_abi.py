@@ -3,7 +3,7 @@
 import ctypes as C
 
 OK, INVALID_INPUT, UNSUPPORTED_OPERATION, COMPUTATION_FAILURE = 0, 1, 2, 3
-INTERRUPTED, DEVICE_ERROR, MISSING_HISTORY = 4, 5, 6
+INTERRUPTED, DEVICE_ERROR, MISSING_HISTORY, NON_CONVERGENCE = 4, 5, 6, 7
 
 PSEUDO_RANDOM, SOBOL = 0, 1
 FP64, FP32_STEP = 0, 1
@@ -158,6 +158,7 @@ AVERAGE_ARITHMETIC, AVERAGE_GEOMETRIC = 0, 1
 BOOK_BARRIER, BOOK_ASIAN, BOOK_AUTOCALL, BOOK_ACCUMULATOR, BOOK_DOUBLE_BARRIER, BOOK_DIGITAL = 0, 1, 2, 3, 4, 5
 GREEK_DELTA, GREEK_GAMMA = 1, 2
 DOUBLE_KNOCK_OUT, DOUBLE_KNOCK_IN = 0, 1
+TOUCH_UNKNOWN, TOUCH_UNTOUCHED, TOUCH_TOUCHED = -1, 0, 1
 CASH_OR_NOTHING, ASSET_OR_NOTHING = 0, 1
 
 
@@ -173,7 +174,7 @@ class BarrierTerms(C.Structure):
         ("effective_time", C.c_int64),
         ("expiry_time", C.c_int64),
         ("n_observations", C.c_int32),
-        ("reserved", C.c_int32),
+        ("touch_state", C.c_int32),
         ("observation_times", C.POINTER(C.c_int64)),
     ]
 
@@ -201,7 +202,7 @@ class DoubleBarrierTerms(C.Structure):
         ("effective_time", C.c_int64),
         ("expiry_time", C.c_int64),
         ("n_observations", C.c_int32),
-        ("reserved", C.c_int32),
+        ("touch_state", C.c_int32),
         ("observation_times", C.POINTER(C.c_int64)),
     ]
 

@@ -20,6 +20,7 @@
 #include <numeric>
 
 #include "book_device.hpp"
+#include "hit_value.hpp"
 
 namespace kg {
 
@@ -140,8 +141,9 @@ __device__ inline void fold_block(const double* pos, const double* neg, const un
   *pairs = wave_merge(p);
 }
 
+// Two waves per SIMD: at most 256 VGPRs in either precision.
 template <bool FP32, int SPT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
     book_kernel(const BookItem* __restrict__ items, const int2* __restrict__ tiles, int n_tiles,
                 int* __restrict__ counter, Moments* __restrict__ partials,
                 uint32_t* __restrict__ flags, const volatile int32_t* cancel) {
@@ -378,8 +380,12 @@ struct Planned {
   bool known = false;
   double value = 0.0;
   Estimator kind = Estimator::Estimate;
-  double control_expected = 0.0;  // arithmetic Asian control mean
-  double control_delta = 0.0, control_gamma = 0.0;  // and its delta and gamma
+  // Known addend of the path mean, with its delta and gamma: the exact
+  // mean of the arithmetic Asian's geometric control (asian.rs:612-616), or
+  // the analytic at-hit rebate value of a continuous knock-out
+  // (barrier.rs:486-531).
+  double offset = 0.0;
+  double offset_delta = 0.0, offset_gamma = 0.0;
   // Spot Greeks requested and kept after the engine's filters, and the
   // values of a contract resolved before sampling.
   int32_t greeks = 0;
@@ -444,7 +450,7 @@ struct BarrierContract {
   bool has_upper;
   double upper;
   int64_t effective_time, expiry_time;
-  int32_t n_observations;
+  int32_t n_observations, touch_state;
   const int64_t* observation_times;
 };
 static BarrierContract contract_of(const kg_barrier_terms& o) {
@@ -452,7 +458,7 @@ static BarrierContract contract_of(const kg_barrier_terms& o) {
     fail(KG_INVALID_INPUT, "unknown barrier type");
   return {o.option_type,    o.barrier_type, o.observation, o.rebate_timing,  o.strike,
           o.barrier,        o.rebate,       false,         0.0,              o.effective_time,
-          o.expiry_time,    o.n_observations, o.observation_times};
+          o.expiry_time,    o.n_observations, o.touch_state, o.observation_times};
 }
 // DoubleBarrierOption::new (instruments/double_barrier.rs:119-144).
 static BarrierContract contract_of(const kg_double_barrier_terms& o) {
@@ -475,6 +481,7 @@ static BarrierContract contract_of(const kg_double_barrier_terms& o) {
           o.effective_time,
           o.expiry_time,
           o.n_observations,
+          o.touch_state,
           o.observation_times};
 }
 
@@ -490,8 +497,10 @@ static void vanilla_spot_greeks(double S, double K, double r, double q, double s
 }
 
 // MonteCarloBarrierEngine::native plan (engines/monte_carlo/barrier.rs:160-290)
-// for flat rates and no prior touch. `request` (KG_GREEK_* bits) and
-// `radius` drive the spot-Greek filter of :186-205.
+// for flat rates. `request` (KG_GREEK_* bits) and `radius` drive the
+// spot-Greek filter of :186-205; a continuous knock-out with an at-hit
+// rebate adds the analytic hit value and its central spot differences
+// over the same radius (:486-531, risk/numerical.rs:359-430).
 static Planned plan_barrier(const BarrierContract& o, const kg_bsm_market& m, Blob& blob, int32_t request = 0,
                             double radius = 0.0) {
   Planned pl;
@@ -505,6 +514,8 @@ static Planned plan_barrier(const BarrierContract& o, const kg_bsm_market& m, Bl
     if (o.observation_times[i] <= o.observation_times[i - 1])
       fail(KG_INVALID_INPUT, "observation times must be strictly increasing");
   if (!(radius >= 0.0) || !std::isfinite(radius)) fail(KG_INVALID_INPUT, "spot_radius must be nonnegative");
+  if (o.touch_state < KG_TOUCH_UNKNOWN || o.touch_state > KG_TOUCH_TOUCHED)
+    fail(KG_INVALID_INPUT, "unknown touch state");
   const int64_t val = m.valuation_time, expiry = o.expiry_time;
   if (val < o.effective_time || val > expiry) fail(KG_INVALID_INPUT, "valuation outside contract life");
   check_market(m);
@@ -514,11 +525,16 @@ static Planned plan_barrier(const BarrierContract& o, const kg_bsm_market& m, Bl
   const bool at_hit = o.rebate_timing == KG_REBATE_AT_HIT;
   const double sign = o.option_type, r = m.rate, q = m.dividend, sigma = m.volatility, S = m.spot;
   const double T = (double)(expiry - val) / (365.0 * 86400e6);
-  // Monitoring state (instruments/monitoring.rs:293-313).
+  // Monitoring state (instruments/monitoring.rs:272-313): the prior touch
+  // from the supplied history, then a touch at the valuation itself.
+  const bool had = continuous ? val > o.effective_time : o.observation_times[0] < val;
+  if (!had && o.touch_state == KG_TOUCH_TOUCHED) fail(KG_INVALID_INPUT, "touch predates monitoring");
+  if (had && o.touch_state == KG_TOUCH_UNKNOWN) fail(KG_MISSING_HISTORY, "prior barrier touch state is required");
+  const bool prior = o.touch_state == KG_TOUCH_TOUCHED;
   bool observed_now = continuous;
   for (int i = 0; i < o.n_observations && !continuous; ++i) observed_now |= o.observation_times[i] == val;
   const bool touched =
-      observed_now && ((up ? S >= o.barrier : S <= o.barrier) || (o.has_upper && S >= o.upper));
+      prior || (observed_now && ((up ? S >= o.barrier : S <= o.barrier) || (o.has_upper && S >= o.upper)));
   const bool finished =
       val == expiry || (!continuous && o.observation_times[o.n_observations - 1] <= val);
   const bool last_is_expiry =
@@ -530,14 +546,17 @@ static Planned plan_barrier(const BarrierContract& o, const kg_bsm_market& m, Bl
   // Spot Greeks: never at expiry, nor when the stencil crosses a monitored
   // level (spot_discontinuity, monitoring.rs:430-445).
   auto crosses = [&](double level) { return S - radius <= level && level <= S + radius; };
-  const bool discontinuity = observed_now && (crosses(o.barrier) || (o.has_upper && crosses(o.upper)));
+  const bool discontinuity = !prior && observed_now && (crosses(o.barrier) || (o.has_upper && crosses(o.upper)));
   pl.greeks = (val != expiry && !discontinuity) ? request & (KG_GREEK_DELTA | KG_GREEK_GAMMA) : 0;
   const double cash = o.rebate == 0.0 ? 0.0 : o.rebate * std::exp(-r * T);
   const bool fixed = o.rebate == 0.0 && requires_touch;
   // resolved_price_with_state (engines/analytic/barrier.rs:32-80).
   if ((touched && !knock_in) || (!touched && finished && knock_in) || (fixed && !knock_in)) {
     pl.known = true;
-    pl.value = (touched && !knock_in) ? (at_hit ? o.rebate : cash) : (!touched && finished && knock_in) ? cash : 0.0;
+    // A prior touch has already paid an at-hit rebate.
+    pl.value = (touched && !knock_in) ? (at_hit ? (prior ? 0.0 : o.rebate) : cash)
+               : (!touched && finished && knock_in) ? cash
+                                                    : 0.0;
     return pl;
   }
   if (touched || finished || fixed) {
@@ -551,9 +570,27 @@ static Planned plan_barrier(const BarrierContract& o, const kg_bsm_market& m, Bl
     }
     return pl;
   }
-  if (continuous && !knock_in && at_hit && o.rebate != 0.0)
-    fail(KG_UNSUPPORTED_OPERATION, "continuous at-hit rebates need the analytic hit value");
   const double carry = r - q, hvr = 0.5 * sigma * sigma, drift = carry - hvr;
+  if (continuous && !knock_in && at_hit && o.rebate != 0.0) {
+    // The paths carry no rebate; its value at the first hit is analytic.
+    auto hit_at = [&](double spot) {
+      if (o.has_upper) return hitv::Band(o.barrier, o.upper, spot, r, q, sigma, T).hit(o.rebate, r);
+      return hitv::single(o.rebate, std::fabs(log_ratio(o.barrier, spot)), up, drift, sigma, T, r);
+    };
+    pl.offset = hit_at(S);
+    if (pl.greeks) {
+      // Central differences over the stencil radius; without a usable
+      // radius the spot Greeks are unavailable.
+      const double h = radius;
+      if (h > 0.0 && S > h) {
+        const double above = hit_at(S + h), below = hit_at(S - h), center = pl.offset;
+        if (pl.greeks & KG_GREEK_DELTA) pl.offset_delta = (above - below) / h / 2.0;
+        if (pl.greeks & KG_GREEK_GAMMA) pl.offset_gamma = ((above - center) - (center - below)) / h / h;
+      } else {
+        pl.greeks = 0;
+      }
+    }
+  }
   auto years = [](int64_t a, int64_t b) { return (double)(b - a) / (365.0 * 86400e6); };
   auto mean_distance = [&](int64_t date, double level) {
     const double e = years(val, date);
@@ -821,10 +858,10 @@ static Planned plan_asian(const kg_asian_terms& o, const kg_bsm_market& m, Blob&
       };
       const double la = lf + log_discount + lcdf(sign * d1);
       const double lc = ls + log_discount + lcdf(sign * d2);
-      pl.control_expected = sign > 0 ? pdiff(la, lc) : pdiff(lc, la);
+      pl.offset = sign > 0 ? pdiff(la, lc) : pdiff(lc, la);
       // geometric_values' delta and gamma (asian.rs:1084-1096)
-      pl.control_delta = sign * std::exp(la - std::log(m.spot));
-      pl.control_gamma = std::exp(lf + log_discount - 2.0 * std::log(m.spot) - std::log(w) - 0.5 * d1 * d1 -
+      pl.offset_delta = sign * std::exp(la - std::log(m.spot));
+      pl.offset_gamma = std::exp(lf + log_discount - 2.0 * std::log(m.spot) - std::log(w) - 0.5 * d1 * d1 -
                                   0.5 * std::log(2.0 * M_PI));
     }
   }
@@ -1349,7 +1386,7 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
       Planned& p = plans[sampled[k]];
       if (p.item.kind == IK_ARITH_ASIAN && p.item.u.a.control) {
         const double mean = scaled_value(results[2 * k].scale, results[2 * k].mean, 0.0);
-        if (mean + p.control_expected < 0.0) redo_index.push_back((int)k);
+        if (mean + p.offset < 0.0) redo_index.push_back((int)k);
       }
     }
     for (int k : redo_index) {
@@ -1367,7 +1404,7 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
       check_flags(ctx);
       KG_HIP(hipMemcpyAsync(&results[2 * k], d_out, 2 * sizeof(Moments), hipMemcpyDeviceToHost, ctx->stream));
       KG_HIP(hipStreamSynchronize(ctx->stream));
-      p.control_expected = 0.0;
+      p.offset = 0.0;
     }
   }
   // Estimates in input order.
@@ -1395,8 +1432,8 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
     const Moments& pm = results[2 * k + 1];
     ++k;
     fill_estimate(p.sampler, m, pm, 0.0, false, true, &out[i]);
-    if (p.kind == Estimator::PathMeans && p.control_expected != 0.0) {
-      out[i].price += p.control_expected;
+    if (p.kind == Estimator::PathMeans && p.offset != 0.0) {
+      out[i].price += p.offset;
       if (out[i].has_standard_error) {
         const double hw = 1.959963984540054 * out[i].standard_error;
         out[i].ci_lower = out[i].price - hw;
@@ -1491,11 +1528,11 @@ static void price_spot_greeks(kg_context* ctx, const kg_settings* settings, Plan
   Moments r[4];
   finish_sampling(ctx, parts, blocks, 4, r, &out->device_ms);
   const bool control = p.item.kind == IK_ARITH_ASIAN && p.item.u.a.control;
-  if (control && scaled_value(r[0].scale, r[0].mean, 0.0) + p.control_expected < 0.0) {
+  if (control && scaled_value(r[0].scale, r[0].mean, 0.0) + p.offset < 0.0) {
     // A controlled price below zero is resampled without the control on the
     // same seed (asian.rs:618-630).
     p.item.u.a.control = 0;
-    p.control_expected = p.control_delta = p.control_gamma = 0.0;
+    p.offset = p.offset_delta = p.offset_gamma = 0.0;
     kg_settings same = *settings;
     same.seed = s.seed;
     same.has_seed = 1;
@@ -1506,8 +1543,8 @@ static void price_spot_greeks(kg_context* ctx, const kg_settings* settings, Plan
   }
   fill_estimate(s, r[0], r[3], 0.0, false, true, out);
   out->path_steps = (double)s.path_count * (double)p.item.dims;
-  if (control) {  // the control's exact means (asian.rs:612-616)
-    out->price += p.control_expected;
+  if (p.offset != 0.0) {  // the control's exact mean, or the analytic hit value
+    out->price += p.offset;
     if (out->has_standard_error) {
       const double hw = 1.959963984540054 * out->standard_error;
       out->ci_lower = out->price - hw;
@@ -1515,8 +1552,8 @@ static void price_spot_greeks(kg_context* ctx, const kg_settings* settings, Plan
     }
   }
   greeks->available = p.greeks;
-  if (p.greeks & KG_GREEK_DELTA) greeks->delta = scaled_value(r[1].scale, r[1].mean, 0.0) + p.control_delta;
-  if (p.greeks & KG_GREEK_GAMMA) greeks->gamma = scaled_value(r[2].scale, r[2].mean, 0.0) + p.control_gamma;
+  if (p.greeks & KG_GREEK_DELTA) greeks->delta = scaled_value(r[1].scale, r[1].mean, 0.0) + p.offset_delta;
+  if (p.greeks & KG_GREEK_GAMMA) greeks->gamma = scaled_value(r[2].scale, r[2].mean, 0.0) + p.offset_gamma;
   if (!std::isfinite(greeks->delta) || !std::isfinite(greeks->gamma)) unstable();
 }
 
@@ -1626,7 +1663,7 @@ static void asian_price_strikes(kg_context* ctx, const kg_settings* settings, co
       finish_sampling(ctx, parts, blocks, 2 * m, r, &ms);
       for (int l = 0; l < m; ++l) {
         const size_t u = sampled[g + l];
-        value[u] = scaled_value(r[2 * l].scale, r[2 * l].mean, 0.0) + plans[u].control_expected;
+        value[u] = scaled_value(r[2 * l].scale, r[2 * l].mean, 0.0) + plans[u].offset;
         if (plans[u].item.u.a.control && value[u] < 0.0) redo.push_back(u);
       }
     }

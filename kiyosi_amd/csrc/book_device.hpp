@@ -459,7 +459,8 @@ __device__ ArithPath arith_path(const ArithAsianDev& p, D& dr, bool live, float 
 // ArithmeticParameters::arithmetic with the geometric control residual
 // (asian.rs:1100-1256, 1016-1061): the discounted payoff of one path state
 // for the strike terms of `p`.
-__device__ inline double arith_payoff(const ArithAsianDev& p, const ArithPath& a, uint32_t& bad) {
+__device__ __attribute__((always_inline)) inline double arith_payoff(const ArithAsianDev& p, const ArithPath& a,
+                                                                   uint32_t& bad) {
   const double inv_n = 1.0 / (double)p.n;
   const double rel = a.rel, log_sum = a.log_sum, gg = a.gg, lg = a.lead_growth;
   const double half = 0.5 * inv_n, rest = 1.0 - half;
@@ -507,7 +508,8 @@ __device__ inline double arith_payoff(const ArithAsianDev& p, const ArithPath& a
 // and geometric_values' for the control residual, :1063-1098) in lr[0] and
 // lr[1]: the last interval is integrated in closed form, so both are
 // smooth functions of the path.
-__device__ inline double arith_payoff_greeks(const ArithAsianDev& p, const ArithPath& a, uint32_t& bad,
+// A call, not inlined: inlined, it spills the FP32 spot-Greek kernel.
+__device__ __attribute__((noinline)) double arith_payoff_greeks(const ArithAsianDev& p, const ArithPath& a, uint32_t& bad,
                                              double* lr) {
   constexpr double HALF_LN_2PI = 0.9189385332046727;
   const double inv_n = 1.0 / (double)p.n;
@@ -572,11 +574,20 @@ __device__ inline double arith_payoff_greeks(const ArithAsianDev& p, const Arith
   return v;
 }
 
+// The FP64 book keeps the strike terms out of line: inlined into the FP64
+// sample body they would lift it past 256 VGPRs (one wave per SIMD).
+__device__ __attribute__((noinline)) double arith_payoff_call(const ArithAsianDev& p, const ArithPath& a,
+                                                              uint32_t& bad) {
+  return arith_payoff(p, a, bad);
+}
 template <bool FP32, class D>
 __device__ double arith_asian(const ArithAsianDev& p, D& dr, bool live, float sign, uint32_t& bad) {
   const ArithPath a = arith_path<FP32>(p, dr, live, sign);
   if (!live) return 0.0;
-  return arith_payoff(p, a, bad);
+  if constexpr (FP32)
+    return arith_payoff(p, a, bad);
+  else
+    return arith_payoff_call(p, a, bad);
 }
 
 // ------------------------------------------------------------------ accumulator

@@ -55,4 +55,6 @@ def raise_for_status(status: int, message: str) -> None:
         raise InterruptedError(message)
     if status == _abi.MISSING_HISTORY:
         raise MissingHistoryError(message)
+    if status == _abi.NON_CONVERGENCE:
+        raise NonConvergenceError(message)
     raise DeviceError(message)

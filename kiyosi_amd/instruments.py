@@ -11,7 +11,7 @@ import datetime as _dt
 import math
 
 from . import _abi
-from .errors import InvalidInputError, MissingHistoryError
+from .errors import InvalidInputError, KiyosiError, MissingHistoryError
 from .market import MICROS_PER_DAY, TradingCalendar, clock_micros, to_micros
 
 
@@ -396,9 +396,8 @@ _BARRIER_TYPES = {"down_and_out": _abi.DOWN_AND_OUT, "down_and_in": _abi.DOWN_AN
 
 class _Monitored:
     """Monitoring state shared by single and double barriers
-    (instruments/monitoring.rs:272-321, 430-445), for contracts without prior
-    touches and Black-Scholes contexts with positive volatility, where no
-    deterministic interval exists."""
+    (instruments/monitoring.rs:272-321, 430-445), for Black-Scholes contexts
+    with positive volatility, where no deterministic interval exists."""
 
     _upper = None
 
@@ -413,16 +412,28 @@ class _Monitored:
     def _levels(self):
         return [self.barrier_level] if self._upper is None else [self.barrier_level, self._upper]
 
+    def _prior(self, time):
+        """Monitoring::prior (monitoring.rs:277-292): whether an observation
+        before `time` touched, from the supplied history."""
+        times = self._times()
+        had = time > to_micros(self.effective_time) if times is None else times[0] < time
+        if not had and self.touch_state == "touched":
+            raise InvalidInputError("touch predates monitoring")
+        if had and self.touch_state is None:
+            raise MissingHistoryError("prior barrier touch state is required")
+        return self.touch_state == "touched"
+
     def _state(self, context):
         """(touched, finished) at the valuation (Monitoring::state)."""
         time, spot = to_micros(context.valuation_time), context.spot_price
+        prior = self._prior(time)
         if self._upper is None:
             hit = spot >= self.barrier_level if self.barrier_type.startswith("up") else spot <= self.barrier_level
         else:
             hit = spot <= self.barrier_level or spot >= self._upper
         times = self._times()
         finished = time == to_micros(self.expiry_time) or (times is not None and times[-1] <= time)
-        return self._monitored(time) and hit, finished
+        return prior or (self._monitored(time) and hit), finished
 
     def _knock_in(self):
         return self.barrier_type.endswith("_in")
@@ -441,7 +452,10 @@ class _Monitored:
     def volatility_affects_price(self, context):
         """OptionContract::volatility_affects_price (instruments/barrier.rs:283-292,
         double_barrier.rs:226-235)."""
-        touched, finished = self._state(context)
+        try:
+            touched, finished = self._state(context)
+        except KiyosiError:
+            return False
         knock_in = self._knock_in()
         return not ((touched and not knock_in) or (not touched and finished and knock_in)
                     or (not knock_in and self._payoff_requires_touch() and self.rebate == 0.0))
@@ -451,11 +465,15 @@ class _Monitored:
         expiry while the payoff is live (spot_discontinuity and
         live_payoff_discontinuity, monitoring.rs:322-340, 430-445)."""
         spot, time = context.spot_price, to_micros(context.valuation_time)
-        if self._monitored(time) and any(spot - radius <= level <= spot + radius for level in self._levels()):
+        if (self.touch_state != "touched" and self._monitored(time)
+                and any(spot - radius <= level <= spot + radius for level in self._levels())):
             return True
         vanilla = EuropeanOption(option_type=self.option_type, strike=self.strike,
                                  effective_time=self.effective_time, expiry_time=self.expiry_time)
-        touched, finished = self._state(context)
+        try:
+            touched, finished = self._state(context)
+        except KiyosiError:
+            return vanilla.at_spot_discontinuity(context, radius)
         if self._knock_in():
             return (touched or not finished) and vanilla.at_spot_discontinuity(context, radius)
         return not touched and vanilla.at_spot_discontinuity(context, radius)
@@ -466,6 +484,16 @@ class _Monitored:
             return None, 0
         arr = (C.c_int64 * max(len(times), 1))(*[to_micros(t) for t in times])
         return arr, len(times)
+
+
+# BarrierTouchState (instruments/barrier.rs:38-47); None is unknown history.
+_TOUCH_STATES = {None: _abi.TOUCH_UNKNOWN, "untouched": _abi.TOUCH_UNTOUCHED, "touched": _abi.TOUCH_TOUCHED}
+
+
+def _touch_state(value):
+    if value not in _TOUCH_STATES:
+        raise InvalidInputError("touch_state must be None, 'untouched' or 'touched'")
+    return value
 
 
 class BarrierOption(_Monitored):
@@ -481,8 +509,7 @@ class BarrierOption(_Monitored):
             raise InvalidInputError("unknown barrier_type")
         if rebate_timing not in ("at_hit", "at_expiry"):
             raise InvalidInputError("rebate_timing must be 'at_hit' or 'at_expiry'")
-        if touch_state == "touched":
-            raise InvalidInputError("touched barriers are resolved by the CPU engine")
+        self.touch_state = _touch_state(touch_state)
         self.option_type = option_type
         self.strike = _positive("strike", strike)
         self.barrier_level = _positive("barrier_level", barrier_level)
@@ -501,7 +528,8 @@ class BarrierOption(_Monitored):
             _abi.OBSERVE_CONTINUOUS if arr is None else _abi.OBSERVE_SCHEDULED,
             _abi.REBATE_AT_HIT if self.rebate_timing == "at_hit" else _abi.REBATE_AT_EXPIRY,
             self.strike, self.barrier_level, self.rebate, to_micros(self.effective_time),
-            to_micros(self.expiry_time), n, 0, C.cast(arr, C.POINTER(C.c_int64)) if arr is not None else None)
+            to_micros(self.expiry_time), n, _TOUCH_STATES[self.touch_state],
+            C.cast(arr, C.POINTER(C.c_int64)) if arr is not None else None)
         terms._keep = arr
         return terms
 
@@ -525,8 +553,7 @@ class DoubleBarrierOption(_Monitored):
             raise InvalidInputError("upper_barrier must exceed lower_barrier")
         if barrier_type == "knock_in" and rebate_timing == "at_hit":
             raise InvalidInputError("knock-in rebates are paid at expiry")
-        if touch_state == "touched":
-            raise InvalidInputError("touched barriers are resolved by the CPU engine")
+        self.touch_state = _touch_state(touch_state)
         self.option_type = option_type
         self.strike = _positive("strike", strike)
         self.barrier_level = lower
@@ -552,7 +579,8 @@ class DoubleBarrierOption(_Monitored):
             _abi.OBSERVE_CONTINUOUS if arr is None else _abi.OBSERVE_SCHEDULED,
             _abi.REBATE_AT_HIT if self.rebate_timing == "at_hit" else _abi.REBATE_AT_EXPIRY,
             self.strike, self.lower_barrier, self.upper_barrier, self.rebate, to_micros(self.effective_time),
-            to_micros(self.expiry_time), n, 0, C.cast(arr, C.POINTER(C.c_int64)) if arr is not None else None)
+            to_micros(self.expiry_time), n, _TOUCH_STATES[self.touch_state],
+            C.cast(arr, C.POINTER(C.c_int64)) if arr is not None else None)
         terms._keep = arr
         return terms
 

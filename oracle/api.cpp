@@ -218,6 +218,21 @@ int oracle_double_survival(double initial, double terminal, double band, double 
   return guarded([&] { double_survival(initial, terminal, band, deviation, derivatives != 0, out, branch); });
 }
 
+int oracle_single_hit_value(double amount, double distance, int up, double drift, double sigma, double time,
+                            double rate, double* out) {
+  return guarded([&] { *out = single_hit_value(amount, distance, up != 0, drift, sigma, time, rate); });
+}
+
+int oracle_double_hit_value(double amount, double lower, double upper, double spot, double rate, double dividend,
+                            double sigma, double time, double* out) {
+  return guarded([&] { *out = double_hit_value(amount, lower, upper, spot, rate, dividend, sigma, time); });
+}
+
+int oracle_double_exit_probability(double lower, double upper, double spot, double rate, double dividend,
+                                   double sigma, double time, double* out) {
+  return guarded([&] { *out = 1.0 - BandSurvival(lower, upper, spot, rate, dividend, sigma)(time); });
+}
+
 int oracle_asian_price(const kg_settings* s, const kg_asian_terms* o, const kg_bsm_market* m,
                        int workers, kg_estimate* out) {
   return guarded([&] {

@@ -6,6 +6,7 @@
 
 #include "../include/kiyosi_amd.h"
 #include "core.hpp"
+#include "hit.hpp"
 #include "plans.hpp"
 
 namespace oracle {
@@ -229,17 +230,17 @@ struct BarrierSpec {
   bool has_upper;
   double upper;
   int64_t effective_time, expiry_time;
-  int n_observations;
+  int n_observations, touch_state;
   const int64_t* observation_times;
 };
 inline BarrierSpec barrier_spec(const kg_barrier_terms& o) {
   return {o.option_type, o.barrier_type, o.observation, o.rebate_timing, o.strike, o.barrier, o.rebate, false,
-          0.0, o.effective_time, o.expiry_time, o.n_observations, o.observation_times};
+          0.0, o.effective_time, o.expiry_time, o.n_observations, o.touch_state, o.observation_times};
 }
 inline BarrierSpec barrier_spec(const kg_double_barrier_terms& o) {
   return {o.option_type, o.knock_type == KG_DOUBLE_KNOCK_IN ? KG_DOWN_AND_IN : KG_DOWN_AND_OUT, o.observation,
           o.rebate_timing, o.strike, o.lower_barrier, o.rebate, true, o.upper_barrier, o.effective_time,
-          o.expiry_time, o.n_observations, o.observation_times};
+          o.expiry_time, o.n_observations, o.touch_state, o.observation_times};
 }
 
 // Delta and gamma of the Black-Scholes vanilla (AnalyticEuropeanEngine's
@@ -260,11 +261,12 @@ struct SpotGreeks {
   double delta = 0.0, gamma = 0.0;
 };
 
-// MonteCarloBarrierEngine::native (barrier.rs:160-541) for flat BSM rates and
-// no prior touch. `request` holds KG_GREEK_* bits; `radius` is the symmetric
-// spot shift whose interval, when it crosses a monitored level, makes the
-// spot Greeks unavailable (at_spot_discontinuity, instruments/monitoring.rs
-// :430-445).
+// MonteCarloBarrierEngine::native (barrier.rs:160-541) for flat BSM rates.
+// `request` holds KG_GREEK_* bits; `radius` is the symmetric spot shift
+// whose interval, when it crosses a monitored level, makes the spot Greeks
+// unavailable (at_spot_discontinuity, instruments/monitoring.rs:430-445),
+// and over which the at-hit rebate value of a continuous knock-out is
+// differenced (complete_greeks, risk/numerical.rs:359-430).
 inline Estimate barrier_native(const Settings& st, const BarrierSpec& o, const kg_bsm_market& m, const Context& cx,
                                int request, double radius, SpotGreeks* greeks) {
   const int64_t val = m.valuation_time, expiry = o.expiry_time;
@@ -278,11 +280,16 @@ inline Estimate barrier_native(const Settings& st, const BarrierSpec& o, const k
   const double sign = o.option_type;
   const double r = m.rate, q = m.dividend, sigma = m.volatility, S = m.spot;
   const double T = years_until(val, expiry);
-  // Monitoring state (instruments/monitoring.rs:293-321) for a contract
-  // without earlier touches.
+  // Monitoring state (instruments/monitoring.rs:272-321): history before
+  // valuation, then a touch at the valuation.
+  const bool had = continuous ? val > o.effective_time : (o.n_observations > 0 && o.observation_times[0] < val);
+  if (!had && o.touch_state == KG_TOUCH_TOUCHED) throw Failure(INVALID_INPUT, "touch predates monitoring");
+  if (had && o.touch_state == KG_TOUCH_UNKNOWN) throw Failure(MISSING_HISTORY, "prior touch state is required");
+  const bool prior = o.touch_state == KG_TOUCH_TOUCHED;
   bool observed_now = continuous;
   for (int i = 0; i < o.n_observations && !continuous; ++i) observed_now |= o.observation_times[i] == val;
-  const bool touched = observed_now && ((up ? S >= o.barrier : S <= o.barrier) || (o.has_upper && S >= o.upper));
+  const bool touched =
+      prior || (observed_now && ((up ? S >= o.barrier : S <= o.barrier) || (o.has_upper && S >= o.upper)));
   const bool finished = val == expiry || (!continuous && o.n_observations > 0 &&
                                           o.observation_times[o.n_observations - 1] <= val);
   const bool last_is_expiry =
@@ -295,9 +302,8 @@ inline Estimate barrier_native(const Settings& st, const BarrierSpec& o, const k
   // The request keeps delta and gamma off expiry and off a spot stencil
   // that crosses a monitored level (barrier.rs:194-205).
   auto crosses = [&](double level) { return S - radius <= level && level <= S + radius; };
-  const bool discontinuity = observed_now && (crosses(o.barrier) || (o.has_upper && crosses(o.upper)));
+  const bool discontinuity = !prior && observed_now && (crosses(o.barrier) || (o.has_upper && crosses(o.upper)));
   int req = (val != expiry && !discontinuity) ? request & (KG_GREEK_DELTA | KG_GREEK_GAMMA) : 0;
-  const bool delta = req & KG_GREEK_DELTA, gamma = req & KG_GREEK_GAMMA;
   if (greeks) *greeks = SpotGreeks{};
   auto cash = [&] { return o.rebate == 0.0 ? 0.0 : o.rebate * std::exp(-r * T); };
   auto vanilla = [&] {
@@ -313,21 +319,39 @@ inline Estimate barrier_native(const Settings& st, const BarrierSpec& o, const k
       if (vanilla_payoff) {  // volatility_affects_price: the vanilla's analytic delta/gamma
         double d, g;
         vanilla_spot_greeks(S, o.strike, r, q, sigma, T, sign, &d, &g);
-        greeks->delta = delta ? d : 0.0;
-        greeks->gamma = gamma ? g : 0.0;
+        greeks->delta = (req & KG_GREEK_DELTA) ? d : 0.0;
+        greeks->gamma = (req & KG_GREEK_GAMMA) ? g : 0.0;
       }
     }
     return known_estimate(st, value);
   };
-  if (touched && !knock_in) return resolved(at_hit ? o.rebate : cash(), false);
+  if (touched && !knock_in) return resolved(at_hit ? (prior ? 0.0 : o.rebate) : cash(), false);
   if (!touched && finished && knock_in) return resolved(cash(), false);
   const bool fixed = o.rebate == 0.0 && requires_touch;
   if (fixed && !knock_in) return resolved(0.0, false);
   if (touched || finished || fixed) return resolved(vanilla(), true);
-  if (continuous && !knock_in && at_hit && o.rebate != 0.0)
-    throw Failure(UNSUPPORTED, "continuous at-hit rebates need the analytic hit value");
   // Plan (barrier.rs:228-290).
   const double carry = r - q, hvr = 0.5 * sigma * sigma, drift = carry - hvr;
+  // A continuous knock-out paid at hit: its paths carry no rebate and the
+  // hit value is added, with central spot differences for the Greeks.
+  double hit = 0.0, hit_delta = 0.0, hit_gamma = 0.0;
+  if (continuous && !knock_in && at_hit && o.rebate != 0.0) {
+    auto hit_at = [&](double spot) {
+      if (o.has_upper) return double_hit_value(o.rebate, o.barrier, o.upper, spot, r, q, sigma, T);
+      return single_hit_value(o.rebate, std::fabs(std::log(o.barrier / spot)), up, drift, sigma, T, r);
+    };
+    hit = hit_at(S);
+    if (req) {
+      if (radius > 0.0 && S > radius) {
+        const double hi = hit_at(S + radius), lo = hit_at(S - radius);
+        hit_delta = (hi - lo) / radius / 2.0;
+        hit_gamma = ((hi - hit) - (hit - lo)) / radius / radius;
+      } else {
+        req = 0;
+      }
+    }
+  }
+  const bool delta = req & KG_GREEK_DELTA, gamma = req & KG_GREEK_GAMMA;
   auto mean_distance = [&](int64_t date, double level) {
     double e = years_until(val, date);
     return log_ratio(S, level) + carry * e - hvr * e;
@@ -458,10 +482,10 @@ inline Estimate barrier_native(const Settings& st, const BarrierSpec& o, const k
   Means mm = path_mean(st, total[0], pairs[0]);
   if (greeks) {
     greeks->available = req;
-    greeks->delta = delta ? scaled_value(total[1].scale, total[1].mean, 0.0) : 0.0;
-    greeks->gamma = gamma ? scaled_value(total[2].scale, total[2].mean, 0.0) : 0.0;
+    greeks->delta = delta ? scaled_value(total[1].scale, total[1].mean, 0.0) + hit_delta : 0.0;
+    greeks->gamma = gamma ? scaled_value(total[2].scale, total[2].mean, 0.0) + hit_gamma : 0.0;
   }
-  return from_sampling(st, mm.mean, mm.has_se, mm.se);
+  return from_sampling(st, mm.mean + hit, mm.has_se, mm.se);
 }
 
 inline Estimate barrier_price(const Settings& st, const kg_barrier_terms& o, const kg_bsm_market& m,

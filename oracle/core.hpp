@@ -28,7 +28,8 @@ struct Failure : std::runtime_error {
   int code;
   Failure(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
-constexpr int INVALID_INPUT = 1, UNSUPPORTED = 2, COMPUTATION_FAILURE = 3, MISSING_HISTORY = 6;
+constexpr int INVALID_INPUT = 1, UNSUPPORTED = 2, COMPUTATION_FAILURE = 3, MISSING_HISTORY = 6,
+              NON_CONVERGENCE = 7;
 // scaled_cash (numerics.rs:175-186)
 inline double scaled_cash(double amount, double exponent) {
   if (amount == 0.0) return 0.0;

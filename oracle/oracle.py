@@ -190,6 +190,31 @@ def double_survival(initial, terminal, band, deviation, derivatives=True, branch
     return list(out)
 
 
+def single_hit_value(amount, distance, up, drift, sigma, time, rate):
+    """E[amount e^{-r tau} 1{tau <= T}] for a barrier at log distance
+    `distance` (up or down) from the spot, with log drift `drift`."""
+    out = C.c_double()
+    _check(lib().oracle_single_hit_value(*[C.c_double(amount), C.c_double(distance)], int(up),
+                                         *[C.c_double(v) for v in (drift, sigma, time, rate)], C.byref(out)))
+    return out.value
+
+
+def double_hit_value(amount, lower, upper, spot, rate, dividend, sigma, time):
+    """The same for the first exit from (lower, upper)."""
+    out = C.c_double()
+    _check(lib().oracle_double_hit_value(*[C.c_double(v) for v in (amount, lower, upper, spot, rate, dividend, sigma,
+                                                                   time)], C.byref(out)))
+    return out.value
+
+
+def double_exit_probability(lower, upper, spot, rate, dividend, sigma, time):
+    """Probability that the spot leaves (lower, upper) before `time`."""
+    out = C.c_double()
+    _check(lib().oracle_double_exit_probability(*[C.c_double(v) for v in (lower, upper, spot, rate, dividend, sigma,
+                                                                          time)], C.byref(out)))
+    return out.value
+
+
 def asian_price(settings, terms, market, workers: int = 1):
     out = _abi.Estimate()
     _check(lib().oracle_asian_price(C.byref(settings), C.byref(terms), C.byref(market), workers, C.byref(out)))

@@ -43,6 +43,12 @@ def test_book_kernels_keep_two_waves_per_simd(tmp_path):
         assert vgpr <= 256, (name, vgpr)
         if "book_kernelILb1E" in name:  # FP32
             assert private == 0, (name, private)
+    # The FP32 spot-Greek kernels keep four waves per SIMD; their scratch is
+    # the call frame of the arithmetic Asian's strike terms, not spills.
+    greeks = {k: v for k, v in res.items() if "spot_greek_kernelILb1E" in k}
+    assert len(greeks) == 2
+    for name, (vgpr, private) in greeks.items():
+        assert vgpr <= 128 and private <= 128, (name, vgpr, private)
 
 
 @pytest.mark.skipif(not pathlib.Path(HIPCC).exists(), reason="hipcc is not installed")
