@@ -608,8 +608,8 @@ typedef struct kg_lsmc_options {
   int32_t out_of_sample;       /* 0: in-sample price on the training paths */
   int64_t training_path_count; /* training paths for out-of-sample pricing */
   int32_t epochs;              /* MLP: passes over each date's samples (default 4) */
-  int32_t batch;               /* MLP: minibatch size (default 16384) */
-  double learning_rate;        /* MLP: Adam step (default 1e-3) */
+  int32_t batch;               /* MLP: minibatch size (default 65536) */
+  double learning_rate;        /* MLP: Adam step (default 2e-3) */
 } kg_lsmc_options;
 
 typedef struct kg_lsmc_report {

@@ -32,11 +32,13 @@
 //   * the 16x16 output tiles of Z2^T hold exactly the units u(mt, i), so
 //     they are the B operand of dH1^T = W2^T . dZ2^T, whose output tiles
 //     again hold u(mt, i) for the GELU derivative of layer 1;
-//   * dW2 = dZ2^T . H1 contracts over samples and reads both operands from
-//     a 32-sample LDS transpose.
+//   * dW2 = dZ2^T . H1 contracts over samples: two waves stage their
+//     tiles' transposes of dZ2 and H1 as one 32-sample slot in LDS, and
+//     each wave forms the dW2 rows of its own 16 output units over all the
+//     workgroup's slots.
 // Gradient partials are summed over the waves of a workgroup in wave order
-// and over workgroups in workgroup order by the update kernel, so training
-// is deterministic.
+// and over workgroups in a fixed slice order by the update kernel, so
+// training is deterministic.
 #include "lsmc_device.hpp"
 
 namespace kg {
@@ -55,7 +57,7 @@ constexpr int PK_W1 = 0, PK_B1 = 192, PK_B2 = 256, PK_W3 = 320, PK_B3 = 384;  //
 constexpr int PACK_BYTES = ((PK_FLOATS + 385 * 4) + 15) & ~15;
 constexpr int PACK_WORDS = PACK_BYTES / 16;
 constexpr int T_ROW = 40;  // padded row of the 32-sample transposes (bf16)
-constexpr int PPW = 1;     // 32-sample pairs per wave in a gradient launch
+constexpr int GRAD_WAVES = 1024;  // waves of a gradient launch (one per SIMD) before tiles per wave grow
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -74,7 +76,7 @@ __device__ inline float gelu(float x, float* derivative) {
   const float k0 = 0.7978845608028654f, k1 = 0.044715f;
   const float x2 = x * x;
   const float a = k0 * fmaf(k1 * x2, x, x);
-  const float t = 1.0f - 2.0f / (f32::exp(2.0f * a) + 1.0f);
+  const float t = 1.0f - 2.0f * __builtin_amdgcn_rcpf(f32::exp(2.0f * a) + 1.0f);
   *derivative = fmaf(0.5f * x * (1.0f - t * t), k0 * fmaf(3.0f * k1, x2, 1.0f), 0.5f * (1.0f + t));
   return 0.5f * x * (1.0f + t);
 }
@@ -152,8 +154,11 @@ __device__ inline void forward(const Pack& P, float u0, float u1, float u2, Fwd&
   f.finite = isfinite(f.out);
 }
 
+// Per-lane gradient accumulators of a wave. dW2 is held only for the
+// wave's own 16 output units, summed over the samples of all the waves of
+// the workgroup.
 struct Acc {
-  f32x4 w2[4][4];  // dW2 [16 mt + 4 g + i][16 nt + c]
+  f32x4 w2[4];  // dW2 [16 wave + 4 g + i][16 nt + c]
   float w1[16][3], b1[16], b2[16], w3[16];
   float b3, loss;
 };
@@ -202,62 +207,81 @@ __device__ inline void backward(const Pack& P, const Fwd& f, float gs, float u0,
   }
 }
 
+// Lane permutation within rows of 16 lanes by a DPP control word.
+template <int CTRL>
+__device__ inline float dpp(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+// Sum over the 16 columns c of a lane group, left in every lane of the row:
+// quad swaps (1 0 3 2), (2 3 0 1), then the half-row and row mirrors.
 __device__ inline float sum_columns(float v) {
-  for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, WAVE);
+  v += dpp<0xB1>(v);
+  v += dpp<0x4E>(v);
+  v += dpp<0x141>(v);
+  v += dpp<0x140>(v);
   return v;
 }
 
+// Small gradients of a partial: W1, b1, b2, W3 (64 x 6), then b3 and the loss.
+constexpr int SMALL = 6 * HID + 2;
+
+constexpr int GRAD_WG_WAVES = 4;  // waves of a gradient workgroup
+
 // Gradient partials of the batch samples[begin, begin + n): one partial of
 // P_STRIDE floats per workgroup (gradient of sum((out - y)^2) * inv_n, loss
-// in slot P_LOSS). Every wave runs PPW pairs so the block barriers are
-// uniform.
-__global__ void __launch_bounds__(256)
+// in slot P_LOSS). Each wave runs `tpw` consecutive 16-sample tiles, one per
+// round. In a round the waves 2 s and 2 s + 1 stage their tiles' dZ2 and H1
+// transposes as the two halves of the 32-sample slot s, and wave w then
+// forms the dW2 rows of its own 16 output units over both slots, so dW2
+// needs no cross-wave reduction. The small gradients are summed over the
+// waves in wave order. Every wave runs every round so the block barriers
+// are uniform.
+__global__ void __launch_bounds__(GRAD_WG_WAVES * WAVE)
     mlp_grad(const uint4* __restrict__ packed, const float4* __restrict__ samples, int64_t begin, int n,
-             float inv_n, float* __restrict__ parts, uint32_t* __restrict__ flags) {
+             int tpw, float inv_n, float* __restrict__ parts, uint32_t* __restrict__ flags) {
   __shared__ uint4 s_pack[PACK_WORDS];
-  __shared__ __attribute__((aligned(16))) uint16_t s_dz[4][HID * T_ROW];
-  __shared__ __attribute__((aligned(16))) uint16_t s_hs[4][HID * T_ROW];
+  __shared__ __attribute__((aligned(16))) uint16_t s_dz[GRAD_WG_WAVES / 2][HID * T_ROW];
+  __shared__ __attribute__((aligned(16))) uint16_t s_hs[GRAD_WG_WAVES / 2][HID * T_ROW];
+  __shared__ float s_small[GRAD_WG_WAVES][SMALL];
   stage(s_pack, packed);
   __syncthreads();
   const Pack P(s_pack);
   const int wave = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1), g = lane >> 4, c = lane & 15;
-  uint16_t* dz = s_dz[wave];
-  uint16_t* hs = s_hs[wave];
   Acc a;
-  for (int mt = 0; mt < 4; ++mt)
-    for (int nt = 0; nt < 4; ++nt) a.w2[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int nt = 0; nt < 4; ++nt) a.w2[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   for (int r = 0; r < 16; ++r) {
     a.w1[r][0] = a.w1[r][1] = a.w1[r][2] = 0.0f;
     a.b1[r] = a.b2[r] = a.w3[r] = 0.0f;
   }
   a.b3 = a.loss = 0.0f;
   bool finite = true;
-  for (int j = 0; j < PPW; ++j) {
-    const int64_t pair = ((int64_t)blockIdx.x * 4 + wave) * PPW + j;
-    for (int tp = 0; tp < 2; ++tp) {
-      const int64_t local = pair * 32 + tp * 16 + c;
-      const bool valid = local < n;
-      const float4 x = valid ? samples[begin + local] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      Fwd f;
-      forward(P, x.x, x.y, x.z, f);
-      finite = finite && f.finite;
-      const float r = valid ? f.out - x.w : 0.0f;
-      const float gs = 2.0f * r * inv_n;
-      a.loss = fmaf(r * inv_n, r, a.loss);
-      a.b3 += gs;
-      backward(P, f, gs, x.x, x.y, x.z, a, dz, hs, tp);
-    }
+  for (int j = 0; j < tpw; ++j) {
+    const int64_t tile = ((int64_t)blockIdx.x * GRAD_WG_WAVES + wave) * tpw + j;
+    const int64_t local = tile * 16 + c;
+    const bool valid = local < n;
+    const float4 x = valid ? samples[begin + local] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    Fwd f;
+    forward(P, x.x, x.y, x.z, f);
+    finite = finite && f.finite;
+    const float r = valid ? f.out - x.w : 0.0f;
+    const float gs = 2.0f * r * inv_n;
+    a.loss = fmaf(r * inv_n, r, a.loss);
+    a.b3 += gs;
+    backward(P, f, gs, x.x, x.y, x.z, a, s_dz[wave >> 1], s_hs[wave >> 1], wave & 1);
     __syncthreads();
-    for (int mt = 0; mt < 4; ++mt) {
-      const uint4 av = *(const uint4*)(dz + (16 * mt + c) * T_ROW + 8 * g);
+    for (int slot = 0; slot < GRAD_WG_WAVES / 2; ++slot) {
+      const uint4 av = *(const uint4*)(s_dz[slot] + (16 * wave + c) * T_ROW + 8 * g);
       for (int nt = 0; nt < 4; ++nt) {
-        const uint4 bv = *(const uint4*)(hs + (16 * nt + c) * T_ROW + 8 * g);
-        a.w2[mt][nt] = mfma(u32x4{av.x, av.y, av.z, av.w}, u32x4{bv.x, bv.y, bv.z, bv.w}, a.w2[mt][nt]);
+        const uint4 bv = *(const uint4*)(s_hs[slot] + (16 * nt + c) * T_ROW + 8 * g);
+        a.w2[nt] = mfma(u32x4{av.x, av.y, av.z, av.w}, u32x4{bv.x, bv.y, bv.z, bv.w}, a.w2[nt]);
       }
     }
     __syncthreads();
   }
   if (!finite) raise_flag(flags, FLAG_NONFINITE);
+  float* out = parts + (int64_t)blockIdx.x * P_STRIDE;
+  for (int nt = 0; nt < 4; ++nt)
+    for (int i = 0; i < 4; ++i) out[P_W2 + (16 * wave + 4 * g + i) * HID + 16 * nt + c] = a.w2[nt][i];
   for (int r = 0; r < 16; ++r) {
     for (int k = 0; k < 3; ++k) a.w1[r][k] = sum_columns(a.w1[r][k]);
     a.b1[r] = sum_columns(a.b1[r]);
@@ -266,33 +290,36 @@ __global__ void __launch_bounds__(256)
   }
   a.b3 = sum_columns(a.b3);
   a.loss = sum_columns(a.loss);
-  // The four waves' gradients are summed in LDS in wave order (the dZ2
-  // transpose is free again), then written as the workgroup's partial.
-  float* red = reinterpret_cast<float*>(&s_dz[0][0]);
-  static_assert(sizeof(s_dz) >= (P_COUNT + 1) * sizeof(float), "reduction buffer");
-  for (int w = 0; w < 4; ++w) {
-    if (wave == w) {
-      auto put = [&](int k, float v) { red[k] = w == 0 ? v : red[k] + v; };
-      for (int mt = 0; mt < 4; ++mt)
-        for (int nt = 0; nt < 4; ++nt)
-          for (int i = 0; i < 4; ++i) put(P_W2 + (16 * mt + 4 * g + i) * HID + 16 * nt + c, a.w2[mt][nt][i]);
-      if (c == 0)
-        for (int r = 0; r < 16; ++r) {
-          const int u = 16 * (r >> 2) + 4 * g + (r & 3);
-          for (int k = 0; k < 3; ++k) put(P_W1 + 3 * u + k, a.w1[r][k]);
-          put(P_B1 + u, a.b1[r]);
-          put(P_B2 + u, a.b2[r]);
-          put(P_W3 + u, a.w3[r]);
-        }
-      if (lane == 0) {
-        put(P_B3, a.b3);
-        put(P_LOSS, a.loss);
-      }
+  float* mine = s_small[wave];
+  if (c == 0)
+    for (int r = 0; r < 16; ++r) {
+      const int u = 16 * (r >> 2) + 4 * g + (r & 3);
+      for (int k = 0; k < 3; ++k) mine[3 * u + k] = a.w1[r][k];
+      mine[3 * HID + u] = a.b1[r];
+      mine[4 * HID + u] = a.b2[r];
+      mine[5 * HID + u] = a.w3[r];
     }
-    __syncthreads();
+  if (lane == 0) {
+    mine[6 * HID] = a.b3;
+    mine[6 * HID + 1] = a.loss;
   }
-  float* out = parts + (int64_t)blockIdx.x * P_STRIDE;
-  for (int k = threadIdx.x; k <= P_COUNT; k += blockDim.x) out[k] = red[k];
+  __syncthreads();
+  for (int k = threadIdx.x; k < SMALL; k += blockDim.x) {
+    float v = s_small[0][k];
+    for (int w = 1; w < GRAD_WG_WAVES; ++w) v += s_small[w][k];
+    int slot;
+    if (k < 3 * HID)
+      slot = P_W1 + k;
+    else if (k < 4 * HID)
+      slot = P_B1 + k - 3 * HID;
+    else if (k < 5 * HID)
+      slot = P_B2 + k - 4 * HID;
+    else if (k < 6 * HID)
+      slot = P_W3 + k - 5 * HID;
+    else
+      slot = k == 6 * HID ? P_B3 : P_LOSS;
+    out[slot] = v;
+  }
 }
 
 enum : int { UPDATE_ADAM = 0, UPDATE_GRADIENT = 1, UPDATE_PACK = 2, UPDATE_ADOPT = 3 };
@@ -303,17 +330,28 @@ enum : int { UPDATE_ADAM = 0, UPDATE_GRADIENT = 1, UPDATE_PACK = 2, UPDATE_ADOPT
 // parameters by their running average and packs them. With avg_k > 0 an
 // Adam step also folds the new weights into the running average of the
 // last avg_k iterates. Slot P_LOSS adds the batch loss to *loss.
+constexpr int UPDATE_PARAMS = 32;                  // parameters per update workgroup
+constexpr int UPDATE_SLICES = 256 / UPDATE_PARAMS;  // slices of the partials summed in parallel
+
 __global__ void __launch_bounds__(256)
     mlp_update(int mode, const float* __restrict__ parts, int n_parts, float* __restrict__ params,
                float* __restrict__ m1, float* __restrict__ m2, float* __restrict__ avg, int avg_k,
                uint4* __restrict__ packed, float lr, float c1, float c2, float* __restrict__ gradient,
                float* __restrict__ loss) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p > P_COUNT) return;
+  __shared__ double s_sum[UPDATE_SLICES][UPDATE_PARAMS];
+  const int slice = threadIdx.x / UPDATE_PARAMS, j = threadIdx.x % UPDATE_PARAMS;
+  const int p = blockIdx.x * UPDATE_PARAMS + j;
+  // Slice s sums partials s, s + UPDATE_SLICES, ... in order; the slices
+  // are then added in slice order, so the sum is the same on every run.
+  double part = 0.0;
+  if ((mode == UPDATE_ADAM || mode == UPDATE_GRADIENT) && p <= P_COUNT)
+#pragma unroll 4
+    for (int w = slice; w < n_parts; w += UPDATE_SLICES) part += (double)parts[(int64_t)w * P_STRIDE + p];
+  s_sum[slice][j] = part;
+  __syncthreads();
+  if (slice != 0 || p > P_COUNT) return;
   double sum = 0.0;
-  if (mode == UPDATE_ADAM || mode == UPDATE_GRADIENT)
-#pragma unroll 8
-    for (int w = 0; w < n_parts; ++w) sum += (double)parts[(int64_t)w * P_STRIDE + p];
+  for (int k = 0; k < UPDATE_SLICES; ++k) sum += s_sum[k][j];
   if (p == P_LOSS) {
     if (loss) *loss += (float)sum;
     return;
@@ -635,11 +673,17 @@ struct Net {
   int max_parts;
 };
 
-// Partials (workgroups of four waves) of a gradient launch over n samples.
+// 16-sample tiles per wave of a gradient launch over n samples: one until
+// the batch gives every SIMD a wave, then as many as keep it so.
+static int tiles_per_wave(int n) {
+  const int tiles = (n + 15) / 16;
+  return std::max(1, (tiles + GRAD_WAVES - 1) / GRAD_WAVES);
+}
+// Partials (workgroups of GRAD_WG_WAVES waves) of a gradient launch over n samples.
 static int parts_for(int n) {
-  const int pairs = (n + 31) / 32;
-  const int waves = (pairs + PPW - 1) / PPW;
-  return (waves + 3) / 4;
+  const int tiles = (n + 15) / 16, tpw = tiles_per_wave(n);
+  const int waves = (tiles + tpw - 1) / tpw;
+  return (waves + GRAD_WG_WAVES - 1) / GRAD_WG_WAVES;
 }
 
 static Net net_buffers(kg_context* ctx, DeviceBuffer& buf, int max_batch, size_t extra_bytes, void** extra) {
@@ -665,7 +709,8 @@ static void launch_update(kg_context* ctx, const Net& net, int mode, int n_parts
                           float* gradient, float* loss, int avg_k = 0) {
   const float c1 = step > 0 ? (float)(1.0 / (1.0 - std::pow(0.9, step))) : 1.0f;
   const float c2 = step > 0 ? (float)(1.0 / (1.0 - std::pow(0.999, step))) : 1.0f;
-  hipLaunchKernelGGL(mlp_update, dim3((P_COUNT + 1 + 255) / 256), dim3(256), 0, ctx->stream, mode, net.parts,
+  hipLaunchKernelGGL(mlp_update, dim3((P_COUNT + UPDATE_PARAMS) / UPDATE_PARAMS), dim3(256), 0, ctx->stream, mode,
+                     net.parts,
                      n_parts, net.params, net.m1, net.m2, net.avg, avg_k, net.packed, lr, c1, c2, gradient, loss);
   KG_HIP(hipGetLastError());
 }
@@ -681,8 +726,8 @@ static int launch_grad(kg_context* ctx, const Net& net, const float4* samples, i
                        uint32_t* flags) {
   const int parts = parts_for(n);
   if (parts > net.max_parts) fail(KG_INVALID_INPUT, "batch larger than the reserved partials");
-  hipLaunchKernelGGL(mlp_grad, dim3(parts), dim3(256), 0, ctx->stream, net.packed, samples, begin, n,
-                     1.0f / (float)n, net.parts, flags);
+  hipLaunchKernelGGL(mlp_grad, dim3(parts), dim3(GRAD_WG_WAVES * WAVE), 0, ctx->stream, net.packed, samples, begin, n,
+                     tiles_per_wave(n), 1.0f / (float)n, net.parts, flags);
   KG_HIP(hipGetLastError());
   return parts;
 }
@@ -717,8 +762,12 @@ static void run(kg_context* ctx, LsmcHost& h, const kg_lsmc_options& opt, kg_lsm
   const LsmcModel& m = h.model;
   const Sampler& tr = h.oos ? h.training : h.pricing;
   const int epochs = opt.epochs > 0 ? opt.epochs : 4;
-  const int batch = opt.batch > 0 ? opt.batch : 16384;
-  const float lr = opt.learning_rate > 0.0 ? (float)opt.learning_rate : 1e-3f;
+  // Large batches keep every SIMD busy on each Adam step and need a quarter
+  // of the sequential steps of 16384; the doubled rate offsets the fewer
+  // steps (on the 2^20-path local-volatility put the out-of-sample price
+  // moves within its paired error).
+  const int batch = opt.batch > 0 ? opt.batch : 65536;
+  const float lr = opt.learning_rate > 0.0 ? (float)opt.learning_rate : 2e-3f;
   if (epochs > 1000 || batch < 32 || batch > (1 << 20) || !(lr < 1.0f))
     fail(KG_INVALID_INPUT, "MLP options: epochs <= 1000, batch in 32..=1048576, learning rate below 1");
   const int S = m.steps;
@@ -757,11 +806,14 @@ static void run(kg_context* ctx, LsmcHost& h, const kg_lsmc_options& opt, kg_lsm
   int32_t* counts = (int32_t*)(base + coord_bytes + 2 * slot_bytes + sample_bytes + idx_bytes);
   int32_t* offsets = (int32_t*)(base + coord_bytes + 2 * slot_bytes + sample_bytes + idx_bytes + count_bytes);
 
-  const size_t nets_bytes = (size_t)S * PACK_BYTES + 256 + ((size_t)S * 4 + 255) / 256 * 256;
+  const size_t flag_bytes = ((size_t)S * 4 + 255) / 256 * 256;
+  const size_t nets_bytes = (size_t)S * PACK_BYTES + 256 + 2 * flag_bytes;
   void* extra = nullptr;
   const Net net = net_buffers(ctx, ctx->mlp_state, batch, nets_bytes, &extra);
   uint4* nets = (uint4*)extra;
   int32_t* d_fitted = (int32_t*)((unsigned char*)extra + (size_t)S * PACK_BYTES + 256);
+  // Per-date sum of the last epoch's batch losses, read once after training.
+  float* d_losses = (float*)((unsigned char*)d_fitted + flag_bytes);
 
   const std::vector<float> init = initial_params(tr.seed);
   KG_HIP(hipMemcpyAsync(net.params, init.data(), P_COUNT * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
@@ -773,9 +825,8 @@ static void run(kg_context* ctx, LsmcHost& h, const kg_lsmc_options& opt, kg_lsm
                      ctx->stream, m, tr.seed, samples, tr.path_count, (int)tr.antithetic, SobolRef{nullptr, nullptr, 0},
                      coords, spot, cash, flags);
   KG_HIP(hipGetLastError());
-  std::vector<int32_t> fitted(S, 0);
-  double loss_sum = 0.0;
-  int loss_dates = 0;
+  clear(ctx, d_losses, S);
+  std::vector<int32_t> fitted(S, 0), steps_last(S, 0);
   bool first = true;
   int adam_step = 0;
   for (int step = S - 2; step >= 0; --step) {
@@ -792,24 +843,20 @@ static void run(kg_context* ctx, LsmcHost& h, const kg_lsmc_options& opt, kg_lsm
     KG_HIP(hipMemcpyAsync(&n, offsets + blocks, sizeof(n), hipMemcpyDeviceToHost, ctx->stream));
     KG_HIP(hipStreamSynchronize(ctx->stream));
     if (n == 0) continue;
-    clear(ctx, net.scalars, 1);
     // The first fitted date starts cold with four times the epochs; later
     // dates continue from the previous network and its Adam moments.
-    const int steps_last = train(ctx, net, data, n, batch, first ? 4 * epochs : epochs, lr, first, true,
-                                 adam_step, flags, net.scalars);
+    steps_last[step] = train(ctx, net, data, n, batch, first ? 4 * epochs : epochs, lr, first, true, adam_step,
+                             flags, d_losses + step);
     first = false;
     hipLaunchKernelGGL(mlp_eval, dim3((n + 63) / 64), dim3(256), 0, ctx->stream, net.packed, data, n, nullptr, m,
                        idx, spot, cash, flags);
     KG_HIP(hipGetLastError());
     KG_HIP(hipMemcpyAsync(nets + (size_t)step * PACK_WORDS, net.packed, PACK_BYTES, hipMemcpyDeviceToDevice,
                           ctx->stream));
-    float loss = 0.0f;
-    KG_HIP(hipMemcpyAsync(&loss, net.scalars, sizeof(loss), hipMemcpyDeviceToHost, ctx->stream));
-    KG_HIP(hipStreamSynchronize(ctx->stream));
-    loss_sum += loss / steps_last;
-    ++loss_dates;
     fitted[step] = 1;
   }
+  std::vector<float> losses(S, 0.0f);
+  KG_HIP(hipMemcpyAsync(losses.data(), d_losses, S * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 2 * blocks);
   hipLaunchKernelGGL((lsmc_final<KG_BASIS_QUADRATIC>), dim3(blocks), dim3(256), 0, ctx->stream, m, nullptr, 0,
                      samples, tr.path_count, (int)tr.antithetic, spot, cash, parts);
@@ -821,7 +868,10 @@ static void run(kg_context* ctx, LsmcHost& h, const kg_lsmc_options& opt, kg_lsm
   int32_t dates = 0;
   for (int k = 0; k < S; ++k) dates += fitted[k];
   out->dates_fitted = dates;
-  out->mean_loss = loss_dates ? loss_sum / loss_dates : 0.0;
+  double loss_sum = 0.0;
+  for (int k = 0; k < S; ++k)
+    if (fitted[k]) loss_sum += losses[k] / steps_last[k];
+  out->mean_loss = dates ? loss_sum / dates : 0.0;
   out->in_sample_price = std::max(scaled_value(r[0].scale, r[0].mean, m.log_discount), h.immediate);
   out->price = out->in_sample_price;
   out->path_steps = (double)tr.path_count * S;

@@ -166,7 +166,7 @@ class MonteCarloVanillaEngine(_MonteCarloEngine):
         return _batched_greeks(engine, option, context, request, settings, lambda scenarios: [scenarios], batch)
 
     def american_report(self, option, context, *, training_path_count=None, basis="quadratic",
-                        epochs=4, batch=16384, learning_rate=1e-3):
+                        epochs=4, batch=65536, learning_rate=2e-3):
         """Least-squares Monte Carlo on the exercise grid of step_count intervals:
         in-sample (kiyosi's price for AmericanOption) or, with training_path_count,
         fitted on separate training paths and priced out of sample. basis is

@@ -203,3 +203,31 @@ def test_config5_mlp_matches_poly4_beats_quadratic_and_meets_the_fd_judge(gpu, o
     assert mlp.price > quad.price
     assert mlp.price <= fd + 2 * se
     assert fd - mlp.price <= se + 0.01
+
+
+@pytest.mark.gpu
+def test_config5_mlp_against_poly4_with_a_paired_error(gpu):
+    """MLP minus poly-4 out-of-sample prices over six seeds: each seed prices
+    both rules on the same fresh pricing paths, so the seed-to-seed spread
+    of the difference is its paired (common-random-number) error, far below
+    either price's own error. The MLP must not be significantly below
+    poly-4."""
+    import math
+
+    import numpy as np
+    from kiyosi_amd import workloads
+    from kiyosi_amd.engines import MonteCarloVanillaEngine
+    option, context, _, training = workloads.american_put_local_vol()
+    diffs, marginal = [], []
+    for seed in range(101, 107):
+        engine = MonteCarloVanillaEngine(path_count=1 << 20, step_count=50, seed=seed, precision="fp32")
+        mlp = engine.american_report(option, context, training_path_count=training, basis="mlp")
+        poly = engine.american_report(option, context, training_path_count=training, basis="poly4")
+        diffs.append(mlp.price - poly.price)
+        marginal.append(mlp.standard_error)
+    d = np.array(diffs)
+    paired = d.std(ddof=1) / math.sqrt(len(d))
+    print(f"MLP - poly4 over {len(d)} seeds: {d.mean():+.5f} +- {paired:.5f} (paired); "
+          f"one price's error {np.mean(marginal):.5f}")
+    assert paired < 0.25 * np.mean(marginal)
+    assert d.mean() > -3.0 * paired
