@@ -39,15 +39,7 @@ __device__ inline D sample_draws(D dr, uint64_t seed, int64_t block, int j) {
   } else {
     const uint64_t first = (uint64_t)j * (uint64_t)dr.dims;
     if (FP32) {
-      dr.g32.init(seed, (uint64_t)block);
-      dr.g32.advance((first >> 1) * 4);
-      dr.has32 = false;
-      if (first & 1) {
-        float e, spare;
-        f32::box_muller_pair(dr.g32, &e, &spare);
-        dr.spare32 = spare;
-        dr.has32 = true;
-      }
+      dr.start32(seed, (uint64_t)block, first);
     } else {
       dr.s.start(seed, (uint64_t)block, first);
     }

@@ -73,23 +73,23 @@ template <typename R>
 struct Tab;
 template <>
 struct Tab<double> {
-  static __device__ inline double drift(const AcPlanDev& p, int i) { return p.steps[i].drift; }
-  static __device__ inline double diff(const AcPlanDev& p, int i) { return p.steps[i].diffusion; }
-  static __device__ inline double weight(const AcPlanDev& p, int i) { return p.weight[i]; }
-  static __device__ inline double f(const AcPlanDev& p, int i) { return p.fill[i]; }
-  static __device__ inline double ko(const AcPlanDev& p, int e) { return p.events[e].knock_out; }
-  static __device__ inline double cb(const AcPlanDev& p, int e) { return p.events[e].coupon_barrier; }
-  static __device__ inline double idrift(const AcPlanDev& p, int k) { return p.intervals[k].drift; }
-  static __device__ inline double idiff(const AcPlanDev& p, int k) { return p.intervals[k].diffusion; }
+  static __device__ inline double drift(const AcPlanDev& p, int i) { return uload(&p.steps[i].drift); }
+  static __device__ inline double diff(const AcPlanDev& p, int i) { return uload(&p.steps[i].diffusion); }
+  static __device__ inline double weight(const AcPlanDev& p, int i) { return uload(p.weight + i); }
+  static __device__ inline double f(const AcPlanDev& p, int i) { return uload(p.fill + i); }
+  static __device__ inline double ko(const AcPlanDev& p, int e) { return uload(&p.events[e].knock_out); }
+  static __device__ inline double cb(const AcPlanDev& p, int e) { return uload(&p.events[e].coupon_barrier); }
+  static __device__ inline double idrift(const AcPlanDev& p, int k) { return uload(&p.intervals[k].drift); }
+  static __device__ inline double idiff(const AcPlanDev& p, int k) { return uload(&p.intervals[k].diffusion); }
 };
 template <>
 struct Tab<float> {
-  static __device__ inline float drift(const AcPlanDev& p, int i) { return p.steps32[2 * i]; }
-  static __device__ inline float diff(const AcPlanDev& p, int i) { return p.steps32[2 * i + 1]; }
-  static __device__ inline float weight(const AcPlanDev& p, int i) { return p.wf32[2 * i]; }
-  static __device__ inline float f(const AcPlanDev& p, int i) { return p.wf32[2 * i + 1]; }
-  static __device__ inline float ko(const AcPlanDev& p, int e) { return p.events32[2 * e]; }
-  static __device__ inline float cb(const AcPlanDev& p, int e) { return p.events32[2 * e + 1]; }
+  static __device__ inline float drift(const AcPlanDev& p, int i) { return uload(p.steps32 + 2 * i); }
+  static __device__ inline float diff(const AcPlanDev& p, int i) { return uload(p.steps32 + 2 * i + 1); }
+  static __device__ inline float weight(const AcPlanDev& p, int i) { return uload(p.wf32 + 2 * i); }
+  static __device__ inline float f(const AcPlanDev& p, int i) { return uload(p.wf32 + 2 * i + 1); }
+  static __device__ inline float ko(const AcPlanDev& p, int e) { return uload(p.events32 + 2 * e); }
+  static __device__ inline float cb(const AcPlanDev& p, int e) { return uload(p.events32 + 2 * e + 1); }
   static __device__ inline float idrift(const AcPlanDev& p, int k) { return p.intervals32[2 * k]; }
   static __device__ inline float idiff(const AcPlanDev& p, int k) { return p.intervals32[2 * k + 1]; }
 };
@@ -179,11 +179,27 @@ struct Draws {
   const uint64_t* shifts;
   uint64_t point;
   bool direct;
-  // PCG
+  // PCG. The FP32 normals of a lane queue up to three deep, so that whenever
+  // some lane needs a new Box-Muller pair every lane draws its next pair on
+  // the same call: lanes whose stream starts on an odd normal then stay in
+  // step with the others instead of diverging on every call. The values and
+  // their order are those of the lane's stream.
   NormalStream s;
   Pcg32 g32;
-  float spare32;
-  bool has32;
+  float q32[3];
+  int avail32;
+
+  // Positions the FP32 stream of sample stream `block` at normal `first`.
+  __device__ inline void start32(uint64_t seed, uint64_t block, uint64_t first) {
+    g32.init(seed, block);
+    g32.advance((first >> 1) * 4);
+    avail32 = 0;
+    if (first & 1) {
+      float e;
+      f32::box_muller_pair(g32, &e, &q32[0]);
+      avail32 = 1;
+    }
+  }
 
   __device__ inline double normal64(int d) {
     if (SOBOL) {
@@ -204,14 +220,24 @@ struct Draws {
   }
   __device__ inline float normal32(int d) {
     if (SOBOL) return f32::normal_from_bits(bits32(d));
-    if (has32) {
-      has32 = false;
-      return spare32;
+    // Lanes consume in step, so at a refill every lane holds zero or one.
+    if (__ballot(avail32 == 0) != 0 && avail32 <= 1) {
+      float e, o;
+      f32::box_muller_pair(g32, &e, &o);
+      if (avail32 == 0) {
+        q32[0] = e;
+        q32[1] = o;
+      } else {
+        q32[1] = e;
+        q32[2] = o;
+      }
+      avail32 += 2;
     }
-    float e;
-    f32::box_muller_pair(g32, &e, &spare32);
-    has32 = true;
-    return e;
+    const float v = q32[0];
+    q32[0] = q32[1];
+    q32[1] = q32[2];
+    --avail32;
+    return v;
   }
   template <typename R>
   __device__ inline R normal(int d) {
@@ -223,12 +249,17 @@ struct Draws {
   __device__ inline void skip(int count) {
     if (SOBOL || count <= 0) return;
     if (FP32) {
-      if (has32) {
-        has32 = false;
-        count -= 1;
+      for (; count > 0 && avail32 > 0; --count) {
+        q32[0] = q32[1];
+        q32[1] = q32[2];
+        --avail32;
       }
       g32.advance((uint64_t)(count >> 1) * 4);
-      if (count & 1) (void)normal32(0);
+      if (count & 1) {  // draw the pair and keep its second normal
+        float e;
+        f32::box_muller_pair(g32, &e, &q32[0]);
+        avail32 = 1;
+      }
     } else {
       s.skip((uint64_t)count);
     }
@@ -332,7 +363,7 @@ __device__ double bridge_path(const AcPlanDev& p, D& dr, bool live, float sign,
   int start = 0;
   for (int k = 0; k < p.n_intervals; ++k) {
     if (__ballot(active) == 0) break;
-    const int end = p.intervals[k].end;
+    const int end = uload(&p.intervals[k].end);
     R target = lr;
     bool filling = false;
     if (active) {
@@ -344,11 +375,11 @@ __device__ double bridge_path(const AcPlanDev& p, D& dr, bool live, float sign,
     lr = target;
     start = end + 1;
     if (active) {
-      if (monitoring && (below || (p.step_observe[end] && lr < level))) {
+      if (monitoring && (below || (uload_u8(p.step_observe, end) && lr < level))) {
         monitoring = false;
         ki = true;
       }
-      const int e = p.step_event[end];
+      const int e = uload(p.step_event + end);
       if (e >= 0) {
         const AcEventD& ev = p.events[e];
         const bool hit = ev.has_knock_out && lr >= Tab<R>::ko(p, e);
@@ -516,8 +547,8 @@ __device__ double step_path(const AcPlanDev& p, D& dr, bool live, float sign,
       double s = discounted_product(p.initial_spot, 1.0, (double)lr);
       if (!isfinite(s) || s <= 0.0) bad = 1;
     }
-    if (p.step_observe[i]) ki = ki || (p.has_ki && lr < level);
-    const int e = p.step_event[i];
+    if (uload_u8(p.step_observe, i)) ki = ki || (p.has_ki && lr < level);
+    const int e = uload(p.step_event + i);
     if (e >= 0) {
       const AcEventD& ev = p.events[e];
       const bool out = ev.has_knock_out && lr >= Tab<R>::ko(p, e);
@@ -1049,7 +1080,8 @@ inline AcHostPlan build_plan(const kg_autocall_terms& t, const kg_bsm_market& m)
 template <class T>
 inline void stage(std::vector<unsigned char>& blob, const std::vector<T>& v, size_t* offset) {
   *offset = (blob.size() + 255) & ~size_t(255);
-  blob.resize(*offset + v.size() * sizeof(T) + 1);
+  // Whole words, so that uload_u8 may read the word holding any entry.
+  blob.resize(*offset + ((v.size() * sizeof(T) + 4) & ~size_t(3)));
   if (!v.empty()) std::memcpy(blob.data() + *offset, v.data(), v.size() * sizeof(T));
 }
 

@@ -61,14 +61,7 @@ __device__ __attribute__((always_inline)) void sample_body(const BookItem& it, i
   if (!SOBOL) {
     const uint64_t first = (uint64_t)j * (uint64_t)it.dims;
     if (FP32) {
-      dr.g32.init(it.seed, (uint64_t)block);
-      dr.g32.advance((first >> 1) * 4);
-      dr.has32 = false;
-      if (first & 1) {
-        float e;
-        f32::box_muller_pair(dr.g32, &e, &dr.spare32);
-        dr.has32 = true;
-      }
+      dr.start32(it.seed, (uint64_t)block, first);
     } else {
       dr.s.start(it.seed, (uint64_t)block, first);
     }
@@ -243,14 +236,7 @@ __global__ void __launch_bounds__(256)
   if (!SOBOL) {
     const uint64_t first = (uint64_t)j * (uint64_t)it.dims;
     if (FP32) {
-      dr.g32.init(it.seed, (uint64_t)block);
-      dr.g32.advance((first >> 1) * 4);
-      dr.has32 = false;
-      if (first & 1) {
-        float e;
-        f32::box_muller_pair(dr.g32, &e, &dr.spare32);
-        dr.has32 = true;
-      }
+      dr.start32(it.seed, (uint64_t)block, first);
     } else {
       dr.s.start(it.seed, (uint64_t)block, first);
     }
@@ -331,14 +317,7 @@ __global__ void __launch_bounds__(256)
   if (!SOBOL) {
     const uint64_t first = (uint64_t)j * (uint64_t)it.dims;
     if (FP32) {
-      dr.g32.init(it.seed, (uint64_t)block);
-      dr.g32.advance((first >> 1) * 4);
-      dr.has32 = false;
-      if (first & 1) {
-        float e;
-        f32::box_muller_pair(dr.g32, &e, &dr.spare32);
-        dr.has32 = true;
-      }
+      dr.start32(it.seed, (uint64_t)block, first);
     } else {
       dr.s.start(it.seed, (uint64_t)block, first);
     }
@@ -410,7 +389,8 @@ struct Blob {
   template <class T>
   size_t add(const std::vector<T>& v) {
     const size_t off = (bytes.size() + 255) & ~size_t(255);
-    bytes.resize(off + std::max<size_t>(v.size() * sizeof(T), 1));
+    // Whole words, so that uload_u8 may read the word holding any entry.
+    bytes.resize(off + ((v.size() * sizeof(T) + 4) & ~size_t(3)));
     if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
     return off;
   }
@@ -1336,21 +1316,33 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
     const bool use_graph = tuning && tuning->use_graph;
     const int replays = tuning ? std::max(0, tuning->replays) : 0;
     if (use_graph) {
-      // Destroys the graph on every exit, a cancelled run's included.
-      struct GraphGuard {
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        ~GraphGuard() {
-          if (exec) (void)hipGraphExecDestroy(exec);
-          if (graph) (void)hipGraphDestroy(graph);
-        }
-      } guard;
-      KG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      run_book_once(ctx, fp32, spt, wpc, d_items, d_tiles, (int)tiles.size(), d_counter, d_parts, flags);
-      hipLaunchKernelGGL(merge_book, dim3(items.size()), dim3(MERGE_THREADS), 0, ctx->stream, d_items, d_parts, d_out);
-      KG_HIP(hipStreamEndCapture(ctx->stream, &guard.graph));
-      KG_HIP(hipGraphInstantiate(&guard.exec, guard.graph, nullptr, nullptr, 0));
-      hipGraphExec_t exec = guard.exec;
+      const uint64_t key[12] = {(uint64_t)fp32,         (uint64_t)spt,          (uint64_t)wpc,
+                                (uint64_t)items.size(), (uint64_t)tiles.size(), (uint64_t)(uintptr_t)d_items,
+                                (uint64_t)(uintptr_t)d_tiles, (uint64_t)(uintptr_t)d_counter,
+                                (uint64_t)(uintptr_t)d_parts, (uint64_t)(uintptr_t)d_out,
+                                (uint64_t)(uintptr_t)flags, (uint64_t)(uintptr_t)cancel_word(ctx)};
+      if (!ctx->book_graph || std::memcmp(key, ctx->book_graph_key, sizeof(key)) != 0) {
+        if (ctx->book_graph) (void)hipGraphExecDestroy(ctx->book_graph);
+        ctx->book_graph = nullptr;
+        // Destroys the captured graph on every exit; only an instantiated
+        // graph is kept.
+        struct GraphGuard {
+          hipGraph_t graph = nullptr;
+          ~GraphGuard() {
+            if (graph) (void)hipGraphDestroy(graph);
+          }
+        } guard;
+        KG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        run_book_once(ctx, fp32, spt, wpc, d_items, d_tiles, (int)tiles.size(), d_counter, d_parts, flags);
+        hipLaunchKernelGGL(merge_book, dim3(items.size()), dim3(MERGE_THREADS), 0, ctx->stream, d_items, d_parts,
+                           d_out);
+        KG_HIP(hipStreamEndCapture(ctx->stream, &guard.graph));
+        hipGraphExec_t made = nullptr;
+        KG_HIP(hipGraphInstantiate(&made, guard.graph, nullptr, nullptr, 0));
+        ctx->book_graph = made;
+        std::memcpy(ctx->book_graph_key, key, sizeof(key));
+      }
+      hipGraphExec_t exec = ctx->book_graph;
       timing_begin(ctx);
       KG_HIP(hipGraphLaunch(exec, ctx->stream));
       device_ms = timing_end(ctx);

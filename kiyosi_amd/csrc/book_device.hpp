@@ -213,8 +213,8 @@ __device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign
     if (__ballot(active) == 0) break;
     if (!active) continue;
     const BarrierStep& s = b.steps[i];
-    const R width = FP32 ? (R)b.steps32[4 * i + 1] : (R)s.width;
-    const R mean = FP32 ? (R)b.steps32[4 * i] : (R)s.mean;
+    const R width = FP32 ? (R)uload(b.steps32 + 4 * i + 1) : (R)uload(&s.width);
+    const R mean = FP32 ? (R)uload(b.steps32 + 4 * i) : (R)uload(&s.mean);
     const R z = (R)sign * dr.template normal<R>(i);
     if (LR && i == 0) z0 = z;
     noise = FP32 ? (R)fmaf((float)width, (float)z, (float)noise) : (R)fma((double)width, (double)z, (double)noise);
@@ -223,11 +223,11 @@ __device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign
       const double sp = discounted_product(b.spot, 1.0, (double)terminal);
       if (!isfinite(sp) || sp <= 0.0) bad = 1;
     }
-    if (s.observe) {
-      const R lower = FP32 ? (R)b.steps32[4 * i + 2] : (R)s.lower;
+    if (uload(&s.observe)) {
+      const R lower = FP32 ? (R)uload(b.steps32 + 4 * i + 2) : (R)uload(&s.lower);
       bool crossed;
       if (b.has_upper) {
-        const R upper = FP32 ? (R)b.steps32[4 * i + 3] : (R)s.upper;
+        const R upper = FP32 ? (R)uload(b.steps32 + 4 * i + 3) : (R)uload(&s.upper);
         crossed = noise <= -lower || noise >= -upper;
       } else {
         crossed = b.up ? noise >= -lower : noise <= -lower;
@@ -628,14 +628,14 @@ __device__ double accum_path(const AccumDev& a, D& dr, bool live, float sign, ui
     if (__ballot(active) == 0) break;
     if (!active) continue;
     const R z = (R)sign * dr.template normal<R>(i);
-    lr += FP32 ? (R)fmaf(a.steps32[2 * i + 1], (float)z, a.steps32[2 * i])
-               : (R)fma(a.steps[5 * i + 1], (double)z, a.steps[5 * i]);
+    lr += FP32 ? (R)fmaf(uload(a.steps32 + 2 * i + 1), (float)z, uload(a.steps32 + 2 * i))
+               : (R)fma(uload(a.steps + 5 * i + 1), (double)z, uload(a.steps + 5 * i));
     if (!((double)lr >= a.safe_lo && (double)lr <= a.safe_hi)) {
       const double sp = discounted_product(a.spot, 1.0, (double)lr);
       if (!isfinite(sp) || sp <= 0.0) bad = 1;
     }
-    const uint8_t f = a.flags[i];
-    if (a.fixed) quantity += a.steps[5 * i + 3];
+    const uint8_t f = uload_u8(a.flags, i);
+    if (a.fixed) quantity += uload(a.steps + 5 * i + 3);
     if ((f & ACC_OBSERVE) && sg * lr >= sg * ko) {
       ko_step = i;
       ko_lr = lr;
@@ -644,7 +644,7 @@ __device__ double accum_path(const AccumDev& a, D& dr, bool live, float sign, ui
       continue;
     }
     if (a.fixed)
-      quantity += a.steps[5 * i + 4];
+      quantity += uload(a.steps + 5 * i + 4);
     else if (f & ACC_ACCUMULATE)
       quantity += a.daily_qty * (sg * lr < sg * sk ? a.accel : 1.0);
     if (!isfinite(quantity)) bad = 1;

@@ -359,6 +359,8 @@ void kg_context_destroy(kg_context* ctx) {
                                 &ctx->book_out, &ctx->book_counter, &ctx->policy, &ctx->lsmc_tables,
                                 &ctx->mlp_state})
       b->release();
+    if (ctx->book_graph) (void)hipGraphExecDestroy(ctx->book_graph);
+    ctx->book_graph = nullptr;
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     ctx->pinned = nullptr;
     if (ctx->start) (void)hipEventDestroy(ctx->start);

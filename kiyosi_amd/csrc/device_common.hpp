@@ -127,41 +127,76 @@ __device__ inline void box_muller_pair(Pcg32& g, double* even, double* odd) {
 // from Box-Muller pair k/2, i.e. PCG outputs 4*(k/2) .. 4*(k/2)+3
 // (monte_carlo.rs:1689-1703, 1830-1845). Positioning is by skip-ahead, so a
 // thread can start at any sample of the block.
+// FP64 normals of a PCG stream, Box-Muller pairs in stream order. They
+// queue up to three deep so that whenever some lane of the wave needs a new
+// pair every lane draws its next pair on the same call: lanes whose stream
+// starts on an odd normal stay in step with the others instead of diverging
+// on every call. The values and their order are those of the stream.
 struct NormalStream {
   Pcg32 g;
-  double spare;
-  bool has_spare;
+  double q[3];
+  int avail;
   __device__ inline void start(uint64_t seed, uint64_t block, uint64_t first_normal) {
     g.init(seed, block);
     g.advance((first_normal >> 1) * 4);
-    has_spare = false;
+    avail = 0;
     if (first_normal & 1) {
       double e;
-      box_muller_pair(g, &e, &spare);
-      has_spare = true;
+      box_muller_pair(g, &e, &q[0]);
+      avail = 1;
     }
   }
   __device__ inline double next() {
-    if (has_spare) {
-      has_spare = false;
-      return spare;
+    // Lanes consume in step, so at a refill every lane holds zero or one.
+    if (__ballot(avail == 0) != 0 && avail <= 1) {
+      double e, o;
+      box_muller_pair(g, &e, &o);
+      if (avail == 0) {
+        q[0] = e;
+        q[1] = o;
+      } else {
+        q[1] = e;
+        q[2] = o;
+      }
+      avail += 2;
     }
-    double e;
-    box_muller_pair(g, &e, &spare);
-    has_spare = true;
-    return e;
+    const double v = q[0];
+    q[0] = q[1];
+    q[1] = q[2];
+    --avail;
+    return v;
   }
   // Moves the stream forward by `count` normals without drawing them.
   __device__ inline void skip(uint64_t count) {
-    if (count == 0) return;
-    if (has_spare) {
-      has_spare = false;
-      count -= 1;
+    for (; count > 0 && avail > 0; --count) {
+      q[0] = q[1];
+      q[1] = q[2];
+      --avail;
     }
     g.advance((count >> 1) * 4);
-    if (count & 1) next();
+    if (count & 1) {  // draw the pair and keep its second normal
+      double e;
+      box_muller_pair(g, &e, &q[0]);
+      avail = 1;
+    }
   }
 };
+
+// A load through the constant address space: tables that no kernel writes
+// while it runs (per-step plans staged before the launch). With a uniform
+// address the compiler then issues a scalar load instead of a vector load
+// that waits on memory every step; a divergent address still loads per lane.
+template <class T>
+__device__ __forceinline__ T uload(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)p;
+}
+// Entry i of a byte table through the word that holds it (scalar memory
+// loads are whole words); the table starts word-aligned and is staged in
+// whole words.
+__device__ __forceinline__ uint8_t uload_u8(const uint8_t* p, int i) {
+  const uint32_t w = uload((const uint32_t*)(p + (i & ~3)));
+  return (uint8_t)(w >> (8 * (i & 3)));
+}
 
 // ------------------------------------------------------------------ ndtri
 // Cephes ndtri in FP64 (engines/monte_carlo/sobol.rs:262-346).

@@ -107,6 +107,11 @@ struct kg_context {
   kg::DeviceBuffer policy, lsmc_tables, mlp_state;
   int tuned_spt[2] = {0, 0};
   int tuned_wpc[2] = {0, 0};
+  // The book's instantiated launch graph, kept across calls while its launch
+  // shape and buffers (the key) stay the same; the contracts and tiles it
+  // reads are uploaded before each launch.
+  hipGraphExec_t book_graph = nullptr;
+  uint64_t book_graph_key[12] = {};
   // Pinned host memory (hipHostMalloc): results and flags of a call in the
   // first PINNED_RESULT_BYTES, then a staging area for asynchronous uploads
   // whose bump offset restarts with every call.

@@ -352,3 +352,16 @@ def test_book_prices_seasoned_and_forward_start_averages(gpu, oracle):
     for (eng, ins, ctx), got in zip(entries, estimates):
         ref = oracle.asian_price(eng._settings(), ins._terms(eng.step_count), ctx.bsm_market(), workers=8)
         assert got.price == pytest.approx(ref.price, rel=1e-10, abs=1e-13)
+
+
+def test_a_kept_launch_graph_prices_the_next_book_of_the_same_shape(gpu):
+    """The second book differs only in its seeds, so it has the same contract
+    and tile counts and the call replays the graph kept from the first; the
+    graph must read the new contracts."""
+    first = small_book(30)
+    second = [(type(e)(**dict(e._kwargs(), seed=e.seed + 1000)), i, c) for e, i, c in first]
+    a, _ = Book(first).price("fp32")
+    b, _ = Book(second).price("fp32")
+    assert [e.price for e in a] != [e.price for e in b]
+    assert [e.price for e in a] == [e.price for e in Book(first).price("fp32", use_graph=False)[0]]
+    assert [e.price for e in b] == [e.price for e in Book(second).price("fp32", use_graph=False)[0]]
