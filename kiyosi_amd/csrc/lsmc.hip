@@ -343,7 +343,7 @@ __global__ void __launch_bounds__(256)
   int bracket[2] = {-1, -1};
   bool alive[2] = {live, paired};
   for (int step = 0; step < m.steps; ++step) {
-    if (LV) lv_stage(m, step, lv_lds, rows);  // block-uniform
+    if (LV) lv_stage<FP32>(m, step, lv_lds, rows);  // block-uniform
     if (!alive[0] && !alive[1]) continue;
     const double e = z.next();
     for (int leg = 0; leg < 2; ++leg) {

@@ -48,19 +48,50 @@ struct LvRows {
   const double* row1;
   double wt;
   int row;
+  // FP32 copies for the FP32-step kernels (tables staged in LDS only): the
+  // spot nodes, the reciprocal node spacings and the two rows.
+  const float* spots32;
+  const float* inv32;
+  const float* row0_32;
+  const float* row1_32;
+  float wt32;
+  bool f32;
 };
 
 // Stages the rows of time index i (year0 + dt i). Every thread of the block
 // calls it with the same i; it synchronises the block only when the row
-// changes.
+// changes. FP32 stages single-precision tables when the spot nodes fit in
+// LDS, so that the FP32 steps interpolate without FP64 arithmetic.
+template <bool FP32>
 __device__ inline void lv_stage(const LsmcModel& m, int i, double* lds, LvRows& t) {
   double wt;
   const int r = lv_bracket(m.years, m.nt, m.year0 + m.dt * i, &wt);
   t.wt = wt;
+  t.wt32 = (float)wt;
+  t.f32 = false;
   if (m.ns > LV_LDS_NODES) {
     t.spots = m.spots;
     t.row0 = m.vols + (size_t)r * m.ns;
     t.row1 = t.row0 + m.ns;
+    t.row = r;
+    return;
+  }
+  if (FP32) {
+    float* f = (float*)lds;  // spots, reciprocal spacings, row r, row r + 1
+    t.f32 = true;
+    t.spots32 = f;
+    t.inv32 = f + m.ns;
+    t.row0_32 = f + 2 * m.ns;
+    t.row1_32 = f + 3 * m.ns;
+    if (t.row == r) return;
+    __syncthreads();
+    if (t.row < 0)
+      for (int k = threadIdx.x; k < m.ns; k += blockDim.x) {
+        f[k] = (float)m.spots[k];
+        f[m.ns + k] = k + 1 < m.ns ? (float)(1.0 / (m.spots[k + 1] - m.spots[k])) : 0.0f;
+      }
+    for (int k = threadIdx.x; k < 2 * m.ns; k += blockDim.x) f[2 * m.ns + k] = (float)m.vols[(size_t)r * m.ns + k];
+    __syncthreads();
     t.row = r;
     return;
   }
@@ -95,6 +126,35 @@ __device__ inline double lv_sigma_at(const LsmcModel& m, const LvRows& t, double
   return lv_interp(lv_interp(t.row0[c], t.row0[c + 1], ws), lv_interp(t.row1[c], t.row1[c + 1], ws), t.wt);
 }
 
+// The same in FP32 on the single-precision tables: the spot weight takes the
+// staged reciprocal spacing instead of a divide.
+__device__ inline float lv_interp32(float low, float high, float w) {
+  return w <= 0.5f ? fmaf(w, high - low, low) : fmaf(1.0f - w, low - high, high);
+}
+__device__ inline float lv_sigma_at32(const LsmcModel& m, const LvRows& t, float spot, float log_rel, int& c) {
+  const int ns = m.ns;
+  if (m.log_uniform) {
+    const float g = floorf((log_rel + (float)(m.log_spot - m.log_s0)) * (float)m.inv_h);
+    c = (int)fminf(fmaxf(g, 0.0f), (float)(ns - 2));
+  } else if (c < 0) {
+    int lo = 0, hi = ns;  // first node above spot
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (t.spots32[mid] <= spot)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    c = lo > 0 ? lo - 1 : 0;
+    if (c > ns - 2) c = ns - 2;
+  }
+  while (c > 0 && spot < t.spots32[c]) --c;
+  while (c < ns - 2 && spot >= t.spots32[c + 1]) ++c;
+  const float ws = fminf(fmaxf((spot - t.spots32[c]) * t.inv32[c], 0.0f), 1.0f);
+  return lv_interp32(lv_interp32(t.row0_32[c], t.row0_32[c + 1], ws), lv_interp32(t.row1_32[c], t.row1_32[c + 1], ws),
+                     t.wt32);
+}
+
 // spots_at (american.rs:1166-1215): spot after step+1 intervals.
 __device__ inline double bsm_spot(const LsmcModel& m, int step, double w) {
   const double lg = m.diffusion * w;
@@ -119,7 +179,9 @@ __device__ inline double continuation(const double* pol, int basis, double x) {
 template <bool FP32>
 __device__ inline double lv_step(const LsmcModel& m, const LvRows& t, double x, double z, int& c) {
   if (FP32) {
-    const float sig = (float)lv_sigma_at(m, t, m.spot * (double)f32::exp((float)x), x, c);
+    const float xf = (float)x;
+    const float sig = t.f32 ? lv_sigma_at32(m, t, (float)m.spot * f32::exp(xf), xf, c)
+                            : (float)lv_sigma_at(m, t, m.spot * (double)f32::exp(xf), x, c);
     const float dt = (float)m.dt;
     return (double)((float)x + fmaf(sig * f32::sqrt(dt), (float)z,
                                     ((float)(m.rate - m.dividend) - 0.5f * sig * sig) * dt));
@@ -256,7 +318,7 @@ __global__ void __launch_bounds__(256)
   double ap = m.spot, an = m.spot;  // anchored spots of the two legs
   bool bad = false;
   for (int k = 0; k < m.steps; ++k) {
-    if (LV) lv_stage(m, k, lv_lds, rows);  // block-uniform
+    if (LV) lv_stage<FP32>(m, k, lv_lds, rows);  // block-uniform
     if (!live) continue;
     const double e = z.next();
     if (KIND == PATHS_ANCHORED) {

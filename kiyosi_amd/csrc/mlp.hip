@@ -579,7 +579,7 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
     if (fit) stage(s_pack, nets + (size_t)step * PACK_WORDS);
     __syncthreads();
-    if (LV) lv_stage(m, step, lv_lds, rows);  // block-uniform
+    if (LV) lv_stage<FP32>(m, step, lv_lds, rows);  // block-uniform
     const double e = live ? z.next() : 0.0;
     for (int leg = 0; leg < 2; ++leg) {
       if (!alive[leg]) continue;
@@ -602,12 +602,14 @@ __global__ void __launch_bounds__(256)
     if (!fit) continue;
     const Pack P(s_pack);
     const double t = m.dt * (step + 1);
-    if (LV) lv_stage(m, step + 1, lv_lds, rows);  // the date's rows; also the next step's start
+    if (LV) lv_stage<FP32>(m, step + 1, lv_lds, rows);  // the date's rows; also the next step's start
     for (int leg = 0; leg < 2; ++leg) {
       const double intrinsic = payoff(m, current[leg]);
       const bool ask = alive[leg] && intrinsic > 0.0;
       if (__ballot(ask) == 0) continue;
-      const double sigma = LV ? lv_sigma_at(m, rows, current[leg], x[leg], bracket[leg]) : m.sigma;
+      const double sigma = !LV ? m.sigma
+                           : rows.f32 ? (double)lv_sigma_at32(m, rows, (float)current[leg], (float)x[leg], bracket[leg])
+                                      : lv_sigma_at(m, rows, current[leg], x[leg], bracket[leg]);
       const float u0 = (float)(4.0 * (current[leg] / m.strike - 1.0));
       const float u1 = (float)((t - sc.t_mid) * sc.t_scale);
       const float u2 = (float)((sigma - sc.s_mid) * sc.s_scale);

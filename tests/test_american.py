@@ -225,13 +225,17 @@ def test_gpu_local_vol_lsmc_matches_the_oracle(gpu, oracle, basis):
 
 
 @pytest.mark.gpu
-def test_gpu_fp32_lsmc_within_half_a_standard_error(gpu):
+@pytest.mark.parametrize("paths", [1 << 18, 1 << 20])
+def test_gpu_fp32_lsmc_within_half_a_standard_error(gpu, paths):
+    """FP32 steps (single-precision local-volatility interpolation included)
+    against FP64 steps on the same draws, up to the full 2^20 training and
+    2^20 pricing paths of the local-volatility put."""
     option, context, engine, _ = workloads.american_put_local_vol()
     model = american_model(context)
     terms = option._american_terms(50)
-    plain = gpu_american(gpu, settings(1 << 18, seed=9), terms, model, _abi.BASIS_POLY4, training=1 << 18)
-    fast = gpu_american(gpu, settings(1 << 18, seed=9, precision=_abi.FP32_STEP), terms, model, _abi.BASIS_POLY4,
-                        training=1 << 18)
+    plain = gpu_american(gpu, settings(paths, seed=9), terms, model, _abi.BASIS_POLY4, training=paths)
+    fast = gpu_american(gpu, settings(paths, seed=9, precision=_abi.FP32_STEP), terms, model, _abi.BASIS_POLY4,
+                        training=paths)
     assert abs(fast.price - plain.price) <= 0.5 * plain.standard_error
 
 
