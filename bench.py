@@ -39,6 +39,7 @@ sys.path.insert(0, str(ROOT))
 
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0  # dense MFMA
 HBM_PEAK_GBS = 8000.0
 # Algorithmic FLOP per path-step (FMA 2, other arithmetic and elementary
 # functions 1, integer RNG work 0).
@@ -49,6 +50,7 @@ C4_FLOP = {"BarrierOption": 10.0, "GeometricAveragePriceOption": 10.0, "Arithmet
            "SnowballOption": 10.0, "PhoenixOption": 10.0, "Accumulator": 13.0}
 C5_FORWARD_FLOP = 25.0  # normal 6, local-vol bilinear 11, log-Euler 6, exp 2
 C5_BYTES_PER_PATH_DATE = 24.0  # FP32 coordinate stored and reloaded, FP64 cash read and written
+MLP_MFMA_FLOP = 3 * 2 * 64 * 64  # per sample and Adam step: layer-2 forward, input gradient, weight gradient
 # HBM bytes per config-2 FP32 pricing of its two sampling kernels
 # (bridge_events_kernel 13,438,765 + bridge_fill_kernel 16,685,122): FETCH_SIZE
 # x 2.0 + WRITE_SIZE x 1.0 from the two counter passes of `bench.py
@@ -271,7 +273,8 @@ def config5(cpu):
         out[basis] = {"price": r.price, "standard_error": r.standard_error, "in_sample": r.in_sample_price,
                       "training_ms": r.training_ms, "pricing_ms": r.pricing_ms, "call_s": wall,
                       "path_steps": r.path_steps, "path_steps_per_s_call": r.path_steps / wall,
-                      "mean_loss": r.mean_loss if basis == "mlp" else None}
+                      "mean_loss": r.mean_loss if basis == "mlp" else None,
+                      "gradient_samples": r.gradient_samples if basis == "mlp" else None}
     train_ms = out["poly4"]["training_ms"]
     algorithmic = training * 50 * C5_BYTES_PER_PATH_DATE
     out["roofline"] = roofline("hbm", algorithmic / (train_ms * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s",
@@ -279,6 +282,13 @@ def config5(cpu):
                                kernel_ms=train_ms,
                                basis="24 algorithmic bytes per training path and date: FP32 coordinate stored "
                                      "and reloaded, FP64 cash read and written")
+    mlp = out["mlp"]
+    out["roofline_mlp"] = roofline(
+        "mfma", MLP_MFMA_FLOP * mlp["gradient_samples"] / (mlp["training_ms"] * 1e-3) / 1e12, BF16_PEAK_TFLOPS,
+        "TFLOP/s", kernel="MLP training phase (mlp_grad + mlp_update per Adam step, per-date sample preparation)",
+        kernel_ms=mlp["training_ms"],
+        basis="24,576 bf16 MFMA FLOP per training sample and Adam step (forward 64x64, input gradient 64x64, "
+              "weight gradient 64x64), over the whole training phase's device time")
     if cpu:
         from kiyosi_amd import _abi
         from kiyosi_amd.market import american_model
@@ -323,7 +333,8 @@ def headline_cpu_baseline(note, context, kw):
 def profile_pass():
     """A short run for hardware-counter passes: a torch elementwise multiply
     that reads and writes 256 MiB each (the calibration of FETCH_SIZE and
-    WRITE_SIZE), three config-2 FP32 pricings and one config-5 poly-4 LSMC."""
+    WRITE_SIZE), three config-2 FP32 pricings, one config-5 poly-4 LSMC and
+    one config-5 MLP fit, and one FP32 pricing of the config-4 book."""
     import torch
 
     import kiyosi_amd as k
@@ -342,6 +353,9 @@ def profile_pass():
     option, context5, _, training = workloads.american_put_local_vol()
     e5 = k.MonteCarloVanillaEngine(path_count=1 << 20, step_count=50, seed=1729, precision="fp32")
     e5.american_report(option, context5, training_path_count=training, basis="poly4")
+    e5.american_report(option, context5, training_path_count=training, basis="mlp")
+    from kiyosi_amd.engines import Book
+    Book(workloads.book_4096()).price("fp32", samples_per_thread=4, workgroups_per_cu=8)
     log("profile pass: done")
 
 

@@ -622,6 +622,7 @@ typedef struct kg_lsmc_report {
   double mean_loss;      /* MLP: mean squared error of the last epoch over all dates */
   int32_t dates_fitted;  /* exercise dates with a fitted continuation value */
   int32_t reserved;
+  double gradient_samples; /* MLP: training samples through the gradient kernel, over all Adam steps */
 } kg_lsmc_report;
 
 kg_status kg_american_price(kg_context* ctx, const kg_settings* settings,

@@ -331,4 +331,5 @@ class LsmcReport(C.Structure):
         ("mean_loss", C.c_double),
         ("dates_fitted", C.c_int32),
         ("reserved", C.c_int32),
+        ("gradient_samples", C.c_double),
     ]

@@ -738,7 +738,8 @@ static int launch_grad(kg_context* ctx, const Net& net, const float4* samples, i
 // counts Adam steps since the moments were last cleared; with `average`
 // the weights become the mean of the last epoch's iterates.
 static int train(kg_context* ctx, const Net& net, const float4* samples, int n, int batch, int epochs, float lr,
-                 bool reset_moments, bool average, int& step, uint32_t* flags, float* loss_last_epoch) {
+                 bool reset_moments, bool average, int& step, uint32_t* flags, float* loss_last_epoch,
+                 double* gradient_samples) {
   if (reset_moments) {
     clear(ctx, net.m1, 2 * (int64_t)P_STRIDE);  // m1 and m2 are contiguous
     step = 0;
@@ -749,6 +750,7 @@ static int train(kg_context* ctx, const Net& net, const float4* samples, int n, 
     for (int b = 0; b < n; b += batch) {
       const int nb = std::min(batch, n - b);
       const int parts = launch_grad(ctx, net, samples, b, nb, flags);
+      *gradient_samples += nb;
       ++step;
       if (final_epoch) ++last;
       launch_update(ctx, net, UPDATE_ADAM, parts, lr, step, nullptr, final_epoch ? loss_last_epoch : nullptr,
@@ -848,7 +850,7 @@ static void run(kg_context* ctx, LsmcHost& h, const kg_lsmc_options& opt, kg_lsm
     // The first fitted date starts cold with four times the epochs; later
     // dates continue from the previous network and its Adam moments.
     steps_last[step] = train(ctx, net, data, n, batch, first ? 4 * epochs : epochs, lr, first, true, adam_step,
-                             flags, d_losses + step);
+                             flags, d_losses + step, &out->gradient_samples);
     first = false;
     hipLaunchKernelGGL(mlp_eval, dim3((n + 63) / 64), dim3(256), 0, ctx->stream, net.packed, data, n, nullptr, m,
                        idx, spot, cash, flags);
