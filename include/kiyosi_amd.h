@@ -629,6 +629,27 @@ kg_status kg_american_price(kg_context* ctx, const kg_settings* settings,
                             const kg_american_terms* option, const kg_american_model* model,
                             const kg_lsmc_options* options, kg_lsmc_report* out);
 
+/* Lanes of one American option on shared paths: lane l is `option` with
+ * strike strikes[l] (option->strike when strikes is NULL) under models[l],
+ * and out[l] equals kg_american_price of that lane alone, bit for bit.
+ * Replaces american_scenarios::<N, S> (engines/monte_carlo/american.rs:955-1455)
+ * as called by MonteCarloVanillaEngine::price_strikes (:677-771, groups of
+ * 4/2/1 strikes) and by the American Greek stencil (greek_prices and
+ * greek_scenario_group, :775-931, scenario groups at one valuation time).
+ * In-sample polynomial fits share one set of stored paths when every lane
+ * has the same valuation time and normalised paths, and either all lanes
+ * are BSM (spot, rate, dividend, volatility and strike may differ: the
+ * Brownian coordinates are model-free) or all share one local-volatility
+ * market (strikes differ); one generation and one backward pass then fit
+ * every lane, with a regression per lane. Other requests (MLP,
+ * out-of-sample, anchored paths, mixed markets) price each lane alone.
+ * Every lane is validated first; a failed request fills no lane. */
+#define KG_MAX_AMERICAN_LANES 16
+kg_status kg_american_price_lanes(kg_context* ctx, const kg_settings* settings,
+                                  const kg_american_terms* option, const kg_american_model* models,
+                                  const double* strikes, int32_t n_lanes, const kg_lsmc_options* options,
+                                  kg_lsmc_report* out);
+
 /* ---- Generator diagnostics, computed on the GPU with per-thread skip-ahead.
  * kg_pcg32_outputs: outputs first..first+count of Pcg32::new(seed, stream)
  * (monte_carlo.rs:2247-2267). kg_sobol_points: the 52-bit scrambled Sobol

@@ -276,6 +276,7 @@ class BookTuning(C.Structure):
 BASIS_QUADRATIC, BASIS_POLY4, BASIS_MLP = 0, 1, 2
 MAX_SCENARIOS = 64  # KG_MAX_SCENARIOS
 MAX_AUTOCALL_SCENARIOS = 16  # KG_MAX_AUTOCALL_SCENARIOS
+MAX_AMERICAN_LANES = 16  # KG_MAX_AMERICAN_LANES
 
 
 class LocalVolSurface(C.Structure):

@@ -70,6 +70,9 @@ SIGNATURES = {
                                        P(_abi.Estimate)]),
     "kg_american_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AmericanTerms), P(_abi.AmericanModel),
                                     P(_abi.LsmcOptions), P(_abi.LsmcReport)]),
+    "kg_american_price_lanes": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AmericanTerms),
+                                          P(_abi.AmericanModel), P(C.c_double), C.c_int32, P(_abi.LsmcOptions),
+                                          P(_abi.LsmcReport)]),
     "kg_price_book": (C.c_int, [C.c_void_p, P(_abi.BookContract), C.c_size_t, C.c_int32, P(_abi.BookTuning),
                                 P(_abi.Estimate)]),
     "kg_mlp_gradient": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_int64, P(C.c_float),

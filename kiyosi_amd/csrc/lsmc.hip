@@ -20,13 +20,20 @@
 namespace kg {
 
 // ------------------------------------------------------------------ device
-__global__ void clear_doubles(double* p, int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+__global__ void clear_doubles(double* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+static void clear_policy(kg_context* ctx, double* p, int64_t n) {
+  const int64_t grid = std::min<int64_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(clear_doubles, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p, n);
+  KG_HIP(hipGetLastError());
 }
 
 // Entries of the normal equations: quadratic keeps the reference's nine
-// sums (x^0..x^4 moments and three right-hand sides); poly4 holds the 15
-// distinct Gram entries of (1, x, .., x^4) and five right-hand sides.
+// sums (x^0..x^4 moments and three right-hand sides); poly4 holds the nine
+// power sums x^0..x^8 (Gram entry [r][c] is power r + c) and the five
+// right-hand sides x^r t.
 template <int BASIS>
 struct Entries;
 template <>
@@ -35,8 +42,52 @@ struct Entries<KG_BASIS_QUADRATIC> {
 };
 template <>
 struct Entries<KG_BASIS_POLY4> {
-  static constexpr int N = 20;
+  static constexpr int N = 14;
 };
+
+// Quadratic block partials: the slot threads write entry k of slot s to
+// lds[k * QUAD_ROW + s]; thread k then sums row k in slot order (the
+// reference's order). The row pitch of 514 doubles puts the nine rows in
+// different LDS banks.
+constexpr int QUAD_ROW = 514;
+// Poly-4 block partials: 14 rows of the 256 threads' sums; the pitch of
+// 264 doubles staggers the rows' banks.
+constexpr int POLY4_ROW = 264;
+
+template <int CTRL>
+__device__ inline double dpp_double(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+// Sum over the 16 lanes of a row, left in every lane of the row: quad swaps
+// (1 0 3 2), (2 3 0 1), then the half-row and row mirrors (each pair adds
+// the same two values, so every lane forms the same bits).
+__device__ inline double sum_row16(double v) {
+  v += dpp_double<0xB1>(v);
+  v += dpp_double<0x4E>(v);
+  v += dpp_double<0x141>(v);
+  v += dpp_double<0x140>(v);
+  return v;
+}
+
+// s + p[0] + p[stride] + ... + p[(n - 1) stride], added in that order.
+// Each batch of 16 values is read before its adds, so the reads overlap;
+// the adds themselves are one dependent chain.
+__device__ inline double ordered_sum(double s, const double* p, int stride, int n) {
+  int k = 0;
+  for (; k + 16 <= n; k += 16) {
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = p[(k + i) * stride];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s += v[i];
+  }
+  for (; k < n; ++k) s += p[k * stride];
+  return s;
+}
+
+__device__ inline double quadratic_row_sum(const double* row) { return ordered_sum(0.0, row, 1, 512); }
 
 template <int BASIS>
 __device__ inline void contributions(double x, double t, double* e) {
@@ -52,51 +103,30 @@ __device__ inline void contributions(double x, double t, double* e) {
     e[7] = x * t;    // [1][3]
     e[8] = sq * t;   // [2][3]
   } else {
-    double p[9];
-    p[0] = 1.0;
-    for (int i = 1; i < 9; ++i) p[i] = p[i - 1] * x;
-    int k = 0;
-    for (int r = 0; r < 5; ++r)
-      for (int c = r; c < 5; ++c) e[k++] = p[r + c];
-    for (int r = 0; r < 5; ++r) e[15 + r] = p[r] * t;
+    e[0] = 1.0;
+    for (int i = 1; i < 9; ++i) e[i] = e[i - 1] * x;
+    for (int r = 0; r < 5; ++r) e[9 + r] = e[r] * t;
   }
 }
 
+// One exercise date of one lane for the slots of this thread's sample:
+// the exercise decision at the later date (its spot and policy row `pol`),
+// the spot at this date from the slot's coordinate `coord[leg]`, the
+// discounted cash, and the normal-equation contributions of in-the-money
+// slots, summed in the reference's order into this lane's block partials.
 template <bool FP32, int KIND, int BASIS>
-__global__ void __launch_bounds__(256)
-    lsmc_backward(LsmcModel m, int step, const double* __restrict__ pol, int have, int64_t samples,
-                  int64_t paths, int antithetic, uint64_t seed, SobolRef so, void* __restrict__ coords,
-                  double* __restrict__ spot, double* __restrict__ cash, double* __restrict__ partials,
-                  uint32_t* __restrict__ flags) {
+__device__ inline void backward_lane(const LsmcModel& m, int step, const double* pol, int have, int64_t n,
+                                     int legs, const bool (&on)[2], const double (&coord)[2],
+                                     double* __restrict__ spot, double* __restrict__ cash,
+                                     double* __restrict__ partials, double* lds, uint32_t& bad) {
   constexpr int E = Entries<BASIS>::N;
-  extern __shared__ double lds[];  // quadratic: [E][512] in slot order
-  __shared__ double red[E * 4];
-  const int64_t block = blockIdx.x;
   const int j = threadIdx.x;
-  const int64_t n = block * BLOCK_SAMPLES + j;
-  const int legs = antithetic ? 2 : 1;
-  const int64_t slots = samples * legs;
   double acc[E];
   for (int e = 0; e < E; ++e) acc[e] = 0.0;
-  uint32_t bad = 0;
-  // Reverse mode: this date's coordinate from the next date's, in place.
-  double w_rev = 0.0;
-  if (KIND == PATHS_BROWNIAN && m.reverse && n < samples) {
-    const int count = (int)(samples - block * BLOCK_SAMPLES < BLOCK_SAMPLES ? samples - block * BLOCK_SAMPLES
-                                                                            : BLOCK_SAMPLES);
-    const int row = m.steps - 1 - step;
-    const double prev = FP32 ? (double)((const float*)coords)[n] : ((const double*)coords)[n];
-    w_rev = bridge_back<FP32>(prev, reverse_normal<FP32>(seed, block, j, count, row, n, so), step, m.steps);
-    if (FP32)
-      ((float*)coords)[n] = (float)w_rev;
-    else
-      ((double*)coords)[n] = w_rev;
-  }
   for (int leg = 0; leg < 2; ++leg) {
     double e[E];
     for (int k = 0; k < E; ++k) e[k] = 0.0;
-    const bool on = n < samples && (leg == 0 || (antithetic && n < paths / 2));
-    if (on) {
+    if (on[leg]) {
       const int64_t slot = n * legs + leg;
       double c = cash[slot];
       if (have && pol[0] != 0.0) {
@@ -110,16 +140,11 @@ __global__ void __launch_bounds__(256)
       }
       double s;
       if (KIND == PATHS_LOCAL_VOL) {
-        const double x = FP32 ? (double)((const float*)coords)[(size_t)step * slots + slot]
-                              : ((const double*)coords)[(size_t)step * slots + slot];
-        s = m.spot * (FP32 ? (double)f32::exp((float)x) : exp(x));
+        s = m.spot * (FP32 ? (double)f32::exp((float)coord[leg]) : exp(coord[leg]));
       } else if (KIND == PATHS_ANCHORED) {
-        s = ((const double*)coords)[(size_t)step * slots + slot];
+        s = coord[leg];
       } else {
-        const double w = m.reverse ? w_rev
-                         : FP32    ? (double)((const float*)coords)[(size_t)step * samples + n]
-                                   : ((const double*)coords)[(size_t)step * samples + n];
-        s = FP32 ? bsm_spot32(m, step, (float)(leg ? -w : w)) : bsm_spot(m, step, leg ? -w : w);
+        s = FP32 ? bsm_spot32(m, step, (float)coord[leg]) : bsm_spot(m, step, coord[leg]);
       }
       c *= m.discount;
       if (!isfinite(c)) bad = 1;
@@ -128,29 +153,96 @@ __global__ void __launch_bounds__(256)
       if (payoff(m, s) > 0.0) contributions<BASIS>(s / m.strike, c / m.strike, e);
     }
     if (BASIS == KG_BASIS_QUADRATIC) {
-      for (int k = 0; k < E; ++k) lds[k * 512 + 2 * j + leg] = e[k];
+      for (int k = 0; k < E; ++k) lds[k * QUAD_ROW + 2 * j + leg] = e[k];
     } else {
       for (int k = 0; k < E; ++k) acc[k] += e[k];
     }
   }
-  if (bad) raise_flag(flags, FLAG_NONFINITE);
+  const int64_t block = blockIdx.x;
   if (BASIS == KG_BASIS_QUADRATIC) {
     __syncthreads();
-    if (j < E) {  // reference order: samples in block order, a pair's legs together
+    // Reference order: samples in block order, a pair's legs together.
+    if (j < E) partials[block * E + j] = quadratic_row_sum(lds + j * QUAD_ROW);
+  } else {
+    // Row k of LDS holds entry k of the 256 threads; the 16 lanes of row
+    // group k sum its elements p, p + 16, .. and the group's DPP butterfly
+    // leaves the block's sum in every lane of the group.
+    for (int k = 0; k < E; ++k) lds[k * POLY4_ROW + j] = acc[k];
+    __syncthreads();
+    if (j < E * 16) {
+      const int k = j >> 4, p = j & 15;
+      double v[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) v[i] = lds[k * POLY4_ROW + p + 16 * i];
       double sum = 0.0;
-      for (int s = 0; s < 512; ++s) sum += lds[j * 512 + s];
-      partials[block * E + j] = sum;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) sum += v[i];
+      sum = sum_row16(sum);
+      if (p == 0) partials[block * E + k] = sum;
+    }
+  }
+}
+
+// One exercise date of the backward loop. With LANES, `nl` lanes (strikes
+// or market scenarios, american_scenarios::<N, S>) share the coordinates:
+// lane l reads its model from lanes[l] and keeps its spot and cash at
+// l * slots, its partials at l * blocks * E and its policy at l * pol_stride
+// doubles; each lane computes exactly what the lane priced alone computes.
+template <bool FP32, int KIND, int BASIS, bool LANES>
+__global__ void __launch_bounds__(256)
+    lsmc_backward(LsmcModel m, const LsmcModel* __restrict__ lanes, int nl, int step, const double* __restrict__ pol,
+                  int64_t pol_stride, int have, int64_t samples, int64_t paths, int antithetic, uint64_t seed,
+                  SobolRef so, void* __restrict__ coords, double* __restrict__ spot, double* __restrict__ cash,
+                  double* __restrict__ partials, uint32_t* __restrict__ flags) {
+  constexpr int E = Entries<BASIS>::N;
+  __shared__ __align__(16) double lds[BASIS == KG_BASIS_QUADRATIC ? E * QUAD_ROW : E * POLY4_ROW];
+  const int64_t block = blockIdx.x;
+  const int j = threadIdx.x;
+  const int64_t n = block * BLOCK_SAMPLES + j;
+  const int legs = antithetic ? 2 : 1;
+  const int64_t slots = samples * legs;
+  uint32_t bad = 0;
+  bool on[2];
+  double coord[2] = {0.0, 0.0};
+  for (int leg = 0; leg < 2; ++leg) on[leg] = n < samples && (leg == 0 || (antithetic && n < paths / 2));
+  if (KIND == PATHS_BROWNIAN) {
+    double w = 0.0;
+    if (m.reverse && n < samples) {
+      // Reverse mode: this date's coordinate from the next date's, in place.
+      const int count = (int)(samples - block * BLOCK_SAMPLES < BLOCK_SAMPLES ? samples - block * BLOCK_SAMPLES
+                                                                              : BLOCK_SAMPLES);
+      const int row = m.steps - 1 - step;
+      const double prev = FP32 ? (double)((const float*)coords)[n] : ((const double*)coords)[n];
+      w = bridge_back<FP32>(prev, reverse_normal<FP32>(seed, block, j, count, row, n, so), step, m.steps);
+      if (FP32)
+        ((float*)coords)[n] = (float)w;
+      else
+        ((double*)coords)[n] = w;
+    } else if (n < samples) {
+      w = FP32 ? (double)((const float*)coords)[(size_t)step * samples + n]
+               : ((const double*)coords)[(size_t)step * samples + n];
+    }
+    coord[0] = w;
+    coord[1] = -w;
+  } else {
+    for (int leg = 0; leg < 2; ++leg)
+      if (on[leg]) {
+        const size_t at = (size_t)step * slots + n * legs + leg;
+        coord[leg] = KIND == PATHS_LOCAL_VOL && FP32 ? (double)((const float*)coords)[at] : ((const double*)coords)[at];
+      }
+  }
+  if (LANES) {
+    const int64_t blocks = gridDim.x;
+    for (int l = 0; l < nl; ++l) {
+      const LsmcModel ml = lanes[l];
+      backward_lane<FP32, KIND, BASIS>(ml, step, pol + l * pol_stride, have, n, legs, on, coord, spot + l * slots,
+                                       cash + l * slots, partials + l * blocks * E, lds, bad);
+      __syncthreads();  // the next lane rewrites the LDS rows
     }
   } else {
-    const int lane = j & (WAVE - 1), wave = j / WAVE;
-    for (int k = 0; k < E; ++k) {
-      double v = acc[k];
-      for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_down(v, o, WAVE);
-      if (lane == 0) red[k * 4 + wave] = v;
-    }
-    __syncthreads();
-    if (j < E) partials[block * E + j] = ((red[j * 4] + red[j * 4 + 1]) + red[j * 4 + 2]) + red[j * 4 + 3];
+    backward_lane<FP32, KIND, BASIS>(m, step, pol, have, n, legs, on, coord, spot, cash, partials, lds, bad);
   }
+  if (bad) raise_flag(flags, FLAG_NONFINITE);
 }
 
 // solve_quadratic (american.rs:1693-1749).
@@ -204,10 +296,9 @@ __device__ inline bool solve_quadratic_d(const double (&s)[3][4], double* out) {
 // while a pivot falls below 1e-13 of the largest diagonal entry.
 __device__ inline bool solve_poly4_d(const double* e, double* out) {
   double g[5][5], b[5];
-  int k = 0;
   for (int r = 0; r < 5; ++r)
-    for (int c = r; c < 5; ++c) g[r][c] = g[c][r] = e[k++];
-  for (int r = 0; r < 5; ++r) b[r] = e[15 + r];
+    for (int c = 0; c < 5; ++c) g[r][c] = e[r + c];
+  for (int r = 0; r < 5; ++r) b[r] = e[9 + r];
   for (int degree = 4; degree >= 0; --degree) {
     double l[5][5] = {};
     double scale = 0.0;
@@ -253,25 +344,44 @@ __device__ inline bool solve_poly4_d(const double* e, double* out) {
 // as the reference does, streaming them through LDS so the sequential adds
 // wait on LDS rather than on global loads; the poly-4 sums use a fixed
 // tree (thread t folds blocks t, t + 256, ..., then waves and threads
-// combine in a fixed order).
-constexpr int SOLVE_CHUNK = 128;
+// combine in a fixed order). Workgroup l solves lane l: its partials at
+// l * blocks * E, its row at l * pol_stride.
+constexpr int SOLVE_CHUNK = 512;
 
 template <int BASIS>
 __global__ void __launch_bounds__(256)
     lsmc_solve(const double* __restrict__ partials, int64_t blocks, double* __restrict__ pol_row,
-               uint32_t* __restrict__ flags) {
+               int64_t pol_stride, uint32_t* __restrict__ flags) {
   constexpr int E = Entries<BASIS>::N;
   __shared__ double sums[E];
   const int t = threadIdx.x;
+  partials += (int64_t)blockIdx.x * blocks * E;
+  pol_row += (int64_t)blockIdx.x * pol_stride;
   if (BASIS == KG_BASIS_QUADRATIC) {
+    // Chunks of 512 blocks pass through LDS; every thread loads the next
+    // chunk into registers while threads 0..8 add the current one, so the
+    // global loads overlap the sequential adds.
     __shared__ double buf[SOLVE_CHUNK * E];
+    constexpr int PER = SOLVE_CHUNK * E / 256;
+    static_assert(PER * 256 == SOLVE_CHUNK * E, "a chunk is a whole number of loads per thread");
+    double next[PER];
+    auto fetch = [&](int64_t b0) {
+      const int64_t count = (blocks - b0 < SOLVE_CHUNK ? blocks - b0 : SOLVE_CHUNK) * E;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int idx = t + i * 256;
+        next[i] = idx < count ? partials[b0 * E + idx] : 0.0;
+      }
+    };
+    fetch(0);
     double s = 0.0;
     for (int64_t b0 = 0; b0 < blocks; b0 += SOLVE_CHUNK) {
       const int nb = (int)(blocks - b0 < SOLVE_CHUNK ? blocks - b0 : SOLVE_CHUNK);
-      for (int i = t; i < nb * E; i += blockDim.x) buf[i] = partials[b0 * E + i];
+#pragma unroll
+      for (int i = 0; i < PER; ++i) buf[t + i * 256] = next[i];
       __syncthreads();
-      if (t < E)
-        for (int k = 0; k < nb; ++k) s += buf[k * E + t];
+      if (b0 + SOLVE_CHUNK < blocks) fetch(b0 + SOLVE_CHUNK);
+      if (t < E) s = ordered_sum(s, buf + t, E, nb);
       __syncthreads();
     }
     if (t < E) sums[t] = s;
@@ -397,26 +507,34 @@ static SobolRef sobol_ref(kg_context* ctx, const Sampler& s, int steps) {
   return SobolRef{(const uint64_t*)so.d_directions.ptr, (const uint64_t*)so.d_shifts.ptr, steps};
 }
 
+// In-sample LSMC of `nl` lanes on one set of paths: ml[l] is lane l's
+// model (host copy) and d_lanes their device copy, or nullptr for a single
+// lane, which runs the single-lane kernels. values[l] receives lane l's
+// price: max(discounted mean, immediate[l]).
 template <bool FP32, int KIND, int BASIS>
-static double run_lsmc(kg_context* ctx, const LsmcModel& m, const Sampler& s, double* d_policy,
-                       int32_t* dates_fitted, double* immediate_out) {
+static void run_lsmc(kg_context* ctx, const LsmcModel* ml, const LsmcModel* d_lanes, int nl, const Sampler& s,
+                     double* d_policy, int32_t* dates_fitted, const double* immediate, double* values) {
+  const LsmcModel& m = ml[0];
+  const bool lanes = d_lanes != nullptr;
   const int64_t samples = s.samples(), blocks = s.blocks();
   const int legs = s.antithetic ? 2 : 1;
   const size_t word = FP32 ? 4 : 8;
   const bool reverse = KIND == PATHS_BROWNIAN && m.reverse;
   // Stored values: one coordinate per sample in reverse mode, else one per
   // date and sample (Brownian) or per date and slot (local vol, anchored).
-  const size_t values = reverse ? (size_t)samples
-                        : (size_t)m.steps * (size_t)(KIND == PATHS_BROWNIAN ? samples : samples * legs);
+  const size_t values_stored = reverse ? (size_t)samples
+                               : (size_t)m.steps * (size_t)(KIND == PATHS_BROWNIAN ? samples : samples * legs);
   // Rounded up so the FP64 arrays after the coordinates stay aligned.
-  const size_t coord_bytes = (values * word + 255) & ~size_t(255);
+  const size_t coord_bytes = (values_stored * word + 255) & ~size_t(255);
   constexpr int E = Entries<BASIS>::N;
-  unsigned char* base = (unsigned char*)ctx->scratch.reserve(coord_bytes + 2 * (size_t)samples * legs * 8 +
-                                                             (size_t)blocks * E * 8 + 1024);
+  const int64_t slots = samples * legs;
+  unsigned char* base = (unsigned char*)ctx->scratch.reserve(coord_bytes + (size_t)nl * 2 * (size_t)slots * 8 +
+                                                             (size_t)nl * (size_t)blocks * E * 8 + 1024);
   void* coords = base;
   double* spot = (double*)(base + coord_bytes);
-  double* cash = spot + samples * legs;
-  double* partials = cash + samples * legs;
+  double* cash = spot + (size_t)nl * slots;
+  double* partials = cash + (size_t)nl * slots;
+  const int64_t pol_stride = (int64_t)m.steps * 8;
   const SobolRef so = sobol_ref(ctx, s, m.steps);
   uint32_t* flags = reset_flags(ctx);
   if (reverse)
@@ -426,29 +544,46 @@ static double run_lsmc(kg_context* ctx, const LsmcModel& m, const Sampler& s, do
     hipLaunchKernelGGL((lsmc_generate<FP32, KIND>), dim3(blocks), dim3(256), 0, ctx->stream, m, s.seed, samples,
                        s.path_count, (int)s.antithetic, so, coords, spot, cash, flags);
   KG_HIP(hipGetLastError());
-  const size_t lds = BASIS == KG_BASIS_QUADRATIC ? (size_t)E * 512 * 8 : 0;
+  if (lanes) {
+    hipLaunchKernelGGL((lsmc_lane_terminal<FP32, KIND>), dim3(blocks), dim3(256), 0, ctx->stream, d_lanes, nl,
+                       samples, s.path_count, (int)s.antithetic, (int)reverse, (const void*)coords, spot, cash);
+    KG_HIP(hipGetLastError());
+  }
+  const size_t lds = 0;
   int have = 0;
   int fitted = 0;
   for (int step = m.steps - 2; step >= 0; --step) {
-    hipLaunchKernelGGL((lsmc_backward<FP32, KIND, BASIS>), dim3(blocks), dim3(256), lds, ctx->stream, m, step,
-                       d_policy + (size_t)(step + 1) * 8, have, samples, s.path_count, (int)s.antithetic, s.seed,
-                       so, coords, spot, cash, partials, flags);
+    if (lanes)
+      hipLaunchKernelGGL((lsmc_backward<FP32, KIND, BASIS, true>), dim3(blocks), dim3(256), lds, ctx->stream, m,
+                         d_lanes, nl, step, d_policy + (size_t)(step + 1) * 8, pol_stride, have, samples,
+                         s.path_count, (int)s.antithetic, s.seed, so, coords, spot, cash, partials, flags);
+    else
+      hipLaunchKernelGGL((lsmc_backward<FP32, KIND, BASIS, false>), dim3(blocks), dim3(256), lds, ctx->stream, m,
+                         (const LsmcModel*)nullptr, 1, step, d_policy + (size_t)(step + 1) * 8, pol_stride, have,
+                         samples, s.path_count, (int)s.antithetic, s.seed, so, coords, spot, cash, partials, flags);
     KG_HIP(hipGetLastError());
-    hipLaunchKernelGGL((lsmc_solve<BASIS>), dim3(1), dim3(256), 0, ctx->stream, partials, blocks,
-                       d_policy + (size_t)step * 8, flags);
+    hipLaunchKernelGGL((lsmc_solve<BASIS>), dim3(nl), dim3(256), 0, ctx->stream, partials, blocks,
+                       d_policy + (size_t)step * 8, pol_stride, flags);
     KG_HIP(hipGetLastError());
     have = 1;  // the kernels read the valid flag of the row they use
     ++fitted;
   }
-  Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 2 * blocks);
-  hipLaunchKernelGGL((lsmc_final<BASIS>), dim3(blocks), dim3(256), 0, ctx->stream, m, d_policy, 1, samples,
-                     s.path_count, (int)s.antithetic, spot, cash, parts);
+  Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 2 * (size_t)nl * blocks);
+  if (lanes)
+    hipLaunchKernelGGL((lsmc_final<BASIS, true>), dim3(blocks), dim3(256), 0, ctx->stream, m, d_lanes, nl,
+                       (const double*)d_policy, pol_stride, 1, samples, s.path_count, (int)s.antithetic,
+                       (const double*)spot, (const double*)cash, parts);
+  else
+    hipLaunchKernelGGL((lsmc_final<BASIS, false>), dim3(blocks), dim3(256), 0, ctx->stream, m,
+                       (const LsmcModel*)nullptr, 1, (const double*)d_policy, pol_stride, 1, samples, s.path_count,
+                       (int)s.antithetic, (const double*)spot, (const double*)cash, parts);
   KG_HIP(hipGetLastError());
   check_flags(ctx);
-  Moments r[2];
-  merge_and_fetch(ctx, parts, blocks, 2, r);
+  std::vector<Moments> r(2 * (size_t)nl);
+  merge_and_fetch(ctx, parts, blocks, 2 * nl, r.data());
   *dates_fitted = fitted;
-  return std::max(scaled_value(r[0].scale, r[0].mean, m.log_discount), *immediate_out);
+  for (int l = 0; l < nl; ++l)
+    values[l] = std::max(scaled_value(r[2 * l].scale, r[2 * l].mean, ml[l].log_discount), immediate[l]);
 }
 
 template <bool FP32, bool LV, int BASIS>
@@ -470,11 +605,12 @@ static void american(kg_context* ctx, const LsmcModel& m, const Sampler& pricing
                      bool oos, double immediate, kg_lsmc_report* out) {
   constexpr bool LV = KIND == PATHS_LOCAL_VOL;
   double* policy = (double*)ctx->policy.reserve((size_t)m.steps * 8 * sizeof(double));
-  hipLaunchKernelGGL(clear_doubles, dim3(1), dim3(256), 0, ctx->stream, policy, m.steps * 8);
-  KG_HIP(hipGetLastError());
+  clear_policy(ctx, policy, (int64_t)m.steps * 8);
   int32_t fitted = 0;
   timing_begin(ctx);
-  const double in_sample = run_lsmc<FP32, KIND, BASIS>(ctx, m, oos ? training : pricing, policy, &fitted, &immediate);
+  double in_sample = 0.0;
+  run_lsmc<FP32, KIND, BASIS>(ctx, &m, nullptr, 1, oos ? training : pricing, policy, &fitted, &immediate,
+                              &in_sample);
   out->training_ms = timing_end(ctx);
   out->in_sample_price = in_sample;
   out->dates_fitted = fitted;
@@ -509,6 +645,97 @@ static void american_basis(kg_context* ctx, const LsmcHost& h, kg_lsmc_report* o
 #undef KG_AMERICAN
 }
 
+// Lanes that can share one set of stored paths: in-sample polynomial fits
+// at one valuation time, normalised (not anchored) paths, and either BSM
+// Brownian coordinates (each lane its own spot, rates, volatility and
+// strike) or the log-spots of one local-volatility market (strikes differ).
+static bool lanes_share_paths(const std::vector<LsmcHost>& hosts, const kg_american_model* models,
+                              const kg_lsmc_options& opt) {
+  if (hosts.size() < 2 || opt.basis == KG_BASIS_MLP || opt.out_of_sample) return false;
+  const LsmcHost& h0 = hosts[0];
+  if (h0.known) return false;
+  for (size_t l = 0; l < hosts.size(); ++l) {
+    const LsmcHost& h = hosts[l];
+    if (h.known || h.model.anchored || models[l].valuation_time != models[0].valuation_time) return false;
+    if (h.model.lv != h0.model.lv || h.model.reverse != h0.model.reverse) return false;
+    if (h.model.lv) {
+      auto same = [](double a, double b) { return memcmp(&a, &b, sizeof(a)) == 0; };
+      if (!same(h.model.spot, h0.model.spot) || !same(h.model.rate, h0.model.rate) ||
+          !same(h.model.dividend, h0.model.dividend) || h.model.nt != h0.model.nt || h.model.ns != h0.model.ns)
+        return false;
+      const size_t n = h.tables.size() - h.lv_offset;
+      if (n != h0.tables.size() - h0.lv_offset ||
+          memcmp(h.tables.data() + h.lv_offset, h0.tables.data() + h0.lv_offset, n * sizeof(double)) != 0)
+        return false;
+    }
+  }
+  return true;
+}
+
+template <bool FP32, int KIND, int BASIS>
+static void shared_lanes(kg_context* ctx, std::vector<LsmcHost>& hosts, kg_lsmc_report* out) {
+  const int nl = (int)hosts.size();
+  const int S = hosts[0].model.steps;
+  // One upload: every lane's tables (256-byte aligned), then the lane models.
+  std::vector<size_t> offset(nl);
+  size_t doubles = 0;
+  for (int l = 0; l < nl; ++l) {
+    offset[l] = doubles;
+    doubles += (hosts[l].tables.size() + 31) & ~size_t(31);
+  }
+  const size_t bytes = doubles * sizeof(double) + (size_t)nl * sizeof(LsmcModel);
+  unsigned char* d = (unsigned char*)ctx->lsmc_tables.reserve(bytes);
+  std::vector<unsigned char> blob(bytes, 0);
+  std::vector<LsmcModel> ml(nl);
+  std::vector<double> immediate(nl), values(nl);
+  for (int l = 0; l < nl; ++l) {
+    memcpy(blob.data() + offset[l] * sizeof(double), hosts[l].tables.data(), hosts[l].tables.size() * sizeof(double));
+    ml[l] = hosts[l].model;
+    bind_tables(ml[l], (const double*)d + offset[l], hosts[l].lv_offset);
+    immediate[l] = hosts[l].immediate;
+  }
+  memcpy(blob.data() + doubles * sizeof(double), ml.data(), (size_t)nl * sizeof(LsmcModel));
+  upload_async(ctx, d, blob.data(), bytes);
+  const LsmcModel* d_lanes = (const LsmcModel*)(d + doubles * sizeof(double));
+  double* policy = (double*)ctx->policy.reserve((size_t)nl * S * 8 * sizeof(double));
+  clear_policy(ctx, policy, (int64_t)nl * S * 8);
+  int32_t fitted = 0;
+  timing_begin(ctx);
+  run_lsmc<FP32, KIND, BASIS>(ctx, ml.data(), d_lanes, nl, hosts[0].pricing, policy, &fitted, immediate.data(),
+                              values.data());
+  const double ms = timing_end(ctx);
+  for (int l = 0; l < nl; ++l) {
+    out[l].price = out[l].in_sample_price = values[l];
+    out[l].training_ms = ms;
+    out[l].dates_fitted = fitted;
+    out[l].path_steps = (double)hosts[0].pricing.path_count * S;
+  }
+}
+
+template <int BASIS>
+static void shared_lanes_basis(kg_context* ctx, std::vector<LsmcHost>& hosts, kg_lsmc_report* out) {
+  const bool fp32 = hosts[0].pricing.fp32;
+  if (hosts[0].model.lv)
+    fp32 ? shared_lanes<true, PATHS_LOCAL_VOL, BASIS>(ctx, hosts, out)
+         : shared_lanes<false, PATHS_LOCAL_VOL, BASIS>(ctx, hosts, out);
+  else
+    fp32 ? shared_lanes<true, PATHS_BROWNIAN, BASIS>(ctx, hosts, out)
+         : shared_lanes<false, PATHS_BROWNIAN, BASIS>(ctx, hosts, out);
+}
+
+static void american_one(kg_context* ctx, LsmcHost& h, const kg_lsmc_options& options, kg_lsmc_report* out) {
+  if (h.known) {
+    out->price = out->in_sample_price = h.known_value;
+    return;
+  }
+  if (options.basis == KG_BASIS_MLP)
+    american_mlp(ctx, h, options, out);
+  else if (options.basis == KG_BASIS_QUADRATIC)
+    american_basis<KG_BASIS_QUADRATIC>(ctx, h, out);
+  else
+    american_basis<KG_BASIS_POLY4>(ctx, h, out);
+}
+
 }  // namespace kg
 
 using namespace kg;
@@ -520,17 +747,43 @@ extern "C" kg_status kg_american_price(kg_context* ctx, const kg_settings* setti
     if (!settings || !option || !model || !options || !out) fail(KG_INVALID_INPUT, "arguments must not be null");
     *out = kg_lsmc_report{};
     LsmcHost h = lsmc_plan(ctx, *settings, *option, *model, *options);
-    if (h.known) {
-      out->price = out->in_sample_price = h.known_value;
+    american_one(ctx, h, *options, out);
+  });
+}
+
+extern "C" kg_status kg_american_price_lanes(kg_context* ctx, const kg_settings* settings,
+                                             const kg_american_terms* option, const kg_american_model* models,
+                                             const double* strikes, int32_t n_lanes, const kg_lsmc_options* options,
+                                             kg_lsmc_report* out) {
+  return guarded(ctx, [&] {
+    if (!settings || !option || !models || !options || !out) fail(KG_INVALID_INPUT, "arguments must not be null");
+    if (n_lanes < 1 || n_lanes > KG_MAX_AMERICAN_LANES)
+      fail(KG_INVALID_INPUT, "n_lanes must be between 1 and KG_MAX_AMERICAN_LANES");
+    for (int l = 0; l < n_lanes; ++l) out[l] = kg_lsmc_report{};
+    // Every lane is validated before any is priced: a failed request
+    // returns no partial prices.
+    std::vector<LsmcHost> hosts;
+    hosts.reserve(n_lanes);
+    for (int l = 0; l < n_lanes; ++l) {
+      kg_american_terms o = *option;
+      if (strikes) o.strike = strikes[l];
+      hosts.push_back(lsmc_plan_host(*settings, o, models[l], *options));
+    }
+    if (lanes_share_paths(hosts, models, *options)) {
+      if (options->basis == KG_BASIS_QUADRATIC)
+        shared_lanes_basis<KG_BASIS_QUADRATIC>(ctx, hosts, out);
+      else
+        shared_lanes_basis<KG_BASIS_POLY4>(ctx, hosts, out);
       return;
     }
-    if (options->basis == KG_BASIS_MLP) {
-      american_mlp(ctx, h, *options, out);
-      return;
+    std::vector<kg_lsmc_report> priced(n_lanes);
+    for (int l = 0; l < n_lanes; ++l) {
+      check_cancel(ctx);
+      kg_american_terms o = *option;
+      if (strikes) o.strike = strikes[l];
+      LsmcHost h = lsmc_plan(ctx, *settings, o, models[l], *options);
+      american_one(ctx, h, *options, &priced[l]);
     }
-    if (options->basis == KG_BASIS_QUADRATIC)
-      american_basis<KG_BASIS_QUADRATIC>(ctx, h, out);
-    else
-      american_basis<KG_BASIS_POLY4>(ctx, h, out);
+    for (int l = 0; l < n_lanes; ++l) out[l] = priced[l];
   });
 }

@@ -57,10 +57,29 @@ struct LsmcHost {
   Sampler pricing{}, training{};
   bool oos = false;
   double immediate = 0.0;
+  // Host tables before upload: per-date scale, group cutoff and screen
+  // flags, then (local volatility) surface years, spots and values from
+  // lv_offset on. bind_tables points the model at their device copy.
+  std::vector<double> tables;
+  size_t lv_offset = 0;
 };
 
-inline LsmcHost lsmc_plan(kg_context* ctx, const kg_settings& settings, const kg_american_terms& o,
-                          const kg_american_model& mk, const kg_lsmc_options& opt) {
+inline void bind_tables(LsmcModel& m, const double* d, size_t lv_offset) {
+  const int S = m.steps;
+  m.scale = d;
+  m.group_cutoff = d + S;
+  m.screen = (const int32_t*)(d + 2 * S);
+  if (m.lv) {
+    m.years = d + lv_offset;
+    m.spots = m.years + m.nt;
+    m.vols = m.spots + m.ns;
+  }
+}
+
+// Validates one request and builds its model and host tables (nothing on
+// the device yet).
+inline LsmcHost lsmc_plan_host(const kg_settings& settings, const kg_american_terms& o,
+                               const kg_american_model& mk, const kg_lsmc_options& opt) {
   LsmcHost h;
   if (o.option_type != 1 && o.option_type != -1) fail(KG_INVALID_INPUT, "option_type must be 1 or -1");
   if (!(o.strike > 0.0) || !std::isfinite(o.strike)) fail(KG_INVALID_INPUT, "strike must be positive");
@@ -110,7 +129,7 @@ inline LsmcHost lsmc_plan(kg_context* ctx, const kg_settings& settings, const kg
   m.diffusion = mk.volatility * std::sqrt(m.dt);
   m.log_discount = -mk.rate * m.dt;
   m.discount = std::exp(m.log_discount);
-  std::vector<double> tables;
+  std::vector<double>& tables = h.tables;
   const int S = o.step_count;
   tables.resize(2 * (size_t)S + (size_t)S);  // scale, cutoff, screen (as int pairs)
   if (!lv) {
@@ -137,7 +156,7 @@ inline LsmcHost lsmc_plan(kg_context* ctx, const kg_settings& settings, const kg
     }
   }
   const kg_local_vol_surface* sf = mk.surface;
-  size_t lv_off = tables.size();
+  h.lv_offset = tables.size();
   if (lv) {
     if (sf->n_times < 2 || sf->n_spots < 2 || !sf->times || !sf->spots || !sf->values)
       fail(KG_INVALID_INPUT, "a local-volatility surface needs at least two rows and columns");
@@ -167,17 +186,19 @@ inline LsmcHost lsmc_plan(kg_context* ctx, const kg_settings& settings, const kg
     for (int i = 0; i < sf->n_spots; ++i) tables.push_back(sf->spots[i]);
     for (int i = 0; i < sf->n_times * sf->n_spots; ++i) tables.push_back(sf->values[i]);
   }
-  double* d = (double*)ctx->lsmc_tables.reserve(std::max<size_t>(tables.size(), 1) * sizeof(double));
-  KG_HIP(hipMemcpyAsync(d, tables.data(), tables.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  return h;
+}
+
+// lsmc_plan_host with its tables uploaded.
+inline LsmcHost lsmc_plan(kg_context* ctx, const kg_settings& settings, const kg_american_terms& o,
+                          const kg_american_model& mk, const kg_lsmc_options& opt) {
+  LsmcHost h = lsmc_plan_host(settings, o, mk, opt);
+  if (h.known) return h;
+  double* d = (double*)ctx->lsmc_tables.reserve(std::max<size_t>(h.tables.size(), 1) * sizeof(double));
+  KG_HIP(hipMemcpyAsync(d, h.tables.data(), h.tables.size() * sizeof(double), hipMemcpyHostToDevice,
+                        ctx->stream));
   KG_HIP(hipStreamSynchronize(ctx->stream));
-  m.scale = d;
-  m.group_cutoff = d + S;
-  m.screen = (const int32_t*)(d + 2 * S);
-  if (lv) {
-    m.years = d + lv_off;
-    m.spots = m.years + sf->n_times;
-    m.vols = m.spots + sf->n_spots;
-  }
+  bind_tables(h.model, d, h.lv_offset);
   return h;
 }
 

@@ -395,17 +395,14 @@ __global__ void __launch_bounds__(256)
 }
 
 // Final exercise at the first date and the mean of the path cash in slot
-// order (american.rs:1437-1454).
+// order (american.rs:1437-1454), for one lane: its moments go to
+// out[0] (block values) and out[1] (unused, zero).
 template <int BASIS>
-__global__ void __launch_bounds__(256)
-    lsmc_final(LsmcModel m, const double* __restrict__ pol, int have, int64_t samples, int64_t paths,
-               int antithetic, const double* __restrict__ spot, const double* __restrict__ cash,
-               Moments* __restrict__ parts) {
-  __shared__ double vals[512];
-  __shared__ Moments smem[4];
-  const int64_t block = blockIdx.x;
+__device__ inline void final_lane(const LsmcModel& m, const double* pol, int have, int64_t n, int64_t samples,
+                                  int64_t paths, int antithetic, const double* __restrict__ spot,
+                                  const double* __restrict__ cash, double* vals, Moments* smem,
+                                  Moments* __restrict__ out) {
   const int j = threadIdx.x;
-  const int64_t n = block * BLOCK_SAMPLES + j;
   const int legs = antithetic ? 2 : 1;
   for (int leg = 0; leg < 2; ++leg) {
     double v = NAN;
@@ -429,8 +426,67 @@ __global__ void __launch_bounds__(256)
       if (!isnan(vals[s])) moments_add(acc, vals[s], true);
     acc = wave_merge(acc);
     if (j == 0) {
-      parts[block * 2] = acc;
-      parts[block * 2 + 1] = moments_zero();
+      out[0] = acc;
+      out[1] = moments_zero();
+    }
+  }
+  __syncthreads();  // vals is rewritten by the next lane
+}
+
+// One workgroup per sample block. With LANES, lane l reads its model from
+// lanes[l], its policy row 0 at l * pol_stride and its spot and cash at
+// l * slots; block b writes lane l's moments to parts[(b * nl + l) * 2].
+template <int BASIS, bool LANES>
+__global__ void __launch_bounds__(256)
+    lsmc_final(LsmcModel m, const LsmcModel* __restrict__ lanes, int nl, const double* __restrict__ pol,
+               int64_t pol_stride, int have, int64_t samples, int64_t paths, int antithetic,
+               const double* __restrict__ spot, const double* __restrict__ cash, Moments* __restrict__ parts) {
+  __shared__ double vals[512];
+  __shared__ Moments smem[4];
+  const int64_t block = blockIdx.x;
+  const int64_t n = block * BLOCK_SAMPLES + threadIdx.x;
+  if (LANES) {
+    const int64_t slots = samples * (antithetic ? 2 : 1);
+    for (int l = 0; l < nl; ++l) {
+      const LsmcModel ml = lanes[l];
+      final_lane<BASIS>(ml, pol + l * pol_stride, have, n, samples, paths, antithetic, spot + l * slots,
+                        cash + l * slots, vals, smem, parts + (block * nl + l) * 2);
+    }
+  } else {
+    final_lane<BASIS>(m, pol, have, n, samples, paths, antithetic, spot, cash, vals, smem, parts + block * 2);
+  }
+}
+
+// Lanes 1 .. nl - 1 of a shared-path request: each slot's terminal spot and
+// payoff from the stored coordinates, as lsmc_generate and
+// lsmc_reverse_start form them for lane 0.
+template <bool FP32, int KIND>
+__global__ void __launch_bounds__(256)
+    lsmc_lane_terminal(const LsmcModel* __restrict__ lanes, int nl, int64_t samples, int64_t paths, int antithetic,
+                       int reverse, const void* __restrict__ coords, double* __restrict__ spot,
+                       double* __restrict__ cash) {
+  const int64_t n = (int64_t)blockIdx.x * BLOCK_SAMPLES + threadIdx.x;
+  if (n >= samples) return;
+  const bool paired = antithetic && n < paths / 2;
+  const int legs = antithetic ? 2 : 1;
+  const int64_t slots = samples * legs;
+  for (int l = 1; l < nl; ++l) {
+    const LsmcModel m = lanes[l];
+    const int last = m.steps - 1;
+    for (int leg = 0; leg < (paired ? 2 : 1); ++leg) {
+      const int64_t slot = n * legs + leg;
+      double s;
+      if (KIND == PATHS_LOCAL_VOL) {
+        const size_t at = (size_t)last * slots + slot;
+        const double x = FP32 ? (double)((const float*)coords)[at] : ((const double*)coords)[at];
+        s = m.spot * exp(x);
+      } else {
+        const size_t at = reverse ? (size_t)n : (size_t)last * samples + n;
+        const double w = FP32 ? (double)((const float*)coords)[at] : ((const double*)coords)[at];
+        s = FP32 ? bsm_spot32(m, last, (float)(leg ? -w : w)) : bsm_spot(m, last, leg ? -w : w);
+      }
+      spot[l * slots + slot] = s;
+      cash[l * slots + slot] = payoff(m, s);
     }
   }
 }

@@ -862,8 +862,8 @@ static void run(kg_context* ctx, LsmcHost& h, const kg_lsmc_options& opt, kg_lsm
   std::vector<float> losses(S, 0.0f);
   KG_HIP(hipMemcpyAsync(losses.data(), d_losses, S * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 2 * blocks);
-  hipLaunchKernelGGL((lsmc_final<KG_BASIS_QUADRATIC>), dim3(blocks), dim3(256), 0, ctx->stream, m, nullptr, 0,
-                     samples, tr.path_count, (int)tr.antithetic, spot, cash, parts);
+  hipLaunchKernelGGL((lsmc_final<KG_BASIS_QUADRATIC, false>), dim3(blocks), dim3(256), 0, ctx->stream, m, nullptr,
+                     1, nullptr, 0, 0, samples, tr.path_count, (int)tr.antithetic, spot, cash, parts);
   KG_HIP(hipGetLastError());
   check_flags(ctx);
   Moments r[2];

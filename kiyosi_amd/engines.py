@@ -153,9 +153,16 @@ class MonteCarloVanillaEngine(_MonteCarloEngine):
         values equal numerical_greeks(engine, ...) with the same seed."""
         from . import greeks
         from .instruments import AmericanOption
-        # American options (a subclass of EuropeanOption here) price every
-        # stencil scenario with the exercise-grid estimator.
-        if not isinstance(option, EuropeanOption) or isinstance(option, AmericanOption):
+        if isinstance(option, AmericanOption):
+            # The exercise-grid estimator in every stencil scenario; the
+            # scenarios at one valuation time are lanes on shared paths
+            # (greek_prices / greek_scenario_group, american.rs:775-931).
+            engine = self._resolved()
+
+            def lanes(scenarios):
+                return [(p, engine.seed) for p in engine._american_lanes(option, scenarios)]
+            return _batched_greeks(engine, option, context, request, settings, _by_valuation_time, lanes)
+        if not isinstance(option, EuropeanOption):
             return greeks.numerical_greeks(self, option, context, request, settings)
         engine = self._resolved()
 
@@ -184,12 +191,31 @@ class MonteCarloVanillaEngine(_MonteCarloEngine):
                  C.byref(report), interruptible=True)
         return report
 
+    def _american_lanes(self, option, contexts, strikes=None):
+        """In-sample American prices of `option` in each context (at
+        strikes[i] when given) through kg_american_price_lanes: lanes at one
+        valuation time share one set of paths, and each price equals
+        ``price`` of that lane alone."""
+        from .market import american_model
+        engine = self._resolved()
+        ctx = _native.context(engine.device)
+        n = len(contexts)
+        kept = [american_model(c) for c in contexts]  # the structs keep their surfaces alive
+        models = (_abi.AmericanModel * n)(*kept)
+        ks = (C.c_double * n)(*strikes) if strikes is not None else None
+        reports = (_abi.LsmcReport * n)()
+        options = _abi.LsmcOptions(_abi.BASIS_QUADRATIC, 0, 0, 4, 65536, 2e-3)
+        ctx.call(_native.lib().kg_american_price_lanes, C.byref(engine._settings()),
+                 C.byref(option._american_terms(engine.step_count)), models, ks, n, C.byref(options), reports,
+                 interruptible=True)
+        return [r.price for r in reports]
+
     def price_strikes(self, instrument, context, *, strikes):
         """American prices at each strike on one resolved seed, with a separate
         exercise rule fitted per distinct strike (MonteCarloVanillaEngine::
-        price_strikes, american.rs:677-709): each equals ``price`` of that
-        strike's option. Distinct strikes are priced once each on the GPU."""
-        import copy
+        price_strikes, american.rs:677-771): each equals ``price`` of that
+        strike's option. The distinct strikes are lanes on one set of paths,
+        up to MAX_AMERICAN_LANES per launch."""
         import math
         from .instruments import AmericanOption
         if not isinstance(instrument, AmericanOption):
@@ -199,11 +225,11 @@ class MonteCarloVanillaEngine(_MonteCarloEngine):
             if not (math.isfinite(k) and k > 0.0):
                 raise InvalidInputError("strikes must be positive and finite")
         engine = self._resolved()
+        distinct = sorted(set(values))
         quotes = {}
-        for k in sorted(set(values)):
-            option = copy.copy(instrument)
-            option.strike = k
-            quotes[k] = engine.price(option, context)
+        for i in range(0, len(distinct), _abi.MAX_AMERICAN_LANES):
+            chunk = distinct[i:i + _abi.MAX_AMERICAN_LANES]
+            quotes.update(zip(chunk, engine._american_lanes(instrument, [context] * len(chunk), chunk)))
         return tuple(quotes[k] for k in values)
 
     def price_american_out_of_sample(self, option, context, *, training_path_count, basis="quadratic"):
@@ -419,6 +445,17 @@ _BOOK_KINDS = {
     "MonteCarloAutocallableEngine": _abi.BOOK_AUTOCALL,
     "MonteCarloAccumulatorEngine": _abi.BOOK_ACCUMULATOR,
 }
+
+
+def _by_valuation_time(scenarios):
+    """Stencil scenarios grouped by valuation time, at most
+    MAX_AMERICAN_LANES per group."""
+    from .market import to_micros
+    groups = {}
+    for scn in scenarios:
+        groups.setdefault(to_micros(scn.valuation_time), []).append(scn)
+    return [g[i:i + _abi.MAX_AMERICAN_LANES] for g in groups.values()
+            for i in range(0, len(g), _abi.MAX_AMERICAN_LANES)]
 
 
 def _batched_greeks(engine, option, context, request, settings, groups, batch):

@@ -704,6 +704,12 @@ class AmericanOption(EuropeanOption):
     """American call or put (instruments/vanilla.rs), exercisable on the
     engine's exercise grid."""
 
+    @property
+    def valuation_start(self):
+        """No valuation before the effective time (OptionContract's default,
+        instruments/vanilla.rs:141-143, which AmericanOption keeps)."""
+        return self.effective_time
+
     def _american_terms(self, step_count):
         return _abi.AmericanTerms(1 if self.option_type == "call" else -1, step_count, self.strike,
                                   to_micros(self.effective_time), to_micros(self.expiry_time))

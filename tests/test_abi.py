@@ -41,3 +41,12 @@ def test_abi_struct_sizes_match_the_header():
     assert ctypes.sizeof(_abi.BsmMarket) == 40
     assert ctypes.sizeof(_abi.EuropeanTerms) == 32
     assert ctypes.sizeof(_abi.AutocallEvent) == 40
+
+
+def test_header_limits_match_the_python_binding():
+    from kiyosi_amd import _abi
+    header = (ROOT / "include" / "kiyosi_amd.h").read_text()
+    limits = dict(re.findall(r"#define KG_(MAX_[A-Z_]+) (\d+)", header))
+    assert limits, "no KG_MAX_* limits found"
+    for name, value in limits.items():
+        assert getattr(_abi, name) == int(value), name
