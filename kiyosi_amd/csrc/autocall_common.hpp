@@ -246,6 +246,25 @@ struct Draws {
     else
       return normal64(d);
   }
+  // The lane's next two normals (dimensions d and d + 1). PCG FP32: every
+  // lane draws one Box-Muller pair on every call, without a branch, and
+  // merges it with the zero, one or two normals it holds queued, so the
+  // queue keeps its length and the values come in stream order.
+  template <typename R>
+  __device__ inline void normal_pair(int d, R& a, R& b) {
+    if constexpr (FP32 && !SOBOL) {
+      float e, o;
+      f32::box_muller_pair(g32, &e, &o);
+      const bool none = avail32 == 0, one = avail32 == 1, two = avail32 == 2;
+      a = none ? e : q32[0];
+      b = none ? o : (one ? e : q32[1]);
+      q32[0] = one ? o : (two ? e : q32[0]);
+      q32[1] = two ? o : q32[1];
+    } else {
+      a = normal<R>(d);
+      b = normal<R>(d + 1);
+    }
+  }
   __device__ inline void skip(int count) {
     if (SOBOL || count <= 0) return;
     if (FP32) {

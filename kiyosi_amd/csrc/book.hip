@@ -4,9 +4,10 @@
 //
 // Every contract becomes a BookItem: a plan built on the host (restating the
 // reference engine's own plan building, cited per builder below) plus its
-// sampler settings. One persistent kernel pulls (contract, 256-sample block)
-// tiles from an atomic queue, longest contracts first, and writes one
-// partial per tile; a second kernel merges each contract's partials in block
+// sampler settings. The book's tiles (contract, 256-sample block) are grouped
+// by path class; each class has its own persistent kernel, which pulls the
+// class's tiles from an atomic queue, longest contracts first, and writes one
+// partial per tile; a last kernel merges each contract's partials in block
 // order. Tile identity fixes the stream of every sample (PCG stream = block
 // index of the contract, Sobol point = sample index), so results do not
 // depend on scheduling. The single-contract entry points run a book of one.
@@ -14,8 +15,8 @@
 // Within a tile the 256 samples are spread over 256/spt threads (spt =
 // samples per thread); per-sample values go through LDS and are folded by
 // the first wave in sample order, so the tile shape chosen by the autotuner
-// does not change the arithmetic. The reset/book/merge sequence is captured
-// once into a hipGraph and replayed.
+// does not change the arithmetic. The reset, class kernels and merge are
+// captured into one hipGraph and replayed.
 #include <algorithm>
 #include <numeric>
 
@@ -44,8 +45,40 @@ struct BookItem {
   } u;
 };
 
+// Path classes of the persistent kernel. Each class has a kernel of its own,
+// so a kernel's register count is that of one path loop, not of the union of
+// all of them.
+enum : int {
+  BC_BARRIER = 0,
+  BC_DIGITAL,
+  BC_GEO_ASIAN,
+  BC_ARITH_ASIAN,
+  BC_AUTOCALL_STEP,
+  BC_AUTOCALL_BRIDGE,
+  BC_ACCUM,
+  BC_COUNT
+};
+static_assert(BC_COUNT == BOOK_CLASSES, "one tuned shape per path class");
+
+__host__ __device__ inline int book_class(const BookItem& it) {
+  switch (it.kind) {
+    case IK_BARRIER:
+      return BC_BARRIER;
+    case IK_DIGITAL:
+      return BC_DIGITAL;
+    case IK_GEO_ASIAN:
+      return BC_GEO_ASIAN;
+    case IK_ARITH_ASIAN:
+      return BC_ARITH_ASIAN;
+    case IK_AUTOCALL:
+      return it.u.ac.bridge ? BC_AUTOCALL_BRIDGE : BC_AUTOCALL_STEP;
+    default:
+      return BC_ACCUM;
+  }
+}
+
 // ------------------------------------------------------------------ device
-template <bool FP32, bool SOBOL>
+template <bool FP32, bool SOBOL, int CLS>
 __device__ __attribute__((always_inline)) void sample_body(const BookItem& it, int64_t block, int j, double* pos,
                                                            double* neg, bool* live_out, bool* paired_out,
                                                            uint32_t& bad) {
@@ -68,21 +101,20 @@ __device__ __attribute__((always_inline)) void sample_body(const BookItem& it, i
   }
   const auto origin = dr;
   auto run = [&](bool on, float sign) -> double {
-    switch (it.kind) {
-      case IK_BARRIER:
-        return barrier_path<FP32, false, false>(it.u.b, dr, on, sign, bad);
-      case IK_DIGITAL:
-        return digital_path<FP32, false>(it.u.dg, dr, on, sign, bad);
-      case IK_GEO_ASIAN:
-        return geo_asian<FP32>(it.u.g, dr, on, sign, bad);
-      case IK_ARITH_ASIAN:
-        return arith_asian<FP32>(it.u.a, dr, on, sign, bad);
-      case IK_AUTOCALL:
-        return it.u.ac.bridge ? bridge_path<SOBOL, FP32>(it.u.ac, dr, on, sign, bad)
-                              : step_path<SOBOL, FP32>(it.u.ac, dr, on, sign, bad);
-      default:
-        return accum_path<FP32>(it.u.acc, dr, on, sign, bad);
-    }
+    if constexpr (CLS == BC_BARRIER)
+      return barrier_path<FP32, false, false>(it.u.b, dr, on, sign, bad);
+    else if constexpr (CLS == BC_DIGITAL)
+      return digital_path<FP32, false>(it.u.dg, dr, on, sign, bad);
+    else if constexpr (CLS == BC_GEO_ASIAN)
+      return geo_asian<FP32>(it.u.g, dr, on, sign, bad);
+    else if constexpr (CLS == BC_ARITH_ASIAN)
+      return arith_asian<FP32>(it.u.a, dr, on, sign, bad);
+    else if constexpr (CLS == BC_AUTOCALL_BRIDGE)
+      return bridge_path<SOBOL, FP32>(it.u.ac, dr, on, sign, bad);
+    else if constexpr (CLS == BC_AUTOCALL_STEP)
+      return step_path<SOBOL, FP32>(it.u.ac, dr, on, sign, bad);
+    else
+      return accum_path<FP32>(it.u.acc, dr, on, sign, bad);
   };
   *pos = run(live, 1.0f);
   *neg = 0.0;
@@ -92,25 +124,6 @@ __device__ __attribute__((always_inline)) void sample_body(const BookItem& it, i
   }
   *live_out = live;
   *paired_out = paired;
-}
-
-// The FP32 kernel inlines the sample body, so the uniform contract fields
-// and step tables are read with scalar loads. The FP64 body stays a call:
-// inlined, its register demand would cut the kernel's occupancy.
-template <bool SOBOL>
-__device__ __attribute__((noinline)) void sample_fp64(const BookItem& it, int64_t block, int j, double* pos,
-                                                      double* neg, bool* live_out, bool* paired_out, uint32_t& bad) {
-  sample_body<false, SOBOL>(it, block, j, pos, neg, live_out, paired_out, bad);
-}
-
-template <bool FP32, bool SOBOL>
-__device__ __attribute__((always_inline)) void book_sample(const BookItem& it, int64_t block, int j, double* pos,
-                                                           double* neg, bool* live_out, bool* paired_out,
-                                                           uint32_t& bad) {
-  if constexpr (FP32)
-    sample_body<true, SOBOL>(it, block, j, pos, neg, live_out, paired_out, bad);
-  else
-    sample_fp64<SOBOL>(it, block, j, pos, neg, live_out, paired_out, bad);
 }
 
 // Moments of one block's 256 sample values staged in LDS (pos, neg and the
@@ -134,8 +147,10 @@ __device__ inline void fold_block(const double* pos, const double* neg, const un
   *pairs = wave_merge(p);
 }
 
-// Two waves per SIMD: at most 256 VGPRs in either precision.
-template <bool FP32, int SPT>
+// One persistent kernel per path class. The uniform contract fields and step
+// tables are read with scalar loads (the tile index is wave-uniform). At
+// least two waves per SIMD: at most 256 VGPRs in either precision.
+template <bool FP32, int SPT, int CLS>
 __global__ void __launch_bounds__(256, 2)
     book_kernel(const BookItem* __restrict__ items, const int2* __restrict__ tiles, int n_tiles,
                 int* __restrict__ counter, Moments* __restrict__ partials,
@@ -168,9 +183,9 @@ __global__ void __launch_bounds__(256, 2)
       double pos, neg;
       bool live, paired;
       if (it.sobol)
-        book_sample<FP32, true>(it, tb.y, j, &pos, &neg, &live, &paired, bad);
+        sample_body<FP32, true, CLS>(it, tb.y, j, &pos, &neg, &live, &paired, bad);
       else
-        book_sample<FP32, false>(it, tb.y, j, &pos, &neg, &live, &paired, bad);
+        sample_body<FP32, false, CLS>(it, tb.y, j, &pos, &neg, &live, &paired, bad);
       s_pos[j] = pos;
       s_neg[j] = neg;
       s_state[j] = (unsigned char)(live | (paired << 1));
@@ -207,9 +222,9 @@ __global__ void __launch_bounds__(MERGE_THREADS)
   }
 }
 
-__global__ void book_reset(int* counter, uint32_t* flags) {
-  *counter = 0;
-  *flags = 0u;
+__global__ void book_reset(int* counters, uint32_t* flags) {
+  if (threadIdx.x < BC_COUNT) counters[threadIdx.x] = 0;
+  if (threadIdx.x == 0) *flags = 0u;
 }
 
 // Value and likelihood-ratio delta/gamma of one barrier or digital contract
@@ -252,7 +267,7 @@ __global__ void __launch_bounds__(256)
         out[0] = geo_asian<FP32, true>(it.u.g, dr, on, sign, bad, out + 1);
         break;
       case IK_ARITH_ASIAN: {
-        const ArithPath a = arith_path<FP32>(it.u.a, dr, on, sign);
+        const ArithPath a = arith_path<FP32, false>(it.u.a, dr, on, sign);
         out[0] = out[1] = out[2] = 0.0;
         if (on) out[0] = arith_payoff_greeks(it.u.a, a, bad, out + 1);
         break;
@@ -1099,70 +1114,129 @@ static int cu_count(kg_context* ctx) {
   return std::max(cus, 1);
 }
 
-template <bool FP32>
-static void launch_book(kg_context* ctx, int spt, int wpc, const BookItem* items, const int2* tiles,
-                        int n_tiles, int* counter, Moments* parts, uint32_t* flags) {
+// The tiles of one path class: a contiguous run of the tile list, and the
+// tile shape its kernel runs with.
+struct ClassRun {
+  int begin = 0, count = 0, spt = 0, wpc = 0;
+};
+
+template <bool FP32, int CLS>
+static void launch_class(kg_context* ctx, int spt, int grid, const BookItem* items, const int2* tiles,
+                         int n_tiles, int* counter, Moments* parts, uint32_t* flags) {
   const int32_t* cancel = cancel_word(ctx);
-  const int threads = BLOCK_SAMPLES / spt;
-  const int grid = std::max(1, std::min(n_tiles, cu_count(ctx) * wpc));
+  const dim3 threads(BLOCK_SAMPLES / spt);
   switch (spt) {
     case 1:
-      hipLaunchKernelGGL((book_kernel<FP32, 1>), dim3(grid), dim3(threads), 0, ctx->stream, items, tiles,
-                         n_tiles, counter, parts, flags, cancel);
+      hipLaunchKernelGGL((book_kernel<FP32, 1, CLS>), dim3(grid), threads, 0, ctx->stream, items, tiles, n_tiles,
+                         counter, parts, flags, cancel);
       break;
     case 2:
-      hipLaunchKernelGGL((book_kernel<FP32, 2>), dim3(grid), dim3(threads), 0, ctx->stream, items, tiles,
-                         n_tiles, counter, parts, flags, cancel);
+      hipLaunchKernelGGL((book_kernel<FP32, 2, CLS>), dim3(grid), threads, 0, ctx->stream, items, tiles, n_tiles,
+                         counter, parts, flags, cancel);
       break;
     default:
-      hipLaunchKernelGGL((book_kernel<FP32, 4>), dim3(grid), dim3(threads), 0, ctx->stream, items, tiles,
-                         n_tiles, counter, parts, flags, cancel);
+      hipLaunchKernelGGL((book_kernel<FP32, 4, CLS>), dim3(grid), threads, 0, ctx->stream, items, tiles, n_tiles,
+                         counter, parts, flags, cancel);
       break;
   }
   KG_HIP(hipGetLastError());
 }
 
-static void run_book_once(kg_context* ctx, bool fp32, int spt, int wpc, const BookItem* items,
-                          const int2* tiles, int n_tiles, int* counter, Moments* parts,
-                          uint32_t* flags) {
-  hipLaunchKernelGGL(book_reset, dim3(1), dim3(1), 0, ctx->stream, counter, flags);
-  KG_HIP(hipGetLastError());
-  if (fp32)
-    launch_book<true>(ctx, spt, wpc, items, tiles, n_tiles, counter, parts, flags);
-  else
-    launch_book<false>(ctx, spt, wpc, items, tiles, n_tiles, counter, parts, flags);
+// Launches the persistent kernel of class `cls` over `n_tiles` tiles,
+// pulling them through counters[cls].
+template <bool FP32>
+static void launch_book(kg_context* ctx, int cls, int spt, int wpc, const BookItem* items, const int2* tiles,
+                        int n_tiles, int* counters, Moments* parts, uint32_t* flags) {
+  const int grid = std::max(1, std::min(n_tiles, cu_count(ctx) * wpc));
+  int* counter = counters + cls;
+  switch (cls) {
+    case BC_BARRIER:
+      launch_class<FP32, BC_BARRIER>(ctx, spt, grid, items, tiles, n_tiles, counter, parts, flags);
+      break;
+    case BC_DIGITAL:
+      launch_class<FP32, BC_DIGITAL>(ctx, spt, grid, items, tiles, n_tiles, counter, parts, flags);
+      break;
+    case BC_GEO_ASIAN:
+      launch_class<FP32, BC_GEO_ASIAN>(ctx, spt, grid, items, tiles, n_tiles, counter, parts, flags);
+      break;
+    case BC_ARITH_ASIAN:
+      launch_class<FP32, BC_ARITH_ASIAN>(ctx, spt, grid, items, tiles, n_tiles, counter, parts, flags);
+      break;
+    case BC_AUTOCALL_STEP:
+      launch_class<FP32, BC_AUTOCALL_STEP>(ctx, spt, grid, items, tiles, n_tiles, counter, parts, flags);
+      break;
+    case BC_AUTOCALL_BRIDGE:
+      launch_class<FP32, BC_AUTOCALL_BRIDGE>(ctx, spt, grid, items, tiles, n_tiles, counter, parts, flags);
+      break;
+    default:
+      launch_class<FP32, BC_ACCUM>(ctx, spt, grid, items, tiles, n_tiles, counter, parts, flags);
+      break;
+  }
 }
 
-// Times each tile shape on a strided eighth of the tiles and keeps the
-// fastest (samples per thread x workgroups per CU).
-static void autotune(kg_context* ctx, bool fp32, const BookItem* items,
-                     const std::vector<int2>& tiles, int* counter, Moments* parts, uint32_t* flags) {
+// One book execution: the counter reset, then each class's persistent
+// kernel over its run of tiles, in class order.
+static void run_book_once(kg_context* ctx, bool fp32, const ClassRun* runs, const BookItem* items,
+                          const int2* tiles, int* counters, Moments* parts, uint32_t* flags) {
+  hipLaunchKernelGGL(book_reset, dim3(1), dim3(64), 0, ctx->stream, counters, flags);
+  KG_HIP(hipGetLastError());
+  for (int c = 0; c < BC_COUNT; ++c) {
+    const ClassRun& r = runs[c];
+    if (r.count == 0) continue;
+    if (fp32)
+      launch_book<true>(ctx, c, r.spt, r.wpc, items, tiles + r.begin, r.count, counters, parts, flags);
+    else
+      launch_book<false>(ctx, c, r.spt, r.wpc, items, tiles + r.begin, r.count, counters, parts, flags);
+  }
+}
+
+// Tile shape of a class when it has too few tiles to time: one wave per
+// workgroup, eight workgroups per CU.
+constexpr int DEFAULT_SPT = 4, DEFAULT_WPC = 8;
+constexpr size_t TUNE_MIN_TILES = 4096;
+
+// Times each tile shape (samples per thread x workgroups per CU) on a strided
+// sample of one class's tiles and keeps the fastest for that class and
+// precision. Classes with fewer than TUNE_MIN_TILES tiles run the default
+// shape and leave the choice open for a larger book.
+static void autotune(kg_context* ctx, bool fp32, int cls, const BookItem* items, const int2* tiles, size_t n_tiles,
+                     int* counters, Moments* parts, uint32_t* flags, int* spt_out, int* wpc_out) {
   const int slot = fp32 ? 1 : 0;
-  if (ctx->tuned_spt[slot]) return;
+  if (ctx->tuned_spt[slot][cls]) {
+    *spt_out = ctx->tuned_spt[slot][cls];
+    *wpc_out = ctx->tuned_wpc[slot][cls];
+    return;
+  }
+  *spt_out = DEFAULT_SPT;
+  *wpc_out = DEFAULT_WPC;
+  if (n_tiles < TUNE_MIN_TILES) return;
   std::vector<int2> sample;
-  const int stride = tiles.size() >= 8192 ? 8 : 1;
-  for (size_t i = 0; i < tiles.size(); i += stride) sample.push_back(tiles[i]);
+  const size_t stride = n_tiles >= 8 * TUNE_MIN_TILES ? 8 : (n_tiles >= 2 * TUNE_MIN_TILES ? 2 : 1);
+  for (size_t i = 0; i < n_tiles; i += stride) sample.push_back(tiles[i]);
   int2* d_sample = (int2*)ctx->scratch.reserve(sample.size() * sizeof(int2));
   KG_HIP(hipMemcpyAsync(d_sample, sample.data(), sample.size() * sizeof(int2), hipMemcpyHostToDevice,
                         ctx->stream));
+  ClassRun runs[BC_COUNT];
+  runs[cls].count = (int)sample.size();
   double best = 1e300;
-  int best_spt = 1, best_wpc = 4;
   for (int spt : {1, 2, 4})
-    for (int wpc : {2, 4, 8}) {
+    for (int wpc : {2, 4, 8, 16}) {
       check_cancel(ctx);
-      run_book_once(ctx, fp32, spt, wpc, items, d_sample, (int)sample.size(), counter, parts, flags);
+      runs[cls].spt = spt;
+      runs[cls].wpc = wpc;
+      run_book_once(ctx, fp32, runs, items, d_sample, counters, parts, flags);
       timing_begin(ctx);
-      run_book_once(ctx, fp32, spt, wpc, items, d_sample, (int)sample.size(), counter, parts, flags);
+      run_book_once(ctx, fp32, runs, items, d_sample, counters, parts, flags);
       const double ms = timing_end(ctx);
       if (ms < best) {
         best = ms;
-        best_spt = spt;
-        best_wpc = wpc;
+        *spt_out = spt;
+        *wpc_out = wpc;
       }
     }
   check_flags(ctx);
-  ctx->tuned_spt[slot] = best_spt;
-  ctx->tuned_wpc[slot] = best_wpc;
+  ctx->tuned_spt[slot][cls] = *spt_out;
+  ctx->tuned_wpc[slot][cls] = *wpc_out;
 }
 
 // Every table a kernel dereferences must point into the uploaded blob.
@@ -1281,7 +1355,8 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
       check_tables(p.item);
       items.push_back(p.item);
     }
-    // Tiles, most expensive contracts first (LPT order for the queue).
+    // Tiles grouped by path class; within a class, most expensive contracts
+    // first (LPT order for the class's queue).
     std::vector<int2> tiles;
     tiles.reserve((size_t)total_blocks);
     for (size_t k = 0; k < items.size(); ++k) {
@@ -1289,13 +1364,20 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
       for (int64_t b = 0; b < blocks; ++b) tiles.push_back(make_int2((int)k, (int)b));
     }
     std::stable_sort(tiles.begin(), tiles.end(), [&](const int2& x, const int2& y) {
+      const int cx = book_class(items[x.x]), cy = book_class(items[y.x]);
+      if (cx != cy) return cx < cy;
       return items[x.x].dims > items[y.x].dims;
     });
+    ClassRun runs[BC_COUNT];
+    for (size_t t = 0; t < tiles.size(); ++t) {
+      ClassRun& r = runs[book_class(items[tiles[t].x])];
+      if (r.count++ == 0) r.begin = (int)t;
+    }
     BookItem* d_items = (BookItem*)ctx->book_items.reserve(items.size() * sizeof(BookItem));
     int2* d_tiles = (int2*)ctx->book_tiles.reserve(tiles.size() * sizeof(int2));
     Moments* d_parts = (Moments*)ctx->book_parts.reserve((size_t)total_blocks * 2 * sizeof(Moments));
     Moments* d_out = (Moments*)ctx->book_out.reserve(items.size() * 2 * sizeof(Moments));
-    int* d_counter = (int*)ctx->book_counter.reserve(256);
+    int* d_counters = (int*)ctx->book_counter.reserve(256);
     uint32_t* flags = (uint32_t*)ctx->flags.reserve(256);
     if (!blob.bytes.empty())
       KG_HIP(hipMemcpyAsync(d_blob, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -1303,24 +1385,42 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
                           ctx->stream));
     KG_HIP(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice,
                           ctx->stream));
-    int spt = tuning ? tuning->samples_per_thread : 0;
-    int wpc = tuning ? tuning->workgroups_per_cu : 0;
+    const int spt = tuning ? tuning->samples_per_thread : 0;
+    const int wpc = tuning ? tuning->workgroups_per_cu : 0;
     if (spt != 0 && spt != 1 && spt != 2 && spt != 4) fail(KG_INVALID_INPUT, "samples_per_thread must be 1, 2 or 4");
     if (wpc < 0 || wpc > 16) fail(KG_INVALID_INPUT, "workgroups_per_cu must be in 0..16");
-    if (!spt || !wpc) {
-      autotune(ctx, fp32, d_items, tiles, d_counter, d_parts, flags);
-      const int slot = fp32 ? 1 : 0;
-      if (!spt) spt = ctx->tuned_spt[slot];
-      if (!wpc) wpc = ctx->tuned_wpc[slot];
+    // The shape reported back is that of the class with the most path-steps.
+    int major = -1;
+    double major_steps = -1.0;
+    for (int c = 0; c < BC_COUNT; ++c) {
+      ClassRun& r = runs[c];
+      if (r.count == 0) continue;
+      int tuned_spt = spt, tuned_wpc = wpc;
+      if (!spt || !wpc)
+        autotune(ctx, fp32, c, d_items, tiles.data() + r.begin, (size_t)r.count, d_counters, d_parts, flags,
+                 &tuned_spt, &tuned_wpc);
+      r.spt = spt ? spt : tuned_spt;
+      r.wpc = wpc ? wpc : tuned_wpc;
+      double steps = 0.0;
+      for (int t = r.begin; t < r.begin + r.count; ++t) steps += (double)items[tiles[t].x].dims;
+      if (steps > major_steps) {
+        major_steps = steps;
+        major = c;
+      }
     }
     const bool use_graph = tuning && tuning->use_graph;
     const int replays = tuning ? std::max(0, tuning->replays) : 0;
     if (use_graph) {
-      const uint64_t key[12] = {(uint64_t)fp32,         (uint64_t)spt,          (uint64_t)wpc,
-                                (uint64_t)items.size(), (uint64_t)tiles.size(), (uint64_t)(uintptr_t)d_items,
-                                (uint64_t)(uintptr_t)d_tiles, (uint64_t)(uintptr_t)d_counter,
-                                (uint64_t)(uintptr_t)d_parts, (uint64_t)(uintptr_t)d_out,
-                                (uint64_t)(uintptr_t)flags, (uint64_t)(uintptr_t)cancel_word(ctx)};
+      uint64_t key[BOOK_GRAPH_KEY];
+      size_t nk = 0;
+      for (uint64_t v : {(uint64_t)fp32, (uint64_t)items.size(), (uint64_t)tiles.size(),
+                         (uint64_t)(uintptr_t)d_items, (uint64_t)(uintptr_t)d_tiles, (uint64_t)(uintptr_t)d_counters,
+                         (uint64_t)(uintptr_t)d_parts, (uint64_t)(uintptr_t)d_out, (uint64_t)(uintptr_t)flags,
+                         (uint64_t)(uintptr_t)cancel_word(ctx)})
+        key[nk++] = v;
+      for (int c = 0; c < BC_COUNT; ++c)
+        for (int v : {runs[c].begin, runs[c].count, runs[c].spt, runs[c].wpc}) key[nk++] = (uint64_t)v;
+      static_assert(10 + 4 * BC_COUNT == BOOK_GRAPH_KEY, "graph key size");
       if (!ctx->book_graph || std::memcmp(key, ctx->book_graph_key, sizeof(key)) != 0) {
         if (ctx->book_graph) (void)hipGraphExecDestroy(ctx->book_graph);
         ctx->book_graph = nullptr;
@@ -1333,7 +1433,7 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
           }
         } guard;
         KG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        run_book_once(ctx, fp32, spt, wpc, d_items, d_tiles, (int)tiles.size(), d_counter, d_parts, flags);
+        run_book_once(ctx, fp32, runs, d_items, d_tiles, d_counters, d_parts, flags);
         hipLaunchKernelGGL(merge_book, dim3(items.size()), dim3(MERGE_THREADS), 0, ctx->stream, d_items, d_parts,
                            d_out);
         KG_HIP(hipStreamEndCapture(ctx->stream, &guard.graph));
@@ -1356,7 +1456,7 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
       for (int k = 0; k <= replays; ++k) {
         if (k) check_flags(ctx);
         timing_begin(ctx);
-        run_book_once(ctx, fp32, spt, wpc, d_items, d_tiles, (int)tiles.size(), d_counter, d_parts, flags);
+        run_book_once(ctx, fp32, runs, d_items, d_tiles, d_counters, d_parts, flags);
         hipLaunchKernelGGL(merge_book, dim3(items.size()), dim3(MERGE_THREADS), 0, ctx->stream, d_items, d_parts, d_out);
         KG_HIP(hipGetLastError());
         device_ms = timing_end(ctx);
@@ -1367,8 +1467,8 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
                           ctx->stream));
     KG_HIP(hipStreamSynchronize(ctx->stream));
     if (tuning) {
-      tuning->samples_per_thread = spt;
-      tuning->workgroups_per_cu = wpc;
+      tuning->samples_per_thread = runs[major].spt;
+      tuning->workgroups_per_cu = runs[major].wpc;
     }
     // Arithmetic Asians whose controlled price is negative are resampled
     // without the control on the same seed (asian.rs:744-752).
@@ -1391,7 +1491,11 @@ static void price_book(kg_context* ctx, const kg_book_contract* contracts, size_
       for (int64_t b = 0; b < blocks; ++b) t2.push_back(make_int2(0, (int)b));
       KG_HIP(hipMemcpyAsync(d_items, &it, sizeof(BookItem), hipMemcpyHostToDevice, ctx->stream));
       KG_HIP(hipMemcpyAsync(d_tiles, t2.data(), t2.size() * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
-      run_book_once(ctx, fp32, spt, wpc, d_items, d_tiles, (int)t2.size(), d_counter, d_parts, flags);
+      ClassRun one[BC_COUNT];
+      one[BC_ARITH_ASIAN] = runs[BC_ARITH_ASIAN];
+      one[BC_ARITH_ASIAN].begin = 0;
+      one[BC_ARITH_ASIAN].count = (int)t2.size();
+      run_book_once(ctx, fp32, one, d_items, d_tiles, d_counters, d_parts, flags);
       hipLaunchKernelGGL(merge_book, dim3(1), dim3(MERGE_THREADS), 0, ctx->stream, d_items, d_parts, d_out);
       check_flags(ctx);
       KG_HIP(hipMemcpyAsync(&results[2 * k], d_out, 2 * sizeof(Moments), hipMemcpyDeviceToHost, ctx->stream));

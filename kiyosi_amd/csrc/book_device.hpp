@@ -209,44 +209,81 @@ __device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign
   // instead of once per lane inside it.
   int hit = -1;
   R hit_noise = 0, hit_terminal = 0, z0 = 0;
-  for (int i = 0; i < b.n_steps; ++i) {
-    if (__ballot(active) == 0) break;
-    if (!active) continue;
+  // The loop body is branch-free per lane: every lane draws its normals and
+  // a finished lane keeps its state through selects, so the only branches
+  // are on contract fields (wave-uniform) and the wave's early exit. The
+  // lane's flags live in one integer (kept in a vector register) and the
+  // steps go two per iteration, one Box-Muller pair per iteration.
+  enum : uint32_t { ACTIVE = 1, TOUCHED = 2, COND = 4 };
+  uint32_t st = active ? ACTIVE : 0u;
+  // The reference tests every step's spot for overflow; the spot is
+  // monotone in the log coordinate, so the lowest and highest visited
+  // coordinates decide that test for all of them (a NaN reaches the last
+  // coordinate, which the payoff tests).
+  R low = terminal, high = terminal;
+  const int n_steps = b.n_steps;
+  const bool up = b.up, has_upper = b.has_upper, knock_in = b.knock_in;
+  const bool condition_on_hit = b.condition_on_hit, bounded = b.bounded;
+  auto step = [&](int i, R z) {
     const BarrierStep& s = b.steps[i];
     const R width = FP32 ? (R)uload(b.steps32 + 4 * i + 1) : (R)uload(&s.width);
     const R mean = FP32 ? (R)uload(b.steps32 + 4 * i) : (R)uload(&s.mean);
-    const R z = (R)sign * dr.template normal<R>(i);
     if (LR && i == 0) z0 = z;
-    noise = FP32 ? (R)fmaf((float)width, (float)z, (float)noise) : (R)fma((double)width, (double)z, (double)noise);
-    terminal = mean + noise;
-    if (!b.bounded) {
-      const double sp = discounted_product(b.spot, 1.0, (double)terminal);
-      if (!isfinite(sp) || sp <= 0.0) bad = 1;
-    }
+    const bool on = st & ACTIVE;
+    const R next = FP32 ? (R)fmaf((float)width, (float)z, (float)noise) : (R)fma((double)width, (double)z, (double)noise);
+    noise = on ? next : noise;
+    terminal = on ? mean + next : terminal;
+    low = fmin(low, terminal);
+    high = fmax(high, terminal);
     if (uload(&s.observe)) {
       const R lower = FP32 ? (R)uload(b.steps32 + 4 * i + 2) : (R)uload(&s.lower);
       bool crossed;
-      if (b.has_upper) {
+      if (has_upper) {
         const R upper = FP32 ? (R)uload(b.steps32 + 4 * i + 3) : (R)uload(&s.upper);
         crossed = noise <= -lower || noise >= -upper;
       } else {
-        crossed = b.up ? noise >= -lower : noise <= -lower;
+        crossed = up ? noise >= -lower : noise <= -lower;
       }
-      if (crossed) {
-        touched = true;
-        if (b.condition_on_hit && b.n_steps - i > 8) {
-          hit = i;
-          hit_noise = noise;
-          hit_terminal = terminal;
-          has_cond = true;
-          active = false;
-          continue;
-        }
-      }
+      crossed = crossed && on;
+      // A knock-in conditioned on the hit stops the path at the hit.
+      const bool cond = crossed && condition_on_hit && n_steps - i > 8;
+      hit = cond ? i : hit;
+      hit_noise = cond ? noise : hit_noise;
+      hit_terminal = cond ? terminal : hit_terminal;
+      st = (st | (crossed ? TOUCHED : 0u) | (cond ? COND : 0u)) & (cond ? ~ACTIVE : ~0u);
     }
-    if (touched && !b.knock_in) {
-      hit = i;
-      active = false;
+    const bool out = !knock_in && (st & (ACTIVE | TOUCHED)) == (ACTIVE | TOUCHED);
+    hit = out ? i : hit;
+    st &= out ? ~ACTIVE : ~0u;
+  };
+  if constexpr (LR) {
+    // The spot-Greek kernel steps one day at a time (its register budget).
+    for (int i = 0; i < n_steps; ++i) {
+      if (__ballot(st & ACTIVE) == 0) break;
+      step(i, (R)sign * dr.template normal<R>(i));
+    }
+  } else {
+    int i = 0;
+    bool running = true;
+    for (; i + 1 < n_steps; i += 2) {
+      if (__ballot(st & ACTIVE) == 0) {
+        running = false;
+        break;
+      }
+      R za, zb;
+      dr.template normal_pair<R>(i, za, zb);
+      step(i, (R)sign * za);
+      step(i + 1, (R)sign * zb);
+    }
+    if (running && i < n_steps && __ballot(st & ACTIVE) != 0) step(i, (R)sign * dr.template normal<R>(i));
+  }
+  active = st & ACTIVE;
+  touched = st & TOUCHED;
+  has_cond = st & COND;
+  if (!bounded && live) {
+    for (const R t : {low, high}) {
+      const double sp = discounted_product(b.spot, 1.0, (double)t);
+      if (!isfinite(sp) || sp <= 0.0) bad = 1;
     }
   }
   if (has_cond) {
@@ -425,8 +462,10 @@ struct ArithPath {
 };
 
 // ArithmeticGrid::path (asian.rs:640-712): the strike-independent loop; a
-// forward start takes its lead step from dimension 0.
-template <bool FP32, class D>
+// forward start takes its lead step from dimension 0. PAIRS steps two days
+// per iteration (one Box-Muller pair); the spot-Greek kernel steps one at a
+// time to stay within its register budget.
+template <bool FP32, bool PAIRS = true, class D>
 __device__ ArithPath arith_path(const ArithAsianDev& p, D& dr, bool live, float sign) {
   using R = typename std::conditional<FP32, float, double>::type;
   const int dims = p.n - 1;
@@ -439,20 +478,31 @@ __device__ ArithPath arith_path(const ArithAsianDev& p, D& dr, bool live, float 
     lead_growth = FP32 ? (double)((float)p.lead_drift + (float)p.lead_width * (float)z)
                        : p.lead_drift + p.lead_width * (double)z;
   }
-  for (int i = 0; i < dims; ++i) {
-    if (__ballot(live) == 0) break;
-    if (!live) continue;
-    const R z = (R)sign * dr.template normal<R>(i + p.lead);
+  // Lanes past the contract's last sample step along; their state is
+  // discarded by the caller. Two steps per iteration, one Box-Muller pair.
+  if (__ballot(live) == 0) return ArithPath{rel, log_sum, gg, lead_growth};
+  const bool control = p.control;
+  auto step = [&](int i, R z) {
     growth += drift + width * z;
     const double fraction = (i + 2 == p.n ? 0.5 : 1.0) * inv_n;
-    if (p.control) gg += fraction * (double)growth;
+    if (control) gg += fraction * (double)growth;
     if ((double)growth > 600.0 || log_sum != -INFINITY) {
       if (log_sum == -INFINITY) log_sum = log(1.0 - 0.5 * inv_n + rel);
       log_sum = d_log_add(log_sum, log(fraction) + (double)growth);
     } else {
       rel += fraction * (FP32 ? (double)f32::expm1((float)growth) : expm1((double)growth));
     }
+  };
+  int i = 0;
+  if constexpr (PAIRS) {
+    for (; i + 1 < dims; i += 2) {
+      R za, zb;
+      dr.template normal_pair<R>(i + p.lead, za, zb);
+      step(i, (R)sign * za);
+      step(i + 1, (R)sign * zb);
+    }
   }
+  for (; i < dims; ++i) step(i, (R)sign * dr.template normal<R>(i + p.lead));
   return ArithPath{rel, log_sum, gg, lead_growth};
 }
 
@@ -624,30 +674,57 @@ __device__ double accum_path(const AccumDev& a, D& dr, bool live, float sign, ui
     if (fabs(distance) < 0.5) return a.sign * a.strike * expm1(distance);
     return a.sign > 0 ? spot - a.strike : a.strike - spot;
   };
-  for (int i = 0; i < a.n_steps; ++i) {
-    if (__ballot(active) == 0) break;
-    if (!active) continue;
-    const R z = (R)sign * dr.template normal<R>(i);
-    lr += FP32 ? (R)fmaf(uload(a.steps32 + 2 * i + 1), (float)z, uload(a.steps32 + 2 * i))
-               : (R)fma(uload(a.steps + 5 * i + 1), (double)z, uload(a.steps + 5 * i));
-    if (!((double)lr >= a.safe_lo && (double)lr <= a.safe_hi)) {
-      const double sp = discounted_product(a.spot, 1.0, (double)lr);
+  // Branch-free per lane, two steps per iteration, as in barrier_path: a
+  // finished lane draws and discards, and keeps its state through selects.
+  const int n_steps = a.n_steps;
+  const bool fixed = a.fixed;
+  uint32_t on = active ? 1u : 0u;
+  // Overflow test of every visited spot through the lowest and highest log
+  // returns (the spot is monotone in them), as in barrier_path.
+  R low = lr, high = lr;
+  auto step = [&](int i, R z) {
+    const bool act = on;
+    const R next = lr + (FP32 ? (R)fmaf(uload(a.steps32 + 2 * i + 1), (float)z, uload(a.steps32 + 2 * i))
+                              : (R)fma(uload(a.steps + 5 * i + 1), (double)z, uload(a.steps + 5 * i)));
+    lr = act ? next : lr;
+    low = fmin(low, lr);
+    high = fmax(high, lr);
+    const uint8_t f = uload_u8(a.flags, i);
+    if (fixed) quantity = act ? quantity + uload(a.steps + 5 * i + 3) : quantity;
+    bool still = act;
+    if (f & ACC_OBSERVE) {
+      const bool out = act && sg * lr >= sg * ko;
+      ko_step = out ? i : ko_step;
+      ko_lr = out ? lr : ko_lr;
+      ko_quantity = out ? quantity : ko_quantity;
+      still = act && !out;
+    }
+    if (fixed)
+      quantity = still ? quantity + uload(a.steps + 5 * i + 4) : quantity;
+    else if (f & ACC_ACCUMULATE)
+      quantity = still ? quantity + a.daily_qty * (sg * lr < sg * sk ? a.accel : 1.0) : quantity;
+    if (still && !isfinite(quantity)) bad = 1;
+    on = still ? 1u : 0u;
+  };
+  int i = 0;
+  bool running = true;
+  for (; i + 1 < n_steps; i += 2) {
+    if (__ballot(on) == 0) {
+      running = false;
+      break;
+    }
+    R za, zb;
+    dr.template normal_pair<R>(i, za, zb);
+    step(i, (R)sign * za);
+    step(i + 1, (R)sign * zb);
+  }
+  if (running && i < n_steps && __ballot(on) != 0) step(i, (R)sign * dr.template normal<R>(i));
+  active = on;
+  if (live && !((double)low >= a.safe_lo && (double)high <= a.safe_hi)) {
+    for (const R t : {low, high}) {
+      const double sp = discounted_product(a.spot, 1.0, (double)t);
       if (!isfinite(sp) || sp <= 0.0) bad = 1;
     }
-    const uint8_t f = uload_u8(a.flags, i);
-    if (a.fixed) quantity += uload(a.steps + 5 * i + 3);
-    if ((f & ACC_OBSERVE) && sg * lr >= sg * ko) {
-      ko_step = i;
-      ko_lr = lr;
-      ko_quantity = quantity;
-      active = false;
-      continue;
-    }
-    if (a.fixed)
-      quantity += uload(a.steps + 5 * i + 4);
-    else if (f & ACC_ACCUMULATE)
-      quantity += a.daily_qty * (sg * lr < sg * sk ? a.accel : 1.0);
-    if (!isfinite(quantity)) bad = 1;
   }
   if (ko_step >= 0)
     value = discounted_product(ko_quantity, unit((double)ko_lr), a.steps[5 * ko_step + 2]);

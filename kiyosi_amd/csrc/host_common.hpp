@@ -76,6 +76,13 @@ struct SobolSetup {
   DeviceBuffer d_directions, d_shifts;
 };
 
+// The book's persistent kernels: one per path class (barrier, digital,
+// geometric Asian, arithmetic Asian, stepped note, bridge note, accumulator).
+constexpr int BOOK_CLASSES = 7;
+// Launch-graph key: precision, sizes, buffer addresses (10 words), then each
+// class's tile run and shape (4 words per class).
+constexpr int BOOK_GRAPH_KEY = 10 + 4 * BOOK_CLASSES;
+
 }  // namespace kg
 
 // A cancellation token: one int32 in pinned, device-visible host memory.
@@ -101,17 +108,18 @@ struct kg_context {
   std::vector<uint32_t> joe_kuo;  // direction-number records, loaded once
   kg::DeviceBuffer sobol_raw;     // unscrambled directions [dim][52], seed-independent
   uint32_t sobol_raw_dims = 0;
-  // Book engine: device buffers and the autotuned tile shape per precision.
+  // Book engine: device buffers and the autotuned tile shape per precision
+  // and path class (0 = not tuned yet).
   kg::DeviceBuffer book_blob, book_items, book_tiles, book_parts, book_out, book_counter;
   // American engine: exercise policy rows, model tables and MLP state.
   kg::DeviceBuffer policy, lsmc_tables, mlp_state;
-  int tuned_spt[2] = {0, 0};
-  int tuned_wpc[2] = {0, 0};
+  int tuned_spt[2][kg::BOOK_CLASSES] = {};
+  int tuned_wpc[2][kg::BOOK_CLASSES] = {};
   // The book's instantiated launch graph, kept across calls while its launch
   // shape and buffers (the key) stay the same; the contracts and tiles it
   // reads are uploaded before each launch.
   hipGraphExec_t book_graph = nullptr;
-  uint64_t book_graph_key[12] = {};
+  uint64_t book_graph_key[kg::BOOK_GRAPH_KEY] = {};
   // Pinned host memory (hipHostMalloc): results and flags of a call in the
   // first PINNED_RESULT_BYTES, then a staging area for asynchronous uploads
   // whose bump offset restarts with every call.
