@@ -613,10 +613,12 @@ static Planned plan_barrier(const BarrierContract& o, const kg_bsm_market& m, Bl
     s.upper = observe && o.has_upper ? mean_distance(date, o.upper) : 0.0;
     s.remaining = years(date, expiry);
     steps.push_back(s);
+    // FP32 rows (mean, width, lower, upper): the levels of a step without a
+    // fixing are NaN, so no comparison with them can report a crossing.
     steps32.push_back((float)s.mean);
     steps32.push_back((float)s.width);
-    steps32.push_back((float)s.lower);
-    steps32.push_back((float)s.upper);
+    steps32.push_back(observe ? (float)s.lower : NAN);
+    steps32.push_back(observe ? (float)s.upper : NAN);
     width_sum += s.width;
     prev = date;
   }

@@ -225,9 +225,26 @@ __device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign
   const bool up = b.up, has_upper = b.has_upper, knock_in = b.knock_in;
   const bool condition_on_hit = b.condition_on_hit, bounded = b.bounded;
   auto step = [&](int i, R z) {
-    const BarrierStep& s = b.steps[i];
-    const R width = FP32 ? (R)uload(b.steps32 + 4 * i + 1) : (R)uload(&s.width);
-    const R mean = FP32 ? (R)uload(b.steps32 + 4 * i) : (R)uload(&s.mean);
+    // FP32: the step's row (16 bytes); FP64: the step record.
+    R width, mean, lower = 0, upper = 0;
+    bool observe;
+    if constexpr (FP32) {
+      const float* row = b.steps32 + 4 * i;  // one 16-byte scalar load
+      mean = uload(row);
+      width = uload(row + 1);
+      lower = uload(row + 2);
+      upper = uload(row + 3);
+      observe = true;  // rows of steps without a fixing hold NaN levels
+    } else {
+      const BarrierStep& s = b.steps[i];
+      width = uload(&s.width);
+      mean = uload(&s.mean);
+      observe = uload(&s.observe);
+      if (observe) {
+        lower = uload(&s.lower);
+        upper = uload(&s.upper);
+      }
+    }
     if (LR && i == 0) z0 = z;
     const bool on = st & ACTIVE;
     const R next = FP32 ? (R)fmaf((float)width, (float)z, (float)noise) : (R)fma((double)width, (double)z, (double)noise);
@@ -235,15 +252,12 @@ __device__ double barrier_path(const BarrierDev& b, D& dr, bool live, float sign
     terminal = on ? mean + next : terminal;
     low = fmin(low, terminal);
     high = fmax(high, terminal);
-    if (uload(&s.observe)) {
-      const R lower = FP32 ? (R)uload(b.steps32 + 4 * i + 2) : (R)uload(&s.lower);
+    if (observe) {
       bool crossed;
-      if (has_upper) {
-        const R upper = FP32 ? (R)uload(b.steps32 + 4 * i + 3) : (R)uload(&s.upper);
+      if (has_upper)
         crossed = noise <= -lower || noise >= -upper;
-      } else {
+      else
         crossed = up ? noise >= -lower : noise <= -lower;
-      }
       crossed = crossed && on;
       // A knock-in conditioned on the hit stops the path at the hit.
       const bool cond = crossed && condition_on_hit && n_steps - i > 8;
@@ -684,8 +698,13 @@ __device__ double accum_path(const AccumDev& a, D& dr, bool live, float sign, ui
   R low = lr, high = lr;
   auto step = [&](int i, R z) {
     const bool act = on;
-    const R next = lr + (FP32 ? (R)fmaf(uload(a.steps32 + 2 * i + 1), (float)z, uload(a.steps32 + 2 * i))
-                              : (R)fma(uload(a.steps + 5 * i + 1), (double)z, uload(a.steps + 5 * i)));
+    R next;
+    if constexpr (FP32) {
+      const float* row = a.steps32 + 2 * i;  // drift, diffusion
+      next = lr + fmaf(uload(row + 1), (float)z, uload(row));
+    } else {
+      next = lr + fma(uload(a.steps + 5 * i + 1), (double)z, uload(a.steps + 5 * i));
+    }
     lr = act ? next : lr;
     low = fmin(low, lr);
     high = fmax(high, lr);

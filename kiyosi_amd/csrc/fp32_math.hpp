@@ -94,7 +94,8 @@ __device__ inline float uniform_hi(uint32_t hi) { return fmaf((float)hi, 0x1p-32
 __device__ inline void box_muller_pair(Pcg32& g, float* even, float* odd) {
   const uint32_t h1 = g.next_skip_one();
   const uint32_t h2 = g.next_skip_one();
-  const float radius = sqrt(-2.0f * log(uniform_hi(h1)));
+  // -2 ln u = -2 ln 2 * log2 u: one multiply after the hardware log2.
+  const float radius = sqrt(__builtin_amdgcn_logf(uniform_hi(h1)) * -1.38629436111989061f);
   const float turns = uniform_hi(h2);
   *even = radius * __builtin_amdgcn_cosf(turns);
   *odd = radius * __builtin_amdgcn_sinf(turns);
