@@ -638,8 +638,9 @@ __device__ __attribute__((noinline)) double arith_payoff_greeks(const ArithAsian
   return v;
 }
 
-// The FP64 book keeps the strike terms out of line: inlined into the FP64
-// sample body they would lift it past 256 VGPRs (one wave per SIMD).
+// The book keeps the strike terms out of line: inlined into the sample body
+// they would set the kernel's register count (FP32: 171 VGPRs inlined, 123
+// out of line; FP64: past 256).
 __device__ __attribute__((noinline)) double arith_payoff_call(const ArithAsianDev& p, const ArithPath& a,
                                                               uint32_t& bad) {
   return arith_payoff(p, a, bad);
@@ -648,10 +649,7 @@ template <bool FP32, class D>
 __device__ double arith_asian(const ArithAsianDev& p, D& dr, bool live, float sign, uint32_t& bad) {
   const ArithPath a = arith_path<FP32>(p, dr, live, sign);
   if (!live) return 0.0;
-  if constexpr (FP32)
-    return arith_payoff(p, a, bad);
-  else
-    return arith_payoff_call(p, a, bad);
+  return arith_payoff_call(p, a, bad);
 }
 
 // ------------------------------------------------------------------ accumulator
@@ -696,6 +694,10 @@ __device__ double accum_path(const AccumDev& a, D& dr, bool live, float sign, ui
   // Overflow test of every visited spot through the lowest and highest log
   // returns (the spot is monotone in them), as in barrier_path.
   R low = lr, high = lr;
+  // The daily purchase below and above the strike (the products the
+  // reference forms each day). A non-finite quantity stays non-finite, so
+  // its test is made once, after the loop.
+  const double buy_accelerated = a.daily_qty * a.accel, buy = a.daily_qty * 1.0;
   auto step = [&](int i, R z) {
     const bool act = on;
     R next;
@@ -718,11 +720,12 @@ __device__ double accum_path(const AccumDev& a, D& dr, bool live, float sign, ui
       ko_quantity = out ? quantity : ko_quantity;
       still = act && !out;
     }
-    if (fixed)
+    if (fixed) {
       quantity = still ? quantity + uload(a.steps + 5 * i + 4) : quantity;
-    else if (f & ACC_ACCUMULATE)
-      quantity = still ? quantity + a.daily_qty * (sg * lr < sg * sk ? a.accel : 1.0) : quantity;
-    if (still && !isfinite(quantity)) bad = 1;
+    } else if (f & ACC_ACCUMULATE) {
+      const double purchase = sg * lr < sg * sk ? buy_accelerated : buy;
+      quantity = still ? quantity + purchase : quantity;
+    }
     on = still ? 1u : 0u;
   };
   int i = 0;
@@ -739,6 +742,7 @@ __device__ double accum_path(const AccumDev& a, D& dr, bool live, float sign, ui
   }
   if (running && i < n_steps && __ballot(on) != 0) step(i, (R)sign * dr.template normal<R>(i));
   active = on;
+  if (live && !isfinite(quantity)) bad = 1;
   if (live && !((double)low >= a.safe_lo && (double)high <= a.safe_hi)) {
     for (const R t : {low, high}) {
       const double sp = discounted_product(a.spot, 1.0, (double)t);

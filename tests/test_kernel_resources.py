@@ -42,9 +42,11 @@ def test_book_kernels_keep_two_waves_per_simd(tmp_path):
     assert len(book) == 2 * 3 * 7
     for name, (vgpr, private) in book.items():
         assert vgpr <= 256, (name, vgpr)
-        if "book_kernelILb1E" in name:  # FP32: no spills, at least two waves per SIMD
-            assert private == 0, (name, private)
-            assert vgpr <= 192, (name, vgpr)
+        if "book_kernelILb1E" in name:  # FP32: no spills, at least three waves per SIMD
+            # The arithmetic Asian (class 3) calls its payoff out of line;
+            # its scratch is that call's frame.
+            assert private <= (64 if "ELi3EEEv" in name else 0), (name, private)
+            assert vgpr <= 168, (name, vgpr)
     # The FP32 spot-Greek kernels keep four waves per SIMD; their scratch is
     # the call frame of the arithmetic Asian's strike terms, not spills.
     greeks = {k: v for k, v in res.items() if "spot_greek_kernelILb1E" in k}
