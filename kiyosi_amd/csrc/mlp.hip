@@ -57,7 +57,7 @@ constexpr int PK_W1 = 0, PK_B1 = 192, PK_B2 = 256, PK_W3 = 320, PK_B3 = 384;  //
 constexpr int PACK_BYTES = ((PK_FLOATS + 385 * 4) + 15) & ~15;
 constexpr int PACK_WORDS = PACK_BYTES / 16;
 constexpr int T_ROW = 40;  // padded row of the 32-sample transposes (bf16)
-constexpr int GRAD_WAVES = 1024;  // waves of a gradient launch (one per SIMD) before tiles per wave grow
+constexpr int GRAD_WAVES = 2048;  // waves of a gradient launch (two per SIMD) before tiles per wave grow
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -71,14 +71,26 @@ __host__ __device__ inline uint32_t bf16_bits(float f) {
 }
 __host__ __device__ inline float bf16_value(uint32_t bits) { return __builtin_bit_cast(float, bits << 16); }
 
-// 0.5 x (1 + tanh(k0 (x + k1 x^3))) and its derivative; tanh(a) = 1 - 2 / (e^{2a} + 1).
+// Two FP32 values rounded to nearest-even bf16 and packed (lo in the low
+// half): one v_cvt_pk_bf16_f32.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t bf16_pack(float lo, float hi) {
+  const f32x2 v = {lo, hi};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
+
+// GELU(x) = 0.5 x (1 + tanh(a)), a = k0 (x + k1 x^3), and its derivative, in
+// the logistic form: 0.5 (1 + tanh(a)) = s = 1 / (1 + e^{-2a}), so
+// GELU(x) = x s and GELU'(x) = s + x s (1 - s) 2 k0 (1 + 3 k1 x^2).
 __device__ inline float gelu(float x, float* derivative) {
-  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+  constexpr float k0 = 0.7978845608028654f, k1 = 0.044715f;
+  constexpr float c0 = -2.0f * 1.4426950408889634f * k0;  // -2 a log2(e) = x (c0 + c0 k1 x^2)
   const float x2 = x * x;
-  const float a = k0 * fmaf(k1 * x2, x, x);
-  const float t = 1.0f - 2.0f * __builtin_amdgcn_rcpf(f32::exp(2.0f * a) + 1.0f);
-  *derivative = fmaf(0.5f * x * (1.0f - t * t), k0 * fmaf(3.0f * k1, x2, 1.0f), 0.5f * (1.0f + t));
-  return 0.5f * x * (1.0f + t);
+  const float s = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(x * fmaf(c0 * k1, x2, c0)));
+  const float v = x * s;
+  *derivative = fmaf(v * (1.0f - s), fmaf(6.0f * k0 * k1, x2, 2.0f * k0), s);
+  return v;
 }
 
 __device__ inline f32x4 mfma(u32x4 a, u32x4 b, f32x4 c) {
@@ -109,27 +121,39 @@ __device__ inline u32x4 row_pair(const uint16_t* row, int base) {
 struct Fwd {
   float dg1[16];   // GELU'(z1) of u(q, i) at 4 q + i
   uint32_t h1[8];  // bf16 H1 of u(q, i): pair (q, 2 p), (q, 2 p + 1) at 2 q + p
-  float h2[16];    // bf16-rounded GELU(z2) of u(mt, i)
+  uint32_t h2[8];  // bf16 GELU(z2) of u(mt, i): pair (mt, 2 p), (mt, 2 p + 1) at 2 mt + p
   float dg2[16];
   float out;
   bool finite;
+  __device__ float h2_at(int r) const {  // unit u(r / 4, r % 4)
+    const uint32_t w = h2[r >> 1];
+    return bf16_value((r & 1) ? (w >> 16) : (w & 0xffffu));
+  }
 };
 
 // Forward pass of the tile sample held by column c; every lane of the wave
-// must call it (the MFMAs span the wave).
-__device__ inline void forward(const Pack& P, float u0, float u1, float u2, Fwd& f) {
+// must call it (the MFMAs span the wave). With `hs`, H1 of the sample is
+// also staged into the 32-sample transpose at column 16 tp + c (training),
+// so it need not stay in registers until the backward pass.
+__device__ inline void forward(const Pack& P, float u0, float u1, float u2, Fwd& f, uint16_t* hs = nullptr,
+                               int tp = 0) {
   const int lane = threadIdx.x & (WAVE - 1), g = lane >> 4, c = lane & 15;
   const float* w1 = P.f + PK_W1;
   const float* b1 = P.f + PK_B1;
   for (int q = 0; q < 4; ++q) {
-    uint32_t bits[4];
+    float h[4];
     for (int i = 0; i < 4; ++i) {
       const int u = 16 * q + 4 * g + i;
       const float z = fmaf(w1[3 * u + 2], u2, fmaf(w1[3 * u + 1], u1, fmaf(w1[3 * u], u0, b1[u])));
-      bits[i] = bf16_bits(gelu(z, &f.dg1[4 * q + i]));
+      h[i] = gelu(z, &f.dg1[4 * q + i]);
     }
-    f.h1[2 * q] = bits[0] | (bits[1] << 16);
-    f.h1[2 * q + 1] = bits[2] | (bits[3] << 16);
+    f.h1[2 * q] = bf16_pack(h[0], h[1]);
+    f.h1[2 * q + 1] = bf16_pack(h[2], h[3]);
+    if (hs)
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t w = f.h1[2 * q + (i >> 1)];
+        hs[(16 * q + 4 * g + i) * T_ROW + 16 * tp + c] = (uint16_t)((i & 1) ? w >> 16 : w);
+      }
   }
   const float* b2 = P.f + PK_B2;
   const float* w3 = P.f + PK_W3;
@@ -141,12 +165,11 @@ __device__ inline void forward(const Pack& P, float u0, float u1, float u2, Fwd&
       const u32x4 b = {f.h1[4 * kt], f.h1[4 * kt + 1], f.h1[4 * kt + 2], f.h1[4 * kt + 3]};
       acc = mfma(row_pair(row, 32 * kt + 4 * g), b, acc);
     }
-    for (int i = 0; i < 4; ++i) {
-      const int u = 16 * mt + 4 * g + i;
-      const float h = bf16_value(bf16_bits(gelu(acc[i] + b2[u], &f.dg2[4 * mt + i])));
-      f.h2[4 * mt + i] = h;
-      part = fmaf(w3[u], h, part);
-    }
+    float h[4];
+    for (int i = 0; i < 4; ++i) h[i] = gelu(acc[i] + b2[16 * mt + 4 * g + i], &f.dg2[4 * mt + i]);
+    f.h2[2 * mt] = bf16_pack(h[0], h[1]);
+    f.h2[2 * mt + 1] = bf16_pack(h[2], h[3]);
+    for (int i = 0; i < 4; ++i) part = fmaf(w3[16 * mt + 4 * g + i], f.h2_at(4 * mt + i), part);
   }
   part += __shfl_xor(part, 16, WAVE);
   part += __shfl_xor(part, 32, WAVE);
@@ -164,29 +187,26 @@ struct Acc {
 };
 
 // Backward pass of one tile: accumulates the per-lane gradients and stages
-// dZ2 and H1 of the tile into the wave's 32-sample transposes at column
-// 16 tp + c.
+// dZ2 of the tile into the wave's 32-sample transpose at column 16 tp + c
+// (the forward pass staged H1).
 __device__ inline void backward(const Pack& P, const Fwd& f, float gs, float u0, float u1, float u2, Acc& a,
-                                uint16_t* dz, uint16_t* hs, int tp) {
+                                uint16_t* dz, int tp) {
   const int lane = threadIdx.x & (WAVE - 1), g = lane >> 4, c = lane & 15;
   const float* w3 = P.f + PK_W3;
   uint32_t d2[8];
   for (int mt = 0; mt < 4; ++mt) {
-    uint32_t bits[4];
+    float d[4];
     for (int i = 0; i < 4; ++i) {
       const int r = 4 * mt + i, u = 16 * mt + 4 * g + i;
-      const float d = gs * w3[u] * f.dg2[r];
-      a.b2[r] += d;
-      a.w3[r] = fmaf(gs, f.h2[r], a.w3[r]);
-      bits[i] = bf16_bits(d);
-      dz[u * T_ROW + 16 * tp + c] = (uint16_t)bits[i];
+      d[i] = gs * w3[u] * f.dg2[r];
+      a.b2[r] += d[i];
+      a.w3[r] = fmaf(gs, f.h2_at(r), a.w3[r]);
     }
-    d2[2 * mt] = bits[0] | (bits[1] << 16);
-    d2[2 * mt + 1] = bits[2] | (bits[3] << 16);
-    for (int p = 0; p < 2; ++p) {
-      const int u = 16 * mt + 4 * g + 2 * p;
-      hs[u * T_ROW + 16 * tp + c] = (uint16_t)(f.h1[2 * mt + p] & 0xffffu);
-      hs[(u + 1) * T_ROW + 16 * tp + c] = (uint16_t)(f.h1[2 * mt + p] >> 16);
+    d2[2 * mt] = bf16_pack(d[0], d[1]);
+    d2[2 * mt + 1] = bf16_pack(d[2], d[3]);
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t w = d2[2 * mt + (i >> 1)];
+      dz[(16 * mt + 4 * g + i) * T_ROW + 16 * tp + c] = (uint16_t)((i & 1) ? w >> 16 : w);
     }
   }
   for (int mt = 0; mt < 4; ++mt) {
@@ -236,7 +256,7 @@ constexpr int GRAD_WG_WAVES = 4;  // waves of a gradient workgroup
 // needs no cross-wave reduction. The small gradients are summed over the
 // waves in wave order. Every wave runs every round so the block barriers
 // are uniform.
-__global__ void __launch_bounds__(GRAD_WG_WAVES * WAVE)
+__global__ void __launch_bounds__(GRAD_WG_WAVES * WAVE, 2)
     mlp_grad(const uint4* __restrict__ packed, const float4* __restrict__ samples, int64_t begin, int n,
              int tpw, float inv_n, float* __restrict__ parts, uint32_t* __restrict__ flags) {
   __shared__ uint4 s_pack[PACK_WORDS];
@@ -261,13 +281,13 @@ __global__ void __launch_bounds__(GRAD_WG_WAVES * WAVE)
     const bool valid = local < n;
     const float4 x = valid ? samples[begin + local] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     Fwd f;
-    forward(P, x.x, x.y, x.z, f);
+    forward(P, x.x, x.y, x.z, f, s_hs[wave >> 1], wave & 1);
     finite = finite && f.finite;
     const float r = valid ? f.out - x.w : 0.0f;
     const float gs = 2.0f * r * inv_n;
     a.loss = fmaf(r * inv_n, r, a.loss);
     a.b3 += gs;
-    backward(P, f, gs, x.x, x.y, x.z, a, s_dz[wave >> 1], s_hs[wave >> 1], wave & 1);
+    backward(P, f, gs, x.x, x.y, x.z, a, s_dz[wave >> 1], wave & 1);
     __syncthreads();
     for (int slot = 0; slot < GRAD_WG_WAVES / 2; ++slot) {
       const uint4 av = *(const uint4*)(s_dz[slot] + (16 * wave + c) * T_ROW + 8 * g);
@@ -330,7 +350,7 @@ enum : int { UPDATE_ADAM = 0, UPDATE_GRADIENT = 1, UPDATE_PACK = 2, UPDATE_ADOPT
 // parameters by their running average and packs them. With avg_k > 0 an
 // Adam step also folds the new weights into the running average of the
 // last avg_k iterates. Slot P_LOSS adds the batch loss to *loss.
-constexpr int UPDATE_PARAMS = 32;                  // parameters per update workgroup
+constexpr int UPDATE_PARAMS = 16;                  // parameters per update workgroup
 constexpr int UPDATE_SLICES = 256 / UPDATE_PARAMS;  // slices of the partials summed in parallel
 
 __global__ void __launch_bounds__(256)
