@@ -240,7 +240,7 @@ def config4(reps, cpu):
     flop = sum(C4_FLOP[type(ins).__name__] * e.path_steps for (_, ins, _), e in zip(entries, est["fp32"]))
     dev = out["fp32"]["device_ms"]
     out["roofline"] = roofline("valu", flop / (dev * 1e-3) / 1e12, FP32_PEAK_TFLOPS, "TFLOP/s",
-                               kernel="book_kernel<fp32> (graph: reset, book, merge)", kernel_ms=dev,
+                               kernel="book_kernel<fp32, class> x 7 (graph: reset, one persistent kernel per path class, merge)", kernel_ms=dev,
                                basis="per path-step: barrier 10, geometric Asian 10, arithmetic Asian 11, "
                                      "snowball/Phoenix 10, accumulator 13 FLOP, weighted by each "
                                      f"contract's path-steps ({flop / out['fp32']['path_steps']:.2f} mean)")
