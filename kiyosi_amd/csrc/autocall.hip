@@ -418,6 +418,68 @@ __global__ void __launch_bounds__(256)
   table[(n + k) * 16 + hix] = (SobolWord<FP32>)(b >> SHIFT);
 }
 
+// What a Sobol bridge note needs before its two passes besides the scrambled
+// directions, in one launch: the call's flag word is cleared; workgroup 0
+// writes the table rows of the intervals' first dimensions as
+// bridge_table_kernel writes them; and with `base`, workgroup g writes the
+// base points of sample blocks [g, g + 1) * BASE_ROWS_PER_GROUP, thread d
+// taking dimension d of each block in turn, so that a block's row is
+// written by consecutive threads. The base point of block m is the Sobol
+// point m * 256: its Gray code is gray(m) shifted up 8 bits with bit 7 = m
+// & 1, and gray(m) differs from gray(m - 1) in bit ctz(m), so one XOR moves a
+// thread from block m - 1 to block m (the same point sobol_block_base forms).
+constexpr int BASE_ROWS_PER_GROUP = 32;
+constexpr int BASE_ROW_COLS = 21;  // directions 7 .. 27: up to 2^20 sample blocks
+
+template <bool FP32>
+__global__ void __launch_bounds__(256)
+    bridge_setup_kernel(int dims, const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shifts,
+                        int64_t blocks, SobolWord<FP32>* __restrict__ base,
+                        const IntervalRow<typename std::conditional<FP32, float, double>::type>* __restrict__ rows,
+                        int n_intervals, SobolWord<FP32>* __restrict__ table, uint32_t* __restrict__ flags) {
+  constexpr int SHIFT = FP32 ? 20 : 0;
+  static_assert(BLOCK_SAMPLES == 256, "the base points step by 2^8");
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) *flags = 0u;
+    for (int t = threadIdx.x; t < n_intervals * 16; t += blockDim.x) {
+      const int k = t >> 4, hix = t & 15;
+      const int d = rows[k].first;
+      uint64_t v[8];  // all eight loads in flight at once
+#pragma unroll
+      for (int c = 0; c < 8; ++c) v[c] = dirs[(size_t)c * dims + d];
+      uint64_t a = 0, b = 0;
+#pragma unroll
+      for (int bit = 0; bit < 4; ++bit) {
+        const uint64_t on = (hix >> bit) & 1 ? ~0ull : 0ull;
+        a ^= v[bit] & on;
+        b ^= v[4 + bit] & on;
+      }
+      table[k * 16 + hix] = (SobolWord<FP32>)(a >> SHIFT);
+      table[(n_intervals + k) * 16 + hix] = (SobolWord<FP32>)(b >> SHIFT);
+    }
+  }
+  if (!base) return;
+  // Thread t keeps the directions 7 .. 7 + cols - 1 of its dimension in LDS
+  // column t (only it reads them): gray(m) for m < blocks has bits below
+  // cols - 1.
+  __shared__ uint64_t col[BASE_ROW_COLS][256];
+  const int cols = 1 + (64 - __builtin_clzll((uint64_t)blocks));
+  const int64_t m0 = (int64_t)blockIdx.x * BASE_ROWS_PER_GROUP;
+  const int64_t m1 = m0 + BASE_ROWS_PER_GROUP < blocks ? m0 + BASE_ROWS_PER_GROUP : blocks;
+  for (int d = threadIdx.x; d < dims; d += blockDim.x) {
+    uint64_t* mine = &col[0][threadIdx.x];
+    for (int c = 0; c < cols; ++c) mine[c * 256] = dirs[(size_t)(7 + c) * dims + d];
+    const uint64_t shift = shifts[d], odd = mine[0];
+    uint64_t g = 0;  // XOR of the directions 8 + i over the bits i of gray(m)
+    for (uint64_t bits = (uint64_t)m0 ^ ((uint64_t)m0 >> 1); bits; bits &= bits - 1)
+      g ^= mine[(__builtin_ctzll(bits) + 1) * 256];
+    for (int64_t m = m0; m < m1; ++m) {
+      if (m > m0) g ^= mine[(__builtin_ctzll((uint64_t)m) + 1) * 256];
+      base[m * dims + d] = (SobolWord<FP32>)((shift ^ g ^ ((m & 1) ? odd : 0)) >> SHIFT);
+    }
+  }
+}
+
 // Scenario lanes on shared draws: lane l's total and pair moments of the
 // block go to partials[block * slots + slot_base + 2l] and [... + 1],
 // formed exactly as the single-plan kernel forms its own.
@@ -490,11 +552,13 @@ static size_t bridge_events_lds(int n_intervals, int dims, bool sobol, bool fp32
 // interval rows (many knock-out events after the lockout) price in one pass.
 constexpr size_t BRIDGE_EVENTS_LDS_LIMIT = 64 * 1024;
 
-// The two-pass launches of a bridge plan (see bridge_events_kernel).
+// The two bridge passes (events, then the daily fill of the queued legs)
+// over the interval rows at d_rows; `base` holds the sample blocks' Sobol
+// base points and the table rows follow d_rows at table_offset (LDS Sobol).
 template <bool SOBOL, bool FP32, bool LDS>
-static void launch_bridge(kg_context* ctx, const AcPlanDev& p, const Sampler& s, const uint64_t* dirs,
-                          const uint64_t* shifts, Moments* parts, uint32_t* flags, const unsigned char* d_rows,
-                          size_t table_offset) {
+static void launch_bridge_passes(kg_context* ctx, const AcPlanDev& p, const Sampler& s, const uint64_t* dirs,
+                                 const uint64_t* shifts, Moments* parts, uint32_t* flags,
+                                 const unsigned char* d_rows, size_t table_offset, const SobolWord<FP32>* base) {
   using R = typename std::conditional<FP32, float, double>::type;
   using W = SobolWord<FP32>;
   const int dims = p.n_steps;
@@ -505,17 +569,12 @@ static void launch_bridge(kg_context* ctx, const AcPlanDev& p, const Sampler& s,
   // Per pass-one wave: a region of REGION items and its count.
   uint32_t* queue = (uint32_t*)ctx->bridge_queue.reserve(sizeof(uint32_t) * regions * (REGION + 1));
   uint32_t* counts = queue + regions * REGION;
-  const W* base = SOBOL && LDS ? bridge_base_rows<FP32>(ctx, dims, s.seed, blocks, dirs, shifts) : nullptr;
   size_t events_lds = (size_t)p.n_intervals * sizeof(IntervalRow<R>), fill_lds = 0;
   if (SOBOL && LDS) {
     events_lds += (size_t)p.n_intervals * 32 * sizeof(W);
     fill_lds = sobol_lds_bytes(dims, FP32) + (size_t)FILL_WAVES * (dims + 1) * sizeof(W);
   }
   if (events_lds > BRIDGE_EVENTS_LDS_LIMIT) fail(KG_DEVICE_ERROR, "bridge plan exceeds its pass-one LDS");
-  if (SOBOL && LDS)
-    hipLaunchKernelGGL(bridge_table_kernel<FP32>, dim3((unsigned)((p.n_intervals * 16 + 255) / 256)), dim3(256), 0,
-                       ctx->stream, (const IntervalRow<R>*)d_rows, p.n_intervals, dirs, dims,
-                       (W*)const_cast<unsigned char*>(d_rows + table_offset));
   // The call's device time covers the two sampling kernels only.
   timing_begin(ctx);
   hipLaunchKernelGGL((bridge_events_kernel<SOBOL, FP32, LDS>), dim3(blocks), dim3(256), events_lds, ctx->stream, p,
@@ -529,6 +588,58 @@ static void launch_bridge(kg_context* ctx, const AcPlanDev& p, const Sampler& s,
                      dim3(FILL_THREADS), fill_lds, ctx->stream, p, s.seed, samples, s.path_count, (int)s.antithetic,
                      blocks, dirs, shifts, base, queue, counts, values, parts, flags);
   KG_HIP(hipGetLastError());
+}
+
+// The two-pass launches of a bridge plan (see bridge_events_kernel): for LDS
+// Sobol, the base rows (kept per dimensions, seed and block count) and the
+// interval table rows first.
+template <bool SOBOL, bool FP32, bool LDS>
+static void launch_bridge(kg_context* ctx, const AcPlanDev& p, const Sampler& s, const uint64_t* dirs,
+                          const uint64_t* shifts, Moments* parts, uint32_t* flags, const unsigned char* d_rows,
+                          size_t table_offset) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  using W = SobolWord<FP32>;
+  const int dims = p.n_steps;
+  const W* base = SOBOL && LDS ? bridge_base_rows<FP32>(ctx, dims, s.seed, s.blocks(), dirs, shifts) : nullptr;
+  if (SOBOL && LDS)
+    hipLaunchKernelGGL(bridge_table_kernel<FP32>, dim3((unsigned)((p.n_intervals * 16 + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (const IntervalRow<R>*)d_rows, p.n_intervals, dirs, dims,
+                       (W*)const_cast<unsigned char*>(d_rows + table_offset));
+  launch_bridge_passes<SOBOL, FP32, LDS>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset, base);
+}
+
+// A Sobol bridge note with LDS tables: the scrambled directions (kept per
+// dimensions and seed), then bridge_setup_kernel (flag word, base rows unless
+// kept for dimensions, seed and block count, interval table rows), then the
+// two passes.
+template <bool FP32>
+static void launch_bridge_fused(kg_context* ctx, const AcPlanDev& p, const Sampler& s, Moments* parts,
+                                uint32_t* flags, const unsigned char* d_rows, size_t table_offset) {
+  using R = typename std::conditional<FP32, float, double>::type;
+  using W = SobolWord<FP32>;
+  const int dims = p.n_steps;
+  const int64_t blocks = s.blocks();
+  const SobolSetup& so = sobol_setup(ctx, (uint32_t)dims, s.seed);
+  const uint64_t* dirs = (const uint64_t*)so.d_directions.ptr;
+  const uint64_t* shifts = (const uint64_t*)so.d_shifts.ptr;
+  BridgeBaseKey& key = ctx->bridge_base_key;
+  const bool have_base =
+      key.valid && key.dims == dims && key.seed == s.seed && key.blocks == blocks && key.fp32 == FP32;
+  key.valid = false;
+  // One word per dimension per sample block, plus one: the fill kernel
+  // copies dims + 1 words of a row.
+  W* base = (W*)(have_base ? ctx->bridge_base.ptr
+                           : ctx->bridge_base.reserve(sizeof(W) * ((size_t)blocks * dims + 1)));
+  if (1 + (64 - __builtin_clzll((uint64_t)blocks)) > BASE_ROW_COLS)
+    fail(KG_INVALID_INPUT, "too many sample blocks for the bridge set-up");
+  const unsigned groups = have_base ? 1u : (unsigned)((blocks + BASE_ROWS_PER_GROUP - 1) / BASE_ROWS_PER_GROUP);
+  hipLaunchKernelGGL(bridge_setup_kernel<FP32>, dim3(groups), dim3(256), 0, ctx->stream, dims, dirs, shifts, blocks,
+                     have_base ? (W*)nullptr : base,
+                     (const IntervalRow<R>*)d_rows, p.n_intervals,
+                     (W*)const_cast<unsigned char*>(d_rows + table_offset), flags);
+  KG_HIP(hipGetLastError());
+  key = BridgeBaseKey{dims, s.seed, blocks, FP32, true};
+  launch_bridge_passes<true, FP32, true>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset, base);
 }
 
 template <bool SOBOL, bool FP32>
@@ -676,63 +787,116 @@ using namespace kg;
 namespace kg {
 
 // One sampled note on its own: the single-plan kernel and the block merge.
-static void price_note(kg_context* ctx, const Sampler& s, AcHostPlan& h, kg_estimate* out) {
+// The tables of one sampled note as uploaded: the plan's tables and, for
+// two-pass bridge plans, the interval rows (and room for the Sobol table
+// rows that bridge_table_kernel fills).
+struct StagedNote {
+  std::vector<unsigned char> blob;
+  AcOffsets off{};
+  size_t rows_at = 0, table_offset = 0;
+  bool two_pass = false;
+};
+
+static void stage_note(kg_context* ctx, const Sampler& s, const AcHostPlan& h, StagedNote& st) {
   // Bridge plans run in two passes (queue items carry the sample index in
   // 31 bits) while pass one's interval rows fit its LDS; other plans, and
   // larger ones, in one. The two-pass interval rows follow the plan's tables
   // in the same upload.
-  const bool two_pass = h.dev.bridge && s.samples() < ((int64_t)1 << 31) &&
-                        bridge_events_lds(h.dev.n_intervals, h.dev.n_steps, s.sobol, s.fp32) <=
-                            BRIDGE_EVENTS_LDS_LIMIT;
-  std::vector<unsigned char> blob;
-  const AcOffsets off = stage_plan(blob, h);
-  size_t rows_at = 0, table_offset = 0;
-  if (two_pass) {
+  st.two_pass = h.dev.bridge && s.samples() < ((int64_t)1 << 31) &&
+                bridge_events_lds(h.dev.n_intervals, h.dev.n_steps, s.sobol, s.fp32) <= BRIDGE_EVENTS_LDS_LIMIT;
+  st.blob.clear();
+  st.off = stage_plan(st.blob, h);
+  st.rows_at = st.table_offset = 0;
+  if (st.two_pass) {
     std::vector<unsigned char> rows;
     if (s.sobol && s.fp32)
-      bridge_rows_any<true, true>(ctx, h, s.seed, rows, &table_offset);
+      bridge_rows_any<true, true>(ctx, h, s.seed, rows, &st.table_offset);
     else if (s.sobol)
-      bridge_rows_any<true, false>(ctx, h, s.seed, rows, &table_offset);
+      bridge_rows_any<true, false>(ctx, h, s.seed, rows, &st.table_offset);
     else if (s.fp32)
-      bridge_rows_any<false, true>(ctx, h, s.seed, rows, &table_offset);
+      bridge_rows_any<false, true>(ctx, h, s.seed, rows, &st.table_offset);
     else
-      bridge_rows_any<false, false>(ctx, h, s.seed, rows, &table_offset);
-    rows_at = (blob.size() + 255) & ~(size_t)255;
-    blob.resize(rows_at);
-    blob.insert(blob.end(), rows.begin(), rows.end());
+      bridge_rows_any<false, false>(ctx, h, s.seed, rows, &st.table_offset);
+    st.rows_at = (st.blob.size() + 255) & ~(size_t)255;
+    st.blob.resize(st.rows_at);
+    st.blob.insert(st.blob.end(), rows.begin(), rows.end());
   }
-  unsigned char* d = (unsigned char*)ctx->scratch.reserve(std::max<size_t>(blob.size(), 1));
-  upload_async(ctx, d, blob.data(), blob.size());
-  const AcPlanDev p = bind_plan(h, d, off);
-  const unsigned char* d_rows = d + rows_at;
+}
+
+static void run_note(kg_context* ctx, const Sampler& s, const AcHostPlan& h, const StagedNote& st,
+                     kg_estimate* out) {
+  unsigned char* d = (unsigned char*)ctx->scratch.reserve(std::max<size_t>(st.blob.size(), 1));
+  upload_async(ctx, d, st.blob.data(), st.blob.size());
+  const AcPlanDev p = bind_plan(h, d, st.off);
+  const unsigned char* d_rows = d + st.rows_at;
+  const size_t table_offset = st.table_offset;
   const int64_t blocks = s.blocks();
   Moments* parts = (Moments*)ctx->partials.reserve(sizeof(Moments) * 2 * blocks);
-  uint32_t* flags = reset_flags(ctx);
-  const uint64_t* dirs = nullptr;
-  const uint64_t* shifts = nullptr;
-  if (s.sobol) {
-    const SobolSetup& so = sobol_setup(ctx, (uint32_t)p.n_steps, s.seed);
-    dirs = (const uint64_t*)so.d_directions.ptr;
-    shifts = (const uint64_t*)so.d_shifts.ptr;
-  }
-  if (two_pass) {
-    if (s.sobol && s.fp32)
-      launch_bridge_any<true, true>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
-    else if (s.sobol)
-      launch_bridge_any<true, false>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
-    else if (s.fp32)
-      launch_bridge_any<false, true>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
+  if (st.two_pass && s.sobol && sobol_lds_bytes(p.n_steps, s.fp32) <= SOBOL_LDS_LIMIT) {
+    // One set-up launch, then the two passes (the flag word is cleared by
+    // the set-up kernel).
+    uint32_t* flags = (uint32_t*)ctx->flags.reserve(256);
+    if (s.fp32)
+      launch_bridge_fused<true>(ctx, p, s, parts, flags, d_rows, table_offset);
     else
-      launch_bridge_any<false, false>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
+      launch_bridge_fused<false>(ctx, p, s, parts, flags, d_rows, table_offset);
   } else {
-    timing_begin(ctx);
-    launch_any(ctx, p, p.n_steps, s, parts, flags);
+    uint32_t* flags = reset_flags(ctx);
+    const uint64_t* dirs = nullptr;
+    const uint64_t* shifts = nullptr;
+    if (s.sobol) {
+      const SobolSetup& so = sobol_setup(ctx, (uint32_t)p.n_steps, s.seed);
+      dirs = (const uint64_t*)so.d_directions.ptr;
+      shifts = (const uint64_t*)so.d_shifts.ptr;
+    }
+    if (st.two_pass) {
+      if (s.sobol && s.fp32)
+        launch_bridge_any<true, true>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
+      else if (s.sobol)
+        launch_bridge_any<true, false>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
+      else if (s.fp32)
+        launch_bridge_any<false, true>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
+      else
+        launch_bridge_any<false, false>(ctx, p, s, dirs, shifts, parts, flags, d_rows, table_offset);
+    } else {
+      timing_begin(ctx);
+      launch_any(ctx, p, p.n_steps, s, parts, flags);
+    }
   }
   Moments r[2];
   *out = kg_estimate{};
   finish_sampling(ctx, parts, blocks, 2, r, &out->device_ms);
   out->path_steps = (double)s.path_count * (double)p.n_steps;
   fill_estimate(s, r[0], r[1], 0.0, false, true, out);
+}
+
+static void price_note(kg_context* ctx, const Sampler& s, AcHostPlan& h, kg_estimate* out) {
+  StagedNote st;
+  stage_note(ctx, s, h, st);
+  run_note(ctx, s, h, st, out);
+}
+
+// The context's last note plan: the terms (with the events they point to),
+// the market, and the settings bits the staged tables depend on, as bytes.
+// A call whose key matches reuses the plan and its staged tables; neither
+// depends on the seed.
+struct NotePlan {
+  std::vector<unsigned char> key;
+  AcHostPlan plan;
+  StagedNote staged;
+  uint32_t staged_bits = 0;
+  bool staged_ok = false;
+};
+
+static std::vector<unsigned char> note_key(const kg_autocall_terms& t, const kg_bsm_market& m) {
+  kg_autocall_terms copy = t;
+  copy.events = nullptr;
+  const size_t events = t.n_events > 0 && t.events ? (size_t)t.n_events * sizeof(kg_autocall_event) : 0;
+  std::vector<unsigned char> key(sizeof copy + events + sizeof m);
+  std::memcpy(key.data(), &copy, sizeof copy);
+  if (events) std::memcpy(key.data() + sizeof copy, t.events, events);
+  std::memcpy(key.data() + sizeof copy + events, &m, sizeof m);
+  return key;
 }
 
 // Up to AC_MAX_SCENARIOS sampled plans on one step grid (same_grid) as lanes
@@ -811,14 +975,36 @@ extern "C" kg_status kg_autocallable_price(kg_context* ctx, const kg_settings* s
                                            const kg_bsm_market* market, kg_estimate* out) {
   return guarded(ctx, [&] {
     if (!note || !market || !out) fail(KG_INVALID_INPUT, "arguments must not be null");
-    AcHostPlan h = build_plan(*note, *market);
+    std::vector<unsigned char> key = note_key(*note, *market);
+    auto cached = std::static_pointer_cast<NotePlan>(ctx->note_plan);
+    if (!cached || cached->key != key) {
+      ctx->note_plan.reset();
+      auto made = std::make_shared<NotePlan>();
+      made->plan = build_plan(*note, *market);
+      made->key = std::move(key);
+      ctx->note_plan = made;
+      cached = made;
+    }
+    const AcHostPlan& h = cached->plan;
     *out = kg_estimate{};
     if (h.fixed) {
       fill_known(h.fixed_value, settings && settings->sampling == KG_SOBOL, out);
       return;
     }
     Sampler s = validate_settings(settings, (uint64_t)h.dev.n_steps);
-    price_note(ctx, s, h, out);
+    // The staged tables depend on the sampling, precision and whether the
+    // samples fit the two-pass queue items; a call differing in those
+    // restages.
+    StagedNote& st = cached->staged;
+    const uint32_t bits = (uint32_t)s.sobol | ((uint32_t)s.fp32 << 1) |
+                          ((uint32_t)(s.samples() < ((int64_t)1 << 31)) << 2);
+    if (!cached->staged_ok || cached->staged_bits != bits) {
+      cached->staged_ok = false;
+      stage_note(ctx, s, h, st);
+      cached->staged_bits = bits;
+      cached->staged_ok = true;
+    }
+    run_note(ctx, s, h, st, out);
   });
 }
 

@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -115,6 +116,10 @@ struct kg_context {
   kg::DeviceBuffer policy, lsmc_tables, mlp_state;
   int tuned_spt[2][kg::BOOK_CLASSES] = {};
   int tuned_wpc[2][kg::BOOK_CLASSES] = {};
+  // The last autocallable note's host plan and staged tables, reused by a
+  // call on the same terms, market and precision (any seed); its type lives
+  // in autocall.hip.
+  std::shared_ptr<void> note_plan;
   // The book's instantiated launch graph, kept across calls while its launch
   // shape and buffers (the key) stay the same; the contracts and tiles it
   // reads are uploaded before each launch.

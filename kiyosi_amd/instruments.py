@@ -54,12 +54,28 @@ class EuropeanOption:
                 or context.deterministic_crosses(self.expiry_time, self.strike, radius))
 
 
-class KnockInObservation:
+class _Record:
+    """An immutable record, as kiyosi's observation classes are
+    (pyclass(frozen), crates/kiyosi-python/src/instruments/autocallable.rs):
+    fields are set by the constructor only."""
+
+    __slots__ = ()
+
+    def __setattr__(self, name, value):
+        raise AttributeError(f"{type(self).__name__} is immutable")
+
+    def _set(self, **fields):
+        for name, value in fields.items():
+            object.__setattr__(self, name, value)
+
+
+class KnockInObservation(_Record):
     """Daily knock-in at a clock time, or knock-in observed only at expiry."""
 
+    __slots__ = ("_kind", "_clock")
+
     def __init__(self, kind, observation_clock_time=None):
-        self._kind = kind
-        self._clock = observation_clock_time
+        self._set(_kind=kind, _clock=observation_clock_time)
 
     @staticmethod
     def every_trading_day(observation_clock_time=_dt.time(0, 0)):
@@ -76,27 +92,31 @@ class KnockInObservation:
         return self._clock
 
 
-class SnowballObservation:
+class SnowballObservation(_Record):
     """Knock-out observation with an annual coupon accrued on knock-out."""
 
+    __slots__ = ("time", "knock_out_level", "coupon_rate")
+
     def __init__(self, time, knock_out_level, coupon_rate):
-        self.time = time
-        self.knock_out_level = _positive("knock_out_level", knock_out_level)
-        self.coupon_rate = _finite("coupon_rate", coupon_rate)
+        self._set(time=time, knock_out_level=_positive("knock_out_level", knock_out_level),
+                  coupon_rate=_finite("coupon_rate", coupon_rate))
 
 
-class KnockOutObservation:
+class KnockOutObservation(_Record):
+    __slots__ = ("time", "knock_out_level")
+
     def __init__(self, time, knock_out_level):
-        self.time = time
-        self.knock_out_level = _positive("knock_out_level", knock_out_level)
+        self._set(time=time, knock_out_level=_positive("knock_out_level", knock_out_level))
 
 
-class CouponObservation:
+class CouponObservation(_Record):
+    __slots__ = ("time", "coupon_barrier_level")
+
     def __init__(self, time, coupon_barrier_level):
-        self.time = time
-        self.coupon_barrier_level = _finite("coupon_barrier_level", coupon_barrier_level)
-        if self.coupon_barrier_level < 0:
+        level = _finite("coupon_barrier_level", coupon_barrier_level)
+        if level < 0:
             raise InvalidInputError("coupon_barrier_level must be nonnegative")
+        self._set(time=time, coupon_barrier_level=level)
 
 
 _STATES = {None: _abi.STATE_UNKNOWN, "no_event": _abi.STATE_NO_EVENT,
@@ -105,10 +125,18 @@ _STATES = {None: _abi.STATE_UNKNOWN, "no_event": _abi.STATE_NO_EVENT,
 
 class _Autocallable:
     """Shared terms of snowball variants and Phoenix notes
-    (instruments/autocallable.rs:700-870)."""
+    (instruments/autocallable.rs:700-870).
+
+    The C form of the terms is built on first use and kept: the observation
+    sequences are tuples of immutable records, and assigning any attribute of
+    the note drops the kept form."""
 
     _coupon_kind = _abi.COUPON_SNOWBALL
     _redemption_kind = _abi.REDEMPTION_DOWNSIDE
+
+    def __setattr__(self, name, value):
+        self.__dict__.pop("_c_terms", None)
+        object.__setattr__(self, name, value)
 
     def _common(self, *, initial_spot, knock_in_level, upper_strike, lower_strike,
                 knock_in_observation, effective_time, expiry_time, barrier_state,
@@ -159,6 +187,13 @@ class _Autocallable:
         return ordered
 
     def _terms(self):
+        terms = self.__dict__.get("_c_terms")
+        if terms is None:
+            terms = self._build_terms()
+            self.__dict__["_c_terms"] = terms
+        return terms
+
+    def _build_terms(self):
         events = self._events()
         array = (_abi.AutocallEvent * max(len(events), 1))()
         for i, e in enumerate(events):
@@ -306,7 +341,7 @@ class SnowballOption(_Autocallable):
                  autocall_observations, barrier_state=None, principal_redemption_ratio=1.0,
                  rebate_rate=0.0, annual_rebate_rate=0.0, calendar=None):
         self.maturity_coupon_rate = _finite("maturity_coupon_rate", maturity_coupon_rate)
-        self.autocall_observations = list(autocall_observations)
+        self.autocall_observations = tuple(autocall_observations)
         if not self.autocall_observations:
             raise InvalidInputError("autocall_observations must be nonempty")
         self._common(initial_spot=initial_spot, knock_in_level=knock_in_level,
@@ -332,8 +367,8 @@ class PhoenixOption(_Autocallable):
                  principal_redemption_ratio=1.0, rebate_rate=0.0, annual_rebate_rate=0.0,
                  calendar=None):
         self.coupon_per_period = _finite("coupon_per_period", coupon_per_period)
-        self.knock_out_observations = list(knock_out_observations)
-        self.coupon_observations = list(coupon_observations)
+        self.knock_out_observations = tuple(knock_out_observations)
+        self.coupon_observations = tuple(coupon_observations)
         self._common(initial_spot=initial_spot, knock_in_level=knock_in_level,
                      upper_strike=upper_strike, lower_strike=lower_strike,
                      knock_in_observation=knock_in_observation, effective_time=effective_time,
@@ -353,7 +388,7 @@ class BinarySnowballOption(_Autocallable):
                  barrier_state=None, principal_redemption_ratio=1.0, rebate_rate=0.0,
                  annual_rebate_rate=0.0, calendar=None, initial_spot=None):
         self.maturity_coupon_rate = _finite("maturity_coupon_rate", maturity_coupon_rate)
-        self.autocall_observations = list(autocall_observations)
+        self.autocall_observations = tuple(autocall_observations)
         spot = initial_spot or self.autocall_observations[0].knock_out_level
         self._common(initial_spot=spot, knock_in_level=None, upper_strike=None, lower_strike=None,
                      knock_in_observation=None, effective_time=effective_time,
@@ -377,7 +412,7 @@ class TernarySnowballOption(_Autocallable):
                  annual_rebate_rate=0.0, calendar=None):
         self.maturity_coupon_rate = _finite("maturity_coupon_rate", maturity_coupon_rate)
         self.minimum_coupon_rate = _finite("minimum_coupon_rate", minimum_coupon_rate)
-        self.autocall_observations = list(autocall_observations)
+        self.autocall_observations = tuple(autocall_observations)
         self._common(initial_spot=knock_in_level, knock_in_level=knock_in_level,
                      upper_strike=None, lower_strike=None,
                      knock_in_observation=knock_in_observation, effective_time=effective_time,

@@ -206,14 +206,11 @@ def _snowball_with_coupon(note, coupon, convention):
     """SnowballCoupon::with_coupon (instruments/autocallable.rs:647-660):
     every observation coupon moves by the shift of the first; with
     ShiftMaturityCoupon the maturity coupon moves too."""
+    from .instruments import SnowballObservation
     out = copy.copy(note)
     shift = coupon - (note.autocall_observations[0].coupon_rate if note.autocall_observations else 0.0)
-    obs = []
-    for o in note.autocall_observations:
-        moved = copy.copy(o)
-        moved.coupon_rate = o.coupon_rate + shift
-        obs.append(moved)
-    out.autocall_observations = obs
+    out.autocall_observations = tuple(SnowballObservation(o.time, o.knock_out_level, o.coupon_rate + shift)
+                                      for o in note.autocall_observations)
     if convention == SHIFT_MATURITY_COUPON:
         out.maturity_coupon_rate = note.maturity_coupon_rate + shift
     return out

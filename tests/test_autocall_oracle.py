@@ -49,10 +49,14 @@ def test_certain_knock_out_at_the_first_observation(oracle):
     assert p.price == pytest.approx((1 + 0.15 * t) * math.exp(-0.02 * t), rel=1e-12)
 
 
+def unreachable(observations):
+    """The same observation dates and coupons with knock-out levels no path reaches."""
+    return tuple(SnowballObservation(o.time, 1e6, o.coupon_rate) for o in observations)
+
+
 def test_unreachable_barriers_pay_the_maturity_coupon(oracle):
     note = workloads.snowball_note(knock_in=1e-6)
-    for obs in note.autocall_observations:
-        obs.knock_out_level = 1e6
+    note.autocall_observations = unreachable(note.autocall_observations)
     p = price(oracle, note, ctx(), settings(2048))
     t = years_until(V, note.expiry_time)
     assert p.price == pytest.approx((1 + 0.15 * t) * math.exp(-0.02 * t), rel=1e-12)
@@ -63,8 +67,7 @@ def test_certain_knock_in_pays_the_capped_downside(oracle):
     # Value = e^{-rT} (1 - E[(100 - S_T)^+] / 100) = 1 e^{-rT} - put/100.
     note = workloads.snowball_note(knock_in=1e6)
     note.knock_in_level = 1e6
-    for obs in note.autocall_observations:
-        obs.knock_out_level = 1e6
+    note.autocall_observations = unreachable(note.autocall_observations)
     p = price(oracle, note, ctx(), settings(400_000))
     t = years_until(V, note.expiry_time)
     d1 = (math.log(100 / 100) + (0.02 + 0.5 * 0.25**2) * t) / (0.25 * math.sqrt(t))
