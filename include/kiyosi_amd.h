@@ -257,19 +257,31 @@ kg_status kg_autocallable_scenarios(kg_context* ctx, const kg_settings* settings
                                     const kg_autocall_terms* note, const kg_bsm_market* markets,
                                     int32_t n, kg_estimate* out);
 
-/* ---- Autocallable portfolio: MonteCarloAutocallableEngine::price_many and
- * price_many_on_shared_grid (engines/monte_carlo/autocallable/portfolio.rs
- * :187-318; Python binding crates/kiyosi-python/src/engines/monte_carlo.rs
- * :699-746). Every note is priced with the one resolved seed of `settings`,
- * and out[i] equals kg_autocallable_price for notes[i] bit for bit, as the
- * reference's batch equals its scalar prices
- * (crates/kiyosi/tests/autocallable_monte_carlo.rs:52-99). Notes whose
- * plans share the step grid (steps, events, knock-in observations and
- * bridge) run as lanes on one path set; the others run alone. At most
- * 1,048,576 notes (binding_support::validate_contract_batch_size). */
+/* ---- Autocallable portfolio: MonteCarloAutocallableEngine::price_many
+ * (engines/monte_carlo/autocallable/portfolio.rs:187-318; Python binding
+ * crates/kiyosi-python/src/engines/monte_carlo.rs:699-746). Every note is
+ * priced with the one resolved seed of `settings`, and out[i] equals
+ * kg_autocallable_price for notes[i] bit for bit, as the reference's batch
+ * equals its scalar prices (crates/kiyosi/tests/autocallable_monte_carlo.rs
+ * :52-99). Notes whose plans share the step grid (steps, events, knock-in
+ * observations and bridge) run as lanes on one path set; the others run
+ * alone. At most 1,048,576 notes (binding_support::validate_contract_batch_size). */
 kg_status kg_autocallable_price_many(kg_context* ctx, const kg_settings* settings,
                                      const kg_autocall_terms* notes, size_t n,
                                      const kg_bsm_market* market, kg_estimate* out);
+
+/* price_many_on_shared_grid (the same portfolio.rs:187-318 with
+ * share_daily_grid): a note with daily knock-in monitoring whose future
+ * events all fall on its daily observation times (or at expiry), and whose
+ * daily grid fits the reference's grid slot (DailyGrids::prepare,
+ * portfolio.rs:75-186), is sampled on that daily grid step by step, without
+ * the Brownian bridge of its scalar plan (autocallable.rs:341-374); the
+ * other notes keep their own plans. out[i] is the note's price on the
+ * batch's draws, which the reference's coupon-family and threshold-strip
+ * estimators (portfolio.rs:570-825) reproduce to rounding. */
+kg_status kg_autocallable_price_many_on_shared_grid(kg_context* ctx, const kg_settings* settings,
+                                                    const kg_autocall_terms* notes, size_t n,
+                                                    const kg_bsm_market* market, kg_estimate* out);
 
 /* ---- Heston path stepping. kiyosi has Heston only in its Fourier and FD
  * engines (market/heston.rs:58-227, engines/analytic/heston.rs,

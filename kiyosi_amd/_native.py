@@ -46,6 +46,8 @@ SIGNATURES = {
                                             P(_abi.BsmMarket), C.c_int32, P(_abi.Estimate)]),
     "kg_autocallable_price_many": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AutocallTerms), C.c_size_t,
                                              P(_abi.BsmMarket), P(_abi.Estimate)]),
+    "kg_autocallable_price_many_on_shared_grid": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.AutocallTerms),
+                                                            C.c_size_t, P(_abi.BsmMarket), P(_abi.Estimate)]),
     "kg_heston_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.HestonTerms), P(_abi.HestonMarket),
                                   P(_abi.Estimate)]),
     "kg_barrier_price": (C.c_int, [C.c_void_p, P(_abi.Settings), P(_abi.BarrierTerms), P(_abi.BsmMarket),

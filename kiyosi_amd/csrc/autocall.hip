@@ -1039,10 +1039,12 @@ extern "C" kg_status kg_autocallable_scenarios(kg_context* ctx, const kg_setting
   });
 }
 
-extern "C" kg_status kg_autocallable_price_many(kg_context* ctx, const kg_settings* settings,
-                                                const kg_autocall_terms* notes, size_t n,
-                                                const kg_bsm_market* market, kg_estimate* out) {
-  return guarded(ctx, [&] {
+// price_many and price_many_on_shared_grid: notes grouped by step grid, each
+// group's notes as lanes of one path set. On shared grids a note with a
+// daily grid is planned without the bridge (build_plan).
+static void price_many(kg_context* ctx, const kg_settings* settings, const kg_autocall_terms* notes, size_t n,
+                       const kg_bsm_market* market, kg_estimate* out, bool shared_grid) {
+  {
     if (n == 0) return;
     if (!notes || !market || !out || !settings) fail(KG_INVALID_INPUT, "arguments must not be null");
     if (n > BATCH_NOTES) fail(KG_INVALID_INPUT, "a batch holds at most 1,048,576 notes");
@@ -1055,7 +1057,7 @@ extern "C" kg_status kg_autocallable_price_many(kg_context* ctx, const kg_settin
       std::unordered_map<uint64_t, std::vector<size_t>> by_hash;
       for (size_t i = c0; i < c1; ++i) {
         AcHostPlan& p = h[i - c0];
-        p = build_plan(notes[i], *market);
+        p = build_plan(notes[i], *market, shared_grid);
         out[i] = kg_estimate{};
         if (p.fixed) {
           fill_known(p.fixed_value, sobol, &out[i]);
@@ -1091,5 +1093,17 @@ extern "C" kg_status kg_autocallable_price_many(kg_context* ctx, const kg_settin
         }
       }
     }
-  });
+  }
+}
+
+extern "C" kg_status kg_autocallable_price_many(kg_context* ctx, const kg_settings* settings,
+                                                const kg_autocall_terms* notes, size_t n,
+                                                const kg_bsm_market* market, kg_estimate* out) {
+  return guarded(ctx, [&] { price_many(ctx, settings, notes, n, market, out, false); });
+}
+
+extern "C" kg_status kg_autocallable_price_many_on_shared_grid(kg_context* ctx, const kg_settings* settings,
+                                                               const kg_autocall_terms* notes, size_t n,
+                                                               const kg_bsm_market* market, kg_estimate* out) {
+  return guarded(ctx, [&] { price_many(ctx, settings, notes, n, market, out, true); });
 }

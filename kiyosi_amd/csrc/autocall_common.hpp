@@ -885,7 +885,42 @@ inline void validate_terms(const kg_autocall_terms& t) {
     fail(KG_INVALID_INPUT, "lower_strike must lie in [0, upper_strike]");
 }
 
-inline AcHostPlan build_plan(const kg_autocall_terms& t, const kg_bsm_market& m) {
+// Whether a batch priced on shared daily grids gives this note the daily
+// grid of its expiry, calendar and clock (DailyGrids::prepare,
+// engines/monte_carlo/autocallable/portfolio.rs:75-186): the grid of
+// daily observation times after valuation (plus expiry when it falls off
+// the grid) must be small enough for the reference's 2 MiB grid slot (a
+// 72-byte path step and an 8-byte time per day, its time vector grown by
+// doubling from four) and must hold every future event. A note on such a
+// grid is sampled step by step, without the Brownian bridge
+// (autocallable.rs:341-374 leaves `bridge` empty); its dates, transitions,
+// events and knock-in fixings are those of its own plan.
+inline bool shared_daily_grid(const kg_autocall_terms& t, const NoteView& note, int64_t valuation) {
+  constexpr uint64_t GRID_BYTES = 2u * 1024u * 1024u, STEP_BYTES = 72, TIME_BYTES = 8;
+  const int64_t days = floor_div(t.expiry_time, MICROS_PER_DAY) - floor_div(valuation, MICROS_PER_DAY) + 1;
+  if ((uint64_t)days >= GRID_BYTES / (TIME_BYTES + STEP_BYTES)) return false;
+  for (int i = 0; i < t.n_events; ++i) {
+    const int64_t time = t.events[i].time;
+    if (time > valuation && time != t.expiry_time && !note.daily_time(time)) return false;
+  }
+  uint64_t n = 0;
+  for (int64_t d = floor_div(valuation, MICROS_PER_DAY); d <= floor_div(t.expiry_time, MICROS_PER_DAY); ++d) {
+    const int64_t at = d * MICROS_PER_DAY + t.observation_clock_micros;
+    if (at > valuation && at <= t.expiry_time && trading_day(t.calendar, d)) ++n;
+  }
+  if (n == 0) return false;
+  uint64_t capacity = 4;
+  while (capacity < n) capacity *= 2;
+  const int64_t last = floor_div(t.expiry_time, MICROS_PER_DAY) * MICROS_PER_DAY + t.observation_clock_micros;
+  const bool on_grid = last == t.expiry_time && trading_day(t.calendar, floor_div(t.expiry_time, MICROS_PER_DAY));
+  if (!on_grid && ++n > capacity) capacity *= 2;
+  return capacity * TIME_BYTES + n * STEP_BYTES <= GRID_BYTES;
+}
+
+// The sampled plan of a note (or its known value). With `shared_grid` the
+// note is planned as a batch on shared daily grids plans it
+// (price_many_on_shared_grid, autocallable/portfolio.rs:187-318).
+inline AcHostPlan build_plan(const kg_autocall_terms& t, const kg_bsm_market& m, bool shared_grid = false) {
   validate_terms(t);
   NoteView note{t};
   AcHostPlan h;
@@ -1026,7 +1061,7 @@ inline AcHostPlan build_plan(const kg_autocall_terms& t, const kg_bsm_market& m)
   // BridgePlan::new (autocallable.rs:463-515)
   const bool safe = excursion >= p.safe_lo && excursion <= p.safe_hi && -excursion >= p.safe_lo &&
                     -excursion <= p.safe_hi;
-  if (daily && safe) {
+  if (daily && safe && !(shared_grid && shared_daily_grid(t, note, valuation))) {
     std::vector<AcIntervalD> iv;
     size_t first = 0;
     bool gap = false;

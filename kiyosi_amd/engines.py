@@ -253,9 +253,14 @@ class MonteCarloAutocallableEngine(_MonteCarloEngine):
     def price_many(self, instruments, context, shared_grid=False):
         """Prices of notes of one product type on one resolved seed
         (price_many / price_many_on_shared_grid, crates/kiyosi-python/src/
-        engines/monte_carlo.rs:699-746): each equals ``price`` of that note
-        with the same seed, bit for bit. Notes sharing a step grid run as
-        lanes of one GPU path set; `shared_grid` selects the same prices."""
+        engines/monte_carlo.rs:699-746). Without `shared_grid` each equals
+        ``price`` of that note with the same seed, bit for bit. With it, a
+        note with daily knock-in monitoring whose events fall on its daily
+        observation times is sampled on its shared daily grid step by step,
+        without the Brownian bridge (autocallable/portfolio.rs:75-186); its
+        price is then that of the batch's draws, as the reference's shared
+        grid estimators give it. Notes sharing a step grid run as lanes of
+        one GPU path set."""
         notes = list(instruments)
         if len(notes) > 1 << 20:
             raise InvalidInputError("a batch holds at most 1,048,576 notes")
@@ -270,8 +275,10 @@ class MonteCarloAutocallableEngine(_MonteCarloEngine):
         array = (_abi.AutocallTerms * len(notes))(*terms)
         out = (_abi.Estimate * len(notes))()
         ctx = _native.context(engine.device)
-        ctx.check(_native.lib().kg_autocallable_price_many(ctx.handle, C.byref(engine._settings()), array,
-                                                           len(notes), C.byref(context.bsm_market()), out))
+        entry = (_native.lib().kg_autocallable_price_many_on_shared_grid if shared_grid
+                 else _native.lib().kg_autocallable_price_many)
+        ctx.check(entry(ctx.handle, C.byref(engine._settings()), array, len(notes), C.byref(context.bsm_market()),
+                        out))
         return tuple(o.price for o in out)
 
     def price_with_greeks(self, note, context, request=None, settings=None):

@@ -157,6 +157,18 @@ int oracle_autocallable_price(const kg_settings* s, const kg_autocall_terms* t,
   });
 }
 
+// The price of one note of a price_many_on_shared_grid batch: the note
+// planned on its shared daily grid when it has one (the batch's prices are
+// each note's price on the shared draws; its coupon-family and threshold-
+// strip estimators reproduce them to rounding).
+int oracle_autocallable_price_on_shared_grid(const kg_settings* s, const kg_autocall_terms* t,
+                                             const kg_bsm_market* m, int workers, kg_estimate* out) {
+  return guarded([&] {
+    Settings st = to_settings(s, workers);
+    write(autocall_price(st, *t, *m, context(), true), out);
+  });
+}
+
 int oracle_heston_price(const kg_settings* s, const kg_heston_terms* o, const kg_heston_market* m,
                         int workers, kg_estimate* out) {
   return guarded([&] {

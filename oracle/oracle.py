@@ -46,6 +46,8 @@ def lib():
             P(_abi.Settings), P(_abi.EuropeanTerms), P(_abi.BsmMarket), C.c_int, P(_abi.Estimate)]
         handle.oracle_autocallable_price.argtypes = [
             P(_abi.Settings), P(_abi.AutocallTerms), P(_abi.BsmMarket), C.c_int, P(_abi.Estimate)]
+        handle.oracle_autocallable_price_on_shared_grid.argtypes = [
+            P(_abi.Settings), P(_abi.AutocallTerms), P(_abi.BsmMarket), C.c_int, P(_abi.Estimate)]
         handle.oracle_heston_price.argtypes = [
             P(_abi.Settings), P(_abi.HestonTerms), P(_abi.HestonMarket), C.c_int, P(_abi.Estimate)]
         for name, terms in (("barrier", _abi.BarrierTerms), ("asian", _abi.AsianTerms),
@@ -137,6 +139,14 @@ def vanilla_price(settings, terms, market, workers: int = 1):
 def autocallable_price(settings, terms, market, workers: int = 1):
     out = _abi.Estimate()
     _check(lib().oracle_autocallable_price(C.byref(settings), C.byref(terms), C.byref(market), workers, C.byref(out)))
+    return out
+
+
+def autocallable_price_on_shared_grid(settings, terms, market, workers: int = 1):
+    """The price of one note of a price_many_on_shared_grid batch."""
+    out = _abi.Estimate()
+    _check(lib().oracle_autocallable_price_on_shared_grid(C.byref(settings), C.byref(terms), C.byref(market),
+                                                          workers, C.byref(out)))
     return out
 
 

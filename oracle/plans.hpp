@@ -232,7 +232,43 @@ struct AutocallPlan {
   }
 };
 
-inline AutocallPlan autocall_plan(const Note& note, const kg_bsm_market& m) {
+// DailyGrids::prepare (engines/monte_carlo/autocallable/portfolio.rs:75-186):
+// whether price_many_on_shared_grid places the note on the shared daily grid
+// of its expiry, calendar and clock. The grid holds the daily observation
+// times after valuation, plus expiry when it is off the grid; it must fit
+// the reference's 2 MiB slot (DAILY_GRID_BYTES; 72-byte PathStep, 8-byte
+// Timestamp, the time vector grown from capacity four by doubling, before
+// and after the expiry push) and contain every future event.
+inline bool shared_daily_grid(const Note& note, int64_t valuation) {
+  const kg_autocall_terms& t = *note.t;
+  const uint64_t grid_bytes = 2u * 1024u * 1024u, step_bytes = 72, time_bytes = 8;
+  const int64_t days = day_of(t.expiry_time) - day_of(valuation) + 1;
+  if ((uint64_t)days >= grid_bytes / (time_bytes + step_bytes)) return false;
+  for (int i = 0; i < t.n_events; ++i) {
+    const int64_t time = t.events[i].time;
+    if (time <= valuation) continue;
+    if (time != t.expiry_time && !note.daily_time(time)) return false;
+  }
+  std::vector<int64_t> dates;
+  for (int64_t d = day_of(valuation); d <= day_of(t.expiry_time); ++d) {
+    const int64_t at = d * MICROS_PER_DAY + t.observation_clock_micros;
+    if (at > valuation && at <= t.expiry_time && is_trading_day(t.calendar, d)) dates.push_back(at);
+  }
+  if (dates.empty()) return false;
+  uint64_t capacity = 4;
+  while (capacity < dates.size()) capacity *= 2;
+  if (dates.back() != t.expiry_time) {
+    dates.push_back(t.expiry_time);
+    if (dates.size() > capacity) capacity *= 2;
+  }
+  return capacity * time_bytes + dates.size() * step_bytes <= grid_bytes;
+}
+
+// The plan of a note; with `shared_grid`, as price_many_on_shared_grid plans
+// it: a note on a shared daily grid keeps its dates, transitions, events and
+// knock-in fixings but is sampled step by step, without the Brownian bridge
+// (autocallable.rs:341-374).
+inline AutocallPlan autocall_plan(const Note& note, const kg_bsm_market& m, bool shared_grid = false) {
   const kg_autocall_terms& t = *note.t;
   AutocallPlan p;
   p.note = &note;
@@ -348,7 +384,7 @@ inline AutocallPlan autocall_plan(const Note& note, const kg_bsm_market& m) {
   // BridgePlan::new (autocallable.rs:463-515)
   bool safe = excursion >= p.safe_lo && excursion <= p.safe_hi && -excursion >= p.safe_lo &&
               -excursion <= p.safe_hi;
-  if (daily && safe) {
+  if (daily && safe && !(shared_grid && shared_daily_grid(note, valuation))) {
     std::vector<BridgeInterval> iv;
     size_t first = 0;
     bool has_gap = false;
@@ -480,9 +516,9 @@ inline double autocall_bridge_payoff(const AutocallPlan& p, SkippingPath& normal
 
 // MonteCarloAutocallableEngine::sample + price_with_statistics (autocallable.rs:154-174, 911-918).
 inline Estimate autocall_price(const Settings& st, const kg_autocall_terms& t,
-                               const kg_bsm_market& m, const Context& cx) {
+                               const kg_bsm_market& m, const Context& cx, bool shared_grid = false) {
   Note note{&t};
-  AutocallPlan p = autocall_plan(note, m);
+  AutocallPlan p = autocall_plan(note, m, shared_grid);
   if (p.fixed) {
     if (!std::isfinite(p.fixed_value)) unstable();
     Moments s;

@@ -4,7 +4,8 @@ batch is priced on one resolved seed and equals its scalar price bit for
 bit, as the reference's own test asserts
 (crates/kiyosi/tests/autocallable_monte_carlo.rs:52-99), for notes that
 share a step grid (lanes), notes on other grids, repeated notes, notes with
-a known value and mixed batches across several lane launches."""
+a known value and mixed batches across several lane launches; and the same
+batches on shared daily grids against the oracle's shared-grid prices."""
 
 import datetime as dt
 
@@ -91,7 +92,18 @@ def test_batch_prices_equal_scalar_prices_bit_for_bit(gpu, oracle, name, notes, 
             assert abs(price - ref.price) <= 0.5 * ref.standard_error
         else:
             assert price == pytest.approx(ref.price, rel=1e-3)
-    assert engine.price_many(notes, context(), shared_grid=True) == got
+    # On shared daily grids the notes with daily knock-in step day by day
+    # (no bridge): each price is the oracle's price of that note so planned.
+    shared = engine.price_many(notes, context(), shared_grid=True)
+    for price, note in zip(shared, notes):
+        ref = oracle.autocallable_price_on_shared_grid(engine._settings(), note._terms(), context().bsm_market(),
+                                                       workers=8)
+        if precision == "fp64":
+            assert price == pytest.approx(ref.price, rel=1e-10, abs=1e-13)
+        elif ref.has_standard_error:
+            assert abs(price - ref.price) <= 0.5 * ref.standard_error
+        else:
+            assert price == pytest.approx(ref.price, rel=1e-3)
 
 
 def test_phoenix_batch_and_known_values(gpu, oracle):
