@@ -204,21 +204,17 @@ __global__ void __launch_bounds__(256, 2)
   if (bad) raise_flag(flags, FLAG_NONFINITE);
 }
 
-// One workgroup per contract: its block partials in order, with the thread
-// layout of merge_partials, so a book result equals the single-contract one.
+// One workgroup per contract: its block partials merged as merge_partials
+// merges them (merge_run), so a book result equals the single-contract one.
 __global__ void __launch_bounds__(MERGE_THREADS)
     merge_book(const BookItem* __restrict__ items, const Moments* __restrict__ partials,
                Moments* __restrict__ out) {
-  __shared__ Moments smem[MERGE_THREADS / WAVE];
+  __shared__ double smem[4 * MERGE_THREADS / WAVE];
   const BookItem& it = items[blockIdx.x];
   const int64_t blocks = (it.samples + BLOCK_SAMPLES - 1) / BLOCK_SAMPLES;
-  const int64_t per = (blocks + blockDim.x - 1) / blockDim.x;
-  const int64_t begin = per * threadIdx.x;
-  const int64_t end = begin + per < blocks ? begin + per : blocks;
   for (int s = 0; s < 2; ++s) {
-    Moments acc[1] = {fold_partials(moments_zero(), partials + it.first_block * 2 + s, 2, begin, end)};
-    block_merge<1>(acc, smem);
-    if (threadIdx.x == 0) out[blockIdx.x * 2 + s] = acc[0];
+    const Moments m = merge_run(partials + it.first_block * 2 + s, 2, blocks, smem);
+    if (threadIdx.x == 0) out[blockIdx.x * 2 + s] = m;
   }
 }
 

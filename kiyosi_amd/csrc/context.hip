@@ -11,25 +11,20 @@ extern "C" const uint32_t kg_joe_kuo_end[];
 namespace kg {
 
 // Merges per-block partials [block][slot] for `slots` independent Moments,
-// one workgroup per slot. Thread t folds a contiguous run of blocks in block
-// order; the runs are then merged as a fixed tree. The result depends only
-// on the partials, not on scheduling, so a seed replays exactly
-// (monte_carlo.rs:1800-1809 merges the same partials sequentially). With
+// one workgroup per slot (merge_run: the parallel form of the reference's
+// merge, monte_carlo.rs:1800-1809 and :2192-2222). The result depends only
+// on the partials, not on scheduling, so a seed replays exactly. With
 // `flags`, workgroup 0 also copies the call's flag word behind the slots, so
 // one copy brings back both.
 __global__ void __launch_bounds__(MERGE_THREADS) merge_partials(const Moments* __restrict__ partials,
                                                                 int64_t blocks, int slots,
                                                                 Moments* __restrict__ out,
                                                                 const uint32_t* __restrict__ flags) {
-  __shared__ Moments smem[MERGE_THREADS / WAVE];
+  __shared__ double smem[4 * MERGE_THREADS / WAVE];
   const int s = blockIdx.x;
   if (flags != nullptr && s == 0 && threadIdx.x == 0) *(uint32_t*)(out + slots) = *flags;
-  const int64_t per = (blocks + blockDim.x - 1) / blockDim.x;
-  const int64_t begin = per * threadIdx.x;
-  const int64_t end = begin + per < blocks ? begin + per : blocks;
-  Moments acc[1] = {fold_partials(moments_zero(), partials + s, slots, begin, end)};
-  block_merge<1>(acc, smem);
-  if (threadIdx.x == 0) out[s] = acc[0];
+  const Moments m = merge_run(partials + s, slots, blocks, smem);
+  if (threadIdx.x == 0) out[s] = m;
 }
 
 void merge_and_fetch(kg_context* ctx, const Moments* partials, int64_t blocks, int slots,

@@ -14,9 +14,10 @@ namespace kg {
 
 constexpr int BLOCK_SAMPLES = 256;  // monte_carlo.rs:20
 constexpr int WAVE = 64;
-// Threads of every kernel that merges per-block partials in block order; one
-// layout everywhere keeps the merge tree, and so the result bits, the same.
-constexpr int MERGE_THREADS = 1024;
+// Threads of every kernel that merges per-block partials (merge_run); one
+// layout everywhere keeps the summation order, and so the result bits, the
+// same.
+constexpr int MERGE_THREADS = 256;
 
 // True when `p` holds in some active lane of the wave. The builtin takes the
 // predicate as a lane mask, so no VGPR copy of it is made (HIP's __ballot
@@ -394,22 +395,6 @@ __device__ inline Moments wave_merge(Moments m, int lanes = WAVE) {
   return m;
 }
 
-// Folds partials[b * stride] for b in [begin, end) into acc in block order,
-// issuing the loads eight at a time so the merge chain does not wait on each
-// load in turn.
-__device__ inline Moments fold_partials(Moments acc, const Moments* __restrict__ partials, int64_t stride,
-                                        int64_t begin, int64_t end) {
-  constexpr int CHUNK = 8;
-  for (int64_t b0 = begin; b0 < end; b0 += CHUNK) {
-    Moments tmp[CHUNK];
-#pragma unroll
-    for (int k = 0; k < CHUNK; ++k) tmp[k] = b0 + k < end ? partials[(b0 + k) * stride] : moments_zero();
-#pragma unroll
-    for (int k = 0; k < CHUNK; ++k) acc = moments_merge(acc, tmp[k]);
-  }
-  return acc;
-}
-
 // Block-wide merge of N independent Moments sets; returns the result in
 // thread 0. Each wave merges its lanes, then wave 0 merges the waves' results
 // the same way. `smem` needs N * (blockDim.x / 64) entries.
@@ -435,6 +420,118 @@ __device__ inline void block_merge(Moments (&m)[N], Moments* smem) {
     }
   }
   __syncthreads();
+}
+
+// Lane l's copy of v from the lane that the DPP pattern CTRL names.
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+  const uint32_t hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, (uint64_t)hi << 32 | lo);
+}
+
+__device__ inline double read_lane(double v, int lane) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+  const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+  return __builtin_bit_cast(double, (uint64_t)hi << 32 | lo);
+}
+
+// Sum (with MAX the largest) of v over the wave, the same bits in every
+// lane: four pairwise levels inside each 16-lane row by DPP (lanes l ^ 1 and
+// l ^ 2, then the half-row and row mirrors; a + b and b + a are the same
+// bits, so both lanes of a pair hold the same value), then the four rows'
+// results in row order.
+template <bool MAX>
+__device__ inline double wave_reduce(double v) {
+#pragma clang fp contract(off)
+  auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
+  v = op(v, dpp_move<0xB1>(v));   // quad_perm [1, 0, 3, 2]
+  v = op(v, dpp_move<0x4E>(v));   // quad_perm [2, 3, 0, 1]
+  v = op(v, dpp_move<0x141>(v));  // row_half_mirror
+  v = op(v, dpp_move<0x140>(v));  // row_mirror
+  return op(op(read_lane(v, 0), read_lane(v, 16)), op(read_lane(v, 32), read_lane(v, 48)));
+}
+
+// The waves' results of one workgroup-wide reduction, folded in wave order
+// in every thread. `slot` holds THREADS / 64 doubles used by no other
+// reduction in flight.
+template <bool MAX, int THREADS>
+__device__ inline double across_waves(double w, double* slot) {
+#pragma clang fp contract(off)
+  if ((threadIdx.x & (WAVE - 1)) == 0) slot[threadIdx.x / WAVE] = w;
+  __syncthreads();
+  double r = slot[0];
+#pragma unroll
+  for (int i = 1; i < THREADS / WAVE; ++i) r = MAX ? fmax(r, slot[i]) : r + slot[i];
+  return r;
+}
+
+// The partials p[b * stride], b in [0, blocks), merged into one set: the
+// reference's merge (Moments::merge, monte_carlo.rs:2192-2222) in its
+// parallel form. With S the largest scale and r_b = scale_b / S, the set has
+// count N = sum n_b, scale S, mean sum(n_b mean_b r_b) / N and
+// M2 sum(m2_b r_b^2 + n_b (mean_b r_b - mean)^2): Chan's update with each
+// block's deviation taken from the final mean, so M2 still comes from
+// deviations, while the chain of dependent pairwise merges (each with its
+// divisions) becomes three workgroup sums. Thread t takes blocks
+// t, t + THREADS, ... in that order (held in registers when there are at most
+// CACHED per thread), so the result depends only on the partials. Called by
+// all THREADS threads; every thread returns the result. `smem`: 4 * THREADS
+// / 64 doubles.
+template <int THREADS = MERGE_THREADS, int CACHED = 16>
+__device__ inline Moments merge_run(const Moments* __restrict__ p, int64_t stride, int64_t blocks,
+                                    double* smem) {
+#pragma clang fp contract(off)
+  constexpr int W = THREADS / WAVE;
+  const bool cached = blocks <= (int64_t)CACHED * THREADS;
+  Moments v[CACHED];
+  if (cached) {
+#pragma unroll
+    for (int k = 0; k < CACHED; ++k) {
+      const int64_t b = threadIdx.x + (int64_t)k * THREADS;
+      v[k] = b < blocks ? p[b * stride] : moments_zero();
+    }
+  }
+  // f over this thread's blocks in order. Held in registers, the entries
+  // past the last block are empty sets, whose terms below are exact zeros.
+  auto each = [&](auto&& f) {
+    if (cached) {
+#pragma unroll
+      for (int k = 0; k < CACHED; ++k) f(v[k]);
+    } else {
+      for (int64_t b = threadIdx.x; b < blocks; b += THREADS) f(p[b * stride]);
+    }
+  };
+  double n = 0.0, top = 0.0;
+  each([&](const Moments& m) {
+    n += m.count;
+    top = fmax(top, m.scale);
+  });
+  const double count = across_waves<false, THREADS>(wave_reduce<false>(n), smem);
+  const double scale = across_waves<true, THREADS>(wave_reduce<true>(top), smem + W);
+  if (count == 0.0) return moments_zero();
+  // r_b is 1 for the block that holds S, and 0 for an empty block or when
+  // every value is zero (S = 0).
+  double t = 0.0;
+  each([&](const Moments& m) {
+    const double r = scale == 0.0 ? 0.0 : m.scale / scale;
+    t += m.count * (m.mean * r);
+  });
+  const double mean = across_waves<false, THREADS>(wave_reduce<false>(t), smem + 2 * W) / count;
+  double q = 0.0;
+  each([&](const Moments& m) {
+    const double r = scale == 0.0 ? 0.0 : m.scale / scale;
+    const double d = m.mean * r - mean;
+    q += m.m2 * r * r + m.count * (d * d);
+  });
+  Moments out;
+  out.count = count;
+  out.scale = scale;
+  out.mean = mean;
+  out.m2 = across_waves<false, THREADS>(wave_reduce<false>(q), smem + 3 * W);
+  return out;
 }
 
 // Moments of the values a block's threads hold (at most K per thread and
