@@ -127,15 +127,24 @@ __global__ void __launch_bounds__(256)
 //                            and the waiting legs (sample << 1 | leg) of each
 //                            wave into the wave's own queue region of 128
 //                            items, with their count;
-//   bridge_fill_kernel     - one wave per sample block: pass two over the
-//                            block's four regions (adding the expiry
-//                            payoff), then the block's moments, merged
-//                            exactly as autocall_kernel merges them.
+//   bridge_fill_kernel     - pass two over the queued legs of a run of
+//                            sample blocks (adding the expiry payoff), then
+//                            each block's moments, merged exactly as
+//                            autocall_kernel merges them.
 template <bool FP32>
 using SobolWord = typename std::conditional<FP32, uint32_t, uint64_t>::type;
-constexpr int FILL_THREADS = 256;
-constexpr int FILL_WAVES = FILL_THREADS / 64;
+constexpr int QUEUE_WAVES = BLOCK_SAMPLES / 64;  // pass-one waves (queue regions) per sample block
 constexpr int REGION = 128;  // queue items per pass-one wave: 64 samples, two legs
+// Sample blocks (one wave each) per fill workgroup: as many waves as a CU
+// holds at the kernel's register budget, FP32 at most 128 VGPRs (16 waves),
+// FP64 at most 256 (8 waves).
+template <bool FP32>
+constexpr int fill_waves() {
+  return FP32 ? 16 : 8;
+}
+// s_getreg operand of HW_ID's SIMD_ID field (bits 5:4): register 4, offset
+// 4, width 2.
+constexpr int HW_ID_SIMD = 4 | 4 << 6 | (2 - 1) << 11;
 
 template <bool FP32>
 __global__ void __launch_bounds__(256)
@@ -210,10 +219,10 @@ __global__ void __launch_bounds__(256)
   const uint64_t m_up = __ballot(want_up), m_down = __ballot(want_down);
   const int lane = j & 63;
   const uint64_t lower = (1ull << lane) - 1ull;
-  uint32_t* region = queue + (size_t)(block * FILL_WAVES + (j >> 6)) * REGION;
+  uint32_t* region = queue + (size_t)(block * QUEUE_WAVES + (j >> 6)) * REGION;
   if (want_up) region[__popcll(m_up & lower)] = (uint32_t)(n << 1);
   if (want_down) region[__popcll(m_up) + __popcll(m_down & lower)] = (uint32_t)(n << 1 | 1);
-  if (lane == 0) counts[block * FILL_WAVES + (j >> 6)] = (uint32_t)(__popcll(m_up) + __popcll(m_down));
+  if (lane == 0) counts[block * QUEUE_WAVES + (j >> 6)] = (uint32_t)(__popcll(m_up) + __popcll(m_down));
   if (bad) raise_flag(flags, FLAG_NONFINITE);
 }
 
@@ -235,15 +244,19 @@ __device__ inline void wave_merge4(Moments (&m)[4], bool full) {
   }
 }
 
-// Pass two over the queue, one wave per sample block: the wave copies its
-// block's base row into LDS once (so every lane's point is two table reads
-// and one broadcast read of the row, XORed) and fills the block's queued
-// legs 64 at a time; then it forms the block's moments from the 256 leg
-// values as block_merge over 256 threads forms them: lane l holds samples
-// l, 64 + l, 128 + l and 192 + l, merges each set across the wave
+// Pass two over the queue. A workgroup takes WAVES consecutive sample
+// blocks, one wave each. Each wave copies its block's base row into LDS (so
+// every lane's point is two table reads and one read of its block's row,
+// XORed); the queued legs of all the blocks (their regions, concatenated)
+// are cut into chunks of 64, and the chunks go to the waves in turn, spread
+// over the CU's four SIMDs, so the SIMDs share the fill evenly whatever the
+// blocks' counts (a leg's value does not depend on the lane that fills it).
+// After a workgroup barrier each wave forms its block's moments from the
+// 256 leg values as block_merge over 256 threads forms them: lane l holds
+// samples l, 64 + l, 128 + l and 192 + l, merges each set across the wave
 // (wave_merge), then the four results ((0, 1), (2, 3)).
 template <bool SOBOL, bool FP32, bool LDS>
-__global__ void __launch_bounds__(FILL_THREADS)
+__global__ void __launch_bounds__(64 * fill_waves<FP32>())
     bridge_fill_kernel(AcPlanDev p, uint64_t seed, int64_t samples, int64_t paths, int antithetic, int64_t blocks,
                        const uint64_t* __restrict__ dirs, const uint64_t* __restrict__ shifts,
                        const void* __restrict__ base_rows, const uint32_t* __restrict__ queue,
@@ -252,10 +265,18 @@ __global__ void __launch_bounds__(FILL_THREADS)
   using R = typename std::conditional<FP32, float, double>::type;
   using W = SobolWord<FP32>;
   constexpr int SHIFT = FP32 ? 20 : 0;
+  constexpr int WAVES = fill_waves<FP32>();
+  constexpr int THREADS = 64 * WAVES;
+  constexpr int REGIONS = WAVES * QUEUE_WAVES;
+  static_assert(REGIONS <= 64, "the region scan runs in one wave");
   extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ uint32_t edge[REGIONS + 1];
+  __shared__ int per_simd[4];
   const int dims = p.n_steps;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t block = (int64_t)blockIdx.x * FILL_WAVES + wave;
+  const int64_t first = (int64_t)blockIdx.x * WAVES;
+  const int64_t block = first + wave;
+  if (threadIdx.x < 4) per_simd[threadIdx.x] = 0;
   // LDS tables without a base point (block-independent), then each wave's
   // base row of dims + 1 words (the last one for the fill loop's final
   // look-ahead, whose value goes unused).
@@ -265,35 +286,58 @@ __global__ void __launch_bounds__(FILL_THREADS)
   proto.dirs = dirs;
   proto.shifts = shifts;
   proto.base = nullptr;
-  W* row = nullptr;
+  W* rows = nullptr;
   if (SOBOL && LDS) {
     W* lo = (W*)lds;
     W* hi = lo + (size_t)dims * 16;
-    for (int d = threadIdx.x; d < dims; d += FILL_THREADS)
+    for (int d = threadIdx.x; d < dims; d += THREADS)
       sobol_table_rows<W, SHIFT>(dirs, dims, d, 0ull, lo + d * 16, hi + d * 16);
-    row = hi + (size_t)dims * 16 + 16 + (size_t)wave * (dims + 1);
+    rows = hi + (size_t)dims * 16 + 16;
     if (block < blocks)
-      for (int d = lane; d <= dims; d += 64) row[d] = ((const W*)base_rows)[block * dims + d];
-    __syncthreads();
+      for (int d = lane; d <= dims; d += 64) rows[wave * (dims + 1) + d] = ((const W*)base_rows)[block * dims + d];
     proto.lo = lo;
     proto.hi = hi;
   }
-  if (block >= blocks) return;
-  // The block's four regions, concatenated.
-  uint32_t edge[FILL_WAVES + 1];
-  edge[0] = 0;
+  // The workgroup's regions (QUEUE_WAVES per block, empty past the last
+  // block) and their running sums: region k holds items edge[k]..edge[k+1].
+  if (wave == 0) {
+    uint32_t c = lane < REGIONS && first * QUEUE_WAVES + lane < blocks * QUEUE_WAVES
+                     ? counts[first * QUEUE_WAVES + lane] : 0u;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t t = __shfl_up(c, o, 64);
+      if (lane >= o) c += t;
+    }
+    if (lane < REGIONS) edge[lane + 1] = c;
+    if (lane == 0) edge[0] = 0u;
+  }
+  __syncthreads();
+  // The wave's place in the order (rank 0 on SIMDs 0..3, then rank 1, ...):
+  // chunk i goes to place i mod WAVES, so consecutive chunks land on
+  // different SIMDs. Any SIMD numbering gives a permutation of the places.
+  const int simd = __builtin_amdgcn_s_getreg(HW_ID_SIMD) & 3;
+  int rank = 0;
+  if (lane == 0) rank = atomicAdd(&per_simd[simd], 1);
+  rank = __shfl(rank, 0, 64);
+  __syncthreads();
+  int place = 0;
 #pragma unroll
-  for (int w = 0; w < FILL_WAVES; ++w) edge[w + 1] = edge[w] + counts[block * FILL_WAVES + w];
+  for (int s = 0; s < 4; ++s) {
+    const int n = per_simd[s];
+    place += (n < rank ? n : rank) + (s < simd && n > rank ? 1 : 0);
+  }
+  const uint32_t total = edge[REGIONS];
   uint32_t bad = 0;
-  for (uint32_t c = lane; c < edge[FILL_WAVES]; c += 64) {
-    int w = 0;
+  for (uint32_t c = (uint32_t)place * 64 + lane; c < total; c += THREADS) {
+    // The region of item c: the last k with edge[k] <= c.
+    int k = 0;
 #pragma unroll
-    for (int u = 1; u < FILL_WAVES; ++u) w += c >= edge[u];
-    const uint32_t item = queue[(size_t)(block * FILL_WAVES + w) * REGION + (c - edge[w])];
+    for (int step = REGIONS / 2; step > 0; step >>= 1) k += edge[k + step] <= c ? step : 0;
+    const uint32_t item = queue[(size_t)(first * QUEUE_WAVES + k) * REGION + (c - edge[k])];
     const int64_t n = item >> 1;
     const int leg = item & 1;
-    auto dr = sample_draws<SOBOL, FP32>(proto, seed, block, (int)(n - block * BLOCK_SAMPLES));
-    if constexpr (SOBOL && LDS) dr.base = row;
+    const int64_t b = first + k / QUEUE_WAVES;
+    auto dr = sample_draws<SOBOL, FP32>(proto, seed, b, (int)(n - b * BLOCK_SAMPLES));
+    if constexpr (SOBOL && LDS) dr.base = rows + (size_t)(k / QUEUE_WAVES) * (dims + 1);
     R lr;
     const bool ki = bridge_knock_in<SOBOL, FP32>(p, dr, leg ? -1.0f : 1.0f, lr);
     double& value = values[leg * samples + n];
@@ -302,10 +346,9 @@ __global__ void __launch_bounds__(FILL_THREADS)
     value = cash;
   }
   if (bad) raise_flag(flags, FLAG_NONFINITE);
-  // The lanes' stores above are read back by other lanes below.
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  // Every wave's stores above are read by the block's wave below.
+  __syncthreads();
+  if (block >= blocks) return;
   // Set 0 (every leg value), then set 1 (the pair midpoints): one set's
   // registers at a time.
   Moments m[4];
@@ -564,7 +607,7 @@ static void launch_bridge_passes(kg_context* ctx, const AcPlanDev& p, const Samp
   const int dims = p.n_steps;
   const int64_t samples = s.samples(), blocks = s.blocks();
   const int legs = s.antithetic ? 2 : 1;
-  const size_t regions = (size_t)blocks * FILL_WAVES;
+  const size_t regions = (size_t)blocks * QUEUE_WAVES;
   double* values = (double*)ctx->bridge_values.reserve(sizeof(double) * (size_t)samples * legs);
   // Per pass-one wave: a region of REGION items and its count.
   uint32_t* queue = (uint32_t*)ctx->bridge_queue.reserve(sizeof(uint32_t) * regions * (REGION + 1));
@@ -572,7 +615,7 @@ static void launch_bridge_passes(kg_context* ctx, const AcPlanDev& p, const Samp
   size_t events_lds = (size_t)p.n_intervals * sizeof(IntervalRow<R>), fill_lds = 0;
   if (SOBOL && LDS) {
     events_lds += (size_t)p.n_intervals * 32 * sizeof(W);
-    fill_lds = sobol_lds_bytes(dims, FP32) + (size_t)FILL_WAVES * (dims + 1) * sizeof(W);
+    fill_lds = sobol_lds_bytes(dims, FP32) + (size_t)fill_waves<FP32>() * (dims + 1) * sizeof(W);
   }
   if (events_lds > BRIDGE_EVENTS_LDS_LIMIT) fail(KG_DEVICE_ERROR, "bridge plan exceeds its pass-one LDS");
   // The call's device time covers the two sampling kernels only.
@@ -584,8 +627,9 @@ static void launch_bridge_passes(kg_context* ctx, const AcPlanDev& p, const Samp
   if (fill_lds > 64 * 1024)
     KG_HIP(hipFuncSetAttribute((const void*)bridge_fill_kernel<SOBOL, FP32, LDS>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)fill_lds));
-  hipLaunchKernelGGL((bridge_fill_kernel<SOBOL, FP32, LDS>), dim3((unsigned)((blocks + FILL_WAVES - 1) / FILL_WAVES)),
-                     dim3(FILL_THREADS), fill_lds, ctx->stream, p, s.seed, samples, s.path_count, (int)s.antithetic,
+  constexpr int WAVES = fill_waves<FP32>();
+  hipLaunchKernelGGL((bridge_fill_kernel<SOBOL, FP32, LDS>), dim3((unsigned)((blocks + WAVES - 1) / WAVES)),
+                     dim3(64 * WAVES), fill_lds, ctx->stream, p, s.seed, samples, s.path_count, (int)s.antithetic,
                      blocks, dirs, shifts, base, queue, counts, values, parts, flags);
   KG_HIP(hipGetLastError());
 }
