@@ -297,6 +297,13 @@ __global__ void __launch_bounds__(64 * fill_waves<FP32>())
       for (int d = lane; d <= dims; d += 64) rows[wave * (dims + 1) + d] = ((const W*)base_rows)[block * dims + d];
     proto.lo = lo;
     proto.hi = hi;
+    if constexpr (FP32) {
+      // The fill coefficients too: read from LDS, the fill loop waits on
+      // LDS reads alone, in order, instead of on every scalar load as well.
+      float* wf = (float*)(rows + (size_t)WAVES * (dims + 1));
+      for (int i = threadIdx.x; i < 2 * dims; i += THREADS) wf[i] = p.wf32[i];
+      p.wf32 = wf;
+    }
   }
   // The workgroup's regions (QUEUE_WAVES per block, empty past the last
   // block) and their running sums: region k holds items edge[k]..edge[k+1].
@@ -615,7 +622,8 @@ static void launch_bridge_passes(kg_context* ctx, const AcPlanDev& p, const Samp
   size_t events_lds = (size_t)p.n_intervals * sizeof(IntervalRow<R>), fill_lds = 0;
   if (SOBOL && LDS) {
     events_lds += (size_t)p.n_intervals * 32 * sizeof(W);
-    fill_lds = sobol_lds_bytes(dims, FP32) + (size_t)fill_waves<FP32>() * (dims + 1) * sizeof(W);
+    fill_lds = sobol_lds_bytes(dims, FP32) + (size_t)fill_waves<FP32>() * (dims + 1) * sizeof(W) +
+               (FP32 ? (size_t)2 * dims * sizeof(float) : 0);
   }
   if (events_lds > BRIDGE_EVENTS_LDS_LIMIT) fail(KG_DEVICE_ERROR, "bridge plan exceeds its pass-one LDS");
   // The call's device time covers the two sampling kernels only.
