@@ -27,17 +27,6 @@ __global__ void __launch_bounds__(MERGE_THREADS) merge_partials(const Moments* _
   if (threadIdx.x == 0) out[s] = m;
 }
 
-void merge_and_fetch(kg_context* ctx, const Moments* partials, int64_t blocks, int slots,
-                     Moments* host_out) {
-  Moments* d_out = (Moments*)ctx->results.reserve(sizeof(Moments) * slots);
-  hipLaunchKernelGGL(merge_partials, dim3(slots), dim3(MERGE_THREADS), 0, ctx->stream, partials, blocks, slots,
-                     d_out, (const uint32_t*)nullptr);
-  KG_HIP(hipGetLastError());
-  KG_HIP(hipMemcpyAsync(host_out, d_out, sizeof(Moments) * slots, hipMemcpyDeviceToHost,
-                        ctx->stream));
-  KG_HIP(hipStreamSynchronize(ctx->stream));
-}
-
 __global__ void clear_word(uint32_t* w) { *w = 0u; }
 
 uint32_t* reset_flags(kg_context* ctx) {
@@ -69,6 +58,27 @@ static void reserve_pinned(kg_context* ctx, size_t bytes) {
   ctx->pinned_bytes = size;
 }
 
+void merge_and_fetch(kg_context* ctx, const Moments* partials, int64_t blocks, int slots,
+                     Moments* host_out) {
+  const size_t bytes = sizeof(Moments) * slots;
+  if (bytes <= PINNED_RESULT_BYTES) {
+    // Merged straight into the pinned host buffer, as finish_sampling does.
+    reserve_pinned(ctx, PINNED_RESULT_BYTES);
+    hipLaunchKernelGGL(merge_partials, dim3(slots), dim3(MERGE_THREADS), 0, ctx->stream, partials, blocks, slots,
+                       (Moments*)ctx->pinned, (const uint32_t*)nullptr);
+    KG_HIP(hipGetLastError());
+    KG_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(host_out, ctx->pinned, bytes);
+    return;
+  }
+  Moments* d_out = (Moments*)ctx->results.reserve(bytes);
+  hipLaunchKernelGGL(merge_partials, dim3(slots), dim3(MERGE_THREADS), 0, ctx->stream, partials, blocks, slots,
+                     d_out, (const uint32_t*)nullptr);
+  KG_HIP(hipGetLastError());
+  KG_HIP(hipMemcpyAsync(host_out, d_out, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP(hipStreamSynchronize(ctx->stream));
+}
+
 void upload_async(kg_context* ctx, void* dst, const void* src, size_t n) {
   if (n == 0) return;
   const size_t at = PINNED_RESULT_BYTES + ((ctx->staging_used + 255) & ~size_t(255));
@@ -93,13 +103,12 @@ void finish_sampling(kg_context* ctx, const Moments* partials, int64_t blocks, i
   reserve_pinned(ctx, PINNED_RESULT_BYTES);
   const size_t result_bytes = sizeof(Moments) * slots;
   if (result_bytes + sizeof(uint32_t) > PINNED_RESULT_BYTES) fail(KG_INVALID_INPUT, "too many merged outputs");
-  // The merged slots, then the flag word (written by merge_partials).
-  Moments* d_out = (Moments*)ctx->results.reserve(result_bytes + sizeof(uint32_t));
-  hipLaunchKernelGGL(merge_partials, dim3(slots), dim3(MERGE_THREADS), 0, ctx->stream, partials, blocks, slots,
-                     d_out, (const uint32_t*)ctx->flags.ptr);
-  KG_HIP(hipGetLastError());
+  // merge_partials writes the merged slots, then the flag word, straight
+  // into the pinned host buffer (device-visible), so no copy follows it.
   unsigned char* pin = (unsigned char*)ctx->pinned;
-  KG_HIP(hipMemcpyAsync(pin, d_out, result_bytes + sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  hipLaunchKernelGGL(merge_partials, dim3(slots), dim3(MERGE_THREADS), 0, ctx->stream, partials, blocks, slots,
+                     (Moments*)pin, (const uint32_t*)ctx->flags.ptr);
+  KG_HIP(hipGetLastError());
   KG_HIP(hipStreamSynchronize(ctx->stream));
   uint32_t h;
   std::memcpy(&h, pin + result_bytes, sizeof(h));
