@@ -193,6 +193,20 @@ def test_antithetic_pair_standard_error_at_small_volatility(gpu, oracle):
         assert got.standard_error == pytest.approx(ref.standard_error, rel=rel)
 
 
+@pytest.mark.parametrize("paths,antithetic", [(4096 * 256, False), (4096 * 256 + 1, False), ((1 << 21) + 5, True)])
+def test_partial_merge_in_registers_and_streamed(gpu, oracle, paths, antithetic):
+    """The block partials are merged from registers up to 4,096 blocks and
+    streamed from memory past that (one more sample crosses over, and a
+    partial last block pads nothing): both forms meet the oracle's sequential
+    merge at 1e-10 in price and 1e-9 in standard error."""
+    o, m = option(), market()
+    s = settings(paths, 99, antithetic=antithetic)
+    got, ref = gpu_price(gpu, s, o, m), oracle.vanilla_price(s, o, m, workers=8)
+    assert got.price == pytest.approx(ref.price, rel=1e-10)
+    assert got.standard_error == pytest.approx(ref.standard_error, rel=1e-9)
+    assert got.uncertainty_status == ref.uncertainty_status
+
+
 def test_zero_variance_payoff_reports_zero_observed_variance(gpu, oracle):
     """A call struck far above any path (sigma 1e-4, strike 200) pays zero on
     every path: the status is ZERO_OBSERVED_VARIANCE with no standard error, as
