@@ -52,10 +52,10 @@ C5_FORWARD_FLOP = 25.0  # normal 6, local-vol bilinear 11, log-Euler 6, exp 2
 C5_BYTES_PER_PATH_DATE = 24.0  # FP32 coordinate stored and reloaded, FP64 cash read and written
 MLP_MFMA_FLOP = 3 * 2 * 64 * 64  # per sample and Adam step: layer-2 forward, input gradient, weight gradient
 # HBM bytes per config-2 FP32 pricing of its two sampling kernels
-# (bridge_events_kernel 13.45 MB + bridge_fill_kernel 16.70 MB): FETCH_SIZE
+# (bridge_events_kernel 13.45 MB + bridge_fill_kernel 16.54 MB): FETCH_SIZE
 # and WRITE_SIZE from the two counter passes of `bench.py --profile-pass`,
 # scaled on a known 256 MiB read/write.
-C2_KERNEL_HBM_BYTES = 30_150_207
+C2_KERNEL_HBM_BYTES = 29_988_010
 # Host threads for the CPU legs: the host share of one GPU on the box.
 CPU_THREADS = 16
 CPU_MIN_RUNS = 15
